@@ -1,0 +1,113 @@
+/* Context-level C ABI of the MI355X GS backend.
+ *
+ * Mirrors the public API of the reference's GSInterface (gs/gs_interface.hpp:251-305)
+ * one call per method, with the Vulkan device argument replaced by a HIP device index.
+ * All calls for one context must come from a single thread, as in the reference.
+ * Functions returning int give 0 on success and a negative value on error.
+ */
+#ifndef PGS_AMD_H
+#define PGS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pgs_context pgs_context;
+
+/* GSOptions (gs/gs_interface.hpp:200-207). super_sampling is the sample count 1/2/4/8/16. */
+typedef struct pgs_options
+{
+	uint32_t super_sampling;
+	uint32_t ordered_super_sampling;
+	uint32_t super_sampled_textures;
+	uint32_t device_index;
+	uint64_t upload_arena_bytes; /* 0 selects the default (256 MiB) */
+} pgs_options;
+
+/* FlushStats (gs/gs_renderer.hpp:55-65), extended with the backend's own counters. */
+typedef struct pgs_flush_stats
+{
+	uint64_t transfer_batches;
+	uint64_t copies;
+	uint64_t copy_blocks;
+	uint64_t copy_records;
+	uint64_t snapshot_blocks;
+	uint64_t copy_bytes;
+	uint64_t render_passes;
+	uint64_t primitives;
+	uint64_t syncs;
+} pgs_flush_stats;
+
+/* GSInterface::GSInterface + GSInterface::init (gs/gs_interface.cpp:15-63). NULL on failure. */
+pgs_context *pgs_create(const pgs_options *options);
+void pgs_destroy(pgs_context *ctx);
+
+/* GSInterface::reset_context_state (gs/gs_interface.cpp:26-39). */
+int pgs_reset_context_state(pgs_context *ctx);
+
+/* GSInterface::gif_transfer (gs/gs_interface.cpp:4290-4397). size is a multiple of 16, path < 4. */
+int pgs_gif_transfer(pgs_context *ctx, uint32_t path, const void *data, size_t size);
+
+/* GSInterface::write_register (gs/gs_interface.cpp:4276-4279). */
+int pgs_write_register(pgs_context *ctx, uint32_t addr, uint64_t payload);
+
+/* GSInterface::flush (gs/gs_interface.cpp:4257-4264). Submits all pending work. */
+int pgs_flush(pgs_context *ctx);
+
+/* GSInterface::map_vram_read (gs/gs_interface.cpp:4230-4255). Blocks until the GPU is idle. */
+const void *pgs_map_vram_read(pgs_context *ctx, size_t offset, size_t size);
+
+/* GSInterface::map_vram_write / end_vram_write (gs/gs_interface.cpp:4184-4228). */
+void *pgs_map_vram_write(pgs_context *ctx, size_t offset, size_t size);
+int pgs_end_vram_write(pgs_context *ctx, size_t offset, size_t size);
+
+/* GSInterface::read_transfer_fifo (gs/gs_interface.cpp:3150-3166). */
+int pgs_read_transfer_fifo(pgs_context *ctx, void *data, uint32_t num_128b_words);
+
+/* GSInterface::clobber_register_state (gs/gs_interface.cpp:4266-4274). */
+int pgs_clobber_register_state(pgs_context *ctx);
+
+/* GSInterface::consume_flush_stats (gs/gs_interface.cpp:4697-4700). */
+int pgs_consume_flush_stats(pgs_context *ctx, pgs_flush_stats *stats);
+
+/* Number of 4 MiB VRAM planes: 1 at 1x, 2 + samples with super-sampling
+ * (plane layout of GSRenderer::init_vram, gs/gs_renderer.cpp:360-427). */
+uint32_t pgs_num_planes(pgs_context *ctx);
+
+/* Copies every plane to dst (pgs_num_planes * 4 MiB). Blocks until the GPU is idle.
+ * Backend extension used by parity tests: the reference exposes plane 0 only. */
+int pgs_read_vram_planes(pgs_context *ctx, void *dst);
+
+/* Launch recording: work flushed between begin/end is kept resident in device memory and
+ * can be re-issued with pgs_replay (optionally as one hipGraph). Backend extension used to
+ * time device execution separately from host parsing. */
+int pgs_begin_recording(pgs_context *ctx);
+int pgs_end_recording(pgs_context *ctx);
+int pgs_replay(pgs_context *ctx, int use_graph);
+/* Per-launch HIP event timing. While enabled, every kernel launch of the context is
+ * bracketed by events on its stream; pgs_profile_collect synchronizes and writes the summed
+ * milliseconds and launch counts per kernel kind (0 = VRAM copy, 1 = snapshot, 2 = triangle
+ * setup, 3 = binning, 4 = shading, ...). */
+int pgs_profile_enable(pgs_context *ctx, int enable);
+int pgs_profile_collect(pgs_context *ctx, double *ms, uint64_t *launches, uint32_t kinds);
+/* Blocks until all submitted work has finished. */
+int pgs_sync(pgs_context *ctx);
+/* Returns the HIP stream (hipStream_t) the context submits on. */
+void *pgs_stream(pgs_context *ctx);
+
+/* Host execution of the swizzle the kernels use: byte address (nibble address for PSMT4) of
+ * pixel (x, y). Same mapping as swizzle_PS2 (gs/shaders/swizzle_utils.h:329-522). */
+uint32_t pgs_host_swizzle_addr(uint32_t x, uint32_t y, uint32_t base, uint32_t stride, uint32_t psm);
+
+/* Host execution of the in-block inverse layout the copy kernel uses: pixel offset (dx, dy)
+ * inside a 256-byte block of element k of dword `lane`. */
+void pgs_host_block_lane_pixel(uint32_t psm, uint32_t lane, uint32_t k, uint32_t *dx, uint32_t *dy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,403 @@
+// Host front end: GIF parsing, register file and transfer state machine.
+#include "gs_interface.hpp"
+#include <cstring>
+#include <algorithm>
+
+namespace pgs
+{
+bool GSInterface::init(const GSOptions &options)
+{
+	options_ = options;
+	uint32_t s = uint32_t(options.super_sampling);
+	uint32_t sample_log2 = s >= 16 ? 4 : s >= 8 ? 3 : s >= 4 ? 2 : s >= 2 ? 1 : 0;
+	if (!renderer_.init(sample_log2, options.upload_arena_bytes))
+		return false;
+	rp_.init(&renderer_, sample_log2);
+	reset_context_state();
+	return true;
+}
+
+// gs/gs_interface.cpp:26-39
+void GSInterface::reset_context_state()
+{
+	regs_ = {};
+	for (auto &p : paths_)
+		p = {};
+	xfer_.host_to_local_active = false;
+	xfer_.payload.clear();
+	rp_.reset_vertex_queue();
+}
+
+void GSInterface::clobber_register_state()
+{
+	flush();
+	rp_.reset_vertex_queue();
+}
+
+// Transfers and draws execute in submission order on one stream. The host keeps at most one
+// kind of work pending: queuing a transfer first flushes pending primitives, and a draw
+// first flushes pending transfers (see RenderPassBuilder). The reference reaches the same
+// ordering through page-hazard flushes (page_tracker.cpp:221-325, 621-673).
+void GSInterface::begin_transfer_work()
+{
+	if (rp_.has_primitives())
+		flush_render_pass();
+}
+
+void GSInterface::flush_render_pass()
+{
+	rp_.flush();
+}
+
+// ---------------------------------------------------------------------------------------
+// GIF packet decoding (gs/gs_interface.cpp:4290-4397).
+
+void GSInterface::gif_transfer(uint32_t path_index, const void *data, size_t size)
+{
+	auto &path = paths_[path_index & 3u];
+	size /= 16;
+	const uint64_t *q = static_cast<const uint64_t *>(data);
+
+	for (size_t i = 0; i < size;)
+	{
+		if (path.loop == path.nloop)
+		{
+			uint64_t lo = q[2 * i], hi = q[2 * i + 1];
+			path.tag_lo = lo;
+			path.nloop = uint32_t(lo & 0x7fff);
+			path.flg = uint32_t((lo >> 58) & 3u);
+			path.nreg = uint32_t((lo >> 60) & 0xf);
+			path.regs = hi;
+			path.loop = 0;
+			path.reg = 0;
+			bool pre = ((lo >> 46) & 1u) != 0;
+			if (path.flg == 0 && pre && path.nloop)
+				write_register(0x00, (lo >> 47) & 0x7ffu);
+			if (path.nloop)
+				regs_.internal_q = 1.0f;
+			i++;
+			continue;
+		}
+
+		const uint32_t nreg = path.nreg ? path.nreg : 16u;
+		if (path.flg == 0)
+		{
+			// PACKED: one register per qword.
+			while (i < size && path.loop < path.nloop)
+			{
+				uint32_t addr = uint32_t(path.regs >> (4u * path.reg)) & 0xfu;
+				packed_write(addr, q + 2 * i);
+				i++;
+				if (++path.reg == nreg)
+				{
+					path.reg = 0;
+					path.loop++;
+				}
+			}
+		}
+		else if (path.flg == 1)
+		{
+			// REGLIST: two 64-bit register values per qword.
+			for (uint32_t j = 0; j < 2; j++)
+			{
+				uint32_t addr = uint32_t(path.regs >> (4u * path.reg)) & 0xfu;
+				if (addr < 0xe)
+					write_register(addr, q[2 * i + j]);
+				if (++path.reg == nreg)
+				{
+					path.reg = 0;
+					path.loop++;
+					if (path.loop == path.nloop)
+						break;
+				}
+			}
+			i++;
+		}
+		else
+		{
+			// IMAGE: raw HWREG payload.
+			size_t n = std::min<size_t>(size - i, path.nloop - path.loop);
+			hwreg(q + 2 * i, n * 2);
+			i += n;
+			path.loop += uint32_t(n);
+		}
+	}
+}
+
+// PACKED register formats (gs/gs_interface.cpp:3992-4138; gs/gs_registers.hpp:443-517).
+void GSInterface::packed_write(uint32_t addr, const uint64_t *w)
+{
+	const uint64_t lo = w[0], hi = w[1];
+	switch (addr)
+	{
+	case 0x0:
+		write_register(0x00, lo & 0x7ffu);
+		break;
+	case 0x1:
+	{
+		uint64_t rgba = (lo & 0xff) | (((lo >> 32) & 0xff) << 8) | ((hi & 0xff) << 16) | (((hi >> 32) & 0xff) << 24);
+		uint32_t qbits;
+		memcpy(&qbits, &regs_.internal_q, 4);
+		regs_.rgbaq = rgba | (uint64_t(qbits) << 32);
+		break;
+	}
+	case 0x2:
+		regs_.st = lo;
+		{
+			uint32_t qbits = uint32_t(hi);
+			memcpy(&regs_.internal_q, &qbits, 4);
+		}
+		break;
+	case 0x3:
+		regs_.uv = (lo & 0x3fffu) | (((lo >> 32) & 0x3fffu) << 16);
+		break;
+	case 0x4: case 0xc:
+	{
+		uint64_t x = lo & 0xffff, y = (lo >> 32) & 0xffff;
+		uint64_t z = (hi >> 4) & 0xffffff, f = (hi >> 36) & 0xff;
+		bool adc = addr == 0xc || ((hi >> 47) & 1u);
+		if (!adc)
+			write_register(0x04, x | y << 16 | z << 32 | f << 56);
+		else
+			write_register(0x0c, x | y << 16 | z << 32 | f << 56);
+		break;
+	}
+	case 0x5: case 0xd:
+	{
+		uint64_t x = lo & 0xffff, y = (lo >> 32) & 0xffff;
+		uint64_t z = hi & 0xffffffffu;
+		bool adc = addr == 0xd || ((hi >> 47) & 1u);
+		write_register(adc ? 0x0d : 0x05, x | y << 16 | z << 32);
+		break;
+	}
+	case 0x6: case 0x7: case 0x8: case 0x9:
+		write_register(addr, lo);
+		break;
+	case 0xa:
+		regs_.fog = ((hi >> 36) & 0xff) << 56;
+		break;
+	case 0xe:
+		write_register(uint32_t(hi & 0x7f), lo);
+		break;
+	default:
+		break;
+	}
+}
+
+// A+D register writes (gs/gs_interface.cpp:3507-3986).
+void GSInterface::write_register(uint32_t addr, uint64_t v)
+{
+	switch (addr)
+	{
+	case 0x00: regs_.prim = v; rp_.reset_vertex_queue(); break;
+	case 0x01: case 0x11: regs_.rgbaq = v; break;
+	case 0x02: regs_.st = v; break;
+	case 0x03: regs_.uv = v; break;
+	case 0x0a: regs_.fog = v; break;
+	case 0x06: case 0x07: regs_.tex0[addr - 0x06] = v; break;
+	case 0x08: case 0x09: regs_.clamp[addr - 0x08] = v; break;
+	case 0x14: case 0x15: regs_.tex1[addr - 0x14] = v; break;
+	case 0x18: case 0x19: regs_.xyoffset[addr - 0x18] = v; break;
+	case 0x1a: regs_.prmodecont = v; break;
+	case 0x1b: regs_.prmode = v; break;
+	case 0x1c: regs_.texclut = v; break;
+	case 0x22: regs_.scanmsk = v; break;
+	case 0x34: case 0x35: regs_.miptbp1[addr - 0x34] = v; break;
+	case 0x36: case 0x37: regs_.miptbp2[addr - 0x36] = v; break;
+	case 0x3b: regs_.texa = v; break;
+	case 0x3d: regs_.fogcol = v; break;
+	case 0x40: case 0x41: regs_.scissor[addr - 0x40] = v; break;
+	case 0x42: case 0x43: regs_.alpha[addr - 0x42] = v; break;
+	case 0x44: regs_.dimx = v; break;
+	case 0x45: regs_.dthe = v; break;
+	case 0x46: regs_.colclamp = v; break;
+	case 0x47: case 0x48: regs_.test[addr - 0x47] = v; break;
+	case 0x49: regs_.pabe = v; break;
+	case 0x4a: case 0x4b: regs_.fba[addr - 0x4a] = v; break;
+	case 0x4c: case 0x4d: regs_.frame[addr - 0x4c] = v; break;
+	case 0x4e: case 0x4f: regs_.zbuf[addr - 0x4e] = v; break;
+	case 0x50: regs_.bitbltbuf = v; break;
+	case 0x51: regs_.trxpos = v; break;
+	case 0x52: regs_.trxreg = v; break;
+	case 0x53: regs_.trxdir = v; init_transfer(); break;
+	case 0x54: hwreg(&v, 1); break;
+	default: break;
+	}
+}
+
+// ---------------------------------------------------------------------------------------
+// Transfers (gs/gs_interface.cpp:3168-3276, 2970-3148).
+
+static CopyDesc decode_copy(const RegisterState &r)
+{
+	CopyDesc c{};
+	uint64_t b = r.bitbltbuf, p = r.trxpos, g = r.trxreg;
+	c.sbp = uint32_t(b & 0x3fff);
+	c.sbw = uint32_t((b >> 16) & 0x3f);
+	c.spsm = uint32_t((b >> 24) & 0x3f);
+	c.dbp = uint32_t((b >> 32) & 0x3fff);
+	c.dbw = uint32_t((b >> 48) & 0x3f);
+	c.dpsm = uint32_t((b >> 56) & 0x3f);
+	c.ssax = uint32_t(p & 0x7ff);
+	c.ssay = uint32_t((p >> 16) & 0x7ff);
+	c.dsax = uint32_t((p >> 32) & 0x7ff);
+	c.dsay = uint32_t((p >> 48) & 0x7ff);
+	c.w = uint32_t(g & 0xfff);
+	c.h = uint32_t((g >> 32) & 0xfff);
+	return c;
+}
+
+void GSInterface::init_transfer()
+{
+	flush_pending_transfer(false);
+	CopyDesc c = decode_copy(regs_);
+	uint32_t xdir = uint32_t(regs_.trxdir & 3u);
+
+	if (xdir == 2)
+	{
+		if (c.w == 0 || c.h == 0)
+			return;
+		begin_transfer_work();
+		renderer_.queue_local_copy(c);
+	}
+	else if (xdir == 0)
+	{
+		xfer_.copy = c;
+		xfer_.required_qwords = uint32_t((uint64_t(c.w) * c.h * psm_bpp(c.dpsm)) / 64u);
+		xfer_.host_to_local_active = xfer_.required_qwords != 0;
+		xfer_.payload.clear();
+		xfer_.flushed_pixels = 0;
+	}
+	else if (xdir == 1)
+	{
+		// Local->host readback into the transfer FIFO (gs/gs_interface.cpp:3213-3275).
+		uint32_t bytes = uint32_t((uint64_t(c.w) * c.h * psm_bpp(c.spsm)) / 8u);
+		xfer_.fifo.assign(bytes, 0);
+		xfer_.fifo_offset_128 = 0;
+		xfer_.fifo_size_128 = bytes / 16u;
+		if (host_vram_.size() < kVramSize)
+			host_vram_.resize(kVramSize);
+		renderer_.read_vram(host_vram_.data(), 0, 1);
+		const uint8_t *v = host_vram_.data();
+		for (uint32_t y = 0; y < c.h; y++)
+			for (uint32_t x = 0; x < c.w; x++)
+			{
+				uint32_t idx = y * c.w + x;
+				uint32_t a = swizzle_addr((x + c.ssax) & 2047u, (y + c.ssay) & 2047u, c.sbp, c.sbw, c.spsm);
+				switch (c.spsm)
+				{
+				case PSMCT32: case PSMZ32:
+					memcpy(&xfer_.fifo[idx * 4], v + a, 4);
+					break;
+				case PSMCT24: case PSMZ24:
+					if (idx * 3 + 3 <= bytes)
+						memcpy(&xfer_.fifo[idx * 3], v + a, 3);
+					break;
+				case PSMCT16: case PSMCT16S: case PSMZ16: case PSMZ16S:
+					memcpy(&xfer_.fifo[idx * 2], v + a, 2);
+					break;
+				case PSMT8:
+					xfer_.fifo[idx] = v[a];
+					break;
+				case PSMT8H:
+					xfer_.fifo[idx] = v[a | 3u];
+					break;
+				default:
+					break;
+				}
+			}
+	}
+}
+
+void GSInterface::hwreg(const uint64_t *data, size_t count)
+{
+	if (!xfer_.host_to_local_active)
+		return;
+	xfer_.payload.insert(xfer_.payload.end(), data, data + count);
+	if (xfer_.payload.size() >= xfer_.required_qwords)
+		flush_pending_transfer(false);
+}
+
+// Emits the pixels whose payload has arrived. With keep_alive the transfer stays active
+// and later payload continues where this flush ended (partial transfers).
+void GSInterface::flush_pending_transfer(bool keep_alive)
+{
+	if (xfer_.host_to_local_active)
+	{
+		const CopyDesc &c = xfer_.copy;
+		uint32_t total = c.w * c.h;
+		uint64_t bits = uint64_t(xfer_.payload.size()) * 64u;
+		uint32_t avail = uint32_t(std::min<uint64_t>(total, bits / psm_bpp(c.dpsm)));
+		if (avail > xfer_.flushed_pixels)
+		{
+			begin_transfer_work();
+			CopyDesc d = c;
+			d.first_pixel = xfer_.flushed_pixels;
+			d.end_pixel = avail;
+			renderer_.queue_host_copy(d, xfer_.payload.data(), xfer_.payload.size() * 8u);
+			xfer_.flushed_pixels = avail;
+		}
+	}
+
+	if (!keep_alive)
+	{
+		xfer_.host_to_local_active = false;
+		xfer_.payload.clear();
+		xfer_.flushed_pixels = 0;
+	}
+}
+
+// gs/gs_interface.cpp:3150-3166
+void GSInterface::read_transfer_fifo(void *data, uint32_t n)
+{
+	uint32_t avail = xfer_.fifo_size_128 - std::min(xfer_.fifo_size_128, xfer_.fifo_offset_128);
+	uint32_t copy = std::min(n, avail);
+	if (copy)
+		memcpy(data, xfer_.fifo.data() + 16u * xfer_.fifo_offset_128, copy * 16u);
+	if (copy < n)
+		memset(static_cast<uint8_t *>(data) + 16u * copy, 0, (n - copy) * 16u);
+	xfer_.fifo_offset_128 += copy;
+}
+
+// ---------------------------------------------------------------------------------------
+// Flush and host VRAM access (gs/gs_interface.cpp:4184-4264).
+
+void GSInterface::flush()
+{
+	flush_pending_transfer(true);
+	flush_render_pass();
+	renderer_.flush_transfers();
+}
+
+const void *GSInterface::map_vram_read(size_t offset, size_t size)
+{
+	flush();
+	if (host_vram_.size() < kVramSize)
+		host_vram_.resize(kVramSize);
+	renderer_.read_vram(host_vram_.data(), 0, 1);
+	if (offset + size > kVramSize)
+		return nullptr;
+	return host_vram_.data() + offset;
+}
+
+void GSInterface::read_all_planes(void *dst)
+{
+	flush();
+	renderer_.read_vram(dst, 0, renderer_.num_planes());
+}
+
+void *GSInterface::map_vram_write(size_t offset, size_t size)
+{
+	if (offset + size > kVramSize)
+		return nullptr;
+	map_vram_read(0, kVramSize);
+	return host_vram_.data() + offset;
+}
+
+void GSInterface::end_vram_write(size_t offset, size_t size)
+{
+	if (offset + size > kVramSize)
+		return;
+	renderer_.write_vram(host_vram_.data() + offset, offset, size);
+}
+}

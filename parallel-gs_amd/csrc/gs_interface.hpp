@@ -1,0 +1,102 @@
+// Host front end of the MI355X GS backend: a drop-in for the reference's GSInterface
+// (gs/gs_interface.hpp:251-305). GIF packets and register writes are decoded on the host;
+// transfers and render passes are handed to the Renderer, which runs HIP kernels.
+#pragma once
+#include "gs_defs.hpp"
+#include "renderer.hpp"
+#include "render_pass.hpp"
+#include <vector>
+#include <cstdint>
+
+namespace pgs
+{
+enum class SuperSampling : uint32_t { X1 = 1, X2 = 2, X4 = 4, X8 = 8, X16 = 16 };
+
+struct GSOptions
+{
+	SuperSampling super_sampling = SuperSampling::X1;
+	uint32_t vram_size = kVramSize;
+	bool dynamic_super_sampling = false;
+	bool ordered_super_sampling = true;
+	bool super_sampled_textures = false;
+	size_t upload_arena_bytes = 256u << 20;
+};
+
+// Register file (the GS privileged-less register set the GIF can write).
+struct RegisterState
+{
+	uint64_t prim = 0;
+	uint64_t rgbaq = 0; // RGBA in the low word, Q (float bits) in the high word
+	uint64_t st = 0;
+	uint64_t uv = 0;
+	uint64_t fog = 0;
+	uint64_t tex0[2] = {}, clamp[2] = {}, tex1[2] = {}, xyoffset[2] = {}, miptbp1[2] = {}, miptbp2[2] = {};
+	uint64_t scissor[2] = {}, alpha[2] = {}, test[2] = {}, fba[2] = {}, frame[2] = {}, zbuf[2] = {};
+	uint64_t prmodecont = 1, prmode = 0, texclut = 0, scanmsk = 0, texa = 0, fogcol = 0;
+	uint64_t dimx = 0, dthe = 0, colclamp = 0, pabe = 0;
+	uint64_t bitbltbuf = 0, trxpos = 0, trxreg = 0, trxdir = 0;
+	float internal_q = 1.0f;
+};
+
+struct GIFPath
+{
+	uint32_t nloop = 0, loop = 0, flg = 0, nreg = 0, reg = 0;
+	uint64_t regs = 0;
+	uint64_t tag_lo = 0;
+};
+
+class GSInterface
+{
+public:
+	GSInterface() = default;
+	bool init(const GSOptions &options);
+	void reset_context_state();
+
+	void gif_transfer(uint32_t path, const void *data, size_t size);
+	void write_register(uint32_t addr, uint64_t payload);
+
+	void *map_vram_write(size_t offset, size_t size);
+	void end_vram_write(size_t offset, size_t size);
+	const void *map_vram_read(size_t offset, size_t size);
+	// Reads every VRAM plane (plane 0 = VRAM; at SSAA: 1 = reference, 2.. = samples).
+	void read_all_planes(void *dst);
+
+	void flush();
+	void clobber_register_state();
+	void read_transfer_fifo(void *data, uint32_t num_128b_words);
+
+	RegisterState &get_register_state() { return regs_; }
+	GIFPath &get_gif_path(uint32_t path) { return paths_[path & 3u]; }
+	FlushStats consume_flush_stats() { return renderer_.consume_stats(); }
+	Renderer &renderer() { return renderer_; }
+	RenderPassBuilder &render_pass() { return rp_; }
+
+private:
+	struct TransferState
+	{
+		CopyDesc copy{};
+		bool host_to_local_active = false;
+		uint32_t required_qwords = 0;
+		uint32_t flushed_pixels = 0;
+		std::vector<uint64_t> payload;
+		std::vector<uint8_t> fifo;
+		uint32_t fifo_offset_128 = 0;
+		uint32_t fifo_size_128 = 0;
+	};
+
+	GSOptions options_;
+	Renderer renderer_;
+	RenderPassBuilder rp_;
+	RegisterState regs_;
+	GIFPath paths_[4];
+	TransferState xfer_;
+	std::vector<uint8_t> host_vram_;
+
+	void packed_write(uint32_t addr, const uint64_t *qword);
+	void init_transfer();
+	void hwreg(const uint64_t *data, size_t count);
+	void flush_pending_transfer(bool keep_alive);
+	void begin_transfer_work();
+	void flush_render_pass();
+};
+}

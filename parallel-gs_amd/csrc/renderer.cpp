@@ -1,0 +1,491 @@
+// Device-side renderer: HBM buffers, upload arena, transfer batching and launch recording.
+#include "renderer.hpp"
+#include <algorithm>
+#include <cstdio>
+#include <stdexcept>
+
+namespace pgs
+{
+namespace
+{
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
+
+// Upper bound on the number of 256-byte blocks a rectangle touches.
+inline size_t rect_block_bound(uint32_t psm, uint32_t w, uint32_t h)
+{
+	const Geometry g = psm_geometry(psm);
+	size_t bx = (size_t(w) >> g.bw) + 2, by = (size_t(h) >> g.bh) + 2;
+	return bx * by;
+}
+
+// Arena bytes a copy's work lists can need at batch close: its descriptor plus, per touched
+// block, one record, one destination-block entry and one snapshot-list entry.
+inline size_t copy_meta_bound(const CopyDesc &d)
+{
+	size_t blocks = rect_block_bound(d.dpsm, d.w, d.h);
+	size_t src_blocks = (d.flags & COPY_LOCAL) ? rect_block_bound(d.spsm, d.w, d.h) : 0;
+	return sizeof(CopyDesc) + blocks * (sizeof(BlockRecord) + sizeof(DestBlock)) + src_blocks * 4 + 4 * kAlign;
+}
+}
+
+Renderer::~Renderer()
+{
+	if (stream_)
+		(void)hipStreamSynchronize(stream_);
+	destroy_graph();
+	auto free_arena = [](Arena &a) {
+		if (a.host) (void)hipHostFree(a.host);
+		if (a.dev) (void)hipFree(a.dev);
+		a = {};
+	};
+	free_arena(arena_);
+	for (auto &a : retired_arenas_)
+		free_arena(a);
+	for (auto &t : timed_)
+	{
+		event_pool_.push_back(t.start);
+		event_pool_.push_back(t.stop);
+	}
+	for (auto e : event_pool_)
+		(void)hipEventDestroy(e);
+	if (vram_) (void)hipFree(vram_);
+	if (shadow_) (void)hipFree(shadow_);
+	if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Renderer::check(hipError_t err)
+{
+	if (err != hipSuccess)
+	{
+		last_error_ = -int(err);
+		fprintf(stderr, "pgs: HIP error %d (%s)\n", int(err), hipGetErrorString(err));
+		throw std::runtime_error(hipGetErrorString(err));
+	}
+}
+
+bool Renderer::init(uint32_t sample_log2, size_t arena_bytes)
+{
+	try
+	{
+		sample_log2_ = sample_log2;
+		num_planes_ = sample_log2 ? 2u + (1u << sample_log2) : 1u;
+		check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+		check(hipMalloc(&vram_, size_t(num_planes_) * kVramSize));
+		check(hipMemsetAsync(vram_, 0, size_t(num_planes_) * kVramSize, stream_));
+		check(hipMalloc(&shadow_, kVramSize));
+		arena_.cap = std::max<size_t>(arena_bytes, 16u << 20);
+		check(hipHostMalloc(&arena_.host, arena_.cap, hipHostMallocDefault));
+		check(hipMalloc(&arena_.dev, arena_.cap));
+		batch_written_.assign(kNumBlocks / 64, 0);
+		block_count_.assign(kNumBlocks, 0);
+		check(hipStreamSynchronize(stream_));
+		return true;
+	}
+	catch (const std::exception &)
+	{
+		return false;
+	}
+}
+
+// ---------------------------------------------------------------------------------------
+// Upload arena. Host writes go to pinned memory; arena_upload() copies the not yet uploaded
+// range to the device mirror on the stream, ahead of the launches that read it.
+
+uint8_t *Renderer::arena_alloc(size_t bytes, size_t &dev_offset)
+{
+	bytes = align_up(std::max<size_t>(bytes, 1), kAlign);
+	if (arena_.used + bytes > arena_.cap)
+		arena_make_room(bytes);
+	dev_offset = arena_.used;
+	uint8_t *p = arena_.host + arena_.used;
+	arena_.used += bytes;
+	return p;
+}
+
+void Renderer::arena_upload()
+{
+	if (arena_.used > arena_.uploaded)
+	{
+		check(hipMemcpyAsync(arena_.dev + arena_.uploaded, arena_.host + arena_.uploaded,
+		                     arena_.used - arena_.uploaded, hipMemcpyHostToDevice, stream_));
+		arena_.uploaded = arena_.used;
+	}
+}
+
+void Renderer::arena_make_room(size_t bytes)
+{
+	arena_upload();
+	if (recording_)
+	{
+		// Keep the filled arena alive for replay and start a fresh one.
+		retired_arenas_.push_back(arena_);
+		arena_ = {};
+		arena_.cap = std::max<size_t>(retired_arenas_.back().cap, align_up(bytes, 1u << 20));
+		check(hipHostMalloc(&arena_.host, arena_.cap, hipHostMallocDefault));
+		check(hipMalloc(&arena_.dev, arena_.cap));
+		return;
+	}
+
+	check(hipStreamSynchronize(stream_));
+	stats_.syncs++;
+	if (bytes > arena_.cap)
+	{
+		(void)hipHostFree(arena_.host);
+		(void)hipFree(arena_.dev);
+		arena_.cap = align_up(bytes, 1u << 20);
+		check(hipHostMalloc(&arena_.host, arena_.cap, hipHostMallocDefault));
+		check(hipMalloc(&arena_.dev, arena_.cap));
+	}
+	arena_.used = 0;
+	arena_.uploaded = 0;
+}
+
+void Renderer::arena_reset_after_sync()
+{
+	if (!recording_)
+	{
+		arena_.used = 0;
+		arena_.uploaded = 0;
+	}
+}
+
+void Renderer::sync()
+{
+	if (!batch_.empty())
+		flush_transfers();
+	arena_upload();
+	check(hipStreamSynchronize(stream_));
+	stats_.syncs++;
+	arena_reset_after_sync();
+}
+
+// ---------------------------------------------------------------------------------------
+// Launches. Every launch goes through the kernel-level C ABI.
+
+hipEvent_t Renderer::get_event()
+{
+	if (!event_pool_.empty())
+	{
+		hipEvent_t e = event_pool_.back();
+		event_pool_.pop_back();
+		return e;
+	}
+	hipEvent_t e;
+	check(hipEventCreate(&e));
+	return e;
+}
+
+void Renderer::profile_enable(bool enable)
+{
+	profiling_ = enable;
+}
+
+void Renderer::profile_collect(double *ms, uint64_t *launches, uint32_t kinds)
+{
+	check(hipStreamSynchronize(stream_));
+	for (uint32_t i = 0; i < kinds; i++)
+	{
+		ms[i] = 0.0;
+		launches[i] = 0;
+	}
+	for (auto &t : timed_)
+	{
+		float e = 0.0f;
+		check(hipEventElapsedTime(&e, t.start, t.stop));
+		if (t.kind < kinds)
+		{
+			ms[t.kind] += e;
+			launches[t.kind]++;
+		}
+		event_pool_.push_back(t.start);
+		event_pool_.push_back(t.stop);
+	}
+	timed_.clear();
+}
+
+void Renderer::launch(const Launch &l)
+{
+	TimedLaunch t{ uint32_t(l.kind), nullptr, nullptr };
+	if (profiling_)
+	{
+		t.start = get_event();
+		t.stop = get_event();
+		check(hipEventRecord(t.start, stream_));
+	}
+	int rc = 0;
+	switch (l.kind)
+	{
+	case LaunchKind::VramCopy: rc = pgs_k_vram_copy(&l.copy, stream_); break;
+	case LaunchKind::VramSnapshot: rc = pgs_k_vram_snapshot(&l.snapshot, stream_); break;
+	}
+	if (rc != 0)
+		check(hipError_t(-rc));
+	if (profiling_)
+	{
+		check(hipEventRecord(t.stop, stream_));
+		timed_.push_back(t);
+	}
+	if (recording_)
+		recorded_.push_back(l);
+}
+
+// ---------------------------------------------------------------------------------------
+// Transfer batching.
+//
+// A batch is a run of copies with no read-after-write hazard: a local copy whose source
+// blocks were written by an earlier copy of the batch closes the batch first (the
+// reference flushes on the same condition, page_tracker.cpp:221-245, at 8 KiB page
+// granularity; here the granularity is the 256-byte block). Write-after-write between
+// copies is resolved inside the batch by the per-block ordered kernel, and local copies
+// whose source is overwritten within the batch read a snapshot taken at batch start.
+
+void Renderer::queue_host_copy(const CopyDesc &desc_in, const void *payload, size_t bytes)
+{
+	CopyDesc desc = desc_in;
+	size_t meta = copy_meta_bound(desc);
+	size_t need = align_up(bytes, kAlign) + meta;
+	size_t reserved = 0;
+	for (auto &p : batch_)
+		reserved += copy_meta_bound(p.desc);
+	if (arena_.used + reserved + need > arena_.cap)
+	{
+		if (!batch_.empty())
+			flush_transfers();
+		arena_make_room(need);
+	}
+
+	size_t off;
+	uint8_t *dst = arena_alloc(bytes, off);
+	memcpy(dst, payload, bytes);
+	desc.payload_offset = off;
+	desc.payload_bytes = uint32_t(bytes);
+	desc.flags &= ~COPY_LOCAL;
+
+	for_each_rect_block(desc.dpsm, desc.dbp, desc.dbw, desc.dsax, desc.dsay, desc.w, desc.h,
+	                    [&](uint32_t b, uint32_t, uint32_t) { batch_written_[b >> 6] |= 1ull << (b & 63); });
+	batch_.push_back({ desc });
+}
+
+void Renderer::queue_local_copy(const CopyDesc &desc_in)
+{
+	CopyDesc desc = desc_in;
+	desc.flags |= COPY_LOCAL;
+	desc.payload_bytes = 0;
+	desc.payload_offset = 0;
+	desc.first_pixel = 0;
+	desc.end_pixel = desc.w * desc.h;
+
+	bool raw = false;
+	for_each_rect_block(desc.spsm, desc.sbp, desc.sbw, desc.ssax, desc.ssay, desc.w, desc.h,
+	                    [&](uint32_t b, uint32_t, uint32_t) {
+		                    if (batch_written_[b >> 6] & (1ull << (b & 63)))
+			                    raw = true;
+	                    });
+	if (raw)
+		flush_transfers();
+
+	size_t meta = copy_meta_bound(desc);
+	size_t reserved = 0;
+	for (auto &p : batch_)
+		reserved += copy_meta_bound(p.desc);
+	if (arena_.used + reserved + meta > arena_.cap)
+	{
+		if (!batch_.empty())
+			flush_transfers();
+		arena_make_room(meta);
+	}
+
+	for_each_rect_block(desc.dpsm, desc.dbp, desc.dbw, desc.dsax, desc.dsay, desc.w, desc.h,
+	                    [&](uint32_t b, uint32_t, uint32_t) { batch_written_[b >> 6] |= 1ull << (b & 63); });
+	batch_.push_back({ desc });
+}
+
+void Renderer::flush_transfers()
+{
+	if (batch_.empty())
+		return;
+
+	const uint32_t ncopies = uint32_t(batch_.size());
+
+	// 1. Local copies whose source the batch overwrites read from the batch-start snapshot.
+	std::vector<uint32_t> snapshot;
+	std::vector<uint64_t> snap_mask(kNumBlocks / 64, 0);
+	for (auto &p : batch_)
+	{
+		if (!(p.desc.flags & COPY_LOCAL))
+			continue;
+		bool hazard = false;
+		for_each_rect_block(p.desc.spsm, p.desc.sbp, p.desc.sbw, p.desc.ssax, p.desc.ssay, p.desc.w, p.desc.h,
+		                    [&](uint32_t b, uint32_t, uint32_t) {
+			                    if (batch_written_[b >> 6] & (1ull << (b & 63)))
+				                    hazard = true;
+		                    });
+		if (!hazard)
+			continue;
+		p.desc.flags |= COPY_SOURCE_SHADOW;
+		for_each_rect_block(p.desc.spsm, p.desc.sbp, p.desc.sbw, p.desc.ssax, p.desc.ssay, p.desc.w, p.desc.h,
+		                    [&](uint32_t b, uint32_t, uint32_t) {
+			                    uint64_t bit = 1ull << (b & 63);
+			                    if (!(snap_mask[b >> 6] & bit))
+			                    {
+				                    snap_mask[b >> 6] |= bit;
+				                    snapshot.push_back(b);
+			                    }
+		                    });
+	}
+
+	// 2. Count records per destination block.
+	size_t nrec = 0;
+	for (auto &p : batch_)
+		for_each_rect_block(p.desc.dpsm, p.desc.dbp, p.desc.dbw, p.desc.dsax, p.desc.dsay, p.desc.w, p.desc.h,
+		                    [&](uint32_t b, uint32_t, uint32_t) {
+			                    if (block_count_[b]++ == 0)
+				                    touched_blocks_.push_back(b);
+			                    nrec++;
+		                    });
+	std::sort(touched_blocks_.begin(), touched_blocks_.end());
+
+	size_t off_desc, off_rec, off_blk, off_snap = 0;
+	auto *descs = reinterpret_cast<CopyDesc *>(arena_alloc(ncopies * sizeof(CopyDesc), off_desc));
+	auto *recs = reinterpret_cast<BlockRecord *>(arena_alloc(nrec * sizeof(BlockRecord), off_rec));
+	auto *blks = reinterpret_cast<DestBlock *>(arena_alloc(touched_blocks_.size() * sizeof(DestBlock), off_blk));
+	uint32_t *snap = nullptr;
+	if (!snapshot.empty())
+		snap = reinterpret_cast<uint32_t *>(arena_alloc(snapshot.size() * 4, off_snap));
+
+	// 3. Prefix sum, then scatter records stably (copy order, then raster order).
+	uint32_t run = 0;
+	for (size_t i = 0; i < touched_blocks_.size(); i++)
+	{
+		uint32_t b = touched_blocks_[i];
+		blks[i] = { b, run, block_count_[b], 0 };
+		uint32_t c = block_count_[b];
+		block_count_[b] = run; // reuse as write cursor
+		run += c;
+	}
+	for (uint32_t ci = 0; ci < ncopies; ci++)
+	{
+		const CopyDesc &d = batch_[ci].desc;
+		descs[ci] = d;
+		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h,
+		                    [&](uint32_t b, uint32_t x0, uint32_t y0) {
+			                    recs[block_count_[b]++] = { ci, uint16_t(x0), uint16_t(y0) };
+		                    });
+	}
+	for (uint32_t b : touched_blocks_)
+		block_count_[b] = 0;
+	if (snap)
+		memcpy(snap, snapshot.data(), snapshot.size() * 4);
+
+	arena_upload();
+
+	if (snap)
+	{
+		Launch l{};
+		l.kind = LaunchKind::VramSnapshot;
+		l.snapshot = { vram_, shadow_, reinterpret_cast<const uint32_t *>(arena_.dev + off_snap),
+		               uint32_t(snapshot.size()), 0 };
+		launch(l);
+	}
+
+	Launch l{};
+	l.kind = LaunchKind::VramCopy;
+	l.copy.vram = vram_;
+	l.copy.ref_plane = sample_log2_ ? vram_ + kVramSize : nullptr;
+	l.copy.shadow = shadow_;
+	l.copy.payload = arena_.dev;
+	l.copy.descs = arena_.dev + off_desc;
+	l.copy.records = arena_.dev + off_rec;
+	l.copy.blocks = arena_.dev + off_blk;
+	l.copy.num_blocks = uint32_t(touched_blocks_.size());
+	launch(l);
+
+	stats_.transfer_batches++;
+	stats_.copies += ncopies;
+	stats_.copy_blocks += touched_blocks_.size();
+	stats_.copy_records += nrec;
+	stats_.snapshot_blocks += snapshot.size();
+	for (auto &p : batch_)
+	{
+		uint64_t pixels = (p.desc.flags & COPY_LOCAL) ? uint64_t(p.desc.w) * p.desc.h
+		                                               : uint64_t(p.desc.end_pixel - p.desc.first_pixel);
+		stats_.copy_bytes += 2u * pixels * psm_bpp(p.desc.dpsm) / 8u;
+	}
+
+	touched_blocks_.clear();
+	batch_.clear();
+	std::fill(batch_written_.begin(), batch_written_.end(), 0);
+}
+
+// ---------------------------------------------------------------------------------------
+// Host VRAM access.
+
+void Renderer::read_vram(void *dst, uint32_t first_plane, uint32_t planes)
+{
+	sync();
+	check(hipMemcpy(dst, vram_ + size_t(first_plane) * kVramSize, size_t(planes) * kVramSize,
+	                hipMemcpyDeviceToHost));
+}
+
+void Renderer::write_vram(const void *src, size_t offset, size_t size)
+{
+	sync();
+	check(hipMemcpy(vram_ + offset, src, size, hipMemcpyHostToDevice));
+}
+
+// ---------------------------------------------------------------------------------------
+// Recording and replay.
+
+void Renderer::begin_recording()
+{
+	sync();
+	destroy_graph();
+	recorded_.clear();
+	recording_ = true;
+}
+
+void Renderer::end_recording()
+{
+	if (!batch_.empty())
+		flush_transfers();
+	arena_upload();
+	check(hipStreamSynchronize(stream_));
+	recording_ = false;
+}
+
+void Renderer::destroy_graph()
+{
+	if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+	if (graph_) (void)hipGraphDestroy(graph_);
+	graph_exec_ = nullptr;
+	graph_ = nullptr;
+}
+
+int Renderer::replay(bool use_graph)
+{
+	try
+	{
+		if (!use_graph)
+		{
+			for (auto &l : recorded_)
+				launch(l);
+			return 0;
+		}
+		if (!graph_exec_)
+		{
+			check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+			for (auto &l : recorded_)
+				launch(l);
+			check(hipStreamEndCapture(stream_, &graph_));
+			check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+		}
+		check(hipGraphLaunch(graph_exec_, stream_));
+		return 0;
+	}
+	catch (const std::exception &)
+	{
+		return last_error_ ? last_error_ : -1;
+	}
+}
+}

@@ -1,0 +1,166 @@
+// Device-side renderer of the MI355X GS backend: owns HBM buffers and issues kernels.
+//
+// Counterpart of GSRenderer (gs/gs_renderer.hpp:244-324). The Vulkan command-buffer,
+// descriptor and timeline machinery is replaced by one in-order HIP stream, a pinned
+// upload arena mirrored in device memory, and the kernel-level C ABI (pgs_kernels.h).
+#pragma once
+#include "gs_defs.hpp"
+#include "pgs_kernels.h"
+#include <vector>
+#include <cstring>
+#include <cstddef>
+
+namespace pgs
+{
+struct FlushStats
+{
+	uint64_t transfer_batches = 0;
+	uint64_t copies = 0;
+	uint64_t copy_blocks = 0;
+	uint64_t copy_records = 0;
+	uint64_t snapshot_blocks = 0;
+	uint64_t copy_bytes = 0;       // algorithmic bytes: pixels x bpp/8, read + write
+	uint64_t render_passes = 0;
+	uint64_t primitives = 0;
+	uint64_t syncs = 0;
+};
+
+class Renderer
+{
+public:
+	Renderer() = default;
+	~Renderer();
+	Renderer(const Renderer &) = delete;
+	Renderer &operator=(const Renderer &) = delete;
+
+	// sample_log2: 0 = 1x, else log2 of the sample count; allocates 2 + samples planes.
+	bool init(uint32_t sample_log2, size_t arena_bytes);
+
+	// ---- transfers -------------------------------------------------------------------
+	// Queue a host->local upload. The payload bytes are copied into the upload arena.
+	void queue_host_copy(const CopyDesc &desc, const void *payload, size_t bytes);
+	// Queue a local->local copy.
+	void queue_local_copy(const CopyDesc &desc);
+	// Close the current transfer batch: build the per-block work lists and launch.
+	void flush_transfers();
+	bool has_pending_transfers() const { return !batch_.empty(); }
+
+	// ---- host access ------------------------------------------------------------------
+	void sync();
+	// Copies `planes` * 4 MiB of device VRAM into `dst` (plane 0 first).
+	void read_vram(void *dst, uint32_t first_plane, uint32_t planes);
+	void write_vram(const void *src, size_t offset, size_t size);
+	uint32_t num_planes() const { return num_planes_; }
+	uint32_t sample_log2() const { return sample_log2_; }
+
+	// ---- launch recording for graph replay ----------------------------------------------
+	// While recording, uploads are kept alive and every launch is appended to a list that
+	// replay() re-issues (optionally captured into one hipGraph).
+	void begin_recording();
+	void end_recording();
+	int replay(bool use_graph);
+	size_t recorded_launches() const { return recorded_.size(); }
+
+	// Per-launch HIP event timing: while enabled every launch is bracketed by two events;
+	// profile_collect() synchronizes and returns the summed milliseconds per kernel kind.
+	void profile_enable(bool enable);
+	// kind: 0 = VRAM copy, 1 = snapshot, further kinds are the render kernels.
+	void profile_collect(double *ms, uint64_t *launches, uint32_t kinds);
+
+	hipStream_t stream() const { return stream_; }
+	FlushStats consume_stats() { FlushStats s = stats_; stats_ = {}; return s; }
+	const FlushStats &stats() const { return stats_; }
+	int last_error() const { return last_error_; }
+
+	uint8_t *vram_device() { return vram_; }
+
+private:
+	struct Arena
+	{
+		uint8_t *host = nullptr;
+		uint8_t *dev = nullptr;
+		size_t cap = 0;
+		size_t used = 0;      // bytes handed out
+		size_t uploaded = 0;  // bytes already copied to the device
+	};
+
+	enum class LaunchKind : uint32_t { VramCopy, VramSnapshot };
+	struct Launch
+	{
+		LaunchKind kind;
+		union
+		{
+			pgs_vram_copy_args copy;
+			pgs_vram_snapshot_args snapshot;
+		};
+	};
+
+	struct PendingCopy
+	{
+		CopyDesc desc;
+	};
+
+	uint32_t sample_log2_ = 0;
+	uint32_t num_planes_ = 1;
+	uint8_t *vram_ = nullptr;    // num_planes_ * 4 MiB
+	uint8_t *shadow_ = nullptr;  // 4 MiB
+	hipStream_t stream_ = nullptr;
+	int last_error_ = 0;
+
+	Arena arena_;
+	std::vector<Arena> retired_arenas_; // kept alive while recording
+	bool recording_ = false;
+	std::vector<Launch> recorded_;
+	hipGraph_t graph_ = nullptr;
+	hipGraphExec_t graph_exec_ = nullptr;
+
+	struct TimedLaunch { uint32_t kind; hipEvent_t start, stop; };
+	bool profiling_ = false;
+	std::vector<TimedLaunch> timed_;
+	std::vector<hipEvent_t> event_pool_;
+	hipEvent_t get_event();
+
+	// Current transfer batch.
+	std::vector<PendingCopy> batch_;
+	std::vector<uint64_t> batch_written_;   // bitset over 16384 blocks
+	std::vector<uint32_t> block_count_;     // per-block record counts (scratch)
+	std::vector<uint32_t> touched_blocks_;  // blocks with nonzero count
+
+	FlushStats stats_;
+
+	void check(hipError_t err);
+	void launch(const Launch &l);
+	// Bump-allocates `bytes` in the arena; returns the host pointer and device offset.
+	uint8_t *arena_alloc(size_t bytes, size_t &dev_offset);
+	void arena_make_room(size_t bytes);
+	void arena_upload();
+	void arena_reset_after_sync();
+	void destroy_graph();
+};
+
+// Calls f(block, x0, y0) for every 256-byte block a w x h rectangle at (sax, say) of a
+// buffer (base, stride, psm) touches, in raster order of block origins. Coordinates wrap
+// at 2048 as the transmission area does (vram_copy.comp:302-306).
+template <typename F>
+inline void for_each_rect_block(uint32_t psm, uint32_t base, uint32_t stride, uint32_t sax, uint32_t say,
+                                uint32_t w, uint32_t h, F &&f)
+{
+	const Geometry g = psm_geometry(psm);
+	const uint32_t bwid = 1u << g.bw, bhei = 1u << g.bh;
+	const bool nibble = psm_storage_class(psm) == SC_4;
+	for (uint32_t ry = 0; ry < h;)
+	{
+		uint32_t y = (say + ry) & 2047u;
+		uint32_t y0 = y & ~(bhei - 1u);
+		for (uint32_t rx = 0; rx < w;)
+		{
+			uint32_t x = (sax + rx) & 2047u;
+			uint32_t x0 = x & ~(bwid - 1u);
+			uint32_t addr = swizzle_addr(x0, y0, base, stride, psm);
+			f(nibble ? (addr >> 9) : (addr >> 8), x0, y0);
+			rx += x0 + bwid - x;
+		}
+		ry += y0 + bhei - y;
+	}
+}
+}

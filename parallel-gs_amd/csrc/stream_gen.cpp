@@ -1,0 +1,255 @@
+// Synthetic GS command-stream generator for the benchmark workloads and the parity tests.
+//
+// It plays the role of the reference's dump generator (dump/gs_dump_generator.cpp:104-185):
+// it emits plain GIF packets (PACKED A+D register writes, PACKED vertex data and IMAGE
+// payloads) that any GSInterface implementation consumes byte for byte.
+//
+// PRNG: xoshiro256** seeded through SplitMix64 (Blackman & Vigna), so the CPU oracle and the
+// HIP path read identical bytes for a given seed.
+#include <cstdint>
+#include <cstring>
+#include <cstdlib>
+#include <vector>
+#include <cmath>
+#include <algorithm>
+
+namespace
+{
+struct Rng
+{
+	uint64_t s[4];
+	explicit Rng(uint64_t seed)
+	{
+		for (auto &v : s)
+		{
+			seed += 0x9e3779b97f4a7c15ull;
+			uint64_t z = seed;
+			z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+			z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+			v = z ^ (z >> 31);
+		}
+	}
+	static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+	uint64_t next()
+	{
+		uint64_t r = rotl(s[1] * 5, 7) * 9;
+		uint64_t t = s[1] << 17;
+		s[2] ^= s[0];
+		s[3] ^= s[1];
+		s[1] ^= s[2];
+		s[0] ^= s[3];
+		s[2] ^= t;
+		s[3] = rotl(s[3], 45);
+		return r;
+	}
+	// Uniform integer in [lo, hi].
+	uint32_t range(uint32_t lo, uint32_t hi) { return lo + uint32_t(next() % (uint64_t(hi) - lo + 1u)); }
+	// Uniform float in [0, 1).
+	double unit() { return double(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+struct Stream
+{
+	std::vector<uint64_t> q; // pairs of 64-bit words = 128-bit GIF qwords
+
+	void tag(uint32_t nloop, bool eop, bool pre, uint32_t prim, uint32_t flg, uint32_t nreg, uint64_t regs)
+	{
+		uint64_t lo = uint64_t(nloop & 0x7fff) | (uint64_t(eop) << 15) | (uint64_t(pre) << 46) |
+		              (uint64_t(prim & 0x7ff) << 47) | (uint64_t(flg & 3) << 58) | (uint64_t(nreg & 0xf) << 60);
+		q.push_back(lo);
+		q.push_back(regs);
+	}
+
+	// A list of A+D register writes in one PACKED GIFtag.
+	void ad(const std::vector<std::pair<uint32_t, uint64_t>> &w)
+	{
+		tag(uint32_t(w.size()), false, false, 0, 0, 1, 0xe);
+		for (auto &r : w)
+		{
+			q.push_back(r.second);
+			q.push_back(r.first);
+		}
+	}
+
+	void image(const uint8_t *bytes, size_t nbytes)
+	{
+		// nbytes is a multiple of 16. IMAGE NLOOP is 15 bits, so split long payloads.
+		size_t qwords = nbytes / 16;
+		size_t off = 0;
+		while (off < qwords)
+		{
+			size_t n = std::min<size_t>(qwords - off, 32767);
+			tag(uint32_t(n), false, false, 0, 2, 0, 0);
+			size_t base = q.size();
+			q.resize(base + n * 2);
+			memcpy(q.data() + base, bytes + off * 16, n * 16);
+			off += n;
+		}
+	}
+};
+
+enum : uint32_t
+{
+	CT32 = 0x00, CT24 = 0x01, CT16 = 0x02, CT16S = 0x0a,
+	T8 = 0x13, T4 = 0x14, T8H = 0x1b, T4HL = 0x24, T4HH = 0x2c,
+	Z32 = 0x30, Z24 = 0x31, Z16 = 0x32, Z16S = 0x3a
+};
+
+uint32_t bpp_of(uint32_t psm)
+{
+	switch (psm)
+	{
+	case CT24: case Z24: return 24;
+	case CT16: case CT16S: case Z16: case Z16S: return 16;
+	case T8: case T8H: return 8;
+	case T4: case T4HL: case T4HH: return 4;
+	default: return 32;
+	}
+}
+
+uint64_t bitbltbuf(uint32_t sbp, uint32_t sbw, uint32_t spsm, uint32_t dbp, uint32_t dbw, uint32_t dpsm)
+{
+	return uint64_t(sbp) | uint64_t(sbw) << 16 | uint64_t(spsm) << 24 |
+	       uint64_t(dbp) << 32 | uint64_t(dbw) << 48 | uint64_t(dpsm) << 56;
+}
+
+uint64_t trxpos(uint32_t ssax, uint32_t ssay, uint32_t dsax, uint32_t dsay)
+{
+	return uint64_t(ssax) | uint64_t(ssay) << 16 | uint64_t(dsax) << 32 | uint64_t(dsay) << 48;
+}
+
+uint64_t trxreg(uint32_t w, uint32_t h) { return uint64_t(w) | uint64_t(h) << 32; }
+
+std::vector<uint8_t> *g_last = nullptr;
+
+const uint8_t *publish(Stream &s, size_t *size)
+{
+	delete g_last;
+	g_last = new std::vector<uint8_t>(s.q.size() * 8);
+	memcpy(g_last->data(), s.q.data(), g_last->size());
+	*size = g_last->size();
+	return g_last->data();
+}
+}
+
+extern "C"
+{
+// VRAM transfer microbenchmark: n_h2l host->local uploads cycling through 12 PSMs with
+// random w,h in 8..max_dim (multiples of 8), interleaved with one local->local copy after
+// every `h2l_per_l2l` uploads (same PSM, page-overlapping source and destination).
+// Returns a pointer to the stream (valid until the next generator call).
+const uint8_t *pgs_gen_transfers(uint64_t seed, uint32_t n_h2l, uint32_t n_l2l, uint32_t max_dim, size_t *size)
+{
+	static const uint32_t psms[12] = { CT32, CT24, CT16, CT16S, T8, T4, T8H, T4HL, T4HH, Z32, Z24, Z16 };
+	Rng rng(seed);
+	Stream s;
+	std::vector<uint8_t> payload;
+	uint32_t h2l_per_l2l = n_l2l ? std::max<uint32_t>(1, n_h2l / n_l2l) : 0;
+	uint32_t l2l_done = 0;
+
+	for (uint32_t i = 0; i < n_h2l; i++)
+	{
+		uint32_t psm = psms[i % 12];
+		uint32_t w = rng.range(1, max_dim / 8) * 8;
+		uint32_t h = rng.range(1, max_dim / 8) * 8;
+		uint32_t dsax = rng.range(0, (2048 - w) / 8) * 8;
+		uint32_t dsay = rng.range(0, (2048 - h) / 8) * 8;
+		uint32_t min_bw = (dsax + w + 63) / 64;
+		if ((psm == T8 || psm == T4) && (min_bw & 1))
+			min_bw++;
+		uint32_t dbw = std::min<uint32_t>(min_bw + rng.range(0, 3), 62);
+		if ((psm == T8 || psm == T4) && (dbw & 1))
+			dbw++;
+		uint32_t dbp = rng.range(0, 16383);
+
+		s.ad({ { 0x50, bitbltbuf(0, 0, 0, dbp, dbw, psm) },
+		       { 0x51, trxpos(0, 0, dsax, dsay) },
+		       { 0x52, trxreg(w, h) },
+		       { 0x53, 0 } });
+
+		size_t nbytes = size_t(w) * h * bpp_of(psm) / 8;
+		payload.resize(nbytes);
+		for (size_t b = 0; b < nbytes; b += 8)
+		{
+			uint64_t r = rng.next();
+			memcpy(payload.data() + b, &r, std::min<size_t>(8, nbytes - b));
+		}
+		s.image(payload.data(), nbytes);
+
+		if (h2l_per_l2l && (i + 1) % h2l_per_l2l == 0 && l2l_done < n_l2l)
+		{
+			l2l_done++;
+			uint32_t lpsm = psms[rng.range(0, 11)];
+			uint32_t lw = rng.range(1, max_dim / 16) * 8;
+			uint32_t lh = rng.range(1, max_dim / 16) * 8;
+			uint32_t ssax = rng.range(8, (2048 - lw - 72) / 8) * 8;
+			uint32_t ssay = rng.range(8, (2048 - lh - 72) / 8) * 8;
+			uint32_t ldsax = ssax + rng.range(0, 16) * 8 - 64;
+			uint32_t ldsay = ssay + rng.range(0, 16) * 8 - 64;
+			uint32_t bw = (std::max(ssax, ldsax) + lw + 63) / 64;
+			if ((lpsm == T8 || lpsm == T4) && (bw & 1))
+				bw++;
+			uint32_t bp = rng.range(0, 16383);
+			s.ad({ { 0x50, bitbltbuf(bp, bw, lpsm, bp, bw, lpsm) },
+			       { 0x51, trxpos(ssax, ssay, ldsax, ldsay) },
+			       { 0x52, trxreg(lw, lh) },
+			       { 0x53, 2 } });
+		}
+	}
+	return publish(s, size);
+}
+
+// Random uploads with fully arbitrary geometry (any w/h, unaligned origins, wrap-around,
+// mixed source/destination PSMs for local copies, strides narrower than the rectangle).
+// Used by the parity tests for edge cases.
+const uint8_t *pgs_gen_transfers_wild(uint64_t seed, uint32_t n, size_t *size)
+{
+	static const uint32_t psms[13] = { CT32, CT24, CT16, CT16S, T8, T4, T8H, T4HL, T4HH, Z32, Z24, Z16, Z16S };
+	Rng rng(seed);
+	Stream s;
+	std::vector<uint8_t> payload;
+	for (uint32_t i = 0; i < n; i++)
+	{
+		uint32_t dpsm = psms[rng.range(0, 12)];
+		uint32_t w = rng.range(1, 80);
+		uint32_t h = rng.range(1, 80);
+		uint32_t dsax = rng.range(0, 2047);
+		uint32_t dsay = rng.range(0, 2047);
+		uint32_t dbw = rng.range(0, 12);
+		uint32_t dbp = rng.range(0, 16383);
+		bool local = rng.range(0, 3) == 0;
+		if (local)
+		{
+			uint32_t spsm = rng.range(0, 1) ? dpsm : psms[rng.range(0, 12)];
+			s.ad({ { 0x50, bitbltbuf(rng.range(0, 16383), rng.range(0, 12), spsm, dbp, dbw, dpsm) },
+			       { 0x51, trxpos(rng.range(0, 2047), rng.range(0, 2047), dsax, dsay) },
+			       { 0x52, trxreg(w, h) },
+			       { 0x53, 2 } });
+			continue;
+		}
+
+		s.ad({ { 0x50, bitbltbuf(0, 0, 0, dbp, dbw, dpsm) },
+		       { 0x51, trxpos(0, 0, dsax, dsay) },
+		       { 0x52, trxreg(w, h) },
+		       { 0x53, 0 } });
+		size_t nbits = size_t(w) * h * bpp_of(dpsm);
+		size_t nbytes = ((nbits / 64) * 8 + 15) & ~size_t(15); // payload rounded up to 128 bits
+		// Sometimes send the transfer in two IMAGE packets, sometimes leave it short.
+		uint32_t mode = rng.range(0, 5);
+		if (mode == 0 && nbytes >= 32)
+			nbytes -= 16;
+		payload.resize(nbytes);
+		for (size_t b = 0; b < nbytes; b++)
+			payload[b] = uint8_t(rng.next());
+		if (mode == 1 && nbytes >= 32)
+		{
+			size_t half = (nbytes / 32) * 16;
+			s.image(payload.data(), half);
+			s.image(payload.data() + half, nbytes - half);
+		}
+		else if (nbytes)
+			s.image(payload.data(), nbytes);
+	}
+	return publish(s, size);
+}
+}

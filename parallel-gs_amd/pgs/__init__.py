@@ -1,0 +1,263 @@
+"""Python bindings (ctypes) for the MI355X GS backend.
+
+The product is the C-ABI shared library ``libpgs_amd.so`` (include/pgs_amd.h and
+include/pgs_kernels.h). This module only loads it and wraps the context-level calls, the
+same way a ctypes integration of the reference's GSInterface would. There is no fallback:
+if the library is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libpgs_amd.so")
+STREAMS_PATH = os.path.join(PKG_DIR, "libpgs_streams.so")
+
+VRAM_SIZE = 4 * 1024 * 1024
+
+PSM = {
+    "CT32": 0x00, "CT24": 0x01, "CT16": 0x02, "CT16S": 0x0A,
+    "T8": 0x13, "T4": 0x14, "T8H": 0x1B, "T4HL": 0x24, "T4HH": 0x2C,
+    "Z32": 0x30, "Z24": 0x31, "Z16": 0x32, "Z16S": 0x3A,
+}
+
+
+class PgsError(RuntimeError):
+    pass
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [
+        ("super_sampling", ctypes.c_uint32),
+        ("ordered_super_sampling", ctypes.c_uint32),
+        ("super_sampled_textures", ctypes.c_uint32),
+        ("device_index", ctypes.c_uint32),
+        ("upload_arena_bytes", ctypes.c_uint64),
+    ]
+
+
+class FlushStats(ctypes.Structure):
+    _fields_ = [
+        ("transfer_batches", ctypes.c_uint64),
+        ("copies", ctypes.c_uint64),
+        ("copy_blocks", ctypes.c_uint64),
+        ("copy_records", ctypes.c_uint64),
+        ("snapshot_blocks", ctypes.c_uint64),
+        ("copy_bytes", ctypes.c_uint64),
+        ("render_passes", ctypes.c_uint64),
+        ("primitives", ctypes.c_uint64),
+        ("syncs", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+_lib: Optional[ctypes.CDLL] = None
+_streams: Optional[ctypes.CDLL] = None
+
+# Every symbol declared in include/pgs_amd.h and include/pgs_kernels.h.
+EXPORTED_SYMBOLS = [
+    "pgs_create", "pgs_destroy", "pgs_reset_context_state", "pgs_gif_transfer",
+    "pgs_write_register", "pgs_flush", "pgs_map_vram_read", "pgs_map_vram_write",
+    "pgs_end_vram_write", "pgs_read_transfer_fifo", "pgs_clobber_register_state",
+    "pgs_consume_flush_stats", "pgs_num_planes", "pgs_read_vram_planes",
+    "pgs_begin_recording", "pgs_end_recording", "pgs_replay", "pgs_sync", "pgs_stream",
+    "pgs_profile_enable", "pgs_profile_collect",
+    "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
+    "pgs_k_vram_copy", "pgs_k_vram_snapshot",
+]
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libpgs_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PgsError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, u32, u64, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+        L.pgs_create.restype = vp
+        L.pgs_create.argtypes = [ctypes.POINTER(_Options)]
+        L.pgs_destroy.argtypes = [vp]
+        L.pgs_destroy.restype = None
+        L.pgs_reset_context_state.argtypes = [vp]
+        L.pgs_gif_transfer.argtypes = [vp, u32, vp, sz]
+        L.pgs_write_register.argtypes = [vp, u32, u64]
+        L.pgs_flush.argtypes = [vp]
+        L.pgs_map_vram_read.argtypes = [vp, sz, sz]
+        L.pgs_map_vram_read.restype = vp
+        L.pgs_map_vram_write.argtypes = [vp, sz, sz]
+        L.pgs_map_vram_write.restype = vp
+        L.pgs_end_vram_write.argtypes = [vp, sz, sz]
+        L.pgs_read_transfer_fifo.argtypes = [vp, vp, u32]
+        L.pgs_clobber_register_state.argtypes = [vp]
+        L.pgs_consume_flush_stats.argtypes = [vp, ctypes.POINTER(FlushStats)]
+        L.pgs_num_planes.argtypes = [vp]
+        L.pgs_num_planes.restype = u32
+        L.pgs_read_vram_planes.argtypes = [vp, vp]
+        L.pgs_begin_recording.argtypes = [vp]
+        L.pgs_end_recording.argtypes = [vp]
+        L.pgs_replay.argtypes = [vp, i32]
+        L.pgs_sync.argtypes = [vp]
+        L.pgs_profile_enable.argtypes = [vp, i32]
+        L.pgs_profile_collect.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64), u32]
+        L.pgs_stream.argtypes = [vp]
+        L.pgs_stream.restype = vp
+        L.pgs_host_swizzle_addr.argtypes = [u32, u32, u32, u32, u32]
+        L.pgs_host_swizzle_addr.restype = u32
+        L.pgs_host_block_lane_pixel.argtypes = [u32, u32, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+        L.pgs_host_block_lane_pixel.restype = None
+        _lib = L
+    return _lib
+
+
+def streams() -> ctypes.CDLL:
+    """Loads the synthetic stream generator library."""
+    global _streams
+    if _streams is None:
+        if not os.path.exists(STREAMS_PATH):
+            raise PgsError(f"{STREAMS_PATH} is missing: run __graft_entry__.build() first")
+        S = ctypes.CDLL(STREAMS_PATH)
+        S.pgs_gen_transfers.restype = ctypes.c_void_p
+        S.pgs_gen_transfers.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_size_t)]
+        S.pgs_gen_transfers_wild.restype = ctypes.c_void_p
+        S.pgs_gen_transfers_wild.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)]
+        _streams = S
+    return _streams
+
+
+def _copy_stream(ptr, size) -> np.ndarray:
+    buf = (ctypes.c_uint8 * size.value).from_address(ptr)
+    return np.frombuffer(buf, dtype=np.uint8).copy()
+
+
+def gen_transfers(seed: int, n_h2l: int, n_l2l: int, max_dim: int = 512) -> np.ndarray:
+    """GIF stream of the VRAM transfer microbenchmark (host->local + local->local)."""
+    size = ctypes.c_size_t()
+    ptr = streams().pgs_gen_transfers(seed, n_h2l, n_l2l, max_dim, ctypes.byref(size))
+    return _copy_stream(ptr, size)
+
+
+def gen_transfers_wild(seed: int, n: int) -> np.ndarray:
+    """GIF stream of small transfers with arbitrary geometry (edge cases)."""
+    size = ctypes.c_size_t()
+    ptr = streams().pgs_gen_transfers_wild(seed, n, ctypes.byref(size))
+    return _copy_stream(ptr, size)
+
+
+class Context:
+    """One GS context (GSInterface) on a HIP device."""
+
+    def __init__(self, super_sampling: int = 1, device_index: int = 0, arena_bytes: int = 0,
+                 ordered: bool = True, super_sampled_textures: bool = False):
+        opts = _Options(super_sampling, int(ordered), int(super_sampled_textures), device_index, arena_bytes)
+        self._L = lib()
+        self._h = self._L.pgs_create(ctypes.byref(opts))
+        if not self._h:
+            raise PgsError("pgs_create failed (no HIP device or out of memory)")
+
+    def close(self):
+        if self._h:
+            self._L.pgs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise PgsError(f"{what} failed with {rc}")
+
+    def gif_transfer(self, data: np.ndarray, path: int = 3):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self._check(self._L.pgs_gif_transfer(self._h, path, data.ctypes.data, data.nbytes), "gif_transfer")
+
+    def write_register(self, addr: int, value: int):
+        self._check(self._L.pgs_write_register(self._h, addr, value & ((1 << 64) - 1)), "write_register")
+
+    def flush(self):
+        self._check(self._L.pgs_flush(self._h), "flush")
+
+    def sync(self):
+        self._check(self._L.pgs_sync(self._h), "sync")
+
+    @property
+    def num_planes(self) -> int:
+        return int(self._L.pgs_num_planes(self._h))
+
+    def vram(self) -> np.ndarray:
+        """Plane 0 (the 4 MiB GS local memory) as uint8."""
+        p = self._L.pgs_map_vram_read(self._h, 0, VRAM_SIZE)
+        if not p:
+            raise PgsError("map_vram_read failed")
+        return np.frombuffer((ctypes.c_uint8 * VRAM_SIZE).from_address(p), dtype=np.uint8).copy()
+
+    def vram_planes(self) -> np.ndarray:
+        out = np.empty(self.num_planes * VRAM_SIZE, dtype=np.uint8)
+        self._check(self._L.pgs_read_vram_planes(self._h, out.ctypes.data), "read_vram_planes")
+        return out
+
+    def write_vram(self, data: np.ndarray, offset: int = 0):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        p = self._L.pgs_map_vram_write(self._h, offset, data.nbytes)
+        if not p:
+            raise PgsError("map_vram_write failed")
+        ctypes.memmove(p, data.ctypes.data, data.nbytes)
+        self._check(self._L.pgs_end_vram_write(self._h, offset, data.nbytes), "end_vram_write")
+
+    def read_transfer_fifo(self, n128: int) -> np.ndarray:
+        out = np.zeros(n128 * 16, dtype=np.uint8)
+        self._check(self._L.pgs_read_transfer_fifo(self._h, out.ctypes.data, n128), "read_transfer_fifo")
+        return out
+
+    def consume_flush_stats(self) -> dict:
+        s = FlushStats()
+        self._check(self._L.pgs_consume_flush_stats(self._h, ctypes.byref(s)), "consume_flush_stats")
+        return s.as_dict()
+
+    def begin_recording(self):
+        self._check(self._L.pgs_begin_recording(self._h), "begin_recording")
+
+    def end_recording(self):
+        self._check(self._L.pgs_end_recording(self._h), "end_recording")
+
+    def replay(self, use_graph: bool = True):
+        self._check(self._L.pgs_replay(self._h, int(use_graph)), "replay")
+
+    def profile_enable(self, enable: bool = True):
+        self._check(self._L.pgs_profile_enable(self._h, int(enable)), "profile_enable")
+
+    def profile_collect(self, kinds: int = 8):
+        ms = (ctypes.c_double * kinds)()
+        n = (ctypes.c_uint64 * kinds)()
+        self._check(self._L.pgs_profile_collect(self._h, ms, n, kinds), "profile_collect")
+        return list(ms), list(n)
+
+    @property
+    def stream(self) -> int:
+        return int(self._L.pgs_stream(self._h) or 0)
+
+
+def host_swizzle_addr(x: int, y: int, base: int, stride: int, psm: int) -> int:
+    return int(lib().pgs_host_swizzle_addr(x, y, base, stride, psm))
+
+
+def host_block_lane_pixel(psm: int, lane: int, k: int):
+    dx, dy = ctypes.c_uint32(), ctypes.c_uint32()
+    lib().pgs_host_block_lane_pixel(psm, lane, k, ctypes.byref(dx), ctypes.byref(dy))
+    return int(dx.value), int(dy.value)

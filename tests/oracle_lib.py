@@ -1,0 +1,90 @@
+"""ctypes loader for the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: the oracle is the checker the HIP path is compared against.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+VRAM_SIZE = 4 * 1024 * 1024
+
+_L = None
+
+
+def oracle_lib() -> ctypes.CDLL:
+    global _L
+    if _L is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"{ORACLE_PATH} missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(ORACLE_PATH)
+        vp, sz, u32, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint64
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [u32]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_destroy.restype = None
+        L.orc_gif_transfer.argtypes = [vp, u32, vp, sz]
+        L.orc_gif_transfer.restype = None
+        L.orc_write_register.argtypes = [vp, u32, u64]
+        L.orc_write_register.restype = None
+        L.orc_flush.argtypes = [vp]
+        L.orc_flush.restype = None
+        L.orc_vram.argtypes = [vp]
+        L.orc_vram.restype = vp
+        L.orc_vram_size.argtypes = [vp]
+        L.orc_vram_size.restype = sz
+        L.orc_set_vram.argtypes = [vp, vp, sz]
+        L.orc_set_vram.restype = None
+        L.orc_swizzle.argtypes = [u32, u32, u32, u32, u32]
+        L.orc_swizzle.restype = u32
+        L.orc_stat.argtypes = [vp, u32]
+        L.orc_stat.restype = u64
+        _L = L
+    return _L
+
+
+class Oracle:
+    """Immediate-mode CPU GS (one context)."""
+
+    def __init__(self, sample_log2: int = 0):
+        self._L = oracle_lib()
+        self._h = self._L.orc_create(sample_log2)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.orc_destroy(self._h)
+            self._h = None
+
+    def gif_transfer(self, data: np.ndarray, path: int = 3):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self._L.orc_gif_transfer(self._h, path, data.ctypes.data, data.nbytes)
+
+    def write_register(self, addr: int, value: int):
+        self._L.orc_write_register(self._h, addr, value & ((1 << 64) - 1))
+
+    def flush(self):
+        self._L.orc_flush(self._h)
+
+    def planes(self) -> np.ndarray:
+        n = self._L.orc_vram_size(self._h)
+        p = self._L.orc_vram(self._h)
+        return np.frombuffer((ctypes.c_uint8 * n).from_address(p), dtype=np.uint8).copy()
+
+    def vram(self) -> np.ndarray:
+        self.flush()
+        return self.planes()[:VRAM_SIZE]
+
+    def set_vram(self, data: np.ndarray):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self._L.orc_set_vram(self._h, data.ctypes.data, data.nbytes)
+
+    def stat(self, which: int) -> int:
+        return int(self._L.orc_stat(self._h, which))
+
+
+def swizzle(x, y, base, stride, psm) -> int:
+    return int(oracle_lib().orc_swizzle(x, y, base, stride, psm))
