@@ -50,6 +50,79 @@ typedef struct pgs_vram_snapshot_args
 
 int pgs_k_vram_snapshot(const pgs_vram_snapshot_args *args, hipStream_t stream);
 
+/* Uploads the reciprocal lookup tables the raster kernels read (the 256-entry fixed-point
+ * seed table and the 256-entry float seed LUT). Replaces GSRenderer::init_luts
+ * (gs/gs_renderer.cpp:3600-3660). Must run once per device before any setup launch. */
+int pgs_k_init_tables(const uint8_t *rcp_seed, const float *rcp_float_lut);
+
+/* Triangle / sprite / line / point setup: one thread per primitive.
+ * Replaces triangle_setup.comp dispatched by GSRenderer::dispatch_triangle_setup
+ * (gs/gs_renderer.cpp:1898-1977). Writes PrimitiveSetup (80 B) and TransformedAttributes
+ * (64 B) per primitive and updates the state / tex words of the primitive records in place. */
+typedef struct pgs_setup_args
+{
+	const void *positions;  /* VertexPosition[3 * num_primitives] */
+	const void *attributes; /* VertexAttribute[3 * num_primitives] */
+	void *prims;            /* PrimitiveAttribute[num_primitives], updated in place */
+	void *setup;            /* PrimitiveSetup[num_primitives] */
+	void *tattr;            /* TransformedAttributes[num_primitives] */
+	uint32_t num_primitives;
+	uint32_t rate_log2;      /* log2 of the super-sampling grid per axis (0 at 1x) */
+	uint32_t opaque_fbmask;  /* FBMSK every primitive of the pass shares */
+	uint32_t pad;
+} pgs_setup_args;
+
+int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stream);
+
+/* Coarse binning: for each coarse tile, the ascending list of primitives whose conservative
+ * edge test overlaps the tile. Replaces binning.comp dispatched by GSRenderer::dispatch_binning
+ * (gs/gs_renderer.cpp:2068-2145). One 256-thread workgroup per coarse tile. */
+typedef struct pgs_bin_args
+{
+	const void *setup;
+	const void *prims;
+	uint16_t *lists;        /* tiles * list_stride entries */
+	uint32_t *counts;       /* per tile */
+	uint32_t num_primitives;
+	uint32_t list_stride;
+	int32_t origin_x, origin_y;
+	uint32_t tile_log2;
+	uint32_t tiles_x, tiles_y;
+	uint32_t pad;
+} pgs_bin_args;
+
+int pgs_k_binning(const pgs_bin_args *args, hipStream_t stream);
+
+/* Shading: one wave64 per 8x8 block of samples; primitives of the block's coarse tile are
+ * applied to every sample in submission order (coverage, Z, texture, combiner, fog, alpha
+ * test, blend, dither, FBMSK) and the results are written back once.
+ * Replaces ubershader.comp dispatched by GSRenderer::dispatch_shading
+ * (gs/gs_renderer.cpp:2489-2682, 2322-2416). */
+typedef struct pgs_shade_args
+{
+	uint8_t *vram;
+	const void *setup;
+	const void *tattr;
+	const void *prims;
+	const void *states;      /* StateVector[] */
+	const uint16_t *lists;
+	const uint32_t *counts;
+	const void *textures;    /* TextureDesc[] */
+	const uint32_t *tex_images;
+	uint32_t list_stride;
+	int32_t origin_x, origin_y;
+	uint32_t tile_log2;
+	uint32_t tiles_x, tiles_y;
+	uint32_t blocks_x, blocks_y;
+	uint32_t fb_psm, fbp, fbw;
+	uint32_t z_psm;          /* 0xffffffff: the pass does not read or write depth */
+	uint32_t zbp;
+	uint32_t rate_log2;
+	uint32_t pad[2];
+} pgs_shade_args;
+
+int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

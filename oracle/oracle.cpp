@@ -242,15 +242,28 @@ void GS::reset_registers()
 	xfer = {};
 }
 
-// A+D register writes (gs/shaders/gs_interface.cpp:3507-3986 restated; only the state is
-// recorded, drawing is handled in oracle_render.cpp).
+void GS::set_reg(uint64_t &reg, uint64_t v, bool marks_dirty)
+{
+	if (reg != v)
+	{
+		reg = v;
+		if (marks_dirty)
+			state_dirty = true;
+	}
+}
+
+// A+D register writes (gs/gs_interface.cpp:3507-3986). Registers that feed primitive state
+// mark the state dirty when their value changes (update_internal_register,
+// gs_interface.cpp:3877-3885); a dirty state blocks parallelogram fusion of the next triangle.
 void GS::write_register(uint32_t addr, uint64_t v)
 {
 	switch (addr)
 	{
-	case 0x00: regs.prim = v; vq_count = 0; break;
-	case 0x01: regs.rgbaq = uint32_t(v); { uint32_t q = uint32_t(v >> 32); memcpy(&regs.rgbaq_q, &q, 4); } break;
-	case 0x11: regs.rgbaq = uint32_t(v); { uint32_t q = uint32_t(v >> 32); memcpy(&regs.rgbaq_q, &q, 4); } break;
+	case 0x00: set_reg(regs.prim, v & 0x7ffu, true); vq_count = 0; break;
+	case 0x01: case 0x11:
+		regs.rgbaq = uint32_t(v);
+		{ uint32_t q = uint32_t(v >> 32); memcpy(&regs.rgbaq_q, &q, 4); }
+		break;
 	case 0x02: { uint32_t s = uint32_t(v), t = uint32_t(v >> 32); memcpy(&regs.s, &s, 4); memcpy(&regs.t, &t, 4); } break;
 	case 0x03: regs.uv = v; break;
 	case 0x04: kick_xyzf(v, false); break;
@@ -258,27 +271,29 @@ void GS::write_register(uint32_t addr, uint64_t v)
 	case 0x0c: kick_xyzf(v, true); break;
 	case 0x0d: kick_xyz(v, true); break;
 	case 0x06: case 0x07: write_tex0(addr - 0x06, v); break;
-	case 0x08: case 0x09: regs.clamp[addr - 0x08] = v; break;
+	case 0x08: case 0x09: set_reg(regs.clamp[addr - 0x08], v, true); break;
 	case 0x0a: regs.fog = uint32_t(v >> 56); break;
-	case 0x14: case 0x15: regs.tex1[addr - 0x14] = v; break;
+	case 0x14: case 0x15: set_reg(regs.tex1[addr - 0x14], v, true); break;
 	case 0x16: case 0x17: write_tex2(addr - 0x16, v); break;
 	case 0x18: case 0x19: regs.xyoffset[addr - 0x18] = v; break;
 	case 0x1a: regs.prmodecont = v; break;
 	case 0x1b: regs.prmode = v; break;
 	case 0x1c: regs.texclut = v; break;
 	case 0x22: regs.scanmsk = v; break;
-	case 0x3b: regs.texa = v; break;
+	case 0x34: case 0x35: regs.miptbp1[addr - 0x34] = v; break;
+	case 0x36: case 0x37: regs.miptbp2[addr - 0x36] = v; break;
+	case 0x3b: set_reg(regs.texa, v, true); break;
 	case 0x3d: regs.fogcol = v; break;
 	case 0x40: case 0x41: regs.scissor[addr - 0x40] = v; break;
-	case 0x42: case 0x43: regs.alpha[addr - 0x42] = v; break;
-	case 0x44: regs.dimx = v; break;
-	case 0x45: regs.dthe = v; break;
-	case 0x46: regs.colclamp = v; break;
-	case 0x47: case 0x48: regs.test[addr - 0x47] = v; break;
-	case 0x49: regs.pabe = v; break;
-	case 0x4a: case 0x4b: regs.fba[addr - 0x4a] = v; break;
-	case 0x4c: case 0x4d: regs.frame[addr - 0x4c] = v; break;
-	case 0x4e: case 0x4f: regs.zbuf[addr - 0x4e] = v; break;
+	case 0x42: case 0x43: set_reg(regs.alpha[addr - 0x42], v, true); break;
+	case 0x44: set_reg(regs.dimx, v, true); break;
+	case 0x45: set_reg(regs.dthe, v, true); break;
+	case 0x46: set_reg(regs.colclamp, v, true); break;
+	case 0x47: case 0x48: set_reg(regs.test[addr - 0x47], v, true); break;
+	case 0x49: set_reg(regs.pabe, v, true); break;
+	case 0x4a: case 0x4b: set_reg(regs.fba[addr - 0x4a], v, true); break;
+	case 0x4c: case 0x4d: set_reg(regs.frame[addr - 0x4c], v, true); break;
+	case 0x4e: case 0x4f: set_reg(regs.zbuf[addr - 0x4e], v, true); break;
 	case 0x50: regs.bitbltbuf = v; break;
 	case 0x51: regs.trxpos = v; break;
 	case 0x52: regs.trxreg = v; break;
@@ -309,7 +324,10 @@ void GS::init_transfer()
 	c.h = uint32_t((r >> 32) & 0xfff);
 
 	if (xdir == 2)
+	{
+		flush_render_pass();
 		local_copy(c);
+	}
 	else if (xdir == 0)
 	{
 		xfer.copy = c;
@@ -357,6 +375,7 @@ void GS::hwreg(const uint64_t *data, size_t count)
 
 void GS::write_host_pixels()
 {
+	flush_render_pass();
 	const Copy &c = xfer.copy;
 	const uint8_t *p = reinterpret_cast<const uint8_t *>(xfer.payload.data());
 	size_t bytes = xfer.payload.size() * 8u;

@@ -40,6 +40,37 @@ struct Registers
 	uint64_t bitbltbuf, trxpos, trxreg, trxdir;
 };
 
+// One assembled primitive of a render pass (gs/shaders/data_structures.h:55-80 content).
+struct Prim
+{
+	Vertex v[3];
+	int32_t bb[4];       // scissor-clipped pixel bounding box
+	uint32_t state, tex, tex2, alpha, fbmsk, fogcol;
+	uint32_t combiner, blend_mode, dimx[2];
+	uint32_t clut_index;  // CLUT snapshot used by a textured primitive
+	uint32_t tex_slot;    // texture slot in the render pass
+};
+
+// Result of triangle setup (gs/shaders/data_structures.h:32-52 content).
+struct Setup
+{
+	int32_t a[3], b[3], c[3];
+	float inv_area, error_i, error_j;
+	int32_t bb[4];
+	uint32_t z[3];
+	float stqf0[4], stqf1[4], stqf2[4];
+	uint32_t rgba[3];
+};
+
+struct TextureSlot
+{
+	uint64_t tex0, clamp, tex1, texa;
+	uint32_t clut_index;
+	uint32_t width, height;
+	std::vector<uint32_t> rgba; // decoded texture cache image (width x height)
+	float region[4];
+};
+
 class GS
 {
 public:
@@ -50,7 +81,7 @@ public:
 	uint64_t stat(uint32_t which) const;
 
 	std::vector<uint8_t> vram;   // plane 0 = VRAM; at SSAA: plane 1 = reference, 2.. = samples
-	std::vector<uint16_t> clut;  // 512 x 16-bit CLUT entries (gs/shaders/clut_write.comp)
+	std::vector<uint16_t> clut;  // current CLUT: 512 x 16-bit entries (gs/shaders/clut_write.comp)
 	uint32_t sample_log2 = 0;
 
 	void (*xfer_hook)(void *, uint32_t dir, uint32_t w, uint32_t h, uint32_t psm) = nullptr;
@@ -74,9 +105,19 @@ private:
 	Vertex vq[3];
 	uint32_t vq_count = 0;
 
+	// Render pass state.
+	std::vector<Prim> rp_prims;
+	std::vector<TextureSlot> rp_textures;
+	std::vector<std::vector<uint16_t>> clut_snapshots;
+	uint64_t rp_frame = 0, rp_zbuf = 0;
+	bool state_dirty = true;          // a draw-state register changed since the last append
+	bool last_parallelogram_candidate = false;
+	int32_t last_order[3] = { 0, 1, 2 };
+
 	// Render statistics for the covered-sample count.
 	uint64_t covered_samples = 0;
 	uint64_t primitives_drawn = 0;
+	uint64_t render_passes = 0;
 
 	void reset_registers();
 	void packed_write(uint32_t addr, const uint64_t *w);
@@ -87,6 +128,7 @@ private:
 	void finish_host_transfer();
 	void partial_flush_transfer();
 	void invalidate_ssaa(uint32_t psm, uint32_t x, uint32_t y, uint32_t base, uint32_t stride);
+	void set_reg(uint64_t &reg, uint64_t v, bool marks_dirty);
 
 	// Drawing (oracle_render.cpp).
 	void write_tex0(uint32_t ctx, uint64_t v);
@@ -95,7 +137,17 @@ private:
 	void kick_xyzf(uint64_t v, bool adc);
 	void push_vertex(int32_t x, int32_t y, uint32_t z, float fog);
 	void drawing_kick(bool adc);
-	void draw(const Vertex *v, uint32_t nverts, uint32_t prim_type);
+	void append(const Vertex *v, uint32_t nverts, bool quad);
+	bool draw_is_degenerate() const;
+	void flush_render_pass();
+	void render(const std::vector<Prim> &prims);
 	void load_clut(uint32_t ctx);
+	uint32_t texture_slot(uint32_t ctx);
 };
+
+// Reciprocal helpers (gs/shaders/math_utils.h:27-101) and their tables
+// (gs/gs_renderer.cpp:3600-3640).
+float rcp_fixed(uint32_t b);
+float rcp_float(float v);
+const uint8_t *rcp_table();
 }

@@ -13,6 +13,7 @@ bool GSInterface::init(const GSOptions &options)
 	if (!renderer_.init(sample_log2, options.upload_arena_bytes))
 		return false;
 	rp_.init(&renderer_, sample_log2);
+	rp_.host = this;
 	reset_context_state();
 	return true;
 }
@@ -47,6 +48,29 @@ void GSInterface::begin_transfer_work()
 void GSInterface::flush_render_pass()
 {
 	rp_.flush();
+}
+
+void GSInterface::before_first_primitive()
+{
+	renderer_.flush_transfers();
+}
+
+void GSInterface::before_texture_read()
+{
+	if (xfer_.host_to_local_active)
+		flush_pending_transfer(true);
+}
+
+// Registers that feed primitive state mark the state dirty when their value changes
+// (update_internal_register, gs/gs_interface.cpp:3877-3885); a dirty state blocks
+// parallelogram fusion of the next triangle.
+void GSInterface::set_state_reg(uint64_t &reg, uint64_t v)
+{
+	if (reg != v)
+	{
+		reg = v;
+		rp_.mark_state_dirty();
+	}
 }
 
 // ---------------------------------------------------------------------------------------
@@ -189,14 +213,30 @@ void GSInterface::write_register(uint32_t addr, uint64_t v)
 {
 	switch (addr)
 	{
-	case 0x00: regs_.prim = v; rp_.reset_vertex_queue(); break;
+	case 0x00: set_state_reg(regs_.prim, v & 0x7ffu); rp_.reset_vertex_queue(); break;
 	case 0x01: case 0x11: regs_.rgbaq = v; break;
 	case 0x02: regs_.st = v; break;
 	case 0x03: regs_.uv = v; break;
 	case 0x0a: regs_.fog = v; break;
-	case 0x06: case 0x07: regs_.tex0[addr - 0x06] = v; break;
-	case 0x08: case 0x09: regs_.clamp[addr - 0x08] = v; break;
-	case 0x14: case 0x15: regs_.tex1[addr - 0x14] = v; break;
+	case 0x04: case 0x0c:
+		rp_.vertex_kick(regs_, uint32_t(v & 0xffff), uint32_t((v >> 16) & 0xffff), uint32_t((v >> 32) & 0xffffff),
+		                float(uint32_t(v >> 56)), addr == 0x0c);
+		break;
+	case 0x05: case 0x0d:
+		rp_.vertex_kick(regs_, uint32_t(v & 0xffff), uint32_t((v >> 16) & 0xffff), uint32_t(v >> 32),
+		                float(uint32_t(regs_.fog >> 56)), addr == 0x0d);
+		break;
+	case 0x06: case 0x07: write_tex0(addr - 0x06, v); break;
+	case 0x08: case 0x09: set_state_reg(regs_.clamp[addr - 0x08], v); break;
+	case 0x14: case 0x15: set_state_reg(regs_.tex1[addr - 0x14], v); break;
+	case 0x16: case 0x17:
+	{
+		// TEX2 rewrites the palette / PSM part of TEX0 (gs/gs_interface.cpp:2950-2968).
+		const uint64_t keep = 0x3fffull | (0x3full << 14) | (0xfull << 26) | (0xfull << 30) | (1ull << 34) | (3ull << 35);
+		uint32_t ctx = addr - 0x16;
+		write_tex0(ctx, (regs_.tex0[ctx] & keep) | (v & ~keep));
+		break;
+	}
 	case 0x18: case 0x19: regs_.xyoffset[addr - 0x18] = v; break;
 	case 0x1a: regs_.prmodecont = v; break;
 	case 0x1b: regs_.prmode = v; break;
@@ -204,18 +244,18 @@ void GSInterface::write_register(uint32_t addr, uint64_t v)
 	case 0x22: regs_.scanmsk = v; break;
 	case 0x34: case 0x35: regs_.miptbp1[addr - 0x34] = v; break;
 	case 0x36: case 0x37: regs_.miptbp2[addr - 0x36] = v; break;
-	case 0x3b: regs_.texa = v; break;
+	case 0x3b: set_state_reg(regs_.texa, v); break;
 	case 0x3d: regs_.fogcol = v; break;
 	case 0x40: case 0x41: regs_.scissor[addr - 0x40] = v; break;
-	case 0x42: case 0x43: regs_.alpha[addr - 0x42] = v; break;
-	case 0x44: regs_.dimx = v; break;
-	case 0x45: regs_.dthe = v; break;
-	case 0x46: regs_.colclamp = v; break;
-	case 0x47: case 0x48: regs_.test[addr - 0x47] = v; break;
-	case 0x49: regs_.pabe = v; break;
-	case 0x4a: case 0x4b: regs_.fba[addr - 0x4a] = v; break;
-	case 0x4c: case 0x4d: regs_.frame[addr - 0x4c] = v; break;
-	case 0x4e: case 0x4f: regs_.zbuf[addr - 0x4e] = v; break;
+	case 0x42: case 0x43: set_state_reg(regs_.alpha[addr - 0x42], v); break;
+	case 0x44: set_state_reg(regs_.dimx, v); break;
+	case 0x45: set_state_reg(regs_.dthe, v); break;
+	case 0x46: set_state_reg(regs_.colclamp, v); break;
+	case 0x47: case 0x48: set_state_reg(regs_.test[addr - 0x47], v); break;
+	case 0x49: set_state_reg(regs_.pabe, v); break;
+	case 0x4a: case 0x4b: set_state_reg(regs_.fba[addr - 0x4a], v); break;
+	case 0x4c: case 0x4d: set_state_reg(regs_.frame[addr - 0x4c], v); break;
+	case 0x4e: case 0x4f: set_state_reg(regs_.zbuf[addr - 0x4e], v); break;
 	case 0x50: regs_.bitbltbuf = v; break;
 	case 0x51: regs_.trxpos = v; break;
 	case 0x52: regs_.trxreg = v; break;
@@ -223,6 +263,12 @@ void GSInterface::write_register(uint32_t addr, uint64_t v)
 	case 0x54: hwreg(&v, 1); break;
 	default: break;
 	}
+}
+
+// TEX0 write: texture state change; palette loads are handled by the texture cache.
+void GSInterface::write_tex0(uint32_t ctx, uint64_t v)
+{
+	set_state_reg(regs_.tex0[ctx], v);
 }
 
 // ---------------------------------------------------------------------------------------

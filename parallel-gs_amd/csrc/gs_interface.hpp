@@ -45,7 +45,7 @@ struct GIFPath
 	uint64_t tag_lo = 0;
 };
 
-class GSInterface
+class GSInterface : private RenderPassBuilder::Host
 {
 public:
 	GSInterface() = default;
@@ -93,10 +93,14 @@ private:
 	std::vector<uint8_t> host_vram_;
 
 	void packed_write(uint32_t addr, const uint64_t *qword);
+	void write_tex0(uint32_t ctx, uint64_t v);
 	void init_transfer();
 	void hwreg(const uint64_t *data, size_t count);
 	void flush_pending_transfer(bool keep_alive);
 	void begin_transfer_work();
 	void flush_render_pass();
+	void set_state_reg(uint64_t &reg, uint64_t v);
+	void before_first_primitive() override;
+	void before_texture_read() override;
 };
 }

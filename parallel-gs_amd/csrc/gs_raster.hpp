@@ -1,0 +1,93 @@
+// Render-pass records shared by the host front end and the raster kernels.
+//
+// Field layouts follow the reference's shared structs (gs/shaders/data_structures.h:32-80)
+// because the front end and the kernels exchange them; state/blend/combiner bit positions are
+// the reference's (data_structures.h:206-307).
+#pragma once
+#include "gs_defs.hpp"
+
+namespace pgs
+{
+enum : uint32_t
+{
+	STATE_INDEX_MASK = 0xfffu,
+	ST_Z_TEST = 1u << 12, ST_Z_WRITE = 1u << 13, ST_OPAQUE = 1u << 14, ST_PARALLELOGRAM = 1u << 15,
+	ST_MULTISAMPLE = 1u << 16, ST_Z_GREATER = 1u << 17, ST_PERSPECTIVE = 1u << 18, ST_IIP = 1u << 19,
+	ST_FIX = 1u << 20, ST_SPRITE = 1u << 21, ST_LINE = 1u << 22, ST_SCANMSK_EVEN = 1u << 23,
+	ST_SCANMSK_ODD = 1u << 24, ST_SNAP_RASTER = 1u << 25, ST_SNAP_ATTRIBUTE = 1u << 26,
+	ST_PROVOKING_SHIFT = 27,
+};
+
+enum : uint32_t
+{
+	TEX_INDEX_MASK = 0xffffu,
+	TEX_MAG_LINEAR = 1u << 16, TEX_MIN_LINEAR = 1u << 17, TEX_MIP_LINEAR = 1u << 18,
+	TEX_CLAMP_S = 1u << 19, TEX_CLAMP_T = 1u << 20,
+	TEX2_FIXED_LOD = 1u << 0,
+};
+
+enum : uint32_t
+{
+	COMB_MODE_MASK = 3u, COMB_TME = 1u << 2, COMB_FOG = 1u << 3, COMB_TCC = 1u << 4,
+	BLEND_ATE = 1u << 1, BLEND_DATE = 1u << 7, BLEND_DATM = 1u << 8, BLEND_ABE = 1u << 9,
+	BLEND_PABE = 1u << 10, BLEND_COLCLAMP = 1u << 19, BLEND_FBA = 1u << 20, BLEND_DTHE = 1u << 21,
+};
+
+struct VertexPosition
+{
+	int32_t x, y;
+	uint32_t z;
+	int32_t pad;
+};
+
+struct VertexAttribute
+{
+	float s, t, q;
+	uint32_t rgba;
+	float fog;
+	uint32_t uv; // u | v << 16, 10.4 fixed point
+};
+
+struct PrimitiveAttribute
+{
+	int16_t bb[4];
+	uint32_t state, tex, tex2, alpha, fbmsk, fogcol;
+};
+static_assert(sizeof(PrimitiveAttribute) == 32, "PrimitiveAttribute layout");
+
+struct StateVector
+{
+	uint32_t combiner, blend_mode, dimx0, dimx1;
+};
+
+struct PrimitiveSetup
+{
+	int32_t a[3];
+	float inv_area;
+	int32_t b[3];
+	float error_i;
+	int32_t c[3];
+	float error_j;
+	int32_t bb[4];
+	uint32_t z[4];
+};
+static_assert(sizeof(PrimitiveSetup) == 80, "PrimitiveSetup layout");
+
+struct TransformedAttributes
+{
+	float stqf0[4], stqf1[4], stqf2[4];
+	uint32_t rgba[3];
+	uint32_t pad;
+};
+static_assert(sizeof(TransformedAttributes) == 64, "TransformedAttributes layout");
+
+// One cached texture of a render pass: a decoded RGBA8 image plus sampling parameters
+// (gs/shaders/data_structures.h:98-105 TexInfo content).
+struct TextureDesc
+{
+	uint32_t image_offset; // offset in dwords into the texture image arena
+	uint32_t width, height;
+	uint32_t pad;
+	float region[4];       // clamp region (min u, min v, max u, max v) in texels
+};
+}

@@ -1,0 +1,511 @@
+// Raster kernels for gfx950: triangle setup, coarse binning, shading.
+//
+// Replaces triangle_setup.comp, binning.comp and ubershader.comp. Integer edge setup and all
+// float expressions follow the reference formulas operation by operation; the translation unit
+// is compiled with -ffp-contract=off so every float operation rounds as written (the reference
+// marks the setup and barycentric expressions `precise`; the remaining expressions are
+// evaluated without contraction as well).
+#include "gs_defs.hpp"
+#include "gs_raster.hpp"
+#include "pgs_kernels.h"
+
+namespace pgs
+{
+// Seed table of the fixed-point reciprocal and the float reciprocal LUT, uploaded at init
+// (pgs_raster_init_tables).
+__constant__ uint8_t c_rcp_seed[256];
+__constant__ float c_rcp_float_lut[256];
+
+namespace
+{
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int32_t uni(int32_t v) { return int32_t(__builtin_amdgcn_readfirstlane(uint32_t(v))); }
+__device__ __forceinline__ float unif(float v) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v))); }
+
+__device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
+__device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
+__device__ __forceinline__ int32_t idot3(int32_t e0, int32_t e1, int32_t e2, int32_t x, int32_t y)
+{
+	return wadd(wadd(wmul(e0, x), wmul(e1, y)), e2);
+}
+
+// math_utils.h:27-61
+__device__ float rcp_fixed(uint32_t b)
+{
+	uint32_t lz = __clz(b);
+	uint32_t nb = b << lz;
+	uint32_t r = uint32_t(c_rcp_seed[(nb >> 23) & 0xffu]) | 0x100u;
+	uint32_t f = (r * r) << 14;
+	uint32_t p = __umulhi(f, nb);
+	r = (r << 23) - p;
+	r <<= 1;
+	p = __umulhi(r, nb);
+	r = __umulhi(r, 0u - p);
+	r = (r + 64u) >> 7;
+	r -= 1u << 23;
+	r += uint32_t(127 - 32 + int32_t(lz)) << 23;
+	return __uint_as_float(r);
+}
+
+// math_utils.h:65-101
+__device__ float rcp_float(float v)
+{
+	uint32_t u = __float_as_uint(v);
+	int32_t e = int32_t((u >> 23) & 0xffu);
+	if (e == 0)
+		return __uint_as_float(0x7f800000u);
+	float x = c_rcp_float_lut[(u >> 15) & 0xffu];
+	float a = __uint_as_float((u & 0x7fffffu) | (127u << 23));
+	for (int it = 0; it < 4; it++)
+	{
+		float x2 = __fmul_rn(x, 2.0f);
+		float sq = __fmul_rn(x, x);
+		float sqa = __fmul_rn(sq, a);
+		x = __fsub_rn(x2, sqa);
+	}
+	return ldexpf(x, 127 - e);
+}
+
+__device__ __forceinline__ bool tie_break(int32_t ex, int32_t ey) { return ey > 0 || (ey == 0 && ex < 0); }
+__device__ __forceinline__ bool order_less(int32_t ax, int32_t ay, int32_t bx, int32_t by, uint32_t az, uint32_t bz)
+{
+	if (bz != az)
+		return az < bz;
+	if (by != ay)
+		return ay < by;
+	return ax < bx;
+}
+__device__ __forceinline__ int32_t quantize64(int64_t c) { return int32_t(uint32_t(uint64_t(c) >> 1)); }
+
+template <typename T> __device__ __forceinline__ void swp(T &a, T &b) { T t = a; a = b; b = t; }
+
+// ---------------------------------------------------------------------------------------
+// Triangle setup (gs/shaders/triangle_setup.comp:127-873), one thread per primitive.
+__global__ void __launch_bounds__(64) k_triangle_setup(pgs_setup_args args)
+{
+	const uint32_t index = blockIdx.x * 64u + threadIdx.x;
+	if (index >= args.num_primitives)
+		return;
+	const VertexPosition *pos = static_cast<const VertexPosition *>(args.positions) + 3u * index;
+	const VertexAttribute *attr = static_cast<const VertexAttribute *>(args.attributes) + 3u * index;
+	PrimitiveAttribute &prim = static_cast<PrimitiveAttribute *>(args.prims)[index];
+	PrimitiveSetup &out = static_cast<PrimitiveSetup *>(args.setup)[index];
+	TransformedAttributes &ta = static_cast<TransformedAttributes *>(args.tattr)[index];
+	const uint32_t rate = args.rate_log2;
+
+	uint32_t state = prim.state;
+	uint32_t tex = prim.tex;
+	const uint32_t tex2 = prim.tex2;
+	const bool parallelogram = state & ST_PARALLELOGRAM, perspective = state & ST_PERSPECTIVE;
+	const bool iip = state & ST_IIP, fix = state & ST_FIX, sprite = state & ST_SPRITE, line = state & ST_LINE;
+	const bool multisample = state & ST_MULTISAMPLE;
+	const uint32_t provoking = (state >> ST_PROVOKING_SHIFT) & 3u;
+
+	int32_t o0 = 0, o1 = 1, o2 = 2;
+	int32_t ax = pos[0].x, ay = pos[0].y, bx = pos[1].x, by = pos[1].y, cx = pos[2].x, cy = pos[2].y;
+	uint32_t za = pos[0].z, zb = pos[1].z, zc = pos[2].z;
+
+	if (line)
+	{
+		bool sax = (ax & 15) == 0, say = (ay & 15) == 0, sbx = (bx & 15) == 0, sby = (by & 15) == 0;
+		if ((sax || sbx) && bx > ax) { ax++; bx++; }
+		if ((say || sby) && by > ay) { ay++; by++; }
+	}
+
+	if ((!parallelogram || line) && !fix)
+	{
+		if (!line && !order_less(bx, by, cx, cy, zb, zc))
+		{
+			swp(o1, o2); swp(bx, cx); swp(by, cy); swp(zb, zc);
+		}
+		if (!order_less(ax, ay, bx, by, za, zb))
+		{
+			swp(o0, o1); swp(ax, bx); swp(ay, by); swp(za, zb);
+		}
+	}
+	else if (provoking == 1)
+	{
+		swp(o0, o1); swp(ax, bx); swp(ay, by); swp(za, zb);
+	}
+	else if (provoking == 2)
+	{
+		swp(o0, o2); swp(ax, cx); swp(ay, cy); swp(za, zc);
+	}
+
+	if (sprite)
+	{
+		cx = ax; cy = by;
+		by = ay;
+	}
+	else if (line)
+	{
+		int32_t dx = bx - ax, dy = by - ay;
+		if (abs(dx) >= abs(dy))
+		{
+			int32_t yd = -((dx > 0) - (dx < 0)) * 8;
+			if (multisample) yd *= 2;
+			cy = ay + yd; ay -= yd; by -= yd;
+			if (!multisample) { ay++; by++; }
+			cx = ax;
+		}
+		else
+		{
+			int32_t xd = ((dy > 0) - (dy < 0)) * 8;
+			if (multisample) xd *= 2;
+			cx = ax + xd; ax -= xd; bx -= xd;
+			if (!multisample) { ax++; bx++; }
+			cy = ay;
+		}
+	}
+
+	int32_t abx = bx - ax, aby = by - ay, acx = cx - ax, acy = cy - ay;
+	const int64_t sa = int64_t(aby) * acx - int64_t(abx) * acy;
+	if (sa == 0)
+	{
+		out.bb[0] = 0; out.bb[1] = 0; out.bb[2] = -1; out.bb[3] = -1;
+		if (rate)
+			prim.state = state | ST_SNAP_ATTRIBUTE;
+		return;
+	}
+
+	uint32_t area;
+	if (sa < 0)
+	{
+		area = 0u - uint32_t(uint64_t(sa));
+		swp(abx, acx); swp(aby, acy); swp(bx, cx); swp(by, cy); swp(zb, zc); swp(o1, o2);
+	}
+	else
+		area = uint32_t(uint64_t(sa));
+
+	const int32_t bcx = cx - bx, bcy = cy - by, cax = -acx, cay = -acy;
+	const int32_t dcdx = aby, dcdy = -abx, dadx = bcy, dady = -bcx, dbdx = cay, dbdy = -cax;
+	int64_t kc = int64_t(-ax) * dcdx - int64_t(ay) * dcdy;
+	int64_t ka = int64_t(-bx) * dadx - int64_t(by) * dady;
+	int64_t kb = int64_t(-cx) * dbdx - int64_t(cy) * dbdy;
+	const bool ta_ = tie_break(bcx, bcy), tb_ = tie_break(cax, cay), tc_ = tie_break(abx, aby);
+	int32_t ea = !ta_, eb = !tb_, ec = !tc_;
+	ka -= ea; kb -= eb; kc -= ec;
+	ea += int32_t(uint32_t(uint64_t(ka)) & 1u);
+	eb += int32_t(uint32_t(uint64_t(kb)) & 1u);
+	ec += int32_t(uint32_t(uint64_t(kc)) & 1u);
+
+	int32_t flip_b = 0, flip_c = 0;
+	if (parallelogram)
+	{
+		int32_t off = 1 - int32_t(tb_) - int32_t(tc_);
+		flip_b = (off + eb + ea + 2) >> 1;
+		flip_c = (off + ec + ea + 2) >> 1;
+	}
+
+	const float inv_area = rcp_fixed(area);
+	const float inv_area_scaled = __fmul_rn(inv_area, 2.0f);
+	float error_i = __fmul_rn(float(eb), inv_area);
+	float error_j = __fmul_rn(float(ec), inv_area);
+
+	if (sprite) { zb = za; zc = za; }
+	else if (line) zc = za;
+	if (fix || sprite) { o0 = 0; o1 = 1; o2 = 2; }
+
+	const VertexAttribute a0 = attr[o0], a1 = attr[o1];
+	const VertexAttribute a2 = attr[(!sprite && !line) ? o2 : 0];
+	float q0[4], q1[4], q2[4];
+	auto load = [&](const VertexAttribute &v, float *o) {
+		if (perspective) { o[0] = v.s; o[1] = v.t; }
+		else { o[0] = __fdiv_rn(float(v.uv & 0xffffu), 16.0f); o[1] = __fdiv_rn(float(v.uv >> 16), 16.0f); }
+		o[2] = v.q; o[3] = v.fog;
+	};
+	load(a0, q0);
+	load(a1, q1);
+	if (!sprite && !line)
+		load(a2, q2);
+	if (sprite)
+	{
+		if (sa < 0)
+		{
+			q2[0] = q1[0]; q2[1] = q0[1]; q2[2] = q0[2]; q2[3] = q0[3];
+			q1[0] = q0[0]; q1[2] = q0[2]; q1[3] = q0[3];
+		}
+		else
+		{
+			q2[0] = q0[0]; q2[1] = q1[1]; q2[2] = q0[2]; q2[3] = q0[3];
+			q1[1] = q0[1]; q1[2] = q0[2]; q1[3] = q0[3];
+		}
+	}
+	else if (line)
+	{
+		for (int k = 0; k < 4; k++) q2[k] = q0[k];
+	}
+
+	if (perspective)
+	{
+		if (q0[2] < 0.0f)
+		{
+			q0[0] = -q0[0]; q0[1] = -q0[1]; q1[0] = -q1[0]; q1[1] = -q1[1]; q2[0] = -q2[0]; q2[1] = -q2[1];
+		}
+		q0[2] = fabsf(q0[2]); q1[2] = fabsf(q1[2]); q2[2] = fabsf(q2[2]);
+		const uint32_t qexp = (__float_as_uint(fmaxf(fmaxf(q0[2], q1[2]), q2[2])) >> 23) & 0xffu;
+		q0[2] = __uint_as_float(__float_as_uint(q0[2]) & ~0xffu);
+		q1[2] = __uint_as_float(__float_as_uint(q1[2]) & ~0xffu);
+		q2[2] = __uint_as_float(__float_as_uint(q2[2]) & ~0xffu);
+		const uint32_t sexp = (__float_as_uint(fmaxf(fmaxf(fabsf(q0[0]), fabsf(q1[0])), fabsf(q2[0]))) >> 23) & 0xffu;
+		const uint32_t texp = (__float_as_uint(fmaxf(fmaxf(fabsf(q0[1]), fabsf(q1[1])), fabsf(q2[1]))) >> 23) & 0xffu;
+		const uint32_t schop = min(9u + max(sexp, qexp) - sexp, 23u);
+		const uint32_t tchop = min(9u + max(texp, qexp) - texp, 23u);
+		const uint32_t smask = ~((1u << schop) - 1u), tmask = ~((1u << tchop) - 1u);
+		q0[0] = __uint_as_float(__float_as_uint(q0[0]) & smask); q0[1] = __uint_as_float(__float_as_uint(q0[1]) & tmask);
+		q1[0] = __uint_as_float(__float_as_uint(q1[0]) & smask); q1[1] = __uint_as_float(__float_as_uint(q1[1]) & tmask);
+		q2[0] = __uint_as_float(__float_as_uint(q2[0]) & smask); q2[1] = __uint_as_float(__float_as_uint(q2[1]) & tmask);
+		const bool needs_lod = (tex2 & TEX2_FIXED_LOD) == 0;
+		if (!needs_lod && q0[2] == q1[2] && q0[2] == q2[2])
+		{
+			const float inv_q = rcp_float(q0[2]);
+			q0[0] = __fmul_rn(q0[0], inv_q); q0[1] = __fmul_rn(q0[1], inv_q); q0[2] = 1.0f;
+			q1[0] = __fmul_rn(q1[0], inv_q); q1[1] = __fmul_rn(q1[1], inv_q); q1[2] = 1.0f;
+			q2[0] = __fmul_rn(q2[0], inv_q); q2[1] = __fmul_rn(q2[1], inv_q); q2[2] = 1.0f;
+		}
+	}
+
+	float d1[4], d2[4];
+	for (int k = 0; k < 4; k++)
+	{
+		d1[k] = fix ? 0.0f : __fsub_rn(q1[k], q0[k]);
+		d2[k] = fix ? 0.0f : __fsub_rn(q2[k], q0[k]);
+		ta.stqf0[k] = q0[k];
+		ta.stqf1[k] = d1[k];
+		ta.stqf2[k] = d2[k];
+	}
+
+	if (iip && !sprite)
+	{
+		uint32_t r0 = a0.rgba, r1 = a1.rgba, r2 = a2.rgba;
+		if (fix) r1 = r0;
+		if (fix || line) r2 = r0;
+		ta.rgba[0] = r0; ta.rgba[1] = r1; ta.rgba[2] = r2;
+	}
+	else
+	{
+		const uint32_t flat = attr[0].rgba;
+		ta.rgba[0] = flat; ta.rgba[1] = flat; ta.rgba[2] = flat;
+	}
+	ta.pad = 0;
+
+	const int32_t pa0 = int32_t((uint32_t(dadx) << 6) | uint32_t(parallelogram) | (uint32_t(flip_b & 3) << 2) |
+	                            (uint32_t(flip_c & 3) << 4));
+
+	const int32_t dx = parallelogram ? bx + cx - ax : cx;
+	const int32_t dy = parallelogram ? by + cy - ay : cy;
+	int32_t di = 0, dj = 0;
+	if (sprite)
+	{
+		const int32_t xd = dx - ax, yd = dy - ay;
+		if (xd > 0 && (xd & (xd - 1)) != 0) { di += dbdx; dj += dcdx; }
+		if (yd > 0 && (yd & (yd - 1)) != 0) { di += dbdy; dj += dcdy; }
+	}
+	else if (parallelogram && !perspective)
+	{
+		const bool nearest = (tex & TEX_MAG_LINEAR) == 0;
+		const bool npot = (area & (area - 1u)) != 0;
+		if (nearest && npot)
+		{
+			const int32_t u0 = int32_t(a0.uv & 0xffffu) - ax, v0 = int32_t(a0.uv >> 16) - ay;
+			const int32_t u1 = int32_t(a1.uv & 0xffffu) - bx, v1 = int32_t(a1.uv >> 16) - by;
+			const int32_t u2 = int32_t(a2.uv & 0xffffu) - cx, v2 = int32_t(a2.uv >> 16) - cy;
+			if (u0 == u1 && v0 == v1 && u0 == u2 && v0 == v2)
+			{
+				const bool mis_x = (min(ax, dx) & 15) != 0, mis_y = (min(ay, dy) & 15) != 0;
+				const float dudx = __fadd_rn(__fmul_rn(d1[0], float(dbdx)), __fmul_rn(d2[0], float(dcdx)));
+				const float dvdx = __fadd_rn(__fmul_rn(d1[1], float(dbdx)), __fmul_rn(d2[1], float(dcdx)));
+				const float dudy = __fadd_rn(__fmul_rn(d1[0], float(dbdy)), __fmul_rn(d2[0], float(dcdy)));
+				const float dvdy = __fadd_rn(__fmul_rn(d1[1], float(dbdy)), __fmul_rn(d2[1], float(dcdy)));
+				const bool adj_x = mis_x && ((dudx > 0.0f && dvdx == 0.0f) || (dvdx > 0.0f && dudx == 0.0f));
+				const bool adj_y = mis_y && ((dudy > 0.0f && dvdy == 0.0f) || (dvdy > 0.0f && dudy == 0.0f));
+				if (adj_x) { di -= dbdx; dj -= dcdx; }
+				if (adj_y) { di -= dbdy; dj -= dcdy; }
+			}
+		}
+	}
+	const float k8 = __fmul_rn(0.125f, inv_area_scaled);
+	error_i = __fadd_rn(error_i, __fmul_rn(float(di), k8));
+	error_j = __fadd_rn(error_j, __fmul_rn(float(dj), k8));
+
+	PrimitiveSetup s;
+	s.a[0] = pa0; s.a[1] = dady; s.a[2] = quantize64(ka);
+	s.b[0] = dbdx; s.b[1] = dbdy; s.b[2] = quantize64(kb);
+	s.c[0] = dcdx; s.c[1] = dcdy; s.c[2] = quantize64(kc);
+	s.inv_area = inv_area_scaled;
+	s.error_i = error_i;
+	s.error_j = error_j;
+	s.z[0] = za;
+	s.z[1] = __float_as_uint(__uint2float_rn(zb - za));
+	s.z[2] = __float_as_uint(__uint2float_rn(zc - za));
+	s.z[3] = 0;
+
+	int32_t lox = min(min(ax, bx), min(cx, dx)), loy = min(min(ay, by), min(cy, dy));
+	int32_t hix = max(max(ax, bx), max(cx, dx)), hiy = max(max(ay, by), max(cy, dy));
+	lox = (lox + (15 >> rate)) >> 4;
+	loy = (loy + (15 >> rate)) >> 4;
+	hix = (hix - 1) >> 4;
+	hiy = (hiy - 1) >> 4;
+	if (multisample && !line) { lox--; loy--; hix++; hiy++; }
+	s.bb[0] = max(lox, int32_t(prim.bb[0]));
+	s.bb[1] = max(loy, int32_t(prim.bb[1]));
+	s.bb[2] = min(hix, int32_t(prim.bb[2]));
+	s.bb[3] = min(hiy, int32_t(prim.bb[3]));
+	out = s;
+
+	if (sprite && !perspective)
+	{
+		const int32_t u0 = int32_t(a0.uv & 0xffffu) - ax, v0 = int32_t(a0.uv >> 16) - ay;
+		const int32_t u1 = int32_t(a1.uv & 0xffffu) - dx, v1 = int32_t(a1.uv >> 16) - dy;
+		const int32_t minu = min(u0, u1), minv = min(v0, v1), maxu = max(u0, u1), maxv = max(v0, v1);
+		const bool cs_u = (minu & 15) == 8, cs_v = (minv & 15) == 8;
+		const int32_t distx = abs(dx - ax), disty = abs(dy - ay);
+		const bool ns_u = (maxu & 15) == 9 && (distx & (distx - 1)) != 0;
+		const bool ns_v = (maxv & 15) == 9 && (disty & (disty - 1)) != 0;
+		const bool no_interp = (maxu - minu) <= 1 && (maxv - minv) <= 1;
+		if ((ns_u || cs_u) && (ns_v || cs_v) && no_interp)
+			tex &= ~TEX_MAG_LINEAR;
+	}
+
+	if (prim.fbmsk != args.opaque_fbmask && (state & ST_OPAQUE))
+		state &= ~ST_OPAQUE;
+
+	if (rate != 0 && !sprite)
+	{
+		bool per_sample = multisample;
+		if (!per_sample && perspective)
+			per_sample = d1[2] != 0.0f || d2[2] != 0.0f;
+		if (!per_sample)
+			per_sample = za != zb || zb != zc;
+		if (!per_sample && !line)
+			state |= ST_SNAP_ATTRIBUTE;
+	}
+
+	prim.state = state;
+	prim.tex = tex;
+}
+
+// intersect.h:24-81: conservative test of a primitive against a pixel rectangle.
+__device__ __forceinline__ bool intersects_tile(const PrimitiveSetup &s, int32_t tlox, int32_t tloy, int32_t thix,
+                                                int32_t thiy, bool multisampled_tri)
+{
+	int32_t lox = max(s.bb[0], tlox), loy = max(s.bb[1], tloy);
+	int32_t hix = min(s.bb[2], thix), hiy = min(s.bb[3], thiy);
+	if (hix < lox || hiy < loy)
+		return false;
+	const bool para = (s.a[0] & 1) != 0;
+	const int32_t offb = int32_t((uint32_t(s.a[0]) >> 2) & 3u) - 1;
+	const int32_t offc = int32_t((uint32_t(s.a[0]) >> 4) & 3u) - 1;
+	const int32_t ax = s.a[0] >> 6;
+	lox <<= 3; loy <<= 3;
+	hix = (hix << 3) | 7; hiy = (hiy << 3) | 7;
+	int32_t da = idot3(ax, s.a[1], s.a[2], ax >= 0 ? hix : lox, s.a[1] >= 0 ? hiy : loy);
+	int32_t db = idot3(s.b[0], s.b[1], s.b[2], s.b[0] >= 0 ? hix : lox, s.b[1] >= 0 ? hiy : loy);
+	int32_t dc = idot3(s.c[0], s.c[1], s.c[2], s.c[0] >= 0 ? hix : lox, s.c[1] >= 0 ? hiy : loy);
+	if (multisampled_tri && !para)
+	{
+		da = wadd(da, 8 * max(abs(ax), abs(s.a[1])));
+		db = wadd(db, 8 * max(abs(s.b[0]), abs(s.b[1])));
+		dc = wadd(dc, 8 * max(abs(s.c[0]), abs(s.c[1])));
+	}
+	bool hit = da >= 0 && db >= 0 && dc >= 0;
+	if (para && !hit)
+	{
+		const bool mxa = ax >= 0, mya = s.a[1] >= 0;
+		int32_t pa = idot3(ax, s.a[1], s.a[2], mxa ? hix : lox, mya ? hiy : loy);
+		int32_t pb = idot3(s.b[0], s.b[1], s.b[2], -s.c[0] >= 0 ? hix : lox, -s.c[1] >= 0 ? hiy : loy);
+		int32_t pc = idot3(s.c[0], s.c[1], s.c[2], -s.b[0] >= 0 ? hix : lox, -s.b[1] >= 0 ? hiy : loy);
+		pb = wadd(pb, wadd(pa, offb));
+		pc = wadd(pc, wadd(pa, offc));
+		int32_t na = idot3(ax, s.a[1], s.a[2], mxa ? lox : hix, mya ? loy : hiy);
+		hit = -1 >= na && pb >= 0 && pc >= 0;
+	}
+	return hit;
+}
+
+// ---------------------------------------------------------------------------------------
+// Coarse binning: one workgroup (4 waves) per coarse tile. Primitives are tested 256 at a time;
+// hits are compacted in ascending order with wave ballots and an LDS exchange of wave counts.
+__global__ void __launch_bounds__(256) k_binning(pgs_bin_args args)
+{
+	const uint32_t tile = blockIdx.x;
+	const uint32_t tx = tile % args.tiles_x, ty = tile / args.tiles_x;
+	const int32_t size = 1 << args.tile_log2;
+	const int32_t lox = args.origin_x + int32_t(tx) * size, loy = args.origin_y + int32_t(ty) * size;
+	const int32_t hix = lox + size - 1, hiy = loy + size - 1;
+	const PrimitiveSetup *setup = static_cast<const PrimitiveSetup *>(args.setup);
+	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(args.prims);
+	uint16_t *list = args.lists + size_t(tile) * args.list_stride;
+
+	__shared__ uint32_t wave_counts[4];
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	uint32_t written = 0;
+
+	for (uint32_t base = 0; base < args.num_primitives; base += 256u)
+	{
+		const uint32_t i = base + threadIdx.x;
+		bool hit = false;
+		if (i < args.num_primitives)
+		{
+			const uint32_t st = prims[i].state;
+			const bool ms_tri = (st & ST_MULTISAMPLE) && !(st & ST_LINE);
+			hit = intersects_tile(setup[i], lox, loy, hix, hiy, ms_tri);
+		}
+		const uint64_t ballot = __ballot(hit);
+		if (lane == 0)
+			wave_counts[wave] = uint32_t(__popcll(ballot));
+		__syncthreads();
+		uint32_t before = 0;
+		for (uint32_t w = 0; w < wave; w++)
+			before += wave_counts[w];
+		const uint32_t total = wave_counts[0] + wave_counts[1] + wave_counts[2] + wave_counts[3];
+		if (hit)
+		{
+			const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(ballot >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(ballot), 0u));
+			list[written + before + rank] = uint16_t(i);
+		}
+		written += total;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		args.counts[tile] = written;
+}
+}
+}
+
+// ---------------------------------------------------------------------------------------
+// Shading lives in kernels_shade.hip; launchers for setup and binning:
+
+extern "C" int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stream)
+{
+	if (!args || !args->positions || !args->attributes || !args->prims || !args->setup || !args->tattr)
+		return -int(hipErrorInvalidValue);
+	if (args->num_primitives == 0)
+		return 0;
+	hipLaunchKernelGGL(pgs::k_triangle_setup, dim3((args->num_primitives + 63u) / 64u), dim3(64), 0, stream, *args);
+	hipError_t err = hipGetLastError();
+	return err == hipSuccess ? 0 : -int(err);
+}
+
+extern "C" int pgs_k_binning(const pgs_bin_args *args, hipStream_t stream)
+{
+	if (!args || !args->setup || !args->prims || !args->lists || !args->counts)
+		return -int(hipErrorInvalidValue);
+	if (args->num_primitives > 65536u || args->list_stride < args->num_primitives)
+		return -int(hipErrorInvalidValue);
+	const uint32_t tiles = args->tiles_x * args->tiles_y;
+	if (tiles == 0)
+		return 0;
+	hipLaunchKernelGGL(pgs::k_binning, dim3(tiles), dim3(256), 0, stream, *args);
+	hipError_t err = hipGetLastError();
+	return err == hipSuccess ? 0 : -int(err);
+}
+
+// Uploads the reciprocal seed table and float LUT (gs/gs_renderer.cpp:3600-3640).
+extern "C" int pgs_k_init_tables(const uint8_t *seed, const float *lut)
+{
+	hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(pgs::c_rcp_seed), seed, 256);
+	if (e == hipSuccess)
+		e = hipMemcpyToSymbol(HIP_SYMBOL(pgs::c_rcp_float_lut), lut, 256 * sizeof(float));
+	return e == hipSuccess ? 0 : -int(e);
+}

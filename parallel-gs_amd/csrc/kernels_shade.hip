@@ -1,0 +1,401 @@
+// Shading kernel for gfx950 (replaces ubershader.comp).
+//
+// One wave64 owns one 8x8 block of pixels: lane L shades pixel (L & 7, L >> 3) of the block.
+// Colour and depth are loaded into VGPRs once (begin_tile, ubershader.comp:435-562), every
+// primitive binned to the block's coarse tile is re-tested against the block with one lane per
+// primitive (64 primitives per ballot), and the hits are applied in ascending primitive order
+// with the primitive's setup read through the scalar cache. Dirty values are written back once
+// at the end (end_tile, ubershader.comp:581-780).
+//
+// Workgroups of four waves cover 16x16 pixels. Workgroup ids are remapped so that the eight
+// XCDs (which receive workgroups round-robin) each own a contiguous band of the screen and
+// keep that band's primitive data in their own L2.
+#include "gs_defs.hpp"
+#include "gs_raster.hpp"
+#include "pgs_kernels.h"
+
+namespace pgs
+{
+namespace
+{
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
+__device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
+__device__ __forceinline__ int32_t idot3(int32_t e0, int32_t e1, int32_t e2, int32_t x, int32_t y)
+{
+	return wadd(wadd(wmul(e0, x), wmul(e1, y)), e2);
+}
+
+__device__ __forceinline__ float unorm8(uint32_t c, uint32_t ch) { return __fdiv_rn(float((c >> (8 * ch)) & 0xffu), 255.0f); }
+
+__device__ __forceinline__ uint32_t pack_unorm(float r, float g, float b, float a)
+{
+	auto q = [](float c) { return uint32_t(rintf(__fmul_rn(fminf(fmaxf(c, 0.0f), 1.0f), 255.0f))); };
+	return q(r) | (q(g) << 8) | (q(b) << 16) | (q(a) << 24);
+}
+
+__device__ __forceinline__ uint32_t rgba16_to_rgba32(uint32_t c16)
+{
+	return ((c16 & 0x1fu) << 3) | (((c16 >> 5) & 0x1fu) << 11) | (((c16 >> 10) & 0x1fu) << 19) |
+	       (((c16 >> 15) & 1u) ? 0x80000000u : 0u);
+}
+__device__ __forceinline__ uint32_t rgba32_to_rgba16(uint32_t c)
+{
+	return ((c >> 3) & 0x1fu) | (((c >> 11) & 0x1fu) << 5) | (((c >> 19) & 0x1fu) << 10) | ((c >> 31) << 15);
+}
+
+__device__ __forceinline__ bool intersects_block(const PrimitiveSetup &s, int32_t tlox, int32_t tloy, int32_t thix,
+                                                 int32_t thiy, bool multisampled_tri)
+{
+	int32_t lox = max(s.bb[0], tlox), loy = max(s.bb[1], tloy);
+	int32_t hix = min(s.bb[2], thix), hiy = min(s.bb[3], thiy);
+	if (hix < lox || hiy < loy)
+		return false;
+	const bool para = (s.a[0] & 1) != 0;
+	const int32_t offb = int32_t((uint32_t(s.a[0]) >> 2) & 3u) - 1;
+	const int32_t offc = int32_t((uint32_t(s.a[0]) >> 4) & 3u) - 1;
+	const int32_t ax = s.a[0] >> 6;
+	lox <<= 3; loy <<= 3;
+	hix = (hix << 3) | 7; hiy = (hiy << 3) | 7;
+	int32_t da = idot3(ax, s.a[1], s.a[2], ax >= 0 ? hix : lox, s.a[1] >= 0 ? hiy : loy);
+	int32_t db = idot3(s.b[0], s.b[1], s.b[2], s.b[0] >= 0 ? hix : lox, s.b[1] >= 0 ? hiy : loy);
+	int32_t dc = idot3(s.c[0], s.c[1], s.c[2], s.c[0] >= 0 ? hix : lox, s.c[1] >= 0 ? hiy : loy);
+	if (multisampled_tri && !para)
+	{
+		da = wadd(da, wmul(8, max(abs(ax), abs(s.a[1]))));
+		db = wadd(db, wmul(8, max(abs(s.b[0]), abs(s.b[1]))));
+		dc = wadd(dc, wmul(8, max(abs(s.c[0]), abs(s.c[1]))));
+	}
+	bool hit = da >= 0 && db >= 0 && dc >= 0;
+	if (para && !hit)
+	{
+		const bool mxa = ax >= 0, mya = s.a[1] >= 0;
+		int32_t pa = idot3(ax, s.a[1], s.a[2], mxa ? hix : lox, mya ? hiy : loy);
+		int32_t pb = idot3(s.b[0], s.b[1], s.b[2], -s.c[0] >= 0 ? hix : lox, -s.c[1] >= 0 ? hiy : loy);
+		int32_t pc = idot3(s.c[0], s.c[1], s.c[2], -s.b[0] >= 0 ? hix : lox, -s.b[1] >= 0 ? hiy : loy);
+		pb = wadd(pb, wadd(pa, offb));
+		pc = wadd(pc, wadd(pa, offc));
+		int32_t na = idot3(ax, s.a[1], s.a[2], mxa ? lox : hix, mya ? loy : hiy);
+		hit = -1 >= na && pb >= 0 && pc >= 0;
+	}
+	return hit;
+}
+
+__device__ __forceinline__ bool alpha_test(uint32_t atst, uint32_t aref, float a)
+{
+	const float r = float(aref);
+	switch (atst)
+	{
+	case 0: return false;
+	case 2: return a < r;
+	case 3: return a <= r;
+	case 4: return a == r;
+	case 5: return a >= r;
+	case 6: return a > r;
+	case 7: return a != r;
+	default: return true;
+	}
+}
+
+struct PassInfo
+{
+	uint32_t fb_psm, z_psm;
+	bool fb16, fb24, fb32, z16, z24, zsens;
+};
+
+// Applies one primitive to this lane's pixel (ubershader.comp shade_early + shade_resolve).
+__device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassInfo &pi, uint32_t prim_index,
+                                            int32_t px, int32_t py, uint32_t &color, uint32_t &depth, bool &cdirty,
+                                            bool &zdirty)
+{
+	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
+	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
+	const PrimitiveSetup &s = setups[prim_index];
+	const uint32_t st = prims[prim_index].state;
+
+	// Coverage (intersect.h:156-244).
+	if (!(px >= s.bb[0] && py >= s.bb[1] && px <= s.bb[2] && py <= s.bb[3]))
+		return;
+	const bool para = (s.a[0] & 1) != 0;
+	const int32_t offb = int32_t((uint32_t(s.a[0]) >> 2) & 3u) - 1;
+	const int32_t offc = int32_t((uint32_t(s.a[0]) >> 4) & 3u) - 1;
+	const int32_t sx = int32_t(uint32_t(px) << 3), sy = int32_t(uint32_t(py) << 3);
+	int32_t A = idot3(s.a[0] >> 6, s.a[1], s.a[2], sx, sy);
+	int32_t B = idot3(s.b[0], s.b[1], s.b[2], sx, sy);
+	int32_t C = idot3(s.c[0], s.c[1], s.c[2], sx, sy);
+	const float fi = __fadd_rn(__fmul_rn(float(B), s.inv_area), s.error_i);
+	const float fj = __fadd_rn(__fmul_rn(float(C), s.inv_area), s.error_j);
+	if (para && A < 0)
+	{
+		B = wadd(B, wadd(A, offb));
+		C = wadd(C, wadd(A, offc));
+		A = 0;
+	}
+	if (!(A >= 0 && B >= 0 && C >= 0))
+		return;
+	if (st & (ST_SCANMSK_EVEN | ST_SCANMSK_ODD))
+	{
+		if ((py & 1) == 0 ? (st & ST_SCANMSK_EVEN) : (st & ST_SCANMSK_ODD))
+			return;
+	}
+
+	// Z (ubershader.comp:852-871).
+	uint32_t z = s.z[0];
+	{
+		const float t0 = __fmul_rn(__uint_as_float(s.z[1]), fi);
+		const float t1 = __fmul_rn(__uint_as_float(s.z[2]), fj);
+		float zf = rintf(__fadd_rn(t0, t1));
+		zf = fminf(fmaxf(zf, 0.0f), 4294967040.0f);
+		uint32_t dz = uint32_t(zf);
+		dz = min(dz, 0xffffffffu - z);
+		z += dz;
+		if (pi.z16) z = min(z, 0xffffu);
+		else if (pi.z24) z = min(z, 0xffffffu);
+	}
+	const bool z_test = pi.zsens && (st & ST_Z_TEST);
+	const bool z_write = pi.zsens && (st & ST_Z_WRITE);
+	if (z_test && !((st & ST_Z_GREATER) ? z > depth : z >= depth))
+		return;
+
+	const PrimitiveAttribute &pa = prims[prim_index];
+	const StateVector sv = static_cast<const StateVector *>(a.states)[st & STATE_INDEX_MASK];
+	const TransformedAttributes &ta = static_cast<const TransformedAttributes *>(a.tattr)[prim_index];
+
+	// Combined colour (ubershader.comp:1260-1339).
+	float comb[4], col[4];
+	for (uint32_t ch = 0; ch < 4; ch++)
+	{
+		const float c0 = unorm8(ta.rgba[0], ch), c1 = unorm8(ta.rgba[1], ch), c2 = unorm8(ta.rgba[2], ch);
+		const float e1 = __fmul_rn(__fsub_rn(c1, c0), fi);
+		const float e2 = __fmul_rn(__fsub_rn(c2, c0), fj);
+		const float sum = __fadd_rn(__fadd_rn(c0, e1), e2);
+		col[ch] = floorf(__fadd_rn(__fmul_rn(255.0f, sum), 1.0f / 256.0f));
+		comb[ch] = col[ch];
+	}
+	if (sv.combiner & COMB_FOG)
+	{
+		const float e1 = __fmul_rn(ta.stqf1[3], fi), e2 = __fmul_rn(ta.stqf2[3], fj);
+		const float fog = rintf(__fadd_rn(__fadd_rn(ta.stqf0[3], e1), e2));
+		for (uint32_t k = 0; k < 3; k++)
+		{
+			const float fc = __fmul_rn(unorm8(pa.fogcol, k), 255.0f);
+			const float d = __fmul_rn(__fsub_rn(comb[k], fc), fog);
+			comb[k] = floorf(__fadd_rn(__fadd_rn(fc, __fmul_rn(d, 1.0f / 256.0f)), 0.5f));
+		}
+	}
+
+	// shade_resolve (ubershader.comp:1464-1607).
+	float fb[4];
+	for (uint32_t k = 0; k < 4; k++)
+		fb[k] = rintf(__fmul_rn(unorm8(color, k), 255.0f));
+	bool write_fb = true, write_zb = true, write_alpha = true;
+	const uint32_t bm = sv.blend_mode;
+	bool blending = bm & BLEND_ABE;
+	if (bm & BLEND_ATE)
+	{
+		if (!alpha_test((bm >> 2) & 7u, pa.alpha & 0xffu, comb[3]))
+		{
+			switch ((bm >> 5) & 3u)
+			{
+			case 0: return;
+			case 1: write_zb = false; break;
+			case 2: write_fb = false; write_alpha = false; break;
+			default:
+				if (pi.fb32) write_alpha = false;
+				write_zb = false;
+				break;
+			}
+		}
+	}
+	if (!pi.fb24 && (bm & BLEND_DATE))
+	{
+		const uint32_t bit = fb[3] >= 128.0f ? 1u : 0u;
+		if (uint32_t((bm & BLEND_DATM) != 0) != bit)
+			return;
+	}
+	if ((bm & BLEND_PABE) && comb[3] < 128.0f)
+		blending = false;
+	float o[4] = { comb[0], comb[1], comb[2], comb[3] };
+	if (blending)
+	{
+		const uint32_t am = (bm >> 11) & 3u, bmd = (bm >> 13) & 3u, cm = (bm >> 15) & 3u, dm = (bm >> 17) & 3u;
+		float C = 0.0f;
+		if (cm == 0) C = comb[3];
+		else if (cm == 1) C = pi.fb24 ? 128.0f : fb[3];
+		else if (cm == 2) C = float((pa.alpha >> 8) & 0xffu);
+		for (int k = 0; k < 3; k++)
+		{
+			const float A_ = am == 0 ? comb[k] : am == 1 ? fb[k] : 0.0f;
+			const float B_ = bmd == 0 ? comb[k] : bmd == 1 ? fb[k] : 0.0f;
+			const float D_ = dm == 0 ? comb[k] : dm == 1 ? fb[k] : 0.0f;
+			o[k] = __fadd_rn(floorf(__fmul_rn(__fmul_rn(__fsub_rn(A_, B_), C), 1.0f / 128.0f)), D_);
+		}
+	}
+	if (bm & BLEND_DTHE)
+	{
+		const int32_t dxp = px & 3, dyp = py & 3;
+		const int32_t bit_index = ((dyp * 4 + dxp) * 4) & 31;
+		const uint32_t dmx = dyp < 2 ? sv.dimx0 : sv.dimx1;
+		const int32_t dv = int32_t(dmx << (29 - bit_index)) >> 29;
+		for (int k = 0; k < 3; k++)
+			o[k] = __fadd_rn(o[k], float(dv));
+	}
+	if (pi.fb24)
+		write_alpha = false;
+	if (!write_alpha)
+		o[3] = fb[3];
+	if (!write_fb)
+	{
+		o[0] = fb[0]; o[1] = fb[1]; o[2] = fb[2];
+	}
+	float n[4];
+	if (!(bm & BLEND_COLCLAMP))
+	{
+		for (int k = 0; k < 4; k++)
+		{
+			const float q = __fdiv_rn(o[k], 256.0f);
+			n[k] = __fmul_rn(__fsub_rn(q, floorf(q)), 256.0f / 255.0f);
+		}
+	}
+	else
+	{
+		for (int k = 0; k < 4; k++)
+			n[k] = __fmul_rn(o[k], 1.0f / 255.0f);
+	}
+	uint32_t packed = pack_unorm(n[0], n[1], n[2], n[3]);
+	packed |= uint32_t((bm & BLEND_FBA) && write_alpha) << 31;
+
+	if (pa.fbmsk != ~0u && (write_fb || write_alpha))
+	{
+		color = pa.fbmsk ? ((packed & ~pa.fbmsk) | (color & pa.fbmsk)) : packed;
+		if (pi.fb16)
+			color &= 0x80f8f8f8u;
+		cdirty = true;
+	}
+	if (z_write && write_zb)
+	{
+		depth = z;
+		zdirty = true;
+	}
+}
+
+__global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a)
+{
+	// XCD-aware remap: workgroup w runs on XCD (w % 8); give each XCD a contiguous range.
+	const uint32_t nwg = gridDim.x;
+	const uint32_t per_xcd = nwg / 8u;
+	const uint32_t logical = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+
+	const uint32_t quads_x = (a.blocks_x + 1u) / 2u;
+	const uint32_t qx = logical % quads_x, qy = logical / quads_x;
+	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+	const uint32_t bx = qx * 2u + (wave & 1u), by = qy * 2u + (wave >> 1);
+	if (bx >= a.blocks_x || by >= a.blocks_y)
+		return;
+
+	const int32_t blox = a.origin_x + int32_t(bx) * 8, bloy = a.origin_y + int32_t(by) * 8;
+	const int32_t px = blox + int32_t(lane & 7u), py = bloy + int32_t(lane >> 3);
+	const uint32_t shift = a.tile_log2 - 3u;
+	const uint32_t tile = (by >> shift) * a.tiles_x + (bx >> shift);
+	const uint32_t count = uni(a.counts[tile]);
+	if (count == 0)
+		return;
+	const uint16_t *list = a.lists + size_t(tile) * a.list_stride;
+
+	PassInfo pi;
+	pi.fb_psm = a.fb_psm;
+	pi.z_psm = a.z_psm;
+	pi.fb16 = psm_is_16(a.fb_psm);
+	pi.fb24 = a.fb_psm == PSMCT24 || a.fb_psm == PSMZ24;
+	pi.fb32 = a.fb_psm == PSMCT32 || a.fb_psm == PSMZ32;
+	pi.zsens = a.z_psm != 0xffffffffu;
+	pi.z16 = pi.zsens && psm_is_16(a.z_psm);
+	pi.z24 = pi.zsens && (a.z_psm == PSMCT24 || a.z_psm == PSMZ24);
+
+	// begin_tile
+	const uint32_t caddr = swizzle_addr(uint32_t(px), uint32_t(py), a.fbp * 32u, a.fbw, a.fb_psm);
+	uint32_t color;
+	if (pi.fb16)
+		color = rgba16_to_rgba32(*reinterpret_cast<const uint16_t *>(a.vram + caddr));
+	else
+		color = *reinterpret_cast<const uint32_t *>(a.vram + caddr);
+	uint32_t zaddr = 0, depth = 0;
+	if (pi.zsens)
+	{
+		zaddr = swizzle_addr(uint32_t(px), uint32_t(py), a.zbp * 32u, a.fbw, a.z_psm);
+		if (pi.z16)
+			depth = *reinterpret_cast<const uint16_t *>(a.vram + zaddr);
+		else
+		{
+			depth = *reinterpret_cast<const uint32_t *>(a.vram + zaddr);
+			if (pi.z24)
+				depth &= 0xffffffu;
+		}
+	}
+	bool cdirty = false, zdirty = false;
+
+	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
+	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
+	for (uint32_t base = 0; base < count; base += 64u)
+	{
+		const uint32_t li = base + lane;
+		uint32_t idx = 0;
+		bool hit = false;
+		if (li < count)
+		{
+			idx = list[li];
+			const uint32_t st = prims[idx].state;
+			hit = intersects_block(setups[idx], blox, bloy, blox + 7, bloy + 7, (st & ST_MULTISAMPLE) && !(st & ST_LINE));
+		}
+		uint64_t ballot = __ballot(hit);
+		while (ballot)
+		{
+			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
+			ballot &= ballot - 1u;
+			const uint32_t prim = uni(uint32_t(__shfl(int(idx), int(bit))));
+			shade_pixel(a, pi, prim, px, py, color, depth, cdirty, zdirty);
+		}
+	}
+
+	// end_tile
+	if (cdirty)
+	{
+		if (pi.fb16)
+			*reinterpret_cast<uint16_t *>(a.vram + caddr) = uint16_t(rgba32_to_rgba16(color));
+		else
+			*reinterpret_cast<uint32_t *>(a.vram + caddr) = color;
+	}
+	if (zdirty)
+	{
+		if (pi.z16)
+			*reinterpret_cast<uint16_t *>(a.vram + zaddr) = uint16_t(depth);
+		else if (pi.z24)
+		{
+			a.vram[zaddr] = uint8_t(depth);
+			a.vram[zaddr + 1] = uint8_t(depth >> 8);
+			a.vram[zaddr + 2] = uint8_t(depth >> 16);
+		}
+		else
+			*reinterpret_cast<uint32_t *>(a.vram + zaddr) = depth;
+	}
+}
+}
+}
+
+extern "C" int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream)
+{
+	if (!args || !args->vram || !args->setup || !args->tattr || !args->prims || !args->states || !args->lists ||
+	    !args->counts)
+		return -int(hipErrorInvalidValue);
+	if (args->tile_log2 < 3 || args->tile_log2 > 7)
+		return -int(hipErrorInvalidValue);
+	const uint32_t quads = ((args->blocks_x + 1u) / 2u) * ((args->blocks_y + 1u) / 2u);
+	if (quads == 0)
+		return 0;
+	if (args->rate_log2 != 0)
+		return -int(hipErrorNotSupported);
+	const uint32_t groups = (quads + 7u) & ~7u;
+	hipLaunchKernelGGL(pgs::k_shade_1x, dim3(groups), dim3(256), 0, stream, *args);
+	hipError_t err = hipGetLastError();
+	return err == hipSuccess ? 0 : -int(err);
+}

@@ -1,0 +1,460 @@
+// Primitive assembly and render-pass batching.
+#include "render_pass.hpp"
+#include "gs_interface.hpp"
+#include <algorithm>
+#include <cstring>
+#include <cstdlib>
+
+namespace pgs
+{
+namespace
+{
+inline uint32_t field(uint64_t v, uint32_t off, uint32_t bits) { return uint32_t((v >> off) & ((1ull << bits) - 1ull)); }
+inline int32_t wrapmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
+
+// gs/gs_util.cpp:237-310
+bool parallelogram_candidate(const VertexPosition *p, const VertexAttribute *a, int32_t lox, int32_t loy,
+                             int32_t hix, int32_t hiy, uint64_t prim, int32_t order[3])
+{
+	const uint32_t type = field(prim, 0, 3);
+	if (field(prim, 7, 1) && type != 6 && type != 0)
+		return false;
+	const bool fst = field(prim, 8, 1), tme = field(prim, 4, 1), fge = field(prim, 5, 1);
+	if (!fst && tme && (a[0].q != a[1].q || a[1].q != a[2].q))
+		return false;
+	if (p[0].z != p[1].z || p[1].z != p[2].z)
+		return false;
+	if (fge && (a[0].fog != a[1].fog || a[1].fog != a[2].fog))
+		return false;
+	if (a[0].rgba != a[1].rgba || a[1].rgba != a[2].rgba)
+		return false;
+	const int32_t abx = p[1].x - p[0].x, aby = p[1].y - p[0].y;
+	const int32_t acx = p[2].x - p[0].x, acy = p[2].y - p[0].y;
+	const int32_t bcx = p[2].x - p[1].x, bcy = p[2].y - p[1].y;
+	const int32_t area = std::abs(int32_t(uint32_t(wrapmul(abx, acy)) - uint32_t(wrapmul(aby, acx))));
+	if (area != wrapmul(hix - lox, hiy - loy))
+		return false;
+	if (abx != 0 && aby != 0)
+	{
+		order[0] = 2; order[1] = 1; order[2] = 0;
+		if (acx * bcx + acy * bcy != 0)
+			return false;
+	}
+	else if (acx != 0 && acy != 0)
+	{
+		order[0] = 1; order[1] = 2; order[2] = 0;
+		if (acx * abx + acy * aby != 0)
+			return false;
+	}
+	else
+	{
+		order[0] = 0; order[1] = 1; order[2] = 2;
+		if (acx * abx + acy * aby != 0)
+			return false;
+	}
+	if (p[order[1]].x > p[order[2]].x)
+		std::swap(order[1], order[2]);
+	return true;
+}
+
+// gs/gs_util.cpp:312-377
+bool triangles_form_parallelogram(const VertexPosition *p, const VertexAttribute *a, const int32_t o[3],
+                                  const VertexPosition *lp, const VertexAttribute *la, const int32_t lo[3],
+                                  uint64_t prim)
+{
+	const int32_t p3x = p[o[1]].x + p[o[2]].x - p[o[0]].x, p3y = p[o[1]].y + p[o[2]].y - p[o[0]].y;
+	if (p3x != lp[lo[0]].x || p3y != lp[lo[0]].y || p[o[1]].x != lp[lo[1]].x || p[o[1]].y != lp[lo[1]].y ||
+	    p[o[2]].x != lp[lo[2]].x || p[o[2]].y != lp[lo[2]].y)
+		return false;
+	const bool fst = field(prim, 8, 1), tme = field(prim, 4, 1), fge = field(prim, 5, 1);
+	if (p[0].z != lp[0].z)
+		return false;
+	if (!fst && tme && a[0].q != la[0].q)
+		return false;
+	if (fge && a[0].fog != la[0].fog)
+		return false;
+	if (a[0].rgba != la[0].rgba)
+		return false;
+	if (tme)
+	{
+		if (fst)
+		{
+			const uint32_t u3 = ((a[o[1]].uv & 0xffffu) + (a[o[2]].uv & 0xffffu) - (a[o[0]].uv & 0xffffu)) & 0xffffu;
+			const uint32_t v3 = ((a[o[1]].uv >> 16) + (a[o[2]].uv >> 16) - (a[o[0]].uv >> 16)) & 0xffffu;
+			if (u3 != (la[lo[0]].uv & 0xffffu) || v3 != (la[lo[0]].uv >> 16) || a[o[1]].uv != la[lo[1]].uv ||
+			    a[o[2]].uv != la[lo[2]].uv)
+				return false;
+		}
+		else
+		{
+			const float es = std::fabs((a[o[2]].s + a[o[1]].s - a[o[0]].s - la[lo[0]].s) / a[0].q);
+			const float et = std::fabs((a[o[2]].t + a[o[1]].t - a[o[0]].t - la[lo[0]].t) / a[0].q);
+			if (a[o[1]].s != la[lo[1]].s || a[o[1]].t != la[lo[1]].t || a[o[2]].s != la[lo[2]].s ||
+			    a[o[2]].t != la[lo[2]].t || es > 1e-4f || et > 1e-4f)
+				return false;
+		}
+	}
+	return true;
+}
+}
+
+void RenderPassBuilder::init(Renderer *renderer, uint32_t sample_log2)
+{
+	renderer_ = renderer;
+	sample_log2_ = sample_log2;
+	// Samples per axis of the ordered grids: 2x and 4x use 2, 8x and 16x use 4
+	// (gs/gs_interface.cpp:65-118).
+	rate_log2_ = sample_log2 ? (sample_log2 + 1) / 2 : 0;
+	pos_.reserve(3u * 65536u);
+	attr_.reserve(3u * 65536u);
+	prims_.reserve(65536u);
+}
+
+void RenderPassBuilder::vertex_kick(const RegisterState &r, uint32_t x, uint32_t y, uint32_t z, float fog, bool adc)
+{
+	if (vq_count_ == 3)
+	{
+		vq_[0] = vq_[1];
+		vq_[1] = vq_[2];
+		vq_count_ = 2;
+	}
+	const uint32_t ctxt = field(r.prim, 9, 1);
+	QueuedVertex &v = vq_[vq_count_++];
+	v.pos.x = int32_t(x) - int32_t(field(r.xyoffset[ctxt], 0, 16));
+	v.pos.y = int32_t(y) - int32_t(field(r.xyoffset[ctxt], 32, 16));
+	v.pos.z = z;
+	v.pos.pad = 0;
+	uint32_t sbits = uint32_t(r.st), tbits = uint32_t(r.st >> 32), qbits = uint32_t(r.rgbaq >> 32);
+	memcpy(&v.attr.s, &sbits, 4);
+	memcpy(&v.attr.t, &tbits, 4);
+	memcpy(&v.attr.q, &qbits, 4);
+	v.attr.rgba = uint32_t(r.rgbaq);
+	v.attr.fog = fog;
+	v.attr.uv = field(r.uv, 0, 14) | (field(r.uv, 16, 14) << 16);
+
+	// gs/gs_interface.cpp:2838-2928
+	const uint32_t type = field(r.prim, 0, 3);
+	static const uint32_t nverts[8] = { 1, 2, 2, 3, 3, 3, 2, 0 };
+	const uint32_t n = nverts[type];
+	if (n == 0)
+	{
+		vq_count_ = 0;
+		return;
+	}
+	if (vq_count_ < n)
+		return;
+	if (!adc && !draw_is_degenerate(r))
+	{
+		QueuedVertex v3[3];
+		if (n == 1)
+		{
+			v3[0] = vq_[vq_count_ - 1];
+			v3[0].pos.x -= 7;
+			v3[0].pos.y -= 7;
+			v3[1] = v3[0];
+			v3[1].pos.x += 16;
+			v3[1].pos.y += 16;
+			v3[2] = v3[0];
+		}
+		else if (n == 2)
+		{
+			v3[0] = vq_[vq_count_ - 1];
+			v3[1] = vq_[vq_count_ - 2];
+			v3[2] = v3[0];
+		}
+		else
+		{
+			v3[0] = vq_[2];
+			v3[1] = vq_[1];
+			v3[2] = vq_[0];
+		}
+		append(r, v3, n, type == 0 || type == 6);
+	}
+	if (type == 5)
+	{
+		vq_[1] = vq_[2];
+		vq_count_ = 2;
+	}
+	else if (type == 0 || type == 1 || type == 3 || type == 6)
+		vq_count_ = 0;
+}
+
+// gs/gs_interface.cpp:2036-2115
+bool RenderPassBuilder::draw_is_degenerate(const RegisterState &r) const
+{
+	const uint32_t ctx = field(r.prim, 9, 1);
+	const uint64_t sc = r.scissor[ctx], test = r.test[ctx], frame = r.frame[ctx], zbuf = r.zbuf[ctx], alpha = r.alpha[ctx];
+	if (field(sc, 0, 11) > field(sc, 16, 11) || field(sc, 32, 11) > field(sc, 48, 11))
+		return true;
+	const uint32_t ate = field(test, 0, 1), atst = field(test, 1, 3), afail = field(test, 12, 2);
+	const uint32_t zte = field(test, 16, 1), ztst = field(test, 17, 2);
+	if (zte && ztst == 0)
+		return true;
+	if (ate && atst == 0 && afail == 0)
+		return true;
+	uint32_t fbmsk = uint32_t(frame >> 32);
+	uint32_t zmsk = field(zbuf, 32, 1);
+	const uint32_t psm = field(frame, 24, 6);
+	if (psm == PSMCT24 || psm == PSMZ24)
+		fbmsk |= 0xffu << 24;
+	if (ate && atst == 0)
+	{
+		if (afail == 2) fbmsk = ~0u;
+		else if (afail == 0) { fbmsk = ~0u; zmsk = 1; }
+		else if (afail == 1) zmsk = 1;
+		else fbmsk |= 0xffu << 24;
+	}
+	if (field(r.prim, 6, 1) && field(alpha, 6, 2) == 1 && field(alpha, 0, 2) == field(alpha, 2, 2))
+		fbmsk |= 0xffffffu;
+	return fbmsk == ~0u && (zmsk || zte == 0);
+}
+
+uint32_t RenderPassBuilder::state_index(const StateVector &sv)
+{
+	uint64_t key = (uint64_t(sv.blend_mode) << 32) ^ (uint64_t(sv.combiner) << 24) ^
+	               (uint64_t(sv.dimx0) * 0x9e3779b97f4a7c15ull) ^ (uint64_t(sv.dimx1) * 0xc2b2ae3d27d4eb4full);
+	auto it = state_map_.find(key);
+	if (it != state_map_.end())
+	{
+		const StateVector &e = states_[it->second];
+		if (e.combiner == sv.combiner && e.blend_mode == sv.blend_mode && e.dimx0 == sv.dimx0 && e.dimx1 == sv.dimx1)
+			return it->second;
+		// Hash collision: fall back to a linear search.
+		for (uint32_t i = 0; i < states_.size(); i++)
+			if (states_[i].combiner == sv.combiner && states_[i].blend_mode == sv.blend_mode &&
+			    states_[i].dimx0 == sv.dimx0 && states_[i].dimx1 == sv.dimx1)
+				return i;
+	}
+	uint32_t idx = uint32_t(states_.size());
+	states_.push_back(sv);
+	state_map_.emplace(key, idx);
+	return idx;
+}
+
+// gs/gs_interface.cpp:2473-2836 (one frame-buffer instance per render pass).
+void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, uint32_t n, bool quad)
+{
+	VertexPosition p[3] = { vin[0].pos, vin[1].pos, vin[2].pos };
+	VertexAttribute a[3] = { vin[0].attr, vin[1].attr, vin[2].attr };
+	const uint64_t prim = r.prim;
+	const uint32_t ctx = field(prim, 9, 1);
+	const bool is_line = !quad && n == 2;
+	const uint32_t type = field(prim, 0, 3);
+	const bool aa1 = field(prim, 7, 1) && type != 6 && type != 0;
+
+	int32_t lox = std::min(p[0].x, p[1].x), loy = std::min(p[0].y, p[1].y);
+	int32_t hix = std::max(p[0].x, p[1].x), hiy = std::max(p[0].y, p[1].y);
+	if (!quad && !is_line)
+	{
+		lox = std::min(lox, p[2].x); loy = std::min(loy, p[2].y);
+		hix = std::max(hix, p[2].x); hiy = std::max(hiy, p[2].y);
+	}
+	const int32_t pre_lox = lox, pre_loy = loy, pre_hix = hix, pre_hiy = hiy;
+	hix -= 1; hiy -= 1;
+	lox += (1 << (4 - int32_t(rate_log2_))) - 1;
+	loy += (1 << (4 - int32_t(rate_log2_))) - 1;
+	lox >>= 4; loy >>= 4; hix >>= 4; hiy >>= 4;
+	if (is_line || aa1)
+	{
+		lox -= 1; loy -= 1; hix += 1; hiy += 1;
+	}
+	const uint64_t sc = r.scissor[ctx];
+	lox = std::max(lox, int32_t(field(sc, 0, 11)));
+	loy = std::max(loy, int32_t(field(sc, 32, 11)));
+	hix = std::min(hix, int32_t(field(sc, 16, 11)));
+	hiy = std::min(hiy, int32_t(field(sc, 48, 11)));
+	if (hix < lox || hiy < loy)
+		return;
+
+	if (field(prim, 4, 1) && host)
+		host->before_texture_read();
+
+	// Frame / depth buffer change ends the render pass.
+	const uint64_t frame = r.frame[ctx], zbuf = r.zbuf[ctx];
+	if (!prims_.empty())
+	{
+		bool pass_z = false;
+		for (auto &pp : prims_)
+			pass_z |= (pp.state & (ST_Z_TEST | ST_Z_WRITE)) != 0;
+		if ((frame_ & 0x3f3f01ffull) != (frame & 0x3f3f01ffull) ||
+		    ((zbuf_ & 0x0f0001ffull) != (zbuf & 0x0f0001ffull) && pass_z))
+			flush();
+	}
+
+	bool candidate = false;
+	int32_t order[3] = { 0, 1, 2 };
+	if (n == 3)
+	{
+		candidate = parallelogram_candidate(p, a, pre_lox, pre_loy, pre_hix, pre_hiy, prim, order);
+		if (!state_dirty_ && candidate && last_candidate_ && !prims_.empty() &&
+		    triangles_form_parallelogram(p, a, order, &pos_[pos_.size() - 3], &attr_[attr_.size() - 3], last_order_,
+		                                 prim))
+		{
+			prims_.back().state |= ST_PARALLELOGRAM | (uint32_t(last_order_[0]) << ST_PROVOKING_SHIFT);
+			last_candidate_ = false;
+			return;
+		}
+		memcpy(last_order_, order, sizeof(order));
+	}
+
+	if (prims_.empty())
+	{
+		if (host)
+			host->before_first_primitive();
+		frame_ = frame;
+		zbuf_ = zbuf;
+		bb_[0] = lox; bb_[1] = loy; bb_[2] = hix; bb_[3] = hiy;
+		color_write_mask_ = 0;
+	}
+	zbuf_ = (zbuf_ & ~0x0f0001ffull) | (zbuf & 0x0f0001ffull);
+	bb_[0] = std::min(bb_[0], lox); bb_[1] = std::min(bb_[1], loy);
+	bb_[2] = std::max(bb_[2], hix); bb_[3] = std::max(bb_[3], hiy);
+
+	const uint64_t test = r.test[ctx], alpha = r.alpha[ctx];
+	const uint32_t ate = field(test, 0, 1), atst = field(test, 1, 3), aref = field(test, 4, 8);
+	const uint32_t afail = field(test, 12, 2), date = field(test, 14, 1), datm = field(test, 15, 1);
+	const uint32_t zte = field(test, 16, 1), ztst = field(test, 17, 2);
+	const bool abe = field(prim, 6, 1), tme = field(prim, 4, 1);
+
+	// State vector (gs/gs_interface.cpp:1065-1128).
+	StateVector sv{};
+	if (field(r.dthe, 0, 1))
+	{
+		sv.blend_mode |= BLEND_DTHE;
+		sv.dimx0 = uint32_t(r.dimx);
+		sv.dimx1 = uint32_t(r.dimx >> 32);
+	}
+	if (ate && atst != 1 && !(atst == 0 && afail == 1))
+		sv.blend_mode |= BLEND_ATE | (atst << 2) | (afail << 5);
+	sv.blend_mode |= date ? BLEND_DATE : 0;
+	sv.blend_mode |= datm ? BLEND_DATM : 0;
+	if (abe || aa1)
+		sv.blend_mode |= field(alpha, 0, 2) << 11 | field(alpha, 2, 2) << 13 | field(alpha, 4, 2) << 15 |
+		                 field(alpha, 6, 2) << 17;
+	if (abe)
+		sv.blend_mode |= BLEND_ABE | (field(r.pabe, 0, 1) ? BLEND_PABE : 0);
+	sv.blend_mode |= field(r.colclamp, 0, 1) ? BLEND_COLCLAMP : 0;
+	sv.blend_mode |= field(r.fba[ctx], 0, 1) ? BLEND_FBA : 0;
+	if (tme)
+	{
+		const uint64_t t0 = r.tex0[ctx];
+		sv.combiner |= COMB_TME | (field(t0, 34, 1) ? COMB_TCC : 0) | field(t0, 35, 2);
+	}
+	sv.combiner |= field(prim, 5, 1) ? COMB_FOG : 0;
+
+	PrimitiveAttribute pa{};
+	uint32_t state = state_index(sv);
+	if (tme)
+	{
+		const uint64_t t1 = r.tex1[ctx], cl = r.clamp[ctx];
+		pa.tex = 0; // texture slot (assigned with the texture cache)
+		pa.tex |= field(t1, 5, 1) ? TEX_MAG_LINEAR : 0;
+		const uint32_t wms = field(cl, 0, 2), wmt = field(cl, 2, 2);
+		pa.tex |= (wms == 1 || wms == 2) ? TEX_CLAMP_S : 0;
+		pa.tex |= (wmt == 1 || wmt == 2) ? TEX_CLAMP_T : 0;
+		pa.tex2 = TEX2_FIXED_LOD;
+	}
+	if (zte)
+	{
+		if (ztst > 1)
+			state |= ST_Z_TEST | (ztst == 3 ? ST_Z_GREATER : 0);
+		const bool implied_z_mask = ate && atst == 0 && afail != 2;
+		if (!implied_z_mask && field(zbuf, 32, 1) == 0)
+			state |= ST_Z_WRITE;
+	}
+	bool needs_prev = false;
+	if (abe || aa1)
+		needs_prev = field(alpha, 0, 2) == 1 || field(alpha, 2, 2) == 1 || field(alpha, 4, 2) == 1 ||
+		             field(alpha, 6, 2) == 1;
+	if (date)
+		needs_prev = true;
+	else if (ate && atst != 1)
+	{
+		const bool can_write_z = (state & ST_Z_WRITE) != 0;
+		if (afail != 1 || (can_write_z && afail != 2))
+			needs_prev = true;
+	}
+	if (!needs_prev)
+		state |= ST_OPAQUE;
+	if (aa1)
+		state |= ST_MULTISAMPLE;
+	const uint32_t msk = field(r.scanmsk, 0, 2);
+	if (msk >= 2)
+		state |= 1u << (23 + msk - 2);
+	if (!field(prim, 8, 1))
+		state |= ST_PERSPECTIVE;
+	if (field(prim, 3, 1))
+		state |= ST_IIP;
+	if (field(prim, 10, 1))
+		state |= ST_FIX;
+	if (quad)
+	{
+		state |= ST_PARALLELOGRAM | ST_SPRITE | ST_SNAP_RASTER | ST_SNAP_ATTRIBUTE;
+		state &= ~ST_MULTISAMPLE;
+	}
+	else if (is_line)
+	{
+		state |= ST_PARALLELOGRAM | ST_LINE;
+		if (state & ST_MULTISAMPLE)
+			state &= ~ST_Z_WRITE;
+	}
+	if (n == 1)
+		state |= ST_FIX | ST_SNAP_RASTER | ST_SNAP_ATTRIBUTE;
+
+	pa.state = state;
+	pa.fbmsk = uint32_t(frame >> 32);
+	pa.fogcol = uint32_t(r.fogcol);
+	pa.alpha = (field(alpha, 32, 8) << 8) | aref;
+	pa.bb[0] = int16_t(lox); pa.bb[1] = int16_t(loy); pa.bb[2] = int16_t(hix); pa.bb[3] = int16_t(hiy);
+	color_write_mask_ |= ~pa.fbmsk;
+
+	prims_.push_back(pa);
+	pos_.insert(pos_.end(), p, p + 3);
+	attr_.insert(attr_.end(), a, a + 3);
+	last_candidate_ = candidate;
+	state_dirty_ = false;
+	if (prims_.size() >= 65536u || states_.size() >= 4096u)
+		flush();
+}
+
+void RenderPassBuilder::flush()
+{
+	if (!prims_.empty())
+	{
+		RenderPassDesc d{};
+		d.positions = pos_.data();
+		d.attributes = attr_.data();
+		d.prims = prims_.data();
+		d.states = states_.data();
+		d.num_primitives = uint32_t(prims_.size());
+		d.num_states = uint32_t(states_.size());
+		d.fb_psm = field(frame_, 24, 6);
+		d.fbp = field(frame_, 0, 9);
+		d.fbw = field(frame_, 16, 6);
+		d.zbp = field(zbuf_, 0, 9);
+		uint32_t zpsm = field(zbuf_, 24, 4) | 0x30u;
+		// deduce_effectice_z_psm (gs/gs_renderer.cpp:2147-2170)
+		if (d.fb_psm == PSMZ32 || d.fb_psm == PSMZ24 || d.fb_psm == PSMZ16S)
+		{
+			if (zpsm == PSMZ32) zpsm = PSMCT32;
+			else if (zpsm == PSMZ24) zpsm = PSMCT24;
+			else if (zpsm == PSMZ16S || zpsm == PSMZ16) zpsm = PSMCT16S;
+		}
+		bool zsens = false;
+		for (auto &pp : prims_)
+			zsens |= (pp.state & (ST_Z_TEST | ST_Z_WRITE)) != 0;
+		d.z_psm = zsens ? zpsm : 0xffffffffu;
+		memcpy(d.bb, bb_, sizeof(bb_));
+		d.opaque_fbmask = ~color_write_mask_;
+		d.rate_log2 = rate_log2_;
+		renderer_->submit_render_pass(d);
+		prims_.clear();
+		pos_.clear();
+		attr_.clear();
+		states_.clear();
+		state_map_.clear();
+	}
+	state_dirty_ = true;
+	last_candidate_ = false;
+}
+}

@@ -1,29 +1,89 @@
 // Primitive assembly and render-pass batching (host side of drawing).
+//
+// Counterpart of the drawing half of GSInterface (gs/gs_interface.cpp:845-2943): vertex
+// kicks, primitive assembly with bounding box / scissor, degenerate-draw rejection,
+// parallelogram fusion, state-vector dedupe and primitive state bits. Primitives accumulate
+// in host arrays until the pass ends; Renderer::submit_render_pass uploads them and runs the
+// setup / binning / shading kernels.
 #pragma once
 #include "gs_defs.hpp"
+#include "gs_raster.hpp"
 #include "renderer.hpp"
 #include <vector>
+#include <unordered_map>
 
 namespace pgs
 {
 struct RegisterState;
 
+struct RenderPassDesc
+{
+	const VertexPosition *positions;
+	const VertexAttribute *attributes;
+	const PrimitiveAttribute *prims;
+	const StateVector *states;
+	uint32_t num_primitives;
+	uint32_t num_states;
+	uint32_t fb_psm, fbp, fbw, z_psm, zbp;
+	int32_t bb[4];            // union of the primitives' scissor-clipped boxes (pixels)
+	uint32_t opaque_fbmask;
+	uint32_t rate_log2;
+};
+
 class RenderPassBuilder
 {
 public:
-	void init(Renderer *renderer, uint32_t sample_log2)
-	{
-		renderer_ = renderer;
-		sample_log2_ = sample_log2;
-	}
-	bool has_primitives() const { return num_primitives_ != 0; }
+	void init(Renderer *renderer, uint32_t sample_log2);
+	bool has_primitives() const { return !prims_.empty(); }
 	void reset_vertex_queue() { vq_count_ = 0; }
-	void flush() {}
+
+	// A register that feeds primitive state changed value (blocks parallelogram fusion).
+	void mark_state_dirty() { state_dirty_ = true; }
+
+	// XYZ2/XYZF2/XYZ3/XYZF3 kick (gs/gs_interface.cpp:858-899, 2883-2928).
+	void vertex_kick(const RegisterState &r, uint32_t x, uint32_t y, uint32_t z, float fog, bool adc);
+
+	// Ends the render pass: submits all pending primitives.
+	void flush();
+
+	// Callbacks into the front end: before_first_primitive runs when a pass starts (pending
+	// transfers must execute first); before_texture_read runs before a textured primitive is
+	// appended (a partially received upload is written first, gs/gs_interface.cpp:2640-2650).
+	struct Host
+	{
+		virtual ~Host() = default;
+		virtual void before_first_primitive() = 0;
+		virtual void before_texture_read() = 0;
+	};
+	Host *host = nullptr;
 
 private:
+	struct QueuedVertex
+	{
+		VertexPosition pos;
+		VertexAttribute attr;
+	};
+
 	Renderer *renderer_ = nullptr;
 	uint32_t sample_log2_ = 0;
-	uint32_t num_primitives_ = 0;
+	uint32_t rate_log2_ = 0;
+	QueuedVertex vq_[3];
 	uint32_t vq_count_ = 0;
+
+	std::vector<VertexPosition> pos_;
+	std::vector<VertexAttribute> attr_;
+	std::vector<PrimitiveAttribute> prims_;
+	std::vector<StateVector> states_;
+	std::unordered_map<uint64_t, uint32_t> state_map_;
+	uint64_t frame_ = 0, zbuf_ = 0;
+	int32_t bb_[4] = { 0, 0, -1, -1 };
+	uint32_t color_write_mask_ = 0;
+	bool state_dirty_ = true;
+	bool last_candidate_ = false;
+	int32_t last_order_[3] = { 0, 1, 2 };
+
+	bool draw_is_degenerate(const RegisterState &r) const;
+	void append(const RegisterState &r, const QueuedVertex *v, uint32_t n, bool quad);
+	uint32_t state_index(const StateVector &sv);
 };
 }

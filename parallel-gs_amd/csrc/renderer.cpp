@@ -1,5 +1,7 @@
 // Device-side renderer: HBM buffers, upload arena, transfer batching and launch recording.
 #include "renderer.hpp"
+#include "render_pass.hpp"
+#include "raster_tables.hpp"
 #include <algorithm>
 #include <cstdio>
 #include <stdexcept>
@@ -49,6 +51,8 @@ Renderer::~Renderer()
 	}
 	for (auto e : event_pool_)
 		(void)hipEventDestroy(e);
+	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_ })
+		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
 	if (stream_) (void)hipStreamDestroy(stream_);
@@ -78,6 +82,21 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes)
 		check(hipHostMalloc(&arena_.host, arena_.cap, hipHostMallocDefault));
 		check(hipMalloc(&arena_.dev, arena_.cap));
 		batch_written_.assign(kNumBlocks / 64, 0);
+		check(hipMalloc(&setup_buf_, 65536u * sizeof(PrimitiveSetup)));
+		check(hipMalloc(&tattr_buf_, 65536u * sizeof(TransformedAttributes)));
+		lists_cap_ = 32u << 20;  // 64 MiB of uint16 entries
+		counts_cap_ = 1u << 16;
+		check(hipMalloc(&lists_buf_, lists_cap_ * sizeof(uint16_t)));
+		check(hipMalloc(&counts_buf_, counts_cap_ * sizeof(uint32_t)));
+		check(hipMalloc(&dummy_buf_, 256));
+		check(hipMemsetAsync(dummy_buf_, 0, 256, stream_));
+		{
+			float lut[256];
+			make_rcp_float_lut(lut);
+			int rc = pgs_k_init_tables(kRcpSeed, lut);
+			if (rc != 0)
+				check(hipError_t(-rc));
+		}
 		block_count_.assign(kNumBlocks, 0);
 		check(hipStreamSynchronize(stream_));
 		return true;
@@ -218,6 +237,9 @@ void Renderer::launch(const Launch &l)
 	{
 	case LaunchKind::VramCopy: rc = pgs_k_vram_copy(&l.copy, stream_); break;
 	case LaunchKind::VramSnapshot: rc = pgs_k_vram_snapshot(&l.snapshot, stream_); break;
+	case LaunchKind::Setup: rc = pgs_k_triangle_setup(&l.setup, stream_); break;
+	case LaunchKind::Binning: rc = pgs_k_binning(&l.bin, stream_); break;
+	case LaunchKind::Shade: rc = pgs_k_shade(&l.shade, stream_); break;
 	}
 	if (rc != 0)
 		check(hipError_t(-rc));
@@ -416,6 +438,105 @@ void Renderer::flush_transfers()
 	touched_blocks_.clear();
 	batch_.clear();
 	std::fill(batch_written_.begin(), batch_written_.end(), 0);
+}
+
+// ---------------------------------------------------------------------------------------
+// Render passes.
+
+void Renderer::submit_render_pass(const RenderPassDesc &d)
+{
+	flush_transfers();
+	const uint32_t n = d.num_primitives;
+	if (n == 0 || n > 65536u)
+		return;
+
+	// Tile grid. At 1x the grid is a free choice (it cannot change a pixel); it starts at the
+	// pass bounding box rounded down to whole 8x8 VRAM blocks.
+	uint32_t tile_log2 = tile_log2_1x_;
+	int32_t ox = d.bb[0] & ~7, oy = d.bb[1] & ~7;
+	auto tiles_for = [&](uint32_t tl, uint32_t &tx, uint32_t &ty) {
+		tx = uint32_t((d.bb[2] - ox) >> tl) + 1u;
+		ty = uint32_t((d.bb[3] - oy) >> tl) + 1u;
+	};
+	uint32_t tiles_x, tiles_y;
+	tiles_for(tile_log2, tiles_x, tiles_y);
+	while ((size_t(tiles_x) * tiles_y * n > lists_cap_ || size_t(tiles_x) * tiles_y > counts_cap_) && tile_log2 < 7)
+	{
+		tile_log2++;
+		tiles_for(tile_log2, tiles_x, tiles_y);
+	}
+
+	const size_t bytes_pos = size_t(n) * 3u * sizeof(VertexPosition);
+	const size_t bytes_attr = size_t(n) * 3u * sizeof(VertexAttribute);
+	const size_t bytes_prim = size_t(n) * sizeof(PrimitiveAttribute);
+	const size_t bytes_state = size_t(d.num_states) * sizeof(StateVector);
+	const size_t need = bytes_pos + bytes_attr + bytes_prim + bytes_state + 4 * kAlign;
+	if (arena_.used + need > arena_.cap)
+		arena_make_room(need);
+	size_t off_pos, off_attr, off_prim, off_state;
+	memcpy(arena_alloc(bytes_pos, off_pos), d.positions, bytes_pos);
+	memcpy(arena_alloc(bytes_attr, off_attr), d.attributes, bytes_attr);
+	memcpy(arena_alloc(bytes_prim, off_prim), d.prims, bytes_prim);
+	memcpy(arena_alloc(bytes_state, off_state), d.states, bytes_state);
+	arena_upload();
+
+	Launch ls{};
+	ls.kind = LaunchKind::Setup;
+	ls.setup.positions = arena_.dev + off_pos;
+	ls.setup.attributes = arena_.dev + off_attr;
+	ls.setup.prims = arena_.dev + off_prim;
+	ls.setup.setup = setup_buf_;
+	ls.setup.tattr = tattr_buf_;
+	ls.setup.num_primitives = n;
+	ls.setup.rate_log2 = d.rate_log2;
+	ls.setup.opaque_fbmask = d.opaque_fbmask;
+	launch(ls);
+
+	Launch lb{};
+	lb.kind = LaunchKind::Binning;
+	lb.bin.setup = setup_buf_;
+	lb.bin.prims = arena_.dev + off_prim;
+	lb.bin.lists = lists_buf_;
+	lb.bin.counts = counts_buf_;
+	lb.bin.num_primitives = n;
+	lb.bin.list_stride = n;
+	lb.bin.origin_x = ox;
+	lb.bin.origin_y = oy;
+	lb.bin.tile_log2 = tile_log2;
+	lb.bin.tiles_x = tiles_x;
+	lb.bin.tiles_y = tiles_y;
+	launch(lb);
+
+	Launch lsh{};
+	lsh.kind = LaunchKind::Shade;
+	pgs_shade_args &a = lsh.shade;
+	a.vram = vram_;
+	a.setup = setup_buf_;
+	a.tattr = tattr_buf_;
+	a.prims = arena_.dev + off_prim;
+	a.states = arena_.dev + off_state;
+	a.lists = lists_buf_;
+	a.counts = counts_buf_;
+	a.textures = dummy_buf_;
+	a.tex_images = dummy_buf_;
+	a.list_stride = n;
+	a.origin_x = ox;
+	a.origin_y = oy;
+	a.tile_log2 = tile_log2;
+	a.tiles_x = tiles_x;
+	a.tiles_y = tiles_y;
+	a.blocks_x = uint32_t((d.bb[2] - ox) >> 3) + 1u;
+	a.blocks_y = uint32_t((d.bb[3] - oy) >> 3) + 1u;
+	a.fb_psm = d.fb_psm;
+	a.fbp = d.fbp;
+	a.fbw = d.fbw;
+	a.z_psm = d.z_psm;
+	a.zbp = d.zbp;
+	a.rate_log2 = d.rate_log2;
+	launch(lsh);
+
+	stats_.render_passes++;
+	stats_.primitives += n;
 }
 
 // ---------------------------------------------------------------------------------------

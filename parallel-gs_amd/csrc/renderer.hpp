@@ -5,6 +5,7 @@
 // upload arena mirrored in device memory, and the kernel-level C ABI (pgs_kernels.h).
 #pragma once
 #include "gs_defs.hpp"
+#include "gs_raster.hpp"
 #include "pgs_kernels.h"
 #include <vector>
 #include <cstring>
@@ -12,6 +13,8 @@
 
 namespace pgs
 {
+struct RenderPassDesc;
+
 struct FlushStats
 {
 	uint64_t transfer_batches = 0;
@@ -44,6 +47,13 @@ public:
 	// Close the current transfer batch: build the per-block work lists and launch.
 	void flush_transfers();
 	bool has_pending_transfers() const { return !batch_.empty(); }
+
+	// ---- render passes ----------------------------------------------------------------
+	// Uploads the pass's primitive arrays and runs triangle setup, binning and shading.
+	void submit_render_pass(const RenderPassDesc &desc);
+	// Coarse tile size (log2 pixels) used at 1x; super-sampled passes use the reference's
+	// binning-cost heuristic because the tile grid decides super- vs single-sampled shading.
+	void set_tile_log2(uint32_t v) { tile_log2_1x_ = v; }
 
 	// ---- host access ------------------------------------------------------------------
 	void sync();
@@ -84,7 +94,7 @@ private:
 		size_t uploaded = 0;  // bytes already copied to the device
 	};
 
-	enum class LaunchKind : uint32_t { VramCopy, VramSnapshot };
+	enum class LaunchKind : uint32_t { VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4 };
 	struct Launch
 	{
 		LaunchKind kind;
@@ -92,8 +102,21 @@ private:
 		{
 			pgs_vram_copy_args copy;
 			pgs_vram_snapshot_args snapshot;
+			pgs_setup_args setup;
+			pgs_bin_args bin;
+			pgs_shade_args shade;
 		};
 	};
+
+	// Device scratch of a render pass, sized for the largest pass (65536 primitives).
+	uint8_t *setup_buf_ = nullptr;    // PrimitiveSetup[65536]
+	uint8_t *tattr_buf_ = nullptr;    // TransformedAttributes[65536]
+	uint16_t *lists_buf_ = nullptr;   // coarse tile lists
+	uint32_t *counts_buf_ = nullptr;  // per coarse tile
+	uint32_t *dummy_buf_ = nullptr;   // bound where a pass has no textures
+	size_t lists_cap_ = 0;            // entries
+	size_t counts_cap_ = 0;
+	uint32_t tile_log2_1x_ = 5;
 
 	struct PendingCopy
 	{

@@ -120,6 +120,56 @@ uint64_t trxpos(uint32_t ssax, uint32_t ssay, uint32_t dsax, uint32_t dsay)
 
 uint64_t trxreg(uint32_t w, uint32_t h) { return uint64_t(w) | uint64_t(h) << 32; }
 
+// PACKED vertex qwords (gs/gs_registers.hpp:443-517 layouts).
+void packed_rgbaq(Stream &s, uint32_t rgba)
+{
+	s.q.push_back(uint64_t(rgba & 0xff) | (uint64_t((rgba >> 8) & 0xff) << 32));
+	s.q.push_back(uint64_t((rgba >> 16) & 0xff) | (uint64_t(rgba >> 24) << 32));
+}
+
+void packed_xyz2(Stream &s, uint32_t x, uint32_t y, uint32_t z)
+{
+	s.q.push_back(uint64_t(x & 0xffff) | (uint64_t(y & 0xffff) << 32));
+	s.q.push_back(uint64_t(z));
+}
+
+void packed_uv(Stream &s, uint32_t u, uint32_t v)
+{
+	s.q.push_back(uint64_t(u & 0x3fff) | (uint64_t(v & 0x3fff) << 32));
+	s.q.push_back(0);
+}
+
+void packed_st(Stream &s, float S, float T, float Q)
+{
+	uint32_t a, b, c;
+	memcpy(&a, &S, 4);
+	memcpy(&b, &T, 4);
+	memcpy(&c, &Q, 4);
+	s.q.push_back(uint64_t(a) | (uint64_t(b) << 32));
+	s.q.push_back(uint64_t(c));
+}
+
+uint64_t frame_reg(uint32_t fbp, uint32_t fbw, uint32_t psm, uint32_t fbmsk)
+{
+	return uint64_t(fbp) | uint64_t(fbw) << 16 | uint64_t(psm) << 24 | uint64_t(fbmsk) << 32;
+}
+
+uint64_t zbuf_reg(uint32_t zbp, uint32_t psm, uint32_t zmsk)
+{
+	return uint64_t(zbp) | uint64_t(psm & 0xf) << 24 | uint64_t(zmsk) << 32;
+}
+
+uint64_t test_reg(uint32_t ate, uint32_t atst, uint32_t aref, uint32_t afail, uint32_t zte, uint32_t ztst)
+{
+	return uint64_t(ate) | uint64_t(atst) << 1 | uint64_t(aref) << 4 | uint64_t(afail) << 12 | uint64_t(zte) << 16 |
+	       uint64_t(ztst) << 17;
+}
+
+uint64_t scissor_reg(uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1)
+{
+	return uint64_t(x0) | uint64_t(x1) << 16 | uint64_t(y0) << 32 | uint64_t(y1) << 48;
+}
+
 std::vector<uint8_t> *g_last = nullptr;
 
 const uint8_t *publish(Stream &s, size_t *size)
@@ -195,6 +245,39 @@ const uint8_t *pgs_gen_transfers(uint64_t seed, uint32_t n_h2l, uint32_t n_l2l, 
 			       { 0x52, trxreg(lw, lh) },
 			       { 0x53, 2 } });
 		}
+	}
+	return publish(s, size);
+}
+
+// Plain rendering workload: one frame of n untextured Gouraud triangles (PRIM=TRIANGLE, IIP=1)
+// with vertices uniform over a width x height window at XY offset 2048.0, Z uniform 24-bit,
+// RGBA uniform, ZTST=GEQUAL into PSMCT32 at FBP 0 with a PSMZ24 depth buffer.
+const uint8_t *pgs_gen_plain(uint64_t seed, uint32_t n_tris, uint32_t width, uint32_t height, size_t *size)
+{
+	Rng rng(seed);
+	Stream s;
+	const uint32_t ofs = 2048u * 16u;
+	const uint32_t fbw = (width + 63) / 64;
+	const uint32_t zbp = (fbw * 64u * height * 4u + 8191u) / 8192u;
+	s.ad({ { 0x1a, 1 },                                   // PRMODECONT: AC=1
+	       { 0x4c, frame_reg(0, fbw, CT32, 0) },          // FRAME_1
+	       { 0x4e, zbuf_reg(zbp, Z24, 0) },               // ZBUF_1
+	       { 0x18, uint64_t(ofs) | uint64_t(ofs) << 32 }, // XYOFFSET_1
+	       { 0x40, scissor_reg(0, width - 1, 0, height - 1) },
+	       { 0x47, test_reg(0, 0, 0, 0, 1, 2) },          // TEST_1: ZTE, GEQUAL
+	       { 0x42, 0 },                                   // ALPHA_1
+	       { 0x46, 0 },                                   // COLCLAMP
+	       { 0x45, 0 } });                                // DTHE
+	// PRIM = TRIANGLE | IIP, set through the GIFtag PRE bit; 2 registers per vertex.
+	s.tag(n_tris * 3, true, true, 0x3 | 0x8, 0, 2, 0x51);
+	for (uint32_t t = 0; t < n_tris * 3; t++)
+	{
+		uint32_t rgba = uint32_t(rng.next());
+		uint32_t x = ofs + rng.range(0, width * 16 - 1);
+		uint32_t y = ofs + rng.range(0, height * 16 - 1);
+		uint32_t z = uint32_t(rng.next() & 0xffffff);
+		packed_rgbaq(s, rgba);
+		packed_xyz2(s, x, y, z);
 	}
 	return publish(s, size);
 }

@@ -69,7 +69,8 @@ EXPORTED_SYMBOLS = [
     "pgs_begin_recording", "pgs_end_recording", "pgs_replay", "pgs_sync", "pgs_stream",
     "pgs_profile_enable", "pgs_profile_collect",
     "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
-    "pgs_k_vram_copy", "pgs_k_vram_snapshot",
+    "pgs_k_vram_copy", "pgs_k_vram_snapshot", "pgs_k_init_tables", "pgs_k_triangle_setup",
+    "pgs_k_binning", "pgs_k_shade",
 ]
 
 
@@ -126,6 +127,9 @@ def streams() -> ctypes.CDLL:
         S.pgs_gen_transfers.restype = ctypes.c_void_p
         S.pgs_gen_transfers.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.POINTER(ctypes.c_size_t)]
+        S.pgs_gen_plain.restype = ctypes.c_void_p
+        S.pgs_gen_plain.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.POINTER(ctypes.c_size_t)]
         S.pgs_gen_transfers_wild.restype = ctypes.c_void_p
         S.pgs_gen_transfers_wild.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)]
         _streams = S
@@ -141,6 +145,13 @@ def gen_transfers(seed: int, n_h2l: int, n_l2l: int, max_dim: int = 512) -> np.n
     """GIF stream of the VRAM transfer microbenchmark (host->local + local->local)."""
     size = ctypes.c_size_t()
     ptr = streams().pgs_gen_transfers(seed, n_h2l, n_l2l, max_dim, ctypes.byref(size))
+    return _copy_stream(ptr, size)
+
+
+def gen_plain(seed: int, n_tris: int = 2000, width: int = 640, height: int = 448) -> np.ndarray:
+    """GIF stream of the plain 1x rendering workload (untextured Gouraud triangles)."""
+    size = ctypes.c_size_t()
+    ptr = streams().pgs_gen_plain(seed, n_tris, width, height, ctypes.byref(size))
     return _copy_stream(ptr, size)
 
 
