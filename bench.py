@@ -131,7 +131,7 @@ def kernel_profile(ctx):
     ctx.replay(False)
     kms, kn = ctx.profile_collect()
     ctx.profile_enable(False)
-    names = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade"]
+    names = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode"]
     return {names[i]: (kms[i], kn[i]) for i in range(len(names)) if kn[i]}
 
 

@@ -123,6 +123,41 @@ typedef struct pgs_shade_args
 
 int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);
 
+/* CLUT load of one TEX0/TEX2 write with CLD that loads.
+ * Replaces the clut_write.comp dispatch of GSRenderer::flush_palette_upload
+ * (gs/gs_renderer.cpp:3662-3819) for a load queued by GSInterface::handle_clut_upload
+ * (gs/gs_interface.cpp:550-797). `clut` is the 512-entry 16-bit CLUT. */
+typedef struct pgs_clut_args
+{
+	const uint8_t *vram;
+	uint16_t *clut;
+	uint32_t cbp;      /* CLUT base, 256-byte blocks */
+	uint32_t cbw;      /* TEXCLUT.CBW (CSM2 only) */
+	uint32_t cou, cov; /* TEXCLUT.COU / COV (CSM2 only) */
+	uint32_t cpsm;     /* PSMCT32 (PSMCT24 passes as PSMCT32), PSMCT16 or PSMCT16S */
+	uint32_t csm;      /* 0 = CSM1, 1 = CSM2 */
+	uint32_t csa;      /* bank offset, 0..31 (0..15 for PSMCT32) */
+	uint32_t index4;   /* 1 for 4-bit index formats (16 entries) */
+} pgs_clut_args;
+
+int pgs_k_clut_write(const pgs_clut_args *args, hipStream_t stream);
+
+/* Decodes the textures of one render pass into RGBA8 images.
+ * Replaces the upload.comp dispatches of GSRenderer::upload_texture
+ * (gs/gs_renderer.cpp:3458-3598). `decodes` is a device array of TexDecode records
+ * (parallel-gs_amd/csrc/texture_layout.hpp), ordered by first workgroup. */
+typedef struct pgs_tex_decode_args
+{
+	const uint8_t *vram;
+	const uint16_t *clut;
+	uint32_t *images;
+	const void *decodes;
+	uint32_t num_textures;
+	uint32_t num_groups;
+} pgs_tex_decode_args;
+
+int pgs_k_texture_decode(const pgs_tex_decode_args *args, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -227,7 +227,7 @@ GS::GS(uint32_t ssaa_log2)
 	sample_log2 = ssaa_log2;
 	uint32_t planes = ssaa_log2 ? (2u + (1u << ssaa_log2)) : 1u;
 	vram.assign(size_t(planes) * VRAM_SIZE, 0);
-	clut.assign(1024u, 0);
+	clut.assign(512u, 0);
 	reset_registers();
 }
 
@@ -571,4 +571,11 @@ uint32_t orc_swizzle(uint32_t x, uint32_t y, uint32_t base, uint32_t stride, uin
 	return orc::swizzle(x, y, base, stride, psm);
 }
 uint64_t orc_stat(void *g, uint32_t which) { return static_cast<orc::GS *>(g)->stat(which); }
+}
+
+extern "C"
+{
+const uint8_t *orc_rcp_table() { return orc::rcp_table(); }
+float orc_rcp_fixed(uint32_t v) { return orc::rcp_fixed(v); }
+float orc_rcp_float(float v) { return orc::rcp_float(v); }
 }

@@ -457,6 +457,13 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 		memcpy(last_order, order, sizeof(order));
 	}
 
+	// A texture read of pages the pending pass writes renders the pass first, so the texture
+	// sees every earlier primitive (the reference's page tracker, gs/page_tracker.cpp, orders
+	// texture reads after pending writes the same way). Fused parallelograms skip the check,
+	// as in the reference (gs/gs_interface.cpp:2570-2599).
+	if (field(prim, 4, 1) && !rp_prims.empty() && texture_hazard(ctx))
+		flush_render_pass();
+
 	if (rp_prims.empty())
 	{
 		rp_frame = frame;
@@ -573,6 +580,16 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 		state |= S_FIX | S_SNAP_RASTER | S_SNAP_ATTRIBUTE;
 	p.state = state;
 
+	if (rp_prims.empty())
+	{
+		memcpy(rp_bb, p.bb, sizeof(rp_bb));
+		rp_zwrite = false;
+	}
+	rp_bb[0] = std::min(rp_bb[0], p.bb[0]);
+	rp_bb[1] = std::min(rp_bb[1], p.bb[1]);
+	rp_bb[2] = std::max(rp_bb[2], p.bb[2]);
+	rp_bb[3] = std::max(rp_bb[3], p.bb[3]);
+	rp_zwrite |= (state & S_Z_WRITE) != 0;
 	rp_prims.push_back(p);
 	last_parallelogram_candidate = candidate;
 	state_dirty = false;
@@ -1140,8 +1157,9 @@ void GS::render(const std::vector<Prim> &prims_in)
 						u = u / sq[2];
 						vv = vv / sq[2];
 					}
-					u = std::min(std::max(u, -2047.0f), 2047.0f);
-					vv = std::min(std::max(vv, -2047.0f), 2047.0f);
+					// NClamp: a NaN coordinate clamps to the lower bound.
+					u = std::fmin(std::fmax(u, -2047.0f), 2047.0f);
+					vv = std::fmin(std::fmax(vv, -2047.0f), 2047.0f);
 					float tx[4];
 					sample_texture(rp_textures[p.tex_slot], p.tex, u, vv, sq[2], persp, tx);
 					const bool tcc = p.combiner & C_TCC;

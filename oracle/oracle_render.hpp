@@ -47,7 +47,6 @@ struct Prim
 	int32_t bb[4];       // scissor-clipped pixel bounding box
 	uint32_t state, tex, tex2, alpha, fbmsk, fogcol;
 	uint32_t combiner, blend_mode, dimx[2];
-	uint32_t clut_index;  // CLUT snapshot used by a textured primitive
 	uint32_t tex_slot;    // texture slot in the render pass
 };
 
@@ -62,14 +61,23 @@ struct Setup
 	uint32_t rgba[3];
 };
 
+// A decoded texture of a render pass (gs/gs_interface.cpp:1754-1781 TexInfo content).
 struct TextureSlot
 {
-	uint64_t tex0, clamp, tex1, texa;
-	uint32_t clut_index;
-	uint32_t width, height;
-	std::vector<uint32_t> rgba; // decoded texture cache image (width x height)
-	float region[4];
+	uint64_t tex0, clamp, texa;
+	uint32_t width, height;     // effective rectangle size
+	std::vector<uint32_t> rgba; // decoded RGBA8 image (width x height)
+	float size[4];              // 2^TW, 2^TH, 1 / width, 1 / height
+	float bias[2];              // -rect origin / rectangle size
+	float region[4];            // clamp region: min u, min v, max u, max v
 };
+
+// Inclusive interval of 8 KiB pages touched by a rectangle of a buffer; `last` may run past
+// the end of VRAM (it wraps).
+struct PageSpan { uint32_t first, last; };
+PageSpan page_span(uint32_t base_block, uint32_t stride, uint32_t psm, uint32_t x0, uint32_t y0, uint32_t x1,
+                   uint32_t y1);
+bool spans_overlap(PageSpan a, PageSpan b);
 
 class GS
 {
@@ -108,7 +116,9 @@ private:
 	// Render pass state.
 	std::vector<Prim> rp_prims;
 	std::vector<TextureSlot> rp_textures;
-	std::vector<std::vector<uint16_t>> clut_snapshots;
+	uint32_t cached_cbp[2] = { 0, 0 }; // CBP0 / CBP1 of CLD 2..5
+	int32_t rp_bb[4] = { 0, 0, 0, 0 };  // union of the pass's primitive boxes
+	bool rp_zwrite = false;            // some primitive of the pass writes depth
 	uint64_t rp_frame = 0, rp_zbuf = 0;
 	bool state_dirty = true;          // a draw-state register changed since the last append
 	bool last_parallelogram_candidate = false;
@@ -143,6 +153,8 @@ private:
 	void render(const std::vector<Prim> &prims);
 	void load_clut(uint32_t ctx);
 	uint32_t texture_slot(uint32_t ctx);
+	PageSpan texture_span(uint32_t ctx) const;
+	bool texture_hazard(uint32_t ctx) const;
 };
 
 // Reciprocal helpers (gs/shaders/math_utils.h:27-101) and their tables

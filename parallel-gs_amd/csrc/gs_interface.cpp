@@ -265,10 +265,58 @@ void GSInterface::write_register(uint32_t addr, uint64_t v)
 	}
 }
 
-// TEX0 write: texture state change; palette loads are handled by the texture cache.
+// TEX0 write: texture state change plus the CLUT load its CLD field asks for.
 void GSInterface::write_tex0(uint32_t ctx, uint64_t v)
 {
 	set_state_reg(regs_.tex0[ctx], v);
+	handle_clut_upload(ctx);
+}
+
+// CLUT load decision and parameters (gs/gs_interface.cpp:550-797): CLD 1 always loads, 2/3
+// load and record CBP0/CBP1, 4/5 load only when CBP differs from CBP0/CBP1. The load reads
+// VRAM as of this register write, so a partially received upload, pending transfers and
+// pending primitives are all executed first.
+void GSInterface::handle_clut_upload(uint32_t ctx)
+{
+	const uint64_t t0 = regs_.tex0[ctx];
+	const uint32_t psm = uint32_t((t0 >> 20) & 0x3f);
+	if (psm != PSMT8 && psm != PSMT4 && psm != PSMT8H && psm != PSMT4HL && psm != PSMT4HH)
+		return;
+	const uint32_t cld = uint32_t((t0 >> 61) & 7);
+	const uint32_t cbp = uint32_t((t0 >> 37) & 0x3fff);
+	bool load = false;
+	switch (cld)
+	{
+	case 1: load = true; break;
+	case 2: case 3: load = true; cached_cbp_[cld & 1] = cbp; break;
+	case 4: case 5: load = cached_cbp_[cld & 1] != cbp; cached_cbp_[cld & 1] = cbp; break;
+	default: break;
+	}
+	if (!load)
+		return;
+	if (xfer_.host_to_local_active)
+		flush_pending_transfer(true);
+	flush_render_pass();
+	renderer_.flush_transfers();
+
+	pgs_clut_args a{};
+	uint32_t cpsm = uint32_t((t0 >> 51) & 0xf);
+	if (cpsm == PSMCT24)
+		cpsm = PSMCT32;
+	a.cpsm = cpsm;
+	a.cbp = cbp;
+	a.csm = uint32_t((t0 >> 55) & 1);
+	a.csa = uint32_t((t0 >> 56) & 0x1f);
+	if (cpsm == PSMCT32)
+		a.csa &= 15u; // a 32-bit CLUT only targets the lower banks (gs/gs_interface.cpp:608-610)
+	a.index4 = (psm == PSMT4 || psm == PSMT4HL || psm == PSMT4HH) ? 1u : 0u;
+	if (a.csm)
+	{
+		a.cbw = uint32_t(regs_.texclut & 0x3f);
+		a.cou = uint32_t((regs_.texclut >> 6) & 0x3f);
+		a.cov = uint32_t((regs_.texclut >> 12) & 0x3ff);
+	}
+	renderer_.clut_load(a);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -91,9 +91,11 @@ private:
 	GIFPath paths_[4];
 	TransferState xfer_;
 	std::vector<uint8_t> host_vram_;
+	uint32_t cached_cbp_[2] = { 0, 0 }; // CBP0 / CBP1 of CLD 2..5
 
 	void packed_write(uint32_t addr, const uint64_t *qword);
 	void write_tex0(uint32_t ctx, uint64_t v);
+	void handle_clut_upload(uint32_t ctx);
 	void init_transfer();
 	void hwreg(const uint64_t *data, size_t count);
 	void flush_pending_transfer(bool keep_alive);

@@ -82,12 +82,16 @@ struct TransformedAttributes
 static_assert(sizeof(TransformedAttributes) == 64, "TransformedAttributes layout");
 
 // One cached texture of a render pass: a decoded RGBA8 image plus sampling parameters
-// (gs/shaders/data_structures.h:98-105 TexInfo content).
+// (gs/shaders/data_structures.h:98-105 TexInfo content, gs/gs_interface.cpp:1754-1781).
 struct TextureDesc
 {
 	uint32_t image_offset; // offset in dwords into the texture image arena
-	uint32_t width, height;
-	uint32_t pad;
+	uint32_t width, height; // effective rectangle size (the decoded image)
+	uint32_t pad0;
 	float region[4];       // clamp region (min u, min v, max u, max v) in texels
+	float size[4];         // 2^TW, 2^TH, 1 / width, 1 / height
+	float bias[2];         // -rectangle origin / rectangle size
+	uint32_t pad1[2];
 };
+static_assert(sizeof(TextureDesc) == 64, "TextureDesc layout");
 }

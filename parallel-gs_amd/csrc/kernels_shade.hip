@@ -97,6 +97,69 @@ __device__ __forceinline__ bool alpha_test(uint32_t atst, uint32_t aref, float a
 	}
 }
 
+// Texel fetch with REPEAT addressing on the decoded image.
+__device__ __forceinline__ uint32_t texel(const uint32_t *img, const TextureDesc &t, int32_t x, int32_t y)
+{
+	const int32_t w = int32_t(t.width), h = int32_t(t.height);
+	x %= w;
+	y %= h;
+	x += x < 0 ? w : 0;
+	y += y < 0 ? h : 0;
+	return img[uint32_t(y) * t.width + uint32_t(x)];
+}
+
+// a + round((b - a) * w / 256) with round-half-up, w an 8-bit fraction.
+__device__ __forceinline__ uint32_t lerp8(uint32_t a, uint32_t b, uint32_t w)
+{
+	const int32_t d = (int32_t(b) - int32_t(a)) * int32_t(w) + 128;
+	return uint32_t(int32_t(a) + (d >> 8));
+}
+
+// Non-array sampling path of sample_texture (ubershader.comp:1042-1258) at LOD 0: UV snap to
+// 1/16 texel with the reference's epsilons, then a NEAREST or LINEAR fetch with REPEAT
+// addressing. LINEAR uses 8-bit fractional weights and an integer lerp (U, then V).
+__device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint32_t *images, uint32_t tex, float u,
+                                               float v, bool perspective, float out[4])
+{
+	const uint32_t *img = images + t.image_offset;
+	float u1 = perspective ? __fmul_rn(u, t.size[0]) : u;
+	float v1 = perspective ? __fmul_rn(v, t.size[1]) : v;
+	if (tex & TEX_CLAMP_S)
+		u1 = fminf(fmaxf(u1, floorf(t.region[0]) + 0.5f), floorf(t.region[2]) + 0.5f);
+	if (tex & TEX_CLAMP_T)
+		v1 = fminf(fmaxf(v1, floorf(t.region[1]) + 0.5f), floorf(t.region[3]) + 0.5f);
+	float un = floorf(__fadd_rn(__fmul_rn(u1, 16.0f), 1.0f / 64.0f));
+	float vn = floorf(__fadd_rn(__fmul_rn(v1, 16.0f), 1.0f / 64.0f));
+	un = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(un, 16.0f / 1024.0f), t.size[2]), 0.0625f), t.bias[0]);
+	vn = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(vn, 16.0f / 1024.0f), t.size[3]), 0.0625f), t.bias[1]);
+	const float fw = float(t.width), fh = float(t.height);
+	uint32_t c;
+	if (!(tex & TEX_MAG_LINEAR))
+	{
+		c = texel(img, t, int32_t(floorf(__fmul_rn(un, fw))), int32_t(floorf(__fmul_rn(vn, fh))));
+	}
+	else
+	{
+		const float xs = __fsub_rn(__fmul_rn(un, fw), 0.5f), ys = __fsub_rn(__fmul_rn(vn, fh), 0.5f);
+		const float x0 = floorf(xs), y0 = floorf(ys);
+		const uint32_t wx = uint32_t(__fmul_rn(__fsub_rn(xs, x0), 256.0f));
+		const uint32_t wy = uint32_t(__fmul_rn(__fsub_rn(ys, y0), 256.0f));
+		const int32_t ix = int32_t(x0), iy = int32_t(y0);
+		const uint32_t c00 = texel(img, t, ix, iy), c10 = texel(img, t, ix + 1, iy);
+		const uint32_t c01 = texel(img, t, ix, iy + 1), c11 = texel(img, t, ix + 1, iy + 1);
+		c = 0;
+		for (uint32_t ch = 0; ch < 4; ch++)
+		{
+			const uint32_t sh = ch * 8u;
+			const uint32_t top = lerp8((c00 >> sh) & 0xffu, (c10 >> sh) & 0xffu, wx);
+			const uint32_t bot = lerp8((c01 >> sh) & 0xffu, (c11 >> sh) & 0xffu, wx);
+			c |= lerp8(top, bot, wy) << sh;
+		}
+	}
+	for (uint32_t ch = 0; ch < 4; ch++)
+		out[ch] = float((c >> (8 * ch)) & 0xffu);
+}
+
 struct PassInfo
 {
 	uint32_t fb_psm, z_psm;
@@ -171,6 +234,47 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		const float sum = __fadd_rn(__fadd_rn(c0, e1), e2);
 		col[ch] = floorf(__fadd_rn(__fmul_rn(255.0f, sum), 1.0f / 256.0f));
 		comb[ch] = col[ch];
+	}
+	if (sv.combiner & COMB_TME)
+	{
+		float sq[3];
+		for (int k = 0; k < 3; k++)
+			sq[k] = __fadd_rn(__fadd_rn(ta.stqf0[k], __fmul_rn(ta.stqf1[k], fi)), __fmul_rn(ta.stqf2[k], fj));
+		const bool persp = (st & ST_PERSPECTIVE) != 0;
+		float u = sq[0], v = sq[1];
+		if (persp && sq[2] != 1.0f)
+		{
+			u = __fdiv_rn(u, sq[2]);
+			v = __fdiv_rn(v, sq[2]);
+		}
+		u = fminf(fmaxf(u, -2047.0f), 2047.0f);
+		v = fminf(fmaxf(v, -2047.0f), 2047.0f);
+		const TextureDesc &td = static_cast<const TextureDesc *>(a.textures)[pa.tex & TEX_INDEX_MASK];
+		float tx[4];
+		sample_texture(td, a.tex_images, pa.tex, u, v, persp, tx);
+		const bool tcc = (sv.combiner & COMB_TCC) != 0;
+		const uint32_t mode = sv.combiner & COMB_MODE_MASK;
+		if (mode == 1) // DECAL
+		{
+			for (int k = 0; k < 4; k++) comb[k] = tx[k];
+			if (!tcc) comb[3] = col[3];
+		}
+		else
+		{
+			// MODULATE, HIGHLIGHT, HIGHLIGHT2 (ubershader.comp:918-945)
+			for (int k = 0; k < 4; k++)
+				comb[k] = fminf(fmaxf(floorf(__fmul_rn(__fmul_rn(col[k], tx[k]), 1.0f / 128.0f)), 0.0f), 255.0f);
+			if (mode == 0)
+			{
+				if (!tcc) comb[3] = col[3];
+			}
+			else
+			{
+				for (int k = 0; k < 3; k++) comb[k] = __fadd_rn(comb[k], col[3]);
+				comb[3] = tcc ? (mode == 2 ? __fadd_rn(tx[3], col[3]) : tx[3]) : col[3];
+				for (int k = 0; k < 4; k++) comb[k] = fminf(fmaxf(comb[k], 0.0f), 255.0f);
+			}
+		}
 	}
 	if (sv.combiner & COMB_FOG)
 	{

@@ -273,11 +273,8 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 	const uint64_t frame = r.frame[ctx], zbuf = r.zbuf[ctx];
 	if (!prims_.empty())
 	{
-		bool pass_z = false;
-		for (auto &pp : prims_)
-			pass_z |= (pp.state & (ST_Z_TEST | ST_Z_WRITE)) != 0;
 		if ((frame_ & 0x3f3f01ffull) != (frame & 0x3f3f01ffull) ||
-		    ((zbuf_ & 0x0f0001ffull) != (zbuf & 0x0f0001ffull) && pass_z))
+		    ((zbuf_ & 0x0f0001ffull) != (zbuf & 0x0f0001ffull) && pass_z_))
 			flush();
 	}
 
@@ -297,6 +294,23 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 		memcpy(last_order_, order, sizeof(order));
 	}
 
+	// A texture read of pages the pending pass writes renders the pass first, so the texture
+	// sees every earlier primitive (the ordering gs/page_tracker.cpp enforces). Fused
+	// parallelograms above skip the check, as in gs/gs_interface.cpp:2570-2599.
+	const bool tme = field(prim, 4, 1);
+	int32_t slot = -1;
+	if (tme)
+	{
+		if (!prims_.empty() && texture_hazard(r, ctx))
+			flush();
+		slot = texture_slot(r, ctx);
+		if (slot < 0)
+		{
+			flush();
+			slot = texture_slot(r, ctx);
+		}
+	}
+
 	if (prims_.empty())
 	{
 		if (host)
@@ -314,7 +328,7 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 	const uint32_t ate = field(test, 0, 1), atst = field(test, 1, 3), aref = field(test, 4, 8);
 	const uint32_t afail = field(test, 12, 2), date = field(test, 14, 1), datm = field(test, 15, 1);
 	const uint32_t zte = field(test, 16, 1), ztst = field(test, 17, 2);
-	const bool abe = field(prim, 6, 1), tme = field(prim, 4, 1);
+	const bool abe = field(prim, 6, 1);
 
 	// State vector (gs/gs_interface.cpp:1065-1128).
 	StateVector sv{};
@@ -347,7 +361,7 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 	if (tme)
 	{
 		const uint64_t t1 = r.tex1[ctx], cl = r.clamp[ctx];
-		pa.tex = 0; // texture slot (assigned with the texture cache)
+		pa.tex = uint32_t(slot);
 		pa.tex |= field(t1, 5, 1) ? TEX_MAG_LINEAR : 0;
 		const uint32_t wms = field(cl, 0, 2), wmt = field(cl, 2, 2);
 		pa.tex |= (wms == 1 || wms == 2) ? TEX_CLAMP_S : 0;
@@ -408,6 +422,8 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 	pa.bb[0] = int16_t(lox); pa.bb[1] = int16_t(loy); pa.bb[2] = int16_t(hix); pa.bb[3] = int16_t(hiy);
 	color_write_mask_ |= ~pa.fbmsk;
 
+	pass_z_ |= (state & (ST_Z_TEST | ST_Z_WRITE)) != 0;
+	pass_zwrite_ |= (state & ST_Z_WRITE) != 0;
 	prims_.push_back(pa);
 	pos_.insert(pos_.end(), p, p + 3);
 	attr_.insert(attr_.end(), a, a + 3);
@@ -440,15 +456,21 @@ void RenderPassBuilder::flush()
 			else if (zpsm == PSMZ24) zpsm = PSMCT24;
 			else if (zpsm == PSMZ16S || zpsm == PSMZ16) zpsm = PSMCT16S;
 		}
-		bool zsens = false;
-		for (auto &pp : prims_)
-			zsens |= (pp.state & (ST_Z_TEST | ST_Z_WRITE)) != 0;
-		d.z_psm = zsens ? zpsm : 0xffffffffu;
+		d.z_psm = pass_z_ ? zpsm : 0xffffffffu;
 		memcpy(d.bb, bb_, sizeof(bb_));
 		d.opaque_fbmask = ~color_write_mask_;
 		d.rate_log2 = rate_log2_;
+		d.decodes = decodes_.data();
+		d.tex_descs = tex_descs_.data();
+		d.num_textures = uint32_t(decodes_.size());
+		d.tex_groups = group_total_;
 		renderer_->submit_render_pass(d);
 		prims_.clear();
+		slot_keys_.clear();
+		decodes_.clear();
+		tex_descs_.clear();
+		texel_total_ = group_total_ = 0;
+		pass_z_ = pass_zwrite_ = false;
 		pos_.clear();
 		attr_.clear();
 		states_.clear();
@@ -456,5 +478,86 @@ void RenderPassBuilder::flush()
 	}
 	state_dirty_ = true;
 	last_candidate_ = false;
+}
+}
+
+namespace pgs
+{
+bool RenderPassBuilder::texture_hazard(const RegisterState &r, uint32_t ctx)
+{
+	if (r.tex0[ctx] != span_tex0_ || r.clamp[ctx] != span_clamp_)
+	{
+		span_tex0_ = r.tex0[ctx];
+		span_clamp_ = r.clamp[ctx];
+		span_ = texture_span(tex_layout(span_tex0_, span_clamp_));
+	}
+	const uint32_t x0 = uint32_t(bb_[0]), y0 = uint32_t(bb_[1]), x1 = uint32_t(bb_[2]), y1 = uint32_t(bb_[3]);
+	const PageSpan fb = page_span(field(frame_, 0, 9) * 32u, field(frame_, 16, 6), field(frame_, 24, 6), x0, y0, x1, y1);
+	if (spans_overlap(span_, fb))
+		return true;
+	if (pass_zwrite_)
+	{
+		const PageSpan z = page_span(field(zbuf_, 0, 9) * 32u, field(frame_, 16, 6), 0x30u | field(zbuf_, 24, 4), x0,
+		                             y0, x1, y1);
+		if (spans_overlap(span_, z))
+			return true;
+	}
+	return false;
+}
+
+int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
+{
+	const uint64_t key0 = r.tex0[ctx] & ((1ull << 61) - 1ull); // CLD does not change the texture
+	const uint64_t keyc = r.clamp[ctx], keya = r.texa;
+	for (uint32_t i = 0; i < slot_keys_.size(); i++)
+		if (slot_keys_[i].tex0 == key0 && slot_keys_[i].clamp == keyc && slot_keys_[i].texa == keya)
+			return int32_t(i);
+
+	const TexLayout l = tex_layout(key0, keyc);
+	const uint32_t texels = l.width * l.height;
+	if (slot_keys_.size() >= kMaxPassTextures || texel_total_ + texels > kTexImageTexels)
+		return prims_.empty() ? -2 : -1;
+
+	// TexInfo (gs/gs_interface.cpp:1752-1781).
+	const uint32_t wms = field(keyc, 0, 2), wmt = field(keyc, 2, 2);
+	TextureDesc td{};
+	td.image_offset = texel_total_;
+	td.width = l.width;
+	td.height = l.height;
+	if (wms == 1) { td.region[0] = 0.0f; td.region[2] = float(l.width) - 1.0f; }
+	else if (wms == 2) { td.region[0] = float(field(keyc, 4, 10)); td.region[2] = float(field(keyc, 14, 10)); }
+	if (wmt == 1) { td.region[1] = 0.0f; td.region[3] = float(l.height) - 1.0f; }
+	else if (wmt == 2) { td.region[1] = float(field(keyc, 24, 10)); td.region[3] = float(field(keyc, 34, 10)); }
+	td.size[0] = float(1u << l.tw);
+	td.size[1] = float(1u << l.th);
+	td.size[2] = 1.0f / float(l.width);
+	td.size[3] = 1.0f / float(l.height);
+	td.bias[0] = -float(l.bx) * td.size[2];
+	td.bias[1] = -float(l.by) * td.size[3];
+
+	TexDecode dd{};
+	dd.image_offset = texel_total_;
+	dd.width = l.width;
+	dd.height = l.height;
+	dd.tbp = l.tbp;
+	dd.tbw = l.tbw;
+	dd.psm = l.psm;
+	dd.bx = l.bx; dd.by = l.by;
+	dd.umsk = l.umsk; dd.ufix = l.ufix; dd.vmsk = l.vmsk; dd.vfix = l.vfix;
+	const uint32_t cpsm = field(key0, 51, 4);
+	dd.clut16 = (cpsm == PSMCT16 || cpsm == PSMCT16S) ? 1u : 0u;
+	// A 32-bit CLUT addresses the lower 16 banks only (gs/gs_interface.cpp:1306-1311).
+	dd.csa = dd.clut16 ? field(key0, 56, 5) : (field(key0, 56, 5) & 15u);
+	dd.aem = field(keya, 15, 1);
+	dd.ta0 = field(keya, 0, 8);
+	dd.ta1 = field(keya, 32, 8);
+	dd.group_begin = group_total_;
+
+	slot_keys_.push_back({ key0, keyc, keya });
+	decodes_.push_back(dd);
+	tex_descs_.push_back(td);
+	texel_total_ += texels;
+	group_total_ += (texels + 255u) / 256u;
+	return int32_t(slot_keys_.size() - 1);
 }
 }

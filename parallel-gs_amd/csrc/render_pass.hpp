@@ -9,6 +9,7 @@
 #include "gs_defs.hpp"
 #include "gs_raster.hpp"
 #include "renderer.hpp"
+#include "texture_layout.hpp"
 #include <vector>
 #include <unordered_map>
 
@@ -28,7 +29,17 @@ struct RenderPassDesc
 	int32_t bb[4];            // union of the primitives' scissor-clipped boxes (pixels)
 	uint32_t opaque_fbmask;
 	uint32_t rate_log2;
+	// Textures of the pass, decoded from VRAM before shading.
+	const TexDecode *decodes;
+	const TextureDesc *tex_descs;
+	uint32_t num_textures;
+	uint32_t tex_groups;      // 256-texel decode workgroups over all textures
 };
+
+// Per-pass texture limits: slot index width of PrimitiveAttribute.tex and the decoded image
+// arena (dwords).
+constexpr uint32_t kMaxPassTextures = 256;
+constexpr uint32_t kTexImageTexels = 16u << 20;
 
 class RenderPassBuilder
 {
@@ -78,6 +89,16 @@ private:
 	uint64_t frame_ = 0, zbuf_ = 0;
 	int32_t bb_[4] = { 0, 0, -1, -1 };
 	uint32_t color_write_mask_ = 0;
+	bool pass_z_ = false;      // some primitive tests or writes depth
+	bool pass_zwrite_ = false; // some primitive writes depth
+
+	struct SlotKey { uint64_t tex0, clamp, texa; };
+	std::vector<SlotKey> slot_keys_;
+	std::vector<TexDecode> decodes_;
+	std::vector<TextureDesc> tex_descs_;
+	uint32_t texel_total_ = 0, group_total_ = 0;
+	uint64_t span_tex0_ = ~0ull, span_clamp_ = ~0ull; // last texture whose page span is cached
+	PageSpan span_{};
 	bool state_dirty_ = true;
 	bool last_candidate_ = false;
 	int32_t last_order_[3] = { 0, 1, 2 };
@@ -85,5 +106,9 @@ private:
 	bool draw_is_degenerate(const RegisterState &r) const;
 	void append(const RegisterState &r, const QueuedVertex *v, uint32_t n, bool quad);
 	uint32_t state_index(const StateVector &sv);
+	// Texture slot of the current TEX0 / CLAMP / TEXA; -1 when the pass has no room for it.
+	int32_t texture_slot(const RegisterState &r, uint32_t ctx);
+	// Does the texture of `ctx` read pages the pending pass writes?
+	bool texture_hazard(const RegisterState &r, uint32_t ctx);
 };
 }

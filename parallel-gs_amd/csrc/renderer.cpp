@@ -51,7 +51,8 @@ Renderer::~Renderer()
 	}
 	for (auto e : event_pool_)
 		(void)hipEventDestroy(e);
-	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_ })
+	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_,
+	                 (void *)clut_buf_, (void *)tex_images_ })
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
@@ -90,6 +91,9 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes)
 		check(hipMalloc(&counts_buf_, counts_cap_ * sizeof(uint32_t)));
 		check(hipMalloc(&dummy_buf_, 256));
 		check(hipMemsetAsync(dummy_buf_, 0, 256, stream_));
+		check(hipMalloc(&clut_buf_, 512u * sizeof(uint16_t)));
+		check(hipMemsetAsync(clut_buf_, 0, 512u * sizeof(uint16_t), stream_));
+		check(hipMalloc(&tex_images_, size_t(kTexImageTexels) * sizeof(uint32_t)));
 		{
 			float lut[256];
 			make_rcp_float_lut(lut);
@@ -240,6 +244,8 @@ void Renderer::launch(const Launch &l)
 	case LaunchKind::Setup: rc = pgs_k_triangle_setup(&l.setup, stream_); break;
 	case LaunchKind::Binning: rc = pgs_k_binning(&l.bin, stream_); break;
 	case LaunchKind::Shade: rc = pgs_k_shade(&l.shade, stream_); break;
+	case LaunchKind::ClutWrite: rc = pgs_k_clut_write(&l.clut, stream_); break;
+	case LaunchKind::TexDecode: rc = pgs_k_texture_decode(&l.decode, stream_); break;
 	}
 	if (rc != 0)
 		check(hipError_t(-rc));
@@ -470,7 +476,9 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	const size_t bytes_attr = size_t(n) * 3u * sizeof(VertexAttribute);
 	const size_t bytes_prim = size_t(n) * sizeof(PrimitiveAttribute);
 	const size_t bytes_state = size_t(d.num_states) * sizeof(StateVector);
-	const size_t need = bytes_pos + bytes_attr + bytes_prim + bytes_state + 4 * kAlign;
+	const size_t bytes_dec = size_t(d.num_textures) * sizeof(TexDecode);
+	const size_t bytes_tex = size_t(d.num_textures) * sizeof(TextureDesc);
+	const size_t need = bytes_pos + bytes_attr + bytes_prim + bytes_state + bytes_dec + bytes_tex + 6 * kAlign;
 	if (arena_.used + need > arena_.cap)
 		arena_make_room(need);
 	size_t off_pos, off_attr, off_prim, off_state;
@@ -478,7 +486,26 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	memcpy(arena_alloc(bytes_attr, off_attr), d.attributes, bytes_attr);
 	memcpy(arena_alloc(bytes_prim, off_prim), d.prims, bytes_prim);
 	memcpy(arena_alloc(bytes_state, off_state), d.states, bytes_state);
+	size_t off_dec = 0, off_tex = 0;
+	if (d.num_textures)
+	{
+		memcpy(arena_alloc(bytes_dec, off_dec), d.decodes, bytes_dec);
+		memcpy(arena_alloc(bytes_tex, off_tex), d.tex_descs, bytes_tex);
+	}
 	arena_upload();
+
+	if (d.num_textures)
+	{
+		Launch ld{};
+		ld.kind = LaunchKind::TexDecode;
+		ld.decode.vram = vram_;
+		ld.decode.clut = clut_buf_;
+		ld.decode.images = tex_images_;
+		ld.decode.decodes = arena_.dev + off_dec;
+		ld.decode.num_textures = d.num_textures;
+		ld.decode.num_groups = d.tex_groups;
+		launch(ld);
+	}
 
 	Launch ls{};
 	ls.kind = LaunchKind::Setup;
@@ -517,8 +544,8 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	a.states = arena_.dev + off_state;
 	a.lists = lists_buf_;
 	a.counts = counts_buf_;
-	a.textures = dummy_buf_;
-	a.tex_images = dummy_buf_;
+	a.textures = d.num_textures ? static_cast<const void *>(arena_.dev + off_tex) : dummy_buf_;
+	a.tex_images = d.num_textures ? tex_images_ : dummy_buf_;
 	a.list_stride = n;
 	a.origin_x = ox;
 	a.origin_y = oy;
@@ -608,5 +635,18 @@ int Renderer::replay(bool use_graph)
 	{
 		return last_error_ ? last_error_ : -1;
 	}
+}
+}
+
+namespace pgs
+{
+void Renderer::clut_load(const pgs_clut_args &args)
+{
+	Launch l{};
+	l.kind = LaunchKind::ClutWrite;
+	l.clut = args;
+	l.clut.vram = vram_;
+	l.clut.clut = clut_buf_;
+	launch(l);
 }
 }

@@ -54,6 +54,8 @@ public:
 	// Coarse tile size (log2 pixels) used at 1x; super-sampled passes use the reference's
 	// binning-cost heuristic because the tile grid decides super- vs single-sampled shading.
 	void set_tile_log2(uint32_t v) { tile_log2_1x_ = v; }
+	// Runs a CLUT load (the caller has ordered it after every VRAM write it must see).
+	void clut_load(const pgs_clut_args &args);
 
 	// ---- host access ------------------------------------------------------------------
 	void sync();
@@ -94,7 +96,10 @@ private:
 		size_t uploaded = 0;  // bytes already copied to the device
 	};
 
-	enum class LaunchKind : uint32_t { VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4 };
+	enum class LaunchKind : uint32_t
+	{
+		VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4, ClutWrite = 5, TexDecode = 6
+	};
 	struct Launch
 	{
 		LaunchKind kind;
@@ -105,6 +110,8 @@ private:
 			pgs_setup_args setup;
 			pgs_bin_args bin;
 			pgs_shade_args shade;
+			pgs_clut_args clut;
+			pgs_tex_decode_args decode;
 		};
 	};
 
@@ -114,6 +121,8 @@ private:
 	uint16_t *lists_buf_ = nullptr;   // coarse tile lists
 	uint32_t *counts_buf_ = nullptr;  // per coarse tile
 	uint32_t *dummy_buf_ = nullptr;   // bound where a pass has no textures
+	uint16_t *clut_buf_ = nullptr;    // the 512-entry CLUT
+	uint32_t *tex_images_ = nullptr;  // decoded RGBA8 textures of the current pass
 	size_t lists_cap_ = 0;            // entries
 	size_t counts_cap_ = 0;
 	uint32_t tile_log2_1x_ = 5;

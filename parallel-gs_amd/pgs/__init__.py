@@ -70,7 +70,7 @@ EXPORTED_SYMBOLS = [
     "pgs_profile_enable", "pgs_profile_collect",
     "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
     "pgs_k_vram_copy", "pgs_k_vram_snapshot", "pgs_k_init_tables", "pgs_k_triangle_setup",
-    "pgs_k_binning", "pgs_k_shade",
+    "pgs_k_binning", "pgs_k_shade", "pgs_k_clut_write", "pgs_k_texture_decode",
 ]
 
 

@@ -110,3 +110,131 @@ def random_render_stream(seed: int, n_batches: int = 12, prims_per_batch: int = 
         for i in range(0, len(writes), 200):
             out.append(gif.ad(writes[i:i + 200]))
     return gif.arr(b"".join(out))
+
+
+# ---------------------------------------------------------------------------------------
+# Texture state (GS register layouts: TEX0, CLAMP, TEX1, TEXA, TEXCLUT, UV, ST).
+
+def tex0(tbp, tbw, psm, tw, th, tcc=1, tfx=1, cbp=0, cpsm=0, csm=0, csa=0, cld=0):
+    return (tbp | tbw << 14 | psm << 20 | tw << 26 | th << 30 | tcc << 34 | tfx << 35 | cbp << 37 |
+            cpsm << 51 | csm << 55 | csa << 56 | cld << 61)
+
+
+def clamp_reg(wms=0, wmt=0, minu=0, maxu=0, minv=0, maxv=0):
+    return wms | wmt << 2 | minu << 4 | maxu << 14 | minv << 24 | maxv << 34
+
+
+def tex1(mmag=0, mmin=0):
+    return mmag << 5 | mmin << 6
+
+
+def texa(ta0=0, aem=0, ta1=0x80):
+    return ta0 | aem << 15 | ta1 << 32
+
+
+def texclut(cbw, cou, cov):
+    return cbw | cou << 6 | cov << 12
+
+
+def uv(u, v):
+    return (u & 0x3FFF) | (v & 0x3FFF) << 16
+
+
+def st(s, t):
+    f = lambda x: struct.unpack("<I", struct.pack("<f", x))[0]
+    return f(s) | f(t) << 32
+
+
+def csm1_position(i):
+    """CLUT entry i of a CSM1 palette: 8x2 column blocks, the two halves of each 32-entry group
+    side by side (the GS CLUT layout for 256-colour palettes)."""
+    group, j = divmod(i, 32)
+    return (j // 16) * 8 + j % 8, 2 * group + (j // 8) % 2
+
+
+TEX_PSMS = [0x00, 0x01, 0x02, 0x0A, 0x13, 0x14, 0x1B, 0x24, 0x2C]
+
+
+def _random_payload(rng, nbytes):
+    return rng.integers(0, 256, size=(nbytes + 15) // 16 * 16, dtype=np.uint8).tobytes()
+
+
+def random_textured_stream(seed: int, n_batches: int = 10, prims_per_batch: int = 30, width: int = 256,
+                           height: int = 192, feedback: bool = True) -> np.ndarray:
+    """Textured drawing: random texture formats / sizes / wrap modes / filters / combiners,
+    CLUT loads (CSM1, CSM2, CSA, CLD 0..5), UV and ST/Q coordinates, and (optionally) textures
+    read from the frame buffer being drawn to (render-pass hazard flushes)."""
+    rng = np.random.default_rng(seed)
+    ofs = 2048 * 16
+    out = [gif.ad([(0x1A, 1), (0x18, ofs | ofs << 32)])]
+    fbw = (width + 63) // 64
+    # Texture and CLUT area: pages 400..511.
+    tex_pages = [400 + 8 * i for i in range(12)]
+    # Fill the texture area with random bytes (as 32-bit pixels, 64 px wide rows).
+    for page in tex_pages:
+        out.append(gif.upload(page * 32, 1, 0x00, 0, 0, 64, 64, _random_payload(rng, 64 * 64 * 4)))
+    clut_bp = 505 * 32
+    out.append(gif.upload(clut_bp, 1, 0x00, 0, 0, 64, 32, _random_payload(rng, 64 * 32 * 4)))
+    for b in range(n_batches):
+        fb_psm = int(rng.choice([0x00, 0x00, 0x01, 0x02]))
+        zbp = 64 + int(rng.integers(0, 8)) * 32
+        psm = int(rng.choice(TEX_PSMS))
+        tw, th = int(rng.integers(2, 8)), int(rng.integers(2, 8))
+        from_fb = feedback and rng.integers(0, 6) == 0
+        if from_fb:
+            psm, tbp, tbw = fb_psm, 0, fbw
+        else:
+            tbp = int(rng.choice(tex_pages)) * 32 + int(rng.integers(0, 4)) * 32
+            tbw = int(rng.integers(1, 5)) * 2
+        cpsm = int(rng.choice([0x00, 0x02, 0x0A]))
+        csm = int(rng.integers(0, 4) == 0)
+        csa = int(rng.integers(0, 32)) if rng.integers(0, 2) else 0
+        cld = int(rng.choice([0, 1, 1, 2, 3, 4, 5]))
+        cbp = clut_bp + int(rng.integers(0, 4)) * 4
+        wms, wmt = int(rng.integers(0, 4)), int(rng.integers(0, 4))
+        lim = lambda n: int(rng.integers(0, 1 << n))
+        cl = clamp_reg(wms, wmt, lim(tw), lim(tw), lim(th), lim(th))
+        if wms == 3:
+            cl = (cl & ~(0xFFFFF << 4)) | (int(rng.integers(0, 1 << tw)) << 4) | (int(rng.integers(0, 64)) << 14)
+        if wmt == 3:
+            cl = (cl & ~(0xFFFFF << 24)) | (int(rng.integers(0, 1 << th)) << 24) | (int(rng.integers(0, 64)) << 34)
+        regs = [
+            (0x4C, frame(0, fbw, fb_psm)),
+            (0x4E, zbuf(zbp, 1, int(rng.integers(0, 3) == 0))),
+            (0x40, scissor(0, width - 1, 0, height - 1)),
+            (0x47, test_reg(ate=int(rng.integers(0, 2)), atst=int(rng.integers(0, 8)), aref=int(rng.integers(0, 256)),
+                            afail=int(rng.integers(0, 4)), zte=1, ztst=int(rng.integers(1, 4)))),
+            (0x42, alpha_reg(0, 1, 0, 1, 0)),
+            (0x3B, texa(int(rng.integers(0, 256)), int(rng.integers(0, 2)), int(rng.integers(0, 256)))),
+            (0x1C, texclut(int(rng.integers(1, 4)), int(rng.integers(0, 4)), int(rng.integers(0, 8)))),
+            (0x14, tex1(int(rng.integers(0, 2)), int(rng.integers(0, 2)))),
+            (0x08, cl),
+            (0x06, tex0(tbp, tbw, psm, tw, th, tcc=int(rng.integers(0, 2)), tfx=int(rng.integers(0, 4)), cbp=cbp,
+                        cpsm=cpsm, csm=csm, csa=csa, cld=cld)),
+            (0x3D, int(rng.integers(0, 2**24))),
+        ]
+        ptype = int(rng.choice([3, 3, 4, 5, 6, 6, 1, 2]))
+        fst = int(rng.integers(0, 3) != 0)
+        prim = prim_reg(ptype, iip=int(rng.integers(0, 2)), tme=1, abe=int(rng.integers(0, 2)),
+                        fge=int(rng.integers(0, 2)), fst=fst)
+        regs.append((0x00, prim))
+        out.append(gif.ad(regs))
+        writes = []
+        nv = {1: 2, 2: 2, 3: 3, 4: 3, 5: 3, 6: 2}[ptype]
+        tsz_u, tsz_v = (1 << tw) * 16, (1 << th) * 16
+        for p in range(prims_per_batch):
+            q_const = float(rng.choice([1.0, 0.5, 2.0]))
+            for v in range(nv):
+                x = ofs + int(rng.integers(-16 * 16, (width + 16) * 16))
+                y = ofs + int(rng.integers(-16 * 16, (height + 16) * 16))
+                z = int(rng.integers(0, 2**24))
+                q = q_const if rng.integers(0, 2) else float(rng.uniform(0.5, 2.0))
+                writes.append((0x01, rgbaq(int(rng.integers(0, 2**32)), q)))
+                if fst:
+                    writes.append((0x03, uv(int(rng.integers(0, 2 * tsz_u)), int(rng.integers(0, 2 * tsz_v)))))
+                else:
+                    writes.append((0x02, st(float(rng.uniform(-0.5, 2.0)) * q, float(rng.uniform(-0.5, 2.0)) * q)))
+                writes.append((0x05, xyz2(x, y, z)))
+        for i in range(0, len(writes), 240):
+            out.append(gif.ad(writes[i:i + 240]))
+    return gif.arr(b"".join(out))
