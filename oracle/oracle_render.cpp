@@ -930,6 +930,11 @@ bool setup_primitive(Prim &p, Setup &s, uint32_t rate_log2, uint32_t opaque_fbma
 			per_sample = za != zb || zb != zc;
 		if (!per_sample && !line)
 			p.state |= S_SNAP_ATTRIBUTE;
+		if (!per_sample)
+		{
+			s.flat = true;
+			s.flat_z = za;
+		}
 	}
 	return true;
 }
@@ -1003,6 +1008,437 @@ bool alpha_test(uint32_t atst, uint32_t aref, float a)
 	default: return true;
 	}
 }
+// triangle_setup_intersects_tile (intersect.h:24-81): conservative test of a primitive against
+// the pixel rectangle [lo, hi], clipped to the primitive's box first.
+bool intersects_tile(const Setup &s, int32_t tlox, int32_t tloy, int32_t thix, int32_t thiy, bool multisampled_tri)
+{
+	int32_t lox = std::max(s.bb[0], tlox), loy = std::max(s.bb[1], tloy);
+	int32_t hix = std::min(s.bb[2], thix), hiy = std::min(s.bb[3], thiy);
+	if (hix < lox || hiy < loy)
+		return false;
+	const bool para = (s.a[0] & 1) != 0;
+	const int32_t offb = int32_t((uint32_t(s.a[0]) >> 2) & 3u) - 1;
+	const int32_t offc = int32_t((uint32_t(s.a[0]) >> 4) & 3u) - 1;
+	const int32_t a_eq[3] = { s.a[0] >> 6, s.a[1], s.a[2] };
+	lox <<= 3;
+	loy <<= 3;
+	hix = (hix << 3) | 7;
+	hiy = (hiy << 3) | 7;
+	auto pick = [&](int32_t e, int32_t lo, int32_t hi) { return e >= 0 ? hi : lo; };
+	int32_t da = idot3(a_eq, pick(a_eq[0], lox, hix), pick(a_eq[1], loy, hiy));
+	int32_t db = idot3(s.b, pick(s.b[0], lox, hix), pick(s.b[1], loy, hiy));
+	int32_t dc = idot3(s.c, pick(s.c[0], lox, hix), pick(s.c[1], loy, hiy));
+	if (multisampled_tri && !para)
+	{
+		da = wadd(da, 8 * std::max(std::abs(a_eq[0]), std::abs(a_eq[1])));
+		db = wadd(db, 8 * std::max(std::abs(s.b[0]), std::abs(s.b[1])));
+		dc = wadd(dc, 8 * std::max(std::abs(s.c[0]), std::abs(s.c[1])));
+	}
+	bool hit = da >= 0 && db >= 0 && dc >= 0;
+	if (para && !hit)
+	{
+		const int32_t xa = pick(a_eq[0], lox, hix), ya = pick(a_eq[1], loy, hiy);
+		int32_t pa = idot3(a_eq, xa, ya);
+		int32_t pb = idot3(s.b, pick(-s.c[0], lox, hix), pick(-s.c[1], loy, hiy));
+		int32_t pc = idot3(s.c, pick(-s.b[0], lox, hix), pick(-s.b[1], loy, hiy));
+		pb = wadd(pb, wadd(pa, offb));
+		pc = wadd(pc, wadd(pa, offc));
+		const int32_t na = idot3(a_eq, a_eq[0] >= 0 ? lox : hix, a_eq[1] >= 0 ? loy : hiy);
+		hit = -1 >= na && pb >= 0 && pc >= 0;
+	}
+	return hit;
+}
+
+// single_sample_heuristic.comp:148-290, run per active depth bucket. Every bucket sees the
+// states as triangle setup left them; only the final step writes states, and only for
+// primitives of its own bucket (the reference runs buckets as concurrent workgroups).
+uint32_t hash_vertex(const Vertex &v)
+{
+	const uint32_t P = 1103515245u;
+	uint32_t x = uint32_t(v.x), y = uint32_t(v.y), z = v.z;
+	for (int r = 0; r < 3; r++)
+	{
+		const uint32_t nx = ((x >> 8) ^ y) * P, ny = ((y >> 8) ^ z) * P, nz = ((z >> 8) ^ x) * P;
+		x = nx; y = ny; z = nz;
+	}
+	return x >> 16;
+}
+
+void single_sample_heuristic(std::vector<Prim> &prims, const std::vector<Setup> &setups, uint32_t depth_psm)
+{
+	uint32_t z_shift_setup, z_shift_bucket, z_shift, z_max;
+	switch (depth_psm)
+	{
+	case Z16: case Z16S: z_shift_setup = 8; z_shift_bucket = 8; z_shift = 0; z_max = 0xffffu; break;
+	case Z32: z_shift_setup = 24; z_shift_bucket = 16; z_shift = 8; z_max = 0xffffffffu; break;
+	default: z_shift_setup = 16; z_shift_bucket = 16; z_shift = 0; z_max = 0xffffffu; break;
+	}
+	std::vector<bool> active(256, false);
+	bool any = false;
+	for (size_t i = 0; i < prims.size(); i++)
+		if (setups[i].flat)
+		{
+			active[std::min(255u, setups[i].flat_z >> z_shift_setup)] = true;
+			any = true;
+		}
+	if (!any)
+		return;
+
+	const uint32_t NO_FULL_3D = S_SPRITE | S_LINE | S_SNAP_ATTRIBUTE;
+	std::vector<uint32_t> pre(prims.size());
+	bool has_full_3d = false;
+	uint32_t min3d = ~0u, max3d = 0;
+	for (size_t i = 0; i < prims.size(); i++)
+	{
+		pre[i] = prims[i].state;
+		if ((pre[i] & NO_FULL_3D) == 0)
+		{
+			has_full_3d = true;
+			for (int k = 0; k < 3; k++)
+			{
+				min3d = std::min(min3d, prims[i].v[k].z);
+				max3d = std::max(max3d, prims[i].v[k].z);
+			}
+		}
+	}
+	const bool force_single = !has_full_3d;
+	auto snapped = [&](uint32_t z) { return std::min(z_max, z) >> z_shift; };
+
+	std::vector<uint8_t> full3d(65536), cand(65536), shared(65536);
+	for (uint32_t b = 0; b < 256; b++)
+	{
+		if (!active[b])
+			continue;
+		const uint32_t zlo = b << z_shift_bucket, zhi = ((b + 1u) << z_shift_bucket) - 1u;
+		auto in_range = [&](uint32_t sz) { return sz >= zlo && sz <= zhi; };
+		std::fill(full3d.begin(), full3d.end(), 0);
+		std::fill(cand.begin(), cand.end(), 0);
+		std::fill(shared.begin(), shared.end(), 0);
+		for (size_t i = 0; i < prims.size(); i++)
+		{
+			const uint32_t st = pre[i];
+			if ((st & NO_FULL_3D) == 0)
+			{
+				for (int k = 0; k < 3; k++)
+				{
+					const uint32_t sz = snapped(prims[i].v[k].z);
+					if (in_range(sz))
+						full3d[sz - zlo] = 1;
+				}
+			}
+			else if ((st & (S_SNAP_ATTRIBUTE | S_SNAP_RASTER)) == S_SNAP_ATTRIBUTE || (st & S_LINE))
+			{
+				const uint32_t sz = snapped(prims[i].v[0].z);
+				if (in_range(sz))
+					cand[sz - zlo] = 1;
+			}
+		}
+		for (size_t i = 0; i < prims.size(); i++)
+		{
+			if ((pre[i] & NO_FULL_3D) != 0)
+				continue;
+			for (int k = 0; k < 3; k++)
+			{
+				const uint32_t sz = snapped(prims[i].v[k].z);
+				if (in_range(sz) && cand[sz - zlo])
+					shared[hash_vertex(prims[i].v[k])] = 1;
+			}
+		}
+		for (size_t i = 0; i < prims.size(); i++)
+		{
+			if ((pre[i] & (S_SNAP_ATTRIBUTE | S_SNAP_RASTER)) != S_SNAP_ATTRIBUTE)
+				continue;
+			const uint32_t sz = snapped(prims[i].v[0].z);
+			if (!in_range(sz) || !full3d[sz - zlo])
+				continue;
+			if (shared[hash_vertex(prims[i].v[0])] || shared[hash_vertex(prims[i].v[1])] ||
+			    shared[hash_vertex(prims[i].v[2])])
+				cand[sz - zlo] = 0;
+		}
+		for (size_t i = 0; i < prims.size(); i++)
+		{
+			const uint32_t z = prims[i].v[0].z;
+			const uint32_t sz = snapped(z);
+			if (!in_range(sz))
+				continue;
+			uint32_t st = pre[i];
+			const bool tri_cand = (st & (S_SNAP_ATTRIBUTE | S_SNAP_RASTER)) == S_SNAP_ATTRIBUTE;
+			const bool line_cand = force_single && (st & S_LINE);
+			if (!tri_cand && !line_cand)
+				continue;
+			if (cand[sz - zlo])
+				prims[i].state = st | S_SNAP_RASTER;
+			else if ((st & (S_Z_TEST | S_Z_WRITE)) && z > min3d && z < max3d)
+				prims[i].state = st & ~S_SNAP_ATTRIBUTE;
+		}
+	}
+}
+
+// Hi-res sample offsets inside a pixel for sample index s (ubershader.comp:133-385: lane bits
+// of the shading wave map to sample positions; sample index = lane & (samples - 1)).
+void sample_offset(uint32_t rate_x, uint32_t rate_y, uint32_t s, int32_t &dx, int32_t &dy)
+{
+	if (rate_x == 0 && rate_y == 1) { dx = int32_t(s); dy = int32_t(s); }
+	else if (rate_x == 1 && rate_y == 1) { dx = int32_t(s >> 1); dy = int32_t(s & 1u); }
+	else if (rate_x == 1 && rate_y == 2)
+	{
+		dx = int32_t(2u * ((s >> 1) & 1u) + (s & 1u));
+		dy = int32_t((s & 1u) + 2u * ((s >> 2) & 1u));
+	}
+	else if (rate_x == 2 && rate_y == 2)
+	{
+		dx = int32_t(((s >> 1) & 1u) + 2u * ((s >> 3) & 1u));
+		dy = int32_t((s & 1u) + 2u * ((s >> 2) & 1u));
+	}
+	else { dx = 0; dy = 0; }
+}
+// A render pass of a super-sampled context (planes: 0 = single-sampled VRAM, 1 = reference
+// marking pixels whose super-samples are valid, 2 + s = sample s).
+//
+// Coarse tiles start at the pass bounding box (gs/gs_interface.cpp:148-187, 214-215); each tile
+// is shaded super-sampled or single-sampled as binning decides (binning.comp:87-125, 272-312).
+// begin_tile / end_tile follow ubershader.comp:435-780: samples load from their planes where
+// the reference marks the pixel, and a dirty pixel whose samples differ stores every sample,
+// the reference mask and the resolved value (float average in butterfly order, minimum depth).
+template <typename Shade>
+uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t context_rate_y, std::vector<Prim> &prims,
+                                   const std::vector<Setup> &setups, const std::vector<bool> &live,
+                                   const PassFormat &f, uint32_t rate_x, uint32_t rate_y, Shade &&shade)
+{
+	const uint32_t n = uint32_t(prims.size());
+	const bool ssaa = rate_y != 0;
+	const uint32_t S = 1u << (rate_x + rate_y);
+	const bool zs = f.z_psm != 0xffffffffu;
+	const bool fb16 = psm16(f.fb_psm), fb24 = psm24(f.fb_psm);
+	const bool z16 = zs && psm16(f.z_psm), z24 = zs && psm24(f.z_psm);
+	const uint32_t fb_base = f.fbp * 32u, z_base = f.zbp * 32u;
+	const size_t P = size_t(4) << 20;
+
+	uint64_t drawn = 0;
+	for (uint32_t i = 0; i < n; i++)
+		drawn += live[i] ? 1u : 0u;
+
+	// Coarse tile size (gs/gs_interface.cpp:148-187), in the reference's 32-bit arithmetic.
+	const uint32_t w8 = uint32_t((bb[2] - bb[0]) >> 3) + 1u, h8 = uint32_t((bb[3] - bb[1]) >> 3) + 1u;
+	const uint32_t cost = w8 * h8 * n;
+	uint32_t tl = cost < 10000u ? 3u : cost < 10000000u ? 4u : cost < 100000000u ? 5u : 6u;
+	if (context_rate_y != 0 && tl > 3)
+		tl--;
+	{
+		const uint32_t cw = uint32_t((bb[2] - bb[0]) >> tl) + 1u, ch = uint32_t((bb[3] - bb[1]) >> tl) + 1u;
+		uint64_t list_bytes = uint64_t(uint32_t(cw * ch * n)) * 2u;
+		while (list_bytes > 48ull * 1024 * 1024)
+		{
+			tl++;
+			list_bytes /= 4;
+		}
+	}
+	const uint32_t cw = uint32_t((bb[2] - bb[0]) >> tl) + 1u, ch = uint32_t((bb[3] - bb[1]) >> tl) + 1u;
+	const int32_t tsize = 1 << tl;
+
+	auto rd32 = [&](size_t a) { uint32_t d; memcpy(&d, v0 + a, 4); return d; };
+	auto rd16 = [&](size_t a) { uint16_t d; memcpy(&d, v0 + a, 2); return uint32_t(d); };
+	auto wr32 = [&](size_t a, uint32_t d) { memcpy(v0 + a, &d, 4); };
+	auto wr16 = [&](size_t a, uint32_t d) { uint16_t h = uint16_t(d); memcpy(v0 + a, &h, 2); };
+
+	std::vector<uint32_t> list;
+	Regs regs[16];
+	for (uint32_t ty = 0; ty < ch; ty++)
+	{
+		for (uint32_t tx = 0; tx < cw; tx++)
+		{
+			const int32_t lox = bb[0] + int32_t(tx) * tsize, loy = bb[1] + int32_t(ty) * tsize;
+			const int32_t hix = lox + tsize - 1, hiy = loy + tsize - 1;
+			list.clear();
+			bool req = false, req_zw = false, has_zw = false;
+			for (uint32_t i = 0; i < n; i++)
+			{
+				if (!live[i])
+					continue;
+				const uint32_t st = prims[i].state;
+				if (!intersects_tile(setups[i], lox, loy, hix, hiy, (st & S_MULTISAMPLE) && !(st & S_LINE)))
+					continue;
+				list.push_back(i);
+				if ((st & (S_SNAP_RASTER | S_Z_TEST | S_OPAQUE)) != (S_SNAP_RASTER | S_OPAQUE))
+					req_zw = true;
+				if ((st & (S_SNAP_RASTER | S_OPAQUE)) != (S_SNAP_RASTER | S_OPAQUE))
+					req = true;
+				if (st & S_Z_WRITE)
+					has_zw = true;
+			}
+			if (list.empty())
+				continue;
+			const bool super = ssaa && (req || (req_zw && has_zw));
+			const uint32_t ns = super ? S : 1u;
+			const uint32_t rate = super ? rate_y : 0u;
+
+			for (int32_t py = loy; py <= hiy; py++)
+			{
+				for (int32_t px = lox; px <= hix; px++)
+				{
+					bool any = false;
+					for (uint32_t i : list)
+					{
+						const Setup &s = setups[i];
+						if (px >= s.bb[0] && px <= s.bb[2] && py >= s.bb[1] && py <= s.bb[3])
+						{
+							any = true;
+							break;
+						}
+					}
+					if (!any)
+						continue;
+
+					// begin_tile
+					const size_t ca = fb16 ? size_t(swizzle(uint32_t(px), uint32_t(py), fb_base, f.fbw, f.fb_psm)) * 2u
+					                       : size_t(swizzle(uint32_t(px), uint32_t(py), fb_base, f.fbw, f.fb_psm)) * 4u;
+					size_t za = 0;
+					if (zs)
+						za = z16 ? size_t(swizzle(uint32_t(px), uint32_t(py), z_base, f.fbw, f.z_psm)) * 2u
+						         : size_t(swizzle(uint32_t(px), uint32_t(py), z_base, f.fbw, f.z_psm)) * 4u;
+					for (uint32_t smp = 0; smp < ns; smp++)
+					{
+						Regs &r = regs[smp];
+						r = Regs{};
+						r.loaded = true;
+						if (fb16)
+						{
+							uint32_t c = rd16(ca);
+							if (super && rd16(ca + P) == 0xffffu)
+								c = rd16(ca + (2u + smp) * P);
+							r.color = rgba16_to_rgba32(uint16_t(c), 0, 0, 0x80);
+						}
+						else
+						{
+							r.color = rd32(ca);
+							if (super)
+							{
+								const uint32_t ref = rd32(ca + P), mask = fb24 ? 0xffffffu : ~0u;
+								if ((ref & mask) == mask)
+								{
+									const uint32_t sc = rd32(ca + (2u + smp) * P);
+									r.color = (fb24 && ref != ~0u) ? ((r.color & 0xff000000u) | (sc & 0xffffffu)) : sc;
+								}
+							}
+						}
+						if (zs)
+						{
+							if (z16)
+							{
+								r.depth = rd16(za);
+								if (super && rd16(za + P) == 0xffffu)
+									r.depth = rd16(za + (2u + smp) * P);
+							}
+							else
+							{
+								const uint32_t mask = z24 ? 0xffffffu : ~0u;
+								r.depth = rd32(za) & mask;
+								if (super && (rd32(za + P) & mask) == mask)
+									r.depth = rd32(za + (2u + smp) * P) & mask;
+							}
+						}
+					}
+
+					for (uint32_t i : list)
+					{
+						const Setup &s = setups[i];
+						if (!(px >= s.bb[0] && px <= s.bb[2] && py >= s.bb[1] && py <= s.bb[3]))
+							continue;
+						for (uint32_t smp = 0; smp < ns; smp++)
+						{
+							int32_t dx = 0, dy = 0;
+							if (super)
+								sample_offset(rate_x, rate_y, smp, dx, dy);
+							const int32_t hx = (px << rate) + dx, hy = (py << rate) + dy;
+							shade(prims[i], s, hx, hy, rate, super, [&]() -> Regs & { return regs[smp]; });
+						}
+					}
+
+					// end_tile
+					bool req_c = false, req_d = false, cdirty = false, zdirty = false;
+					for (uint32_t smp = 0; smp < ns; smp++)
+					{
+						req_c |= regs[smp].color != regs[0].color;
+						req_d |= zs && regs[smp].depth != regs[0].depth;
+						cdirty |= regs[smp].cdirty;
+						zdirty |= zs && regs[smp].zdirty;
+					}
+					uint32_t color = regs[0].color, depth = regs[0].depth;
+					if (req_c)
+					{
+						if (cdirty)
+							for (uint32_t smp = 0; smp < ns; smp++)
+							{
+								if (fb16)
+									wr16(ca + (2u + smp) * P, rgba32_to_rgba16(regs[smp].color));
+								else
+									wr32(ca + (2u + smp) * P, regs[smp].color);
+							}
+						float acc[16][4];
+						for (uint32_t smp = 0; smp < ns; smp++)
+							for (uint32_t k = 0; k < 4; k++)
+								acc[smp][k] = unorm(regs[smp].color, k);
+						for (uint32_t m = ns >> 1; m >= 1; m >>= 1)
+						{
+							float nx[16][4];
+							for (uint32_t smp = 0; smp < ns; smp++)
+								for (uint32_t k = 0; k < 4; k++)
+									nx[smp][k] = acc[smp][k] + acc[smp ^ m][k];
+							memcpy(acc, nx, sizeof(acc));
+						}
+						float avg[4];
+						for (uint32_t k = 0; k < 4; k++)
+							avg[k] = acc[0][k] / float(ns);
+						color = pack_unorm(avg);
+					}
+					if (req_d)
+					{
+						if (zdirty)
+							for (uint32_t smp = 0; smp < ns; smp++)
+							{
+								if (z16)
+									wr16(za + (2u + smp) * P, regs[smp].depth);
+								else
+									wr32(za + (2u + smp) * P, regs[smp].depth);
+							}
+						for (uint32_t smp = 1; smp < ns; smp++)
+							depth = std::min(depth, regs[smp].depth);
+					}
+					if (cdirty)
+					{
+						if (fb16)
+						{
+							wr16(ca + P, req_c ? 0xffffu : 0u);
+							wr16(ca, rgba32_to_rgba16(color));
+						}
+						else
+						{
+							wr32(ca + P, req_c ? ~0u : 0u);
+							wr32(ca, color);
+						}
+					}
+					if (zdirty)
+					{
+						if (z16)
+						{
+							wr16(za + P, req_d ? 0xffffu : 0u);
+							wr16(za, depth);
+						}
+						else
+						{
+							wr32(za + P, req_d ? (z24 ? 0xffffffu : ~0u) : 0u);
+							if (z24)
+								memcpy(v0 + za, &depth, 3);
+							else
+								wr32(za, depth);
+						}
+					}
+				}
+			}
+		}
+	}
+	return drawn;
+}
 }
 
 // Texture sampling lives in oracle_texture.cpp.
@@ -1025,22 +1461,255 @@ void GS::render(const std::vector<Prim> &prims_in)
 		else if (zpsm == Z24) zpsm = CT24;
 		else if (zpsm == Z16S || zpsm == Z16) zpsm = CT16S;
 	}
-	bool z_sensitive = false;
+	bool z_sensitive = false, z_write_any = false;
 	uint32_t write_mask = 0;
 	for (auto &p : prims)
 	{
 		z_sensitive |= (p.state & (S_Z_TEST | S_Z_WRITE)) != 0;
+		z_write_any |= (p.state & S_Z_WRITE) != 0;
 		write_mask |= ~p.fbmsk;
 	}
 	f.z_psm = z_sensitive ? zpsm : 0xffffffffu;
 	const uint32_t opaque_fbmask = ~write_mask;
-	const uint32_t rate = 0; // 1x rendering path
-	(void)rate;
+
+	// Sampling rate of the pass (gs/gs_interface.cpp:212-260): x/y log2 from the context's
+	// sample count (2x: 0/1, 4x: 1/1, 8x: 1/2, 16x: 2/2); channel-sliced 16-bit colour masks
+	// and colour/depth aliasing with depth writes render single-sampled.
+	uint32_t rate_x = sample_log2 / 2, rate_y = (sample_log2 + 1) / 2;
+	if (sample_log2)
+	{
+		const bool fb_32_24 = f.fb_psm == CT32 || f.fb_psm == CT24 || f.fb_psm == Z32 || f.fb_psm == Z24;
+		const uint32_t m = write_mask & 0xf8f8f8u;
+		const bool channel_slice = !fb_32_24 && m != 0xf8f8f8u && m != 0;
+		const bool aliased = z_write_any && f.zbp == f.fbp;
+		if (channel_slice || aliased)
+			rate_x = rate_y = 0;
+	}
 
 	std::vector<Setup> setups(prims.size());
 	std::vector<bool> live(prims.size());
 	for (size_t i = 0; i < prims.size(); i++)
-		live[i] = setup_primitive(prims[i], setups[i], sample_log2 ? (sample_log2 + 1) / 2 : 0, opaque_fbmask);
+		live[i] = setup_primitive(prims[i], setups[i], rate_y, opaque_fbmask);
+	if (rate_y != 0)
+		single_sample_heuristic(prims, setups, z_sensitive ? (field(rp_zbuf, 24, 4) | 0x30u) : uint32_t(Z24));
+
+	const int32_t snap_mask = -(1 << rate_y);
+
+	// One primitive at one sample: hi-res coordinate (hx, hy) of pixel (hx >> rate, hy >> rate).
+	auto shade = [&](const Prim &p, const Setup &s, int32_t hx, int32_t hy, uint32_t rate, bool super_sampled,
+	                 auto &&get_regs) {
+		const uint32_t st = p.state;
+		const bool z_test = f.z_psm != 0xffffffffu && (st & S_Z_TEST);
+		const bool z_write = f.z_psm != 0xffffffffu && (st & S_Z_WRITE);
+		const bool z_greater = st & S_Z_GREATER;
+		const bool persp = st & S_PERSPECTIVE;
+		// shade_early (ubershader.comp:782-910): snapped primitives rasterize at the
+		// pixel's first sample position.
+		const bool snap_raster = super_sampled && (st & S_SNAP_RASTER);
+		const int32_t rx = snap_raster ? (hx & snap_mask) : hx, ry = snap_raster ? (hy & snap_mask) : hy;
+		float fi, fj;
+		int32_t cov = evaluate_coverage(s, rx, ry, rate, fi, fj);
+		if (cov == 0)
+			return;
+		if (st & (S_SCANMSK_EVEN | S_SCANMSK_ODD))
+		{
+			const int32_t sy = (st & S_SPRITE) ? (hy >> rate) : hy;
+			if ((sy & 1) == 0 ? (st & S_SCANMSK_EVEN) : (st & S_SCANMSK_ODD))
+				return;
+		}
+		covered_samples++;
+		Regs &r = get_regs();
+
+		// Z (ubershader.comp:852-871): no FMA contraction of dzdi * i + dzdj * j.
+		uint32_t z = s.z[0];
+		float t0 = bitsf(s.z[1]) * fi;
+		float t1 = bitsf(s.z[2]) * fj;
+		float zf = std::nearbyint(t0 + t1);
+		zf = std::min(std::max(zf, 0.0f), 4294967040.0f);
+		uint32_t dz = uint32_t(zf);
+		dz = std::min(dz, 0xffffffffu - z);
+		z += dz;
+		if (f.z_psm != 0xffffffffu)
+		{
+			if (psm16(f.z_psm)) z = std::min(z, 0xffffu);
+			else if (psm24(f.z_psm)) z = std::min(z, 0xffffffu);
+		}
+
+		if (z_test && !(z_greater ? z > r.depth : z >= r.depth))
+			return;
+		// SNAP_ATTRIBUTE without SNAP_RASTER: attributes at the pixel's first sample
+		// (classic MSAA); depth stays per sample.
+		if (super_sampled && (st & (S_SNAP_RASTER | S_SNAP_ATTRIBUTE)) == S_SNAP_ATTRIBUTE)
+			barycentric(s, hx & snap_mask, hy & snap_mask, rate, fi, fj);
+
+		// Combined colour (ubershader.comp:1260-1339).
+		float col[4];
+		for (uint32_t ch = 0; ch < 4; ch++)
+		{
+			float c0 = unorm(s.rgba[0], ch), c1 = unorm(s.rgba[1], ch), c2 = unorm(s.rgba[2], ch);
+			float d1 = c1 - c0, d2 = c2 - c0;
+			float e1 = d1 * fi, e2 = d2 * fj;
+			float sum = c0 + e1;
+			sum = sum + e2;
+			col[ch] = std::floor(255.0f * sum + 1.0f / 256.0f);
+		}
+		float comb[4] = { col[0], col[1], col[2], col[3] };
+		if (p.combiner & C_TME)
+		{
+			float sq[3];
+			for (int k = 0; k < 3; k++)
+			{
+				float e1 = s.stqf1[k] * fi, e2 = s.stqf2[k] * fj;
+				float sum = s.stqf0[k] + e1;
+				sq[k] = sum + e2;
+			}
+			float u = sq[0], vv = sq[1];
+			if (persp && sq[2] != 1.0f)
+			{
+				u = u / sq[2];
+				vv = vv / sq[2];
+			}
+			// NClamp: a NaN coordinate clamps to the lower bound.
+			u = std::fmin(std::fmax(u, -2047.0f), 2047.0f);
+			vv = std::fmin(std::fmax(vv, -2047.0f), 2047.0f);
+			float tx[4];
+			sample_texture(rp_textures[p.tex_slot], p.tex, u, vv, sq[2], persp, tx);
+			const bool tcc = p.combiner & C_TCC;
+			switch (p.combiner & 3u)
+			{
+			case 1: // DECAL
+				for (int k = 0; k < 4; k++) comb[k] = tx[k];
+				if (!tcc) comb[3] = col[3];
+				break;
+			case 0: // MODULATE
+				for (int k = 0; k < 4; k++)
+					comb[k] = std::min(std::max(std::floor(col[k] * tx[k] * (1.0f / 128.0f)), 0.0f), 255.0f);
+				if (!tcc) comb[3] = col[3];
+				break;
+			default: // HIGHLIGHT / HIGHLIGHT2
+			{
+				for (int k = 0; k < 4; k++)
+					comb[k] = std::min(std::max(std::floor(col[k] * tx[k] * (1.0f / 128.0f)), 0.0f), 255.0f);
+				for (int k = 0; k < 3; k++) comb[k] += col[3];
+				comb[3] = tcc ? ((p.combiner & 3u) == 2 ? tx[3] + col[3] : tx[3]) : col[3];
+				for (int k = 0; k < 4; k++) comb[k] = std::min(std::max(comb[k], 0.0f), 255.0f);
+				break;
+			}
+			}
+		}
+		if (p.combiner & C_FOG)
+		{
+			float e1 = s.stqf1[3] * fi, e2 = s.stqf2[3] * fj;
+			float fog = std::nearbyint((s.stqf0[3] + e1) + e2);
+			for (uint32_t k = 0; k < 3; k++)
+			{
+				float fc = unorm(p.fogcol, k) * 255.0f;
+				float d = (comb[k] - fc) * fog;
+				comb[k] = std::floor(fc + d * (1.0f / 256.0f) + 0.5f);
+			}
+		}
+
+		// shade_resolve (ubershader.comp:1464-1607)
+		float fb[4];
+		for (uint32_t k = 0; k < 4; k++)
+			fb[k] = std::nearbyint(unorm(r.color, k) * 255.0f);
+		bool write_fb = true, write_zb = true, write_alpha = true;
+		const uint32_t bm = p.blend_mode;
+		bool blending = bm & B_ABE;
+		bool keep_all = false;
+		if (bm & B_ATE)
+		{
+			if (!alpha_test((bm >> 2) & 7u, p.alpha & 0xffu, comb[3]))
+			{
+				switch ((bm >> 5) & 3u)
+				{
+				case 0: keep_all = true; break;
+				case 1: write_zb = false; break;
+				case 2: write_fb = false; write_alpha = false; break;
+				case 3:
+					if (psm32(f.fb_psm)) write_alpha = false;
+					write_zb = false;
+					break;
+				}
+			}
+		}
+		if (keep_all)
+			return;
+		if (!psm24(f.fb_psm) && (bm & B_DATE))
+		{
+			uint32_t bit = fb[3] >= 128.0f ? 1u : 0u;
+			if (uint32_t((bm & B_DATM) != 0) != bit)
+				return;
+		}
+		if ((bm & B_PABE) && comb[3] < 128.0f)
+			blending = false;
+		float out[4] = { comb[0], comb[1], comb[2], comb[3] };
+		if (blending)
+		{
+			float A[3] = { 0, 0, 0 }, B[3] = { 0, 0, 0 }, D[3] = { 0, 0, 0 }, C = 0.0f;
+			uint32_t am = (bm >> 11) & 3u, bmd = (bm >> 13) & 3u, cm = (bm >> 15) & 3u, dm = (bm >> 17) & 3u;
+			for (int k = 0; k < 3; k++)
+			{
+				A[k] = am == 0 ? comb[k] : am == 1 ? fb[k] : 0.0f;
+				B[k] = bmd == 0 ? comb[k] : bmd == 1 ? fb[k] : 0.0f;
+				D[k] = dm == 0 ? comb[k] : dm == 1 ? fb[k] : 0.0f;
+			}
+			if (cm == 0) C = comb[3];
+			else if (cm == 1) C = psm24(f.fb_psm) ? 128.0f : fb[3];
+			else if (cm == 2) C = float((p.alpha >> 8) & 0xffu);
+			for (int k = 0; k < 3; k++)
+				out[k] = std::floor((A[k] - B[k]) * C * (1.0f / 128.0f)) + D[k];
+		}
+		if (bm & B_DTHE)
+		{
+			int32_t px = (hx >> rate) & 3, py = (hy >> rate) & 3;
+			int32_t bit_index = ((py * 4 + px) * 4) & 31;
+			uint32_t dm = py < 2 ? p.dimx[0] : p.dimx[1];
+			int32_t dv = int32_t(dm << (29 - bit_index)) >> 29;
+			for (int k = 0; k < 3; k++)
+				out[k] += float(dv);
+		}
+		if (psm24(f.fb_psm))
+			write_alpha = false;
+		if (!write_alpha)
+			out[3] = fb[3];
+		if (!write_fb)
+			for (int k = 0; k < 3; k++) out[k] = fb[k];
+		float norm[4];
+		if (!(bm & B_COLCLAMP))
+		{
+			for (int k = 0; k < 4; k++)
+			{
+				float q = out[k] / 256.0f;
+				norm[k] = (q - std::floor(q)) * (256.0f / 255.0f);
+			}
+		}
+		else
+			for (int k = 0; k < 4; k++)
+				norm[k] = out[k] * (1.0f / 255.0f);
+		uint32_t packed = pack_unorm(norm);
+		packed |= uint32_t((bm & B_FBA) && write_alpha) << 31;
+
+		if (p.fbmsk != ~0u && (write_fb || write_alpha))
+		{
+			r.color = p.fbmsk ? ((packed & ~p.fbmsk) | (r.color & p.fbmsk)) : packed;
+			if (psm16(f.fb_psm))
+				r.color &= 0x80f8f8f8u;
+			r.cdirty = true;
+		}
+		if (z_write && write_zb)
+		{
+			r.depth = z;
+			r.zdirty = true;
+		}
+	};
+
+	if (sample_log2 != 0)
+	{
+		primitives_drawn += render_super_sampled_pass(vram.data(), rp_bb, (sample_log2 + 1) / 2, prims, setups, live, f,
+		                                              rate_x, rate_y, shade);
+		return;
+	}
 
 	const uint32_t fb_base = f.fbp * 32u, z_base = f.zbp * 32u;
 	uint8_t *v0 = vram.data();
@@ -1090,208 +1759,9 @@ void GS::render(const std::vector<Prim> &prims_in)
 		const Prim &p = prims[pi];
 		const Setup &s = setups[pi];
 		primitives_drawn++;
-		const uint32_t st = p.state;
-		const bool z_test = f.z_psm != 0xffffffffu && (st & S_Z_TEST);
-		const bool z_write = f.z_psm != 0xffffffffu && (st & S_Z_WRITE);
-		const bool z_greater = st & S_Z_GREATER;
-		const bool persp = st & S_PERSPECTIVE;
-
 		for (int32_t y = s.bb[1]; y <= s.bb[3]; y++)
-		{
 			for (int32_t x = s.bb[0]; x <= s.bb[2]; x++)
-			{
-				float fi, fj;
-				int32_t cov = evaluate_coverage(s, x, y, 0, fi, fj);
-				if (cov == 0)
-					continue;
-				if (st & (S_SCANMSK_EVEN | S_SCANMSK_ODD))
-				{
-					if ((y & 1) == 0 ? (st & S_SCANMSK_EVEN) : (st & S_SCANMSK_ODD))
-						continue;
-				}
-				covered_samples++;
-				Regs &r = load(x, y);
-
-				// Z (ubershader.comp:852-871): no FMA contraction of dzdi * i + dzdj * j.
-				uint32_t z = s.z[0];
-				float t0 = bitsf(s.z[1]) * fi;
-				float t1 = bitsf(s.z[2]) * fj;
-				float zf = std::nearbyint(t0 + t1);
-				zf = std::min(std::max(zf, 0.0f), 4294967040.0f);
-				uint32_t dz = uint32_t(zf);
-				dz = std::min(dz, 0xffffffffu - z);
-				z += dz;
-				if (f.z_psm != 0xffffffffu)
-				{
-					if (psm16(f.z_psm)) z = std::min(z, 0xffffu);
-					else if (psm24(f.z_psm)) z = std::min(z, 0xffffffu);
-				}
-
-				if (z_test && !(z_greater ? z > r.depth : z >= r.depth))
-					continue;
-
-				// Combined colour (ubershader.comp:1260-1339).
-				float col[4];
-				for (uint32_t ch = 0; ch < 4; ch++)
-				{
-					float c0 = unorm(s.rgba[0], ch), c1 = unorm(s.rgba[1], ch), c2 = unorm(s.rgba[2], ch);
-					float d1 = c1 - c0, d2 = c2 - c0;
-					float e1 = d1 * fi, e2 = d2 * fj;
-					float sum = c0 + e1;
-					sum = sum + e2;
-					col[ch] = std::floor(255.0f * sum + 1.0f / 256.0f);
-				}
-				float comb[4] = { col[0], col[1], col[2], col[3] };
-				if (p.combiner & C_TME)
-				{
-					float sq[3];
-					for (int k = 0; k < 3; k++)
-					{
-						float e1 = s.stqf1[k] * fi, e2 = s.stqf2[k] * fj;
-						float sum = s.stqf0[k] + e1;
-						sq[k] = sum + e2;
-					}
-					float u = sq[0], vv = sq[1];
-					if (persp && sq[2] != 1.0f)
-					{
-						u = u / sq[2];
-						vv = vv / sq[2];
-					}
-					// NClamp: a NaN coordinate clamps to the lower bound.
-					u = std::fmin(std::fmax(u, -2047.0f), 2047.0f);
-					vv = std::fmin(std::fmax(vv, -2047.0f), 2047.0f);
-					float tx[4];
-					sample_texture(rp_textures[p.tex_slot], p.tex, u, vv, sq[2], persp, tx);
-					const bool tcc = p.combiner & C_TCC;
-					switch (p.combiner & 3u)
-					{
-					case 1: // DECAL
-						for (int k = 0; k < 4; k++) comb[k] = tx[k];
-						if (!tcc) comb[3] = col[3];
-						break;
-					case 0: // MODULATE
-						for (int k = 0; k < 4; k++)
-							comb[k] = std::min(std::max(std::floor(col[k] * tx[k] * (1.0f / 128.0f)), 0.0f), 255.0f);
-						if (!tcc) comb[3] = col[3];
-						break;
-					default: // HIGHLIGHT / HIGHLIGHT2
-					{
-						for (int k = 0; k < 4; k++)
-							comb[k] = std::min(std::max(std::floor(col[k] * tx[k] * (1.0f / 128.0f)), 0.0f), 255.0f);
-						for (int k = 0; k < 3; k++) comb[k] += col[3];
-						comb[3] = tcc ? ((p.combiner & 3u) == 2 ? tx[3] + col[3] : tx[3]) : col[3];
-						for (int k = 0; k < 4; k++) comb[k] = std::min(std::max(comb[k], 0.0f), 255.0f);
-						break;
-					}
-					}
-				}
-				if (p.combiner & C_FOG)
-				{
-					float e1 = s.stqf1[3] * fi, e2 = s.stqf2[3] * fj;
-					float fog = std::nearbyint((s.stqf0[3] + e1) + e2);
-					for (uint32_t k = 0; k < 3; k++)
-					{
-						float fc = unorm(p.fogcol, k) * 255.0f;
-						float d = (comb[k] - fc) * fog;
-						comb[k] = std::floor(fc + d * (1.0f / 256.0f) + 0.5f);
-					}
-				}
-
-				// shade_resolve (ubershader.comp:1464-1607)
-				float fb[4];
-				for (uint32_t k = 0; k < 4; k++)
-					fb[k] = std::nearbyint(unorm(r.color, k) * 255.0f);
-				bool write_fb = true, write_zb = true, write_alpha = true;
-				const uint32_t bm = p.blend_mode;
-				bool blending = bm & B_ABE;
-				bool keep_all = false;
-				if (bm & B_ATE)
-				{
-					if (!alpha_test((bm >> 2) & 7u, p.alpha & 0xffu, comb[3]))
-					{
-						switch ((bm >> 5) & 3u)
-						{
-						case 0: keep_all = true; break;
-						case 1: write_zb = false; break;
-						case 2: write_fb = false; write_alpha = false; break;
-						case 3:
-							if (psm32(f.fb_psm)) write_alpha = false;
-							write_zb = false;
-							break;
-						}
-					}
-				}
-				if (keep_all)
-					continue;
-				if (!psm24(f.fb_psm) && (bm & B_DATE))
-				{
-					uint32_t bit = fb[3] >= 128.0f ? 1u : 0u;
-					if (uint32_t((bm & B_DATM) != 0) != bit)
-						continue;
-				}
-				if ((bm & B_PABE) && comb[3] < 128.0f)
-					blending = false;
-				float out[4] = { comb[0], comb[1], comb[2], comb[3] };
-				if (blending)
-				{
-					float A[3] = { 0, 0, 0 }, B[3] = { 0, 0, 0 }, D[3] = { 0, 0, 0 }, C = 0.0f;
-					uint32_t am = (bm >> 11) & 3u, bmd = (bm >> 13) & 3u, cm = (bm >> 15) & 3u, dm = (bm >> 17) & 3u;
-					for (int k = 0; k < 3; k++)
-					{
-						A[k] = am == 0 ? comb[k] : am == 1 ? fb[k] : 0.0f;
-						B[k] = bmd == 0 ? comb[k] : bmd == 1 ? fb[k] : 0.0f;
-						D[k] = dm == 0 ? comb[k] : dm == 1 ? fb[k] : 0.0f;
-					}
-					if (cm == 0) C = comb[3];
-					else if (cm == 1) C = psm24(f.fb_psm) ? 128.0f : fb[3];
-					else if (cm == 2) C = float((p.alpha >> 8) & 0xffu);
-					for (int k = 0; k < 3; k++)
-						out[k] = std::floor((A[k] - B[k]) * C * (1.0f / 128.0f)) + D[k];
-				}
-				if (bm & B_DTHE)
-				{
-					int32_t px = x & 3, py = y & 3;
-					int32_t bit_index = ((py * 4 + px) * 4) & 31;
-					uint32_t dm = py < 2 ? p.dimx[0] : p.dimx[1];
-					int32_t dv = int32_t(dm << (29 - bit_index)) >> 29;
-					for (int k = 0; k < 3; k++)
-						out[k] += float(dv);
-				}
-				if (psm24(f.fb_psm))
-					write_alpha = false;
-				if (!write_alpha)
-					out[3] = fb[3];
-				if (!write_fb)
-					for (int k = 0; k < 3; k++) out[k] = fb[k];
-				float norm[4];
-				if (!(bm & B_COLCLAMP))
-				{
-					for (int k = 0; k < 4; k++)
-					{
-						float q = out[k] / 256.0f;
-						norm[k] = (q - std::floor(q)) * (256.0f / 255.0f);
-					}
-				}
-				else
-					for (int k = 0; k < 4; k++)
-						norm[k] = out[k] * (1.0f / 255.0f);
-				uint32_t packed = pack_unorm(norm);
-				packed |= uint32_t((bm & B_FBA) && write_alpha) << 31;
-
-				if (p.fbmsk != ~0u && (write_fb || write_alpha))
-				{
-					r.color = p.fbmsk ? ((packed & ~p.fbmsk) | (r.color & p.fbmsk)) : packed;
-					if (psm16(f.fb_psm))
-						r.color &= 0x80f8f8f8u;
-					r.cdirty = true;
-				}
-				if (z_write && write_zb)
-				{
-					r.depth = z;
-					r.zdirty = true;
-				}
-			}
-		}
+				shade(p, s, x, y, 0, false, [&]() -> Regs & { return load(x, y); });
 	}
 
 	// Write-back in 8x8 tile order: colour then depth per tile.

@@ -59,6 +59,8 @@ struct Setup
 	uint32_t z[3];
 	float stqf0[4], stqf1[4], stqf2[4];
 	uint32_t rgba[3];
+	bool flat = false;     // super-sampled pass: flat primitive entered into a depth bucket
+	uint32_t flat_z = 0;   // its depth (triangle_setup.comp:820-872)
 };
 
 // A decoded texture of a render pass (gs/gs_interface.cpp:1754-1781 TexInfo content).
