@@ -131,7 +131,8 @@ def kernel_profile(ctx):
     ctx.replay(False)
     kms, kn = ctx.profile_collect()
     ctx.profile_enable(False)
-    names = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode"]
+    names = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode",
+             "ssaa_heuristic", "shade_super"]
     return {names[i]: (kms[i], kn[i]) for i in range(len(names)) if kn[i]}
 
 

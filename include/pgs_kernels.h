@@ -66,13 +66,37 @@ typedef struct pgs_setup_args
 	void *prims;            /* PrimitiveAttribute[num_primitives], updated in place */
 	void *setup;            /* PrimitiveSetup[num_primitives] */
 	void *tattr;            /* TransformedAttributes[num_primitives] */
+	uint32_t *state_copy;   /* super-sampled passes: final state word of every primitive (or NULL) */
+	uint32_t *flat_buckets; /* super-sampled passes: 256-bit mask of depth buckets that hold flat
+	                           primitives (triangle_setup.comp:820-872); the heuristic clears it */
 	uint32_t num_primitives;
 	uint32_t rate_log2;      /* log2 of the super-sampling grid per axis (0 at 1x) */
 	uint32_t opaque_fbmask;  /* FBMSK every primitive of the pass shares */
-	uint32_t pad;
+	uint32_t z_bucket_shift; /* depth >> shift = bucket: 8, 16, 24 for 16, 24, 32-bit depth */
 } pgs_setup_args;
 
 int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stream);
+
+/* Single-sample heuristic of super-sampled passes: flat primitives in a depth bucket with no
+ * 3D geometry sharing their vertices are snapped to the pixel grid; flat Z-tested primitives
+ * inside the 3D depth range lose attribute snapping.
+ * Replaces single_sample_heuristic.comp dispatched by
+ * GSRenderer::dispatch_single_sample_heuristic (gs/gs_renderer.cpp:1979-2040). One 512-thread
+ * workgroup per possible bucket; buckets not in `flat_buckets` exit at once. Reads the states
+ * from `state_copy` (as setup left them) and writes updated states into `prims`. */
+typedef struct pgs_ssaa_heuristic_args
+{
+	const void *positions;   /* VertexPosition[3 * num_primitives] */
+	void *prims;             /* PrimitiveAttribute[num_primitives] */
+	const uint32_t *state_copy;
+	uint32_t *flat_buckets;
+	uint32_t num_primitives;
+	uint32_t z_shift_bucket; /* gs/gs_renderer.cpp:2008-2031 */
+	uint32_t z_shift;
+	uint32_t z_max;
+} pgs_ssaa_heuristic_args;
+
+int pgs_k_ssaa_heuristic(const pgs_ssaa_heuristic_args *args, hipStream_t stream);
 
 /* Coarse binning: for each coarse tile, the ascending list of primitives whose conservative
  * edge test overlaps the tile. Replaces binning.comp dispatched by GSRenderer::dispatch_binning
@@ -88,14 +112,19 @@ typedef struct pgs_bin_args
 	int32_t origin_x, origin_y;
 	uint32_t tile_log2;
 	uint32_t tiles_x, tiles_y;
-	uint32_t pad;
+	uint32_t super_sampled;  /* pass is super-sampled: decide per tile (binning.comp:87-125) */
+	uint32_t *tile_modes;    /* per tile: 0 = no work, 1 = single-sampled, 2 = super-sampled */
+	uint32_t pad2[2];
 } pgs_bin_args;
 
 int pgs_k_binning(const pgs_bin_args *args, hipStream_t stream);
 
-/* Shading: one wave64 per 8x8 block of samples; primitives of the block's coarse tile are
- * applied to every sample in submission order (coverage, Z, texture, combiner, fog, alpha
- * test, blend, dither, FBMSK) and the results are written back once.
+/* Shading: primitives of each block's coarse tile are applied to every sample in submission
+ * order (coverage, Z, texture, combiner, fog, alpha test, blend, dither, FBMSK) and the results
+ * are written back once per tile. Single-sampled tiles run one wave64 per 8x8 pixel block;
+ * super-sampled tiles run one wave64 per 64 samples (the samples of 64 / S pixels) and resolve
+ * in the wave. In a super-sampled context `vram` holds 2 + S planes of 4 MiB (plane 1 marks
+ * pixels with valid samples) and `tile_modes` selects which tiles each launch shades.
  * Replaces ubershader.comp dispatched by GSRenderer::dispatch_shading
  * (gs/gs_renderer.cpp:2489-2682, 2322-2416). */
 typedef struct pgs_shade_args
@@ -117,8 +146,12 @@ typedef struct pgs_shade_args
 	uint32_t fb_psm, fbp, fbw;
 	uint32_t z_psm;          /* 0xffffffff: the pass does not read or write depth */
 	uint32_t zbp;
-	uint32_t rate_log2;
-	uint32_t pad[2];
+	uint32_t rate_log2;      /* 0: single-sampled shading; else the pass's y rate (x: rate_x_log2) */
+	uint32_t rate_x_log2;
+	const uint32_t *tile_modes; /* NULL: every tile with primitives; else shade tiles whose mode
+	                               is 1 (single-sampled launch) or 2 (super-sampled launch) */
+	uint32_t has_ref_plane;  /* context is super-sampled: dirty pixels update plane 1 */
+	uint32_t pad;
 } pgs_shade_args;
 
 int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);

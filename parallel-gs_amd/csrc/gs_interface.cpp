@@ -10,6 +10,10 @@ bool GSInterface::init(const GSOptions &options)
 	options_ = options;
 	uint32_t s = uint32_t(options.super_sampling);
 	uint32_t sample_log2 = s >= 16 ? 4 : s >= 8 ? 3 : s >= 4 ? 2 : s >= 2 ? 1 : 0;
+	// Sampling grids (gs/gs_interface.cpp:65-117): 2x = 1x2, 4x = 2x2, 8x = 2x4, 16x = 4x4.
+	// The sparse 4x (1x4) and 16x (2x8) grids of the reference are not provided.
+	if ((sample_log2 == 2 || sample_log2 == 4) && !options.ordered_super_sampling)
+		return false;
 	if (!renderer_.init(sample_log2, options.upload_arena_bytes))
 		return false;
 	rp_.init(&renderer_, sample_log2);

@@ -163,8 +163,11 @@ __global__ void __launch_bounds__(64) k_triangle_setup(pgs_setup_args args)
 	if (sa == 0)
 	{
 		out.bb[0] = 0; out.bb[1] = 0; out.bb[2] = -1; out.bb[3] = -1;
+		// Degenerate primitives must not count as 3D geometry for the heuristic.
 		if (rate)
 			prim.state = state | ST_SNAP_ATTRIBUTE;
+		if (args.state_copy)
+			args.state_copy[index] = prim.state;
 		return;
 	}
 
@@ -379,10 +382,150 @@ __global__ void __launch_bounds__(64) k_triangle_setup(pgs_setup_args args)
 			per_sample = za != zb || zb != zc;
 		if (!per_sample && !line)
 			state |= ST_SNAP_ATTRIBUTE;
+		// Flat primitive: its depth bucket gets a heuristic workgroup.
+		if (!per_sample && args.flat_buckets)
+		{
+			const uint32_t bucket = min(255u, za >> args.z_bucket_shift);
+			atomicOr(&args.flat_buckets[bucket >> 5], 1u << (bucket & 31u));
+		}
 	}
 
 	prim.state = state;
 	prim.tex = tex;
+	if (args.state_copy)
+		args.state_copy[index] = state;
+}
+
+// ---------------------------------------------------------------------------------------
+// Single-sample heuristic (single_sample_heuristic.comp:148-290), one workgroup per depth
+// bucket. Masks live in LDS: 65536 bits each for 3D depths, demotion candidates and hashed
+// vertices that 3D primitives share with candidates.
+constexpr uint32_t kHeurThreads = 512;
+constexpr uint32_t kMaskWords = 65536 / 32;
+
+__device__ __forceinline__ uint32_t hash_vertex(const VertexPosition &v)
+{
+	const uint32_t P = 1103515245u;
+	uint32_t x = uint32_t(v.x), y = uint32_t(v.y), z = v.z;
+	for (int r = 0; r < 3; r++)
+	{
+		const uint32_t nx = ((x >> 8) ^ y) * P, ny = ((y >> 8) ^ z) * P, nz = ((z >> 8) ^ x) * P;
+		x = nx; y = ny; z = nz;
+	}
+	return x >> 16;
+}
+
+__global__ void __launch_bounds__(kHeurThreads) k_ssaa_heuristic(pgs_ssaa_heuristic_args a)
+{
+	const uint32_t bucket = blockIdx.x;
+	if (((a.flat_buckets[bucket >> 5] >> (bucket & 31u)) & 1u) == 0)
+		return;
+
+	__shared__ uint32_t full3d[kMaskWords], cand[kMaskWords], shared_v[kMaskWords];
+	__shared__ uint32_t any3d, min3d, max3d;
+	for (uint32_t i = threadIdx.x; i < kMaskWords; i += kHeurThreads)
+	{
+		full3d[i] = 0;
+		cand[i] = 0;
+		shared_v[i] = 0;
+	}
+	if (threadIdx.x == 0)
+	{
+		any3d = 0;
+		min3d = ~0u;
+		max3d = 0;
+	}
+	__syncthreads();
+
+	const VertexPosition *pos = static_cast<const VertexPosition *>(a.positions);
+	PrimitiveAttribute *prims = static_cast<PrimitiveAttribute *>(a.prims);
+	const uint32_t zlo = bucket << a.z_shift_bucket;
+	const uint32_t zhi = ((bucket + 1u) << a.z_shift_bucket) - 1u;
+	auto snapped = [&](uint32_t z) { return min(a.z_max, z) >> a.z_shift; };
+	auto in_range = [&](uint32_t sz) { return sz >= zlo && sz <= zhi; };
+	const uint32_t NO_FULL_3D = ST_SPRITE | ST_LINE | ST_SNAP_ATTRIBUTE;
+	const uint32_t SA = ST_SNAP_ATTRIBUTE, SR = ST_SNAP_RASTER;
+
+	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
+	{
+		const uint32_t st = a.state_copy[i];
+		if ((st & NO_FULL_3D) == 0)
+		{
+			for (uint32_t k = 0; k < 3; k++)
+			{
+				const uint32_t z = pos[3u * i + k].z;
+				atomicMin(&min3d, z);
+				atomicMax(&max3d, z);
+				const uint32_t sz = snapped(z);
+				if (in_range(sz))
+					atomicOr(&full3d[(sz - zlo) >> 5], 1u << ((sz - zlo) & 31u));
+			}
+			any3d = 1;
+		}
+		else if ((st & (SA | SR)) == SA || (st & ST_LINE))
+		{
+			const uint32_t sz = snapped(pos[3u * i].z);
+			if (in_range(sz))
+				atomicOr(&cand[(sz - zlo) >> 5], 1u << ((sz - zlo) & 31u));
+		}
+	}
+	__syncthreads();
+
+	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
+	{
+		if ((a.state_copy[i] & NO_FULL_3D) != 0)
+			continue;
+		for (uint32_t k = 0; k < 3; k++)
+		{
+			const VertexPosition v = pos[3u * i + k];
+			const uint32_t sz = snapped(v.z);
+			if (in_range(sz) && ((cand[(sz - zlo) >> 5] >> ((sz - zlo) & 31u)) & 1u))
+			{
+				const uint32_t h = hash_vertex(v);
+				atomicOr(&shared_v[h >> 5], 1u << (h & 31u));
+			}
+		}
+	}
+	__syncthreads();
+
+	auto shared_bit = [&](const VertexPosition &v) {
+		const uint32_t h = hash_vertex(v);
+		return ((shared_v[h >> 5] >> (h & 31u)) & 1u) != 0;
+	};
+	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
+	{
+		if ((a.state_copy[i] & (SA | SR)) != SA)
+			continue;
+		const VertexPosition va = pos[3u * i];
+		const uint32_t sz = snapped(va.z);
+		if (!in_range(sz) || ((full3d[(sz - zlo) >> 5] >> ((sz - zlo) & 31u)) & 1u) == 0)
+			continue;
+		if (shared_bit(va) || shared_bit(pos[3u * i + 1]) || shared_bit(pos[3u * i + 2]))
+			atomicAnd(&cand[(sz - zlo) >> 5], ~(1u << ((sz - zlo) & 31u)));
+	}
+	__syncthreads();
+
+	const bool force_single = any3d == 0;
+	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
+	{
+		const uint32_t z = pos[3u * i].z;
+		const uint32_t sz = snapped(z);
+		if (!in_range(sz))
+			continue;
+		const uint32_t st = a.state_copy[i];
+		const bool tri_cand = (st & (SA | SR)) == SA;
+		const bool line_cand = force_single && (st & ST_LINE);
+		if (!tri_cand && !line_cand)
+			continue;
+		if ((cand[(sz - zlo) >> 5] >> ((sz - zlo) & 31u)) & 1u)
+			prims[i].state = st | SR;
+		else if ((st & (ST_Z_TEST | ST_Z_WRITE)) && z > min3d && z < max3d)
+			prims[i].state = st & ~SA;
+	}
+	__syncthreads();
+	// Leave the bucket mask clear for the next pass.
+	if (threadIdx.x == 0)
+		atomicAnd(&a.flat_buckets[bucket >> 5], ~(1u << (bucket & 31u)));
 }
 
 // intersect.h:24-81: conservative test of a primitive against a pixel rectangle.
@@ -438,8 +581,11 @@ __global__ void __launch_bounds__(256) k_binning(pgs_bin_args args)
 	uint16_t *list = args.lists + size_t(tile) * args.list_stride;
 
 	__shared__ uint32_t wave_counts[4];
+	__shared__ uint32_t wave_flags[4];
 	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 	uint32_t written = 0;
+	// binning.comp:87-125: flags of the primitives binned to this tile.
+	bool req = false, req_zw = false, has_zw = false;
 
 	for (uint32_t base = 0; base < args.num_primitives; base += 256u)
 	{
@@ -450,6 +596,15 @@ __global__ void __launch_bounds__(256) k_binning(pgs_bin_args args)
 			const uint32_t st = prims[i].state;
 			const bool ms_tri = (st & ST_MULTISAMPLE) && !(st & ST_LINE);
 			hit = intersects_tile(setup[i], lox, loy, hix, hiy, ms_tri);
+			if (hit)
+			{
+				if ((st & (ST_SNAP_RASTER | ST_Z_TEST | ST_OPAQUE)) != (ST_SNAP_RASTER | ST_OPAQUE))
+					req_zw = true;
+				if ((st & (ST_SNAP_RASTER | ST_OPAQUE)) != (ST_SNAP_RASTER | ST_OPAQUE))
+					req = true;
+				if (st & ST_Z_WRITE)
+					has_zw = true;
+			}
 		}
 		const uint64_t ballot = __ballot(hit);
 		if (lane == 0)
@@ -467,6 +622,19 @@ __global__ void __launch_bounds__(256) k_binning(pgs_bin_args args)
 		written += total;
 		__syncthreads();
 	}
+	if (args.tile_modes)
+	{
+		const uint32_t f = (__ballot(req) ? 1u : 0u) | (__ballot(req_zw) ? 2u : 0u) | (__ballot(has_zw) ? 4u : 0u);
+		if (lane == 0)
+			wave_flags[wave] = f;
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			const uint32_t all = wave_flags[0] | wave_flags[1] | wave_flags[2] | wave_flags[3];
+			const bool super = args.super_sampled && ((all & 1u) || ((all & 2u) && (all & 4u)));
+			args.tile_modes[tile] = written == 0 ? 0u : (super ? 2u : 1u);
+		}
+	}
 	if (threadIdx.x == 0)
 		args.counts[tile] = written;
 }
@@ -483,6 +651,19 @@ extern "C" int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stre
 	if (args->num_primitives == 0)
 		return 0;
 	hipLaunchKernelGGL(pgs::k_triangle_setup, dim3((args->num_primitives + 63u) / 64u), dim3(64), 0, stream, *args);
+	hipError_t err = hipGetLastError();
+	return err == hipSuccess ? 0 : -int(err);
+}
+
+extern "C" int pgs_k_ssaa_heuristic(const pgs_ssaa_heuristic_args *args, hipStream_t stream)
+{
+	if (!args || !args->positions || !args->prims || !args->state_copy || !args->flat_buckets)
+		return -int(hipErrorInvalidValue);
+	if (args->num_primitives == 0)
+		return 0;
+	if (args->z_shift_bucket > 16 || args->z_shift > 8)
+		return -int(hipErrorInvalidValue);
+	hipLaunchKernelGGL(pgs::k_ssaa_heuristic, dim3(256), dim3(pgs::kHeurThreads), 0, stream, *args);
 	hipError_t err = hipGetLastError();
 	return err == hipSuccess ? 0 : -int(err);
 }

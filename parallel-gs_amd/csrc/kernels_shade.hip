@@ -166,15 +166,23 @@ struct PassInfo
 	bool fb16, fb24, fb32, z16, z24, zsens;
 };
 
-// Applies one primitive to this lane's pixel (ubershader.comp shade_early + shade_resolve).
+// Applies one primitive to this lane's sample (ubershader.comp shade_early + shade_resolve).
+// (hx, hy) is the sample's coordinate on the grid of 2^rate samples per pixel axis; the pixel
+// is (hx >> rate, hy >> rate). Single-sampled shading passes rate 0 and super_sampled false.
 __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassInfo &pi, uint32_t prim_index,
-                                            int32_t px, int32_t py, uint32_t &color, uint32_t &depth, bool &cdirty,
-                                            bool &zdirty)
+                                            int32_t hx, int32_t hy, uint32_t rate, bool super_sampled,
+                                            uint32_t &color, uint32_t &depth, bool &cdirty, bool &zdirty)
 {
 	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
 	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
 	const PrimitiveSetup &s = setups[prim_index];
 	const uint32_t st = prims[prim_index].state;
+
+	// Snapped primitives rasterize at the pixel's first sample (ubershader.comp:815-825).
+	const int32_t snap_mask = -(1 << rate);
+	const bool snap_raster = super_sampled && (st & ST_SNAP_RASTER);
+	const int32_t rx = snap_raster ? (hx & snap_mask) : hx, ry = snap_raster ? (hy & snap_mask) : hy;
+	const int32_t px = hx >> rate, py = hy >> rate;
 
 	// Coverage (intersect.h:156-244).
 	if (!(px >= s.bb[0] && py >= s.bb[1] && px <= s.bb[2] && py <= s.bb[3]))
@@ -182,12 +190,12 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	const bool para = (s.a[0] & 1) != 0;
 	const int32_t offb = int32_t((uint32_t(s.a[0]) >> 2) & 3u) - 1;
 	const int32_t offc = int32_t((uint32_t(s.a[0]) >> 4) & 3u) - 1;
-	const int32_t sx = int32_t(uint32_t(px) << 3), sy = int32_t(uint32_t(py) << 3);
+	const int32_t sx = int32_t(uint32_t(rx) << (3u - rate)), sy = int32_t(uint32_t(ry) << (3u - rate));
 	int32_t A = idot3(s.a[0] >> 6, s.a[1], s.a[2], sx, sy);
 	int32_t B = idot3(s.b[0], s.b[1], s.b[2], sx, sy);
 	int32_t C = idot3(s.c[0], s.c[1], s.c[2], sx, sy);
-	const float fi = __fadd_rn(__fmul_rn(float(B), s.inv_area), s.error_i);
-	const float fj = __fadd_rn(__fmul_rn(float(C), s.inv_area), s.error_j);
+	float fi = __fadd_rn(__fmul_rn(float(B), s.inv_area), s.error_i);
+	float fj = __fadd_rn(__fmul_rn(float(C), s.inv_area), s.error_j);
 	if (para && A < 0)
 	{
 		B = wadd(B, wadd(A, offb));
@@ -198,7 +206,9 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		return;
 	if (st & (ST_SCANMSK_EVEN | ST_SCANMSK_ODD))
 	{
-		if ((py & 1) == 0 ? (st & ST_SCANMSK_EVEN) : (st & ST_SCANMSK_ODD))
+		// Sprites mask per pixel row, everything else per sample row.
+		const int32_t sy_row = (st & ST_SPRITE) ? py : hy;
+		if ((sy_row & 1) == 0 ? (st & ST_SCANMSK_EVEN) : (st & ST_SCANMSK_ODD))
 			return;
 	}
 
@@ -219,6 +229,15 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	const bool z_write = pi.zsens && (st & ST_Z_WRITE);
 	if (z_test && !((st & ST_Z_GREATER) ? z > depth : z >= depth))
 		return;
+	// Attribute snapping without raster snapping: attributes at the pixel's first sample,
+	// depth per sample (ubershader.comp:888-898).
+	if (super_sampled && (st & (ST_SNAP_RASTER | ST_SNAP_ATTRIBUTE)) == ST_SNAP_ATTRIBUTE)
+	{
+		const int32_t bx_ = int32_t(uint32_t(hx & snap_mask) << (3u - rate));
+		const int32_t by_ = int32_t(uint32_t(hy & snap_mask) << (3u - rate));
+		fi = __fadd_rn(__fmul_rn(float(idot3(s.b[0], s.b[1], s.b[2], bx_, by_)), s.inv_area), s.error_i);
+		fj = __fadd_rn(__fmul_rn(float(idot3(s.c[0], s.c[1], s.c[2], bx_, by_)), s.inv_area), s.error_j);
+	}
 
 	const PrimitiveAttribute &pa = prims[prim_index];
 	const StateVector sv = static_cast<const StateVector *>(a.states)[st & STATE_INDEX_MASK];
@@ -401,6 +420,8 @@ __global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a)
 	const int32_t px = blox + int32_t(lane & 7u), py = bloy + int32_t(lane >> 3);
 	const uint32_t shift = a.tile_log2 - 3u;
 	const uint32_t tile = (by >> shift) * a.tiles_x + (bx >> shift);
+	if (a.tile_modes && uni(a.tile_modes[tile]) != 1u)
+		return;
 	const uint32_t count = uni(a.counts[tile]);
 	if (count == 0)
 		return;
@@ -457,20 +478,35 @@ __global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a)
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
 			const uint32_t prim = uni(uint32_t(__shfl(int(idx), int(bit))));
-			shade_pixel(a, pi, prim, px, py, color, depth, cdirty, zdirty);
+			shade_pixel(a, pi, prim, px, py, 0u, false, color, depth, cdirty, zdirty);
 		}
 	}
 
-	// end_tile
+	// end_tile. In a super-sampled context the reference plane marks the pixel single-sampled.
 	if (cdirty)
 	{
 		if (pi.fb16)
+		{
+			if (a.has_ref_plane)
+				*reinterpret_cast<uint16_t *>(a.vram + kVramSize + caddr) = 0;
 			*reinterpret_cast<uint16_t *>(a.vram + caddr) = uint16_t(rgba32_to_rgba16(color));
+		}
 		else
+		{
+			if (a.has_ref_plane)
+				*reinterpret_cast<uint32_t *>(a.vram + kVramSize + caddr) = 0;
 			*reinterpret_cast<uint32_t *>(a.vram + caddr) = color;
+		}
 	}
 	if (zdirty)
 	{
+		if (a.has_ref_plane)
+		{
+			if (pi.z16)
+				*reinterpret_cast<uint16_t *>(a.vram + kVramSize + zaddr) = 0;
+			else
+				*reinterpret_cast<uint32_t *>(a.vram + kVramSize + zaddr) = 0;
+		}
 		if (pi.z16)
 			*reinterpret_cast<uint16_t *>(a.vram + zaddr) = uint16_t(depth);
 		else if (pi.z24)
@@ -481,6 +517,217 @@ __global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a)
 		}
 		else
 			*reinterpret_cast<uint32_t *>(a.vram + zaddr) = depth;
+	}
+}
+
+// Hi-res offset of sample s inside its pixel: the lane-bit layout of ubershader.comp:133-385
+// (sample index = lane & (S - 1)), written out per sampling grid.
+template <uint32_t RX, uint32_t RY>
+__device__ __forceinline__ void sample_offset(uint32_t s, int32_t &dx, int32_t &dy)
+{
+	if (RX == 0 && RY == 1) { dx = int32_t(s); dy = int32_t(s); }
+	else if (RX == 1 && RY == 1) { dx = int32_t(s >> 1); dy = int32_t(s & 1u); }
+	else if (RX == 1 && RY == 2)
+	{
+		dx = int32_t(2u * ((s >> 1) & 1u) + (s & 1u));
+		dy = int32_t((s & 1u) + 2u * ((s >> 2) & 1u));
+	}
+	else
+	{
+		dx = int32_t(((s >> 1) & 1u) + 2u * ((s >> 3) & 1u));
+		dy = int32_t((s & 1u) + 2u * ((s >> 2) & 1u));
+	}
+}
+
+// Super-sampled tiles: one wave64 holds the S samples of 64 / S pixels (lane = pixel * S + s),
+// so the end-of-tile resolve is a butterfly over S consecutive lanes, summing in the
+// reference's xor order (8, 4, 2, 1) and taking the minimum depth (ubershader.comp:581-780).
+template <uint32_t RX, uint32_t RY>
+__global__ void __launch_bounds__(256) k_shade_ssaa(pgs_shade_args a)
+{
+	constexpr uint32_t SL = RX + RY, S = 1u << SL;
+	constexpr uint32_t RW = SL <= 2 ? (SL == 1 ? 8u : 4u) : (SL == 3 ? 4u : 2u); // wave rectangle
+	constexpr uint32_t RH = (64u >> SL) / RW;
+	constexpr uint32_t SUB_X = 8u / RW;
+
+	const uint32_t nwg = gridDim.x;
+	const uint32_t per_xcd = nwg / 8u;
+	const uint32_t logical = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t wave_global = logical * 4u + (threadIdx.x >> 6);
+	const uint32_t block = wave_global / S, sub = wave_global % S;
+	const uint32_t bx = block % a.blocks_x, by = block / a.blocks_x;
+	if (by >= a.blocks_y)
+		return;
+	const uint32_t shift = a.tile_log2 - 3u;
+	const uint32_t tile = (by >> shift) * a.tiles_x + (bx >> shift);
+	if (uni(a.tile_modes[tile]) != 2u)
+		return;
+	const uint32_t count = uni(a.counts[tile]);
+	if (count == 0)
+		return;
+	const uint16_t *list = a.lists + size_t(tile) * a.list_stride;
+
+	const int32_t rox = a.origin_x + int32_t(bx * 8u + (sub % SUB_X) * RW);
+	const int32_t roy = a.origin_y + int32_t(by * 8u + (sub / SUB_X) * RH);
+	const uint32_t smp = lane & (S - 1u), pix = lane >> SL;
+	const int32_t px = rox + int32_t(pix % RW), py = roy + int32_t(pix / RW);
+	int32_t dx, dy;
+	sample_offset<RX, RY>(smp, dx, dy);
+	const int32_t hx = (px << RY) + dx, hy = (py << RY) + dy;
+
+	PassInfo pi;
+	pi.fb_psm = a.fb_psm;
+	pi.z_psm = a.z_psm;
+	pi.fb16 = psm_is_16(a.fb_psm);
+	pi.fb24 = a.fb_psm == PSMCT24 || a.fb_psm == PSMZ24;
+	pi.fb32 = a.fb_psm == PSMCT32 || a.fb_psm == PSMZ32;
+	pi.zsens = a.z_psm != 0xffffffffu;
+	pi.z16 = pi.zsens && psm_is_16(a.z_psm);
+	pi.z24 = pi.zsens && (a.z_psm == PSMCT24 || a.z_psm == PSMZ24);
+
+	// begin_tile: samples come from their planes where the reference plane marks the pixel.
+	uint8_t *v = a.vram;
+	const size_t P = kVramSize;
+	const uint32_t caddr = swizzle_addr(uint32_t(px), uint32_t(py), a.fbp * 32u, a.fbw, a.fb_psm);
+	uint32_t color;
+	if (pi.fb16)
+	{
+		uint32_t c = *reinterpret_cast<const uint16_t *>(v + caddr);
+		if (*reinterpret_cast<const uint16_t *>(v + P + caddr) == 0xffffu)
+			c = *reinterpret_cast<const uint16_t *>(v + (2u + smp) * P + caddr);
+		color = rgba16_to_rgba32(c);
+	}
+	else
+	{
+		color = *reinterpret_cast<const uint32_t *>(v + caddr);
+		const uint32_t ref = *reinterpret_cast<const uint32_t *>(v + P + caddr);
+		const uint32_t mask = pi.fb24 ? 0xffffffu : ~0u;
+		if ((ref & mask) == mask)
+		{
+			const uint32_t sc = *reinterpret_cast<const uint32_t *>(v + (2u + smp) * P + caddr);
+			// A 24-bit frame keeps the single-sampled alpha unless the whole word is marked.
+			color = (pi.fb24 && ref != ~0u) ? ((color & 0xff000000u) | (sc & 0xffffffu)) : sc;
+		}
+	}
+	uint32_t zaddr = 0, depth = 0;
+	if (pi.zsens)
+	{
+		zaddr = swizzle_addr(uint32_t(px), uint32_t(py), a.zbp * 32u, a.fbw, a.z_psm);
+		if (pi.z16)
+		{
+			depth = *reinterpret_cast<const uint16_t *>(v + zaddr);
+			if (*reinterpret_cast<const uint16_t *>(v + P + zaddr) == 0xffffu)
+				depth = *reinterpret_cast<const uint16_t *>(v + (2u + smp) * P + zaddr);
+		}
+		else
+		{
+			const uint32_t mask = pi.z24 ? 0xffffffu : ~0u;
+			depth = *reinterpret_cast<const uint32_t *>(v + zaddr) & mask;
+			if ((*reinterpret_cast<const uint32_t *>(v + P + zaddr) & mask) == mask)
+				depth = *reinterpret_cast<const uint32_t *>(v + (2u + smp) * P + zaddr) & mask;
+		}
+	}
+	bool cdirty = false, zdirty = false;
+
+	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
+	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
+	const int32_t rhx = rox + int32_t(RW) - 1, rhy = roy + int32_t(RH) - 1;
+	for (uint32_t base = 0; base < count; base += 64u)
+	{
+		const uint32_t li = base + lane;
+		uint32_t idx = 0;
+		bool hit = false;
+		if (li < count)
+		{
+			idx = list[li];
+			const uint32_t st = prims[idx].state;
+			hit = intersects_block(setups[idx], rox, roy, rhx, rhy, (st & ST_MULTISAMPLE) && !(st & ST_LINE));
+		}
+		uint64_t ballot = __ballot(hit);
+		while (ballot)
+		{
+			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
+			ballot &= ballot - 1u;
+			const uint32_t prim = uni(uint32_t(__shfl(int(idx), int(bit))));
+			shade_pixel(a, pi, prim, hx, hy, RY, true, color, depth, cdirty, zdirty);
+		}
+	}
+
+	// end_tile
+	const uint32_t group = lane & ~(S - 1u);
+	const uint64_t gmask = ((1ull << S) - 1ull) << group;
+	const uint32_t first_color = uint32_t(__shfl(int(color), int(group)));
+	const uint32_t first_depth = uint32_t(__shfl(int(depth), int(group)));
+	const bool req_c = (__ballot(color != first_color) & gmask) != 0;
+	const bool req_d = pi.zsens && (__ballot(depth != first_depth) & gmask) != 0;
+	const bool cd = (__ballot(cdirty) & gmask) != 0;
+	const bool zd = pi.zsens && (__ballot(zdirty) & gmask) != 0;
+
+	float f[4];
+	for (uint32_t k = 0; k < 4; k++)
+		f[k] = unorm8(color, k);
+	for (uint32_t m = S >> 1; m >= 1u; m >>= 1)
+		for (uint32_t k = 0; k < 4; k++)
+			f[k] = __fadd_rn(f[k], __shfl_xor(f[k], int(m)));
+	for (uint32_t k = 0; k < 4; k++)
+		f[k] = __fdiv_rn(f[k], float(S));
+	const uint32_t resolved = pack_unorm(f[0], f[1], f[2], f[3]);
+	uint32_t dmin = depth;
+	for (uint32_t m = S >> 1; m >= 1u; m >>= 1)
+		dmin = min(dmin, uint32_t(__shfl_xor(int(dmin), int(m))));
+
+	if (req_c && cd)
+	{
+		if (pi.fb16)
+			*reinterpret_cast<uint16_t *>(v + (2u + smp) * P + caddr) = uint16_t(rgba32_to_rgba16(color));
+		else
+			*reinterpret_cast<uint32_t *>(v + (2u + smp) * P + caddr) = color;
+	}
+	if (req_d && zd)
+	{
+		if (pi.z16)
+			*reinterpret_cast<uint16_t *>(v + (2u + smp) * P + zaddr) = uint16_t(depth);
+		else
+			*reinterpret_cast<uint32_t *>(v + (2u + smp) * P + zaddr) = depth;
+	}
+	if (smp == 0)
+	{
+		const uint32_t out_color = req_c ? resolved : first_color;
+		const uint32_t out_depth = req_d ? dmin : first_depth;
+		if (cd)
+		{
+			if (pi.fb16)
+			{
+				*reinterpret_cast<uint16_t *>(v + P + caddr) = req_c ? 0xffffu : 0u;
+				*reinterpret_cast<uint16_t *>(v + caddr) = uint16_t(rgba32_to_rgba16(out_color));
+			}
+			else
+			{
+				*reinterpret_cast<uint32_t *>(v + P + caddr) = req_c ? ~0u : 0u;
+				*reinterpret_cast<uint32_t *>(v + caddr) = out_color;
+			}
+		}
+		if (zd)
+		{
+			if (pi.z16)
+			{
+				*reinterpret_cast<uint16_t *>(v + P + zaddr) = req_d ? 0xffffu : 0u;
+				*reinterpret_cast<uint16_t *>(v + zaddr) = uint16_t(out_depth);
+			}
+			else
+			{
+				*reinterpret_cast<uint32_t *>(v + P + zaddr) = req_d ? (pi.z24 ? 0xffffffu : ~0u) : 0u;
+				if (pi.z24)
+				{
+					v[zaddr] = uint8_t(out_depth);
+					v[zaddr + 1] = uint8_t(out_depth >> 8);
+					v[zaddr + 2] = uint8_t(out_depth >> 16);
+				}
+				else
+					*reinterpret_cast<uint32_t *>(v + zaddr) = out_depth;
+			}
+		}
 	}
 }
 }
@@ -496,10 +743,29 @@ extern "C" int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream)
 	const uint32_t quads = ((args->blocks_x + 1u) / 2u) * ((args->blocks_y + 1u) / 2u);
 	if (quads == 0)
 		return 0;
-	if (args->rate_log2 != 0)
-		return -int(hipErrorNotSupported);
-	const uint32_t groups = (quads + 7u) & ~7u;
-	hipLaunchKernelGGL(pgs::k_shade_1x, dim3(groups), dim3(256), 0, stream, *args);
+	if (args->rate_log2 == 0)
+	{
+		const uint32_t groups = (quads + 7u) & ~7u;
+		hipLaunchKernelGGL(pgs::k_shade_1x, dim3(groups), dim3(256), 0, stream, *args);
+	}
+	else
+	{
+		if (!args->tile_modes)
+			return -int(hipErrorInvalidValue);
+		const uint32_t rx = args->rate_x_log2, ry = args->rate_log2;
+		const uint64_t waves = uint64_t(args->blocks_x) * args->blocks_y << (rx + ry);
+		const uint32_t groups = uint32_t(((waves + 3u) / 4u + 7u) & ~uint64_t(7));
+		if (rx == 0 && ry == 1)
+			hipLaunchKernelGGL((pgs::k_shade_ssaa<0, 1>), dim3(groups), dim3(256), 0, stream, *args);
+		else if (rx == 1 && ry == 1)
+			hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 1>), dim3(groups), dim3(256), 0, stream, *args);
+		else if (rx == 1 && ry == 2)
+			hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 2>), dim3(groups), dim3(256), 0, stream, *args);
+		else if (rx == 2 && ry == 2)
+			hipLaunchKernelGGL((pgs::k_shade_ssaa<2, 2>), dim3(groups), dim3(256), 0, stream, *args);
+		else
+			return -int(hipErrorNotSupported);
+	}
 	hipError_t err = hipGetLastError();
 	return err == hipSuccess ? 0 : -int(err);
 }

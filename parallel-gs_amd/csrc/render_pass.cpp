@@ -460,6 +460,9 @@ void RenderPassBuilder::flush()
 		memcpy(d.bb, bb_, sizeof(bb_));
 		d.opaque_fbmask = ~color_write_mask_;
 		d.rate_log2 = rate_log2_;
+		d.z_sensitive = pass_z_;
+		d.z_write = pass_zwrite_;
+		d.zbuf_psm = field(zbuf_, 24, 4) | 0x30u;
 		d.decodes = decodes_.data();
 		d.tex_descs = tex_descs_.data();
 		d.num_textures = uint32_t(decodes_.size());

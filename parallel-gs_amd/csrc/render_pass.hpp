@@ -29,6 +29,9 @@ struct RenderPassDesc
 	int32_t bb[4];            // union of the primitives' scissor-clipped boxes (pixels)
 	uint32_t opaque_fbmask;
 	uint32_t rate_log2;
+	bool z_sensitive;         // some primitive tests or writes depth
+	bool z_write;             // some primitive writes depth
+	uint32_t zbuf_psm;        // ZBUF.PSM as written (0x30 | low bits), before format deduction
 	// Textures of the pass, decoded from VRAM before shading.
 	const TexDecode *decodes;
 	const TextureDesc *tex_descs;

@@ -52,7 +52,8 @@ Renderer::~Renderer()
 	for (auto e : event_pool_)
 		(void)hipEventDestroy(e);
 	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_,
-	                 (void *)clut_buf_, (void *)tex_images_ })
+	                 (void *)clut_buf_, (void *)tex_images_, (void *)state_copy_, (void *)flat_buckets_,
+	                 (void *)tile_modes_ })
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
@@ -94,6 +95,10 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes)
 		check(hipMalloc(&clut_buf_, 512u * sizeof(uint16_t)));
 		check(hipMemsetAsync(clut_buf_, 0, 512u * sizeof(uint16_t), stream_));
 		check(hipMalloc(&tex_images_, size_t(kTexImageTexels) * sizeof(uint32_t)));
+		check(hipMalloc(&state_copy_, 65536u * sizeof(uint32_t)));
+		check(hipMalloc(&flat_buckets_, 8u * sizeof(uint32_t)));
+		check(hipMemsetAsync(flat_buckets_, 0, 8u * sizeof(uint32_t), stream_));
+		check(hipMalloc(&tile_modes_, counts_cap_ * sizeof(uint32_t)));
 		{
 			float lut[256];
 			make_rcp_float_lut(lut);
@@ -246,6 +251,8 @@ void Renderer::launch(const Launch &l)
 	case LaunchKind::Shade: rc = pgs_k_shade(&l.shade, stream_); break;
 	case LaunchKind::ClutWrite: rc = pgs_k_clut_write(&l.clut, stream_); break;
 	case LaunchKind::TexDecode: rc = pgs_k_texture_decode(&l.decode, stream_); break;
+	case LaunchKind::SsaaHeuristic: rc = pgs_k_ssaa_heuristic(&l.heuristic, stream_); break;
+	case LaunchKind::ShadeSuper: rc = pgs_k_shade(&l.shade, stream_); break;
 	}
 	if (rc != 0)
 		check(hipError_t(-rc));
@@ -456,10 +463,45 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	if (n == 0 || n > 65536u)
 		return;
 
-	// Tile grid. At 1x the grid is a free choice (it cannot change a pixel); it starts at the
-	// pass bounding box rounded down to whole 8x8 VRAM blocks.
+	// Sampling rate of the pass (gs/gs_interface.cpp:212-260): channel-sliced 16-bit colour
+	// masks and colour/depth aliasing with depth writes render single-sampled.
+	const bool ssaa_context = num_planes_ > 1;
+	uint32_t rate_x = sample_log2_ / 2u, rate_y = (sample_log2_ + 1u) / 2u;
+	if (ssaa_context)
+	{
+		const bool fb_32_24 = d.fb_psm == PSMCT32 || d.fb_psm == PSMCT24 || d.fb_psm == PSMZ32 || d.fb_psm == PSMZ24;
+		const uint32_t m = ~d.opaque_fbmask & 0xf8f8f8u;
+		const bool channel_slice = !fb_32_24 && m != 0xf8f8f8u && m != 0;
+		const bool aliased = d.z_write && d.zbp == d.fbp;
+		if (channel_slice || aliased)
+			rate_x = rate_y = 0;
+	}
+	const bool super_pass = rate_y != 0;
+
+	// Tile grid. Single-sampled passes may pick any grid (it cannot change a pixel): the pass
+	// box rounded down to whole 8x8 VRAM blocks. Super-sampled passes decide super- vs
+	// single-sampled shading per coarse tile, so they use the reference's grid: origin at the
+	// pass box and the binning-cost tile size (gs/gs_interface.cpp:148-187, 214-215), computed
+	// in the same 32-bit arithmetic.
 	uint32_t tile_log2 = tile_log2_1x_;
 	int32_t ox = d.bb[0] & ~7, oy = d.bb[1] & ~7;
+	if (super_pass)
+	{
+		ox = d.bb[0];
+		oy = d.bb[1];
+		const uint32_t cost = (uint32_t((d.bb[2] - d.bb[0]) >> 3) + 1u) * (uint32_t((d.bb[3] - d.bb[1]) >> 3) + 1u) * n;
+		tile_log2 = cost < 10000u ? 3u : cost < 10000000u ? 4u : cost < 100000000u ? 5u : 6u;
+		if (tile_log2 > 3)
+			tile_log2--;
+		const uint32_t cw = uint32_t((d.bb[2] - d.bb[0]) >> tile_log2) + 1u;
+		const uint32_t ch = uint32_t((d.bb[3] - d.bb[1]) >> tile_log2) + 1u;
+		uint64_t list_bytes = uint64_t(uint32_t(cw * ch * n)) * 2u;
+		while (list_bytes > 48ull * 1024 * 1024)
+		{
+			tile_log2++;
+			list_bytes /= 4;
+		}
+	}
 	auto tiles_for = [&](uint32_t tl, uint32_t &tx, uint32_t &ty) {
 		tx = uint32_t((d.bb[2] - ox) >> tl) + 1u;
 		ty = uint32_t((d.bb[3] - oy) >> tl) + 1u;
@@ -468,6 +510,12 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	tiles_for(tile_log2, tiles_x, tiles_y);
 	while ((size_t(tiles_x) * tiles_y * n > lists_cap_ || size_t(tiles_x) * tiles_y > counts_cap_) && tile_log2 < 7)
 	{
+		if (super_pass)
+		{
+			// The reference's grid always fits: 48 MiB of lists and at most 256 x 256 tiles.
+			last_error_ = -int(hipErrorInvalidValue);
+			return;
+		}
 		tile_log2++;
 		tiles_for(tile_log2, tiles_x, tiles_y);
 	}
@@ -515,9 +563,42 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	ls.setup.setup = setup_buf_;
 	ls.setup.tattr = tattr_buf_;
 	ls.setup.num_primitives = n;
-	ls.setup.rate_log2 = d.rate_log2;
+	ls.setup.rate_log2 = rate_y;
 	ls.setup.opaque_fbmask = d.opaque_fbmask;
+	// Depth format of the single-sample heuristic (gs/gs_renderer.cpp:1918-1940, 2001-2031);
+	// 24-bit when the pass does not use depth.
+	const uint32_t depth_psm = d.z_sensitive ? d.zbuf_psm : uint32_t(PSMZ24);
+	if (super_pass)
+	{
+		ls.setup.state_copy = state_copy_;
+		ls.setup.flat_buckets = flat_buckets_;
+		ls.setup.z_bucket_shift = (depth_psm == PSMZ16 || depth_psm == PSMZ16S) ? 8u : depth_psm == PSMZ32 ? 24u : 16u;
+	}
 	launch(ls);
+
+	if (super_pass)
+	{
+		Launch lh{};
+		lh.kind = LaunchKind::SsaaHeuristic;
+		lh.heuristic.positions = arena_.dev + off_pos;
+		lh.heuristic.prims = arena_.dev + off_prim;
+		lh.heuristic.state_copy = state_copy_;
+		lh.heuristic.flat_buckets = flat_buckets_;
+		lh.heuristic.num_primitives = n;
+		if (depth_psm == PSMZ16 || depth_psm == PSMZ16S)
+		{
+			lh.heuristic.z_shift_bucket = 8; lh.heuristic.z_shift = 0; lh.heuristic.z_max = 0xffffu;
+		}
+		else if (depth_psm == PSMZ32)
+		{
+			lh.heuristic.z_shift_bucket = 16; lh.heuristic.z_shift = 8; lh.heuristic.z_max = ~0u;
+		}
+		else
+		{
+			lh.heuristic.z_shift_bucket = 16; lh.heuristic.z_shift = 0; lh.heuristic.z_max = 0xffffffu;
+		}
+		launch(lh);
+	}
 
 	Launch lb{};
 	lb.kind = LaunchKind::Binning;
@@ -532,6 +613,8 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	lb.bin.tile_log2 = tile_log2;
 	lb.bin.tiles_x = tiles_x;
 	lb.bin.tiles_y = tiles_y;
+	lb.bin.super_sampled = super_pass ? 1u : 0u;
+	lb.bin.tile_modes = super_pass ? tile_modes_ : nullptr;
 	launch(lb);
 
 	Launch lsh{};
@@ -559,8 +642,19 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	a.fbw = d.fbw;
 	a.z_psm = d.z_psm;
 	a.zbp = d.zbp;
-	a.rate_log2 = d.rate_log2;
+	a.rate_log2 = 0;
+	a.rate_x_log2 = 0;
+	a.tile_modes = super_pass ? tile_modes_ : nullptr;
+	a.has_ref_plane = ssaa_context ? 1u : 0u;
 	launch(lsh);
+	if (super_pass)
+	{
+		Launch lss = lsh;
+		lss.kind = LaunchKind::ShadeSuper;
+		lss.shade.rate_log2 = rate_y;
+		lss.shade.rate_x_log2 = rate_x;
+		launch(lss);
+	}
 
 	stats_.render_passes++;
 	stats_.primitives += n;

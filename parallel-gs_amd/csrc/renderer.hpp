@@ -98,7 +98,8 @@ private:
 
 	enum class LaunchKind : uint32_t
 	{
-		VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4, ClutWrite = 5, TexDecode = 6
+		VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4, ClutWrite = 5, TexDecode = 6,
+		SsaaHeuristic = 7, ShadeSuper = 8
 	};
 	struct Launch
 	{
@@ -112,6 +113,7 @@ private:
 			pgs_shade_args shade;
 			pgs_clut_args clut;
 			pgs_tex_decode_args decode;
+			pgs_ssaa_heuristic_args heuristic;
 		};
 	};
 
@@ -123,6 +125,9 @@ private:
 	uint32_t *dummy_buf_ = nullptr;   // bound where a pass has no textures
 	uint16_t *clut_buf_ = nullptr;    // the 512-entry CLUT
 	uint32_t *tex_images_ = nullptr;  // decoded RGBA8 textures of the current pass
+	uint32_t *state_copy_ = nullptr;  // super-sampled passes: post-setup states
+	uint32_t *flat_buckets_ = nullptr; // super-sampled passes: 256-bit bucket mask
+	uint32_t *tile_modes_ = nullptr;  // super-sampled passes: per coarse tile
 	size_t lists_cap_ = 0;            // entries
 	size_t counts_cap_ = 0;
 	uint32_t tile_log2_1x_ = 5;

@@ -282,6 +282,132 @@ const uint8_t *pgs_gen_plain(uint64_t seed, uint32_t n_tris, uint32_t width, uin
 	return publish(s, size);
 }
 
+// Textured + blended workload: `frames` frames of n_prims primitives (70% triangles, 30%
+// sprites) into a width x height PSMCT32 frame with a PSMZ24 depth buffer. Each frame uploads
+// a PSMT8 256x256 texture with uniform indices and a 256-entry PSMCT32 CLUT (CSM1, CLD=1),
+// then draws with bilinear MODULATE (TCC=1), blend (Cs - Cd) * As >> 7 + Cd with vertex alpha
+// uniform in 0..128, ZTST=GEQUAL with ZMSK=0; 20% of primitives use ST/Q with Q uniform in
+// [0.5, 2], the rest UV. Placement: a centre uniform over the window; with pareto_alpha = 0 a
+// triangle's vertices lie within +-r of it (r uniform 8..64 px) and a sprite is 8..64 px per
+// side; with pareto_alpha > 0 the primitive's area is Pareto(alpha) with a 64 px^2 minimum
+// (capped at the window) and the side / radius is its square root.
+const uint8_t *pgs_gen_textured(uint64_t seed, uint32_t n_prims, uint32_t frames, uint32_t width, uint32_t height,
+                                double pareto_alpha, size_t *size)
+{
+	Rng rng(seed);
+	Stream s;
+	const uint32_t ofs = 2048u * 16u;
+	const uint32_t fbw = (width + 63) / 64;
+	const uint32_t zbp = (fbw * 64u * height * 4u + 8191u) / 8192u;
+	const uint32_t tbp = (zbp * 2u + 8u) * 32u; // texture after the depth buffer
+	const uint32_t cbp = tbp + 32u * 8u + 64u;  // CLUT after the 64 KiB texture
+	s.ad({ { 0x1a, 1 },
+	       { 0x4c, frame_reg(0, fbw, CT32, 0) },
+	       { 0x4e, zbuf_reg(zbp, Z24, 0) },
+	       { 0x18, uint64_t(ofs) | uint64_t(ofs) << 32 },
+	       { 0x40, scissor_reg(0, width - 1, 0, height - 1) },
+	       { 0x47, test_reg(0, 0, 0, 0, 1, 2) },
+	       { 0x42, uint64_t(0) | 1u << 2 | 0u << 4 | 1u << 6 }, // A=Cs B=Cd C=As D=Cd
+	       { 0x14, uint64_t(1) << 5 | uint64_t(1) << 6 },       // TEX1: MMAG = MMIN = LINEAR
+	       { 0x08, 0 },                                          // CLAMP: REPEAT
+	       { 0x46, 0 },
+	       { 0x45, 0 } });
+	std::vector<uint8_t> payload(256 * 256);
+	std::vector<uint8_t> clut(256 * 4);
+	const double area_cap = double(width) * double(height);
+	for (uint32_t f = 0; f < frames; f++)
+	{
+		for (size_t b = 0; b < payload.size(); b += 8)
+		{
+			uint64_t r = rng.next();
+			memcpy(payload.data() + b, &r, 8);
+		}
+		s.ad({ { 0x50, bitbltbuf(0, 0, 0, tbp, 4, T8) }, { 0x51, trxpos(0, 0, 0, 0) }, { 0x52, trxreg(256, 256) },
+		       { 0x53, 0 } });
+		s.image(payload.data(), payload.size());
+		for (size_t b = 0; b < clut.size(); b += 8)
+		{
+			uint64_t r = rng.next();
+			memcpy(clut.data() + b, &r, 8);
+		}
+		s.ad({ { 0x50, bitbltbuf(0, 0, 0, cbp, 1, CT32) }, { 0x51, trxpos(0, 0, 0, 0) }, { 0x52, trxreg(16, 16) },
+		       { 0x53, 0 } });
+		s.image(clut.data(), clut.size());
+		const uint64_t tex0 = uint64_t(tbp) | uint64_t(4) << 14 | uint64_t(T8) << 20 | uint64_t(8) << 26 |
+		                      uint64_t(8) << 30 | uint64_t(1) << 34 | uint64_t(0) << 35 | uint64_t(cbp) << 37 |
+		                      uint64_t(CT32) << 51 | uint64_t(1) << 61;
+		s.ad({ { 0x06, tex0 } });
+
+		uint64_t last_prim = ~0ull;
+		std::vector<std::pair<uint32_t, uint64_t>> w;
+		for (uint32_t i = 0; i < n_prims; i++)
+		{
+			const bool tri = rng.range(0, 9) < 7;
+			const bool stq = rng.range(0, 4) == 0;
+			const uint64_t prim = (tri ? 3u : 6u) | 0x8u | 0x10u | 0x40u | (stq ? 0u : 0x100u);
+			if (prim != last_prim)
+			{
+				w.push_back({ 0x00, prim });
+				last_prim = prim;
+			}
+			const uint32_t cx = rng.range(0, width * 16 - 1), cy = rng.range(0, height * 16 - 1);
+			uint32_t extent;
+			if (pareto_alpha > 0.0)
+			{
+				const double u = std::max(rng.unit(), 1e-12);
+				const double area = std::min(64.0 * std::pow(u, -1.0 / pareto_alpha), area_cap);
+				extent = uint32_t(std::sqrt(area) * 16.0);
+			}
+			else
+				extent = rng.range(8, 64) * 16;
+			const uint32_t nv = tri ? 3 : 2;
+			const uint32_t q_all = rng.range(0, 1);
+			const float q_const = float(0.5 + 1.5 * rng.unit());
+			for (uint32_t v = 0; v < nv; v++)
+			{
+				uint32_t x, y;
+				if (tri)
+				{
+					x = cx + rng.range(0, 2 * extent) - extent;
+					y = cy + rng.range(0, 2 * extent) - extent;
+				}
+				else
+				{
+					// Sprite: extent wide, 8 px .. extent tall.
+					x = v == 0 ? cx : cx + extent;
+					y = v == 0 ? cy : cy + rng.range(8 * 16, std::max(8u * 16u, extent));
+				}
+				const uint32_t rgb = uint32_t(rng.next()) & 0xffffffu;
+				const uint32_t a = rng.range(0, 128);
+				const uint32_t z = uint32_t(rng.next() & 0xffffff);
+				const float q = q_all ? q_const : float(0.5 + 1.5 * rng.unit());
+				uint32_t qbits;
+				memcpy(&qbits, &q, 4);
+				w.push_back({ 0x01, uint64_t(rgb | a << 24) | uint64_t(qbits) << 32 });
+				if (stq)
+				{
+					const float S = float(rng.unit() * 2.0) * q, T = float(rng.unit() * 2.0) * q;
+					uint32_t sb, tb;
+					memcpy(&sb, &S, 4);
+					memcpy(&tb, &T, 4);
+					w.push_back({ 0x02, uint64_t(sb) | uint64_t(tb) << 32 });
+				}
+				else
+					w.push_back({ 0x03, uint64_t(rng.range(0, 8191)) | uint64_t(rng.range(0, 8191)) << 16 });
+				w.push_back({ 0x05, uint64_t(ofs + x) | uint64_t(ofs + y) << 16 | uint64_t(z) << 32 });
+			}
+			if (w.size() > 200)
+			{
+				s.ad(w);
+				w.clear();
+			}
+		}
+		if (!w.empty())
+			s.ad(w);
+	}
+	return publish(s, size);
+}
+
 // Random uploads with fully arbitrary geometry (any w/h, unaligned origins, wrap-around,
 // mixed source/destination PSMs for local copies, strides narrower than the rectangle).
 // Used by the parity tests for edge cases.

@@ -70,7 +70,7 @@ EXPORTED_SYMBOLS = [
     "pgs_profile_enable", "pgs_profile_collect",
     "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
     "pgs_k_vram_copy", "pgs_k_vram_snapshot", "pgs_k_init_tables", "pgs_k_triangle_setup",
-    "pgs_k_binning", "pgs_k_shade", "pgs_k_clut_write", "pgs_k_texture_decode",
+    "pgs_k_binning", "pgs_k_shade", "pgs_k_clut_write", "pgs_k_texture_decode", "pgs_k_ssaa_heuristic",
 ]
 
 
@@ -130,6 +130,9 @@ def streams() -> ctypes.CDLL:
         S.pgs_gen_plain.restype = ctypes.c_void_p
         S.pgs_gen_plain.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                     ctypes.POINTER(ctypes.c_size_t)]
+        S.pgs_gen_textured.restype = ctypes.c_void_p
+        S.pgs_gen_textured.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_size_t)]
         S.pgs_gen_transfers_wild.restype = ctypes.c_void_p
         S.pgs_gen_transfers_wild.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)]
         _streams = S
@@ -152,6 +155,16 @@ def gen_plain(seed: int, n_tris: int = 2000, width: int = 640, height: int = 448
     """GIF stream of the plain 1x rendering workload (untextured Gouraud triangles)."""
     size = ctypes.c_size_t()
     ptr = streams().pgs_gen_plain(seed, n_tris, width, height, ctypes.byref(size))
+    return _copy_stream(ptr, size)
+
+
+def gen_textured(seed: int, n_prims: int, frames: int, width: int = 640, height: int = 448,
+                 pareto_alpha: float = 0.0) -> np.ndarray:
+    """Textured + blended stream: per frame a PSMT8 256x256 texture + PSMCT32 CLUT, n_prims
+    primitives (70% triangles, 30% sprites), bilinear MODULATE, (Cs - Cd) * As >> 7 + Cd,
+    GEQUAL depth, 20% ST/Q. pareto_alpha > 0 draws primitive areas from Pareto(alpha)."""
+    size = ctypes.c_size_t(0)
+    ptr = streams().pgs_gen_textured(seed, n_prims, frames, width, height, pareto_alpha, ctypes.byref(size))
     return _copy_stream(ptr, size)
 
 
