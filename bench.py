@@ -133,18 +133,18 @@ def kernel_profile(ctx):
     ctx.profile_enable(False)
     names = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode",
              "ssaa_heuristic", "shade_super"]
-    return {names[i]: (kms[i], kn[i]) for i in range(len(names)) if kn[i]}
+    return {names[i]: (kms[i], kn[i]) for i in range(min(len(names), len(kn))) if kn[i]}
 
 
-def bench_plain(args):
-    """Plain 1x rendering: 2,000 Gouraud triangles, 640x448 PSMCT32 + PSMZ24, GEQUAL."""
+def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames_stream=None, cpu_frames=1):
+    """Times graph replays of a recorded rendering stream and reports Mpix/s of output."""
     import torch
     import pgs
     from oracle_lib import Oracle
 
-    seed, w, h, ntri = 0x5EED0001, 640, 448, 2000
-    stream = pgs.gen_plain(seed, ntri, w, h)
-    ctx = pgs.Context()
+    w, h = 640, 448
+    log2 = {1: 0, 2: 1, 4: 2, 8: 3, 16: 4}[ssaa]
+    ctx = pgs.Context(super_sampling=ssaa)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ctx.begin_recording()
@@ -156,42 +156,73 @@ def bench_plain(args):
     prof = kernel_profile(ctx)
     ctx.close()
 
-    o = Oracle()
+    # CPU baseline: the single-threaded oracle over a bounded sample of the same workload.
+    sample = cpu_frames_stream if cpu_frames_stream is not None else stream
+    o = Oracle(log2)
     t0 = time.perf_counter()
-    o.gif_transfer(stream)
+    o.gif_transfer(sample)
     o.flush()
     cpu_s = time.perf_counter() - t0
-    covered = o.stat(0)
+    covered_cpu = o.stat(0)
 
-    pixels = w * h
+    pixels = w * h * frames
     value = pixels / (ms / 1e3) / 1e6
-    shade_ms, shade_n = prof.get("shade", (0.0, 1))
-    # Algorithmic bytes of the shade kernel per frame: colour + depth read and written once per
-    # pixel, plus the primitive setup (80 B) and attributes (64 B) read once.
-    algo_bytes = pixels * 8 * 2 + ntri * (80 + 64)
-    achieved = algo_bytes / (shade_ms / shade_n / 1e3) / 1e9
+    # Dominant kernel: the super-sampled shader where the pass is super-sampled, else the
+    # single-sampled one. Algorithmic bytes per launch: every plane of colour and depth read
+    # and written once per pixel ((1 or 2 + S) x 8 B x 2), plus setup (80 B) and attributes
+    # (64 B) read once per primitive.
+    kname = "shade_super" if "shade_super" in prof else "shade"
+    k_ms, k_n = prof[kname]
+    planes = 1 if ssaa == 1 else 2 + ssaa
+    bytes_per_launch = (w * h * planes * 8 * 2 * frames + n_prims * frames * (80 + 64)) / max(k_n, 1)
+    achieved = bytes_per_launch / (k_ms / k_n / 1e3) / 1e9
     return {
         "metric": METRIC,
         "value": round(value, 2),
         "unit": "Mpix/s",
         "ms_per_step": round(ms, 4),
         "dtype": "fp32+int",
-        "config": {"workload": "plain 1x: 2000 Gouraud triangles, 640x448 PSMCT32 + PSMZ24, ZTST=GEQUAL",
-                   "seed": seed, "frames_per_step": 1, "covered_samples_per_frame": covered,
-                   "render_passes": stats["render_passes"]},
-        "frames_per_s": round(1e3 / ms, 1),
-        "covered_samples_per_s": round(covered / (ms / 1e3), 1),
+        "config": {"workload": workload, "seed": seed, "frames_per_step": frames, "ssaa": ssaa,
+                   "render_passes": stats["render_passes"], "primitives": stats["primitives"]},
+        "frames_per_s": round(frames * 1e3 / ms, 1),
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in prof.items()},
+        "kernel_launches": {k: v[1] for k, v in prof.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_shade_1x",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "kernel": "k_shade_ssaa" if kname == "shade_super" else "k_shade_1x",
                      "note": "VALU-issue bound; byte roofline fraction is small by construction"},
-        "cpu_baseline": {"value": round(pixels / cpu_s / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
-                         "sample": "one whole frame, single-threaded CPU oracle"},
+        "cpu_baseline": {"value": round(w * h * cpu_frames / cpu_s / 1e6, 3), "unit": "Mpix/s", "cores": 1,
+                         "kind": "port",
+                         "sample": f"{cpu_frames} frame(s) of the same workload, single-threaded CPU oracle "
+                                   f"({covered_cpu} covered samples)"},
         "live_ms": round(live_s * 1e3, 3),
     }
 
 
-WORKLOADS = {"transfers": bench_transfers, "plain": bench_plain}
+def bench_plain(args):
+    """Plain 1x rendering: 2,000 Gouraud triangles, 640x448 PSMCT32 + PSMZ24, GEQUAL."""
+    import pgs
+
+    seed = 0x5EED0001
+    stream = pgs.gen_plain(seed, 2000, 640, 448)
+    return bench_render(args, stream, 1, 1, 2000,
+                        "plain 1x: 2000 Gouraud triangles, 640x448 PSMCT32 + PSMZ24, ZTST=GEQUAL", seed)
+
+
+def bench_textured(args):
+    """Textured + blended 4x SSAA: 8 frames of 10,000 primitives (70% triangles, 30% sprites)."""
+    import pgs
+
+    seed = 0x5EED0003
+    stream = pgs.gen_textured(seed, 10000, 8, 640, 448, 0.0)
+    sample = pgs.gen_textured(seed, 10000, 1, 640, 448, 0.0)
+    return bench_render(args, stream, 4, 8, 10000,
+                        "textured 4x SSAA: 8 frames x 10000 prims (70% tri / 30% sprite), PSMT8 256x256 + "
+                        "CT32 CLUT, bilinear MODULATE, (Cs-Cd)*As>>7+Cd, GEQUAL, 20% ST/Q",
+                        seed, cpu_frames_stream=sample, cpu_frames=1)
+
+
+WORKLOADS = {"transfers": bench_transfers, "plain": bench_plain, "textured": bench_textured}
 
 
 def main():
@@ -199,7 +230,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="transfers", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="textured", choices=sorted(WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported; GS contexts are independent and this build "

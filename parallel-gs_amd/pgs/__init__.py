@@ -266,7 +266,7 @@ class Context:
     def profile_enable(self, enable: bool = True):
         self._check(self._L.pgs_profile_enable(self._h, int(enable)), "profile_enable")
 
-    def profile_collect(self, kinds: int = 8):
+    def profile_collect(self, kinds: int = 16):
         ms = (ctypes.c_double * kinds)()
         n = (ctypes.c_uint64 * kinds)()
         self._check(self._L.pgs_profile_collect(self._h, ms, n, kinds), "profile_collect")
