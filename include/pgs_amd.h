@@ -106,6 +106,44 @@ uint32_t pgs_host_swizzle_addr(uint32_t x, uint32_t y, uint32_t base, uint32_t s
  * inside a 256-byte block of element k of dword `lane`. */
 void pgs_host_block_lane_pixel(uint32_t psm, uint32_t lane, uint32_t k, uint32_t *dx, uint32_t *dy);
 
+/* ---- Batched replay of many contexts (no counterpart in the reference, which renders one
+ * context per device queue; for callers that run many independent GS command streams).
+ * Every context must have finished a recording (pgs_begin_recording / gif_transfer / ... /
+ * pgs_end_recording) and must not record again while the batch exists. A replay restores each
+ * context's VRAM as it was when its recording began, then runs all recorded work with
+ * launches of the same kernel merged across contexts (each context keeps its own order). */
+typedef struct pgs_batch pgs_batch;
+
+typedef struct pgs_batch_tuning
+{
+	uint32_t tile_log2_1x;    /* coarse tile of single-sampled passes: 0 = as recorded, else 3..6 */
+	uint32_t waves_per_group; /* waves per shading workgroup: 1, 2 or 4 */
+} pgs_batch_tuning;
+
+typedef struct pgs_batch_tuning_result
+{
+	pgs_batch_tuning tuning;
+	float ms_per_replay;
+} pgs_batch_tuning_result;
+
+/* flags: PGS_BATCH_SEQUENTIAL issues every context's launches one context after another on
+ * the batch stream without merging (the single-stream baseline the merged batch is measured
+ * against). */
+#define PGS_BATCH_SEQUENTIAL 1u
+pgs_batch *pgs_batch_create(pgs_context *const *contexts, uint32_t count, uint32_t flags);
+void pgs_batch_destroy(pgs_batch *batch);
+int pgs_batch_apply_tuning(pgs_batch *batch, const pgs_batch_tuning *tuning);
+/* Times every candidate tuning over `reps` graph replays, applies the fastest and writes up to
+ * `max_results` timings (returns the number of candidates timed, or a negative error). */
+int pgs_batch_autotune(pgs_batch *batch, uint32_t reps, pgs_batch_tuning_result *results, uint32_t max_results);
+int pgs_batch_replay(pgs_batch *batch, int use_graph);
+int pgs_batch_sync(pgs_batch *batch);
+void *pgs_batch_stream(pgs_batch *batch);
+uint32_t pgs_batch_num_launches(pgs_batch *batch);
+/* One replay with HIP events around every merged launch: milliseconds and launch counts per
+ * kernel kind (same numbering as pgs_profile_collect). */
+int pgs_batch_profile(pgs_batch *batch, double *ms, uint64_t *launches, uint32_t kinds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -151,7 +151,7 @@ typedef struct pgs_shade_args
 	const uint32_t *tile_modes; /* NULL: every tile with primitives; else shade tiles whose mode
 	                               is 1 (single-sampled launch) or 2 (super-sampled launch) */
 	uint32_t has_ref_plane;  /* context is super-sampled: dirty pixels update plane 1 */
-	uint32_t pad;
+	uint32_t waves_per_group; /* waves per workgroup: 1, 2 or 4 (0 means 4) */
 } pgs_shade_args;
 
 int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);

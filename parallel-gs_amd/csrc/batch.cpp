@@ -1,0 +1,389 @@
+// Batched replay of many GS contexts; see batch.hpp.
+#include "batch.hpp"
+#include "multi_dispatch.hpp"
+#include <algorithm>
+#include <map>
+#include <stdexcept>
+#include <cstring>
+
+namespace pgs
+{
+namespace
+{
+void ok(hipError_t e)
+{
+	if (e != hipSuccess)
+		throw std::runtime_error(hipGetErrorString(e));
+}
+
+uint32_t grid_of(const Renderer::Launch &l)
+{
+	using K = Renderer::LaunchKind;
+	switch (l.kind)
+	{
+	case K::VramCopy: return pgs_grid_vram_copy(l.copy);
+	case K::VramSnapshot: return pgs_grid_vram_snapshot(l.snapshot);
+	case K::Setup: return pgs_grid_triangle_setup(l.setup);
+	case K::Binning: return pgs_grid_binning(l.bin);
+	case K::Shade: case K::ShadeSuper: return pgs_grid_shade(l.shade);
+	case K::ClutWrite: return pgs_grid_clut_write(l.clut);
+	case K::TexDecode: return pgs_grid_texture_decode(l.decode);
+	case K::SsaaHeuristic: return pgs_grid_ssaa_heuristic(l.heuristic);
+	}
+	return 0;
+}
+
+size_t args_size(Renderer::LaunchKind k)
+{
+	using K = Renderer::LaunchKind;
+	switch (k)
+	{
+	case K::VramCopy: return sizeof(pgs_vram_copy_args);
+	case K::VramSnapshot: return sizeof(pgs_vram_snapshot_args);
+	case K::Setup: return sizeof(pgs_setup_args);
+	case K::Binning: return sizeof(pgs_bin_args);
+	case K::Shade: case K::ShadeSuper: return sizeof(pgs_shade_args);
+	case K::ClutWrite: return sizeof(pgs_clut_args);
+	case K::TexDecode: return sizeof(pgs_tex_decode_args);
+	case K::SsaaHeuristic: return sizeof(pgs_ssaa_heuristic_args);
+	}
+	return 0;
+}
+
+// Launches that may share one multi-launch: same kernel, and for shading the same sampling grid.
+uint64_t key_of(const Renderer::Launch &l)
+{
+	uint64_t k = uint64_t(l.kind) << 16;
+	if (l.kind == Renderer::LaunchKind::Shade || l.kind == Renderer::LaunchKind::ShadeSuper)
+		k |= uint64_t(l.shade.rate_x_log2) << 8 | l.shade.rate_log2;
+	return k;
+}
+
+size_t align256(size_t v) { return (v + 255u) & ~size_t(255); }
+}
+
+Batch::~Batch()
+{
+	destroy_graph();
+	if (dev_)
+		(void)hipFree(dev_);
+	if (stream_)
+		(void)hipStreamDestroy(stream_);
+}
+
+void Batch::destroy_graph()
+{
+	if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+	if (graph_) (void)hipGraphDestroy(graph_);
+	graph_exec_ = nullptr;
+	graph_ = nullptr;
+}
+
+bool Batch::build(Renderer *const *contexts, uint32_t count, bool merge)
+{
+	contexts_.assign(contexts, contexts + count);
+	for (Renderer *r : contexts_)
+		r->sync();
+	if (!stream_)
+		ok(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+	merged_.clear();
+	if (!merge)
+	{
+		for (uint32_t c = 0; c < count; c++)
+		{
+			const auto &rec = contexts_[c]->recorded();
+			for (uint32_t i = 0; i < rec.size(); i++)
+			{
+				Merged m;
+				m.kind = rec[i].kind;
+				m.refs.push_back({ c, i });
+				merged_.push_back(std::move(m));
+			}
+		}
+		return apply(tuning_);
+	}
+
+	// Greedy merge: the key most contexts have next goes first; per-context order is kept.
+	std::vector<uint32_t> cursor(count, 0);
+	for (;;)
+	{
+		std::map<uint64_t, uint32_t> votes;
+		for (uint32_t c = 0; c < count; c++)
+		{
+			const auto &rec = contexts_[c]->recorded();
+			if (cursor[c] < rec.size())
+				votes[key_of(rec[cursor[c]])]++;
+		}
+		if (votes.empty())
+			break;
+		uint64_t best = votes.begin()->first;
+		uint32_t best_n = 0;
+		for (auto &v : votes)
+			if (v.second > best_n)
+			{
+				best = v.first;
+				best_n = v.second;
+			}
+		Merged m;
+		for (uint32_t c = 0; c < count; c++)
+		{
+			const auto &rec = contexts_[c]->recorded();
+			if (cursor[c] < rec.size() && key_of(rec[cursor[c]]) == best)
+			{
+				m.kind = rec[cursor[c]].kind;
+				m.refs.push_back({ c, cursor[c] });
+				cursor[c]++;
+			}
+		}
+		merged_.push_back(std::move(m));
+	}
+	return apply(tuning_);
+}
+
+bool Batch::apply(const BatchTuning &t)
+{
+	if (t.waves_per_group != 1 && t.waves_per_group != 2 && t.waves_per_group != 4)
+		return false;
+	if (t.tile_log2_1x != 0 && (t.tile_log2_1x < 3 || t.tile_log2_1x > 6))
+		return false;
+	using K = Renderer::LaunchKind;
+	std::vector<std::vector<Renderer::Launch>> out(contexts_.size());
+	for (size_t c = 0; c < contexts_.size(); c++)
+	{
+		out[c] = contexts_[c]->recorded();
+		uint32_t tl = 0, tx = 0, ty = 0;
+		bool retiled = false;
+		for (auto &l : out[c])
+		{
+			if (l.kind == K::Binning)
+			{
+				retiled = false;
+				if (t.tile_log2_1x != 0 && !l.bin.super_sampled)
+				{
+					const uint32_t old = l.bin.tile_log2;
+					const uint32_t ex = l.bin.tiles_x << old, ey = l.bin.tiles_y << old;
+					tl = t.tile_log2_1x;
+					tx = (ex + (1u << tl) - 1u) >> tl;
+					ty = (ey + (1u << tl) - 1u) >> tl;
+					if (size_t(tx) * ty * l.bin.list_stride > contexts_[c]->lists_capacity() ||
+					    size_t(tx) * ty > contexts_[c]->counts_capacity())
+						return false;
+					l.bin.tile_log2 = tl;
+					l.bin.tiles_x = tx;
+					l.bin.tiles_y = ty;
+					retiled = true;
+				}
+			}
+			else if (l.kind == K::Shade || l.kind == K::ShadeSuper)
+			{
+				l.shade.waves_per_group = t.waves_per_group;
+				if (retiled && !l.shade.tile_modes)
+				{
+					l.shade.tile_log2 = tl;
+					l.shade.tiles_x = tx;
+					l.shade.tiles_y = ty;
+				}
+			}
+		}
+	}
+	launches_ = std::move(out);
+	tuning_ = t;
+	return upload();
+}
+
+bool Batch::upload()
+{
+	// Layout: per merged launch, its argument blocks then its workgroup prefix.
+	size_t bytes = 0;
+	for (auto &m : merged_)
+	{
+		m.args_offset = bytes;
+		bytes = align256(bytes + args_size(m.kind) * m.refs.size());
+		m.prefix_offset = bytes;
+		bytes = align256(bytes + sizeof(uint32_t) * (m.refs.size() + 1u));
+	}
+	std::vector<uint8_t> host(bytes);
+	for (auto &m : merged_)
+	{
+		const size_t sz = args_size(m.kind);
+		uint32_t *prefix = reinterpret_cast<uint32_t *>(host.data() + m.prefix_offset);
+		uint32_t total = 0;
+		for (size_t i = 0; i < m.refs.size(); i++)
+		{
+			const Renderer::Launch &l = launches_[m.refs[i].context][m.refs[i].index];
+			memcpy(host.data() + m.args_offset + i * sz, &l.copy, sz);
+			prefix[i] = total;
+			total += grid_of(l);
+		}
+		prefix[m.refs.size()] = total;
+		m.total = total;
+		if (m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper)
+		{
+			const Renderer::Launch &l0 = launches_[m.refs[0].context][m.refs[0].index];
+			m.rate_x = l0.shade.rate_x_log2;
+			m.rate_y = l0.shade.rate_log2;
+		}
+	}
+	if (bytes > dev_bytes_)
+	{
+		if (dev_)
+			ok(hipFree(dev_));
+		ok(hipMalloc(&dev_, bytes));
+		dev_bytes_ = bytes;
+	}
+	ok(hipMemcpy(dev_, host.data(), bytes, hipMemcpyHostToDevice));
+	destroy_graph();
+	return true;
+}
+
+void Batch::launch(const Merged &m)
+{
+	if (m.total == 0)
+		return;
+	using K = Renderer::LaunchKind;
+	const void *args = dev_ + m.args_offset;
+	const uint32_t *prefix = reinterpret_cast<const uint32_t *>(dev_ + m.prefix_offset);
+	const uint32_t n = uint32_t(m.refs.size());
+	int rc = 0;
+	switch (m.kind)
+	{
+	case K::VramCopy: rc = pgs_multi_vram_copy(static_cast<const pgs_vram_copy_args *>(args), prefix, n, m.total, stream_); break;
+	case K::VramSnapshot:
+		rc = pgs_multi_vram_snapshot(static_cast<const pgs_vram_snapshot_args *>(args), prefix, n, m.total, stream_);
+		break;
+	case K::Setup: rc = pgs_multi_triangle_setup(static_cast<const pgs_setup_args *>(args), prefix, n, m.total, stream_); break;
+	case K::Binning: rc = pgs_multi_binning(static_cast<const pgs_bin_args *>(args), prefix, n, m.total, stream_); break;
+	case K::Shade: case K::ShadeSuper:
+		rc = pgs_multi_shade(static_cast<const pgs_shade_args *>(args), prefix, n, m.total, m.rate_x, m.rate_y,
+		                     tuning_.waves_per_group, stream_);
+		break;
+	case K::ClutWrite: rc = pgs_multi_clut_write(static_cast<const pgs_clut_args *>(args), prefix, n, m.total, stream_); break;
+	case K::TexDecode:
+		rc = pgs_multi_texture_decode(static_cast<const pgs_tex_decode_args *>(args), prefix, n, m.total, stream_);
+		break;
+	case K::SsaaHeuristic:
+		rc = pgs_multi_ssaa_heuristic(static_cast<const pgs_ssaa_heuristic_args *>(args), prefix, n, m.total, stream_);
+		break;
+	}
+	if (rc != 0)
+		throw std::runtime_error("multi-launch failed");
+}
+
+int Batch::replay(bool use_graph)
+{
+	try
+	{
+		auto issue = [&]() {
+			for (Renderer *r : contexts_)
+				r->restore_snapshot(stream_);
+			for (auto &m : merged_)
+				launch(m);
+		};
+		if (!use_graph)
+		{
+			issue();
+			return 0;
+		}
+		if (!graph_exec_)
+		{
+			ok(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+			issue();
+			ok(hipStreamEndCapture(stream_, &graph_));
+			ok(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+		}
+		ok(hipGraphLaunch(graph_exec_, stream_));
+		return 0;
+	}
+	catch (const std::exception &)
+	{
+		return -int(hipErrorLaunchFailure);
+	}
+}
+
+void Batch::sync()
+{
+	ok(hipStreamSynchronize(stream_));
+}
+
+// Autotuner: every candidate (tile size of single-sampled passes x waves per shading
+// workgroup) is applied and timed over `reps` graph replays after one warm-up replay; the
+// fastest is kept. Candidates whose tile lists do not fit are skipped.
+bool Batch::autotune(uint32_t reps, std::vector<TuningResult> *table)
+{
+	static const uint32_t tiles[] = { 0, 3, 4, 5 };
+	static const uint32_t waves[] = { 1, 2, 4 };
+	hipEvent_t e0, e1;
+	ok(hipEventCreate(&e0));
+	ok(hipEventCreate(&e1));
+	BatchTuning best = tuning_;
+	float best_ms = -1.0f;
+	for (uint32_t tl : tiles)
+		for (uint32_t w : waves)
+		{
+			BatchTuning t;
+			t.tile_log2_1x = tl;
+			t.waves_per_group = w;
+			if (!apply(t))
+				continue;
+			if (replay(true) != 0)
+				return false;
+			ok(hipEventRecord(e0, stream_));
+			for (uint32_t r = 0; r < reps; r++)
+				if (replay(true) != 0)
+					return false;
+			ok(hipEventRecord(e1, stream_));
+			ok(hipEventSynchronize(e1));
+			float ms = 0.0f;
+			ok(hipEventElapsedTime(&ms, e0, e1));
+			ms /= float(std::max(reps, 1u));
+			if (table)
+				table->push_back({ t, ms });
+			if (best_ms < 0.0f || ms < best_ms)
+			{
+				best_ms = ms;
+				best = t;
+			}
+		}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	return apply(best);
+}
+
+void Batch::profile(double *ms, uint64_t *launches, uint32_t kinds)
+{
+	for (uint32_t k = 0; k < kinds; k++)
+	{
+		ms[k] = 0.0;
+		launches[k] = 0;
+	}
+	for (Renderer *r : contexts_)
+		r->restore_snapshot(stream_);
+	std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> ev;
+	for (auto &m : merged_)
+	{
+		if (m.total == 0)
+			continue;
+		hipEvent_t a, b;
+		ok(hipEventCreate(&a));
+		ok(hipEventCreate(&b));
+		ok(hipEventRecord(a, stream_));
+		launch(m);
+		ok(hipEventRecord(b, stream_));
+		ev.push_back({ uint32_t(m.kind), { a, b } });
+	}
+	ok(hipStreamSynchronize(stream_));
+	for (auto &e : ev)
+	{
+		float t = 0.0f;
+		ok(hipEventElapsedTime(&t, e.second.first, e.second.second));
+		if (e.first < kinds)
+		{
+			ms[e.first] += t;
+			launches[e.first]++;
+		}
+		(void)hipEventDestroy(e.second.first);
+		(void)hipEventDestroy(e.second.second);
+	}
+}
+}

@@ -1,0 +1,81 @@
+// Batched replay of many GS contexts (no counterpart in the reference, which drives one
+// context per Vulkan device queue).
+//
+// Each context records its work once (Renderer::begin_recording / end_recording). A Batch
+// merges the recordings: it repeatedly takes the kernel kind that the most contexts have as
+// their next recorded launch and issues one multi-launch for all of them, so every context's
+// launches keep their order while 64 contexts cost about as many launches as one. Replay
+// restores each context's VRAM snapshot first, so every replay redoes the recorded work, and
+// the whole sequence can be captured into one hipGraph.
+//
+// Tuning knobs applied at build time (an autotuner times candidates): the coarse tile size of
+// single-sampled passes (free there; super-sampled passes keep the reference's grid because it
+// decides per-tile shading) and the waves per shading workgroup.
+#pragma once
+#include "renderer.hpp"
+#include <vector>
+
+namespace pgs
+{
+struct BatchTuning
+{
+	uint32_t tile_log2_1x = 0;    // 0: keep each pass's recorded tile size; else 3..6
+	uint32_t waves_per_group = 4; // 1, 2 or 4
+};
+
+struct TuningResult
+{
+	BatchTuning tuning;
+	float ms; // per replay
+};
+
+class Batch
+{
+public:
+	Batch() = default;
+	~Batch();
+	Batch(const Batch &) = delete;
+	Batch &operator=(const Batch &) = delete;
+
+	// merge = false keeps every launch separate, context after context (the single-stream baseline).
+	bool build(Renderer *const *contexts, uint32_t count, bool merge = true);
+	// Rewrites the launch arguments for a tuning (no recording needed). Returns false if a
+	// context's tile lists would not fit.
+	bool apply(const BatchTuning &t);
+	int replay(bool use_graph);
+	bool autotune(uint32_t reps, std::vector<TuningResult> *table);
+	// One non-graph replay with HIP events around every merged launch: summed milliseconds and
+	// launch counts per kernel kind (Renderer::LaunchKind numbering).
+	void profile(double *ms, uint64_t *launches, uint32_t kinds);
+	void sync();
+	hipStream_t stream() const { return stream_; }
+	uint32_t num_launches() const { return uint32_t(merged_.size()); }
+	uint32_t num_contexts() const { return uint32_t(contexts_.size()); }
+	const BatchTuning &tuning() const { return tuning_; }
+
+private:
+	struct Ref { uint32_t context, index; };
+	struct Merged
+	{
+		Renderer::LaunchKind kind;
+		uint32_t rate_x = 0, rate_y = 0;
+		std::vector<Ref> refs;
+		size_t args_offset = 0, prefix_offset = 0;
+		uint32_t total = 0;
+	};
+
+	std::vector<Renderer *> contexts_;
+	std::vector<std::vector<Renderer::Launch>> launches_; // per context, after tuning
+	std::vector<Merged> merged_;
+	BatchTuning tuning_;
+	hipStream_t stream_ = nullptr;
+	uint8_t *dev_ = nullptr;       // argument blocks + workgroup prefixes of every merged launch
+	size_t dev_bytes_ = 0;
+	hipGraph_t graph_ = nullptr;
+	hipGraphExec_t graph_exec_ = nullptr;
+
+	bool upload();
+	void destroy_graph();
+	void launch(const Merged &m);
+};
+}

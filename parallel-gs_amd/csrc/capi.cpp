@@ -2,17 +2,41 @@
 // let a C++ exception cross the boundary.
 #include "pgs_amd.h"
 #include "gs_interface.hpp"
+#include "batch.hpp"
+#include <vector>
 #include <exception>
 #include <cstdio>
 #include <new>
+#include <stdexcept>
 
 struct pgs_context
 {
 	pgs::GSInterface gs;
 };
 
+struct pgs_batch
+{
+	pgs::Batch batch;
+};
+
 namespace
 {
+template <typename F>
+int batch_guard(pgs_batch *b, F &&f)
+{
+	if (!b)
+		return -1;
+	try
+	{
+		return f();
+	}
+	catch (const std::exception &e)
+	{
+		fprintf(stderr, "pgs: %s\n", e.what());
+		return -1;
+	}
+}
+
 template <typename F>
 int guard(pgs_context *ctx, F &&f)
 {
@@ -193,4 +217,92 @@ extern "C" uint32_t pgs_host_swizzle_addr(uint32_t x, uint32_t y, uint32_t base,
 extern "C" void pgs_host_block_lane_pixel(uint32_t psm, uint32_t lane, uint32_t k, uint32_t *dx, uint32_t *dy)
 {
 	pgs::block_lane_pixel(pgs::psm_storage_class(psm), lane, k, *dx, *dy);
+}
+
+extern "C"
+{
+pgs_batch *pgs_batch_create(pgs_context *const *contexts, uint32_t count, uint32_t flags)
+{
+	if (!contexts || count == 0)
+		return nullptr;
+	auto *b = new (std::nothrow) pgs_batch;
+	if (!b)
+		return nullptr;
+	try
+	{
+		std::vector<pgs::Renderer *> r;
+		for (uint32_t i = 0; i < count; i++)
+		{
+			if (!contexts[i])
+				throw std::runtime_error("null context");
+			r.push_back(&contexts[i]->gs.renderer());
+		}
+		if (!b->batch.build(r.data(), count, (flags & PGS_BATCH_SEQUENTIAL) == 0))
+			throw std::runtime_error("batch build failed");
+		return b;
+	}
+	catch (const std::exception &e)
+	{
+		fprintf(stderr, "pgs: %s\n", e.what());
+		delete b;
+		return nullptr;
+	}
+}
+
+void pgs_batch_destroy(pgs_batch *batch) { delete batch; }
+
+int pgs_batch_apply_tuning(pgs_batch *batch, const pgs_batch_tuning *tuning)
+{
+	return batch_guard(batch, [&] {
+		if (!tuning)
+			return -1;
+		pgs::BatchTuning t;
+		t.tile_log2_1x = tuning->tile_log2_1x;
+		t.waves_per_group = tuning->waves_per_group;
+		return batch->batch.apply(t) ? 0 : -1;
+	});
+}
+
+int pgs_batch_autotune(pgs_batch *batch, uint32_t reps, pgs_batch_tuning_result *results, uint32_t max_results)
+{
+	return batch_guard(batch, [&] {
+		std::vector<pgs::TuningResult> table;
+		if (!batch->batch.autotune(reps, &table))
+			return -1;
+		for (uint32_t i = 0; i < table.size() && i < max_results && results; i++)
+		{
+			results[i].tuning.tile_log2_1x = table[i].tuning.tile_log2_1x;
+			results[i].tuning.waves_per_group = table[i].tuning.waves_per_group;
+			results[i].ms_per_replay = table[i].ms;
+		}
+		return int(table.size());
+	});
+}
+
+int pgs_batch_replay(pgs_batch *batch, int use_graph)
+{
+	return batch_guard(batch, [&] { return batch->batch.replay(use_graph != 0); });
+}
+
+int pgs_batch_sync(pgs_batch *batch)
+{
+	return batch_guard(batch, [&] {
+		batch->batch.sync();
+		return 0;
+	});
+}
+
+void *pgs_batch_stream(pgs_batch *batch) { return batch ? static_cast<void *>(batch->batch.stream()) : nullptr; }
+
+uint32_t pgs_batch_num_launches(pgs_batch *batch) { return batch ? batch->batch.num_launches() : 0; }
+
+int pgs_batch_profile(pgs_batch *batch, double *ms, uint64_t *launches, uint32_t kinds)
+{
+	return batch_guard(batch, [&] {
+		if (!ms || !launches)
+			return -1;
+		batch->batch.profile(ms, launches, kinds);
+		return 0;
+	});
+}
 }

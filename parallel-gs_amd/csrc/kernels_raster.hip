@@ -8,6 +8,8 @@
 #include "gs_defs.hpp"
 #include "gs_raster.hpp"
 #include "pgs_kernels.h"
+#include "multi_launch.hpp"
+#include "multi_dispatch.hpp"
 
 namespace pgs
 {
@@ -81,9 +83,9 @@ template <typename T> __device__ __forceinline__ void swp(T &a, T &b) { T t = a;
 
 // ---------------------------------------------------------------------------------------
 // Triangle setup (gs/shaders/triangle_setup.comp:127-873), one thread per primitive.
-__global__ void __launch_bounds__(64) k_triangle_setup(pgs_setup_args args)
+__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t)
 {
-	const uint32_t index = blockIdx.x * 64u + threadIdx.x;
+	const uint32_t index = wg * 64u + threadIdx.x;
 	if (index >= args.num_primitives)
 		return;
 	const VertexPosition *pos = static_cast<const VertexPosition *>(args.positions) + 3u * index;
@@ -415,9 +417,9 @@ __device__ __forceinline__ uint32_t hash_vertex(const VertexPosition &v)
 	return x >> 16;
 }
 
-__global__ void __launch_bounds__(kHeurThreads) k_ssaa_heuristic(pgs_ssaa_heuristic_args a)
+__device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_args &a, uint32_t wg, uint32_t)
 {
-	const uint32_t bucket = blockIdx.x;
+	const uint32_t bucket = wg;
 	if (((a.flat_buckets[bucket >> 5] >> (bucket & 31u)) & 1u) == 0)
 		return;
 
@@ -569,9 +571,9 @@ __device__ __forceinline__ bool intersects_tile(const PrimitiveSetup &s, int32_t
 // ---------------------------------------------------------------------------------------
 // Coarse binning: one workgroup (4 waves) per coarse tile. Primitives are tested 256 at a time;
 // hits are compacted in ascending order with wave ballots and an LDS exchange of wave counts.
-__global__ void __launch_bounds__(256) k_binning(pgs_bin_args args)
+__device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t wg, uint32_t)
 {
-	const uint32_t tile = blockIdx.x;
+	const uint32_t tile = wg;
 	const uint32_t tx = tile % args.tiles_x, ty = tile / args.tiles_x;
 	const int32_t size = 1 << args.tile_log2;
 	const int32_t lox = args.origin_x + int32_t(tx) * size, loy = args.origin_y + int32_t(ty) * size;
@@ -638,6 +640,13 @@ __global__ void __launch_bounds__(256) k_binning(pgs_bin_args args)
 	if (threadIdx.x == 0)
 		args.counts[tile] = written;
 }
+
+__global__ void __launch_bounds__(64) k_triangle_setup(pgs_setup_args a) { triangle_setup_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kHeurThreads) k_ssaa_heuristic(pgs_ssaa_heuristic_args a) { ssaa_heuristic_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) k_binning(pgs_bin_args a) { binning_body(a, blockIdx.x, gridDim.x); }
+PGS_DEFINE_MULTI(64, k_triangle_setup_multi, triangle_setup_body, pgs_setup_args)
+PGS_DEFINE_MULTI(kHeurThreads, k_ssaa_heuristic_multi, ssaa_heuristic_body, pgs_ssaa_heuristic_args)
+PGS_DEFINE_MULTI(256, k_binning_multi, binning_body, pgs_bin_args)
 }
 }
 
@@ -689,4 +698,28 @@ extern "C" int pgs_k_init_tables(const uint8_t *seed, const float *lut)
 	if (e == hipSuccess)
 		e = hipMemcpyToSymbol(HIP_SYMBOL(pgs::c_rcp_float_lut), lut, 256 * sizeof(float));
 	return e == hipSuccess ? 0 : -int(e);
+}
+
+uint32_t pgs_grid_triangle_setup(const pgs_setup_args &a) { return (a.num_primitives + 63u) / 64u; }
+uint32_t pgs_grid_ssaa_heuristic(const pgs_ssaa_heuristic_args &a) { return a.num_primitives ? 256u : 0u; }
+uint32_t pgs_grid_binning(const pgs_bin_args &a) { return a.tiles_x * a.tiles_y; }
+
+int pgs_multi_triangle_setup(const pgs_setup_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
+                             hipStream_t stream)
+{
+	hipLaunchKernelGGL(pgs::k_triangle_setup_multi, dim3(total), dim3(64), 0, stream, args, prefix, n);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+}
+
+int pgs_multi_ssaa_heuristic(const pgs_ssaa_heuristic_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
+                             hipStream_t stream)
+{
+	hipLaunchKernelGGL(pgs::k_ssaa_heuristic_multi, dim3(total), dim3(pgs::kHeurThreads), 0, stream, args, prefix, n);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+}
+
+int pgs_multi_binning(const pgs_bin_args *args, const uint32_t *prefix, uint32_t n, uint32_t total, hipStream_t stream)
+{
+	hipLaunchKernelGGL(pgs::k_binning_multi, dim3(total), dim3(256), 0, stream, args, prefix, n);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }

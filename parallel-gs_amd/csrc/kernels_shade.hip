@@ -13,6 +13,8 @@
 #include "gs_defs.hpp"
 #include "gs_raster.hpp"
 #include "pgs_kernels.h"
+#include "multi_launch.hpp"
+#include "multi_dispatch.hpp"
 
 namespace pgs
 {
@@ -402,17 +404,21 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	}
 }
 
-__global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a)
+__device__ __forceinline__ uint32_t waves_of(const pgs_shade_args &a) { return a.waves_per_group ? a.waves_per_group : 4u; }
+
+__device__ __forceinline__ void shade_1x_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
 {
 	// XCD-aware remap: workgroup w runs on XCD (w % 8); give each XCD a contiguous range.
-	const uint32_t nwg = gridDim.x;
 	const uint32_t per_xcd = nwg / 8u;
-	const uint32_t logical = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
 
-	const uint32_t quads_x = (a.blocks_x + 1u) / 2u;
-	const uint32_t qx = logical % quads_x, qy = logical / quads_x;
+	// A workgroup of 1, 2 or 4 waves covers 1x1, 2x1 or 2x2 blocks of 8x8 pixels.
+	const uint32_t waves = waves_of(a);
+	const uint32_t gw = waves >= 2u ? 2u : 1u, gh = waves >= 4u ? 2u : 1u;
+	const uint32_t groups_x = (a.blocks_x + gw - 1u) / gw;
+	const uint32_t qx = logical % groups_x, qy = logical / groups_x;
 	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-	const uint32_t bx = qx * 2u + (wave & 1u), by = qy * 2u + (wave >> 1);
+	const uint32_t bx = qx * gw + (wave % gw), by = qy * gh + (wave / gw);
 	if (bx >= a.blocks_x || by >= a.blocks_y)
 		return;
 
@@ -543,18 +549,17 @@ __device__ __forceinline__ void sample_offset(uint32_t s, int32_t &dx, int32_t &
 // so the end-of-tile resolve is a butterfly over S consecutive lanes, summing in the
 // reference's xor order (8, 4, 2, 1) and taking the minimum depth (ubershader.comp:581-780).
 template <uint32_t RX, uint32_t RY>
-__global__ void __launch_bounds__(256) k_shade_ssaa(pgs_shade_args a)
+__device__ __forceinline__ void shade_ssaa_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
 {
 	constexpr uint32_t SL = RX + RY, S = 1u << SL;
 	constexpr uint32_t RW = SL <= 2 ? (SL == 1 ? 8u : 4u) : (SL == 3 ? 4u : 2u); // wave rectangle
 	constexpr uint32_t RH = (64u >> SL) / RW;
 	constexpr uint32_t SUB_X = 8u / RW;
 
-	const uint32_t nwg = gridDim.x;
 	const uint32_t per_xcd = nwg / 8u;
-	const uint32_t logical = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
 	const uint32_t lane = threadIdx.x & 63u;
-	const uint32_t wave_global = logical * 4u + (threadIdx.x >> 6);
+	const uint32_t wave_global = logical * waves_of(a) + (threadIdx.x >> 6);
 	const uint32_t block = wave_global / S, sub = wave_global % S;
 	const uint32_t bx = block % a.blocks_x, by = block / a.blocks_x;
 	if (by >= a.blocks_y)
@@ -730,42 +735,92 @@ __global__ void __launch_bounds__(256) k_shade_ssaa(pgs_shade_args a)
 		}
 	}
 }
+
+__global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a) { shade_1x_body(a, blockIdx.x, gridDim.x); }
+template <uint32_t RX, uint32_t RY>
+__global__ void __launch_bounds__(256) k_shade_ssaa(pgs_shade_args a) { shade_ssaa_body<RX, RY>(a, blockIdx.x, gridDim.x); }
+PGS_DEFINE_MULTI(256, k_shade_1x_multi, shade_1x_body, pgs_shade_args)
+PGS_DEFINE_MULTI(256, k_shade_ssaa_01_multi, (shade_ssaa_body<0, 1>), pgs_shade_args)
+PGS_DEFINE_MULTI(256, k_shade_ssaa_11_multi, (shade_ssaa_body<1, 1>), pgs_shade_args)
+PGS_DEFINE_MULTI(256, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_args)
+PGS_DEFINE_MULTI(256, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
 }
+}
+
+// Workgroups of one shading launch (padded to a multiple of 8 for the XCD remap).
+uint32_t pgs_grid_shade(const pgs_shade_args &a)
+{
+	const uint32_t waves = a.waves_per_group ? a.waves_per_group : 4u;
+	uint64_t groups;
+	if (a.rate_log2 == 0)
+	{
+		const uint32_t gw = waves >= 2u ? 2u : 1u, gh = waves >= 4u ? 2u : 1u;
+		groups = uint64_t((a.blocks_x + gw - 1u) / gw) * ((a.blocks_y + gh - 1u) / gh);
+	}
+	else
+	{
+		const uint64_t total_waves = uint64_t(a.blocks_x) * a.blocks_y << (a.rate_x_log2 + a.rate_log2);
+		groups = (total_waves + waves - 1u) / waves;
+	}
+	return groups ? uint32_t((groups + 7u) & ~uint64_t(7)) : 0u;
+}
+
+static bool shade_args_valid(const pgs_shade_args *args)
+{
+	if (!args || !args->vram || !args->setup || !args->tattr || !args->prims || !args->states || !args->lists ||
+	    !args->counts)
+		return false;
+	if (args->tile_log2 < 3 || args->tile_log2 > 7)
+		return false;
+	if (args->waves_per_group != 0 && args->waves_per_group != 1 && args->waves_per_group != 2 &&
+	    args->waves_per_group != 4)
+		return false;
+	if (args->rate_log2 != 0 && !args->tile_modes)
+		return false;
+	return true;
 }
 
 extern "C" int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream)
 {
-	if (!args || !args->vram || !args->setup || !args->tattr || !args->prims || !args->states || !args->lists ||
-	    !args->counts)
+	if (!shade_args_valid(args))
 		return -int(hipErrorInvalidValue);
-	if (args->tile_log2 < 3 || args->tile_log2 > 7)
-		return -int(hipErrorInvalidValue);
-	const uint32_t quads = ((args->blocks_x + 1u) / 2u) * ((args->blocks_y + 1u) / 2u);
-	if (quads == 0)
+	const uint32_t groups = pgs_grid_shade(*args);
+	if (groups == 0)
 		return 0;
-	if (args->rate_log2 == 0)
-	{
-		const uint32_t groups = (quads + 7u) & ~7u;
-		hipLaunchKernelGGL(pgs::k_shade_1x, dim3(groups), dim3(256), 0, stream, *args);
-	}
+	const uint32_t threads = 64u * (args->waves_per_group ? args->waves_per_group : 4u);
+	const uint32_t rx = args->rate_x_log2, ry = args->rate_log2;
+	if (ry == 0)
+		hipLaunchKernelGGL(pgs::k_shade_1x, dim3(groups), dim3(threads), 0, stream, *args);
+	else if (rx == 0 && ry == 1)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<0, 1>), dim3(groups), dim3(threads), 0, stream, *args);
+	else if (rx == 1 && ry == 1)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 1>), dim3(groups), dim3(threads), 0, stream, *args);
+	else if (rx == 1 && ry == 2)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 2>), dim3(groups), dim3(threads), 0, stream, *args);
+	else if (rx == 2 && ry == 2)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<2, 2>), dim3(groups), dim3(threads), 0, stream, *args);
 	else
-	{
-		if (!args->tile_modes)
-			return -int(hipErrorInvalidValue);
-		const uint32_t rx = args->rate_x_log2, ry = args->rate_log2;
-		const uint64_t waves = uint64_t(args->blocks_x) * args->blocks_y << (rx + ry);
-		const uint32_t groups = uint32_t(((waves + 3u) / 4u + 7u) & ~uint64_t(7));
-		if (rx == 0 && ry == 1)
-			hipLaunchKernelGGL((pgs::k_shade_ssaa<0, 1>), dim3(groups), dim3(256), 0, stream, *args);
-		else if (rx == 1 && ry == 1)
-			hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 1>), dim3(groups), dim3(256), 0, stream, *args);
-		else if (rx == 1 && ry == 2)
-			hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 2>), dim3(groups), dim3(256), 0, stream, *args);
-		else if (rx == 2 && ry == 2)
-			hipLaunchKernelGGL((pgs::k_shade_ssaa<2, 2>), dim3(groups), dim3(256), 0, stream, *args);
-		else
-			return -int(hipErrorNotSupported);
-	}
+		return -int(hipErrorNotSupported);
 	hipError_t err = hipGetLastError();
 	return err == hipSuccess ? 0 : -int(err);
+}
+
+// Every launch of a multi-launch shares rates and waves per group (the caller groups them).
+int pgs_multi_shade(const pgs_shade_args *args, const uint32_t *prefix, uint32_t n, uint32_t total, uint32_t rate_x,
+                    uint32_t rate_y, uint32_t waves, hipStream_t stream)
+{
+	const uint32_t threads = 64u * (waves ? waves : 4u);
+	if (rate_y == 0)
+		hipLaunchKernelGGL(pgs::k_shade_1x_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	else if (rate_x == 0 && rate_y == 1)
+		hipLaunchKernelGGL(pgs::k_shade_ssaa_01_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	else if (rate_x == 1 && rate_y == 1)
+		hipLaunchKernelGGL(pgs::k_shade_ssaa_11_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	else if (rate_x == 1 && rate_y == 2)
+		hipLaunchKernelGGL(pgs::k_shade_ssaa_12_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	else if (rate_x == 2 && rate_y == 2)
+		hipLaunchKernelGGL(pgs::k_shade_ssaa_22_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	else
+		return -int(hipErrorNotSupported);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }

@@ -11,6 +11,8 @@
 #include "gs_defs.hpp"
 #include "texture_layout.hpp"
 #include "pgs_kernels.h"
+#include "multi_launch.hpp"
+#include "multi_dispatch.hpp"
 
 namespace pgs
 {
@@ -33,7 +35,7 @@ __device__ __forceinline__ uint32_t rgb24_expand(uint32_t c, uint32_t aem, uint3
 // clut_write.comp:46-155. CSM1 reads the palette from a 16x16 (8-bit indices) or 8x2 (4-bit)
 // rectangle whose entries are arranged in 8x2 blocks; CSM2 reads a row of the buffer at
 // (COU * 16, COV) with stride CBW.
-__global__ void __launch_bounds__(256) k_clut_write(pgs_clut_args a)
+__device__ __forceinline__ void clut_write_body(const pgs_clut_args &a, uint32_t, uint32_t)
 {
 	const uint32_t lane = threadIdx.x;
 	const bool ct32 = a.cpsm == PSMCT32;
@@ -96,14 +98,14 @@ __device__ __forceinline__ uint32_t clut_color(const uint16_t *clut, const TexDe
 }
 
 // upload.comp:108-228 (single-sampled source).
-__global__ void __launch_bounds__(256) k_texture_decode(pgs_tex_decode_args a)
+__device__ __forceinline__ void texture_decode_body(const pgs_tex_decode_args &a, uint32_t wg, uint32_t)
 {
 	const TexDecode *decodes = static_cast<const TexDecode *>(a.decodes);
 	uint32_t s = 0;
-	while (s + 1u < a.num_textures && decodes[s + 1u].group_begin <= blockIdx.x)
+	while (s + 1u < a.num_textures && decodes[s + 1u].group_begin <= wg)
 		s++;
 	const TexDecode d = decodes[s];
-	const uint32_t t = (blockIdx.x - d.group_begin) * 256u + threadIdx.x;
+	const uint32_t t = (wg - d.group_begin) * 256u + threadIdx.x;
 	if (t >= d.width * d.height)
 		return;
 	const uint32_t x = t % d.width, y = t / d.width;
@@ -128,6 +130,11 @@ __global__ void __launch_bounds__(256) k_texture_decode(pgs_tex_decode_args a)
 	}
 	a.images[d.image_offset + t] = c;
 }
+
+__global__ void __launch_bounds__(256) k_clut_write(pgs_clut_args a) { clut_write_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) k_texture_decode(pgs_tex_decode_args a) { texture_decode_body(a, blockIdx.x, gridDim.x); }
+PGS_DEFINE_MULTI(256, k_clut_write_multi, clut_write_body, pgs_clut_args)
+PGS_DEFINE_MULTI(256, k_texture_decode_multi, texture_decode_body, pgs_tex_decode_args)
 }
 }
 
@@ -149,4 +156,21 @@ extern "C" int pgs_k_texture_decode(const pgs_tex_decode_args *args, hipStream_t
 	hipLaunchKernelGGL(pgs::k_texture_decode, dim3(args->num_groups), dim3(256), 0, stream, *args);
 	hipError_t err = hipGetLastError();
 	return err == hipSuccess ? 0 : -int(err);
+}
+
+uint32_t pgs_grid_clut_write(const pgs_clut_args &) { return 1u; }
+uint32_t pgs_grid_texture_decode(const pgs_tex_decode_args &a) { return a.num_textures ? a.num_groups : 0u; }
+
+int pgs_multi_clut_write(const pgs_clut_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
+                         hipStream_t stream)
+{
+	hipLaunchKernelGGL(pgs::k_clut_write_multi, dim3(total), dim3(256), 0, stream, args, prefix, n);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+}
+
+int pgs_multi_texture_decode(const pgs_tex_decode_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
+                             hipStream_t stream)
+{
+	hipLaunchKernelGGL(pgs::k_texture_decode_multi, dim3(total), dim3(256), 0, stream, args, prefix, n);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }

@@ -9,6 +9,8 @@
 // atomics are needed. Per-pixel read/merge rules follow vram_copy.comp:199-613.
 #include "gs_defs.hpp"
 #include "pgs_kernels.h"
+#include "multi_launch.hpp"
+#include "multi_dispatch.hpp"
 
 namespace pgs
 {
@@ -68,9 +70,9 @@ __device__ __forceinline__ bool read_source_host(const uint8_t *p, uint32_t byte
 	}
 }
 
-__global__ void __launch_bounds__(256) k_resolve_blocks(pgs_vram_copy_args a)
+__device__ __forceinline__ void resolve_blocks_body(const pgs_vram_copy_args &a, uint32_t wg, uint32_t)
 {
-	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint32_t wave = (wg * blockDim.x + threadIdx.x) >> 6;
 	const uint32_t lane = threadIdx.x & 63u;
 	if (wave >= a.num_blocks)
 		return;
@@ -154,9 +156,9 @@ __global__ void __launch_bounds__(256) k_resolve_blocks(pgs_vram_copy_args a)
 
 // Snapshots whole blocks of VRAM into the shadow plane before a batch whose local copies read
 // memory the batch also writes (gs/gs_renderer.cpp:3921-3942 shadow-page sync).
-__global__ void __launch_bounds__(256) k_snapshot_blocks(pgs_vram_snapshot_args a)
+__device__ __forceinline__ void snapshot_blocks_body(const pgs_vram_snapshot_args &a, uint32_t wg, uint32_t)
 {
-	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint32_t wave = (wg * blockDim.x + threadIdx.x) >> 6;
 	const uint32_t lane = threadIdx.x & 63u;
 	if (wave >= a.num_blocks)
 		return;
@@ -164,7 +166,29 @@ __global__ void __launch_bounds__(256) k_snapshot_blocks(pgs_vram_snapshot_args 
 	const uint32_t word = block * 64u + lane;
 	reinterpret_cast<uint32_t *>(a.shadow)[word] = reinterpret_cast<const uint32_t *>(a.vram)[word];
 }
+
+__global__ void __launch_bounds__(256) k_resolve_blocks(pgs_vram_copy_args a) { resolve_blocks_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) k_snapshot_blocks(pgs_vram_snapshot_args a) { snapshot_blocks_body(a, blockIdx.x, gridDim.x); }
+PGS_DEFINE_MULTI(256, k_resolve_blocks_multi, resolve_blocks_body, pgs_vram_copy_args)
+PGS_DEFINE_MULTI(256, k_snapshot_blocks_multi, snapshot_blocks_body, pgs_vram_snapshot_args)
 }
+}
+
+uint32_t pgs_grid_vram_copy(const pgs_vram_copy_args &a) { return (a.num_blocks + 3u) / 4u; }
+uint32_t pgs_grid_vram_snapshot(const pgs_vram_snapshot_args &a) { return (a.num_blocks + 3u) / 4u; }
+
+int pgs_multi_vram_copy(const pgs_vram_copy_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
+                        hipStream_t stream)
+{
+	hipLaunchKernelGGL(pgs::k_resolve_blocks_multi, dim3(total), dim3(256), 0, stream, args, prefix, n);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+}
+
+int pgs_multi_vram_snapshot(const pgs_vram_snapshot_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
+                            hipStream_t stream)
+{
+	hipLaunchKernelGGL(pgs::k_snapshot_blocks_multi, dim3(total), dim3(256), 0, stream, args, prefix, n);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }
 
 extern "C" int pgs_k_vram_copy(const pgs_vram_copy_args *args, hipStream_t stream)

@@ -1,0 +1,33 @@
+// Multi-launch support: one kernel launch runs the workgroups of several launches of the same
+// kernel, typically one per GS context of a batch. Launch l owns the launch's workgroups
+// [prefix[l], prefix[l + 1]) and its body sees them as 0 .. count - 1, exactly as in a launch
+// of its own.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace pgs
+{
+__device__ __forceinline__ uint32_t find_launch(const uint32_t *prefix, uint32_t n, uint32_t wg)
+{
+	uint32_t lo = 0, hi = n; // prefix[lo] <= wg < prefix[hi]
+	while (hi - lo > 1u)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if (prefix[mid] <= wg)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+}
+
+// Defines `NAME(const ARGS *args, const uint32_t *prefix, uint32_t n)` running BODY for the
+// launch that owns each workgroup.
+#define PGS_DEFINE_MULTI(THREADS, NAME, BODY, ARGS)                                               \
+	__global__ void __launch_bounds__(THREADS) NAME(const ARGS *args, const uint32_t *prefix, uint32_t n) \
+	{                                                                                               \
+		const uint32_t l = ::pgs::find_launch(prefix, n, blockIdx.x);                               \
+		BODY(args[l], blockIdx.x - prefix[l], prefix[l + 1] - prefix[l]);                            \
+	}

@@ -53,7 +53,7 @@ Renderer::~Renderer()
 		(void)hipEventDestroy(e);
 	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_,
 	                 (void *)clut_buf_, (void *)tex_images_, (void *)state_copy_, (void *)flat_buckets_,
-	                 (void *)tile_modes_ })
+	                 (void *)tile_modes_, (void *)snapshot_ })
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
@@ -679,12 +679,27 @@ void Renderer::write_vram(const void *src, size_t offset, size_t size)
 // ---------------------------------------------------------------------------------------
 // Recording and replay.
 
+// Recording keeps every upload alive and remembers every launch; the VRAM planes and the
+// CLUT at the start are snapshotted so that each replay starts from the same state and
+// redoes exactly the recorded work.
 void Renderer::begin_recording()
 {
 	sync();
 	destroy_graph();
 	recorded_.clear();
+	const size_t vram_bytes = size_t(num_planes_) * kVramSize;
+	if (!snapshot_)
+		check(hipMalloc(&snapshot_, vram_bytes + 1024u));
+	check(hipMemcpyAsync(snapshot_, vram_, vram_bytes, hipMemcpyDeviceToDevice, stream_));
+	check(hipMemcpyAsync(snapshot_ + vram_bytes, clut_buf_, 1024u, hipMemcpyDeviceToDevice, stream_));
 	recording_ = true;
+}
+
+void Renderer::restore_snapshot(hipStream_t stream)
+{
+	const size_t vram_bytes = size_t(num_planes_) * kVramSize;
+	check(hipMemcpyAsync(vram_, snapshot_, vram_bytes, hipMemcpyDeviceToDevice, stream));
+	check(hipMemcpyAsync(clut_buf_, snapshot_ + vram_bytes, 1024u, hipMemcpyDeviceToDevice, stream));
 }
 
 void Renderer::end_recording()
@@ -708,8 +723,11 @@ int Renderer::replay(bool use_graph)
 {
 	try
 	{
+		if (!snapshot_)
+			return -int(hipErrorInvalidValue);
 		if (!use_graph)
 		{
+			restore_snapshot(stream_);
 			for (auto &l : recorded_)
 				launch(l);
 			return 0;
@@ -717,6 +735,7 @@ int Renderer::replay(bool use_graph)
 		if (!graph_exec_)
 		{
 			check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+			restore_snapshot(stream_);
 			for (auto &l : recorded_)
 				launch(l);
 			check(hipStreamEndCapture(stream_, &graph_));

@@ -85,17 +85,10 @@ public:
 	int last_error() const { return last_error_; }
 
 	uint8_t *vram_device() { return vram_; }
+	size_t lists_capacity() const { return lists_cap_; }
+	size_t counts_capacity() const { return counts_cap_; }
 
-private:
-	struct Arena
-	{
-		uint8_t *host = nullptr;
-		uint8_t *dev = nullptr;
-		size_t cap = 0;
-		size_t used = 0;      // bytes handed out
-		size_t uploaded = 0;  // bytes already copied to the device
-	};
-
+	// A kernel launch as recorded for replay (kind + argument block).
 	enum class LaunchKind : uint32_t
 	{
 		VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4, ClutWrite = 5, TexDecode = 6,
@@ -116,6 +109,19 @@ private:
 			pgs_ssaa_heuristic_args heuristic;
 		};
 	};
+	const std::vector<Launch> &recorded() const { return recorded_; }
+	// Restores VRAM planes and the CLUT to their state at begin_recording() (on `stream`).
+	void restore_snapshot(hipStream_t stream);
+
+private:
+	struct Arena
+	{
+		uint8_t *host = nullptr;
+		uint8_t *dev = nullptr;
+		size_t cap = 0;
+		size_t used = 0;      // bytes handed out
+		size_t uploaded = 0;  // bytes already copied to the device
+	};
 
 	// Device scratch of a render pass, sized for the largest pass (65536 primitives).
 	uint8_t *setup_buf_ = nullptr;    // PrimitiveSetup[65536]
@@ -128,6 +134,7 @@ private:
 	uint32_t *state_copy_ = nullptr;  // super-sampled passes: post-setup states
 	uint32_t *flat_buckets_ = nullptr; // super-sampled passes: 256-bit bucket mask
 	uint32_t *tile_modes_ = nullptr;  // super-sampled passes: per coarse tile
+	uint8_t *snapshot_ = nullptr;     // VRAM planes + CLUT at begin_recording()
 	size_t lists_cap_ = 0;            // entries
 	size_t counts_cap_ = 0;
 	uint32_t tile_log2_1x_ = 5;

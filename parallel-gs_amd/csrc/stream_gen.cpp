@@ -292,28 +292,40 @@ const uint8_t *pgs_gen_plain(uint64_t seed, uint32_t n_tris, uint32_t width, uin
 // side; with pareto_alpha > 0 the primitive's area is Pareto(alpha) with a 64 px^2 minimum
 // (capped at the window) and the side / radius is its square root.
 const uint8_t *pgs_gen_textured(uint64_t seed, uint32_t n_prims, uint32_t frames, uint32_t width, uint32_t height,
-                                double pareto_alpha, size_t *size)
+                                double pareto_alpha, uint32_t tex_psm, uint32_t blend_preset, size_t *size)
 {
+	// Blend presets (A, B, C, D): 0 = (Cs, Cd, As, Cd) alpha blend, 1 = (Cs, 0, As, Cd) additive,
+	// 2 = (0, Cs, As, Cd) subtractive, 3 = (Cs, Cd, FIX = 64, Cd) fixed 50% blend.
+	static const uint64_t presets[4] = {
+		uint64_t(0) | 1u << 2 | 0u << 4 | 1u << 6,
+		uint64_t(0) | 2u << 2 | 0u << 4 | 1u << 6,
+		uint64_t(2) | 0u << 2 | 0u << 4 | 1u << 6,
+		uint64_t(0) | 1u << 2 | 2u << 4 | 1u << 6 | uint64_t(64) << 32,
+	};
+	if (tex_psm != CT32 && tex_psm != T8 && tex_psm != T4)
+		tex_psm = T8;
 	Rng rng(seed);
 	Stream s;
 	const uint32_t ofs = 2048u * 16u;
 	const uint32_t fbw = (width + 63) / 64;
 	const uint32_t zbp = (fbw * 64u * height * 4u + 8191u) / 8192u;
 	const uint32_t tbp = (zbp * 2u + 8u) * 32u; // texture after the depth buffer
-	const uint32_t cbp = tbp + 32u * 8u + 64u;  // CLUT after the 64 KiB texture
+	const uint32_t cbp = tbp + 32u * 32u + 64u; // CLUT after the texture (at most 256 KiB)
 	s.ad({ { 0x1a, 1 },
 	       { 0x4c, frame_reg(0, fbw, CT32, 0) },
 	       { 0x4e, zbuf_reg(zbp, Z24, 0) },
 	       { 0x18, uint64_t(ofs) | uint64_t(ofs) << 32 },
 	       { 0x40, scissor_reg(0, width - 1, 0, height - 1) },
 	       { 0x47, test_reg(0, 0, 0, 0, 1, 2) },
-	       { 0x42, uint64_t(0) | 1u << 2 | 0u << 4 | 1u << 6 }, // A=Cs B=Cd C=As D=Cd
+	       { 0x42, presets[blend_preset & 3u] },
 	       { 0x14, uint64_t(1) << 5 | uint64_t(1) << 6 },       // TEX1: MMAG = MMIN = LINEAR
 	       { 0x08, 0 },                                          // CLAMP: REPEAT
 	       { 0x46, 0 },
 	       { 0x45, 0 } });
-	std::vector<uint8_t> payload(256 * 256);
-	std::vector<uint8_t> clut(256 * 4);
+	const uint32_t tex_bytes = 256u * 256u * bpp_of(tex_psm) / 8u;
+	std::vector<uint8_t> payload(tex_bytes);
+	const uint32_t clut_w = tex_psm == T4 ? 8u : 16u, clut_h = tex_psm == T4 ? 2u : 16u;
+	std::vector<uint8_t> clut(clut_w * clut_h * 4u);
 	const double area_cap = double(width) * double(height);
 	for (uint32_t f = 0; f < frames; f++)
 	{
@@ -322,20 +334,23 @@ const uint8_t *pgs_gen_textured(uint64_t seed, uint32_t n_prims, uint32_t frames
 			uint64_t r = rng.next();
 			memcpy(payload.data() + b, &r, 8);
 		}
-		s.ad({ { 0x50, bitbltbuf(0, 0, 0, tbp, 4, T8) }, { 0x51, trxpos(0, 0, 0, 0) }, { 0x52, trxreg(256, 256) },
-		       { 0x53, 0 } });
+		s.ad({ { 0x50, bitbltbuf(0, 0, 0, tbp, 4, tex_psm) }, { 0x51, trxpos(0, 0, 0, 0) },
+		       { 0x52, trxreg(256, 256) }, { 0x53, 0 } });
 		s.image(payload.data(), payload.size());
 		for (size_t b = 0; b < clut.size(); b += 8)
 		{
 			uint64_t r = rng.next();
 			memcpy(clut.data() + b, &r, 8);
 		}
-		s.ad({ { 0x50, bitbltbuf(0, 0, 0, cbp, 1, CT32) }, { 0x51, trxpos(0, 0, 0, 0) }, { 0x52, trxreg(16, 16) },
-		       { 0x53, 0 } });
-		s.image(clut.data(), clut.size());
-		const uint64_t tex0 = uint64_t(tbp) | uint64_t(4) << 14 | uint64_t(T8) << 20 | uint64_t(8) << 26 |
+		if (tex_psm != CT32)
+		{
+			s.ad({ { 0x50, bitbltbuf(0, 0, 0, cbp, 1, CT32) }, { 0x51, trxpos(0, 0, 0, 0) },
+			       { 0x52, trxreg(clut_w, clut_h) }, { 0x53, 0 } });
+			s.image(clut.data(), clut.size());
+		}
+		const uint64_t tex0 = uint64_t(tbp) | uint64_t(4) << 14 | uint64_t(tex_psm) << 20 | uint64_t(8) << 26 |
 		                      uint64_t(8) << 30 | uint64_t(1) << 34 | uint64_t(0) << 35 | uint64_t(cbp) << 37 |
-		                      uint64_t(CT32) << 51 | uint64_t(1) << 61;
+		                      uint64_t(CT32) << 51 | uint64_t(tex_psm != CT32 ? 1 : 0) << 61;
 		s.ad({ { 0x06, tex0 } });
 
 		uint64_t last_prim = ~0ull;

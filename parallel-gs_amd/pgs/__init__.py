@@ -71,6 +71,8 @@ EXPORTED_SYMBOLS = [
     "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
     "pgs_k_vram_copy", "pgs_k_vram_snapshot", "pgs_k_init_tables", "pgs_k_triangle_setup",
     "pgs_k_binning", "pgs_k_shade", "pgs_k_clut_write", "pgs_k_texture_decode", "pgs_k_ssaa_heuristic",
+    "pgs_batch_create", "pgs_batch_destroy", "pgs_batch_apply_tuning", "pgs_batch_autotune", "pgs_batch_replay",
+    "pgs_batch_sync", "pgs_batch_stream", "pgs_batch_num_launches", "pgs_batch_profile",
 ]
 
 
@@ -132,7 +134,8 @@ def streams() -> ctypes.CDLL:
                                     ctypes.POINTER(ctypes.c_size_t)]
         S.pgs_gen_textured.restype = ctypes.c_void_p
         S.pgs_gen_textured.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                       ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_size_t)]
+                                       ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_size_t)]
         S.pgs_gen_transfers_wild.restype = ctypes.c_void_p
         S.pgs_gen_transfers_wild.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)]
         _streams = S
@@ -159,12 +162,15 @@ def gen_plain(seed: int, n_tris: int = 2000, width: int = 640, height: int = 448
 
 
 def gen_textured(seed: int, n_prims: int, frames: int, width: int = 640, height: int = 448,
-                 pareto_alpha: float = 0.0) -> np.ndarray:
-    """Textured + blended stream: per frame a PSMT8 256x256 texture + PSMCT32 CLUT, n_prims
-    primitives (70% triangles, 30% sprites), bilinear MODULATE, (Cs - Cd) * As >> 7 + Cd,
-    GEQUAL depth, 20% ST/Q. pareto_alpha > 0 draws primitive areas from Pareto(alpha)."""
+                 pareto_alpha: float = 0.0, tex_psm: int = 0x13, blend_preset: int = 0) -> np.ndarray:
+    """Textured + blended stream: per frame a 256x256 texture (PSMCT32, PSMT8 or PSMT4; the
+    palette formats with a PSMCT32 CLUT loaded by CLD=1), n_prims primitives (70% triangles,
+    30% sprites), bilinear MODULATE, a blend preset (0: (Cs-Cd)*As+Cd, 1: Cs*As+Cd,
+    2: Cd-Cs*As, 3: (Cs-Cd)*64+Cd), GEQUAL depth, 20% ST/Q. pareto_alpha > 0 draws primitive
+    areas from Pareto(alpha)."""
     size = ctypes.c_size_t(0)
-    ptr = streams().pgs_gen_textured(seed, n_prims, frames, width, height, pareto_alpha, ctypes.byref(size))
+    ptr = streams().pgs_gen_textured(seed, n_prims, frames, width, height, pareto_alpha, tex_psm, blend_preset,
+                                     ctypes.byref(size))
     return _copy_stream(ptr, size)
 
 
@@ -285,3 +291,106 @@ def host_block_lane_pixel(psm: int, lane: int, k: int):
     dx, dy = ctypes.c_uint32(), ctypes.c_uint32()
     lib().pgs_host_block_lane_pixel(psm, lane, k, ctypes.byref(dx), ctypes.byref(dy))
     return int(dx.value), int(dy.value)
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("tile_log2_1x", ctypes.c_uint32), ("waves_per_group", ctypes.c_uint32)]
+
+
+class _TuningResult(ctypes.Structure):
+    _fields_ = [("tuning", _Tuning), ("ms_per_replay", ctypes.c_float)]
+
+
+class Batch:
+    """Batched replay of several contexts' recordings (merged multi-context launches)."""
+
+    def __init__(self, contexts, sequential: bool = False):
+        L = lib()
+        L.pgs_batch_create.restype = ctypes.c_void_p
+        L.pgs_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32, ctypes.c_uint32]
+        L.pgs_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.pgs_batch_apply_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Tuning)]
+        L.pgs_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(_TuningResult), ctypes.c_uint32]
+        L.pgs_batch_replay.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.pgs_batch_sync.argtypes = [ctypes.c_void_p]
+        L.pgs_batch_stream.restype = ctypes.c_void_p
+        L.pgs_batch_stream.argtypes = [ctypes.c_void_p]
+        L.pgs_batch_num_launches.restype = ctypes.c_uint32
+        L.pgs_batch_num_launches.argtypes = [ctypes.c_void_p]
+        self._L = L
+        self._contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self._contexts))(*[c._h for c in self._contexts])
+        self._h = L.pgs_batch_create(arr, len(self._contexts), 1 if sequential else 0)
+        if not self._h:
+            raise PgsError("pgs_batch_create failed")
+
+    def close(self):
+        if self._h:
+            self._L.pgs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def replay(self, use_graph: bool = True):
+        rc = self._L.pgs_batch_replay(self._h, int(use_graph))
+        if rc != 0:
+            raise PgsError(f"pgs_batch_replay failed with {rc}")
+
+    def sync(self):
+        if self._L.pgs_batch_sync(self._h) != 0:
+            raise PgsError("pgs_batch_sync failed")
+
+    def apply_tuning(self, tile_log2_1x: int, waves_per_group: int):
+        t = _Tuning(tile_log2_1x, waves_per_group)
+        if self._L.pgs_batch_apply_tuning(self._h, ctypes.byref(t)) != 0:
+            raise PgsError("pgs_batch_apply_tuning failed")
+
+    def autotune(self, reps: int = 3):
+        res = (_TuningResult * 64)()
+        n = self._L.pgs_batch_autotune(self._h, reps, res, 64)
+        if n < 0:
+            raise PgsError("pgs_batch_autotune failed")
+        return [(r.tuning.tile_log2_1x, r.tuning.waves_per_group, r.ms_per_replay) for r in res[:n]]
+
+    @property
+    def stream(self) -> int:
+        return int(self._L.pgs_batch_stream(self._h) or 0)
+
+    @property
+    def num_launches(self) -> int:
+        return int(self._L.pgs_batch_num_launches(self._h))
+
+    def profile(self, kinds: int = 16):
+        """Per-kind (ms, launches) of one event-bracketed replay."""
+        self._L.pgs_batch_profile.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32]
+        ms = (ctypes.c_double * kinds)()
+        n = (ctypes.c_uint64 * kinds)()
+        if self._L.pgs_batch_profile(self._h, ms, n, kinds) != 0:
+            raise PgsError("pgs_batch_profile failed")
+        return list(ms), list(n)
+
+
+def batched_specs(seed: int, n_contexts: int, n_min: int = 500, n_max: int = 50000, ssaa_choices=(1, 2, 4),
+                  pareto_alpha: float = 0.0, tex_psms=(0x00, 0x13, 0x14), blend_presets=(0, 1, 2, 3)):
+    """Per-context workload choices of the batched configurations: primitive count log-uniform
+    in [n_min, n_max], frame 512x448 or 640x448, a sampling rate, a texture format and a blend
+    preset, each drawn from a seeded numpy generator. Returns a list of dicts."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n_contexts):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        specs.append(dict(seed=int(seed) * 1000 + i, n_prims=n, width=int(rng.choice([512, 640])), height=448,
+                          ssaa=int(rng.choice(ssaa_choices)), tex_psm=int(rng.choice(tex_psms)),
+                          blend=int(rng.choice(blend_presets)), pareto_alpha=float(pareto_alpha)))
+    return specs
+
+
+def spec_stream(spec) -> np.ndarray:
+    """The GIF stream of one batched-context spec (one frame)."""
+    return gen_textured(spec["seed"], spec["n_prims"], 1, spec["width"], spec["height"], spec["pareto_alpha"],
+                        spec["tex_psm"], spec["blend"])

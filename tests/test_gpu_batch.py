@@ -1,0 +1,91 @@
+"""GPU: batched replay of several contexts (merged multi-context launches, hipGraph) gives each
+context exactly the VRAM its own live run and the CPU oracle give, for every tuning."""
+import numpy as np
+import pytest
+
+import pgs
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+LOG2 = {1: 0, 2: 1, 4: 2, 8: 3, 16: 4}
+
+
+def make_contexts(specs):
+    ctxs, live = [], []
+    for sp in specs:
+        stream = pgs.spec_stream(sp)
+        print(f"[case] context n={sp['n_prims']} ssaa={sp['ssaa']} tex={sp['tex_psm']:#x} blend={sp['blend']}",
+              flush=True)
+        ctx = pgs.Context(super_sampling=sp["ssaa"], arena_bytes=32 << 20)
+        ctx.begin_recording()
+        ctx.gif_transfer(stream)
+        ctx.end_recording()
+        live.append(ctx.vram_planes())
+        ctxs.append(ctx)
+    return ctxs, live
+
+
+def oracle_planes(sp):
+    o = Oracle(LOG2[sp["ssaa"]])
+    o.gif_transfer(pgs.spec_stream(sp))
+    o.flush()
+    return o.planes()
+
+
+def test_batch_matches_live_and_oracle(hip_device):
+    specs = pgs.batched_specs(0x5EED0004, 6, n_min=300, n_max=3000)
+    ctxs, live = make_contexts(specs)
+    for sp, planes in zip(specs, live):
+        assert np.array_equal(planes, oracle_planes(sp))
+    batch = pgs.Batch(ctxs)
+    print("merged launches", batch.num_launches, flush=True)
+    assert batch.num_launches < sum(1 for _ in ctxs) * 6
+    for tuning in [(0, 4), (3, 1), (4, 2), (5, 4)]:
+        batch.apply_tuning(*tuning)
+        batch.replay(True)
+        batch.replay(True)
+        batch.sync()
+        for i, ctx in enumerate(ctxs):
+            got = ctx.vram_planes()
+            bad = np.flatnonzero(got != live[i])
+            assert bad.size == 0, (tuning, i, bad[:8])
+    batch.close()
+    seq = pgs.Batch(ctxs, sequential=True)
+    seq.replay(True)
+    seq.sync()
+    for i, ctx in enumerate(ctxs):
+        assert np.array_equal(ctx.vram_planes(), live[i])
+    seq.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_autotune_keeps_pixels(hip_device):
+    specs = pgs.batched_specs(0x5EED0005, 4, n_min=300, n_max=2000, ssaa_choices=(1, 4))
+    ctxs, live = make_contexts(specs)
+    batch = pgs.Batch(ctxs)
+    table = batch.autotune(2)
+    print(table, flush=True)
+    assert len(table) >= 6
+    batch.replay(True)
+    batch.sync()
+    for i, ctx in enumerate(ctxs):
+        assert np.array_equal(ctx.vram_planes(), live[i])
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_replay_restores_vram(hip_device):
+    """A single context's graph replay starts from the recording's initial VRAM every time."""
+    stream = pgs.gen_textured(0x5EED0003, 1500, 2, 640, 448, 0.0)
+    with pgs.Context(super_sampling=4) as ctx:
+        ctx.begin_recording()
+        ctx.gif_transfer(stream)
+        ctx.end_recording()
+        live = ctx.vram_planes()
+        for _ in range(3):
+            ctx.replay(True)
+        ctx.sync()
+        assert np.array_equal(ctx.vram_planes(), live)
