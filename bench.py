@@ -230,6 +230,7 @@ def time_batch(batch, steps, warmup):
     """Milliseconds per graph replay of a batch, timed with HIP events on the batch's stream."""
     import torch
 
+    torch.cuda.synchronize()  # initialises torch's device state before wrapping the stream
     for _ in range(warmup):
         batch.replay(True)
     batch.sync()
@@ -351,6 +352,9 @@ def main():
         print(f"bench.py: only --gpus 1 is supported; GS contexts are independent and this build "
               f"measures one GPU's share (asked for {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+    import torch
+
+    torch.cuda.synchronize()  # torch's HIP state first: the library's contexts are created after it
     out = WORKLOADS[args.workload](args)
     line = {"metric": out.pop("metric"), "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"),
