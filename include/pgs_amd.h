@@ -144,6 +144,34 @@ uint32_t pgs_batch_num_launches(pgs_batch *batch);
  * kernel kind (same numbering as pgs_profile_collect). */
 int pgs_batch_profile(pgs_batch *batch, double *ms, uint64_t *launches, uint32_t kinds);
 
+/* Shading schedule of a batch: PGS_SCHEDULE_HARDWARE (one multi-launch per merged shading
+ * step), PGS_SCHEDULE_QUEUE_STATIC (a work queue of resident workgroups in item order) or
+ * PGS_SCHEDULE_QUEUE_LEARNED (the queue ordered by a cost model's predictions, largest first;
+ * `model` must outlive the batch). Pixels are identical under every schedule. */
+#define PGS_SCHEDULE_HARDWARE 0u
+#define PGS_SCHEDULE_QUEUE_STATIC 1u
+#define PGS_SCHEDULE_QUEUE_LEARNED 2u
+typedef struct pgs_cost_model pgs_cost_model;
+int pgs_batch_set_schedule(pgs_batch *batch, uint32_t schedule, const pgs_cost_model *model);
+/* One replay with the static work queue measuring every shading workgroup: writes up to
+ * `max_items` feature rows (PGS_MLP_INPUTS floats) and log2 shader-clock costs; returns the
+ * number of items written (or a negative error). */
+int pgs_batch_collect_costs(pgs_batch *batch, float *features, float *log2_costs, uint32_t max_items);
+
+/* ---- Learned tile-cost model (no counterpart in the reference): the 24 -> 64 -> 64 -> 1 BF16
+ * MLP of pgs_kernels.h, trained on the GPU, that orders a batch's persistent shading queue.
+ * Parameters are PGS_MLP_PARAMS floats in the layout of pgs_kernels.h. All pointers are host
+ * pointers. */
+pgs_cost_model *pgs_cost_model_create(uint64_t seed);
+void pgs_cost_model_destroy(pgs_cost_model *model);
+int pgs_cost_model_set_params(pgs_cost_model *model, const float *params);
+int pgs_cost_model_get_params(pgs_cost_model *model, float *params);
+int pgs_cost_model_predict(pgs_cost_model *model, const float *features, uint32_t count, float *out);
+int pgs_cost_model_gradients(pgs_cost_model *model, const float *features, const float *targets, uint32_t count,
+                             float *grads);
+int pgs_cost_model_train(pgs_cost_model *model, const float *features, const float *targets, uint32_t count,
+                         uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *mse);
+
 #ifdef __cplusplus
 }
 #endif

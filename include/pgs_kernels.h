@@ -156,6 +156,58 @@ typedef struct pgs_shade_args
 
 int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);
 
+/* Work-queue shading of a multi-launch (batched renderer; no counterpart in the reference):
+ * a grid of resident workgroups pops the multi-launch's workgroup indices in `order` (the
+ * identity when NULL) with the atomic `counter` (0 on entry) and runs the shading body of
+ * each. All launches share rate_x / rate_y / waves_per_group. With `costs` set, each item's
+ * shader-clock cycles are stored at costs[item]. The order changes only which workgroup runs
+ * when: every tile is shaded exactly once, so pixels do not depend on it. */
+typedef struct pgs_queue_args
+{
+	const void *args;          /* device pgs_shade_args[num_launches] */
+	const uint32_t *prefix;    /* device workgroup prefix, num_launches + 1 entries */
+	const uint32_t *order;     /* device permutation of 0 .. total - 1, or NULL */
+	uint32_t *counter;         /* device counter, must be 0 */
+	uint32_t *costs;           /* device costs[total], or NULL */
+	uint32_t num_launches;
+	uint32_t total;
+	uint32_t rate_x, rate_y;
+	uint32_t waves_per_group;
+	uint32_t pad;
+} pgs_queue_args;
+
+int pgs_k_shade_queue(const pgs_queue_args *args, hipStream_t stream);
+
+/* Features of each workgroup item of a shading multi-launch for the tile-cost model: one
+ * wave per item scans the item's coarse-tile list. Writes PGS_MLP_INPUTS floats per item. */
+typedef struct pgs_features_args
+{
+	const void *args;          /* device pgs_shade_args[num_launches] */
+	const uint32_t *prefix;
+	float *features;           /* total x PGS_MLP_INPUTS */
+	uint32_t num_launches;
+	uint32_t total;
+} pgs_features_args;
+
+int pgs_k_tile_features(const pgs_features_args *args, hipStream_t stream);
+
+/* Orders `count` items by predicted cost, largest first (256 buckets of 1/8 in log2 cycles;
+ * order within a bucket is arbitrary), writes the permutation to `order` and zeroes
+ * `counter` for the work-queue launch that follows. One workgroup. */
+typedef struct pgs_order_args
+{
+	const float *predicted;    /* log2 cycles per item */
+	uint32_t *order;
+	uint32_t *counter;
+	uint32_t count;
+	uint32_t pad;
+} pgs_order_args;
+
+int pgs_k_queue_order(const pgs_order_args *args, hipStream_t stream);
+
+/* Writes `value` to `count` words (graph-safe replacement of a memset). */
+int pgs_k_fill_u32(uint32_t *dst, uint32_t count, uint32_t value, hipStream_t stream);
+
 /* CLUT load of one TEX0/TEX2 write with CLD that loads.
  * Replaces the clut_write.comp dispatch of GSRenderer::flush_palette_upload
  * (gs/gs_renderer.cpp:3662-3819) for a load queued by GSInterface::handle_clut_upload
@@ -190,6 +242,54 @@ typedef struct pgs_tex_decode_args
 } pgs_tex_decode_args;
 
 int pgs_k_texture_decode(const pgs_tex_decode_args *args, hipStream_t stream);
+
+/* ---- Tile-cost MLP of the learned tile-queue order (no counterpart in the reference; used
+ * by the batched renderer to order its persistent shading queue, never to compute a pixel).
+ * 24 features -> 64 -> 64 -> 1, ReLU, BF16 weights and layer inputs, FP32 accumulation.
+ * Parameter vector layout: W1[64][24], b1[64], W2[64][64], b2[64], w3[64], b3. */
+#define PGS_MLP_INPUTS 24
+#define PGS_MLP_HIDDEN 64
+#define PGS_MLP_PARAMS 5825
+#define PGS_MLP_SAMPLE_FLOATS 281 /* per-sample training scratch: x, h1, h2, dz1, dz2, dy */
+
+typedef struct pgs_mlp_forward_args
+{
+	const uint16_t *weights; /* BF16 parameter vector */
+	const float *features;   /* count x 24 */
+	float *out;              /* count predictions */
+	uint32_t count;
+	uint32_t pad;
+} pgs_mlp_forward_args;
+
+int pgs_k_mlp_forward(const pgs_mlp_forward_args *args, hipStream_t stream);
+
+/* Gradient of mean((y - target)^2) over `count` samples into `grads` (FP32, parameter layout). */
+typedef struct pgs_mlp_grad_args
+{
+	const uint16_t *weights;
+	const float *features;   /* count x 24 */
+	const float *targets;    /* count */
+	float *scratch;          /* count x PGS_MLP_SAMPLE_FLOATS */
+	float *grads;            /* PGS_MLP_PARAMS */
+	float *loss;             /* optional: count squared errors */
+	uint32_t count;
+	uint32_t pad;
+} pgs_mlp_grad_args;
+
+int pgs_k_mlp_gradients(const pgs_mlp_grad_args *args, hipStream_t stream);
+
+typedef struct pgs_mlp_adam_args
+{
+	float *master;           /* FP32 parameters */
+	uint16_t *weights;       /* BF16 copy, refreshed */
+	const float *grads;
+	float *m, *v;            /* Adam moments */
+	float lr, beta1, beta2, eps;
+	float beta1_t, beta2_t;  /* beta1^t, beta2^t of this step */
+} pgs_mlp_adam_args;
+
+int pgs_k_mlp_adam(const pgs_mlp_adam_args *args, hipStream_t stream);
+int pgs_k_mlp_quantize(const float *master, uint16_t *weights, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -47,22 +47,33 @@ def _layer(w: np.ndarray, b: np.ndarray, x: np.ndarray) -> np.ndarray:
     return acc
 
 
-def forward(p: np.ndarray, X: np.ndarray, keep: bool = False):
-    q = {k: bf16(v) for k, v in unpack(np.asarray(p, dtype=np.float32)).items()}
-    xb = bf16(X.astype(np.float32))
-    z1 = _layer(q["W1"], q["b1"], xb)
-    h1 = bf16(np.maximum(z1, 0.0).astype(np.float32))
-    z2 = _layer(q["W2"], q["b2"], h1)
-    h2 = bf16(np.maximum(z2, 0.0).astype(np.float32))
-    y = _layer(q["w3"][None, :], q["b3"], h2)[:, 0]
+def forward(p: np.ndarray, X: np.ndarray, keep: bool = False, quantize: bool = True):
+    """quantize=False evaluates the same network in float64 without BF16 rounding (used to pin
+    the gradient formulas against finite differences)."""
+    if quantize:
+        q = {k: bf16(v) for k, v in unpack(np.asarray(p, dtype=np.float32)).items()}
+        xb = bf16(X.astype(np.float32))
+        z1 = _layer(q["W1"], q["b1"], xb)
+        h1 = bf16(np.maximum(z1, 0.0).astype(np.float32))
+        z2 = _layer(q["W2"], q["b2"], h1)
+        h2 = bf16(np.maximum(z2, 0.0).astype(np.float32))
+        y = _layer(q["w3"][None, :], q["b3"], h2)[:, 0]
+    else:
+        q = {k: v.astype(np.float64) for k, v in unpack(np.asarray(p, dtype=np.float64)).items()}
+        xb = X.astype(np.float64)
+        z1 = xb @ q["W1"].T + q["b1"]
+        h1 = np.maximum(z1, 0.0)
+        z2 = h1 @ q["W2"].T + q["b2"]
+        h2 = np.maximum(z2, 0.0)
+        y = h2 @ q["w3"] + q["b3"][0]
     if keep:
         return y, dict(q=q, xb=xb, z1=z1, h1=h1, z2=z2, h2=h2)
     return y
 
 
-def gradients(p: np.ndarray, X: np.ndarray, t: np.ndarray) -> np.ndarray:
+def gradients(p: np.ndarray, X: np.ndarray, t: np.ndarray, quantize: bool = True) -> np.ndarray:
     """d mean((y - t)^2) / d params, float64, same layout as the parameter vector."""
-    y, c = forward(p, X, keep=True)
+    y, c = forward(p, X, keep=True, quantize=quantize)
     q = {k: v.astype(np.float64) for k, v in c["q"].items()}
     B = X.shape[0]
     dy = 2.0 * (y.astype(np.float64) - t.astype(np.float64)) / B

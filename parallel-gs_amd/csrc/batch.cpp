@@ -5,6 +5,7 @@
 #include <map>
 #include <stdexcept>
 #include <cstring>
+#include <cmath>
 
 namespace pgs
 {
@@ -67,6 +68,8 @@ Batch::~Batch()
 	destroy_graph();
 	if (dev_)
 		(void)hipFree(dev_);
+	if (sched_)
+		(void)hipFree(sched_);
 	if (stream_)
 		(void)hipStreamDestroy(stream_);
 }
@@ -233,8 +236,135 @@ bool Batch::upload()
 		dev_bytes_ = bytes;
 	}
 	ok(hipMemcpy(dev_, host.data(), bytes, hipMemcpyHostToDevice));
+
+	// Work-queue buffers per shading launch: features, predictions, order, costs, counter.
+	size_t sb = 0;
+	for (auto &m : merged_)
+		if (m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper)
+		{
+			m.sched_offset = sb;
+			sb = align256(sb + size_t(m.total) * (PGS_MLP_INPUTS * 4u + 12u) + 4u);
+		}
+	if (sb > sched_bytes_)
+	{
+		if (sched_)
+			ok(hipFree(sched_));
+		ok(hipMalloc(&sched_, sb));
+		sched_bytes_ = sb;
+	}
 	destroy_graph();
 	return true;
+}
+
+bool Batch::set_schedule(Schedule s, const CostModel *model)
+{
+	if (s == Schedule::QueueLearned && !model)
+		return false;
+	schedule_ = s;
+	model_ = model;
+	destroy_graph();
+	return true;
+}
+
+// Shading launch under the current schedule.
+void Batch::launch_shade(const Merged &m)
+{
+	const void *args = dev_ + m.args_offset;
+	const uint32_t *prefix = reinterpret_cast<const uint32_t *>(dev_ + m.prefix_offset);
+	const uint32_t n = uint32_t(m.refs.size());
+	if (schedule_ == Schedule::Hardware && !collecting_)
+	{
+		if (pgs_multi_shade(static_cast<const pgs_shade_args *>(args), prefix, n, m.total, m.rate_x, m.rate_y,
+		                    tuning_.waves_per_group, stream_) != 0)
+			throw std::runtime_error("shade multi-launch failed");
+		return;
+	}
+	uint8_t *base = sched_ + m.sched_offset;
+	float *features = reinterpret_cast<float *>(base);
+	float *pred = features + size_t(m.total) * PGS_MLP_INPUTS;
+	uint32_t *order = reinterpret_cast<uint32_t *>(pred + m.total);
+	uint32_t *costs = order + m.total;
+	uint32_t *counter = costs + m.total;
+
+	const bool learned = schedule_ == Schedule::QueueLearned && !collecting_;
+	if (learned || collecting_)
+	{
+		pgs_features_args f{};
+		f.args = args;
+		f.prefix = prefix;
+		f.features = features;
+		f.num_launches = n;
+		f.total = m.total;
+		if (pgs_k_tile_features(&f, stream_) != 0)
+			throw std::runtime_error("features launch failed");
+	}
+	if (learned)
+	{
+		model_->predict_device(features, m.total, pred, stream_);
+		pgs_order_args o{};
+		o.predicted = pred;
+		o.order = order;
+		o.counter = counter;
+		o.count = m.total;
+		if (pgs_k_queue_order(&o, stream_) != 0)
+			throw std::runtime_error("order launch failed");
+	}
+	else if (pgs_k_fill_u32(counter, 1, 0, stream_) != 0)
+		throw std::runtime_error("fill launch failed");
+
+	pgs_queue_args q{};
+	q.args = args;
+	q.prefix = prefix;
+	q.order = learned ? order : nullptr;
+	q.counter = counter;
+	q.costs = collecting_ ? costs : nullptr;
+	q.num_launches = n;
+	q.total = m.total;
+	q.rate_x = m.rate_x;
+	q.rate_y = m.rate_y;
+	q.waves_per_group = tuning_.waves_per_group;
+	if (pgs_k_shade_queue(&q, stream_) != 0)
+		throw std::runtime_error("work-queue shading launch failed");
+}
+
+uint32_t Batch::collect_costs(float *features_out, float *log2_costs, uint32_t max_items)
+{
+	collecting_ = true;
+	try
+	{
+		for (Renderer *r : contexts_)
+			r->restore_snapshot(stream_);
+		for (auto &m : merged_)
+			launch(m);
+		ok(hipStreamSynchronize(stream_));
+	}
+	catch (...)
+	{
+		collecting_ = false;
+		throw;
+	}
+	collecting_ = false;
+	uint32_t written = 0;
+	std::vector<float> f;
+	std::vector<uint32_t> c;
+	for (auto &m : merged_)
+	{
+		if (!(m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper) || m.total == 0)
+			continue;
+		const uint8_t *base = sched_ + m.sched_offset;
+		f.resize(size_t(m.total) * PGS_MLP_INPUTS);
+		c.resize(m.total);
+		ok(hipMemcpy(f.data(), base, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+		ok(hipMemcpy(c.data(), base + size_t(m.total) * (PGS_MLP_INPUTS + 2u) * 4u, c.size() * 4u,
+		             hipMemcpyDeviceToHost));
+		for (uint32_t i = 0; i < m.total && written < max_items; i++, written++)
+		{
+			memcpy(features_out + size_t(written) * PGS_MLP_INPUTS, &f[size_t(i) * PGS_MLP_INPUTS],
+			       PGS_MLP_INPUTS * sizeof(float));
+			log2_costs[written] = std::log2(float(std::max(c[i], 1u)));
+		}
+	}
+	return written;
 }
 
 void Batch::launch(const Merged &m)
@@ -255,8 +385,7 @@ void Batch::launch(const Merged &m)
 	case K::Setup: rc = pgs_multi_triangle_setup(static_cast<const pgs_setup_args *>(args), prefix, n, m.total, stream_); break;
 	case K::Binning: rc = pgs_multi_binning(static_cast<const pgs_bin_args *>(args), prefix, n, m.total, stream_); break;
 	case K::Shade: case K::ShadeSuper:
-		rc = pgs_multi_shade(static_cast<const pgs_shade_args *>(args), prefix, n, m.total, m.rate_x, m.rate_y,
-		                     tuning_.waves_per_group, stream_);
+		launch_shade(m);
 		break;
 	case K::ClutWrite: rc = pgs_multi_clut_write(static_cast<const pgs_clut_args *>(args), prefix, n, m.total, stream_); break;
 	case K::TexDecode:

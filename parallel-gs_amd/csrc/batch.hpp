@@ -13,6 +13,7 @@
 // decides per-tile shading) and the waves per shading workgroup.
 #pragma once
 #include "renderer.hpp"
+#include "cost_model.hpp"
 #include <vector>
 
 namespace pgs
@@ -22,6 +23,10 @@ struct BatchTuning
 	uint32_t tile_log2_1x = 0;    // 0: keep each pass's recorded tile size; else 3..6
 	uint32_t waves_per_group = 4; // 1, 2 or 4
 };
+
+// How a batch runs its shading launches: the hardware dispatcher (one multi-launch), or a
+// work queue of resident workgroups in static order or in the order of predicted cost.
+enum class Schedule : uint32_t { Hardware = 0, QueueStatic = 1, QueueLearned = 2 };
 
 struct TuningResult
 {
@@ -47,6 +52,11 @@ public:
 	// One non-graph replay with HIP events around every merged launch: summed milliseconds and
 	// launch counts per kernel kind (Renderer::LaunchKind numbering).
 	void profile(double *ms, uint64_t *launches, uint32_t kinds);
+	// QueueLearned needs a model (kept by the caller for the batch's lifetime).
+	bool set_schedule(Schedule s, const CostModel *model);
+	// One replay with the static work queue that measures every shading workgroup: features
+	// (PGS_MLP_INPUTS floats) and log2 shader cycles per item. Returns the item count.
+	uint32_t collect_costs(float *features, float *log2_costs, uint32_t max_items);
 	void sync();
 	hipStream_t stream() const { return stream_; }
 	uint32_t num_launches() const { return uint32_t(merged_.size()); }
@@ -62,6 +72,7 @@ private:
 		std::vector<Ref> refs;
 		size_t args_offset = 0, prefix_offset = 0;
 		uint32_t total = 0;
+		size_t sched_offset = 0; // shading launches: features, predictions, order, costs, counter
 	};
 
 	std::vector<Renderer *> contexts_;
@@ -73,9 +84,15 @@ private:
 	size_t dev_bytes_ = 0;
 	hipGraph_t graph_ = nullptr;
 	hipGraphExec_t graph_exec_ = nullptr;
+	Schedule schedule_ = Schedule::Hardware;
+	const CostModel *model_ = nullptr;
+	uint8_t *sched_ = nullptr;    // work-queue buffers of the shading launches
+	size_t sched_bytes_ = 0;
+	bool collecting_ = false;
 
 	bool upload();
 	void destroy_graph();
 	void launch(const Merged &m);
+	void launch_shade(const Merged &m);
 };
 }

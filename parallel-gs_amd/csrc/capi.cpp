@@ -3,6 +3,7 @@
 #include "pgs_amd.h"
 #include "gs_interface.hpp"
 #include "batch.hpp"
+#include "cost_model.hpp"
 #include <vector>
 #include <exception>
 #include <cstdio>
@@ -19,8 +20,29 @@ struct pgs_batch
 	pgs::Batch batch;
 };
 
+struct pgs_cost_model
+{
+	pgs::CostModel model;
+};
+
 namespace
 {
+template <typename T, typename F>
+int ptr_guard(T *p, F &&f)
+{
+	if (!p)
+		return -1;
+	try
+	{
+		return f();
+	}
+	catch (const std::exception &e)
+	{
+		fprintf(stderr, "pgs: %s\n", e.what());
+		return -1;
+	}
+}
+
 template <typename F>
 int batch_guard(pgs_batch *b, F &&f)
 {
@@ -303,6 +325,98 @@ int pgs_batch_profile(pgs_batch *batch, double *ms, uint64_t *launches, uint32_t
 			return -1;
 		batch->batch.profile(ms, launches, kinds);
 		return 0;
+	});
+}
+
+pgs_cost_model *pgs_cost_model_create(uint64_t seed)
+{
+	auto *m = new (std::nothrow) pgs_cost_model;
+	if (!m)
+		return nullptr;
+	try
+	{
+		m->model.init(seed);
+		return m;
+	}
+	catch (const std::exception &e)
+	{
+		fprintf(stderr, "pgs: %s\n", e.what());
+		delete m;
+		return nullptr;
+	}
+}
+
+void pgs_cost_model_destroy(pgs_cost_model *model) { delete model; }
+
+int pgs_cost_model_set_params(pgs_cost_model *model, const float *params)
+{
+	return ptr_guard(model, [&] {
+		if (!params)
+			return -1;
+		model->model.set_params(params);
+		return 0;
+	});
+}
+
+int pgs_cost_model_get_params(pgs_cost_model *model, float *params)
+{
+	return ptr_guard(model, [&] {
+		if (!params)
+			return -1;
+		model->model.get_params(params);
+		return 0;
+	});
+}
+
+int pgs_cost_model_predict(pgs_cost_model *model, const float *features, uint32_t count, float *out)
+{
+	return ptr_guard(model, [&] {
+		if (!features || !out)
+			return -1;
+		model->model.predict(features, count, out);
+		return 0;
+	});
+}
+
+int pgs_cost_model_gradients(pgs_cost_model *model, const float *features, const float *targets, uint32_t count,
+                             float *grads)
+{
+	return ptr_guard(model, [&] {
+		if (!features || !targets || !grads || count == 0)
+			return -1;
+		model->model.gradients(features, targets, count, grads);
+		return 0;
+	});
+}
+
+int pgs_cost_model_train(pgs_cost_model *model, const float *features, const float *targets, uint32_t count,
+                         uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *mse)
+{
+	return ptr_guard(model, [&] {
+		if (!features || !targets)
+			return -1;
+		const float e = model->model.train(features, targets, count, epochs, batch, lr, seed);
+		if (mse)
+			*mse = e;
+		return 0;
+	});
+}
+
+int pgs_batch_set_schedule(pgs_batch *batch, uint32_t schedule, const pgs_cost_model *model)
+{
+	return batch_guard(batch, [&] {
+		if (schedule > PGS_SCHEDULE_QUEUE_LEARNED)
+			return -1;
+		return batch->batch.set_schedule(pgs::Schedule(schedule), model ? &model->model : nullptr) ? 0 : -1;
+	});
+}
+
+int pgs_batch_collect_costs(pgs_batch *batch, float *features, float *log2_costs, uint32_t max_items)
+{
+	return batch_guard(batch, [&] {
+		if (!features || !log2_costs)
+			return -1;
+		return int(batch->batch.collect_costs(features, log2_costs, max_items));
 	});
 }
 }

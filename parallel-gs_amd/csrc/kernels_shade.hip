@@ -14,6 +14,7 @@
 #include "gs_raster.hpp"
 #include "pgs_kernels.h"
 #include "multi_launch.hpp"
+#include <algorithm>
 #include "multi_dispatch.hpp"
 
 namespace pgs
@@ -744,6 +745,170 @@ PGS_DEFINE_MULTI(256, k_shade_ssaa_01_multi, (shade_ssaa_body<0, 1>), pgs_shade_
 PGS_DEFINE_MULTI(256, k_shade_ssaa_11_multi, (shade_ssaa_body<1, 1>), pgs_shade_args)
 PGS_DEFINE_MULTI(256, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_args)
 PGS_DEFINE_MULTI(256, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
+
+// ---------------------------------------------------------------------------------------
+// Work-queue shading: a grid of resident workgroups pops work items (workgroup indices in the
+// numbering of a multi-launch) in `order` (identity when null) with one atomic counter, until
+// the queue is empty; every wave of a workgroup sees the same item, so all leave together.
+// With `costs` set, thread 0 records the item's shader-clock cycles.
+template <typename Body>
+__device__ __forceinline__ void queue_loop(const pgs_queue_args &q, Body body)
+{
+	__shared__ uint32_t slot;
+	const pgs_shade_args *args = static_cast<const pgs_shade_args *>(q.args);
+	for (;;)
+	{
+		if (threadIdx.x == 0)
+			slot = atomicAdd(q.counter, 1u);
+		__syncthreads();
+		const uint32_t pos = slot;
+		__syncthreads();
+		if (pos >= q.total)
+			return;
+		const uint32_t g = q.order ? q.order[pos] : pos;
+		const uint32_t l = find_launch(q.prefix, q.num_launches, g);
+		const uint64_t t0 = uint64_t(clock64());
+		body(args[l], g - q.prefix[l], q.prefix[l + 1] - q.prefix[l]);
+		__syncthreads();
+		if (q.costs && threadIdx.x == 0)
+		{
+			const uint64_t dt = uint64_t(clock64()) - t0;
+			q.costs[g] = dt > 0xffffffffull ? 0xffffffffu : uint32_t(dt);
+		}
+	}
+}
+
+#define PGS_QUEUE_KERNEL(NAME, BODY)                                                              \
+	__global__ void __launch_bounds__(256) NAME(pgs_queue_args q)                                \
+	{                                                                                            \
+		queue_loop(q, [](const pgs_shade_args &a, uint32_t w, uint32_t n) { BODY(a, w, n); });   \
+	}
+PGS_QUEUE_KERNEL(k_shade_1x_queue, shade_1x_body)
+PGS_QUEUE_KERNEL(k_shade_ssaa_01_queue, (shade_ssaa_body<0, 1>))
+PGS_QUEUE_KERNEL(k_shade_ssaa_11_queue, (shade_ssaa_body<1, 1>))
+PGS_QUEUE_KERNEL(k_shade_ssaa_12_queue, (shade_ssaa_body<1, 2>))
+PGS_QUEUE_KERNEL(k_shade_ssaa_22_queue, (shade_ssaa_body<2, 2>))
+
+// Pixel rectangle and coarse tile of shading workgroup `wg` (the first block it covers),
+// following the workgroup-to-block mapping of the shading bodies.
+__device__ __forceinline__ bool wg_block(const pgs_shade_args &a, uint32_t wg, uint32_t nwg, int32_t &x0, int32_t &y0,
+                                         int32_t &x1, int32_t &y1, uint32_t &tile)
+{
+	const uint32_t per_xcd = nwg / 8u;
+	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
+	const uint32_t waves = waves_of(a);
+	uint32_t bx0, by0, bx1, by1;
+	if (a.rate_log2 == 0)
+	{
+		const uint32_t gw = waves >= 2u ? 2u : 1u, gh = waves >= 4u ? 2u : 1u;
+		const uint32_t groups_x = (a.blocks_x + gw - 1u) / gw;
+		bx0 = (logical % groups_x) * gw;
+		by0 = (logical / groups_x) * gh;
+		bx1 = min(bx0 + gw - 1u, a.blocks_x - 1u);
+		by1 = min(by0 + gh - 1u, a.blocks_y - 1u);
+	}
+	else
+	{
+		const uint32_t S = 1u << (a.rate_x_log2 + a.rate_log2);
+		const uint32_t block = (logical * waves) / S;
+		bx0 = bx1 = block % a.blocks_x;
+		by0 = by1 = block / a.blocks_x;
+	}
+	if (by0 >= a.blocks_y || bx0 >= a.blocks_x)
+		return false;
+	x0 = a.origin_x + int32_t(bx0 * 8u);
+	y0 = a.origin_y + int32_t(by0 * 8u);
+	x1 = a.origin_x + int32_t(bx1 * 8u + 7u);
+	y1 = a.origin_y + int32_t(by1 * 8u + 7u);
+	const uint32_t shift = a.tile_log2 - 3u;
+	tile = (by0 >> shift) * a.tiles_x + (bx0 >> shift);
+	return true;
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+	for (int m = 32; m >= 1; m >>= 1)
+		v += __shfl_xor(v, m);
+	return v;
+}
+
+// Per-workgroup features of a multi-launch's shading work (one wave per workgroup item):
+// primitive counts, covered area and the state mix of the primitives whose boxes overlap the
+// workgroup's pixels. Layout: PGS_MLP_INPUTS floats per item.
+__global__ void __launch_bounds__(256) k_tile_features(pgs_features_args f)
+{
+	const uint32_t item = blockIdx.x * 4u + (threadIdx.x >> 6);
+	const uint32_t lane = threadIdx.x & 63u;
+	if (item >= f.total)
+		return;
+	const pgs_shade_args *args = static_cast<const pgs_shade_args *>(f.args);
+	const uint32_t l = find_launch(f.prefix, f.num_launches, item);
+	const pgs_shade_args &a = args[l];
+	float *out = f.features + size_t(item) * PGS_MLP_INPUTS;
+	int32_t x0, y0, x1, y1;
+	uint32_t tile;
+	const bool valid = wg_block(a, item - f.prefix[l], f.prefix[l + 1] - f.prefix[l], x0, y0, x1, y1, tile);
+	const uint32_t want = a.rate_log2 ? 2u : 1u;
+	uint32_t count = 0;
+	if (valid && (!a.tile_modes || a.tile_modes[tile] == want))
+		count = a.counts[tile];
+	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
+	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
+	const StateVector *states = static_cast<const StateVector *>(a.states);
+	const uint16_t *list = a.lists + size_t(valid ? tile : 0u) * a.list_stride;
+	float acc[16] = {};
+	for (uint32_t base = 0; base < count; base += 64u)
+	{
+		const uint32_t li = base + lane;
+		if (li >= count)
+			break;
+		const uint32_t idx = list[li];
+		const PrimitiveSetup &s = setups[idx];
+		const int32_t ix0 = max(s.bb[0], x0), iy0 = max(s.bb[1], y0), ix1 = min(s.bb[2], x1), iy1 = min(s.bb[3], y1);
+		if (ix1 < ix0 || iy1 < iy0)
+			continue;
+		const PrimitiveAttribute &pa = prims[idx];
+		const uint32_t st = pa.state;
+		const StateVector &sv = states[st & STATE_INDEX_MASK];
+		const float bbw = float(s.bb[2] - s.bb[0] + 1), bbh = float(s.bb[3] - s.bb[1] + 1);
+		acc[0] += 1.0f;
+		acc[1] += float((ix1 - ix0 + 1) * (iy1 - iy0 + 1));
+		acc[2] += (sv.combiner & COMB_TME) ? 1.0f : 0.0f;
+		acc[3] += ((sv.combiner & COMB_TME) && (pa.tex & TEX_MAG_LINEAR)) ? 1.0f : 0.0f;
+		acc[4] += (st & ST_PERSPECTIVE) ? 1.0f : 0.0f;
+		acc[5] += (st & ST_Z_TEST) ? 1.0f : 0.0f;
+		acc[6] += (st & ST_Z_WRITE) ? 1.0f : 0.0f;
+		acc[7] += (sv.blend_mode & BLEND_ABE) ? 1.0f : 0.0f;
+		acc[8] += (st & ST_SPRITE) ? 1.0f : 0.0f;
+		acc[9] += (st & ST_SNAP_RASTER) ? 1.0f : 0.0f;
+		acc[10] += (sv.blend_mode & BLEND_ATE) ? 1.0f : 0.0f;
+		acc[11] += (sv.combiner & COMB_FOG) ? 1.0f : 0.0f;
+		acc[12] += pa.fbmsk ? 1.0f : 0.0f;
+		acc[13] += (ix0 <= x0 && iy0 <= y0 && ix1 >= x1 && iy1 >= y1) ? 1.0f : 0.0f;
+		acc[14] += log2f(bbw * bbh);
+		acc[15] += (st & ST_PARALLELOGRAM) ? 1.0f : 0.0f;
+	}
+	for (uint32_t k = 0; k < 16; k++)
+		acc[k] = wave_sum(acc[k]);
+	if (lane != 0)
+		return;
+	const float hits = acc[0], inv = hits > 0.0f ? 1.0f / hits : 0.0f;
+	const float samples = float(1u << (a.rate_x_log2 + a.rate_log2));
+	out[0] = log2f(1.0f + float(count));
+	out[1] = log2f(1.0f + hits);
+	out[2] = log2f(1.0f + acc[1]);
+	out[3] = log2f(1.0f + acc[1] * samples);
+	for (uint32_t k = 2; k <= 13; k++)
+		out[2 + k] = acc[k] * inv;
+	out[16] = hits > 0.0f ? acc[14] * inv / 16.0f : 0.0f;
+	out[17] = acc[15] * inv;
+	out[18] = log2f(samples) / 4.0f;
+	out[19] = a.rate_log2 ? 1.0f : 0.0f;
+	out[20] = psm_is_16(a.fb_psm) ? 1.0f : 0.0f;
+	out[21] = a.z_psm != 0xffffffffu ? 1.0f : 0.0f;
+	out[22] = log2f(1.0f + float(a.list_stride)) / 16.0f;
+	out[23] = 1.0f;
+}
 }
 }
 
@@ -822,5 +987,49 @@ int pgs_multi_shade(const pgs_shade_args *args, const uint32_t *prefix, uint32_t
 		hipLaunchKernelGGL(pgs::k_shade_ssaa_22_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
 	else
 		return -int(hipErrorNotSupported);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+}
+
+// Grid of resident workgroups for the work-queue shading kernels.
+static uint32_t queue_grid(const void *kernel, uint32_t threads)
+{
+	int dev = 0, cus = 0, per_cu = 0;
+	(void)hipGetDevice(&dev);
+	(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, int(threads), 0);
+	return uint32_t(std::max(1, cus) * std::max(1, per_cu));
+}
+
+extern "C" int pgs_k_shade_queue(const pgs_queue_args *q, hipStream_t stream)
+{
+	if (!q || !q->args || !q->prefix || !q->counter || q->num_launches == 0)
+		return -int(hipErrorInvalidValue);
+	if (q->total == 0)
+		return 0;
+	const uint32_t waves = q->waves_per_group ? q->waves_per_group : 4u;
+	if (waves != 1 && waves != 2 && waves != 4)
+		return -int(hipErrorInvalidValue);
+	const uint32_t threads = 64u * waves;
+	const void *k;
+	if (q->rate_y == 0) k = reinterpret_cast<const void *>(pgs::k_shade_1x_queue);
+	else if (q->rate_x == 0 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_queue);
+	else if (q->rate_x == 1 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_queue);
+	else if (q->rate_x == 1 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_queue);
+	else if (q->rate_x == 2 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_queue);
+	else return -int(hipErrorNotSupported);
+	const uint32_t grid = std::min(q->total, queue_grid(k, threads));
+	void *params[] = { const_cast<pgs_queue_args *>(q) };
+	if (hipLaunchKernel(k, dim3(grid), dim3(threads), params, 0, stream) != hipSuccess)
+		return -int(hipErrorLaunchFailure);
+	return 0;
+}
+
+extern "C" int pgs_k_tile_features(const pgs_features_args *f, hipStream_t stream)
+{
+	if (!f || !f->args || !f->prefix || !f->features)
+		return -int(hipErrorInvalidValue);
+	if (f->total == 0)
+		return 0;
+	hipLaunchKernelGGL(pgs::k_tile_features, dim3((f->total + 3u) / 4u), dim3(256), 0, stream, *f);
 	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }

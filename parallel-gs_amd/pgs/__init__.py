@@ -73,6 +73,11 @@ EXPORTED_SYMBOLS = [
     "pgs_k_binning", "pgs_k_shade", "pgs_k_clut_write", "pgs_k_texture_decode", "pgs_k_ssaa_heuristic",
     "pgs_batch_create", "pgs_batch_destroy", "pgs_batch_apply_tuning", "pgs_batch_autotune", "pgs_batch_replay",
     "pgs_batch_sync", "pgs_batch_stream", "pgs_batch_num_launches", "pgs_batch_profile",
+    "pgs_cost_model_create", "pgs_cost_model_destroy", "pgs_cost_model_set_params", "pgs_cost_model_get_params",
+    "pgs_cost_model_predict", "pgs_cost_model_gradients", "pgs_cost_model_train",
+    "pgs_k_mlp_forward", "pgs_k_mlp_gradients", "pgs_k_mlp_adam", "pgs_k_mlp_quantize",
+    "pgs_batch_set_schedule", "pgs_batch_collect_costs", "pgs_k_shade_queue", "pgs_k_tile_features",
+    "pgs_k_queue_order", "pgs_k_fill_u32",
 ]
 
 
@@ -364,6 +369,24 @@ class Batch:
     def num_launches(self) -> int:
         return int(self._L.pgs_batch_num_launches(self._h))
 
+    SCHEDULES = {"hardware": 0, "static": 1, "learned": 2}
+
+    def set_schedule(self, name: str, model=None):
+        self._L.pgs_batch_set_schedule.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        self._model = model  # kept alive with the batch
+        if self._L.pgs_batch_set_schedule(self._h, self.SCHEDULES[name], model._h if model else None) != 0:
+            raise PgsError("pgs_batch_set_schedule failed")
+
+    def collect_costs(self, max_items: int = 1 << 22):
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._L.pgs_batch_collect_costs.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_uint32]
+        feats = np.zeros((max_items, MLP_INPUTS), np.float32)
+        costs = np.zeros(max_items, np.float32)
+        n = self._L.pgs_batch_collect_costs(self._h, feats.ctypes.data_as(fp), costs.ctypes.data_as(fp), max_items)
+        if n < 0:
+            raise PgsError("pgs_batch_collect_costs failed")
+        return feats[:n].copy(), costs[:n].copy()
+
     def profile(self, kinds: int = 16):
         """Per-kind (ms, launches) of one event-bracketed replay."""
         self._L.pgs_batch_profile.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
@@ -394,3 +417,79 @@ def spec_stream(spec) -> np.ndarray:
     """The GIF stream of one batched-context spec (one frame)."""
     return gen_textured(spec["seed"], spec["n_prims"], 1, spec["width"], spec["height"], spec["pareto_alpha"],
                         spec["tex_psm"], spec["blend"])
+
+
+MLP_INPUTS, MLP_PARAMS = 24, 5825
+
+
+class CostModel:
+    """The learned tile-cost MLP (parameters on the GPU; training runs on the GPU)."""
+
+    def __init__(self, seed: int = 1):
+        L = lib()
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.pgs_cost_model_create.restype = ctypes.c_void_p
+        L.pgs_cost_model_create.argtypes = [ctypes.c_uint64]
+        L.pgs_cost_model_destroy.argtypes = [ctypes.c_void_p]
+        L.pgs_cost_model_set_params.argtypes = [ctypes.c_void_p, fp]
+        L.pgs_cost_model_get_params.argtypes = [ctypes.c_void_p, fp]
+        L.pgs_cost_model_predict.argtypes = [ctypes.c_void_p, fp, ctypes.c_uint32, fp]
+        L.pgs_cost_model_gradients.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_uint32, fp]
+        L.pgs_cost_model_train.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_float, ctypes.c_uint64, fp]
+        self._L = L
+        self._h = L.pgs_cost_model_create(seed)
+        if not self._h:
+            raise PgsError("pgs_cost_model_create failed")
+
+    @staticmethod
+    def _f(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def close(self):
+        if self._h:
+            self._L.pgs_cost_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        a, ptr = self._f(p)
+        assert a.size == MLP_PARAMS
+        if self._L.pgs_cost_model_set_params(self._h, ptr) != 0:
+            raise PgsError("set_params failed")
+
+    def get_params(self):
+        out = np.zeros(MLP_PARAMS, np.float32)
+        if self._L.pgs_cost_model_get_params(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
+            raise PgsError("get_params failed")
+        return out
+
+    def predict(self, X):
+        x, xp = self._f(X)
+        n = x.shape[0]
+        out = np.zeros(n, np.float32)
+        if self._L.pgs_cost_model_predict(self._h, xp, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
+            raise PgsError("predict failed")
+        return out
+
+    def gradients(self, X, t):
+        x, xp = self._f(X)
+        tt, tp = self._f(t)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        if self._L.pgs_cost_model_gradients(self._h, xp, tp, x.shape[0], g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
+            raise PgsError("gradients failed")
+        return g
+
+    def train(self, X, t, epochs=10, batch=1024, lr=1e-3, seed=7):
+        x, xp = self._f(X)
+        tt, tp = self._f(t)
+        mse = ctypes.c_float(0.0)
+        if self._L.pgs_cost_model_train(self._h, xp, tp, x.shape[0], epochs, batch, lr, seed, ctypes.byref(mse)) != 0:
+            raise PgsError("train failed")
+        return float(mse.value)

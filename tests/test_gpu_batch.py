@@ -89,3 +89,27 @@ def test_replay_restores_vram(hip_device):
             ctx.replay(True)
         ctx.sync()
         assert np.array_equal(ctx.vram_planes(), live)
+
+
+def test_schedules_are_bit_exact(hip_device):
+    """Hardware dispatch, static work queue and learned work queue shade the same tiles: the
+    VRAM of every context matches its live run under each schedule."""
+    specs = pgs.batched_specs(0x5EED0006, 5, n_min=300, n_max=3000, ssaa_choices=(1, 2, 4, 16))
+    ctxs, live = make_contexts(specs)
+    batch = pgs.Batch(ctxs)
+    model = pgs.CostModel(4)
+    for name in ["static", "learned", "hardware", "learned"]:
+        batch.set_schedule(name, model if name == "learned" else None)
+        batch.replay(True)
+        batch.replay(True)
+        batch.sync()
+        for i, ctx in enumerate(ctxs):
+            assert np.array_equal(ctx.vram_planes(), live[i]), (name, i)
+    feats, costs = batch.collect_costs()
+    print("collected", feats.shape, "cost range", costs.min(), costs.max(), flush=True)
+    assert feats.shape[0] > 100 and np.all(np.isfinite(feats)) and np.all(costs > 0)
+    for i, ctx in enumerate(ctxs):
+        assert np.array_equal(ctx.vram_planes(), live[i])
+    batch.close()
+    for c in ctxs:
+        c.close()
