@@ -338,7 +338,131 @@ def bench_batched(args):
                                              "SSAA 1/2/4x, CT32/T8/T4 + CLUT, 4 blend presets", seed)
 
 
-WORKLOADS = {"transfers": bench_transfers, "plain": bench_plain, "textured": bench_textured, "batched": bench_batched}
+def record_contexts(specs):
+    import pgs
+
+    ctxs = []
+    for sp in specs:
+        ctx = pgs.Context(super_sampling=sp["ssaa"], arena_bytes=64 << 20)
+        ctx.begin_recording()
+        ctx.gif_transfer(pgs.spec_stream(sp))
+        ctx.end_recording()
+        ctxs.append(ctx)
+    return ctxs
+
+
+def spearman(a, b):
+    import numpy as np
+
+    ra = np.argsort(np.argsort(a)).astype(np.float64)
+    rb = np.argsort(np.argsort(b)).astype(np.float64)
+    ra -= ra.mean()
+    rb -= rb.mean()
+    return float((ra * rb).sum() / np.sqrt((ra * ra).sum() * (rb * rb).sum()))
+
+
+def bench_learned(args):
+    """16x SSAA, 64 contexts with Pareto(1.2) primitive areas: the tile-cost MLP is trained on
+    the GPU from costs measured on a separate training batch, then orders the work queue of
+    the evaluation batch; compared with the static queue, hardware dispatch and the sequential
+    single-stream replay. Pixels are compared across schedules."""
+    import numpy as np
+    import pgs
+
+    seed_train, seed = 0x5EED0015, 0x5EED0005
+    n_ctx = 64
+    common = dict(ssaa_choices=(16,), pareto_alpha=1.2, tex_psms=(0x13,), blend_presets=(0,))
+    t0 = time.perf_counter()
+    train_ctxs = record_contexts(pgs.batched_specs(seed_train, n_ctx, **common))
+    train_batch = pgs.Batch(train_ctxs)
+    feats, costs = train_batch.collect_costs()
+    train_batch.close()
+    for c in train_ctxs:
+        c.close()
+    rng = np.random.default_rng(1)
+    pick = rng.choice(feats.shape[0], size=min(200000, feats.shape[0]), replace=False)
+    model = pgs.CostModel(7)
+    t1 = time.perf_counter()
+    mse = model.train(feats[pick], costs[pick], epochs=8, batch=1024, lr=2e-3, seed=3)
+    train_s = time.perf_counter() - t1
+    log(f"collected {feats.shape[0]} workgroup costs, trained on {pick.size} in {train_s:.2f} s, mse {mse:.4f}, "
+        f"cost variance {float(np.var(costs[pick])):.4f}")
+
+    specs = pgs.batched_specs(seed, n_ctx, **common)
+    ctxs = record_contexts(specs)
+    prep_s = time.perf_counter() - t0
+
+    seq = pgs.Batch(ctxs, sequential=True)
+    ms_plain = time_batch(seq, args.steps, args.warmup)
+    seq.close()
+
+    batch = pgs.Batch(ctxs)
+    eval_feats, eval_costs = batch.collect_costs()
+    rho = spearman(model.predict(eval_feats), eval_costs)
+    times, planes = {}, {}
+    for name in ["hardware", "static", "learned"]:
+        batch.set_schedule(name, model if name == "learned" else None)
+        times[name] = time_batch(batch, args.steps, args.warmup)
+        batch.sync()
+        planes[name] = [ctxs[i].vram_planes() for i in (0, n_ctx // 2, n_ctx - 1)]
+    exact = all(np.array_equal(a, b) for name in ("static", "learned")
+                for a, b in zip(planes["hardware"], planes[name]))
+    batch.set_schedule("learned", model)
+    kms, kn = batch.profile()
+    prof = {KIND_NAMES[i]: (kms[i], kn[i]) for i in range(len(KIND_NAMES)) if kn[i]}
+    launches = batch.num_launches
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+    ms = times["learned"]
+    pixels = sum(sp["width"] * sp["height"] for sp in specs)
+    value = pixels / (ms / 1e3) / 1e6
+    k_ms, k_n = prof["shade_super"]
+    bytes_total = sum(sp["width"] * sp["height"] * (2 + sp["ssaa"]) * 16 + sp["n_prims"] * 144 for sp in specs)
+    achieved = bytes_total / k_n / (k_ms / k_n / 1e3) / 1e9
+
+    from oracle_lib import Oracle
+    cpu_s = 0.0
+    sp = specs[0]
+    o = Oracle(4)
+    st = pgs.spec_stream(sp)
+    t0 = time.perf_counter()
+    o.gif_transfer(st)
+    o.flush()
+    cpu_s = time.perf_counter() - t0
+    return {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpix/s",
+        "ms_per_step": round(ms, 4),
+        "dtype": "fp32+int (tile-cost MLP: bf16 weights, fp32 accumulation)",
+        "config": {"workload": "learned schedule: 64 contexts, 16x SSAA, Pareto(1.2) primitive areas, N log-uniform "
+                               "500..50000, PSMT8 + CLUT, (Cs-Cd)*As>>7+Cd", "seed": seed, "train_seed": seed_train,
+                   "contexts": n_ctx, "primitives": sum(s_["n_prims"] for s_ in specs), "merged_launches": launches},
+        "plain_single_stream": {"value": round(pixels / (ms_plain / 1e3) / 1e6, 2), "unit": "Mpix/s",
+                                "ms_per_step": round(ms_plain, 4)},
+        "speedup_vs_plain": round(ms_plain / ms, 3),
+        "schedules_ms": {k: round(v, 4) for k, v in times.items()},
+        "learned_gain_vs_static_queue": round(times["static"] / times["learned"], 4),
+        "learned_gain_vs_hardware": round(times["hardware"] / times["learned"], 4),
+        "schedules_bit_exact": bool(exact),
+        "mlp": {"train_samples": int(pick.size), "collected": int(feats.shape[0]), "train_s": round(train_s, 3),
+                "train_mse": round(mse, 5), "cost_var": round(float(np.var(costs[pick])), 5),
+                "eval_spearman": round(rho, 4)},
+        "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_shade_ssaa_22_queue",
+                     "note": "VALU-issue bound; byte roofline fraction is small by construction"},
+        "cpu_baseline": {"value": round(sp["width"] * sp["height"] / cpu_s / 1e6, 4), "unit": "Mpix/s", "cores": 1,
+                         "kind": "port", "sample": f"context 0 of the evaluation batch ({sp['n_prims']} primitives), "
+                                                   "single-threaded CPU oracle"},
+        "prep_s": round(prep_s, 2),
+    }
+
+
+WORKLOADS = {"transfers": bench_transfers, "plain": bench_plain, "textured": bench_textured, "batched": bench_batched,
+             "learned": bench_learned}
 
 
 def main():
