@@ -156,21 +156,22 @@ typedef struct pgs_shade_args
 
 int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);
 
-/* Work-queue shading of a multi-launch (batched renderer; no counterpart in the reference):
- * a grid of resident workgroups pops the multi-launch's workgroup indices in `order` (the
- * identity when NULL) with the atomic `counter` (0 on entry) and runs the shading body of
- * each. All launches share rate_x / rate_y / waves_per_group. With `costs` set, each item's
- * shader-clock cycles are stored at costs[item]. The order changes only which workgroup runs
- * when: every tile is shaded exactly once, so pixels do not depend on it. */
+/* Work-queue shading of a multi-launch by coarse tile (batched renderer; no counterpart in
+ * the reference): a grid of resident workgroups pops tiles (tile numbering: launch l owns
+ * tiles [prefix[l], prefix[l + 1])) from `order` with the atomic counter[0] (0 on entry) until
+ * counter[1] tiles are taken, and shades every block of each tile. All launches share rate_x /
+ * rate_y / waves_per_group. With `costs` set, each tile's shader-clock cycles are stored at
+ * costs[tile]. The order only decides which tile runs when: every tile is shaded exactly once
+ * by one workgroup, so pixels do not depend on it. */
 typedef struct pgs_queue_args
 {
 	const void *args;          /* device pgs_shade_args[num_launches] */
-	const uint32_t *prefix;    /* device workgroup prefix, num_launches + 1 entries */
-	const uint32_t *order;     /* device permutation of 0 .. total - 1, or NULL */
-	uint32_t *counter;         /* device counter, must be 0 */
+	const uint32_t *prefix;    /* device tile prefix, num_launches + 1 entries */
+	const uint32_t *order;     /* device list of tiles to shade (pgs_k_queue_order) */
+	uint32_t *counter;         /* device: [0] pop counter (0 on entry), [1] tiles in `order` */
 	uint32_t *costs;           /* device costs[total], or NULL */
 	uint32_t num_launches;
-	uint32_t total;
+	uint32_t total;            /* tiles over all launches */
 	uint32_t rate_x, rate_y;
 	uint32_t waves_per_group;
 	uint32_t pad;
@@ -178,8 +179,9 @@ typedef struct pgs_queue_args
 
 int pgs_k_shade_queue(const pgs_queue_args *args, hipStream_t stream);
 
-/* Features of each workgroup item of a shading multi-launch for the tile-cost model: one
- * wave per item scans the item's coarse-tile list. Writes PGS_MLP_INPUTS floats per item. */
+/* Features of every coarse tile of a shading multi-launch for the tile-cost model: one wave
+ * per tile scans the tile's list. Writes PGS_MLP_INPUTS floats per tile (all zero but the
+ * last, constant input, for tiles without work for this launch). */
 typedef struct pgs_features_args
 {
 	const void *args;          /* device pgs_shade_args[num_launches] */
@@ -191,14 +193,16 @@ typedef struct pgs_features_args
 
 int pgs_k_tile_features(const pgs_features_args *args, hipStream_t stream);
 
-/* Orders `count` items by predicted cost, largest first (256 buckets of 1/8 in log2 cycles;
- * order within a bucket is arbitrary), writes the permutation to `order` and zeroes
- * `counter` for the work-queue launch that follows. One workgroup. */
+/* Builds the queue of tiles with work (feature 0 nonzero): in tile order (`predicted` NULL)
+ * or by predicted cost, largest first (256 buckets of 1/8 in log2 cycles; order within a bucket
+ * is arbitrary). Writes the list to `order`, its length to counter[1] and zeroes counter[0]
+ * for the work-queue launch that follows. One workgroup. */
 typedef struct pgs_order_args
 {
-	const float *predicted;    /* log2 cycles per item */
+	const float *features;     /* count x PGS_MLP_INPUTS */
+	const float *predicted;    /* log2 cycles per tile, or NULL for tile order */
 	uint32_t *order;
-	uint32_t *counter;
+	uint32_t *counter;         /* [0], [1] as in pgs_queue_args */
 	uint32_t count;
 	uint32_t pad;
 } pgs_order_args;

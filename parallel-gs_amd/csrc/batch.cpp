@@ -197,7 +197,11 @@ bool Batch::apply(const BatchTuning &t)
 
 bool Batch::upload()
 {
-	// Layout: per merged launch, its argument blocks then its workgroup prefix.
+	// Layout: per merged launch, its argument blocks, its workgroup prefix and (shading) its
+	// coarse-tile prefix.
+	auto is_shade = [](const Merged &m) {
+		return m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper;
+	};
 	size_t bytes = 0;
 	for (auto &m : merged_)
 	{
@@ -205,6 +209,9 @@ bool Batch::upload()
 		bytes = align256(bytes + args_size(m.kind) * m.refs.size());
 		m.prefix_offset = bytes;
 		bytes = align256(bytes + sizeof(uint32_t) * (m.refs.size() + 1u));
+		m.tile_prefix_offset = bytes;
+		if (is_shade(m))
+			bytes = align256(bytes + sizeof(uint32_t) * (m.refs.size() + 1u));
 	}
 	std::vector<uint8_t> host(bytes);
 	for (auto &m : merged_)
@@ -221,6 +228,19 @@ bool Batch::upload()
 		}
 		prefix[m.refs.size()] = total;
 		m.total = total;
+		if (is_shade(m))
+		{
+			uint32_t *tp = reinterpret_cast<uint32_t *>(host.data() + m.tile_prefix_offset);
+			uint32_t tiles = 0;
+			for (size_t i = 0; i < m.refs.size(); i++)
+			{
+				const Renderer::Launch &l = launches_[m.refs[i].context][m.refs[i].index];
+				tp[i] = tiles;
+				tiles += l.shade.tiles_x * l.shade.tiles_y;
+			}
+			tp[m.refs.size()] = tiles;
+			m.tiles = tiles;
+		}
 		if (m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper)
 		{
 			const Renderer::Launch &l0 = launches_[m.refs[0].context][m.refs[0].index];
@@ -240,10 +260,10 @@ bool Batch::upload()
 	// Work-queue buffers per shading launch: features, predictions, order, costs, counter.
 	size_t sb = 0;
 	for (auto &m : merged_)
-		if (m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper)
+		if (is_shade(m))
 		{
 			m.sched_offset = sb;
-			sb = align256(sb + size_t(m.total) * (PGS_MLP_INPUTS * 4u + 12u) + 4u);
+			sb = align256(sb + size_t(m.tiles) * (PGS_MLP_INPUTS * 4u + 12u) + 8u);
 		}
 	if (sb > sched_bytes_)
 	{
@@ -281,45 +301,44 @@ void Batch::launch_shade(const Merged &m)
 	}
 	uint8_t *base = sched_ + m.sched_offset;
 	float *features = reinterpret_cast<float *>(base);
-	float *pred = features + size_t(m.total) * PGS_MLP_INPUTS;
-	uint32_t *order = reinterpret_cast<uint32_t *>(pred + m.total);
-	uint32_t *costs = order + m.total;
-	uint32_t *counter = costs + m.total;
+	float *pred = features + size_t(m.tiles) * PGS_MLP_INPUTS;
+	uint32_t *order = reinterpret_cast<uint32_t *>(pred + m.tiles);
+	uint32_t *costs = order + m.tiles;
+	uint32_t *counter = costs + m.tiles;
+	const uint32_t *tile_prefix = reinterpret_cast<const uint32_t *>(dev_ + m.tile_prefix_offset);
+	if (m.tiles == 0)
+		return;
 
+	// Features of every tile (the queue holds only tiles with work), the model's predictions,
+	// the queue, then the resident-workgroup shading.
+	pgs_features_args f{};
+	f.args = args;
+	f.prefix = tile_prefix;
+	f.features = features;
+	f.num_launches = n;
+	f.total = m.tiles;
+	if (pgs_k_tile_features(&f, stream_) != 0)
+		throw std::runtime_error("features launch failed");
 	const bool learned = schedule_ == Schedule::QueueLearned && !collecting_;
-	if (learned || collecting_)
-	{
-		pgs_features_args f{};
-		f.args = args;
-		f.prefix = prefix;
-		f.features = features;
-		f.num_launches = n;
-		f.total = m.total;
-		if (pgs_k_tile_features(&f, stream_) != 0)
-			throw std::runtime_error("features launch failed");
-	}
 	if (learned)
-	{
-		model_->predict_device(features, m.total, pred, stream_);
-		pgs_order_args o{};
-		o.predicted = pred;
-		o.order = order;
-		o.counter = counter;
-		o.count = m.total;
-		if (pgs_k_queue_order(&o, stream_) != 0)
-			throw std::runtime_error("order launch failed");
-	}
-	else if (pgs_k_fill_u32(counter, 1, 0, stream_) != 0)
-		throw std::runtime_error("fill launch failed");
+		model_->predict_device(features, m.tiles, pred, stream_);
+	pgs_order_args o{};
+	o.features = features;
+	o.predicted = learned ? pred : nullptr;
+	o.order = order;
+	o.counter = counter;
+	o.count = m.tiles;
+	if (pgs_k_queue_order(&o, stream_) != 0)
+		throw std::runtime_error("order launch failed");
 
 	pgs_queue_args q{};
 	q.args = args;
-	q.prefix = prefix;
-	q.order = learned ? order : nullptr;
+	q.prefix = tile_prefix;
+	q.order = order;
 	q.counter = counter;
 	q.costs = collecting_ ? costs : nullptr;
 	q.num_launches = n;
-	q.total = m.total;
+	q.total = m.tiles;
 	q.rate_x = m.rate_x;
 	q.rate_y = m.rate_y;
 	q.waves_per_group = tuning_.waves_per_group;
@@ -349,19 +368,23 @@ uint32_t Batch::collect_costs(float *features_out, float *log2_costs, uint32_t m
 	std::vector<uint32_t> c;
 	for (auto &m : merged_)
 	{
-		if (!(m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper) || m.total == 0)
+		if (!(m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper) || m.tiles == 0)
 			continue;
 		const uint8_t *base = sched_ + m.sched_offset;
-		f.resize(size_t(m.total) * PGS_MLP_INPUTS);
-		c.resize(m.total);
+		f.resize(size_t(m.tiles) * PGS_MLP_INPUTS);
+		c.resize(m.tiles);
 		ok(hipMemcpy(f.data(), base, f.size() * sizeof(float), hipMemcpyDeviceToHost));
-		ok(hipMemcpy(c.data(), base + size_t(m.total) * (PGS_MLP_INPUTS + 2u) * 4u, c.size() * 4u,
+		ok(hipMemcpy(c.data(), base + size_t(m.tiles) * (PGS_MLP_INPUTS + 2u) * 4u, c.size() * 4u,
 		             hipMemcpyDeviceToHost));
-		for (uint32_t i = 0; i < m.total && written < max_items; i++, written++)
+		// Only tiles that were shaded carry a measurement (feature 0 is their primitive count).
+		for (uint32_t i = 0; i < m.tiles && written < max_items; i++)
 		{
+			if (f[size_t(i) * PGS_MLP_INPUTS] == 0.0f)
+				continue;
 			memcpy(features_out + size_t(written) * PGS_MLP_INPUTS, &f[size_t(i) * PGS_MLP_INPUTS],
 			       PGS_MLP_INPUTS * sizeof(float));
 			log2_costs[written] = std::log2(float(std::max(c[i], 1u)));
+			written++;
 		}
 	}
 	return written;

@@ -72,7 +72,9 @@ private:
 		std::vector<Ref> refs;
 		size_t args_offset = 0, prefix_offset = 0;
 		uint32_t total = 0;
-		size_t sched_offset = 0; // shading launches: features, predictions, order, costs, counter
+		size_t sched_offset = 0;       // shading launches: features, predictions, order, costs, counter
+		size_t tile_prefix_offset = 0; // shading launches: coarse-tile prefix over the launches
+		uint32_t tiles = 0;
 	};
 
 	std::vector<Renderer *> contexts_;

@@ -407,19 +407,10 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 
 __device__ __forceinline__ uint32_t waves_of(const pgs_shade_args &a) { return a.waves_per_group ? a.waves_per_group : 4u; }
 
-__device__ __forceinline__ void shade_1x_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
+// One wave shades 8x8 block (bx, by) of a single-sampled pass (or single-sampled tile).
+__device__ __forceinline__ void shade_1x_block(const pgs_shade_args &a, uint32_t bx, uint32_t by)
 {
-	// XCD-aware remap: workgroup w runs on XCD (w % 8); give each XCD a contiguous range.
-	const uint32_t per_xcd = nwg / 8u;
-	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
-
-	// A workgroup of 1, 2 or 4 waves covers 1x1, 2x1 or 2x2 blocks of 8x8 pixels.
-	const uint32_t waves = waves_of(a);
-	const uint32_t gw = waves >= 2u ? 2u : 1u, gh = waves >= 4u ? 2u : 1u;
-	const uint32_t groups_x = (a.blocks_x + gw - 1u) / gw;
-	const uint32_t qx = logical % groups_x, qy = logical / groups_x;
-	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-	const uint32_t bx = qx * gw + (wave % gw), by = qy * gh + (wave / gw);
+	const uint32_t lane = threadIdx.x & 63u;
 	if (bx >= a.blocks_x || by >= a.blocks_y)
 		return;
 
@@ -527,6 +518,20 @@ __device__ __forceinline__ void shade_1x_body(const pgs_shade_args &a, uint32_t 
 	}
 }
 
+__device__ __forceinline__ void shade_1x_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
+{
+	// XCD-aware remap: workgroup w runs on XCD (w % 8); give each XCD a contiguous range.
+	const uint32_t per_xcd = nwg / 8u;
+	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
+	// A workgroup of 1, 2 or 4 waves covers 1x1, 2x1 or 2x2 blocks of 8x8 pixels.
+	const uint32_t waves = waves_of(a);
+	const uint32_t gw = waves >= 2u ? 2u : 1u, gh = waves >= 4u ? 2u : 1u;
+	const uint32_t groups_x = (a.blocks_x + gw - 1u) / gw;
+	const uint32_t qx = logical % groups_x, qy = logical / groups_x;
+	const uint32_t wave = threadIdx.x >> 6;
+	shade_1x_block(a, qx * gw + (wave % gw), qy * gh + (wave / gw));
+}
+
 // Hi-res offset of sample s inside its pixel: the lane-bit layout of ubershader.comp:133-385
 // (sample index = lane & (S - 1)), written out per sampling grid.
 template <uint32_t RX, uint32_t RY>
@@ -549,21 +554,17 @@ __device__ __forceinline__ void sample_offset(uint32_t s, int32_t &dx, int32_t &
 // Super-sampled tiles: one wave64 holds the S samples of 64 / S pixels (lane = pixel * S + s),
 // so the end-of-tile resolve is a butterfly over S consecutive lanes, summing in the
 // reference's xor order (8, 4, 2, 1) and taking the minimum depth (ubershader.comp:581-780).
+// One wave shades the 64 samples of sub-rectangle `sub` of 8x8 block (bx, by).
 template <uint32_t RX, uint32_t RY>
-__device__ __forceinline__ void shade_ssaa_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
+__device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t sub)
 {
 	constexpr uint32_t SL = RX + RY, S = 1u << SL;
 	constexpr uint32_t RW = SL <= 2 ? (SL == 1 ? 8u : 4u) : (SL == 3 ? 4u : 2u); // wave rectangle
 	constexpr uint32_t RH = (64u >> SL) / RW;
 	constexpr uint32_t SUB_X = 8u / RW;
 
-	const uint32_t per_xcd = nwg / 8u;
-	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
 	const uint32_t lane = threadIdx.x & 63u;
-	const uint32_t wave_global = logical * waves_of(a) + (threadIdx.x >> 6);
-	const uint32_t block = wave_global / S, sub = wave_global % S;
-	const uint32_t bx = block % a.blocks_x, by = block / a.blocks_x;
-	if (by >= a.blocks_y)
+	if (bx >= a.blocks_x || by >= a.blocks_y)
 		return;
 	const uint32_t shift = a.tile_log2 - 3u;
 	const uint32_t tile = (by >> shift) * a.tiles_x + (bx >> shift);
@@ -737,6 +738,17 @@ __device__ __forceinline__ void shade_ssaa_body(const pgs_shade_args &a, uint32_
 	}
 }
 
+template <uint32_t RX, uint32_t RY>
+__device__ __forceinline__ void shade_ssaa_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
+{
+	constexpr uint32_t S = 1u << (RX + RY);
+	const uint32_t per_xcd = nwg / 8u;
+	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
+	const uint32_t wave_global = logical * waves_of(a) + (threadIdx.x >> 6);
+	const uint32_t block = wave_global / S, sub = wave_global % S;
+	shade_ssaa_wave<RX, RY>(a, block % a.blocks_x, block / a.blocks_x, sub);
+}
+
 __global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a) { shade_1x_body(a, blockIdx.x, gridDim.x); }
 template <uint32_t RX, uint32_t RY>
 __global__ void __launch_bounds__(256) k_shade_ssaa(pgs_shade_args a) { shade_ssaa_body<RX, RY>(a, blockIdx.x, gridDim.x); }
@@ -747,15 +759,19 @@ PGS_DEFINE_MULTI(256, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_
 PGS_DEFINE_MULTI(256, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
 
 // ---------------------------------------------------------------------------------------
-// Work-queue shading: a grid of resident workgroups pops work items (workgroup indices in the
-// numbering of a multi-launch) in `order` (identity when null) with one atomic counter, until
-// the queue is empty; every wave of a workgroup sees the same item, so all leave together.
-// With `costs` set, thread 0 records the item's shader-clock cycles.
-template <typename Body>
-__device__ __forceinline__ void queue_loop(const pgs_queue_args &q, Body body)
+// Work-queue shading by coarse tile: a grid of resident workgroups pops coarse tiles (in the
+// tile numbering of a multi-launch: launch l owns tiles [prefix[l], prefix[l + 1])) in the
+// order `order` until counter[1] tiles have been taken; every wave of a workgroup sees the same
+// tile, so all leave together. The workgroup's waves walk the tile's blocks (and, super-
+// sampled, each block's S sample waves). With `costs` set, thread 0 records the tile's
+// shader-clock cycles.
+template <uint32_t S, typename WaveFn>
+__device__ __forceinline__ void tile_queue_loop(const pgs_queue_args &q, uint32_t want_mode, WaveFn fn)
 {
 	__shared__ uint32_t slot;
 	const pgs_shade_args *args = static_cast<const pgs_shade_args *>(q.args);
+	const uint32_t limit = q.counter[1];
+	const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6;
 	for (;;)
 	{
 		if (threadIdx.x == 0)
@@ -763,12 +779,24 @@ __device__ __forceinline__ void queue_loop(const pgs_queue_args &q, Body body)
 		__syncthreads();
 		const uint32_t pos = slot;
 		__syncthreads();
-		if (pos >= q.total)
+		if (pos >= limit)
 			return;
-		const uint32_t g = q.order ? q.order[pos] : pos;
+		const uint32_t g = q.order[pos];
 		const uint32_t l = find_launch(q.prefix, q.num_launches, g);
+		const pgs_shade_args &a = args[l];
+		const uint32_t t = g - q.prefix[l];
 		const uint64_t t0 = uint64_t(clock64());
-		body(args[l], g - q.prefix[l], q.prefix[l + 1] - q.prefix[l]);
+		const bool work = a.tile_modes ? a.tile_modes[t] == want_mode : a.counts[t] != 0;
+		if (work)
+		{
+			const uint32_t B = 1u << (a.tile_log2 - 3u);
+			const uint32_t tx = t % a.tiles_x, ty = t / a.tiles_x;
+			for (uint32_t w = wave; w < B * B * S; w += waves)
+			{
+				const uint32_t block = w / S;
+				fn(a, tx * B + block % B, ty * B + block / B, w % S);
+			}
+		}
 		__syncthreads();
 		if (q.costs && threadIdx.x == 0)
 		{
@@ -778,52 +806,20 @@ __device__ __forceinline__ void queue_loop(const pgs_queue_args &q, Body body)
 	}
 }
 
-#define PGS_QUEUE_KERNEL(NAME, BODY)                                                              \
-	__global__ void __launch_bounds__(256) NAME(pgs_queue_args q)                                \
-	{                                                                                            \
-		queue_loop(q, [](const pgs_shade_args &a, uint32_t w, uint32_t n) { BODY(a, w, n); });   \
-	}
-PGS_QUEUE_KERNEL(k_shade_1x_queue, shade_1x_body)
-PGS_QUEUE_KERNEL(k_shade_ssaa_01_queue, (shade_ssaa_body<0, 1>))
-PGS_QUEUE_KERNEL(k_shade_ssaa_11_queue, (shade_ssaa_body<1, 1>))
-PGS_QUEUE_KERNEL(k_shade_ssaa_12_queue, (shade_ssaa_body<1, 2>))
-PGS_QUEUE_KERNEL(k_shade_ssaa_22_queue, (shade_ssaa_body<2, 2>))
-
-// Pixel rectangle and coarse tile of shading workgroup `wg` (the first block it covers),
-// following the workgroup-to-block mapping of the shading bodies.
-__device__ __forceinline__ bool wg_block(const pgs_shade_args &a, uint32_t wg, uint32_t nwg, int32_t &x0, int32_t &y0,
-                                         int32_t &x1, int32_t &y1, uint32_t &tile)
+__global__ void __launch_bounds__(256) k_shade_1x_queue(pgs_queue_args q)
 {
-	const uint32_t per_xcd = nwg / 8u;
-	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
-	const uint32_t waves = waves_of(a);
-	uint32_t bx0, by0, bx1, by1;
-	if (a.rate_log2 == 0)
-	{
-		const uint32_t gw = waves >= 2u ? 2u : 1u, gh = waves >= 4u ? 2u : 1u;
-		const uint32_t groups_x = (a.blocks_x + gw - 1u) / gw;
-		bx0 = (logical % groups_x) * gw;
-		by0 = (logical / groups_x) * gh;
-		bx1 = min(bx0 + gw - 1u, a.blocks_x - 1u);
-		by1 = min(by0 + gh - 1u, a.blocks_y - 1u);
-	}
-	else
-	{
-		const uint32_t S = 1u << (a.rate_x_log2 + a.rate_log2);
-		const uint32_t block = (logical * waves) / S;
-		bx0 = bx1 = block % a.blocks_x;
-		by0 = by1 = block / a.blocks_x;
-	}
-	if (by0 >= a.blocks_y || bx0 >= a.blocks_x)
-		return false;
-	x0 = a.origin_x + int32_t(bx0 * 8u);
-	y0 = a.origin_y + int32_t(by0 * 8u);
-	x1 = a.origin_x + int32_t(bx1 * 8u + 7u);
-	y1 = a.origin_y + int32_t(by1 * 8u + 7u);
-	const uint32_t shift = a.tile_log2 - 3u;
-	tile = (by0 >> shift) * a.tiles_x + (bx0 >> shift);
-	return true;
+	tile_queue_loop<1>(q, 1u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t) { shade_1x_block(a, bx, by); });
 }
+#define PGS_SSAA_QUEUE_KERNEL(NAME, RX, RY)                                                         \
+	__global__ void __launch_bounds__(256) NAME(pgs_queue_args q)                                  \
+	{                                                                                              \
+		tile_queue_loop<(1u << (RX + RY))>(q, 2u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, \
+		                                               uint32_t sub) { shade_ssaa_wave<RX, RY>(a, bx, by, sub); }); \
+	}
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_01_queue, 0, 1)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_11_queue, 1, 1)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_12_queue, 1, 2)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_22_queue, 2, 2)
 
 __device__ __forceinline__ float wave_sum(float v)
 {
@@ -832,9 +828,10 @@ __device__ __forceinline__ float wave_sum(float v)
 	return v;
 }
 
-// Per-workgroup features of a multi-launch's shading work (one wave per workgroup item):
-// primitive counts, covered area and the state mix of the primitives whose boxes overlap the
-// workgroup's pixels. Layout: PGS_MLP_INPUTS floats per item.
+// Per-tile features of a shading multi-launch's work for the tile-cost model (one wave per
+// coarse tile): primitive count, covered area and the state mix of the tile's primitives.
+// Tiles without work for this launch (no primitives, or the other shading mode) get all-zero
+// features except the constant input. Layout: PGS_MLP_INPUTS floats per tile.
 __global__ void __launch_bounds__(256) k_tile_features(pgs_features_args f)
 {
 	const uint32_t item = blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -844,18 +841,20 @@ __global__ void __launch_bounds__(256) k_tile_features(pgs_features_args f)
 	const pgs_shade_args *args = static_cast<const pgs_shade_args *>(f.args);
 	const uint32_t l = find_launch(f.prefix, f.num_launches, item);
 	const pgs_shade_args &a = args[l];
+	const uint32_t t = item - f.prefix[l];
 	float *out = f.features + size_t(item) * PGS_MLP_INPUTS;
-	int32_t x0, y0, x1, y1;
-	uint32_t tile;
-	const bool valid = wg_block(a, item - f.prefix[l], f.prefix[l + 1] - f.prefix[l], x0, y0, x1, y1, tile);
 	const uint32_t want = a.rate_log2 ? 2u : 1u;
-	uint32_t count = 0;
-	if (valid && (!a.tile_modes || a.tile_modes[tile] == want))
-		count = a.counts[tile];
+	const bool work = a.tile_modes ? a.tile_modes[t] == want : a.counts[t] != 0;
+	const uint32_t count = work ? a.counts[t] : 0u;
+	const uint32_t B = 1u << (a.tile_log2 - 3u);
+	const uint32_t tx = t % a.tiles_x, ty = t / a.tiles_x;
+	const uint32_t bx1 = min((tx + 1u) * B, a.blocks_x), by1 = min((ty + 1u) * B, a.blocks_y);
+	const int32_t x0 = a.origin_x + int32_t(tx * B * 8u), y0 = a.origin_y + int32_t(ty * B * 8u);
+	const int32_t x1 = a.origin_x + int32_t(bx1 * 8u) - 1, y1 = a.origin_y + int32_t(by1 * 8u) - 1;
 	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
 	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
 	const StateVector *states = static_cast<const StateVector *>(a.states);
-	const uint16_t *list = a.lists + size_t(valid ? tile : 0u) * a.list_stride;
+	const uint16_t *list = a.lists + size_t(t) * a.list_stride;
 	float acc[16] = {};
 	for (uint32_t base = 0; base < count; base += 64u)
 	{
@@ -892,8 +891,16 @@ __global__ void __launch_bounds__(256) k_tile_features(pgs_features_args f)
 		acc[k] = wave_sum(acc[k]);
 	if (lane != 0)
 		return;
+	if (!work)
+	{
+		for (uint32_t k = 0; k < PGS_MLP_INPUTS - 1u; k++)
+			out[k] = 0.0f;
+		out[PGS_MLP_INPUTS - 1u] = 1.0f;
+		return;
+	}
 	const float hits = acc[0], inv = hits > 0.0f ? 1.0f / hits : 0.0f;
 	const float samples = float(1u << (a.rate_x_log2 + a.rate_log2));
+	const float tile_px = float((x1 - x0 + 1) * (y1 - y0 + 1));
 	out[0] = log2f(1.0f + float(count));
 	out[1] = log2f(1.0f + hits);
 	out[2] = log2f(1.0f + acc[1]);
@@ -903,7 +910,7 @@ __global__ void __launch_bounds__(256) k_tile_features(pgs_features_args f)
 	out[16] = hits > 0.0f ? acc[14] * inv / 16.0f : 0.0f;
 	out[17] = acc[15] * inv;
 	out[18] = log2f(samples) / 4.0f;
-	out[19] = a.rate_log2 ? 1.0f : 0.0f;
+	out[19] = log2f(tile_px * samples) / 16.0f;
 	out[20] = psm_is_16(a.fb_psm) ? 1.0f : 0.0f;
 	out[21] = a.z_psm != 0xffffffffu ? 1.0f : 0.0f;
 	out[22] = log2f(1.0f + float(a.list_stride)) / 16.0f;
@@ -1002,7 +1009,7 @@ static uint32_t queue_grid(const void *kernel, uint32_t threads)
 
 extern "C" int pgs_k_shade_queue(const pgs_queue_args *q, hipStream_t stream)
 {
-	if (!q || !q->args || !q->prefix || !q->counter || q->num_launches == 0)
+	if (!q || !q->args || !q->prefix || !q->counter || !q->order || q->num_launches == 0)
 		return -int(hipErrorInvalidValue);
 	if (q->total == 0)
 		return 0;
