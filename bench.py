@@ -19,6 +19,11 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "shaded Mpix/s + frames/s at bit-exact VRAM; batched/learned-sched vs plain"
 HBM_PEAK_GBS = 8000.0
+# L2-to-fabric bytes per launch of each workload's dominant kernel, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by what the counter reads for a
+# 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the runs are summarised in
+# profiles/hbm_traffic_pmc__plain_and_textured4x.md.
+MEASURED_TRAFFIC = {"plain": 6829670, "textured": 20671590}
 
 
 def log(msg):
@@ -136,7 +141,8 @@ def kernel_profile(ctx):
     return {names[i]: (kms[i], kn[i]) for i in range(min(len(names), len(kn))) if kn[i]}
 
 
-def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames_stream=None, cpu_frames=1):
+def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames_stream=None, cpu_frames=1,
+                 traffic=None):
     """Times graph replays of a recorded rendering stream and reports Mpix/s of output."""
     import torch
     import pgs
@@ -188,7 +194,7 @@ def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in prof.items()},
         "kernel_launches": {k: v[1] for k, v in prof.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "kernel": "k_shade_ssaa" if kname == "shade_super" else "k_shade_1x",
                      "note": "VALU-issue bound; byte roofline fraction is small by construction"},
         "cpu_baseline": {"value": round(w * h * cpu_frames / cpu_s / 1e6, 3), "unit": "Mpix/s", "cores": 1,
@@ -206,7 +212,8 @@ def bench_plain(args):
     seed = 0x5EED0001
     stream = pgs.gen_plain(seed, 2000, 640, 448)
     return bench_render(args, stream, 1, 1, 2000,
-                        "plain 1x: 2000 Gouraud triangles, 640x448 PSMCT32 + PSMZ24, ZTST=GEQUAL", seed)
+                        "plain 1x: 2000 Gouraud triangles, 640x448 PSMCT32 + PSMZ24, ZTST=GEQUAL", seed,
+                        traffic=MEASURED_TRAFFIC["plain"])
 
 
 def bench_textured(args):
@@ -219,7 +226,7 @@ def bench_textured(args):
     return bench_render(args, stream, 4, 8, 10000,
                         "textured 4x SSAA: 8 frames x 10000 prims (70% tri / 30% sprite), PSMT8 256x256 + "
                         "CT32 CLUT, bilinear MODULATE, (Cs-Cd)*As>>7+Cd, GEQUAL, 20% ST/Q",
-                        seed, cpu_frames_stream=sample, cpu_frames=1)
+                        seed, cpu_frames_stream=sample, cpu_frames=1, traffic=MEASURED_TRAFFIC["textured"])
 
 
 KIND_NAMES = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode",
@@ -461,8 +468,37 @@ def bench_learned(args):
     }
 
 
+def bench_copy(args):
+    """HBM calibration: device-to-device copies of 256 MiB (a plain copy kernel of known size,
+    used to scale the FETCH_SIZE / WRITE_SIZE counters of the rendering kernels)."""
+    import torch
+
+    n = 256 << 20
+    src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2 * n / (ms / 1e3) / 1e9
+    return {
+        "metric": "device copy GB/s (calibration)", "value": round(gbs, 2), "unit": "GB/s", "ms_per_step": round(ms, 4),
+        "dtype": "u8", "config": {"workload": "copy 256 MiB device to device", "bytes_read": n, "bytes_written": n},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": 2 * n, "kernel": "torch copy"},
+        "cpu_baseline": None,
+    }
+
+
 WORKLOADS = {"transfers": bench_transfers, "plain": bench_plain, "textured": bench_textured, "batched": bench_batched,
-             "learned": bench_learned}
+             "learned": bench_learned, "copy": bench_copy}
 
 
 def main():
