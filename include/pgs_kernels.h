@@ -73,6 +73,16 @@ typedef struct pgs_setup_args
 	uint32_t rate_log2;      /* log2 of the super-sampling grid per axis (0 at 1x) */
 	uint32_t opaque_fbmask;  /* FBMSK every primitive of the pass shares */
 	uint32_t z_bucket_shift; /* depth >> shift = bucket: 8, 16, 24 for 16, 24, 32-bit depth */
+	/* Optional first level of binning (NULL: none). coarse_mask[ct * mask_words + w] bit b is set
+	 * when primitive 64 w + b has a bounding box touching coarse tile ct, a square of
+	 * 2^coarse_log2 pixels of the grid at (coarse_origin_x, coarse_origin_y), row-major
+	 * coarse_x by coarse_y; mask_words = ceil(num_primitives / 64). Has no counterpart in the
+	 * reference, whose binning tests every primitive against every tile. */
+	uint64_t *coarse_mask;
+	int32_t coarse_origin_x, coarse_origin_y;
+	uint32_t coarse_log2;
+	uint32_t coarse_x, coarse_y;
+	uint32_t mask_words;
 } pgs_setup_args;
 
 int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stream);
@@ -114,7 +124,13 @@ typedef struct pgs_bin_args
 	uint32_t tiles_x, tiles_y;
 	uint32_t super_sampled;  /* pass is super-sampled: decide per tile (binning.comp:87-125) */
 	uint32_t *tile_modes;    /* per tile: 0 = no work, 1 = single-sampled, 2 = super-sampled */
-	uint32_t pad2[2];
+	/* Optional coarse mask from triangle setup (same grid origin; a coarse tile is
+	 * 2^coarse_shift by 2^coarse_shift tiles): only primitives with their bit set are tested.
+	 * NULL: every primitive is tested. The lists are the same either way. */
+	const uint64_t *coarse_mask;
+	uint32_t coarse_shift;
+	uint32_t coarse_x, coarse_y;
+	uint32_t mask_words;
 } pgs_bin_args;
 
 int pgs_k_binning(const pgs_bin_args *args, hipStream_t stream);
@@ -167,9 +183,12 @@ typedef struct pgs_queue_args
 {
 	const void *args;          /* device pgs_shade_args[num_launches] */
 	const uint32_t *prefix;    /* device tile prefix, num_launches + 1 entries */
-	const uint32_t *order;     /* device list of tiles to shade (pgs_k_queue_order) */
-	uint32_t *counter;         /* device: [0] pop counter (0 on entry), [1] tiles in `order` */
-	uint32_t *costs;           /* device costs[total], or NULL */
+	const uint32_t *order;     /* device queue (pgs_k_queue_order): entry = tile << 8 | part << 4 |
+	                              (parts - 1); the item covers part `part` of `parts` equal ranges
+	                              of the tile's waves */
+	uint32_t *counter;         /* device: [0] pop counter (0 on entry), [1] entries in `order` */
+	uint32_t *costs;           /* device costs[total] (shader clocks added per item; cleared by the
+	                              caller), or NULL */
 	uint32_t num_launches;
 	uint32_t total;            /* tiles over all launches */
 	uint32_t rate_x, rate_y;
@@ -201,9 +220,11 @@ typedef struct pgs_order_args
 {
 	const float *features;     /* count x PGS_MLP_INPUTS */
 	const float *predicted;    /* log2 cycles per tile, or NULL for tile order */
-	uint32_t *order;
+	uint32_t *order;           /* capacity entries */
 	uint32_t *counter;         /* [0], [1] as in pgs_queue_args */
-	uint32_t count;
+	uint32_t count;            /* tiles, below 2^24 */
+	uint32_t slots;            /* resident workgroups of the queue launch */
+	uint32_t capacity;         /* at least count + 2 * slots + 1024 */
 	uint32_t pad;
 } pgs_order_args;
 

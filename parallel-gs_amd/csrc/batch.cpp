@@ -61,6 +61,9 @@ uint64_t key_of(const Renderer::Launch &l)
 }
 
 size_t align256(size_t v) { return (v + 255u) & ~size_t(255); }
+// Queue entries of a shading launch: its tiles plus what cost-based splitting may add (at most
+// 2 * slots, and a launch has far fewer than 65536 resident workgroups).
+uint32_t order_capacity(uint32_t tiles) { return tiles + 2u * 65536u + 1024u; }
 }
 
 Batch::~Batch()
@@ -155,11 +158,13 @@ bool Batch::apply(const BatchTuning &t)
 	for (size_t c = 0; c < contexts_.size(); c++)
 	{
 		out[c] = contexts_[c]->recorded();
-		uint32_t tl = 0, tx = 0, ty = 0;
+		uint32_t tl = 0, tx = 0, ty = 0, coarse_log2 = 0;
 		bool retiled = false;
 		for (auto &l : out[c])
 		{
-			if (l.kind == K::Binning)
+			if (l.kind == K::Setup)
+				coarse_log2 = l.setup.coarse_log2;
+			else if (l.kind == K::Binning)
 			{
 				retiled = false;
 				if (t.tile_log2_1x != 0 && !l.bin.super_sampled)
@@ -175,6 +180,13 @@ bool Batch::apply(const BatchTuning &t)
 					l.bin.tile_log2 = tl;
 					l.bin.tiles_x = tx;
 					l.bin.tiles_y = ty;
+					// The coarse mask keeps the grid of its setup (same origin, coarse tiles of
+					// 2^coarse_log2 pixels); the new tiles still nest in it while they are not
+					// larger, otherwise every primitive is tested.
+					if (l.bin.coarse_mask && coarse_log2 >= tl)
+						l.bin.coarse_shift = coarse_log2 - tl;
+					else
+						l.bin.coarse_mask = nullptr;
 					retiled = true;
 				}
 			}
@@ -257,13 +269,15 @@ bool Batch::upload()
 	}
 	ok(hipMemcpy(dev_, host.data(), bytes, hipMemcpyHostToDevice));
 
-	// Work-queue buffers per shading launch: features, predictions, order, costs, counter.
+	// Work-queue buffers per shading launch: features, predictions, costs, counter, then the
+	// queue, which holds a tile's split items too (at most tiles + 2 * slots entries).
 	size_t sb = 0;
 	for (auto &m : merged_)
 		if (is_shade(m))
 		{
 			m.sched_offset = sb;
-			sb = align256(sb + size_t(m.tiles) * (PGS_MLP_INPUTS * 4u + 12u) + 8u);
+			sb = align256(sb + size_t(m.tiles) * (PGS_MLP_INPUTS * 4u + 8u) + 8u +
+			              size_t(order_capacity(m.tiles)) * 4u);
 		}
 	if (sb > sched_bytes_)
 	{
@@ -302,12 +316,14 @@ void Batch::launch_shade(const Merged &m)
 	uint8_t *base = sched_ + m.sched_offset;
 	float *features = reinterpret_cast<float *>(base);
 	float *pred = features + size_t(m.tiles) * PGS_MLP_INPUTS;
-	uint32_t *order = reinterpret_cast<uint32_t *>(pred + m.tiles);
-	uint32_t *costs = order + m.tiles;
+	uint32_t *costs = reinterpret_cast<uint32_t *>(pred + m.tiles);
 	uint32_t *counter = costs + m.tiles;
+	uint32_t *order = counter + 2;
 	const uint32_t *tile_prefix = reinterpret_cast<const uint32_t *>(dev_ + m.tile_prefix_offset);
 	if (m.tiles == 0)
 		return;
+	if (m.tiles >= (1u << 24))
+		throw std::runtime_error("work-queue shading: too many tiles");
 
 	// Features of every tile (the queue holds only tiles with work), the model's predictions,
 	// the queue, then the resident-workgroup shading.
@@ -328,8 +344,12 @@ void Batch::launch_shade(const Merged &m)
 	o.order = order;
 	o.counter = counter;
 	o.count = m.tiles;
+	o.slots = pgs_queue_slots(m.rate_x, m.rate_y, tuning_.waves_per_group, m.tiles);
+	o.capacity = order_capacity(m.tiles);
 	if (pgs_k_queue_order(&o, stream_) != 0)
 		throw std::runtime_error("order launch failed");
+	if (collecting_ && pgs_k_fill_u32(costs, m.tiles, 0u, stream_) != 0)
+		throw std::runtime_error("cost clear failed");
 
 	pgs_queue_args q{};
 	q.args = args;
@@ -374,7 +394,7 @@ uint32_t Batch::collect_costs(float *features_out, float *log2_costs, uint32_t m
 		f.resize(size_t(m.tiles) * PGS_MLP_INPUTS);
 		c.resize(m.tiles);
 		ok(hipMemcpy(f.data(), base, f.size() * sizeof(float), hipMemcpyDeviceToHost));
-		ok(hipMemcpy(c.data(), base + size_t(m.tiles) * (PGS_MLP_INPUTS + 2u) * 4u, c.size() * 4u,
+		ok(hipMemcpy(c.data(), base + size_t(m.tiles) * (PGS_MLP_INPUTS + 1u) * 4u, c.size() * 4u,
 		             hipMemcpyDeviceToHost));
 		// Only tiles that were shaded carry a measurement (feature 0 is their primitive count).
 		for (uint32_t i = 0; i < m.tiles && written < max_items; i++)

@@ -78,8 +78,12 @@ struct TransformedAttributes
 	float stqf0[4], stqf1[4], stqf2[4];
 	uint32_t rgba[3];
 	uint32_t pad;
+	// The vertex colours as unorm floats, written once by setup for the shader's interpolation:
+	// c0 = rgba[0] / 255 and the differences c1 - c0, c2 - c0, per channel, each rounded as the
+	// shader's own single-precision operations would round it.
+	float col0[4], dcol1[4], dcol2[4];
 };
-static_assert(sizeof(TransformedAttributes) == 64, "TransformedAttributes layout");
+static_assert(sizeof(TransformedAttributes) == 112, "TransformedAttributes layout");
 
 // One cached texture of a render pass: a decoded RGBA8 image plus sampling parameters
 // (gs/shaders/data_structures.h:98-105 TexInfo content, gs/gs_interface.cpp:1754-1781).

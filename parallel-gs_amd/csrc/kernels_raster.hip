@@ -83,11 +83,8 @@ template <typename T> __device__ __forceinline__ void swp(T &a, T &b) { T t = a;
 
 // ---------------------------------------------------------------------------------------
 // Triangle setup (gs/shaders/triangle_setup.comp:127-873), one thread per primitive.
-__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t)
+__device__ __forceinline__ void triangle_setup_one(const pgs_setup_args &args, uint32_t index)
 {
-	const uint32_t index = wg * 64u + threadIdx.x;
-	if (index >= args.num_primitives)
-		return;
 	const VertexPosition *pos = static_cast<const VertexPosition *>(args.positions) + 3u * index;
 	const VertexAttribute *attr = static_cast<const VertexAttribute *>(args.attributes) + 3u * index;
 	PrimitiveAttribute &prim = static_cast<PrimitiveAttribute *>(args.prims)[index];
@@ -293,6 +290,14 @@ __device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, 
 		ta.rgba[0] = flat; ta.rgba[1] = flat; ta.rgba[2] = flat;
 	}
 	ta.pad = 0;
+	for (uint32_t ch = 0; ch < 4; ch++)
+	{
+		auto unorm = [&](uint32_t c) { return __fdiv_rn(float((c >> (8u * ch)) & 0xffu), 255.0f); };
+		const float c0 = unorm(ta.rgba[0]);
+		ta.col0[ch] = c0;
+		ta.dcol1[ch] = __fsub_rn(unorm(ta.rgba[1]), c0);
+		ta.dcol2[ch] = __fsub_rn(unorm(ta.rgba[2]), c0);
+	}
 
 	const int32_t pa0 = int32_t((uint32_t(dadx) << 6) | uint32_t(parallelogram) | (uint32_t(flip_b & 3) << 2) |
 	                            (uint32_t(flip_c & 3) << 4));
@@ -396,6 +401,48 @@ __device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, 
 	prim.tex = tex;
 	if (args.state_copy)
 		args.state_copy[index] = state;
+}
+
+// First level of the binning hierarchy: for every coarse tile (a square of 2^coarse_log2
+// pixels on the binning grid), bit `lane` of word `wg` says whether primitive wg * 64 + lane
+// has a bounding box that touches the tile. The bounding-box test is the first test of the
+// exact per-tile test, and a fine tile lies inside its coarse tile, so every primitive binned
+// to a fine tile has its bit set. Every word is written, so the mask needs no clearing.
+__device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uint32_t wg, uint32_t index)
+{
+	int32_t cx0 = 1, cy0 = 1, cx1 = 0, cy1 = 0; // empty range
+	if (index < args.num_primitives)
+	{
+		const PrimitiveSetup &s = static_cast<const PrimitiveSetup *>(args.setup)[index];
+		if (s.bb[2] >= s.bb[0] && s.bb[3] >= s.bb[1])
+		{
+			const uint32_t l = args.coarse_log2;
+			cx0 = max((s.bb[0] - args.coarse_origin_x) >> l, 0);
+			cy0 = max((s.bb[1] - args.coarse_origin_y) >> l, 0);
+			cx1 = min((s.bb[2] - args.coarse_origin_x) >> l, int32_t(args.coarse_x) - 1);
+			cy1 = min((s.bb[3] - args.coarse_origin_y) >> l, int32_t(args.coarse_y) - 1);
+		}
+	}
+	uint64_t *mask = args.coarse_mask + wg;
+	for (uint32_t cy = 0; cy < args.coarse_y; cy++)
+	{
+		const bool row = int32_t(cy) >= cy0 && int32_t(cy) <= cy1;
+		for (uint32_t cx = 0; cx < args.coarse_x; cx++)
+		{
+			const uint64_t bits = __ballot(row && int32_t(cx) >= cx0 && int32_t(cx) <= cx1);
+			if (threadIdx.x == 0)
+				mask[size_t(cy * args.coarse_x + cx) * args.mask_words] = bits;
+		}
+	}
+}
+
+__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t)
+{
+	const uint32_t index = wg * 64u + threadIdx.x;
+	if (index < args.num_primitives)
+		triangle_setup_one(args, index);
+	if (args.coarse_mask)
+		coarse_mask_wave(args, wg, index);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -569,8 +616,16 @@ __device__ __forceinline__ bool intersects_tile(const PrimitiveSetup &s, int32_t
 }
 
 // ---------------------------------------------------------------------------------------
-// Coarse binning: one workgroup (4 waves) per coarse tile. Primitives are tested 256 at a time;
-// hits are compacted in ascending order with wave ballots and an LDS exchange of wave counts.
+// Binning: one workgroup (4 waves) per tile. Candidates are tested 256 at a time; hits are
+// compacted in ascending primitive order with wave ballots and an LDS exchange of wave counts.
+// With a coarse mask (written by triangle setup) the candidates are the primitives whose bit
+// is set in the tile's coarse tile, taken window by window: each thread owns one 64-bit mask
+// word, a block-wide prefix sum of the word popcounts places the set bits' indices in LDS in
+// ascending order, and only those primitives are tested. Without one, every primitive is a
+// candidate. Both give the same lists: the coarse bit is set for every primitive that passes
+// the exact test.
+constexpr uint32_t kBinWindowWords = 256;
+
 __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t wg, uint32_t)
 {
 	const uint32_t tile = wg;
@@ -584,16 +639,17 @@ __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t 
 
 	__shared__ uint32_t wave_counts[4];
 	__shared__ uint32_t wave_flags[4];
+	__shared__ uint32_t window_counts[4];
+	__shared__ uint16_t candidates[kBinWindowWords * 64];
 	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 	uint32_t written = 0;
 	// binning.comp:87-125: flags of the primitives binned to this tile.
 	bool req = false, req_zw = false, has_zw = false;
 
-	for (uint32_t base = 0; base < args.num_primitives; base += 256u)
-	{
-		const uint32_t i = base + threadIdx.x;
+	// Tests primitive i (when `valid`) and appends the workgroup's hits; ends on a barrier.
+	auto test_chunk = [&](bool valid, uint32_t i) {
 		bool hit = false;
-		if (i < args.num_primitives)
+		if (valid)
 		{
 			const uint32_t st = prims[i].state;
 			const bool ms_tri = (st & ST_MULTISAMPLE) && !(st & ST_LINE);
@@ -623,6 +679,51 @@ __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t 
 		}
 		written += total;
 		__syncthreads();
+	};
+
+	if (!args.coarse_mask)
+	{
+		for (uint32_t base = 0; base < args.num_primitives; base += 256u)
+			test_chunk(base + threadIdx.x < args.num_primitives, base + threadIdx.x);
+	}
+	else
+	{
+		const uint32_t ctile = (ty >> args.coarse_shift) * args.coarse_x + (tx >> args.coarse_shift);
+		const uint64_t *cmask = args.coarse_mask + size_t(ctile) * args.mask_words;
+		for (uint32_t win = 0; win < args.mask_words; win += kBinWindowWords)
+		{
+			const uint32_t w = win + threadIdx.x;
+			uint64_t m = w < args.mask_words ? cmask[w] : 0ull;
+			const uint32_t c = uint32_t(__popcll(m));
+			uint32_t incl = c;
+			for (uint32_t d = 1; d < 64u; d <<= 1)
+			{
+				const uint32_t t = __shfl_up(incl, d, 64);
+				if (lane >= d)
+					incl += t;
+			}
+			if (lane == 63)
+				window_counts[wave] = incl;
+			__syncthreads();
+			uint32_t pos = incl - c, total = 0;
+			for (uint32_t q = 0; q < 4; q++)
+			{
+				pos += q < wave ? window_counts[q] : 0u;
+				total += window_counts[q];
+			}
+			while (m)
+			{
+				candidates[pos++] = uint16_t(w * 64u + uint32_t(__builtin_ctzll(m)));
+				m &= m - 1u;
+			}
+			__syncthreads();
+			for (uint32_t base = 0; base < total; base += 256u)
+			{
+				const uint32_t j = base + threadIdx.x;
+				test_chunk(j < total, j < total ? uint32_t(candidates[j]) : 0u);
+			}
+			__syncthreads(); // window_counts and candidates are rewritten by the next window
+		}
 	}
 	if (args.tile_modes)
 	{
@@ -657,6 +758,9 @@ extern "C" int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stre
 {
 	if (!args || !args->positions || !args->attributes || !args->prims || !args->setup || !args->tattr)
 		return -int(hipErrorInvalidValue);
+	if (args->coarse_mask && (args->coarse_x == 0 || args->coarse_y == 0 || args->coarse_log2 > 16 ||
+	                          args->mask_words != (args->num_primitives + 63u) / 64u))
+		return -int(hipErrorInvalidValue);
 	if (args->num_primitives == 0)
 		return 0;
 	hipLaunchKernelGGL(pgs::k_triangle_setup, dim3((args->num_primitives + 63u) / 64u), dim3(64), 0, stream, *args);
@@ -682,6 +786,11 @@ extern "C" int pgs_k_binning(const pgs_bin_args *args, hipStream_t stream)
 	if (!args || !args->setup || !args->prims || !args->lists || !args->counts)
 		return -int(hipErrorInvalidValue);
 	if (args->num_primitives > 65536u || args->list_stride < args->num_primitives)
+		return -int(hipErrorInvalidValue);
+	if (args->coarse_mask && args->tiles_x && args->tiles_y &&
+	    (args->coarse_shift > 16 || args->mask_words != (args->num_primitives + 63u) / 64u ||
+	     ((args->tiles_x - 1u) >> args->coarse_shift) >= args->coarse_x ||
+	     ((args->tiles_y - 1u) >> args->coarse_shift) >= args->coarse_y))
 		return -int(hipErrorInvalidValue);
 	const uint32_t tiles = args->tiles_x * args->tiles_y;
 	if (tiles == 0)

@@ -29,7 +29,25 @@ __device__ __forceinline__ int32_t idot3(int32_t e0, int32_t e1, int32_t e2, int
 	return wadd(wadd(wmul(e0, x), wmul(e1, y)), e2);
 }
 
-__device__ __forceinline__ float unorm8(uint32_t c, uint32_t ch) { return __fdiv_rn(float((c >> (8 * ch)) & 0xffu), 255.0f); }
+// b / 255 correctly rounded for every byte (the constant folder divides in IEEE single
+// precision, round to nearest even): for wave-uniform bytes the lookup is a scalar load where
+// the division would be ten vector instructions.
+struct Unorm8Table
+{
+	float v[256];
+};
+constexpr Unorm8Table make_unorm8_table()
+{
+	Unorm8Table t{};
+	for (int i = 0; i < 256; i++)
+		t.v[i] = float(i) / 255.0f;
+	return t;
+}
+__constant__ Unorm8Table c_unorm8 = make_unorm8_table();
+__device__ __forceinline__ float unorm8(uint32_t c, uint32_t ch) { return c_unorm8.v[(c >> (8 * ch)) & 0xffu]; }
+// rint(unorm8(b) * 255) == b and unorm8(b) * 255 == b hold for all 256 bytes (exhaustive), so
+// the frame buffer and fog colour channels are the bytes themselves.
+__device__ __forceinline__ float byte_f(uint32_t c, uint32_t ch) { return float((c >> (8 * ch)) & 0xffu); }
 
 __device__ __forceinline__ uint32_t pack_unorm(float r, float g, float b, float a)
 {
@@ -250,10 +268,10 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	float comb[4], col[4];
 	for (uint32_t ch = 0; ch < 4; ch++)
 	{
-		const float c0 = unorm8(ta.rgba[0], ch), c1 = unorm8(ta.rgba[1], ch), c2 = unorm8(ta.rgba[2], ch);
-		const float e1 = __fmul_rn(__fsub_rn(c1, c0), fi);
-		const float e2 = __fmul_rn(__fsub_rn(c2, c0), fj);
-		const float sum = __fadd_rn(__fadd_rn(c0, e1), e2);
+		// c0 + (c1 - c0) * i + (c2 - c0) * j with the per-primitive terms from setup.
+		const float e1 = __fmul_rn(ta.dcol1[ch], fi);
+		const float e2 = __fmul_rn(ta.dcol2[ch], fj);
+		const float sum = __fadd_rn(__fadd_rn(ta.col0[ch], e1), e2);
 		col[ch] = floorf(__fadd_rn(__fmul_rn(255.0f, sum), 1.0f / 256.0f));
 		comb[ch] = col[ch];
 	}
@@ -304,7 +322,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		const float fog = rintf(__fadd_rn(__fadd_rn(ta.stqf0[3], e1), e2));
 		for (uint32_t k = 0; k < 3; k++)
 		{
-			const float fc = __fmul_rn(unorm8(pa.fogcol, k), 255.0f);
+			const float fc = byte_f(pa.fogcol, k);
 			const float d = __fmul_rn(__fsub_rn(comb[k], fc), fog);
 			comb[k] = floorf(__fadd_rn(__fadd_rn(fc, __fmul_rn(d, 1.0f / 256.0f)), 0.5f));
 		}
@@ -313,7 +331,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	// shade_resolve (ubershader.comp:1464-1607).
 	float fb[4];
 	for (uint32_t k = 0; k < 4; k++)
-		fb[k] = rintf(__fmul_rn(unorm8(color, k), 255.0f));
+		fb[k] = byte_f(color, k);
 	bool write_fb = true, write_zb = true, write_alpha = true;
 	const uint32_t bm = sv.blend_mode;
 	bool blending = bm & BLEND_ABE;
@@ -374,21 +392,18 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	{
 		o[0] = fb[0]; o[1] = fb[1]; o[2] = fb[2];
 	}
-	float n[4];
-	if (!(bm & BLEND_COLCLAMP))
+	// Every o[k] is an integer-valued float of magnitude below 4096 (each path ends in floor,
+	// rint, a byte or a sum of such). On those values the float wrap-or-clamp and pack of
+	// shade_resolve, rint(clamp(fract(o / 256) * (256 / 255), 0, 1) * 255) and
+	// rint(clamp(o * (1 / 255), 0, 1) * 255), equal int(o) & 255 and clamp(int(o), 0, 255)
+	// (exhaustive over -4096..4096).
+	uint32_t packed = 0;
+	for (int k = 0; k < 4; k++)
 	{
-		for (int k = 0; k < 4; k++)
-		{
-			const float q = __fdiv_rn(o[k], 256.0f);
-			n[k] = __fmul_rn(__fsub_rn(q, floorf(q)), 256.0f / 255.0f);
-		}
+		const int32_t i = int32_t(o[k]);
+		const uint32_t c = (bm & BLEND_COLCLAMP) ? uint32_t(min(max(i, 0), 255)) : uint32_t(i) & 0xffu;
+		packed |= c << (8 * k);
 	}
-	else
-	{
-		for (int k = 0; k < 4; k++)
-			n[k] = __fmul_rn(o[k], 1.0f / 255.0f);
-	}
-	uint32_t packed = pack_unorm(n[0], n[1], n[2], n[3]);
 	packed |= uint32_t((bm & BLEND_FBA) && write_alpha) << 31;
 
 	if (pa.fbmsk != ~0u && (write_fb || write_alpha))
@@ -678,7 +693,7 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 		for (uint32_t k = 0; k < 4; k++)
 			f[k] = __fadd_rn(f[k], __shfl_xor(f[k], int(m)));
 	for (uint32_t k = 0; k < 4; k++)
-		f[k] = __fdiv_rn(f[k], float(S));
+		f[k] = __fmul_rn(f[k], 1.0f / float(S)); // S is a power of two: exact, as the division
 	const uint32_t resolved = pack_unorm(f[0], f[1], f[2], f[3]);
 	uint32_t dmin = depth;
 	for (uint32_t m = S >> 1; m >= 1u; m >>= 1)
@@ -759,30 +774,37 @@ PGS_DEFINE_MULTI(256, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_
 PGS_DEFINE_MULTI(256, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
 
 // ---------------------------------------------------------------------------------------
-// Work-queue shading by coarse tile: a grid of resident workgroups pops coarse tiles (in the
-// tile numbering of a multi-launch: launch l owns tiles [prefix[l], prefix[l + 1])) in the
-// order `order` until counter[1] tiles have been taken; every wave of a workgroup sees the same
-// tile, so all leave together. The workgroup's waves walk the tile's blocks (and, super-
-// sampled, each block's S sample waves). With `costs` set, thread 0 records the tile's
-// shader-clock cycles.
+// Work-queue shading by coarse tile: a grid of resident workgroups pops queue items (coarse
+// tiles in the tile numbering of a multi-launch, where launch l owns tiles [prefix[l],
+// prefix[l + 1]), or equal parts of a tile's waves) in the order `order` until counter[1]
+// items have been taken; every wave of a workgroup sees the same item, so all leave together.
+// Inside an item the waves take the tile's waves (blocks, and super-sampled each block's S
+// sample waves) one at a time from an LDS counter, so the workgroup waits only for the last
+// one. With `costs` set, thread 0 adds the item's shader-clock cycles to its tile's count.
 template <uint32_t S, typename WaveFn>
-__device__ __forceinline__ void tile_queue_loop(const pgs_queue_args &q, uint32_t want_mode, WaveFn fn)
+__device__ __forceinline__ void tile_queue_loop(const pgs_queue_args &q, const pgs_shade_args *__restrict__ args,
+                                                uint32_t want_mode, WaveFn fn)
 {
-	__shared__ uint32_t slot;
-	const pgs_shade_args *args = static_cast<const pgs_shade_args *>(q.args);
-	const uint32_t limit = q.counter[1];
-	const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6;
+	__shared__ uint32_t slot, next;
+	// Item numbers, launch indices and wave numbers are wave-uniform: readfirstlane keeps them
+	// (and every address derived from them) in scalar registers.
+	const uint32_t limit = uni(q.counter[1]);
+	const uint32_t lane = threadIdx.x & 63u;
 	for (;;)
 	{
 		if (threadIdx.x == 0)
+		{
 			slot = atomicAdd(q.counter, 1u);
+			next = 0;
+		}
 		__syncthreads();
-		const uint32_t pos = slot;
+		const uint32_t pos = uni(slot);
 		__syncthreads();
 		if (pos >= limit)
 			return;
-		const uint32_t g = q.order[pos];
-		const uint32_t l = find_launch(q.prefix, q.num_launches, g);
+		const uint32_t e = uni(q.order[pos]);
+		const uint32_t g = e >> 8, part = (e >> 4) & 15u, parts = (e & 15u) + 1u;
+		const uint32_t l = uni(find_launch(q.prefix, q.num_launches, g));
 		const pgs_shade_args &a = args[l];
 		const uint32_t t = g - q.prefix[l];
 		const uint64_t t0 = uint64_t(clock64());
@@ -791,8 +813,15 @@ __device__ __forceinline__ void tile_queue_loop(const pgs_queue_args &q, uint32_
 		{
 			const uint32_t B = 1u << (a.tile_log2 - 3u);
 			const uint32_t tx = t % a.tiles_x, ty = t / a.tiles_x;
-			for (uint32_t w = wave; w < B * B * S; w += waves)
+			const uint32_t W = B * B * S, begin = W * part / parts, end = W * (part + 1u) / parts;
+			for (;;)
 			{
+				uint32_t k = 0;
+				if (lane == 0)
+					k = atomicAdd(&next, 1u);
+				const uint32_t w = begin + uni(k);
+				if (w >= end)
+					break;
 				const uint32_t block = w / S;
 				fn(a, tx * B + block % B, ty * B + block / B, w % S);
 			}
@@ -801,19 +830,23 @@ __device__ __forceinline__ void tile_queue_loop(const pgs_queue_args &q, uint32_
 		if (q.costs && threadIdx.x == 0)
 		{
 			const uint64_t dt = uint64_t(clock64()) - t0;
-			q.costs[g] = dt > 0xffffffffull ? 0xffffffffu : uint32_t(dt);
+			atomicAdd(&q.costs[g], dt > 0xffffffffull ? 0xffffffffu : uint32_t(dt));
 		}
 	}
 }
 
-__global__ void __launch_bounds__(256) k_shade_1x_queue(pgs_queue_args q)
+// The argument blocks come in as a restrict kernel parameter: with it the compiler may load
+// their fields through the scalar cache inside the loop, although the loop stores to VRAM.
+__global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 k_shade_1x_queue(pgs_queue_args q,
+                                                                        const pgs_shade_args *__restrict__ args)
 {
-	tile_queue_loop<1>(q, 1u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t) { shade_1x_block(a, bx, by); });
+	tile_queue_loop<1>(q, args, 1u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t) { shade_1x_block(a, bx, by); });
 }
 #define PGS_SSAA_QUEUE_KERNEL(NAME, RX, RY)                                                         \
-	__global__ void __launch_bounds__(256) NAME(pgs_queue_args q)                                  \
+	__global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 NAME(pgs_queue_args q,                  \
+	                                                            const pgs_shade_args *__restrict__ args) \
 	{                                                                                              \
-		tile_queue_loop<(1u << (RX + RY))>(q, 2u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, \
+		tile_queue_loop<(1u << (RX + RY))>(q, args, 2u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, \
 		                                               uint32_t sub) { shade_ssaa_wave<RX, RY>(a, bx, by, sub); }); \
 	}
 PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_01_queue, 0, 1)
@@ -1007,6 +1040,25 @@ static uint32_t queue_grid(const void *kernel, uint32_t threads)
 	return uint32_t(std::max(1, cus) * std::max(1, per_cu));
 }
 
+static const void *queue_kernel(uint32_t rate_x, uint32_t rate_y)
+{
+	if (rate_y == 0) return reinterpret_cast<const void *>(pgs::k_shade_1x_queue);
+	if (rate_x == 0 && rate_y == 1) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_queue);
+	if (rate_x == 1 && rate_y == 1) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_queue);
+	if (rate_x == 1 && rate_y == 2) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_queue);
+	if (rate_x == 2 && rate_y == 2) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_queue);
+	return nullptr;
+}
+
+uint32_t pgs_queue_slots(uint32_t rate_x, uint32_t rate_y, uint32_t waves_per_group, uint32_t total)
+{
+	const void *k = queue_kernel(rate_x, rate_y);
+	const uint32_t waves = waves_per_group ? waves_per_group : 4u;
+	if (!k || (waves != 1 && waves != 2 && waves != 4))
+		return 0;
+	return std::min(total, queue_grid(k, 64u * waves));
+}
+
 extern "C" int pgs_k_shade_queue(const pgs_queue_args *q, hipStream_t stream)
 {
 	if (!q || !q->args || !q->prefix || !q->counter || !q->order || q->num_launches == 0)
@@ -1016,17 +1068,13 @@ extern "C" int pgs_k_shade_queue(const pgs_queue_args *q, hipStream_t stream)
 	const uint32_t waves = q->waves_per_group ? q->waves_per_group : 4u;
 	if (waves != 1 && waves != 2 && waves != 4)
 		return -int(hipErrorInvalidValue);
-	const uint32_t threads = 64u * waves;
-	const void *k;
-	if (q->rate_y == 0) k = reinterpret_cast<const void *>(pgs::k_shade_1x_queue);
-	else if (q->rate_x == 0 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_queue);
-	else if (q->rate_x == 1 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_queue);
-	else if (q->rate_x == 1 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_queue);
-	else if (q->rate_x == 2 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_queue);
-	else return -int(hipErrorNotSupported);
-	const uint32_t grid = std::min(q->total, queue_grid(k, threads));
-	void *params[] = { const_cast<pgs_queue_args *>(q) };
-	if (hipLaunchKernel(k, dim3(grid), dim3(threads), params, 0, stream) != hipSuccess)
+	const void *k = queue_kernel(q->rate_x, q->rate_y);
+	if (!k)
+		return -int(hipErrorNotSupported);
+	const uint32_t grid = pgs_queue_slots(q->rate_x, q->rate_y, waves, q->total);
+	const pgs_shade_args *shade_args = static_cast<const pgs_shade_args *>(q->args);
+	void *params[] = { const_cast<pgs_queue_args *>(q), &shade_args };
+	if (hipLaunchKernel(k, dim3(grid), dim3(64u * waves), params, 0, stream) != hipSuccess)
 		return -int(hipErrorLaunchFailure);
 	return 0;
 }

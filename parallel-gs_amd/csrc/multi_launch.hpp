@@ -23,10 +23,15 @@ __device__ __forceinline__ uint32_t find_launch(const uint32_t *prefix, uint32_t
 }
 }
 
+// Register budget for 8 waves per SIMD (64 VGPRs). Without it the shading multi-launch
+// kernels take 80 VGPRs (6 waves per SIMD); at 64 they do not spill, and the extra waves hide
+// more of the memory latency of the per-primitive loads.
+#define PGS_OCCUPANCY_8 __attribute__((amdgpu_waves_per_eu(8)))
+
 // Defines `NAME(const ARGS *args, const uint32_t *prefix, uint32_t n)` running BODY for the
 // launch that owns each workgroup.
 #define PGS_DEFINE_MULTI(THREADS, NAME, BODY, ARGS)                                               \
-	__global__ void __launch_bounds__(THREADS) NAME(const ARGS *args, const uint32_t *prefix, uint32_t n) \
+	__global__ void __launch_bounds__(THREADS) PGS_OCCUPANCY_8 NAME(const ARGS *args, const uint32_t *prefix, uint32_t n) \
 	{                                                                                               \
 		const uint32_t l = ::pgs::find_launch(prefix, n, blockIdx.x);                               \
 		BODY(args[l], blockIdx.x - prefix[l], prefix[l + 1] - prefix[l]);                            \

@@ -53,7 +53,7 @@ Renderer::~Renderer()
 		(void)hipEventDestroy(e);
 	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_,
 	                 (void *)clut_buf_, (void *)tex_images_, (void *)state_copy_, (void *)flat_buckets_,
-	                 (void *)tile_modes_, (void *)snapshot_ })
+	                 (void *)tile_modes_, (void *)coarse_mask_, (void *)snapshot_ })
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
@@ -99,6 +99,7 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes)
 		check(hipMalloc(&flat_buckets_, 8u * sizeof(uint32_t)));
 		check(hipMemsetAsync(flat_buckets_, 0, 8u * sizeof(uint32_t), stream_));
 		check(hipMalloc(&tile_modes_, counts_cap_ * sizeof(uint32_t)));
+		check(hipMalloc(&coarse_mask_, size_t(kMaxCoarseTiles) * kMaxMaskWords * sizeof(uint64_t)));
 		{
 			float lut[256];
 			make_rcp_float_lut(lut);
@@ -520,6 +521,14 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 		tiles_for(tile_log2, tiles_x, tiles_y);
 	}
 
+	// First binning level: coarse tiles of 4 x 4 tiles, larger where the grid would need more
+	// than kMaxCoarseTiles of them.
+	uint32_t coarse_shift = 2;
+	while (((((tiles_x - 1u) >> coarse_shift) + 1u) * (((tiles_y - 1u) >> coarse_shift) + 1u)) > kMaxCoarseTiles)
+		coarse_shift++;
+	const uint32_t coarse_x = ((tiles_x - 1u) >> coarse_shift) + 1u, coarse_y = ((tiles_y - 1u) >> coarse_shift) + 1u;
+	const uint32_t mask_words = (n + 63u) / 64u;
+
 	const size_t bytes_pos = size_t(n) * 3u * sizeof(VertexPosition);
 	const size_t bytes_attr = size_t(n) * 3u * sizeof(VertexAttribute);
 	const size_t bytes_prim = size_t(n) * sizeof(PrimitiveAttribute);
@@ -574,6 +583,13 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 		ls.setup.flat_buckets = flat_buckets_;
 		ls.setup.z_bucket_shift = (depth_psm == PSMZ16 || depth_psm == PSMZ16S) ? 8u : depth_psm == PSMZ32 ? 24u : 16u;
 	}
+	ls.setup.coarse_mask = coarse_mask_;
+	ls.setup.coarse_origin_x = ox;
+	ls.setup.coarse_origin_y = oy;
+	ls.setup.coarse_log2 = tile_log2 + coarse_shift;
+	ls.setup.coarse_x = coarse_x;
+	ls.setup.coarse_y = coarse_y;
+	ls.setup.mask_words = mask_words;
 	launch(ls);
 
 	if (super_pass)
@@ -615,6 +631,11 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	lb.bin.tiles_y = tiles_y;
 	lb.bin.super_sampled = super_pass ? 1u : 0u;
 	lb.bin.tile_modes = super_pass ? tile_modes_ : nullptr;
+	lb.bin.coarse_mask = coarse_mask_;
+	lb.bin.coarse_shift = coarse_shift;
+	lb.bin.coarse_x = coarse_x;
+	lb.bin.coarse_y = coarse_y;
+	lb.bin.mask_words = mask_words;
 	launch(lb);
 
 	Launch lsh{};

@@ -134,6 +134,8 @@ private:
 	uint32_t *state_copy_ = nullptr;  // super-sampled passes: post-setup states
 	uint32_t *flat_buckets_ = nullptr; // super-sampled passes: 256-bit bucket mask
 	uint32_t *tile_modes_ = nullptr;  // super-sampled passes: per coarse tile
+	uint64_t *coarse_mask_ = nullptr; // first binning level: kMaxCoarseTiles x kMaxMaskWords
+	static constexpr uint32_t kMaxCoarseTiles = 256, kMaxMaskWords = 1024;
 	uint8_t *snapshot_ = nullptr;     // VRAM planes + CLUT at begin_recording()
 	size_t lists_cap_ = 0;            // entries
 	size_t counts_cap_ = 0;

@@ -133,21 +133,6 @@ void packed_xyz2(Stream &s, uint32_t x, uint32_t y, uint32_t z)
 	s.q.push_back(uint64_t(z));
 }
 
-void packed_uv(Stream &s, uint32_t u, uint32_t v)
-{
-	s.q.push_back(uint64_t(u & 0x3fff) | (uint64_t(v & 0x3fff) << 32));
-	s.q.push_back(0);
-}
-
-void packed_st(Stream &s, float S, float T, float Q)
-{
-	uint32_t a, b, c;
-	memcpy(&a, &S, 4);
-	memcpy(&b, &T, 4);
-	memcpy(&c, &Q, 4);
-	s.q.push_back(uint64_t(a) | (uint64_t(b) << 32));
-	s.q.push_back(uint64_t(c));
-}
 
 uint64_t frame_reg(uint32_t fbp, uint32_t fbw, uint32_t psm, uint32_t fbmsk)
 {

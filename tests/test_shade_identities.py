@@ -1,0 +1,48 @@
+"""Exhaustive checks of the float identities the shading kernels use in place of the reference's
+float sequences (ubershader.comp shade_resolve and the unorm unpack of colours), evaluated in
+IEEE single precision with numpy.
+
+* frame buffer and fog colour: rint((b / 255) * 255) == b and (b / 255) * 255 == b for every
+  byte b, so the kernels use the byte itself;
+* the colour write: for integer-valued o in -4096..4096,
+  rint(clamp(fract(o / 256) * (256 / 255), 0, 1) * 255) == o & 255 (wrap) and
+  rint(clamp(o * (1 / 255), 0, 1) * 255) == clamp(o, 0, 255) (COLCLAMP);
+* division by a power of two equals multiplication by its reciprocal;
+* b / 255 is not b * fl(1 / 255) for every byte, which is why the kernels keep an exact table.
+"""
+import numpy as np
+
+F = np.float32
+
+
+def unorm8(b):
+    return np.divide(b.astype(F), F(255))
+
+
+def test_frame_buffer_and_fog_bytes_round_trip():
+    b = np.arange(256, dtype=F)
+    assert np.array_equal(np.rint((unorm8(b) * F(255)).astype(F)), b)
+    assert np.array_equal((unorm8(b) * F(255)).astype(F), b)
+
+
+def test_colour_write_wrap_and_clamp_are_integer_ops():
+    o = np.arange(-4096, 4097).astype(F)
+    q = (o / F(256)).astype(F)
+    n = ((q - np.floor(q)).astype(F) * (F(256) / F(255))).astype(F)
+    wrap = np.rint((np.clip(n, F(0), F(1)) * F(255)).astype(F)).astype(np.int64)
+    assert np.array_equal(wrap, o.astype(np.int64) & 255)
+    n2 = (o * (F(1) / F(255))).astype(F)
+    clamp = np.rint((np.clip(n2, F(0), F(1)) * F(255)).astype(F)).astype(np.int64)
+    assert np.array_equal(clamp, np.clip(o.astype(np.int64), 0, 255))
+
+
+def test_power_of_two_division_is_reciprocal_multiplication():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([np.arange(-70000, 70000).astype(F), rng.standard_normal(100000).astype(F) * F(300)])
+    for s in (2, 4, 8, 16, 256):
+        assert np.array_equal((x / F(s)).astype(F), (x * F(1.0 / s)).astype(F))
+
+
+def test_reciprocal_multiplication_is_not_exact_for_255():
+    b = np.arange(256, dtype=F)
+    assert not np.array_equal(unorm8(b), (b * (F(1) / F(255))).astype(F))
