@@ -407,12 +407,12 @@ def bench_learned(args):
     eval_feats, eval_costs = batch.collect_costs()
     rho = spearman(model.predict(eval_feats), eval_costs)
     times, planes = {}, {}
-    for name in ["hardware", "static", "learned"]:
-        batch.set_schedule(name, model if name == "learned" else None)
+    for name in ["hardware", "static", "learned", "dispatch_static", "dispatch_learned"]:
+        batch.set_schedule(name, model if "learned" in name else None)
         times[name] = time_batch(batch, args.steps, args.warmup)
         batch.sync()
         planes[name] = [ctxs[i].vram_planes() for i in (0, n_ctx // 2, n_ctx - 1)]
-    exact = all(np.array_equal(a, b) for name in ("static", "learned")
+    exact = all(np.array_equal(a, b) for name in ("static", "learned", "dispatch_static", "dispatch_learned")
                 for a, b in zip(planes["hardware"], planes[name]))
     batch.set_schedule("learned", model)
     kms, kn = batch.profile()

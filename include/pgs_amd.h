@@ -145,12 +145,17 @@ uint32_t pgs_batch_num_launches(pgs_batch *batch);
 int pgs_batch_profile(pgs_batch *batch, double *ms, uint64_t *launches, uint32_t kinds);
 
 /* Shading schedule of a batch: PGS_SCHEDULE_HARDWARE (one multi-launch per merged shading
- * step), PGS_SCHEDULE_QUEUE_STATIC (a work queue of resident workgroups in item order) or
- * PGS_SCHEDULE_QUEUE_LEARNED (the queue ordered by a cost model's predictions, largest first;
- * `model` must outlive the batch). Pixels are identical under every schedule. */
+ * step), PGS_SCHEDULE_QUEUE_STATIC (a work queue of resident workgroups in item order),
+ * PGS_SCHEDULE_QUEUE_LEARNED (the queue ordered by a cost model's predictions, largest first,
+ * heavy tiles split), PGS_SCHEDULE_DISPATCH_STATIC (one workgroup per chunk of a tile's waves,
+ * started by the hardware dispatcher in tile order) or PGS_SCHEDULE_DISPATCH_LEARNED (the same
+ * with tiles in predicted-cost order, largest first). Learned schedules need `model`, which
+ * must outlive the batch. Pixels are identical under every schedule. */
 #define PGS_SCHEDULE_HARDWARE 0u
 #define PGS_SCHEDULE_QUEUE_STATIC 1u
 #define PGS_SCHEDULE_QUEUE_LEARNED 2u
+#define PGS_SCHEDULE_DISPATCH_STATIC 3u
+#define PGS_SCHEDULE_DISPATCH_LEARNED 4u
 typedef struct pgs_cost_model pgs_cost_model;
 int pgs_batch_set_schedule(pgs_batch *batch, uint32_t schedule, const pgs_cost_model *model);
 /* One replay with the static work queue measuring every shading workgroup: writes up to

@@ -198,6 +198,12 @@ typedef struct pgs_queue_args
 
 int pgs_k_shade_queue(const pgs_queue_args *args, hipStream_t stream);
 
+/* Ordered dispatch: the same shading with the hardware dispatcher as the queue. Workgroup b
+ * (of `grid`, at least counter[1]) shades chunk order[b] & 0xfff of tile order[b] >> 12, the
+ * chunk being waves_per_group consecutive waves of the tile; workgroups from counter[1] on exit
+ * at once. The dispatcher starts workgroups in index order, so `order` is the start order. */
+int pgs_k_shade_ordered(const pgs_queue_args *args, uint32_t grid, hipStream_t stream);
+
 /* Features of every coarse tile of a shading multi-launch for the tile-cost model: one wave
  * per tile scans the tile's list. Writes PGS_MLP_INPUTS floats per tile (all zero but the
  * last, constant input, for tiles without work for this launch). */
@@ -206,6 +212,7 @@ typedef struct pgs_features_args
 	const void *args;          /* device pgs_shade_args[num_launches] */
 	const uint32_t *prefix;
 	float *features;           /* total x PGS_MLP_INPUTS */
+	uint32_t *tile_waves;      /* optional: per tile, the shading waves it needs (0 without work) */
 	uint32_t num_launches;
 	uint32_t total;
 } pgs_features_args;
@@ -214,17 +221,27 @@ int pgs_k_tile_features(const pgs_features_args *args, hipStream_t stream);
 
 /* Builds the queue of tiles with work (feature 0 nonzero): in tile order (`predicted` NULL)
  * or by predicted cost, largest first (256 buckets of 1/8 in log2 cycles; order within a bucket
- * is arbitrary). Writes the list to `order`, its length to counter[1] and zeroes counter[0]
- * for the work-queue launch that follows. One workgroup. */
+ * is arbitrary). A work-queue tile becomes items of at most 16 waves (cost order may split a
+ * heavy tile further, up to 16 items); a dispatch-order tile becomes one entry per chunk.
+ * Writes the entries to `order`, their number to counter[1] and zeroes counter[0] for the
+ * launch that follows. One workgroup places the tiles, a second kernel writes the entries. */
 typedef struct pgs_order_args
 {
 	const float *features;     /* count x PGS_MLP_INPUTS */
 	const float *predicted;    /* log2 cycles per tile, or NULL for tile order */
 	uint32_t *order;           /* capacity entries */
 	uint32_t *counter;         /* [0], [1] as in pgs_queue_args */
-	uint32_t count;            /* tiles, below 2^24 */
+	const uint32_t *tile_waves;/* the waves of every tile (pgs_k_tile_features) */
+	uint32_t *run_start;       /* scratch, count entries: first entry of each tile */
+	uint32_t *run_weight;      /* scratch, count entries: entries of each tile */
+	uint32_t count;            /* tiles, below 2^24 (2^20 for dispatch order) */
 	uint32_t slots;            /* resident workgroups of the queue launch */
-	uint32_t capacity;         /* at least count + 2 * slots + 1024 */
+	uint32_t capacity;         /* entries `order` holds: queue, at least count + 2 * slots + 1024
+	                              plus the waves of all tiles / 16; dispatch, the waves of all
+	                              tiles */
+	uint32_t group_waves;      /* dispatch order: waves per workgroup */
+	uint32_t dispatch;         /* 1: entries tile << 12 | chunk for pgs_k_shade_ordered, one per
+	                              chunk of group_waves waves; 0: work-queue items */
 	uint32_t pad;
 } pgs_order_args;
 

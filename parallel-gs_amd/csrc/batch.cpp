@@ -64,6 +64,15 @@ size_t align256(size_t v) { return (v + 255u) & ~size_t(255); }
 // Queue entries of a shading launch: its tiles plus what cost-based splitting may add (at most
 // 2 * slots, and a launch has far fewer than 65536 resident workgroups).
 uint32_t order_capacity(uint32_t tiles) { return tiles + 2u * 65536u + 1024u; }
+// Entries of a shading launch's queue or dispatch order: at most one per wave (base items,
+// chunks) plus what cost splitting adds.
+uint32_t order_entries(uint32_t tiles, uint32_t waves)
+{
+	const uint64_t n = uint64_t(order_capacity(tiles)) + waves;
+	if (n > 0x7fffffffull)
+		throw std::runtime_error("batch: too many shading waves for a schedule buffer");
+	return uint32_t(n);
+}
 }
 
 Batch::~Batch()
@@ -244,14 +253,18 @@ bool Batch::upload()
 		{
 			uint32_t *tp = reinterpret_cast<uint32_t *>(host.data() + m.tile_prefix_offset);
 			uint32_t tiles = 0;
+			uint64_t waves = 0;
 			for (size_t i = 0; i < m.refs.size(); i++)
 			{
 				const Renderer::Launch &l = launches_[m.refs[i].context][m.refs[i].index];
 				tp[i] = tiles;
 				tiles += l.shade.tiles_x * l.shade.tiles_y;
+				const uint64_t B = 1ull << (l.shade.tile_log2 - 3u);
+				waves += uint64_t(l.shade.tiles_x) * l.shade.tiles_y * B * B << (l.shade.rate_x_log2 + l.shade.rate_log2);
 			}
 			tp[m.refs.size()] = tiles;
 			m.tiles = tiles;
+			m.waves = uint32_t(std::min<uint64_t>(waves, 0xffffffffu));
 		}
 		if (m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper)
 		{
@@ -269,15 +282,15 @@ bool Batch::upload()
 	}
 	ok(hipMemcpy(dev_, host.data(), bytes, hipMemcpyHostToDevice));
 
-	// Work-queue buffers per shading launch: features, predictions, costs, counter, then the
-	// queue, which holds a tile's split items too (at most tiles + 2 * slots entries).
+	// Schedule buffers per shading launch: features, predictions, costs, waves per tile, the
+	// two per-tile placement words of the order build, the counter, then the entries: at most
+	// one per wave (work-queue base items, dispatch chunks) plus what cost splitting adds.
 	size_t sb = 0;
 	for (auto &m : merged_)
 		if (is_shade(m))
 		{
 			m.sched_offset = sb;
-			sb = align256(sb + size_t(m.tiles) * (PGS_MLP_INPUTS * 4u + 8u) + 8u +
-			              size_t(order_capacity(m.tiles)) * 4u);
+			sb = align256(sb + size_t(m.tiles) * (PGS_MLP_INPUTS + 5u) * 4u + 8u + size_t(order_entries(m.tiles, m.waves)) * 4u);
 		}
 	if (sb > sched_bytes_)
 	{
@@ -292,7 +305,7 @@ bool Batch::upload()
 
 bool Batch::set_schedule(Schedule s, const CostModel *model)
 {
-	if (s == Schedule::QueueLearned && !model)
+	if ((s == Schedule::QueueLearned || s == Schedule::DispatchLearned) && !model)
 		return false;
 	schedule_ = s;
 	model_ = model;
@@ -317,13 +330,18 @@ void Batch::launch_shade(const Merged &m)
 	float *features = reinterpret_cast<float *>(base);
 	float *pred = features + size_t(m.tiles) * PGS_MLP_INPUTS;
 	uint32_t *costs = reinterpret_cast<uint32_t *>(pred + m.tiles);
-	uint32_t *counter = costs + m.tiles;
+	uint32_t *tile_waves = costs + m.tiles;
+	uint32_t *run_start = tile_waves + m.tiles;
+	uint32_t *run_weight = run_start + m.tiles;
+	uint32_t *counter = run_weight + m.tiles;
 	uint32_t *order = counter + 2;
 	const uint32_t *tile_prefix = reinterpret_cast<const uint32_t *>(dev_ + m.tile_prefix_offset);
 	if (m.tiles == 0)
 		return;
-	if (m.tiles >= (1u << 24))
-		throw std::runtime_error("work-queue shading: too many tiles");
+	const bool dispatch = (schedule_ == Schedule::DispatchStatic || schedule_ == Schedule::DispatchLearned) &&
+	                      !collecting_;
+	if (m.tiles >= (dispatch ? (1u << 20) : (1u << 24)))
+		throw std::runtime_error("scheduled shading: too many tiles");
 
 	// Features of every tile (the queue holds only tiles with work), the model's predictions,
 	// the queue, then the resident-workgroup shading.
@@ -331,11 +349,12 @@ void Batch::launch_shade(const Merged &m)
 	f.args = args;
 	f.prefix = tile_prefix;
 	f.features = features;
+	f.tile_waves = tile_waves;
 	f.num_launches = n;
 	f.total = m.tiles;
 	if (pgs_k_tile_features(&f, stream_) != 0)
 		throw std::runtime_error("features launch failed");
-	const bool learned = schedule_ == Schedule::QueueLearned && !collecting_;
+	const bool learned = (schedule_ == Schedule::QueueLearned || schedule_ == Schedule::DispatchLearned) && !collecting_;
 	if (learned)
 		model_->predict_device(features, m.tiles, pred, stream_);
 	pgs_order_args o{};
@@ -345,7 +364,13 @@ void Batch::launch_shade(const Merged &m)
 	o.counter = counter;
 	o.count = m.tiles;
 	o.slots = pgs_queue_slots(m.rate_x, m.rate_y, tuning_.waves_per_group, m.tiles);
-	o.capacity = order_capacity(m.tiles);
+	o.capacity = order_entries(m.tiles, m.waves);
+	const uint32_t group_waves = tuning_.waves_per_group ? tuning_.waves_per_group : 4u;
+	o.tile_waves = tile_waves;
+	o.run_start = run_start;
+	o.run_weight = run_weight;
+	o.group_waves = group_waves;
+	o.dispatch = dispatch ? 1u : 0u;
 	if (pgs_k_queue_order(&o, stream_) != 0)
 		throw std::runtime_error("order launch failed");
 	if (collecting_ && pgs_k_fill_u32(costs, m.tiles, 0u, stream_) != 0)
@@ -362,6 +387,22 @@ void Batch::launch_shade(const Merged &m)
 	q.rate_x = m.rate_x;
 	q.rate_y = m.rate_y;
 	q.waves_per_group = tuning_.waves_per_group;
+	if (dispatch)
+	{
+		// One workgroup per chunk; the grid counts the chunks of every tile, with or without
+		// work (each tile's waves are rounded up to whole chunks separately).
+		uint64_t grid = 0;
+		for (const Ref &r : m.refs)
+		{
+			const pgs_shade_args &sa = launches_[r.context][r.index].shade;
+			const uint64_t B = 1ull << (sa.tile_log2 - 3u);
+			const uint64_t w = B * B << (sa.rate_x_log2 + sa.rate_log2);
+			grid += uint64_t(sa.tiles_x) * sa.tiles_y * ((w + group_waves - 1u) / group_waves);
+		}
+		if (grid > 0x7fffffffull || pgs_k_shade_ordered(&q, uint32_t(grid), stream_) != 0)
+			throw std::runtime_error("ordered shading launch failed");
+		return;
+	}
 	if (pgs_k_shade_queue(&q, stream_) != 0)
 		throw std::runtime_error("work-queue shading launch failed");
 }

@@ -26,7 +26,7 @@ struct BatchTuning
 
 // How a batch runs its shading launches: the hardware dispatcher (one multi-launch), or a
 // work queue of resident workgroups in static order or in the order of predicted cost.
-enum class Schedule : uint32_t { Hardware = 0, QueueStatic = 1, QueueLearned = 2 };
+enum class Schedule : uint32_t { Hardware = 0, QueueStatic = 1, QueueLearned = 2, DispatchStatic = 3, DispatchLearned = 4 };
 
 struct TuningResult
 {
@@ -75,6 +75,7 @@ private:
 		size_t sched_offset = 0;       // shading launches: features, predictions, order, costs, counter
 		size_t tile_prefix_offset = 0; // shading launches: coarse-tile prefix over the launches
 		uint32_t tiles = 0;
+		uint32_t waves = 0; // shading launches: waves of all tiles (bound on dispatch-order entries)
 	};
 
 	std::vector<Renderer *> contexts_;

@@ -405,7 +405,7 @@ int pgs_cost_model_train(pgs_cost_model *model, const float *features, const flo
 int pgs_batch_set_schedule(pgs_batch *batch, uint32_t schedule, const pgs_cost_model *model)
 {
 	return batch_guard(batch, [&] {
-		if (schedule > PGS_SCHEDULE_QUEUE_LEARNED)
+		if (schedule > PGS_SCHEDULE_DISPATCH_LEARNED)
 			return -1;
 		return batch->batch.set_schedule(pgs::Schedule(schedule), model ? &model->model : nullptr) ? 0 : -1;
 	});

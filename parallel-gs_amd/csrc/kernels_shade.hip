@@ -835,6 +835,29 @@ __device__ __forceinline__ void tile_queue_loop(const pgs_queue_args &q, const p
 	}
 }
 
+// Ordered dispatch: one workgroup per chunk of waves_per_group consecutive waves of a tile,
+// taken from `order` by workgroup index (the start order of the hardware dispatcher).
+template <uint32_t S, typename WaveFn>
+__device__ __forceinline__ void ordered_chunk(const pgs_queue_args &q, const pgs_shade_args *__restrict__ args,
+                                              uint32_t want_mode, WaveFn fn)
+{
+	if (blockIdx.x >= uni(q.counter[1]))
+		return;
+	const uint32_t e = uni(q.order[blockIdx.x]);
+	const uint32_t g = e >> 12, chunk = e & 0xfffu;
+	const uint32_t l = uni(find_launch(q.prefix, q.num_launches, g));
+	const pgs_shade_args &a = args[l];
+	const uint32_t t = g - q.prefix[l];
+	if (a.tile_modes ? a.tile_modes[t] != want_mode : a.counts[t] == 0)
+		return;
+	const uint32_t B = 1u << (a.tile_log2 - 3u);
+	const uint32_t w = chunk * (blockDim.x >> 6) + uni(threadIdx.x >> 6);
+	if (w >= B * B * S)
+		return;
+	const uint32_t tx = t % a.tiles_x, ty = t / a.tiles_x, block = w / S;
+	fn(a, tx * B + block % B, ty * B + block / B, w % S);
+}
+
 // The argument blocks come in as a restrict kernel parameter: with it the compiler may load
 // their fields through the scalar cache inside the loop, although the loop stores to VRAM.
 __global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 k_shade_1x_queue(pgs_queue_args q,
@@ -853,6 +876,23 @@ PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_01_queue, 0, 1)
 PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_11_queue, 1, 1)
 PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_12_queue, 1, 2)
 PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_22_queue, 2, 2)
+
+__global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 k_shade_1x_ordered(pgs_queue_args q,
+                                                                          const pgs_shade_args *__restrict__ args)
+{
+	ordered_chunk<1>(q, args, 1u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t) { shade_1x_block(a, bx, by); });
+}
+#define PGS_SSAA_ORDERED_KERNEL(NAME, RX, RY)                                                       \
+	__global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 NAME(pgs_queue_args q,                  \
+	                                                            const pgs_shade_args *__restrict__ args) \
+	{                                                                                              \
+		ordered_chunk<(1u << (RX + RY))>(q, args, 2u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, \
+		                                                 uint32_t sub) { shade_ssaa_wave<RX, RY>(a, bx, by, sub); }); \
+	}
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_01_ordered, 0, 1)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_11_ordered, 1, 1)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_12_ordered, 1, 2)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_22_ordered, 2, 2)
 
 __device__ __forceinline__ float wave_sum(float v)
 {
@@ -924,6 +964,8 @@ __global__ void __launch_bounds__(256) k_tile_features(pgs_features_args f)
 		acc[k] = wave_sum(acc[k]);
 	if (lane != 0)
 		return;
+	if (f.tile_waves)
+		f.tile_waves[item] = work ? B * B * (1u << (a.rate_x_log2 + a.rate_log2)) : 0u;
 	if (!work)
 	{
 		for (uint32_t k = 0; k < PGS_MLP_INPUTS - 1u; k++)
@@ -1072,6 +1114,29 @@ extern "C" int pgs_k_shade_queue(const pgs_queue_args *q, hipStream_t stream)
 	if (!k)
 		return -int(hipErrorNotSupported);
 	const uint32_t grid = pgs_queue_slots(q->rate_x, q->rate_y, waves, q->total);
+	const pgs_shade_args *shade_args = static_cast<const pgs_shade_args *>(q->args);
+	void *params[] = { const_cast<pgs_queue_args *>(q), &shade_args };
+	if (hipLaunchKernel(k, dim3(grid), dim3(64u * waves), params, 0, stream) != hipSuccess)
+		return -int(hipErrorLaunchFailure);
+	return 0;
+}
+
+extern "C" int pgs_k_shade_ordered(const pgs_queue_args *q, uint32_t grid, hipStream_t stream)
+{
+	if (!q || !q->args || !q->prefix || !q->counter || !q->order || q->num_launches == 0)
+		return -int(hipErrorInvalidValue);
+	if (grid == 0)
+		return 0;
+	const uint32_t waves = q->waves_per_group ? q->waves_per_group : 4u;
+	if (waves != 1 && waves != 2 && waves != 4)
+		return -int(hipErrorInvalidValue);
+	const void *k = nullptr;
+	if (q->rate_y == 0) k = reinterpret_cast<const void *>(pgs::k_shade_1x_ordered);
+	else if (q->rate_x == 0 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_ordered);
+	else if (q->rate_x == 1 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_ordered);
+	else if (q->rate_x == 1 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_ordered);
+	else if (q->rate_x == 2 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_ordered);
+	else return -int(hipErrorNotSupported);
 	const pgs_shade_args *shade_args = static_cast<const pgs_shade_args *>(q->args);
 	void *params[] = { const_cast<pgs_queue_args *>(q), &shade_args };
 	if (hipLaunchKernel(k, dim3(grid), dim3(64u * waves), params, 0, stream) != hipSuccess)

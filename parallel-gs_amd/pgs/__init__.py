@@ -77,7 +77,7 @@ EXPORTED_SYMBOLS = [
     "pgs_cost_model_predict", "pgs_cost_model_gradients", "pgs_cost_model_train",
     "pgs_k_mlp_forward", "pgs_k_mlp_gradients", "pgs_k_mlp_adam", "pgs_k_mlp_quantize",
     "pgs_batch_set_schedule", "pgs_batch_collect_costs", "pgs_k_shade_queue", "pgs_k_tile_features",
-    "pgs_k_queue_order", "pgs_k_fill_u32",
+    "pgs_k_queue_order", "pgs_k_fill_u32", "pgs_k_shade_ordered",
 ]
 
 
@@ -369,7 +369,9 @@ class Batch:
     def num_launches(self) -> int:
         return int(self._L.pgs_batch_num_launches(self._h))
 
-    SCHEDULES = {"hardware": 0, "static": 1, "learned": 2}
+    # "static" / "learned": the persistent work queue; "dispatch_static" / "dispatch_learned":
+    # the hardware dispatcher started in tile order or in predicted-cost order.
+    SCHEDULES = {"hardware": 0, "static": 1, "learned": 2, "dispatch_static": 3, "dispatch_learned": 4}
 
     def set_schedule(self, name: str, model=None):
         self._L.pgs_batch_set_schedule.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]

@@ -92,19 +92,24 @@ def test_replay_restores_vram(hip_device):
 
 
 def test_schedules_are_bit_exact(hip_device):
-    """Hardware dispatch, static work queue and learned work queue shade the same tiles: the
-    VRAM of every context matches its live run under each schedule."""
+    """Hardware dispatch, the static and learned work queues and the static and learned
+    dispatch orders shade the same tiles: the VRAM of every context matches its live run under
+    each schedule, also with 1 and 2 waves per workgroup (chunks of other sizes)."""
     specs = pgs.batched_specs(0x5EED0006, 5, n_min=300, n_max=3000, ssaa_choices=(1, 2, 4, 16))
     ctxs, live = make_contexts(specs)
     batch = pgs.Batch(ctxs)
     model = pgs.CostModel(4)
-    for name in ["static", "learned", "hardware", "learned"]:
-        batch.set_schedule(name, model if name == "learned" else None)
+    runs = [(4, "static"), (4, "learned"), (4, "hardware"), (4, "learned"), (4, "dispatch_static"),
+            (4, "dispatch_learned"), (1, "dispatch_learned"), (2, "dispatch_static"), (2, "learned"), (4, "hardware")]
+    for waves, name in runs:
+        print(f"[case] schedule {name}, {waves} waves per workgroup", flush=True)
+        batch.apply_tuning(0, waves)
+        batch.set_schedule(name, model if "learned" in name else None)
         batch.replay(True)
         batch.replay(True)
         batch.sync()
         for i, ctx in enumerate(ctxs):
-            assert np.array_equal(ctx.vram_planes(), live[i]), (name, i)
+            assert np.array_equal(ctx.vram_planes(), live[i]), (name, waves, i)
     feats, costs = batch.collect_costs()
     print("collected", feats.shape, "cost range", costs.min(), costs.max(), flush=True)
     assert feats.shape[0] > 100 and np.all(np.isfinite(feats)) and np.all(costs > 0)
