@@ -190,6 +190,11 @@ struct PassInfo
 // Applies one primitive to this lane's sample (ubershader.comp shade_early + shade_resolve).
 // (hx, hy) is the sample's coordinate on the grid of 2^rate samples per pixel axis; the pixel
 // is (hx >> rate, hy >> rate). Single-sampled shading passes rate 0 and super_sampled false.
+// kPredicated: per-lane failures only clear `alive` and the wave leaves early when no lane is
+// left (measured faster for single-sampled shading, where large primitives cover whole waves);
+// otherwise every per-lane failure returns at once (faster for super-sampled shading of small
+// primitives). Both give the same pixels.
+template <bool kPredicated>
 __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassInfo &pi, uint32_t prim_index,
                                             int32_t hx, int32_t hy, uint32_t rate, bool super_sampled,
                                             uint32_t &color, uint32_t &depth, bool &cdirty, bool &zdirty)
@@ -205,9 +210,15 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	const int32_t rx = snap_raster ? (hx & snap_mask) : hx, ry = snap_raster ? (hy & snap_mask) : hy;
 	const int32_t px = hx >> rate, py = hy >> rate;
 
+	// Predicated form: per-lane conditions only clear `alive` (a divergent branch saves nothing
+	// while some lane still runs it) and the colour and depth writes at the end are selects.
+	// Nothing before them touches memory except loads of the primitive's data and in-bounds
+	// texel loads.
 	// Coverage (intersect.h:156-244).
-	if (!(px >= s.bb[0] && py >= s.bb[1] && px <= s.bb[2] && py <= s.bb[3]))
-		return;
+	bool alive = px >= s.bb[0] && py >= s.bb[1] && px <= s.bb[2] && py <= s.bb[3];
+	if constexpr (!kPredicated)
+		if (!alive)
+			return;
 	const bool para = (s.a[0] & 1) != 0;
 	const int32_t offb = int32_t((uint32_t(s.a[0]) >> 2) & 3u) - 1;
 	const int32_t offc = int32_t((uint32_t(s.a[0]) >> 4) & 3u) - 1;
@@ -223,15 +234,20 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		C = wadd(C, wadd(A, offc));
 		A = 0;
 	}
-	if (!(A >= 0 && B >= 0 && C >= 0))
-		return;
+	alive = alive && A >= 0 && B >= 0 && C >= 0;
 	if (st & (ST_SCANMSK_EVEN | ST_SCANMSK_ODD))
 	{
 		// Sprites mask per pixel row, everything else per sample row.
 		const int32_t sy_row = (st & ST_SPRITE) ? py : hy;
-		if ((sy_row & 1) == 0 ? (st & ST_SCANMSK_EVEN) : (st & ST_SCANMSK_ODD))
+		alive = alive && !((sy_row & 1) == 0 ? (st & ST_SCANMSK_EVEN) : (st & ST_SCANMSK_ODD));
+	}
+	if constexpr (kPredicated)
+	{
+		if (__ballot(alive) == 0)
 			return;
 	}
+	else if (!alive)
+		return;
 
 	// Z (ubershader.comp:852-871).
 	uint32_t z = s.z[0];
@@ -248,8 +264,17 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	}
 	const bool z_test = pi.zsens && (st & ST_Z_TEST);
 	const bool z_write = pi.zsens && (st & ST_Z_WRITE);
-	if (z_test && !((st & ST_Z_GREATER) ? z > depth : z >= depth))
-		return;
+	if (z_test)
+	{
+		alive = alive && ((st & ST_Z_GREATER) ? z > depth : z >= depth);
+		if constexpr (kPredicated)
+		{
+			if (__ballot(alive) == 0)
+				return;
+		}
+		else if (!alive)
+			return;
+	}
 	// Attribute snapping without raster snapping: attributes at the pixel's first sample,
 	// depth per sample (ubershader.comp:888-898).
 	if (super_sampled && (st & (ST_SNAP_RASTER | ST_SNAP_ATTRIBUTE)) == ST_SNAP_ATTRIBUTE)
@@ -282,7 +307,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 			sq[k] = __fadd_rn(__fadd_rn(ta.stqf0[k], __fmul_rn(ta.stqf1[k], fi)), __fmul_rn(ta.stqf2[k], fj));
 		const bool persp = (st & ST_PERSPECTIVE) != 0;
 		float u = sq[0], v = sq[1];
-		if (persp && sq[2] != 1.0f)
+		if (persp) // q == 1 divides exactly, as the reference's q != 1 test skips
 		{
 			u = __fdiv_rn(u, sq[2]);
 			v = __fdiv_rn(v, sq[2]);
@@ -337,31 +362,36 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	bool blending = bm & BLEND_ABE;
 	if (bm & BLEND_ATE)
 	{
-		if (!alpha_test((bm >> 2) & 7u, pa.alpha & 0xffu, comb[3]))
+		const bool pass = alpha_test((bm >> 2) & 7u, pa.alpha & 0xffu, comb[3]);
+		switch ((bm >> 5) & 3u)
 		{
-			switch ((bm >> 5) & 3u)
-			{
-			case 0: return;
-			case 1: write_zb = false; break;
-			case 2: write_fb = false; write_alpha = false; break;
-			default:
-				if (pi.fb32) write_alpha = false;
-				write_zb = false;
-				break;
-			}
+		case 0:
+			alive = alive && pass;
+			if constexpr (!kPredicated)
+				if (!alive)
+					return;
+			break;
+		case 1: write_zb = pass; break;
+		case 2: write_fb = pass; write_alpha = pass; break;
+		default:
+			if (pi.fb32) write_alpha = pass;
+			write_zb = pass;
+			break;
 		}
 	}
 	if (!pi.fb24 && (bm & BLEND_DATE))
 	{
 		const uint32_t bit = fb[3] >= 128.0f ? 1u : 0u;
-		if (uint32_t((bm & BLEND_DATM) != 0) != bit)
-			return;
+		alive = alive && uint32_t((bm & BLEND_DATM) != 0) == bit;
+		if constexpr (!kPredicated)
+			if (!alive)
+				return;
 	}
-	if ((bm & BLEND_PABE) && comb[3] < 128.0f)
-		blending = false;
 	float o[4] = { comb[0], comb[1], comb[2], comb[3] };
 	if (blending)
 	{
+		// PABE turns blending off per lane (source alpha below 128): a select, not a branch.
+		const bool lane_blends = !((bm & BLEND_PABE) && comb[3] < 128.0f);
 		const uint32_t am = (bm >> 11) & 3u, bmd = (bm >> 13) & 3u, cm = (bm >> 15) & 3u, dm = (bm >> 17) & 3u;
 		float C = 0.0f;
 		if (cm == 0) C = comb[3];
@@ -372,7 +402,8 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 			const float A_ = am == 0 ? comb[k] : am == 1 ? fb[k] : 0.0f;
 			const float B_ = bmd == 0 ? comb[k] : bmd == 1 ? fb[k] : 0.0f;
 			const float D_ = dm == 0 ? comb[k] : dm == 1 ? fb[k] : 0.0f;
-			o[k] = __fadd_rn(floorf(__fmul_rn(__fmul_rn(__fsub_rn(A_, B_), C), 1.0f / 128.0f)), D_);
+			const float b = __fadd_rn(floorf(__fmul_rn(__fmul_rn(__fsub_rn(A_, B_), C), 1.0f / 128.0f)), D_);
+			o[k] = lane_blends ? b : o[k];
 		}
 	}
 	if (bm & BLEND_DTHE)
@@ -406,17 +437,20 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	}
 	packed |= uint32_t((bm & BLEND_FBA) && write_alpha) << 31;
 
-	if (pa.fbmsk != ~0u && (write_fb || write_alpha))
+	if (pa.fbmsk != ~0u)
 	{
-		color = pa.fbmsk ? ((packed & ~pa.fbmsk) | (color & pa.fbmsk)) : packed;
+		uint32_t c = pa.fbmsk ? ((packed & ~pa.fbmsk) | (color & pa.fbmsk)) : packed;
 		if (pi.fb16)
-			color &= 0x80f8f8f8u;
-		cdirty = true;
+			c &= 0x80f8f8f8u;
+		const bool wc = alive && (write_fb || write_alpha);
+		color = wc ? c : color;
+		cdirty = cdirty || wc;
 	}
-	if (z_write && write_zb)
+	if (z_write)
 	{
-		depth = z;
-		zdirty = true;
+		const bool wz = alive && write_zb;
+		depth = wz ? z : depth;
+		zdirty = zdirty || wz;
 	}
 }
 
@@ -491,7 +525,7 @@ __device__ __forceinline__ void shade_1x_block(const pgs_shade_args &a, uint32_t
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
 			const uint32_t prim = uni(uint32_t(__shfl(int(idx), int(bit))));
-			shade_pixel(a, pi, prim, px, py, 0u, false, color, depth, cdirty, zdirty);
+			shade_pixel<true>(a, pi, prim, px, py, 0u, false, color, depth, cdirty, zdirty);
 		}
 	}
 
@@ -672,7 +706,7 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
 			const uint32_t prim = uni(uint32_t(__shfl(int(idx), int(bit))));
-			shade_pixel(a, pi, prim, hx, hy, RY, true, color, depth, cdirty, zdirty);
+			shade_pixel<false>(a, pi, prim, hx, hy, RY, true, color, depth, cdirty, zdirty);
 		}
 	}
 
