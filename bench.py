@@ -19,6 +19,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "shaded Mpix/s + frames/s at bit-exact VRAM; batched/learned-sched vs plain"
 HBM_PEAK_GBS = 8000.0
+# Launch kinds of the renderer's profile counters (LaunchKind in renderer.hpp).
+KIND_NAMES = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode",
+              "ssaa_heuristic", "shade_super", "sampler_feedback"]
 # L2-to-fabric bytes per launch of each workload's dominant kernel, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by what the counter reads for a
 # 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the runs are summarised in
@@ -136,9 +139,7 @@ def kernel_profile(ctx):
     ctx.replay(False)
     kms, kn = ctx.profile_collect()
     ctx.profile_enable(False)
-    names = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode",
-             "ssaa_heuristic", "shade_super"]
-    return {names[i]: (kms[i], kn[i]) for i in range(min(len(names), len(kn))) if kn[i]}
+    return {KIND_NAMES[i]: (kms[i], kn[i]) for i in range(min(len(KIND_NAMES), len(kn))) if kn[i]}
 
 
 def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames_stream=None, cpu_frames=1,
@@ -229,8 +230,7 @@ def bench_textured(args):
                         seed, cpu_frames_stream=sample, cpu_frames=1, traffic=MEASURED_TRAFFIC["textured"])
 
 
-KIND_NAMES = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode",
-              "ssaa_heuristic", "shade_super"]
+
 
 
 def time_batch(batch, steps, warmup):

@@ -25,6 +25,10 @@ typedef struct pgs_options
 	uint32_t super_sampled_textures;
 	uint32_t device_index;
 	uint64_t upload_arena_bytes; /* 0 selects the default (256 MiB) */
+	/* Sampler feedback (sparse texture decode, gs_renderer.cpp:1388-1422): 0 where a render pass
+	 * decodes at least 2^20 texels, 1 always, 2 never. Never changes a pixel. */
+	uint32_t sampler_feedback;
+	uint32_t reserved;
 } pgs_options;
 
 /* FlushStats (gs/gs_renderer.hpp:55-65), extended with the backend's own counters. */

@@ -83,6 +83,10 @@ typedef struct pgs_setup_args
 	uint32_t coarse_log2;
 	uint32_t coarse_x, coarse_y;
 	uint32_t mask_words;
+	/* Optional: words zeroed by the launch (the sampler-feedback masks of the pass). */
+	uint32_t *clear_words;
+	uint32_t clear_count;
+	uint32_t pad;
 } pgs_setup_args;
 
 int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stream);
@@ -279,11 +283,38 @@ typedef struct pgs_tex_decode_args
 	const uint16_t *clut;
 	uint32_t *images;
 	const void *decodes;
+	const uint32_t *feedback; /* optional sampler-feedback masks (pgs_k_sampler_feedback): only texels
+	                             of marked 8x8 blocks are decoded; NULL decodes every texel */
 	uint32_t num_textures;
 	uint32_t num_groups;
 } pgs_tex_decode_args;
 
 int pgs_k_texture_decode(const pgs_tex_decode_args *args, hipStream_t stream);
+
+/* Sampler feedback: marks, for every texture of a pass, the 8x8 texel blocks of its decoded image
+ * that a primitive can sample, so that the decode can skip the rest. The texel box of a primitive
+ * comes from its texture coordinates at the corners of its pixel box grown by one pixel (every
+ * position the shader evaluates attributes at, snapped ones included), computed with the shader's
+ * own edge functions and barycentrics, then taken through the shader's clamp, 1/16 snap, bilinear
+ * footprint and wrap. One thread per primitive, atomicOr into masks that must be zero on entry (triangle setup
+ * clears them: pgs_setup_args::clear_words). Mask of texture i: PGS_FEEDBACK_WORDS words at
+ * i * PGS_FEEDBACK_WORDS, 4 words per row of blocks; images above 1024 texels in either
+ * direction are decoded whole. Replaces the sampler_feedback.comp dispatch of
+ * GSRenderer::dispatch_texture_analysis (gs/gs_renderer.cpp:3421-3456). */
+#define PGS_FEEDBACK_WORDS 512
+typedef struct pgs_feedback_args
+{
+	const void *setup;      /* PrimitiveSetup[num_primitives] */
+	const void *prims;      /* PrimitiveAttribute[num_primitives] */
+	const void *tattr;      /* TransformedAttributes[num_primitives] (after setup) */
+	const void *states;     /* StateVector[] */
+	const void *textures;   /* TextureDesc[num_textures] */
+	uint32_t *masks;        /* num_textures * PGS_FEEDBACK_WORDS */
+	uint32_t num_primitives;
+	uint32_t num_textures;
+} pgs_feedback_args;
+
+int pgs_k_sampler_feedback(const pgs_feedback_args *args, hipStream_t stream);
 
 /* ---- Tile-cost MLP of the learned tile-queue order (no counterpart in the reference; used
  * by the batched renderer to order its persistent shading queue, never to compute a pixel).

@@ -30,6 +30,7 @@ uint32_t grid_of(const Renderer::Launch &l)
 	case K::ClutWrite: return pgs_grid_clut_write(l.clut);
 	case K::TexDecode: return pgs_grid_texture_decode(l.decode);
 	case K::SsaaHeuristic: return pgs_grid_ssaa_heuristic(l.heuristic);
+	case K::SamplerFeedback: return pgs_grid_sampler_feedback(l.feedback);
 	}
 	return 0;
 }
@@ -47,6 +48,7 @@ size_t args_size(Renderer::LaunchKind k)
 	case K::ClutWrite: return sizeof(pgs_clut_args);
 	case K::TexDecode: return sizeof(pgs_tex_decode_args);
 	case K::SsaaHeuristic: return sizeof(pgs_ssaa_heuristic_args);
+	case K::SamplerFeedback: return sizeof(pgs_feedback_args);
 	}
 	return 0;
 }
@@ -477,6 +479,9 @@ void Batch::launch(const Merged &m)
 		break;
 	case K::SsaaHeuristic:
 		rc = pgs_multi_ssaa_heuristic(static_cast<const pgs_ssaa_heuristic_args *>(args), prefix, n, m.total, stream_);
+		break;
+	case K::SamplerFeedback:
+		rc = pgs_multi_sampler_feedback(static_cast<const pgs_feedback_args *>(args), prefix, n, m.total, stream_);
 		break;
 	}
 	if (rc != 0)

@@ -97,6 +97,9 @@ pgs_context *pgs_create(const pgs_options *options)
 		o.super_sampled_textures = options->super_sampled_textures != 0;
 		if (options->upload_arena_bytes)
 			o.upload_arena_bytes = options->upload_arena_bytes;
+		if (options->sampler_feedback > 2u)
+			return nullptr;
+		o.sampler_feedback = options->sampler_feedback;
 		if (hipSetDevice(int(options->device_index)) != hipSuccess)
 			return nullptr;
 	}

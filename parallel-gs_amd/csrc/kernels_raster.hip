@@ -436,13 +436,17 @@ __device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uin
 	}
 }
 
-__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t)
+__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t nwg)
 {
 	const uint32_t index = wg * 64u + threadIdx.x;
 	if (index < args.num_primitives)
 		triangle_setup_one(args, index);
 	if (args.coarse_mask)
 		coarse_mask_wave(args, wg, index);
+	// The pass's sampler-feedback masks start from zero; the feedback launch follows setup.
+	if (args.clear_words)
+		for (uint32_t w = index; w < args.clear_count; w += nwg * 64u)
+			args.clear_words[w] = 0;
 }
 
 // ---------------------------------------------------------------------------------------

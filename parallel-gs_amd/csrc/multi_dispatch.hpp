@@ -12,6 +12,7 @@ uint32_t pgs_grid_binning(const pgs_bin_args &a);
 uint32_t pgs_grid_shade(const pgs_shade_args &a);
 uint32_t pgs_grid_clut_write(const pgs_clut_args &a);
 uint32_t pgs_grid_texture_decode(const pgs_tex_decode_args &a);
+uint32_t pgs_grid_sampler_feedback(const pgs_feedback_args &a);
 
 int pgs_multi_vram_copy(const pgs_vram_copy_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
                         hipStream_t stream);
@@ -30,3 +31,5 @@ int pgs_multi_clut_write(const pgs_clut_args *args, const uint32_t *prefix, uint
                          hipStream_t stream);
 int pgs_multi_texture_decode(const pgs_tex_decode_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
                              hipStream_t stream);
+int pgs_multi_sampler_feedback(const pgs_feedback_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
+                               hipStream_t stream);

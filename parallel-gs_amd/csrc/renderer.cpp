@@ -53,7 +53,7 @@ Renderer::~Renderer()
 		(void)hipEventDestroy(e);
 	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_,
 	                 (void *)clut_buf_, (void *)tex_images_, (void *)state_copy_, (void *)flat_buckets_,
-	                 (void *)tile_modes_, (void *)coarse_mask_, (void *)snapshot_ })
+	                 (void *)tile_modes_, (void *)coarse_mask_, (void *)feedback_masks_, (void *)snapshot_ })
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
@@ -70,11 +70,12 @@ void Renderer::check(hipError_t err)
 	}
 }
 
-bool Renderer::init(uint32_t sample_log2, size_t arena_bytes)
+bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback)
 {
 	try
 	{
 		sample_log2_ = sample_log2;
+		sampler_feedback_ = sampler_feedback;
 		num_planes_ = sample_log2 ? 2u + (1u << sample_log2) : 1u;
 		check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 		check(hipMalloc(&vram_, size_t(num_planes_) * kVramSize));
@@ -100,6 +101,7 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes)
 		check(hipMemsetAsync(flat_buckets_, 0, 8u * sizeof(uint32_t), stream_));
 		check(hipMalloc(&tile_modes_, counts_cap_ * sizeof(uint32_t)));
 		check(hipMalloc(&coarse_mask_, size_t(kMaxCoarseTiles) * kMaxMaskWords * sizeof(uint64_t)));
+		check(hipMalloc(&feedback_masks_, size_t(kMaxPassTextures) * PGS_FEEDBACK_WORDS * sizeof(uint32_t)));
 		{
 			float lut[256];
 			make_rcp_float_lut(lut);
@@ -254,6 +256,7 @@ void Renderer::launch(const Launch &l)
 	case LaunchKind::TexDecode: rc = pgs_k_texture_decode(&l.decode, stream_); break;
 	case LaunchKind::SsaaHeuristic: rc = pgs_k_ssaa_heuristic(&l.heuristic, stream_); break;
 	case LaunchKind::ShadeSuper: rc = pgs_k_shade(&l.shade, stream_); break;
+	case LaunchKind::SamplerFeedback: rc = pgs_k_sampler_feedback(&l.feedback, stream_); break;
 	}
 	if (rc != 0)
 		check(hipError_t(-rc));
@@ -551,18 +554,14 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	}
 	arena_upload();
 
-	if (d.num_textures)
-	{
-		Launch ld{};
-		ld.kind = LaunchKind::TexDecode;
-		ld.decode.vram = vram_;
-		ld.decode.clut = clut_buf_;
-		ld.decode.images = tex_images_;
-		ld.decode.decodes = arena_.dev + off_dec;
-		ld.decode.num_textures = d.num_textures;
-		ld.decode.num_groups = d.tex_groups;
-		launch(ld);
-	}
+	// Sparse texture decode (sampler feedback) where a pass decodes many texels: setup clears
+	// the masks, the feedback launch marks the 8x8 blocks the primitives can sample, the decode
+	// skips the others. Small passes decode whole textures (the extra launch costs more).
+	uint64_t pass_texels = 0;
+	for (uint32_t t = 0; t < d.num_textures; t++)
+		pass_texels += uint64_t(d.decodes[t].width) * d.decodes[t].height;
+	const bool feedback = d.num_textures != 0 && sampler_feedback_ != 2u &&
+	                      (sampler_feedback_ == 1u || pass_texels >= kFeedbackAutoTexels);
 
 	Launch ls{};
 	ls.kind = LaunchKind::Setup;
@@ -590,7 +589,40 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	ls.setup.coarse_x = coarse_x;
 	ls.setup.coarse_y = coarse_y;
 	ls.setup.mask_words = mask_words;
+	if (feedback)
+	{
+		ls.setup.clear_words = feedback_masks_;
+		ls.setup.clear_count = d.num_textures * PGS_FEEDBACK_WORDS;
+	}
 	launch(ls);
+
+	if (feedback)
+	{
+		Launch lf{};
+		lf.kind = LaunchKind::SamplerFeedback;
+		lf.feedback.setup = setup_buf_;
+		lf.feedback.prims = arena_.dev + off_prim;
+		lf.feedback.tattr = tattr_buf_;
+		lf.feedback.states = arena_.dev + off_state;
+		lf.feedback.textures = arena_.dev + off_tex;
+		lf.feedback.masks = feedback_masks_;
+		lf.feedback.num_primitives = n;
+		lf.feedback.num_textures = d.num_textures;
+		launch(lf);
+	}
+	if (d.num_textures)
+	{
+		Launch ld{};
+		ld.kind = LaunchKind::TexDecode;
+		ld.decode.vram = vram_;
+		ld.decode.clut = clut_buf_;
+		ld.decode.images = tex_images_;
+		ld.decode.decodes = arena_.dev + off_dec;
+		ld.decode.feedback = feedback ? feedback_masks_ : nullptr;
+		ld.decode.num_textures = d.num_textures;
+		ld.decode.num_groups = d.tex_groups;
+		launch(ld);
+	}
 
 	if (super_pass)
 	{

@@ -37,7 +37,9 @@ public:
 	Renderer &operator=(const Renderer &) = delete;
 
 	// sample_log2: 0 = 1x, else log2 of the sample count; allocates 2 + samples planes.
-	bool init(uint32_t sample_log2, size_t arena_bytes);
+	// sampler_feedback: 0 decodes sparsely when a pass decodes at least kFeedbackAutoTexels
+	// texels, 1 always, 2 never.
+	bool init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback = 0);
 
 	// ---- transfers -------------------------------------------------------------------
 	// Queue a host->local upload. The payload bytes are copied into the upload arena.
@@ -92,7 +94,7 @@ public:
 	enum class LaunchKind : uint32_t
 	{
 		VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4, ClutWrite = 5, TexDecode = 6,
-		SsaaHeuristic = 7, ShadeSuper = 8
+		SsaaHeuristic = 7, ShadeSuper = 8, SamplerFeedback = 9
 	};
 	struct Launch
 	{
@@ -107,6 +109,7 @@ public:
 			pgs_clut_args clut;
 			pgs_tex_decode_args decode;
 			pgs_ssaa_heuristic_args heuristic;
+			pgs_feedback_args feedback;
 		};
 	};
 	const std::vector<Launch> &recorded() const { return recorded_; }
@@ -135,6 +138,9 @@ private:
 	uint32_t *flat_buckets_ = nullptr; // super-sampled passes: 256-bit bucket mask
 	uint32_t *tile_modes_ = nullptr;  // super-sampled passes: per coarse tile
 	uint64_t *coarse_mask_ = nullptr; // first binning level: kMaxCoarseTiles x kMaxMaskWords
+	uint32_t *feedback_masks_ = nullptr; // sampler feedback: kMaxPassTextures x PGS_FEEDBACK_WORDS
+	uint32_t sampler_feedback_ = 0;
+	static constexpr uint64_t kFeedbackAutoTexels = 1u << 20;
 	static constexpr uint32_t kMaxCoarseTiles = 256, kMaxMaskWords = 1024;
 	uint8_t *snapshot_ = nullptr;     // VRAM planes + CLUT at begin_recording()
 	size_t lists_cap_ = 0;            // entries

@@ -37,6 +37,8 @@ class _Options(ctypes.Structure):
         ("super_sampled_textures", ctypes.c_uint32),
         ("device_index", ctypes.c_uint32),
         ("upload_arena_bytes", ctypes.c_uint64),
+        ("sampler_feedback", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -77,7 +79,7 @@ EXPORTED_SYMBOLS = [
     "pgs_cost_model_predict", "pgs_cost_model_gradients", "pgs_cost_model_train",
     "pgs_k_mlp_forward", "pgs_k_mlp_gradients", "pgs_k_mlp_adam", "pgs_k_mlp_quantize",
     "pgs_batch_set_schedule", "pgs_batch_collect_costs", "pgs_k_shade_queue", "pgs_k_tile_features",
-    "pgs_k_queue_order", "pgs_k_fill_u32", "pgs_k_shade_ordered",
+    "pgs_k_queue_order", "pgs_k_fill_u32", "pgs_k_shade_ordered", "pgs_k_sampler_feedback",
 ]
 
 
@@ -189,9 +191,12 @@ def gen_transfers_wild(seed: int, n: int) -> np.ndarray:
 class Context:
     """One GS context (GSInterface) on a HIP device."""
 
+    SAMPLER_FEEDBACK = {"auto": 0, "always": 1, "never": 2}
+
     def __init__(self, super_sampling: int = 1, device_index: int = 0, arena_bytes: int = 0,
-                 ordered: bool = True, super_sampled_textures: bool = False):
-        opts = _Options(super_sampling, int(ordered), int(super_sampled_textures), device_index, arena_bytes)
+                 ordered: bool = True, super_sampled_textures: bool = False, sampler_feedback: str = "auto"):
+        opts = _Options(super_sampling, int(ordered), int(super_sampled_textures), device_index, arena_bytes,
+                        self.SAMPLER_FEEDBACK[sampler_feedback], 0)
         self._L = lib()
         self._h = self._L.pgs_create(ctypes.byref(opts))
         if not self._h:
