@@ -24,9 +24,13 @@ KIND_NAMES = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade"
               "ssaa_heuristic", "shade_super", "sampler_feedback"]
 # L2-to-fabric bytes per launch of each workload's dominant kernel, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by what the counter reads for a
-# 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the runs are summarised in
+# 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), and the kernel's waves and VALU
+# instructions per wave from an SQ_WAVES / SQ_INSTS_VALU run; the runs are summarised in
 # profiles/hbm_traffic_pmc__plain_and_textured4x.md.
-MEASURED_TRAFFIC = {"plain": 6829670, "textured": 20671590}
+MEASURED_TRAFFIC = {"plain": 6907183, "textured": 20959007}
+MEASURED_VALU = {"plain": (4480, 10865), "textured": (17920, 2342)}  # (waves, VALU instructions per wave)
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a SIMD for 4 cycles at 2.4 GHz.
+VALU_ISSUE_S_PER_WAVE_INST = 4.0 / 1024 / 2.4e9
 
 
 def log(msg):
@@ -143,7 +147,7 @@ def kernel_profile(ctx):
 
 
 def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames_stream=None, cpu_frames=1,
-                 traffic=None):
+                 traffic=None, valu=None):
     """Times graph replays of a recorded rendering stream and reports Mpix/s of output."""
     import torch
     import pgs
@@ -197,7 +201,10 @@ def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "kernel": "k_shade_ssaa" if kname == "shade_super" else "k_shade_1x",
-                     "note": "VALU-issue bound; byte roofline fraction is small by construction"},
+                     "valu_issue_frac": (round(valu[0] * valu[1] * VALU_ISSUE_S_PER_WAVE_INST / (k_ms / k_n / 1e3), 3)
+                                         if valu else None),
+                     "note": "VALU-issue bound; byte roofline fraction is small by construction; valu_issue_frac = "
+                             "measured VALU instructions x 4 cycles / 1024 SIMDs / 2.4 GHz over the launch time"},
         "cpu_baseline": {"value": round(w * h * cpu_frames / cpu_s / 1e6, 3), "unit": "Mpix/s", "cores": 1,
                          "kind": "port",
                          "sample": f"{cpu_frames} frame(s) of the same workload, single-threaded CPU oracle "
@@ -214,7 +221,7 @@ def bench_plain(args):
     stream = pgs.gen_plain(seed, 2000, 640, 448)
     return bench_render(args, stream, 1, 1, 2000,
                         "plain 1x: 2000 Gouraud triangles, 640x448 PSMCT32 + PSMZ24, ZTST=GEQUAL", seed,
-                        traffic=MEASURED_TRAFFIC["plain"])
+                        traffic=MEASURED_TRAFFIC["plain"], valu=MEASURED_VALU["plain"])
 
 
 def bench_textured(args):
@@ -227,7 +234,8 @@ def bench_textured(args):
     return bench_render(args, stream, 4, 8, 10000,
                         "textured 4x SSAA: 8 frames x 10000 prims (70% tri / 30% sprite), PSMT8 256x256 + "
                         "CT32 CLUT, bilinear MODULATE, (Cs-Cd)*As>>7+Cd, GEQUAL, 20% ST/Q",
-                        seed, cpu_frames_stream=sample, cpu_frames=1, traffic=MEASURED_TRAFFIC["textured"])
+                        seed, cpu_frames_stream=sample, cpu_frames=1, traffic=MEASURED_TRAFFIC["textured"],
+                        valu=MEASURED_VALU["textured"])
 
 
 
