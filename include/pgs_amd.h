@@ -43,6 +43,8 @@ typedef struct pgs_flush_stats
 	uint64_t render_passes;
 	uint64_t primitives;
 	uint64_t syncs;
+	uint64_t textures_decoded;   /* textures render passes decoded */
+	uint64_t texture_cache_hits; /* textures a pass reused from an earlier pass's decode */
 } pgs_flush_stats;
 
 /* GSInterface::GSInterface + GSInterface::init (gs/gs_interface.cpp:15-63). NULL on failure. */

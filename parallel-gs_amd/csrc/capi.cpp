@@ -183,6 +183,8 @@ int pgs_consume_flush_stats(pgs_context *ctx, pgs_flush_stats *out)
 			out->render_passes = s.render_passes;
 			out->primitives = s.primitives;
 			out->syncs = s.syncs;
+			out->textures_decoded = s.textures_decoded;
+			out->texture_cache_hits = s.texture_cache_hits;
 		}
 	});
 }
@@ -198,6 +200,7 @@ int pgs_begin_recording(pgs_context *ctx)
 {
 	return guard(ctx, [&] {
 		ctx->gs.flush();
+		ctx->gs.render_pass().clear_texture_cache();
 		ctx->gs.renderer().begin_recording();
 	});
 }

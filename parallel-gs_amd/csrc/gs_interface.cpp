@@ -320,6 +320,7 @@ void GSInterface::handle_clut_upload(uint32_t ctx)
 		a.cou = uint32_t((regs_.texclut >> 6) & 0x3f);
 		a.cov = uint32_t((regs_.texclut >> 12) & 0x3ff);
 	}
+	rp_.writes().clut_loaded();
 	renderer_.clut_load(a);
 }
 
@@ -356,10 +357,14 @@ void GSInterface::init_transfer()
 		if (c.w == 0 || c.h == 0)
 			return;
 		begin_transfer_work();
+		rp_.writes().write(page_span(c.dbp, c.dbw, c.dpsm, c.dsax, c.dsay, c.dsax + c.w - 1u, c.dsay + c.h - 1u));
 		renderer_.queue_local_copy(c);
 	}
 	else if (xdir == 0)
 	{
+		// The texture cache must not reuse an image of the destination from here on: a texture
+		// lookup can come before the payload is flushed (each flush logs its pages again).
+		rp_.writes().write(page_span(c.dbp, c.dbw, c.dpsm, c.dsax, c.dsay, c.dsax + c.w - 1u, c.dsay + c.h - 1u));
 		xfer_.copy = c;
 		xfer_.required_qwords = uint32_t((uint64_t(c.w) * c.h * psm_bpp(c.dpsm)) / 64u);
 		xfer_.host_to_local_active = xfer_.required_qwords != 0;
@@ -432,6 +437,7 @@ void GSInterface::flush_pending_transfer(bool keep_alive)
 			CopyDesc d = c;
 			d.first_pixel = xfer_.flushed_pixels;
 			d.end_pixel = avail;
+			rp_.writes().write(page_span(c.dbp, c.dbw, c.dpsm, c.dsax, c.dsay, c.dsax + c.w - 1u, c.dsay + c.h - 1u));
 			renderer_.queue_host_copy(d, xfer_.payload.data(), xfer_.payload.size() * 8u);
 			xfer_.flushed_pixels = avail;
 		}
@@ -496,6 +502,7 @@ void GSInterface::end_vram_write(size_t offset, size_t size)
 {
 	if (offset + size > kVramSize)
 		return;
+	rp_.writes().write_all();
 	renderer_.write_vram(host_vram_.data() + offset, offset, size);
 }
 }

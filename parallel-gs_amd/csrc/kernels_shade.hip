@@ -801,11 +801,11 @@ __device__ __forceinline__ void shade_ssaa_body(const pgs_shade_args &a, uint32_
 __global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a) { shade_1x_body(a, blockIdx.x, gridDim.x); }
 template <uint32_t RX, uint32_t RY>
 __global__ void __launch_bounds__(256) k_shade_ssaa(pgs_shade_args a) { shade_ssaa_body<RX, RY>(a, blockIdx.x, gridDim.x); }
-PGS_DEFINE_MULTI(256, k_shade_1x_multi, shade_1x_body, pgs_shade_args)
-PGS_DEFINE_MULTI(256, k_shade_ssaa_01_multi, (shade_ssaa_body<0, 1>), pgs_shade_args)
-PGS_DEFINE_MULTI(256, k_shade_ssaa_11_multi, (shade_ssaa_body<1, 1>), pgs_shade_args)
-PGS_DEFINE_MULTI(256, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_args)
-PGS_DEFINE_MULTI(256, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_1x_multi, shade_1x_body, pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_01_multi, (shade_ssaa_body<0, 1>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_11_multi, (shade_ssaa_body<1, 1>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
 
 // ---------------------------------------------------------------------------------------
 // Work-queue shading by coarse tile: a grid of resident workgroups pops queue items (coarse

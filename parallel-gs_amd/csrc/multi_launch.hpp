@@ -29,10 +29,11 @@ __device__ __forceinline__ uint32_t find_launch(const uint32_t *prefix, uint32_t
 #define PGS_OCCUPANCY_8 __attribute__((amdgpu_waves_per_eu(8)))
 
 // Defines `NAME(const ARGS *args, const uint32_t *prefix, uint32_t n)` running BODY for the
-// launch that owns each workgroup.
-#define PGS_DEFINE_MULTI(THREADS, NAME, BODY, ARGS)                                               \
-	__global__ void __launch_bounds__(THREADS) PGS_OCCUPANCY_8 NAME(const ARGS *args, const uint32_t *prefix, uint32_t n) \
+// launch that owns each workgroup; the _ATTR form adds a kernel attribute (PGS_OCCUPANCY_8).
+#define PGS_DEFINE_MULTI_ATTR(THREADS, ATTR, NAME, BODY, ARGS)                                      \
+	__global__ void __launch_bounds__(THREADS) ATTR NAME(const ARGS *args, const uint32_t *prefix, uint32_t n) \
 	{                                                                                               \
 		const uint32_t l = ::pgs::find_launch(prefix, n, blockIdx.x);                               \
 		BODY(args[l], blockIdx.x - prefix[l], prefix[l + 1] - prefix[l]);                            \
 	}
+#define PGS_DEFINE_MULTI(THREADS, NAME, BODY, ARGS) PGS_DEFINE_MULTI_ATTR(THREADS, , NAME, BODY, ARGS)

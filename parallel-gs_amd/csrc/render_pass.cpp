@@ -467,9 +467,34 @@ void RenderPassBuilder::flush()
 		d.tex_descs = tex_descs_.data();
 		d.num_textures = uint32_t(decodes_.size());
 		d.tex_groups = group_total_;
+		const uint32_t mode = renderer_->sampler_feedback_mode();
+		d.sparse_decode = mode == 1u || (mode == 0u && texel_total_ >= Renderer::kFeedbackAutoTexels);
 		renderer_->submit_render_pass(d);
+
+		// Whole decoded images join the cache (a sparse decode holds only this pass's blocks),
+		// stamped before this pass's own frame-buffer and depth writes are logged.
+		uint32_t decoded = 0;
+		for (SlotImage &im : slot_images_)
+			if (im.decoded)
+			{
+				decoded++;
+				if (!d.sparse_decode)
+				{
+					im.entry.decoded_at = writes_.now();
+					cache_.push_back(im.entry);
+				}
+			}
+		renderer_->note_textures(decoded, uint32_t(slot_images_.size()) - decoded);
+		if (bb_[2] >= bb_[0] && bb_[3] >= bb_[1])
+		{
+			const uint32_t x0 = uint32_t(bb_[0]), y0 = uint32_t(bb_[1]), x1 = uint32_t(bb_[2]), y1 = uint32_t(bb_[3]);
+			writes_.write(page_span(d.fbp * 32u, d.fbw, d.fb_psm, x0, y0, x1, y1));
+			if (d.z_write)
+				writes_.write(page_span(d.zbp * 32u, d.fbw, d.zbuf_psm, x0, y0, x1, y1));
+		}
 		prims_.clear();
 		slot_keys_.clear();
+		slot_images_.clear();
 		decodes_.clear();
 		tex_descs_.clear();
 		texel_total_ = group_total_ = 0;
@@ -508,6 +533,26 @@ bool RenderPassBuilder::texture_hazard(const RegisterState &r, uint32_t ctx)
 	return false;
 }
 
+// Image arena as a ring: the next image goes after the last one, or at the start when it
+// does not fit before the end. Cached images it overlaps are dropped; an image of the current
+// pass must not be overwritten, so then the pass has to end first (false).
+bool RenderPassBuilder::allocate_image(uint32_t texels, uint32_t &offset)
+{
+	uint32_t off = ring_;
+	if (uint64_t(off) + texels > kTexImageTexels)
+		off = 0;
+	auto overlaps = [&](uint32_t o, uint32_t n) { return o < off + texels && off < o + n; };
+	for (const SlotImage &im : slot_images_)
+		if (overlaps(im.offset, im.texels))
+			return false;
+	cache_.erase(std::remove_if(cache_.begin(), cache_.end(),
+	                            [&](const CachedTexture &c) { return overlaps(c.offset, c.texels); }),
+	             cache_.end());
+	ring_ = off + texels;
+	offset = off;
+	return true;
+}
+
 int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 {
 	const uint64_t key0 = r.tex0[ctx] & ((1ull << 61) - 1ull); // CLD does not change the texture
@@ -518,13 +563,42 @@ int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 
 	const TexLayout l = tex_layout(key0, keyc);
 	const uint32_t texels = l.width * l.height;
-	if (slot_keys_.size() >= kMaxPassTextures || texel_total_ + texels > kTexImageTexels)
+	if (slot_keys_.size() >= kMaxPassTextures || texels > kTexImageTexels)
 		return prims_.empty() ? -2 : -1;
+
+	// A valid cached image of an earlier pass is reused; otherwise the pass decodes one.
+	const bool palette = l.psm == PSMT8 || l.psm == PSMT4 || l.psm == PSMT8H || l.psm == PSMT4HL || l.psm == PSMT4HH;
+	const SlotKey key{ key0, keyc, keya };
+	SlotImage im{};
+	bool hit = false;
+	for (size_t i = 0; i < cache_.size(); i++)
+	{
+		const CachedTexture &c = cache_[i];
+		if (c.key.tex0 != key0 || c.key.clamp != keyc || c.key.texa != keya)
+			continue;
+		if (!writes_.written_since(c.span, c.decoded_at) && (!c.palette || c.clut_gen == writes_.clut_generation()))
+		{
+			im.offset = c.offset;
+			im.texels = c.texels;
+			hit = true;
+		}
+		else
+			cache_.erase(cache_.begin() + std::ptrdiff_t(i));
+		break;
+	}
+	if (!hit)
+	{
+		if (!allocate_image(texels, im.offset))
+			return prims_.empty() ? -2 : -1;
+		im.texels = texels;
+		im.decoded = true;
+		im.entry = { key, im.offset, texels, 0, writes_.clut_generation(), palette, texture_span(l) };
+	}
 
 	// TexInfo (gs/gs_interface.cpp:1752-1781).
 	const uint32_t wms = field(keyc, 0, 2), wmt = field(keyc, 2, 2);
 	TextureDesc td{};
-	td.image_offset = texel_total_;
+	td.image_offset = im.offset;
 	td.width = l.width;
 	td.height = l.height;
 	if (wms == 1) { td.region[0] = 0.0f; td.region[2] = float(l.width) - 1.0f; }
@@ -538,10 +612,12 @@ int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 	td.bias[0] = -float(l.bx) * td.size[2];
 	td.bias[1] = -float(l.by) * td.size[3];
 
+	// Decode work; a reused image gets an empty entry (no workgroups), which keeps decode,
+	// descriptor and sampler-feedback indices equal to the slot.
 	TexDecode dd{};
-	dd.image_offset = texel_total_;
-	dd.width = l.width;
-	dd.height = l.height;
+	dd.image_offset = im.offset;
+	dd.width = im.decoded ? l.width : 0u;
+	dd.height = im.decoded ? l.height : 0u;
 	dd.tbp = l.tbp;
 	dd.tbw = l.tbw;
 	dd.psm = l.psm;
@@ -556,11 +632,15 @@ int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 	dd.ta1 = field(keya, 32, 8);
 	dd.group_begin = group_total_;
 
-	slot_keys_.push_back({ key0, keyc, keya });
+	slot_keys_.push_back(key);
+	slot_images_.push_back(im);
 	decodes_.push_back(dd);
 	tex_descs_.push_back(td);
-	texel_total_ += texels;
-	group_total_ += (texels + 255u) / 256u;
+	if (im.decoded)
+	{
+		texel_total_ += texels;
+		group_total_ += (texels + 255u) / 256u;
+	}
 	return int32_t(slot_keys_.size() - 1);
 }
 }

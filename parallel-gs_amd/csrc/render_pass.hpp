@@ -6,6 +6,8 @@
 // in host arrays until the pass ends; Renderer::submit_render_pass uploads them and runs the
 // setup / binning / shading kernels.
 #pragma once
+#include <algorithm>
+#include <array>
 #include "gs_defs.hpp"
 #include "gs_raster.hpp"
 #include "renderer.hpp"
@@ -37,6 +39,48 @@ struct RenderPassDesc
 	const TextureDesc *tex_descs;
 	uint32_t num_textures;
 	uint32_t tex_groups;      // 256-texel decode workgroups over all textures
+	bool sparse_decode;       // decode only the 8x8 blocks primitives can sample (sampler feedback)
+};
+
+// Write generations of the VRAM pages and of the CLUT, for the texture cache: a decoded image
+// stays valid while no page it was decoded from has been written since, and (palette formats)
+// the CLUT has not been reloaded. Writes are logged when they are queued; the GPU executes
+// them in the same order as the passes that read the cache.
+class VramWriteLog
+{
+public:
+	void write(PageSpan s)
+	{
+		clock_++;
+		if (s.last - s.first + 1u >= kNumPages)
+		{
+			page_.fill(clock_);
+			return;
+		}
+		for (uint32_t p = s.first; p <= s.last; p++)
+			page_[p % kNumPages] = clock_;
+	}
+	void write_all()
+	{
+		clock_++;
+		page_.fill(clock_);
+	}
+	void clut_loaded() { clut_gen_++; }
+	uint64_t now() const { return clock_; }
+	uint64_t clut_generation() const { return clut_gen_; }
+	bool written_since(PageSpan s, uint64_t t) const
+	{
+		if (s.last - s.first + 1u >= kNumPages)
+			return std::any_of(page_.begin(), page_.end(), [&](uint64_t g) { return g > t; });
+		for (uint32_t p = s.first; p <= s.last; p++)
+			if (page_[p % kNumPages] > t)
+				return true;
+		return false;
+	}
+
+private:
+	uint64_t clock_ = 1, clut_gen_ = 0;
+	std::array<uint64_t, kNumPages> page_{};
 };
 
 // Per-pass texture limits: slot index width of PrimitiveAttribute.tex and the decoded image
@@ -59,6 +103,11 @@ public:
 
 	// Ends the render pass: submits all pending primitives.
 	void flush();
+
+	// VRAM and CLUT writes the texture cache must see (transfers, CLUT loads, host writes).
+	VramWriteLog &writes() { return writes_; }
+	// Forgets every cached texture image (a recording must decode everything it samples).
+	void clear_texture_cache() { cache_.clear(); }
 
 	// Callbacks into the front end: before_first_primitive runs when a pass starts (pending
 	// transfers must execute first); before_texture_read runs before a textured primitive is
@@ -97,6 +146,28 @@ private:
 
 	struct SlotKey { uint64_t tex0, clamp, texa; };
 	std::vector<SlotKey> slot_keys_;
+	// Texture cache across passes (gs/gs_renderer.cpp:1317-1386 create_cached_texture, with
+	// page_tracker.cpp:842-933 invalidation): fully decoded images of earlier passes, kept in
+	// the image arena used as a ring.
+	struct CachedTexture
+	{
+		SlotKey key;
+		uint32_t offset, texels;
+		uint64_t decoded_at, clut_gen;
+		bool palette;
+		PageSpan span;
+	};
+	struct SlotImage
+	{
+		uint32_t offset, texels;
+		bool decoded; // decoded by this pass (else reused from the cache)
+		CachedTexture entry;
+	};
+	std::vector<CachedTexture> cache_;
+	std::vector<SlotImage> slot_images_;
+	uint32_t ring_ = 0; // next free texel of the image arena
+	VramWriteLog writes_;
+	bool allocate_image(uint32_t texels, uint32_t &offset);
 	std::vector<TexDecode> decodes_;
 	std::vector<TextureDesc> tex_descs_;
 	uint32_t texel_total_ = 0, group_total_ = 0;

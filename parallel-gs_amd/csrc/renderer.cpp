@@ -554,14 +554,10 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	}
 	arena_upload();
 
-	// Sparse texture decode (sampler feedback) where a pass decodes many texels: setup clears
-	// the masks, the feedback launch marks the 8x8 blocks the primitives can sample, the decode
-	// skips the others. Small passes decode whole textures (the extra launch costs more).
-	uint64_t pass_texels = 0;
-	for (uint32_t t = 0; t < d.num_textures; t++)
-		pass_texels += uint64_t(d.decodes[t].width) * d.decodes[t].height;
-	const bool feedback = d.num_textures != 0 && sampler_feedback_ != 2u &&
-	                      (sampler_feedback_ == 1u || pass_texels >= kFeedbackAutoTexels);
+	// Sparse texture decode (sampler feedback, chosen by the pass builder): setup clears the
+	// masks, the feedback launch marks the 8x8 blocks the primitives can sample, the decode
+	// skips the others.
+	const bool feedback = d.num_textures != 0 && d.sparse_decode;
 
 	Launch ls{};
 	ls.kind = LaunchKind::Setup;

@@ -26,6 +26,8 @@ struct FlushStats
 	uint64_t render_passes = 0;
 	uint64_t primitives = 0;
 	uint64_t syncs = 0;
+	uint64_t textures_decoded = 0;   // textures decoded by render passes
+	uint64_t texture_cache_hits = 0; // textures a pass reused from an earlier pass
 };
 
 class Renderer
@@ -56,6 +58,12 @@ public:
 	// Coarse tile size (log2 pixels) used at 1x; super-sampled passes use the reference's
 	// binning-cost heuristic because the tile grid decides super- vs single-sampled shading.
 	void set_tile_log2(uint32_t v) { tile_log2_1x_ = v; }
+	uint32_t sampler_feedback_mode() const { return sampler_feedback_; }
+	void note_textures(uint32_t decoded, uint32_t hits)
+	{
+		stats_.textures_decoded += decoded;
+		stats_.texture_cache_hits += hits;
+	}
 	// Runs a CLUT load (the caller has ordered it after every VRAM write it must see).
 	void clut_load(const pgs_clut_args &args);
 
@@ -140,7 +148,12 @@ private:
 	uint64_t *coarse_mask_ = nullptr; // first binning level: kMaxCoarseTiles x kMaxMaskWords
 	uint32_t *feedback_masks_ = nullptr; // sampler feedback: kMaxPassTextures x PGS_FEEDBACK_WORDS
 	uint32_t sampler_feedback_ = 0;
+
+public:
+	// Sparse decode where a pass decodes at least this many texels (sampler_feedback 0).
 	static constexpr uint64_t kFeedbackAutoTexels = 1u << 20;
+
+private:
 	static constexpr uint32_t kMaxCoarseTiles = 256, kMaxMaskWords = 1024;
 	uint8_t *snapshot_ = nullptr;     // VRAM planes + CLUT at begin_recording()
 	size_t lists_cap_ = 0;            // entries

@@ -53,6 +53,8 @@ class FlushStats(ctypes.Structure):
         ("render_passes", ctypes.c_uint64),
         ("primitives", ctypes.c_uint64),
         ("syncs", ctypes.c_uint64),
+        ("textures_decoded", ctypes.c_uint64),
+        ("texture_cache_hits", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -98,3 +98,66 @@ def test_feedback_large_texture_windows(hip_device, ssaa):
     assert_equal(render(stream, ssaa, "always"), ref)
     assert_equal(render(stream, ssaa, "never"), ref)
     assert_equal(render(stream, ssaa, "auto"), ref)
+
+
+def texture_cache_stream(seed: int) -> np.ndarray:
+    """Passes that reuse a texture and passes after each kind of write that invalidates it:
+    - pass 1 decodes a 64x64 CT32 texture, pass 2 (another frame buffer) reuses it;
+    - an upload into the texture's pages, then pass 3 must decode again, pass 4 reuses;
+    - a T8 texture with CLUT load (pass 5), reused without a load (pass 6), CLD=1 again (a new
+      CLUT load) decodes again (pass 7);
+    - a pass drawing into the texture's pages (pass 8), after which pass 9 decodes again.
+    Expected reuses: passes 2, 4 and 6."""
+    rng = np.random.default_rng(seed)
+    ofs = 2048 * 16
+    out = [gif.ad([(0x1A, 1), (0x18, ofs | ofs << 32)])]
+    t32, t8, clut = 300 * 32, 320 * 32, 340 * 32
+    out.append(gif.upload(t32, 1, 0x00, 0, 0, 64, 64, rng.integers(0, 2**32, 64 * 64, dtype=np.uint32).tobytes()))
+    out.append(gif.upload(t8, 1, 0x13, 0, 0, 64, 64, rng.integers(0, 256, 64 * 64, dtype=np.uint8).tobytes()))
+    out.append(gif.upload(clut, 1, 0x00, 0, 0, 16, 16, rng.integers(0, 2**32, 256, dtype=np.uint32).tobytes()))
+
+    def draw(fbp, tex0, textured=True, full=False):
+        regs = [(0x4C, R.frame(fbp, 4, 0x00)), (0x4E, R.zbuf(64, 1, 1)), (0x40, R.scissor(0, 255, 0, 191)),
+                (0x47, R.test_reg(zte=1, ztst=1)),
+                (0x14, R.tex1(1, 1)), (0x08, R.clamp_reg(0, 0)), (0x06, tex0),
+                (0x00, R.prim_reg(6, tme=int(textured), fst=1))]
+        writes = []
+        for k in range(1 if full else 4):
+            x0, y0, size = int(rng.integers(0, 200)), int(rng.integers(0, 140)), int(rng.integers(8, 48))
+            if full:
+                x0, y0, size = 0, 0, 256
+            u0, v0 = int(rng.integers(0, 64)), int(rng.integers(0, 64))
+            for dx in (0, size):
+                writes.append((0x01, R.rgbaq(int(rng.integers(0, 2**32)))))
+                writes.append((0x03, R.uv((u0 + dx) * 16, (v0 + dx) * 16)))
+                writes.append((0x05, R.xyz2(ofs + (x0 + dx) * 16, ofs + (y0 + dx) * 16, 0)))
+        out.append(gif.ad(regs + writes))
+
+    T32 = R.tex0(t32, 1, 0x00, 6, 6, tcc=1, tfx=1)
+    T8_load = R.tex0(t8, 1, 0x13, 6, 6, tcc=1, tfx=1, cbp=clut, cpsm=0, csm=0, cld=1)
+    T8_keep = R.tex0(t8, 1, 0x13, 6, 6, tcc=1, tfx=1, cbp=clut, cpsm=0, csm=0, cld=0)
+    draw(0, T32)                                   # 1: decode
+    draw(100, T32)                                 # 2: reuse
+    out.append(gif.upload(t32, 1, 0x00, 16, 16, 8, 8, rng.integers(0, 2**32, 64, dtype=np.uint32).tobytes()))
+    draw(0, T32)                                   # 3: decode (pages written)
+    draw(100, T32)                                 # 4: reuse
+    draw(0, T8_load)                               # 5: CLUT load, decode
+    draw(100, T8_keep)                             # 6: reuse
+    draw(0, T8_load)                               # 7: CLUT load again, decode
+    draw(300, T32, textured=False, full=True)      # 8: draws over the T32 pages
+    draw(0, T32)                                   # 9: decode
+    draw(100, T32)                                 # flushes pass 9
+    return gif.arr(b"".join(out))
+
+
+def test_texture_cache_reuse_and_invalidation(hip_device):
+    stream = texture_cache_stream(31)
+    print(f"[case] texture cache stream {len(stream)} bytes", flush=True)
+    with pgs.Context(sampler_feedback="never") as ctx:
+        ctx.gif_transfer(stream)
+        got = ctx.vram_planes()
+        stats = ctx.consume_flush_stats()
+    assert_equal(got, oracle(stream, 1))
+    print("decoded", stats["textures_decoded"], "reused", stats["texture_cache_hits"], flush=True)
+    assert stats["texture_cache_hits"] == 4  # passes 2, 4, 6 and the last pass (reusing pass 9's decode)
+    assert stats["textures_decoded"] == 5    # passes 1, 3, 5, 7, 9
