@@ -85,6 +85,8 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_f
 		check(hipHostMalloc(&arena_.host, arena_.cap, hipHostMallocDefault));
 		check(hipMalloc(&arena_.dev, arena_.cap));
 		batch_written_.assign(kNumBlocks / 64, 0);
+		level_written_.assign(kNumBlocks, -1);
+		level_read_.assign(kNumBlocks, -1);
 		check(hipMalloc(&setup_buf_, 65536u * sizeof(PrimitiveSetup)));
 		check(hipMalloc(&tattr_buf_, 65536u * sizeof(TransformedAttributes)));
 		lists_cap_ = 32u << 20;  // 64 MiB of uint16 entries
@@ -272,22 +274,21 @@ void Renderer::launch(const Launch &l)
 // ---------------------------------------------------------------------------------------
 // Transfer batching.
 //
-// A batch is a run of copies with no read-after-write hazard: a local copy whose source
-// blocks were written by an earlier copy of the batch closes the batch first (the
-// reference flushes on the same condition, page_tracker.cpp:221-245, at 8 KiB page
-// granularity; here the granularity is the 256-byte block). Write-after-write between
-// copies is resolved inside the batch by the per-block ordered kernel, and local copies
-// whose source is overwritten within the batch read a snapshot taken at batch start.
+// Copies queue until something needs VRAM (a render pass, a CLUT load, a readback, a full
+// arena); then each copy gets a dependency level and every level runs as one launch. The
+// reference flushes a batch whenever a local copy reads what the batch wrote
+// (page_tracker.cpp:221-245, at 8 KiB page granularity); levels at the 256-byte block
+// granularity give one launch per step of the read-after-write chain instead, and copies
+// that do not depend on that chain join earlier launches. Inside a level, write-after-write
+// is resolved by the per-block ordered kernel, and local copies whose source the level
+// overwrites read a snapshot taken at level start.
 
 void Renderer::queue_host_copy(const CopyDesc &desc_in, const void *payload, size_t bytes)
 {
 	CopyDesc desc = desc_in;
 	size_t meta = copy_meta_bound(desc);
 	size_t need = align_up(bytes, kAlign) + meta;
-	size_t reserved = 0;
-	for (auto &p : batch_)
-		reserved += copy_meta_bound(p.desc);
-	if (arena_.used + reserved + need > arena_.cap)
+	if (arena_.used + batch_meta_ + need > arena_.cap)
 	{
 		if (!batch_.empty())
 			flush_transfers();
@@ -301,9 +302,8 @@ void Renderer::queue_host_copy(const CopyDesc &desc_in, const void *payload, siz
 	desc.payload_bytes = uint32_t(bytes);
 	desc.flags &= ~COPY_LOCAL;
 
-	for_each_rect_block(desc.dpsm, desc.dbp, desc.dbw, desc.dsax, desc.dsay, desc.w, desc.h,
-	                    [&](uint32_t b, uint32_t, uint32_t) { batch_written_[b >> 6] |= 1ull << (b & 63); });
 	batch_.push_back({ desc });
+	batch_meta_ += copy_meta_bound(desc);
 }
 
 void Renderer::queue_local_copy(const CopyDesc &desc_in)
@@ -315,29 +315,16 @@ void Renderer::queue_local_copy(const CopyDesc &desc_in)
 	desc.first_pixel = 0;
 	desc.end_pixel = desc.w * desc.h;
 
-	bool raw = false;
-	for_each_rect_block(desc.spsm, desc.sbp, desc.sbw, desc.ssax, desc.ssay, desc.w, desc.h,
-	                    [&](uint32_t b, uint32_t, uint32_t) {
-		                    if (batch_written_[b >> 6] & (1ull << (b & 63)))
-			                    raw = true;
-	                    });
-	if (raw)
-		flush_transfers();
-
 	size_t meta = copy_meta_bound(desc);
-	size_t reserved = 0;
-	for (auto &p : batch_)
-		reserved += copy_meta_bound(p.desc);
-	if (arena_.used + reserved + meta > arena_.cap)
+	if (arena_.used + batch_meta_ + meta > arena_.cap)
 	{
 		if (!batch_.empty())
 			flush_transfers();
 		arena_make_room(meta);
 	}
 
-	for_each_rect_block(desc.dpsm, desc.dbp, desc.dbw, desc.dsax, desc.dsay, desc.w, desc.h,
-	                    [&](uint32_t b, uint32_t, uint32_t) { batch_written_[b >> 6] |= 1ull << (b & 63); });
 	batch_.push_back({ desc });
+	batch_meta_ += copy_meta_bound(desc);
 }
 
 void Renderer::flush_transfers()
@@ -345,44 +332,94 @@ void Renderer::flush_transfers()
 	if (batch_.empty())
 		return;
 
-	const uint32_t ncopies = uint32_t(batch_.size());
+	// Level of every queued copy, in queue order: after the levels of the copies that last
+	// wrote its source blocks (read-after-write), and not before the levels of the copies
+	// that last wrote or read its destination blocks (write-after-write, write-after-read:
+	// inside one level destination blocks apply copies in queue order and sources that the
+	// level overwrites are read from the level-start snapshot).
+	std::fill(level_written_.begin(), level_written_.end(), -1);
+	std::fill(level_read_.begin(), level_read_.end(), -1);
+	std::vector<uint32_t> level(batch_.size());
+	uint32_t levels = 0;
+	for (size_t j = 0; j < batch_.size(); j++)
+	{
+		const CopyDesc &d = batch_[j].desc;
+		const bool local = (d.flags & COPY_LOCAL) != 0;
+		int32_t L = 0;
+		if (local)
+			for_each_rect_block(d.spsm, d.sbp, d.sbw, d.ssax, d.ssay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
+				L = std::max(L, level_written_[b] + 1);
+			});
+		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
+			L = std::max(L, std::max(level_written_[b], level_read_[b]));
+		});
+		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h,
+		                    [&](uint32_t b, uint32_t, uint32_t) { level_written_[b] = L; });
+		if (local)
+			for_each_rect_block(d.spsm, d.sbp, d.sbw, d.ssax, d.ssay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
+				level_read_[b] = std::max(level_read_[b], L);
+			});
+		level[j] = uint32_t(L);
+		levels = std::max(levels, uint32_t(L) + 1u);
+	}
+	std::vector<std::vector<uint32_t>> members(levels);
+	for (uint32_t j = 0; j < uint32_t(batch_.size()); j++)
+		members[level[j]].push_back(j);
+	for (const auto &m : members)
+		emit_transfer_level(m);
+	batch_.clear();
+	batch_meta_ = 0;
+}
 
-	// 1. Local copies whose source the batch overwrites read from the batch-start snapshot.
+// One level of queued copies: one snapshot launch (if a local copy reads a block the level
+// writes) and one copy launch.
+void Renderer::emit_transfer_level(const std::vector<uint32_t> &members)
+{
+	const uint32_t ncopies = uint32_t(members.size());
+	std::fill(batch_written_.begin(), batch_written_.end(), 0);
+	for (uint32_t j : members)
+	{
+		const CopyDesc &d = batch_[j].desc;
+		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h,
+		                    [&](uint32_t b, uint32_t, uint32_t) { batch_written_[b >> 6] |= 1ull << (b & 63); });
+	}
+
+	// 1. Local copies whose source the level overwrites read from the level-start snapshot.
+	std::vector<CopyDesc> level_descs(ncopies);
 	std::vector<uint32_t> snapshot;
 	std::vector<uint64_t> snap_mask(kNumBlocks / 64, 0);
-	for (auto &p : batch_)
+	for (uint32_t ci = 0; ci < ncopies; ci++)
 	{
-		if (!(p.desc.flags & COPY_LOCAL))
+		CopyDesc &d = level_descs[ci];
+		d = batch_[members[ci]].desc;
+		if (!(d.flags & COPY_LOCAL))
 			continue;
 		bool hazard = false;
-		for_each_rect_block(p.desc.spsm, p.desc.sbp, p.desc.sbw, p.desc.ssax, p.desc.ssay, p.desc.w, p.desc.h,
-		                    [&](uint32_t b, uint32_t, uint32_t) {
-			                    if (batch_written_[b >> 6] & (1ull << (b & 63)))
-				                    hazard = true;
-		                    });
+		for_each_rect_block(d.spsm, d.sbp, d.sbw, d.ssax, d.ssay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
+			if (batch_written_[b >> 6] & (1ull << (b & 63)))
+				hazard = true;
+		});
 		if (!hazard)
 			continue;
-		p.desc.flags |= COPY_SOURCE_SHADOW;
-		for_each_rect_block(p.desc.spsm, p.desc.sbp, p.desc.sbw, p.desc.ssax, p.desc.ssay, p.desc.w, p.desc.h,
-		                    [&](uint32_t b, uint32_t, uint32_t) {
-			                    uint64_t bit = 1ull << (b & 63);
-			                    if (!(snap_mask[b >> 6] & bit))
-			                    {
-				                    snap_mask[b >> 6] |= bit;
-				                    snapshot.push_back(b);
-			                    }
-		                    });
+		d.flags |= COPY_SOURCE_SHADOW;
+		for_each_rect_block(d.spsm, d.sbp, d.sbw, d.ssax, d.ssay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
+			uint64_t bit = 1ull << (b & 63);
+			if (!(snap_mask[b >> 6] & bit))
+			{
+				snap_mask[b >> 6] |= bit;
+				snapshot.push_back(b);
+			}
+		});
 	}
 
 	// 2. Count records per destination block.
 	size_t nrec = 0;
-	for (auto &p : batch_)
-		for_each_rect_block(p.desc.dpsm, p.desc.dbp, p.desc.dbw, p.desc.dsax, p.desc.dsay, p.desc.w, p.desc.h,
-		                    [&](uint32_t b, uint32_t, uint32_t) {
-			                    if (block_count_[b]++ == 0)
-				                    touched_blocks_.push_back(b);
-			                    nrec++;
-		                    });
+	for (const CopyDesc &d : level_descs)
+		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
+			if (block_count_[b]++ == 0)
+				touched_blocks_.push_back(b);
+			nrec++;
+		});
 	std::sort(touched_blocks_.begin(), touched_blocks_.end());
 
 	size_t off_desc, off_rec, off_blk, off_snap = 0;
@@ -405,12 +442,11 @@ void Renderer::flush_transfers()
 	}
 	for (uint32_t ci = 0; ci < ncopies; ci++)
 	{
-		const CopyDesc &d = batch_[ci].desc;
+		const CopyDesc &d = level_descs[ci];
 		descs[ci] = d;
-		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h,
-		                    [&](uint32_t b, uint32_t x0, uint32_t y0) {
-			                    recs[block_count_[b]++] = { ci, uint16_t(x0), uint16_t(y0) };
-		                    });
+		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h, [&](uint32_t b, uint32_t x0, uint32_t y0) {
+			recs[block_count_[b]++] = { ci, uint16_t(x0), uint16_t(y0) };
+		});
 	}
 	for (uint32_t b : touched_blocks_)
 		block_count_[b] = 0;
@@ -445,16 +481,12 @@ void Renderer::flush_transfers()
 	stats_.copy_blocks += touched_blocks_.size();
 	stats_.copy_records += nrec;
 	stats_.snapshot_blocks += snapshot.size();
-	for (auto &p : batch_)
+	for (const CopyDesc &d : level_descs)
 	{
-		uint64_t pixels = (p.desc.flags & COPY_LOCAL) ? uint64_t(p.desc.w) * p.desc.h
-		                                               : uint64_t(p.desc.end_pixel - p.desc.first_pixel);
-		stats_.copy_bytes += 2u * pixels * psm_bpp(p.desc.dpsm) / 8u;
+		uint64_t pixels = (d.flags & COPY_LOCAL) ? uint64_t(d.w) * d.h : uint64_t(d.end_pixel - d.first_pixel);
+		stats_.copy_bytes += 2u * pixels * psm_bpp(d.dpsm) / 8u;
 	}
-
 	touched_blocks_.clear();
-	batch_.clear();
-	std::fill(batch_written_.begin(), batch_written_.end(), 0);
 }
 
 // ---------------------------------------------------------------------------------------

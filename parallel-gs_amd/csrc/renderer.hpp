@@ -164,6 +164,7 @@ private:
 	{
 		CopyDesc desc;
 	};
+	void emit_transfer_level(const std::vector<uint32_t> &members);
 
 	uint32_t sample_log2_ = 0;
 	uint32_t num_planes_ = 1;
@@ -187,7 +188,10 @@ private:
 
 	// Current transfer batch.
 	std::vector<PendingCopy> batch_;
-	std::vector<uint64_t> batch_written_;   // bitset over 16384 blocks
+	std::vector<uint64_t> batch_written_;   // bitset over 16384 blocks: written by the level being emitted
+	std::vector<int32_t> level_written_;    // per block: level of the last queued copy writing it (-1: none)
+	std::vector<int32_t> level_read_;       // per block: highest level of a queued copy reading it
+	size_t batch_meta_ = 0;                 // arena bytes the queued copies' metadata needs at flush
 	std::vector<uint32_t> block_count_;     // per-block record counts (scratch)
 	std::vector<uint32_t> touched_blocks_;  // blocks with nonzero count
 
