@@ -119,14 +119,18 @@ __device__ __forceinline__ bool alpha_test(uint32_t atst, uint32_t aref, float a
 }
 
 // Texel fetch with REPEAT addressing on the decoded image.
-__device__ __forceinline__ uint32_t texel(const uint32_t *img, const TextureDesc &t, int32_t x, int32_t y)
+// Wrap of a texel coordinate into [0, size) (REPEAT on the decoded image), for |v| < 2^22:
+// v and q * size are exact in float, floor(v * (1 / size)) is off by at most one, and the two
+// selects correct that, so the result is the mathematical remainder v mod size for every size
+// (seven full-rate instructions instead of an integer division).
+__device__ __forceinline__ uint32_t wrap_texel(int32_t v, float size, float inv_size)
 {
-	const int32_t w = int32_t(t.width), h = int32_t(t.height);
-	x %= w;
-	y %= h;
-	x += x < 0 ? w : 0;
-	y += y < 0 ? h : 0;
-	return img[uint32_t(y) * t.width + uint32_t(x)];
+	const float vf = float(v);
+	const float q = floorf(__fmul_rn(vf, inv_size));
+	float r = __fsub_rn(vf, __fmul_rn(q, size));
+	r = r < 0.0f ? __fadd_rn(r, size) : r;
+	r = r >= size ? __fsub_rn(r, size) : r;
+	return uint32_t(r);
 }
 
 // a + round((b - a) * w / 256) with round-half-up, w an 8-bit fraction.
@@ -157,7 +161,9 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 	uint32_t c;
 	if (!(tex & TEX_MAG_LINEAR))
 	{
-		c = texel(img, t, int32_t(floorf(__fmul_rn(un, fw))), int32_t(floorf(__fmul_rn(vn, fh))));
+		const uint32_t x = wrap_texel(int32_t(floorf(__fmul_rn(un, fw))), fw, t.size[2]);
+		const uint32_t y = wrap_texel(int32_t(floorf(__fmul_rn(vn, fh))), fh, t.size[3]);
+		c = img[y * t.width + x];
 	}
 	else
 	{
@@ -165,9 +171,11 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 		const float x0 = floorf(xs), y0 = floorf(ys);
 		const uint32_t wx = uint32_t(__fmul_rn(__fsub_rn(xs, x0), 256.0f));
 		const uint32_t wy = uint32_t(__fmul_rn(__fsub_rn(ys, y0), 256.0f));
-		const int32_t ix = int32_t(x0), iy = int32_t(y0);
-		const uint32_t c00 = texel(img, t, ix, iy), c10 = texel(img, t, ix + 1, iy);
-		const uint32_t c01 = texel(img, t, ix, iy + 1), c11 = texel(img, t, ix + 1, iy + 1);
+		// REPEAT on the decoded image; the second texel of each axis wraps from the first.
+		const uint32_t ax = wrap_texel(int32_t(x0), fw, t.size[2]), ay = wrap_texel(int32_t(y0), fh, t.size[3]);
+		const uint32_t bx = ax + 1u == t.width ? 0u : ax + 1u, by = ay + 1u == t.height ? 0u : ay + 1u;
+		const uint32_t r0 = ay * t.width, r1 = by * t.width;
+		const uint32_t c00 = img[r0 + ax], c10 = img[r0 + bx], c01 = img[r1 + ax], c11 = img[r1 + bx];
 		c = 0;
 		for (uint32_t ch = 0; ch < 4; ch++)
 		{

@@ -8,6 +8,8 @@ IEEE single precision with numpy.
   rint(clamp(fract(o / 256) * (256 / 255), 0, 1) * 255) == o & 255 (wrap) and
   rint(clamp(o * (1 / 255), 0, 1) * 255) == clamp(o, 0, 255) (COLCLAMP);
 * division by a power of two equals multiplication by its reciprocal;
+* the texel wrap done in float (floor of v times the reciprocal size, two corrections) is the
+  mathematical remainder for every image size up to 1024 and |v| < 2^22;
 * b / 255 is not b * fl(1 / 255) for every byte, which is why the kernels keep an exact table.
 """
 import numpy as np
@@ -46,3 +48,28 @@ def test_power_of_two_division_is_reciprocal_multiplication():
 def test_reciprocal_multiplication_is_not_exact_for_255():
     b = np.arange(256, dtype=F)
     assert not np.array_equal(unorm8(b), (b * (F(1) / F(255))).astype(F))
+
+
+def wrap_texel(v, size):
+    """The shader's texel wrap in float32 steps (kernels_shade.hip wrap_texel)."""
+    vf = v.astype(F)
+    inv = F(1) / F(size)
+    q = np.floor((vf * inv).astype(F)).astype(F)
+    r = (vf - (q * F(size)).astype(F)).astype(F)
+    r = np.where(r < 0, (r + F(size)).astype(F), r)
+    r = np.where(r >= F(size), (r - F(size)).astype(F), r)
+    return r.astype(np.int64)
+
+
+def test_float_texel_wrap_is_the_remainder():
+    """For every image size 1..1024 and coordinates below 2^22 in magnitude (multiples of the
+    size and their neighbours, where the float quotient can round across an integer, the
+    extremes, and random values), the float wrap equals the mathematical remainder."""
+    rng = np.random.default_rng(9)
+    lim = (1 << 22) - 1
+    for size in range(1, 1025):
+        k = np.arange(-(lim // size), lim // size + 1, max(1, (lim // size) // 512), dtype=np.int64)
+        mult = np.concatenate([k * size - 1, k * size, k * size + 1])
+        v = np.concatenate([mult, rng.integers(-lim, lim, 2048), [-lim, lim, 0, -1, 1]])
+        v = v[np.abs(v) <= lim]
+        assert np.array_equal(wrap_texel(v, size), np.mod(v, size)), size
