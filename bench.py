@@ -117,12 +117,13 @@ def bench_transfers(args):
     }
 
 
-def replay_timed(ctx, steps, warmup):
-    """Times `steps` graph replays of the recorded stream with HIP events on its stream."""
+def replay_timed(ctx, steps, warmup, use_graph=True):
+    """Times `steps` replays (graph replays by default) of the recorded stream with HIP events on
+    its stream; each replay ends with the context's texture stream joined to that stream."""
     import torch
 
     for _ in range(warmup):
-        ctx.replay(True)
+        ctx.replay(use_graph)
     ctx.sync()
     s = torch.cuda.ExternalStream(ctx.stream)
     e0 = torch.cuda.Event(enable_timing=True)
@@ -130,7 +131,7 @@ def replay_timed(ctx, steps, warmup):
     torch.cuda.synchronize()
     e0.record(s)
     for _ in range(steps):
-        ctx.replay(True)
+        ctx.replay(use_graph)
     e1.record(s)
     e1.synchronize()
     torch.cuda.synchronize()
@@ -155,7 +156,7 @@ def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames
 
     w, h = 640, 448
     log2 = {1: 0, 2: 1, 4: 2, 8: 3, 16: 4}[ssaa]
-    ctx = pgs.Context(super_sampling=ssaa)
+    ctx = pgs.Context(super_sampling=ssaa, texture_stream=args.texture_stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ctx.begin_recording()
@@ -163,7 +164,7 @@ def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames
     ctx.end_recording()
     live_s = time.perf_counter() - t0
     stats = ctx.consume_flush_stats()
-    ms = replay_timed(ctx, args.steps, args.warmup)
+    ms = replay_timed(ctx, args.steps, args.warmup, use_graph=not args.no_graph)
     prof = kernel_profile(ctx)
     ctx.close()
 
@@ -194,7 +195,8 @@ def bench_render(args, stream, ssaa, frames, n_prims, workload, seed, cpu_frames
         "ms_per_step": round(ms, 4),
         "dtype": "fp32+int",
         "config": {"workload": workload, "seed": seed, "frames_per_step": frames, "ssaa": ssaa,
-                   "render_passes": stats["render_passes"], "primitives": stats["primitives"]},
+                   "render_passes": stats["render_passes"], "primitives": stats["primitives"],
+                   "streams": 2 if args.texture_stream else 1, "replay": "eager" if args.no_graph else "hipGraph"},
         "frames_per_s": round(frames * 1e3 / ms, 1),
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in prof.items()},
         "kernel_launches": {k: v[1] for k, v in prof.items()},
@@ -515,6 +517,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="textured", choices=sorted(WORKLOADS))
+    ap.add_argument("--texture-stream", action="store_true",
+                    help="run copies, CLUT loads and decodes on a second stream beside setup and binning")
+    ap.add_argument("--no-graph", action="store_true", help="time eager replays instead of hipGraph replays")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported; GS contexts are independent and this build "

@@ -28,7 +28,10 @@ typedef struct pgs_options
 	/* Sampler feedback (sparse texture decode, gs_renderer.cpp:1388-1422): 0 where a render pass
 	 * decodes at least 2^20 texels, 1 always, 2 never. Never changes a pixel. */
 	uint32_t sampler_feedback;
-	uint32_t reserved;
+	/* 1: VRAM copies, CLUT loads and texture decodes run on a second HIP stream, beside triangle
+	 * setup, the sampling heuristic and binning, with event waits between the streams; 0: one
+	 * stream. Never changes a pixel. */
+	uint32_t texture_stream;
 } pgs_options;
 
 /* FlushStats (gs/gs_renderer.hpp:55-65), extended with the backend's own counters. */
@@ -101,7 +104,8 @@ int pgs_profile_enable(pgs_context *ctx, int enable);
 int pgs_profile_collect(pgs_context *ctx, double *ms, uint64_t *launches, uint32_t kinds);
 /* Blocks until all submitted work has finished. */
 int pgs_sync(pgs_context *ctx);
-/* Returns the HIP stream (hipStream_t) the context submits on. */
+/* Returns the HIP stream (hipStream_t) the context submits on (its main stream: a replay ends
+ * with the texture stream's work joined to it). */
 void *pgs_stream(pgs_context *ctx);
 
 /* Host execution of the swizzle the kernels use: byte address (nibble address for PSMT4) of

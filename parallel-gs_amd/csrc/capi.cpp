@@ -100,6 +100,9 @@ pgs_context *pgs_create(const pgs_options *options)
 		if (options->sampler_feedback > 2u)
 			return nullptr;
 		o.sampler_feedback = options->sampler_feedback;
+		if (options->texture_stream > 1u)
+			return nullptr;
+		o.texture_stream = options->texture_stream != 0;
 		if (hipSetDevice(int(options->device_index)) != hipSuccess)
 			return nullptr;
 	}

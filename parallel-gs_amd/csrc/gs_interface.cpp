@@ -14,7 +14,7 @@ bool GSInterface::init(const GSOptions &options)
 	// The sparse 4x (1x4) and 16x (2x8) grids of the reference are not provided.
 	if ((sample_log2 == 2 || sample_log2 == 4) && !options.ordered_super_sampling)
 		return false;
-	if (!renderer_.init(sample_log2, options.upload_arena_bytes, options.sampler_feedback))
+	if (!renderer_.init(sample_log2, options.upload_arena_bytes, options.sampler_feedback, options.texture_stream))
 		return false;
 	rp_.init(&renderer_, sample_log2);
 	rp_.host = this;

@@ -21,6 +21,7 @@ struct GSOptions
 	bool super_sampled_textures = false;
 	size_t upload_arena_bytes = 256u << 20;
 	uint32_t sampler_feedback = 0; // 0 automatic, 1 always, 2 never
+	bool texture_stream = false;   // copies, CLUT loads and decodes beside setup and binning
 };
 
 // Register file (the GS privileged-less register set the GIF can write).

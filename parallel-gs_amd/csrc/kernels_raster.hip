@@ -25,10 +25,13 @@ __device__ __forceinline__ int32_t uni(int32_t v) { return int32_t(__builtin_amd
 __device__ __forceinline__ float unif(float v) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v))); }
 
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
-__device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
+// Edge function at a pixel corner, wrapping in 32 bits. The 24-bit multiply gives the 32-bit
+// product's low bits because both operands fit in signed 24 bits: edge slopes are differences of
+// vertex coordinates (16-bit fields minus a 16-bit offset, below 2^18 in magnitude) and corners
+// are pixel positions (below 2^11) times 8.
 __device__ __forceinline__ int32_t idot3(int32_t e0, int32_t e1, int32_t e2, int32_t x, int32_t y)
 {
-	return wadd(wadd(wmul(e0, x), wmul(e1, y)), e2);
+	return wadd(wadd(__mul24(e0, x), __mul24(e1, y)), e2);
 }
 
 // math_utils.h:27-61

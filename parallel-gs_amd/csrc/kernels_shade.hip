@@ -24,9 +24,14 @@ namespace
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
 __device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
+// Edge function at a sample, wrapping in 32 bits like the reference's int arithmetic. The
+// products use the full-rate 24-bit multiply, whose low 32 bits equal the 32-bit product when
+// both operands fit in signed 24 bits: the edge slopes are differences of vertex coordinates
+// (16-bit register fields minus a 16-bit offset, moved by at most 17 for lines), so below 2^18
+// in magnitude, and the sample coordinates are pixel positions (below 2^11) times 8.
 __device__ __forceinline__ int32_t idot3(int32_t e0, int32_t e1, int32_t e2, int32_t x, int32_t y)
 {
-	return wadd(wadd(wmul(e0, x), wmul(e1, y)), e2);
+	return wadd(wadd(__mul24(e0, x), __mul24(e1, y)), e2);
 }
 
 // b / 255 correctly rounded for every byte (the constant folder divides in IEEE single
@@ -118,14 +123,14 @@ __device__ __forceinline__ bool alpha_test(uint32_t atst, uint32_t aref, float a
 	}
 }
 
-// Texel fetch with REPEAT addressing on the decoded image.
-// Wrap of a texel coordinate into [0, size) (REPEAT on the decoded image), for |v| < 2^22:
-// v and q * size are exact in float, floor(v * (1 / size)) is off by at most one, and the two
-// selects correct that, so the result is the mathematical remainder v mod size for every size
-// (seven full-rate instructions instead of an integer division).
-__device__ __forceinline__ uint32_t wrap_texel(int32_t v, float size, float inv_size)
+// Wrap of an integer-valued texel coordinate vf into [0, size) (REPEAT on the decoded image), for
+// |vf| < 2^22: q * size is exact in float, floor(vf * (1 / size)) is off by at most one, and the
+// two selects correct that, so the result is the mathematical remainder vf mod size for every
+// size (seven full-rate instructions instead of an integer division). The sampler's coordinates
+// are floors of texel positions below 2^22 in magnitude (|u| <= 2047 texture widths of at most
+// 1024), so they arrive here as floats without an integer round trip.
+__device__ __forceinline__ uint32_t wrap_texel(float vf, float size, float inv_size)
 {
-	const float vf = float(v);
 	const float q = floorf(__fmul_rn(vf, inv_size));
 	float r = __fsub_rn(vf, __fmul_rn(q, size));
 	r = r < 0.0f ? __fadd_rn(r, size) : r;
@@ -133,16 +138,24 @@ __device__ __forceinline__ uint32_t wrap_texel(int32_t v, float size, float inv_
 	return uint32_t(r);
 }
 
-// a + round((b - a) * w / 256) with round-half-up, w an 8-bit fraction.
-__device__ __forceinline__ uint32_t lerp8(uint32_t a, uint32_t b, uint32_t w)
+// Per byte of two RGBA8 words, a + floor(((b - a) * w + 128) / 256) for a weight w in 0..256
+// (round-half-up lerp). Since 256 a + (b - a) w + 128 = a (256 - w) + b w + 128, which is at most
+// 255 * 256 + 128 < 2^16, two bytes held 16 bits apart (mask 0x00ff00ff) are blended in one
+// 24-bit multiply-add each without carrying into each other, and the field's bits 8..15 are
+// the result.
+__device__ __forceinline__ uint32_t lerp8x4(uint32_t a, uint32_t b, uint32_t w)
 {
-	const int32_t d = (int32_t(b) - int32_t(a)) * int32_t(w) + 128;
-	return uint32_t(int32_t(a) + (d >> 8));
+	constexpr uint32_t M = 0x00ff00ffu;
+	const uint32_t iw = 256u - w;
+	const uint32_t even = __umul24(a & M, iw) + __umul24(b & M, w) + 0x00800080u;
+	const uint32_t odd = __umul24((a >> 8) & M, iw) + __umul24((b >> 8) & M, w) + 0x00800080u;
+	return ((even >> 8) & M) | (odd & ~M);
 }
 
 // Non-array sampling path of sample_texture (ubershader.comp:1042-1258) at LOD 0: UV snap to
 // 1/16 texel with the reference's epsilons, then a NEAREST or LINEAR fetch with REPEAT
-// addressing. LINEAR uses 8-bit fractional weights and an integer lerp (U, then V).
+// addressing. LINEAR uses 8-bit fractional weights and an integer lerp (U, then V). Decoded
+// images are at most 1024 texels on a side, so texel offsets fit the 24-bit multiply.
 __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint32_t *images, uint32_t tex, float u,
                                                float v, bool perspective, float out[4])
 {
@@ -161,9 +174,9 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 	uint32_t c;
 	if (!(tex & TEX_MAG_LINEAR))
 	{
-		const uint32_t x = wrap_texel(int32_t(floorf(__fmul_rn(un, fw))), fw, t.size[2]);
-		const uint32_t y = wrap_texel(int32_t(floorf(__fmul_rn(vn, fh))), fh, t.size[3]);
-		c = img[y * t.width + x];
+		const uint32_t x = wrap_texel(floorf(__fmul_rn(un, fw)), fw, t.size[2]);
+		const uint32_t y = wrap_texel(floorf(__fmul_rn(vn, fh)), fh, t.size[3]);
+		c = img[__umul24(y, t.width) + x];
 	}
 	else
 	{
@@ -172,18 +185,13 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 		const uint32_t wx = uint32_t(__fmul_rn(__fsub_rn(xs, x0), 256.0f));
 		const uint32_t wy = uint32_t(__fmul_rn(__fsub_rn(ys, y0), 256.0f));
 		// REPEAT on the decoded image; the second texel of each axis wraps from the first.
-		const uint32_t ax = wrap_texel(int32_t(x0), fw, t.size[2]), ay = wrap_texel(int32_t(y0), fh, t.size[3]);
+		const uint32_t ax = wrap_texel(x0, fw, t.size[2]), ay = wrap_texel(y0, fh, t.size[3]);
 		const uint32_t bx = ax + 1u == t.width ? 0u : ax + 1u, by = ay + 1u == t.height ? 0u : ay + 1u;
-		const uint32_t r0 = ay * t.width, r1 = by * t.width;
+		const uint32_t r0 = __umul24(ay, t.width), r1 = __umul24(by, t.width);
 		const uint32_t c00 = img[r0 + ax], c10 = img[r0 + bx], c01 = img[r1 + ax], c11 = img[r1 + bx];
-		c = 0;
-		for (uint32_t ch = 0; ch < 4; ch++)
-		{
-			const uint32_t sh = ch * 8u;
-			const uint32_t top = lerp8((c00 >> sh) & 0xffu, (c10 >> sh) & 0xffu, wx);
-			const uint32_t bot = lerp8((c01 >> sh) & 0xffu, (c11 >> sh) & 0xffu, wx);
-			c |= lerp8(top, bot, wy) << sh;
-		}
+		// wx, wy: fractions below 1 times 256, so at most 256 (a fraction just below 1 can round
+		// to 1).
+		c = lerp8x4(lerp8x4(c00, c10, wx), lerp8x4(c01, c11, wx), wy);
 	}
 	for (uint32_t ch = 0; ch < 4; ch++)
 		out[ch] = float((c >> (8 * ch)) & 0xffu);

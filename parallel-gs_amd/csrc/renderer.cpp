@@ -33,6 +33,8 @@ inline size_t copy_meta_bound(const CopyDesc &d)
 
 Renderer::~Renderer()
 {
+	if (side_)
+		(void)hipStreamSynchronize(side_);
 	if (stream_)
 		(void)hipStreamSynchronize(stream_);
 	destroy_graph();
@@ -57,6 +59,9 @@ Renderer::~Renderer()
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
+	if (fork_event_) (void)hipEventDestroy(fork_event_);
+	if (join_event_) (void)hipEventDestroy(join_event_);
+	if (side_) (void)hipStreamDestroy(side_);
 	if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -70,7 +75,7 @@ void Renderer::check(hipError_t err)
 	}
 }
 
-bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback)
+bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback, bool texture_stream)
 {
 	try
 	{
@@ -78,6 +83,12 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_f
 		sampler_feedback_ = sampler_feedback;
 		num_planes_ = sample_log2 ? 2u + (1u << sample_log2) : 1u;
 		check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+		if (texture_stream)
+		{
+			check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+			check(hipEventCreateWithFlags(&fork_event_, hipEventDisableTiming));
+			check(hipEventCreateWithFlags(&join_event_, hipEventDisableTiming));
+		}
 		check(hipMalloc(&vram_, size_t(num_planes_) * kVramSize));
 		check(hipMemsetAsync(vram_, 0, size_t(num_planes_) * kVramSize, stream_));
 		check(hipMalloc(&shadow_, kVramSize));
@@ -143,6 +154,7 @@ void Renderer::arena_upload()
 		check(hipMemcpyAsync(arena_.dev + arena_.uploaded, arena_.host + arena_.uploaded,
 		                     arena_.used - arena_.uploaded, hipMemcpyHostToDevice, stream_));
 		arena_.uploaded = arena_.used;
+		main_pending_ = true;
 	}
 }
 
@@ -160,7 +172,7 @@ void Renderer::arena_make_room(size_t bytes)
 		return;
 	}
 
-	check(hipStreamSynchronize(stream_));
+	sync_streams();
 	stats_.syncs++;
 	if (bytes > arena_.cap)
 	{
@@ -188,9 +200,15 @@ void Renderer::sync()
 	if (!batch_.empty())
 		flush_transfers();
 	arena_upload();
-	check(hipStreamSynchronize(stream_));
+	sync_streams();
 	stats_.syncs++;
 	arena_reset_after_sync();
+}
+
+void Renderer::sync_streams()
+{
+	join_texture_stream();
+	check(hipStreamSynchronize(stream_));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -216,7 +234,7 @@ void Renderer::profile_enable(bool enable)
 
 void Renderer::profile_collect(double *ms, uint64_t *launches, uint32_t kinds)
 {
-	check(hipStreamSynchronize(stream_));
+	sync_streams();
 	for (uint32_t i = 0; i < kinds; i++)
 	{
 		ms[i] = 0.0;
@@ -237,38 +255,92 @@ void Renderer::profile_collect(double *ms, uint64_t *launches, uint32_t kinds)
 	timed_.clear();
 }
 
+bool Renderer::on_texture_stream(const Launch &l) const
+{
+	switch (l.kind)
+	{
+	case LaunchKind::VramCopy:
+	case LaunchKind::VramSnapshot:
+	case LaunchKind::ClutWrite: return true;
+	// With sampler feedback the decode reads the masks the feedback launch writes after setup.
+	case LaunchKind::TexDecode: return l.decode.feedback == nullptr;
+	default: return false;
+	}
+}
+
 void Renderer::launch(const Launch &l)
 {
+	if (side_)
+	{
+		if (on_texture_stream(l))
+		{
+			if (main_pending_)
+				issue(Launch{ LaunchKind::Fork, {} });
+		}
+		else if (side_pending_ && l.kind != LaunchKind::Setup && l.kind != LaunchKind::Binning &&
+		         l.kind != LaunchKind::SsaaHeuristic && l.kind != LaunchKind::SamplerFeedback)
+			issue(Launch{ LaunchKind::Join, {} });
+	}
+	issue(l);
+}
+
+void Renderer::join_texture_stream()
+{
+	if (side_ && side_pending_)
+		issue(Launch{ LaunchKind::Join, {} });
+}
+
+void Renderer::issue(const Launch &l)
+{
+	if (recording_)
+		replay_.push_back(l);
+	if (l.kind == LaunchKind::Fork || l.kind == LaunchKind::Join)
+	{
+		const bool fork = l.kind == LaunchKind::Fork;
+		if (side_)
+		{
+			hipEvent_t e = fork ? fork_event_ : join_event_;
+			check(hipEventRecord(e, fork ? stream_ : side_));
+			check(hipStreamWaitEvent(fork ? side_ : stream_, e, 0));
+		}
+		(fork ? main_pending_ : side_pending_) = false;
+		return;
+	}
+	if (recording_)
+		recorded_.push_back(l);
+	const bool side = side_ && on_texture_stream(l);
+	hipStream_t s = side ? side_ : stream_;
 	TimedLaunch t{ uint32_t(l.kind), nullptr, nullptr };
 	if (profiling_)
 	{
 		t.start = get_event();
 		t.stop = get_event();
-		check(hipEventRecord(t.start, stream_));
+		check(hipEventRecord(t.start, s));
 	}
 	int rc = 0;
 	switch (l.kind)
 	{
-	case LaunchKind::VramCopy: rc = pgs_k_vram_copy(&l.copy, stream_); break;
-	case LaunchKind::VramSnapshot: rc = pgs_k_vram_snapshot(&l.snapshot, stream_); break;
-	case LaunchKind::Setup: rc = pgs_k_triangle_setup(&l.setup, stream_); break;
-	case LaunchKind::Binning: rc = pgs_k_binning(&l.bin, stream_); break;
-	case LaunchKind::Shade: rc = pgs_k_shade(&l.shade, stream_); break;
-	case LaunchKind::ClutWrite: rc = pgs_k_clut_write(&l.clut, stream_); break;
-	case LaunchKind::TexDecode: rc = pgs_k_texture_decode(&l.decode, stream_); break;
-	case LaunchKind::SsaaHeuristic: rc = pgs_k_ssaa_heuristic(&l.heuristic, stream_); break;
-	case LaunchKind::ShadeSuper: rc = pgs_k_shade(&l.shade, stream_); break;
-	case LaunchKind::SamplerFeedback: rc = pgs_k_sampler_feedback(&l.feedback, stream_); break;
+	case LaunchKind::VramCopy: rc = pgs_k_vram_copy(&l.copy, s); break;
+	case LaunchKind::VramSnapshot: rc = pgs_k_vram_snapshot(&l.snapshot, s); break;
+	case LaunchKind::Setup: rc = pgs_k_triangle_setup(&l.setup, s); break;
+	case LaunchKind::Binning: rc = pgs_k_binning(&l.bin, s); break;
+	case LaunchKind::Shade: rc = pgs_k_shade(&l.shade, s); break;
+	case LaunchKind::ClutWrite: rc = pgs_k_clut_write(&l.clut, s); break;
+	case LaunchKind::TexDecode: rc = pgs_k_texture_decode(&l.decode, s); break;
+	case LaunchKind::SsaaHeuristic: rc = pgs_k_ssaa_heuristic(&l.heuristic, s); break;
+	case LaunchKind::ShadeSuper: rc = pgs_k_shade(&l.shade, s); break;
+	case LaunchKind::SamplerFeedback: rc = pgs_k_sampler_feedback(&l.feedback, s); break;
+	case LaunchKind::Fork:
+	case LaunchKind::Join: break;
 	}
 	if (rc != 0)
 		check(hipError_t(-rc));
+	(side ? side_pending_ : main_pending_) = true;
 	if (profiling_)
 	{
-		check(hipEventRecord(t.stop, stream_));
+		check(hipEventRecord(t.stop, s));
 		timed_.push_back(t);
 	}
-	if (recording_)
-		recorded_.push_back(l);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -590,6 +662,22 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	// masks, the feedback launch marks the 8x8 blocks the primitives can sample, the decode
 	// skips the others.
 	const bool feedback = d.num_textures != 0 && d.sparse_decode;
+	// A full decode depends only on VRAM and the CLUT: issued first, it runs on the texture
+	// stream beside setup, the heuristic and binning. A sparse decode follows the feedback.
+	auto decode = [&]() {
+		Launch ld{};
+		ld.kind = LaunchKind::TexDecode;
+		ld.decode.vram = vram_;
+		ld.decode.clut = clut_buf_;
+		ld.decode.images = tex_images_;
+		ld.decode.decodes = arena_.dev + off_dec;
+		ld.decode.feedback = feedback ? feedback_masks_ : nullptr;
+		ld.decode.num_textures = d.num_textures;
+		ld.decode.num_groups = d.tex_groups;
+		launch(ld);
+	};
+	if (d.num_textures && !feedback)
+		decode();
 
 	Launch ls{};
 	ls.kind = LaunchKind::Setup;
@@ -637,19 +725,7 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 		lf.feedback.num_primitives = n;
 		lf.feedback.num_textures = d.num_textures;
 		launch(lf);
-	}
-	if (d.num_textures)
-	{
-		Launch ld{};
-		ld.kind = LaunchKind::TexDecode;
-		ld.decode.vram = vram_;
-		ld.decode.clut = clut_buf_;
-		ld.decode.images = tex_images_;
-		ld.decode.decodes = arena_.dev + off_dec;
-		ld.decode.feedback = feedback ? feedback_masks_ : nullptr;
-		ld.decode.num_textures = d.num_textures;
-		ld.decode.num_groups = d.tex_groups;
-		launch(ld);
+		decode();
 	}
 
 	if (super_pass)
@@ -768,11 +844,13 @@ void Renderer::begin_recording()
 	sync();
 	destroy_graph();
 	recorded_.clear();
+	replay_.clear();
 	const size_t vram_bytes = size_t(num_planes_) * kVramSize;
 	if (!snapshot_)
 		check(hipMalloc(&snapshot_, vram_bytes + 1024u));
 	check(hipMemcpyAsync(snapshot_, vram_, vram_bytes, hipMemcpyDeviceToDevice, stream_));
 	check(hipMemcpyAsync(snapshot_ + vram_bytes, clut_buf_, 1024u, hipMemcpyDeviceToDevice, stream_));
+	main_pending_ = true;
 	recording_ = true;
 }
 
@@ -788,7 +866,7 @@ void Renderer::end_recording()
 	if (!batch_.empty())
 		flush_transfers();
 	arena_upload();
-	check(hipStreamSynchronize(stream_));
+	sync_streams();
 	recording_ = false;
 }
 
@@ -806,19 +884,26 @@ int Renderer::replay(bool use_graph)
 	{
 		if (!snapshot_)
 			return -int(hipErrorInvalidValue);
+		// The replay list holds the recorded ordering points; the first texture-stream launch
+		// is preceded by a Fork (recording starts with a snapshot copy on the main stream), so
+		// it waits for the restore below. The closing join gives the main stream the whole
+		// replay (and closes the capture's fork).
+		join_texture_stream();
 		if (!use_graph)
 		{
 			restore_snapshot(stream_);
-			for (auto &l : recorded_)
-				launch(l);
+			for (auto &l : replay_)
+				issue(l);
+			join_texture_stream();
 			return 0;
 		}
 		if (!graph_exec_)
 		{
 			check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
 			restore_snapshot(stream_);
-			for (auto &l : recorded_)
-				launch(l);
+			for (auto &l : replay_)
+				issue(l);
+			join_texture_stream();
 			check(hipStreamEndCapture(stream_, &graph_));
 			check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
 		}

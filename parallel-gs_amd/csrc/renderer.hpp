@@ -41,7 +41,9 @@ public:
 	// sample_log2: 0 = 1x, else log2 of the sample count; allocates 2 + samples planes.
 	// sampler_feedback: 0 decodes sparsely when a pass decodes at least kFeedbackAutoTexels
 	// texels, 1 always, 2 never.
-	bool init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback = 0);
+	// texture_stream: run VRAM copies, CLUT loads and texture decodes on a second stream,
+	// overlapping triangle setup, the sampling heuristic and binning.
+	bool init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback = 0, bool texture_stream = false);
 
 	// ---- transfers -------------------------------------------------------------------
 	// Queue a host->local upload. The payload bytes are copied into the upload arena.
@@ -82,6 +84,7 @@ public:
 	void end_recording();
 	int replay(bool use_graph);
 	size_t recorded_launches() const { return recorded_.size(); }
+	bool texture_stream() const { return side_ != nullptr; }
 
 	// Per-launch HIP event timing: while enabled every launch is bracketed by two events;
 	// profile_collect() synchronizes and returns the summed milliseconds per kernel kind.
@@ -98,11 +101,14 @@ public:
 	size_t lists_capacity() const { return lists_cap_; }
 	size_t counts_capacity() const { return counts_cap_; }
 
-	// A kernel launch as recorded for replay (kind + argument block).
+	// A kernel launch as recorded for replay (kind + argument block). Fork and Join are the
+	// ordering points between the two streams, recorded in place so that replays (and graph
+	// captures) keep the same dependencies: Fork makes the texture stream wait for everything
+	// issued on the main stream so far, Join makes the main stream wait for the texture stream.
 	enum class LaunchKind : uint32_t
 	{
 		VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4, ClutWrite = 5, TexDecode = 6,
-		SsaaHeuristic = 7, ShadeSuper = 8, SamplerFeedback = 9
+		SsaaHeuristic = 7, ShadeSuper = 8, SamplerFeedback = 9, Fork = 10, Join = 11
 	};
 	struct Launch
 	{
@@ -120,6 +126,7 @@ public:
 			pgs_feedback_args feedback;
 		};
 	};
+	// The recorded kernel launches, in issue order (without the stream ordering points).
 	const std::vector<Launch> &recorded() const { return recorded_; }
 	// Restores VRAM planes and the CLUT to their state at begin_recording() (on `stream`).
 	void restore_snapshot(hipStream_t stream);
@@ -171,12 +178,23 @@ private:
 	uint8_t *vram_ = nullptr;    // num_planes_ * 4 MiB
 	uint8_t *shadow_ = nullptr;  // 4 MiB
 	hipStream_t stream_ = nullptr;
+	// Texture stream: VRAM copies, snapshots, CLUT loads and (without sampler feedback) texture
+	// decodes. Before one of them runs, the texture stream waits for the main stream when the
+	// main stream has had work since the last such wait; the shading kernels (and a decode on the
+	// main stream) wait for the texture stream when it has had work since the last join. Setup,
+	// binning, the heuristic and the feedback marking touch no VRAM, CLUT or decoded image, so
+	// they run beside it. Every host synchronization joins first.
+	hipStream_t side_ = nullptr;
+	hipEvent_t fork_event_ = nullptr, join_event_ = nullptr;
+	bool main_pending_ = true;  // main stream work the texture stream has not waited for
+	bool side_pending_ = false; // texture stream work the main stream has not waited for
 	int last_error_ = 0;
 
 	Arena arena_;
 	std::vector<Arena> retired_arenas_; // kept alive while recording
 	bool recording_ = false;
-	std::vector<Launch> recorded_;
+	std::vector<Launch> recorded_;   // kernels
+	std::vector<Launch> replay_;     // kernels and stream ordering points
 	hipGraph_t graph_ = nullptr;
 	hipGraphExec_t graph_exec_ = nullptr;
 
@@ -198,7 +216,13 @@ private:
 	FlushStats stats_;
 
 	void check(hipError_t err);
+	// Issues a launch with the stream ordering points it needs; issue() runs one entry as is.
 	void launch(const Launch &l);
+	void issue(const Launch &l);
+	bool on_texture_stream(const Launch &l) const;
+	// Makes the main stream wait for the texture stream's outstanding work.
+	void join_texture_stream();
+	void sync_streams();
 	// Bump-allocates `bytes` in the arena; returns the host pointer and device offset.
 	uint8_t *arena_alloc(size_t bytes, size_t &dev_offset);
 	void arena_make_room(size_t bytes);

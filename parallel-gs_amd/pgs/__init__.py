@@ -38,7 +38,7 @@ class _Options(ctypes.Structure):
         ("device_index", ctypes.c_uint32),
         ("upload_arena_bytes", ctypes.c_uint64),
         ("sampler_feedback", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("texture_stream", ctypes.c_uint32),
     ]
 
 
@@ -196,9 +196,12 @@ class Context:
     SAMPLER_FEEDBACK = {"auto": 0, "always": 1, "never": 2}
 
     def __init__(self, super_sampling: int = 1, device_index: int = 0, arena_bytes: int = 0,
-                 ordered: bool = True, super_sampled_textures: bool = False, sampler_feedback: str = "auto"):
+                 ordered: bool = True, super_sampled_textures: bool = False, sampler_feedback: str = "auto",
+                 texture_stream: bool = False):
+        """texture_stream=True runs VRAM copies, CLUT loads and texture decodes on a second HIP
+        stream beside setup and binning; by default every launch goes to one stream."""
         opts = _Options(super_sampling, int(ordered), int(super_sampled_textures), device_index, arena_bytes,
-                        self.SAMPLER_FEEDBACK[sampler_feedback], 0)
+                        self.SAMPLER_FEEDBACK[sampler_feedback], int(texture_stream))
         self._L = lib()
         self._h = self._L.pgs_create(ctypes.byref(opts))
         if not self._h:

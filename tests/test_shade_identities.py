@@ -10,7 +10,9 @@ IEEE single precision with numpy.
 * division by a power of two equals multiplication by its reciprocal;
 * the texel wrap done in float (floor of v times the reciprocal size, two corrections) is the
   mathematical remainder for every image size up to 1024 and |v| < 2^22;
-* b / 255 is not b * fl(1 / 255) for every byte, which is why the kernels keep an exact table.
+* b / 255 is not b * fl(1 / 255) for every byte, which is why the kernels keep an exact table;
+* the bilinear filter's lerp of two bytes held 16 bits apart in one word equals the per-byte
+  lerp of the oracle for every pair of bytes and every weight 0..256.
 """
 import numpy as np
 
@@ -73,3 +75,52 @@ def test_float_texel_wrap_is_the_remainder():
         v = np.concatenate([mult, rng.integers(-lim, lim, 2048), [-lim, lim, 0, -1, 1]])
         v = v[np.abs(v) <= lim]
         assert np.array_equal(wrap_texel(v, size), np.mod(v, size)), size
+
+
+def lerp8_ref(a, b, w):
+    """Per-byte lerp of the oracle's bilinear filter: a + floor(((b - a) w + 128) / 256)."""
+    return a + np.floor_divide((b - a) * w + 128, 256)
+
+
+def lerp8x4(a, b, w):
+    """The shader's packed form (kernels_shade.hip lerp8x4): two bytes per 16-bit field."""
+    M = np.uint64(0x00FF00FF)
+    a, b, w = a.astype(np.uint64), b.astype(np.uint64), w.astype(np.uint64)
+    iw = np.uint64(256) - w
+    even = ((a & M) * iw + (b & M) * w + np.uint64(0x00800080)) & np.uint64(0xFFFFFFFF)
+    odd = (((a >> np.uint64(8)) & M) * iw + ((b >> np.uint64(8)) & M) * w + np.uint64(0x00800080)) & np.uint64(0xFFFFFFFF)
+    return ((even >> np.uint64(8)) & M) | (odd & ~M & np.uint64(0xFFFFFFFF))
+
+
+def test_packed_bilinear_lerp_matches_per_byte_lerp():
+    """Exhaustive over bytes a, b and weights 0..256: a (256 - w) + b w + 128 stays below 2^16
+    and its bits 8..15 are the lerp; then the packed four-byte form on random words and all
+    weights equals the lerp of each byte."""
+    a, b = np.meshgrid(np.arange(256, dtype=np.int64), np.arange(256, dtype=np.int64), indexing="ij")
+    for w in range(257):
+        s = a * (256 - w) + b * w + 128
+        assert s.max() < 1 << 16
+        assert np.array_equal(s >> 8, lerp8_ref(a, b, w)), w
+    rng = np.random.default_rng(3)
+    wa = rng.integers(0, 2**32, 4096, dtype=np.uint64)
+    wb = rng.integers(0, 2**32, 4096, dtype=np.uint64)
+    for w in range(257):
+        ww = np.full(wa.shape, w, dtype=np.uint64)
+        got = lerp8x4(wa, wb, ww)
+        want = np.zeros_like(wa)
+        for ch in range(4):
+            sh = np.uint64(8 * ch)
+            x = ((wa >> sh) & np.uint64(255)).astype(np.int64)
+            y = ((wb >> sh) & np.uint64(255)).astype(np.int64)
+            want |= lerp8_ref(x, y, w).astype(np.uint64) << sh
+        assert np.array_equal(got, want), w
+
+
+def test_edge_products_fit_24_bits():
+    """The shaders multiply edge slopes by sample coordinates with the 24-bit multiply: slopes
+    are differences of vertex coordinates (16-bit register fields minus a 16-bit offset, lines
+    moved by at most 17), sample coordinates are pixel positions below 2^11 times 8."""
+    coord_max = 0xFFFF + 17
+    slope_max = 2 * coord_max
+    assert slope_max < 1 << 23
+    assert (2047 << 3) | 7 < 1 << 23
