@@ -76,8 +76,8 @@ typedef struct pgs_setup_args
 	/* Optional first level of binning (NULL: none). coarse_mask[ct * mask_words + w] bit b is set
 	 * when primitive 64 w + b has a bounding box touching coarse tile ct, a square of
 	 * 2^coarse_log2 pixels of the grid at (coarse_origin_x, coarse_origin_y), row-major
-	 * coarse_x by coarse_y; mask_words = ceil(num_primitives / 64). Has no counterpart in the
-	 * reference, whose binning tests every primitive against every tile. */
+	 * coarse_x by coarse_y (each at most 256); mask_words = ceil(num_primitives / 64). Has no
+	 * counterpart in the reference, whose binning tests every primitive against every tile. */
 	uint64_t *coarse_mask;
 	int32_t coarse_origin_x, coarse_origin_y;
 	uint32_t coarse_log2;
@@ -172,6 +172,9 @@ typedef struct pgs_shade_args
 	                               is 1 (single-sampled launch) or 2 (super-sampled launch) */
 	uint32_t has_ref_plane;  /* context is super-sampled: dirty pixels update plane 1 */
 	uint32_t waves_per_group; /* waves per workgroup: 1, 2 or 4 (0 means 4) */
+	uint32_t single_tiles;   /* super-sampled launch only: 1 also shades the tiles whose mode is 1,
+	                            single-sampled, with the first of each block's sample waves (the
+	                            pass then needs no single-sampled launch); 0 leaves them */
 } pgs_shade_args;
 
 int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);

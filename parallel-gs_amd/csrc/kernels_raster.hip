@@ -411,6 +411,9 @@ __device__ __forceinline__ void triangle_setup_one(const pgs_setup_args &args, u
 // has a bounding box that touches the tile. The bounding-box test is the first test of the
 // exact per-tile test, and a fine tile lies inside its coarse tile, so every primitive binned
 // to a fine tile has its bit set. Every word is written, so the mask needs no clearing.
+// Largest coarse grid side (the launcher checks coarse_x, coarse_y and their product).
+constexpr uint32_t kMaxCoarseSide = 256;
+
 __device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uint32_t wg, uint32_t index)
 {
 	int32_t cx0 = 1, cy0 = 1, cx1 = 0, cy1 = 0; // empty range
@@ -426,16 +429,31 @@ __device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uin
 			cy1 = min((s.bb[3] - args.coarse_origin_y) >> l, int32_t(args.coarse_y) - 1);
 		}
 	}
-	uint64_t *mask = args.coarse_mask + wg;
+	// A bounding box is a column range times a row range, so the word of coarse tile (cx, cy)
+	// is the ballot of the lanes whose columns cover cx AND the ballot of the lanes whose rows
+	// cover cy: coarse_x + coarse_y ballots, kept in LDS, then every lane writes the words of
+	// tiles lane, lane + 64, ... (the launch is one wave per workgroup).
+	__shared__ uint64_t col_bits[kMaxCoarseSide], row_bits[kMaxCoarseSide];
+	const uint32_t lane = threadIdx.x & 63u;
+	for (uint32_t cx = 0; cx < args.coarse_x; cx++)
+	{
+		const uint64_t bits = __ballot(int32_t(cx) >= cx0 && int32_t(cx) <= cx1);
+		if (lane == 0)
+			col_bits[cx] = bits;
+	}
 	for (uint32_t cy = 0; cy < args.coarse_y; cy++)
 	{
-		const bool row = int32_t(cy) >= cy0 && int32_t(cy) <= cy1;
-		for (uint32_t cx = 0; cx < args.coarse_x; cx++)
-		{
-			const uint64_t bits = __ballot(row && int32_t(cx) >= cx0 && int32_t(cx) <= cx1);
-			if (threadIdx.x == 0)
-				mask[size_t(cy * args.coarse_x + cx) * args.mask_words] = bits;
-		}
+		const uint64_t bits = __ballot(int32_t(cy) >= cy0 && int32_t(cy) <= cy1);
+		if (lane == 0)
+			row_bits[cy] = bits;
+	}
+	__syncthreads();
+	uint64_t *mask = args.coarse_mask + wg;
+	const uint32_t tiles = args.coarse_x * args.coarse_y;
+	for (uint32_t t = lane; t < tiles; t += 64u)
+	{
+		const uint32_t cy = t / args.coarse_x, cx = t - cy * args.coarse_x;
+		mask[size_t(t) * args.mask_words] = col_bits[cx] & row_bits[cy];
 	}
 }
 
@@ -502,6 +520,10 @@ __device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_arg
 	const uint32_t NO_FULL_3D = ST_SPRITE | ST_LINE | ST_SNAP_ATTRIBUTE;
 	const uint32_t SA = ST_SNAP_ATTRIBUTE, SR = ST_SNAP_RASTER;
 
+	// The 3D depth range is reduced in registers and across the wave first: one LDS atomic pair
+	// per wave instead of one per vertex on the same two words.
+	uint32_t zmin = ~0u, zmax = 0u;
+	bool has3d = false;
 	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
 	{
 		const uint32_t st = a.state_copy[i];
@@ -510,13 +532,13 @@ __device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_arg
 			for (uint32_t k = 0; k < 3; k++)
 			{
 				const uint32_t z = pos[3u * i + k].z;
-				atomicMin(&min3d, z);
-				atomicMax(&max3d, z);
+				zmin = min(zmin, z);
+				zmax = max(zmax, z);
 				const uint32_t sz = snapped(z);
 				if (in_range(sz))
 					atomicOr(&full3d[(sz - zlo) >> 5], 1u << ((sz - zlo) & 31u));
 			}
-			any3d = 1;
+			has3d = true;
 		}
 		else if ((st & (SA | SR)) == SA || (st & ST_LINE))
 		{
@@ -524,6 +546,17 @@ __device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_arg
 			if (in_range(sz))
 				atomicOr(&cand[(sz - zlo) >> 5], 1u << ((sz - zlo) & 31u));
 		}
+	}
+	for (int m = 32; m >= 1; m >>= 1)
+	{
+		zmin = min(zmin, uint32_t(__shfl_xor(int(zmin), m)));
+		zmax = max(zmax, uint32_t(__shfl_xor(int(zmax), m)));
+	}
+	if (__ballot(has3d) != 0 && (threadIdx.x & 63u) == 0)
+	{
+		atomicMin(&min3d, zmin);
+		atomicMax(&max3d, zmax);
+		any3d = 1;
 	}
 	__syncthreads();
 
@@ -766,6 +799,7 @@ extern "C" int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stre
 	if (!args || !args->positions || !args->attributes || !args->prims || !args->setup || !args->tattr)
 		return -int(hipErrorInvalidValue);
 	if (args->coarse_mask && (args->coarse_x == 0 || args->coarse_y == 0 || args->coarse_log2 > 16 ||
+	                          args->coarse_x > pgs::kMaxCoarseSide || args->coarse_y > pgs::kMaxCoarseSide ||
 	                          args->mask_words != (args->num_primitives + 63u) / 64u))
 		return -int(hipErrorInvalidValue);
 	if (args->num_primitives == 0)

@@ -619,7 +619,10 @@ __device__ __forceinline__ void sample_offset(uint32_t s, int32_t &dx, int32_t &
 // Super-sampled tiles: one wave64 holds the S samples of 64 / S pixels (lane = pixel * S + s),
 // so the end-of-tile resolve is a butterfly over S consecutive lanes, summing in the
 // reference's xor order (8, 4, 2, 1) and taking the minimum depth (ubershader.comp:581-780).
-// One wave shades the 64 samples of sub-rectangle `sub` of 8x8 block (bx, by).
+// One wave shades the 64 samples of sub-rectangle `sub` of 8x8 block (bx, by). In a launch
+// with single_tiles, the first wave of a block of a single-sampled tile shades its 64 pixels at
+// 1x through the same loop (the sampling rate is then a wave-uniform 0), as the single-sampled
+// kernel would, and the block's other waves leave.
 template <uint32_t RX, uint32_t RY>
 __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t sub)
 {
@@ -633,20 +636,45 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 		return;
 	const uint32_t shift = a.tile_log2 - 3u;
 	const uint32_t tile = (by >> shift) * a.tiles_x + (bx >> shift);
-	if (uni(a.tile_modes[tile]) != 2u)
+	const uint32_t mode = uni(a.tile_modes[tile]);
+	const bool single = mode == 1u && a.single_tiles != 0;
+	if (single ? sub != 0u : mode != 2u)
 		return;
 	const uint32_t count = uni(a.counts[tile]);
 	if (count == 0)
 		return;
 	const uint16_t *list = a.lists + size_t(tile) * a.list_stride;
 
-	const int32_t rox = a.origin_x + int32_t(bx * 8u + (sub % SUB_X) * RW);
-	const int32_t roy = a.origin_y + int32_t(by * 8u + (sub / SUB_X) * RH);
-	const uint32_t smp = lane & (S - 1u), pix = lane >> SL;
-	const int32_t px = rox + int32_t(pix % RW), py = roy + int32_t(pix / RW);
-	int32_t dx, dy;
-	sample_offset<RX, RY>(smp, dx, dy);
-	const int32_t hx = (px << RY) + dx, hy = (py << RY) + dy;
+	int32_t rox, roy, rhx, rhy, px, py, hx, hy;
+	uint32_t smp;
+	if (single)
+	{
+		rox = a.origin_x + int32_t(bx * 8u);
+		roy = a.origin_y + int32_t(by * 8u);
+		rhx = rox + 7;
+		rhy = roy + 7;
+		px = rox + int32_t(lane & 7u);
+		py = roy + int32_t(lane >> 3);
+		hx = px;
+		hy = py;
+		smp = 0;
+	}
+	else
+	{
+		rox = a.origin_x + int32_t(bx * 8u + (sub % SUB_X) * RW);
+		roy = a.origin_y + int32_t(by * 8u + (sub / SUB_X) * RH);
+		rhx = rox + int32_t(RW) - 1;
+		rhy = roy + int32_t(RH) - 1;
+		smp = lane & (S - 1u);
+		const uint32_t pix = lane >> SL;
+		px = rox + int32_t(pix % RW);
+		py = roy + int32_t(pix / RW);
+		int32_t dx, dy;
+		sample_offset<RX, RY>(smp, dx, dy);
+		hx = (px << RY) + dx;
+		hy = (py << RY) + dy;
+	}
+	const uint32_t rate = single ? 0u : RY;
 
 	PassInfo pi;
 	pi.fb_psm = a.fb_psm;
@@ -658,7 +686,8 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 	pi.z16 = pi.zsens && psm_is_16(a.z_psm);
 	pi.z24 = pi.zsens && (a.z_psm == PSMCT24 || a.z_psm == PSMZ24);
 
-	// begin_tile: samples come from their planes where the reference plane marks the pixel.
+	// begin_tile: samples come from their planes where the reference plane marks the pixel; a
+	// single-sampled tile reads plane 0 only.
 	uint8_t *v = a.vram;
 	const size_t P = kVramSize;
 	const uint32_t caddr = swizzle_addr(uint32_t(px), uint32_t(py), a.fbp * 32u, a.fbw, a.fb_psm);
@@ -666,14 +695,14 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 	if (pi.fb16)
 	{
 		uint32_t c = *reinterpret_cast<const uint16_t *>(v + caddr);
-		if (*reinterpret_cast<const uint16_t *>(v + P + caddr) == 0xffffu)
+		if (!single && *reinterpret_cast<const uint16_t *>(v + P + caddr) == 0xffffu)
 			c = *reinterpret_cast<const uint16_t *>(v + (2u + smp) * P + caddr);
 		color = rgba16_to_rgba32(c);
 	}
 	else
 	{
 		color = *reinterpret_cast<const uint32_t *>(v + caddr);
-		const uint32_t ref = *reinterpret_cast<const uint32_t *>(v + P + caddr);
+		const uint32_t ref = single ? 0u : *reinterpret_cast<const uint32_t *>(v + P + caddr);
 		const uint32_t mask = pi.fb24 ? 0xffffffu : ~0u;
 		if ((ref & mask) == mask)
 		{
@@ -689,14 +718,14 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 		if (pi.z16)
 		{
 			depth = *reinterpret_cast<const uint16_t *>(v + zaddr);
-			if (*reinterpret_cast<const uint16_t *>(v + P + zaddr) == 0xffffu)
+			if (!single && *reinterpret_cast<const uint16_t *>(v + P + zaddr) == 0xffffu)
 				depth = *reinterpret_cast<const uint16_t *>(v + (2u + smp) * P + zaddr);
 		}
 		else
 		{
 			const uint32_t mask = pi.z24 ? 0xffffffu : ~0u;
 			depth = *reinterpret_cast<const uint32_t *>(v + zaddr) & mask;
-			if ((*reinterpret_cast<const uint32_t *>(v + P + zaddr) & mask) == mask)
+			if (!single && (*reinterpret_cast<const uint32_t *>(v + P + zaddr) & mask) == mask)
 				depth = *reinterpret_cast<const uint32_t *>(v + (2u + smp) * P + zaddr) & mask;
 		}
 	}
@@ -704,7 +733,6 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 
 	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
 	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
-	const int32_t rhx = rox + int32_t(RW) - 1, rhy = roy + int32_t(RH) - 1;
 	for (uint32_t base = 0; base < count; base += 64u)
 	{
 		const uint32_t li = base + lane;
@@ -722,13 +750,15 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
 			const uint32_t prim = uni(uint32_t(__shfl(int(idx), int(bit))));
-			shade_pixel<false>(a, pi, prim, hx, hy, RY, true, color, depth, cdirty, zdirty);
+			shade_pixel<false>(a, pi, prim, hx, hy, rate, !single, color, depth, cdirty, zdirty);
 		}
 	}
 
-	// end_tile
-	const uint32_t group = lane & ~(S - 1u);
-	const uint64_t gmask = ((1ull << S) - 1ull) << group;
+	// end_tile. A single-sampled pixel is its own group of one lane: nothing differs from its
+	// first sample, so it writes plane 0 and clears its reference-plane mark where dirty, as
+	// the single-sampled kernel does.
+	const uint32_t group = single ? lane : lane & ~(S - 1u);
+	const uint64_t gmask = single ? (1ull << lane) : ((1ull << S) - 1ull) << group;
 	const uint32_t first_color = uint32_t(__shfl(int(color), int(group)));
 	const uint32_t first_depth = uint32_t(__shfl(int(depth), int(group)));
 	const bool req_c = (__ballot(color != first_color) & gmask) != 0;
@@ -823,6 +853,14 @@ PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_11_multi, (shade_ssaa_b
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_args)
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
 
+// Whether a launch shading `want_mode` tiles works on a tile of `mode`: a super-sampled launch
+// with single_tiles also takes the single-sampled tiles (with the same S waves per block, of
+// which the first shades the block).
+__device__ __forceinline__ bool tile_has_work(const pgs_shade_args &a, uint32_t mode, uint32_t want_mode)
+{
+	return mode == want_mode || (want_mode == 2u && mode == 1u && a.single_tiles);
+}
+
 // ---------------------------------------------------------------------------------------
 // Work-queue shading by coarse tile: a grid of resident workgroups pops queue items (coarse
 // tiles in the tile numbering of a multi-launch, where launch l owns tiles [prefix[l],
@@ -858,7 +896,7 @@ __device__ __forceinline__ void tile_queue_loop(const pgs_queue_args &q, const p
 		const pgs_shade_args &a = args[l];
 		const uint32_t t = g - q.prefix[l];
 		const uint64_t t0 = uint64_t(clock64());
-		const bool work = a.tile_modes ? a.tile_modes[t] == want_mode : a.counts[t] != 0;
+		const bool work = a.tile_modes ? tile_has_work(a, a.tile_modes[t], want_mode) : a.counts[t] != 0;
 		if (work)
 		{
 			const uint32_t B = 1u << (a.tile_log2 - 3u);
@@ -898,7 +936,7 @@ __device__ __forceinline__ void ordered_chunk(const pgs_queue_args &q, const pgs
 	const uint32_t l = uni(find_launch(q.prefix, q.num_launches, g));
 	const pgs_shade_args &a = args[l];
 	const uint32_t t = g - q.prefix[l];
-	if (a.tile_modes ? a.tile_modes[t] != want_mode : a.counts[t] == 0)
+	if (a.tile_modes ? !tile_has_work(a, a.tile_modes[t], want_mode) : a.counts[t] == 0)
 		return;
 	const uint32_t B = 1u << (a.tile_log2 - 3u);
 	const uint32_t w = chunk * (blockDim.x >> 6) + uni(threadIdx.x >> 6);
@@ -967,7 +1005,7 @@ __global__ void __launch_bounds__(256) k_tile_features(pgs_features_args f)
 	const uint32_t t = item - f.prefix[l];
 	float *out = f.features + size_t(item) * PGS_MLP_INPUTS;
 	const uint32_t want = a.rate_log2 ? 2u : 1u;
-	const bool work = a.tile_modes ? a.tile_modes[t] == want : a.counts[t] != 0;
+	const bool work = a.tile_modes ? tile_has_work(a, a.tile_modes[t], want) : a.counts[t] != 0;
 	const uint32_t count = work ? a.counts[t] : 0u;
 	const uint32_t B = 1u << (a.tile_log2 - 3u);
 	const uint32_t tx = t % a.tiles_x, ty = t / a.tiles_x;
@@ -1073,6 +1111,8 @@ static bool shade_args_valid(const pgs_shade_args *args)
 	    args->waves_per_group != 4)
 		return false;
 	if (args->rate_log2 != 0 && !args->tile_modes)
+		return false;
+	if (args->single_tiles > 1u || (args->single_tiles && args->rate_log2 == 0))
 		return false;
 	return true;
 }

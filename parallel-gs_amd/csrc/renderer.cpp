@@ -803,13 +803,17 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	a.rate_x_log2 = 0;
 	a.tile_modes = super_pass ? tile_modes_ : nullptr;
 	a.has_ref_plane = ssaa_context ? 1u : 0u;
-	launch(lsh);
-	if (super_pass)
+	if (!super_pass)
+		launch(lsh);
+	else
 	{
+		// One launch for both kinds of tile: the first sample wave of each block shades the
+		// single-sampled tiles.
 		Launch lss = lsh;
 		lss.kind = LaunchKind::ShadeSuper;
 		lss.shade.rate_log2 = rate_y;
 		lss.shade.rate_x_log2 = rate_x;
+		lss.shade.single_tiles = 1;
 		launch(lss);
 	}
 
