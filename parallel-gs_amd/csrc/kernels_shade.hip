@@ -217,8 +217,10 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 {
 	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
 	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
-	const PrimitiveSetup &s = setups[prim_index];
+	// Copies: the records are wave-uniform, so they come in with a few wide scalar loads issued
+	// together before any test reads them, instead of one load and wait per field.
 	const uint32_t st = prims[prim_index].state;
+	const PrimitiveSetup s = setups[prim_index];
 
 	// Snapped primitives rasterize at the pixel's first sample (ubershader.comp:815-825).
 	const int32_t snap_mask = -(1 << rate);
@@ -231,7 +233,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	// Nothing before them touches memory except loads of the primitive's data and in-bounds
 	// texel loads.
 	// Coverage (intersect.h:156-244).
-	bool alive = px >= s.bb[0] && py >= s.bb[1] && px <= s.bb[2] && py <= s.bb[3];
+	bool alive = (px >= s.bb[0]) & (py >= s.bb[1]) & (px <= s.bb[2]) & (py <= s.bb[3]);
 	if constexpr (!kPredicated)
 		if (!alive)
 			return;
@@ -300,6 +302,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		fi = __fadd_rn(__fmul_rn(float(idot3(s.b[0], s.b[1], s.b[2], bx_, by_)), s.inv_area), s.error_i);
 		fj = __fadd_rn(__fmul_rn(float(idot3(s.c[0], s.c[1], s.c[2], bx_, by_)), s.inv_area), s.error_j);
 	}
+
 
 	const PrimitiveAttribute &pa = prims[prim_index];
 	const StateVector sv = static_cast<const StateVector *>(a.states)[st & STATE_INDEX_MASK];
