@@ -92,7 +92,10 @@ struct TextureDesc
 	uint32_t image_offset; // offset in dwords into the texture image arena
 	uint32_t width, height; // effective rectangle size (the decoded image)
 	uint32_t pad0;
-	float region[4];       // clamp region (min u, min v, max u, max v) in texels
+	// Clamp bounds of the texel coordinate (min u, min v, max u, max v): the sampler's
+	// floor(region) + 0.5 on an axis with CLAMP or REGION_CLAMP, -inf / +inf on the others, so
+	// that the clamp is two unconditional min/max operations.
+	float region[4];
 	float size[4];         // 2^TW, 2^TH, 1 / width, 1 / height
 	float bias[2];         // -rectangle origin / rectangle size
 	uint32_t pad1[2];

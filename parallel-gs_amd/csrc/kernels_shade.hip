@@ -162,10 +162,10 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 	const uint32_t *img = images + t.image_offset;
 	float u1 = perspective ? __fmul_rn(u, t.size[0]) : u;
 	float v1 = perspective ? __fmul_rn(v, t.size[1]) : v;
-	if (tex & TEX_CLAMP_S)
-		u1 = fminf(fmaxf(u1, floorf(t.region[0]) + 0.5f), floorf(t.region[2]) + 0.5f);
-	if (tex & TEX_CLAMP_T)
-		v1 = fminf(fmaxf(v1, floorf(t.region[1]) + 0.5f), floorf(t.region[3]) + 0.5f);
+	// CLAMP / REGION_CLAMP; the descriptor holds floor(region) + 0.5, or infinities on an axis
+	// that does not clamp (u1 and v1 are finite: u and v were clamped to +-2047).
+	u1 = fminf(fmaxf(u1, t.region[0]), t.region[2]);
+	v1 = fminf(fmaxf(v1, t.region[1]), t.region[3]);
 	float un = floorf(__fadd_rn(__fmul_rn(u1, 16.0f), 1.0f / 64.0f));
 	float vn = floorf(__fadd_rn(__fmul_rn(v1, 16.0f), 1.0f / 64.0f));
 	un = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(un, 16.0f / 1024.0f), t.size[2]), 0.0625f), t.bias[0]);
@@ -234,6 +234,12 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	// texel loads.
 	// Coverage (intersect.h:156-244).
 	bool alive = (px >= s.bb[0]) & (py >= s.bb[1]) & (px <= s.bb[2]) & (py <= s.bb[3]);
+	if (st & (ST_SCANMSK_EVEN | ST_SCANMSK_ODD))
+	{
+		// Sprites mask per pixel row, everything else per sample row.
+		const int32_t sy_row = (st & ST_SPRITE) ? py : hy;
+		alive = alive && !((sy_row & 1) == 0 ? (st & ST_SCANMSK_EVEN) : (st & ST_SCANMSK_ODD));
+	}
 	if constexpr (!kPredicated)
 		if (!alive)
 			return;
@@ -253,21 +259,9 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		A = 0;
 	}
 	alive = alive && A >= 0 && B >= 0 && C >= 0;
-	if (st & (ST_SCANMSK_EVEN | ST_SCANMSK_ODD))
-	{
-		// Sprites mask per pixel row, everything else per sample row.
-		const int32_t sy_row = (st & ST_SPRITE) ? py : hy;
-		alive = alive && !((sy_row & 1) == 0 ? (st & ST_SCANMSK_EVEN) : (st & ST_SCANMSK_ODD));
-	}
-	if constexpr (kPredicated)
-	{
-		if (__ballot(alive) == 0)
-			return;
-	}
-	else if (!alive)
-		return;
 
-	// Z (ubershader.comp:852-871).
+	// Z (ubershader.comp:852-871), before the coverage exit: it reads the rest of the setup
+	// record, which then arrives with the first part.
 	uint32_t z = s.z[0];
 	{
 		const float t0 = __fmul_rn(__uint_as_float(s.z[1]), fi);
@@ -280,6 +274,14 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		if (pi.z16) z = min(z, 0xffffu);
 		else if (pi.z24) z = min(z, 0xffffffu);
 	}
+	if constexpr (kPredicated)
+	{
+		if (__ballot(alive) == 0)
+			return;
+	}
+	else if (!alive)
+		return;
+
 	const bool z_test = pi.zsens && (st & ST_Z_TEST);
 	const bool z_write = pi.zsens && (st & ST_Z_WRITE);
 	if (z_test)
@@ -293,49 +295,56 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		else if (!alive)
 			return;
 	}
+
+	// The primitive's other records, read in one place: every interpolant is computed from them
+	// here for each primitive, so their scalar loads leave together (one round trip) instead of
+	// one wait at each first use further down.
+	const PrimitiveAttribute &pa = prims[prim_index];
+	const StateVector sv = static_cast<const StateVector *>(a.states)[st & STATE_INDEX_MASK];
+	const TransformedAttributes &ta = static_cast<const TransformedAttributes *>(a.tattr)[prim_index];
 	// Attribute snapping without raster snapping: attributes at the pixel's first sample,
 	// depth per sample (ubershader.comp:888-898).
+	float ai = fi, aj = fj;
 	if (super_sampled && (st & (ST_SNAP_RASTER | ST_SNAP_ATTRIBUTE)) == ST_SNAP_ATTRIBUTE)
 	{
 		const int32_t bx_ = int32_t(uint32_t(hx & snap_mask) << (3u - rate));
 		const int32_t by_ = int32_t(uint32_t(hy & snap_mask) << (3u - rate));
-		fi = __fadd_rn(__fmul_rn(float(idot3(s.b[0], s.b[1], s.b[2], bx_, by_)), s.inv_area), s.error_i);
-		fj = __fadd_rn(__fmul_rn(float(idot3(s.c[0], s.c[1], s.c[2], bx_, by_)), s.inv_area), s.error_j);
+		ai = __fadd_rn(__fmul_rn(float(idot3(s.b[0], s.b[1], s.b[2], bx_, by_)), s.inv_area), s.error_i);
+		aj = __fadd_rn(__fmul_rn(float(idot3(s.c[0], s.c[1], s.c[2], bx_, by_)), s.inv_area), s.error_j);
 	}
-
-
-	const PrimitiveAttribute &pa = prims[prim_index];
-	const StateVector sv = static_cast<const StateVector *>(a.states)[st & STATE_INDEX_MASK];
-	const TransformedAttributes &ta = static_cast<const TransformedAttributes *>(a.tattr)[prim_index];
-
-	// Combined colour (ubershader.comp:1260-1339).
-	float comb[4], col[4];
+	// Vertex colour: c0 + (c1 - c0) * i + (c2 - c0) * j with the per-primitive terms from setup
+	// (ubershader.comp:1260-1339).
+	float col[4];
 	for (uint32_t ch = 0; ch < 4; ch++)
 	{
-		// c0 + (c1 - c0) * i + (c2 - c0) * j with the per-primitive terms from setup.
-		const float e1 = __fmul_rn(ta.dcol1[ch], fi);
-		const float e2 = __fmul_rn(ta.dcol2[ch], fj);
+		const float e1 = __fmul_rn(ta.dcol1[ch], ai);
+		const float e2 = __fmul_rn(ta.dcol2[ch], aj);
 		const float sum = __fadd_rn(__fadd_rn(ta.col0[ch], e1), e2);
 		col[ch] = floorf(__fadd_rn(__fmul_rn(255.0f, sum), 1.0f / 256.0f));
-		comb[ch] = col[ch];
 	}
+	const uint32_t p_tex = pa.tex, p_alpha = pa.alpha, p_fbmsk = pa.fbmsk, p_fogcol = pa.fogcol;
+	// S, T, Q and fog at the attribute position.
+	auto stqf = [&](int k) {
+		return __fadd_rn(__fadd_rn(ta.stqf0[k], __fmul_rn(ta.stqf1[k], ai)), __fmul_rn(ta.stqf2[k], aj));
+	};
+
+	// Combined colour (ubershader.comp:1260-1339).
+	float comb[4] = { col[0], col[1], col[2], col[3] };
 	if (sv.combiner & COMB_TME)
 	{
-		float sq[3];
-		for (int k = 0; k < 3; k++)
-			sq[k] = __fadd_rn(__fadd_rn(ta.stqf0[k], __fmul_rn(ta.stqf1[k], fi)), __fmul_rn(ta.stqf2[k], fj));
 		const bool persp = (st & ST_PERSPECTIVE) != 0;
-		float u = sq[0], v = sq[1];
+		float u = stqf(0), v = stqf(1);
 		if (persp) // q == 1 divides exactly, as the reference's q != 1 test skips
 		{
-			u = __fdiv_rn(u, sq[2]);
-			v = __fdiv_rn(v, sq[2]);
+			const float q = stqf(2);
+			u = __fdiv_rn(u, q);
+			v = __fdiv_rn(v, q);
 		}
 		u = fminf(fmaxf(u, -2047.0f), 2047.0f);
 		v = fminf(fmaxf(v, -2047.0f), 2047.0f);
-		const TextureDesc &td = static_cast<const TextureDesc *>(a.textures)[pa.tex & TEX_INDEX_MASK];
+		const TextureDesc &td = static_cast<const TextureDesc *>(a.textures)[p_tex & TEX_INDEX_MASK];
 		float tx[4];
-		sample_texture(td, a.tex_images, pa.tex, u, v, persp, tx);
+		sample_texture(td, a.tex_images, p_tex, u, v, persp, tx);
 		const bool tcc = (sv.combiner & COMB_TCC) != 0;
 		const uint32_t mode = sv.combiner & COMB_MODE_MASK;
 		if (mode == 1) // DECAL
@@ -362,11 +371,10 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	}
 	if (sv.combiner & COMB_FOG)
 	{
-		const float e1 = __fmul_rn(ta.stqf1[3], fi), e2 = __fmul_rn(ta.stqf2[3], fj);
-		const float fog = rintf(__fadd_rn(__fadd_rn(ta.stqf0[3], e1), e2));
+		const float fog = rintf(stqf(3));
 		for (uint32_t k = 0; k < 3; k++)
 		{
-			const float fc = byte_f(pa.fogcol, k);
+			const float fc = byte_f(p_fogcol, k);
 			const float d = __fmul_rn(__fsub_rn(comb[k], fc), fog);
 			comb[k] = floorf(__fadd_rn(__fadd_rn(fc, __fmul_rn(d, 1.0f / 256.0f)), 0.5f));
 		}
@@ -381,7 +389,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	bool blending = bm & BLEND_ABE;
 	if (bm & BLEND_ATE)
 	{
-		const bool pass = alpha_test((bm >> 2) & 7u, pa.alpha & 0xffu, comb[3]);
+		const bool pass = alpha_test((bm >> 2) & 7u, p_alpha & 0xffu, comb[3]);
 		switch ((bm >> 5) & 3u)
 		{
 		case 0:
@@ -415,7 +423,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		float C = 0.0f;
 		if (cm == 0) C = comb[3];
 		else if (cm == 1) C = pi.fb24 ? 128.0f : fb[3];
-		else if (cm == 2) C = float((pa.alpha >> 8) & 0xffu);
+		else if (cm == 2) C = float((p_alpha >> 8) & 0xffu);
 		for (int k = 0; k < 3; k++)
 		{
 			const float A_ = am == 0 ? comb[k] : am == 1 ? fb[k] : 0.0f;
@@ -456,9 +464,9 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	}
 	packed |= uint32_t((bm & BLEND_FBA) && write_alpha) << 31;
 
-	if (pa.fbmsk != ~0u)
+	if (p_fbmsk != ~0u)
 	{
-		uint32_t c = pa.fbmsk ? ((packed & ~pa.fbmsk) | (color & pa.fbmsk)) : packed;
+		uint32_t c = p_fbmsk ? ((packed & ~p_fbmsk) | (color & p_fbmsk)) : packed;
 		if (pi.fb16)
 			c &= 0x80f8f8f8u;
 		const bool wc = alive && (write_fb || write_alpha);

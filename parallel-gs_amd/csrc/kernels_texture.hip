@@ -247,8 +247,8 @@ __device__ __forceinline__ void sampler_feedback_body(const pgs_feedback_args &a
 	if (!all)
 	{
 		// The shader's chain: clamp to +-2047, scale (STQ), region clamp, snap, texel index.
-		auto axis = [&](float lo, float hi, float size, bool clamp, float rlo, float rhi, float inv, float bias,
-		                int32_t extent, int32_t r[4]) -> uint32_t {
+		auto axis = [&](float lo, float hi, float size, float rlo, float rhi, float inv, float bias, int32_t extent,
+		                int32_t r[4]) -> uint32_t {
 			lo = fminf(fmaxf(lo, -2047.0f), 2047.0f);
 			hi = fminf(fmaxf(hi, -2047.0f), 2047.0f);
 			if (persp)
@@ -256,21 +256,17 @@ __device__ __forceinline__ void sampler_feedback_body(const pgs_feedback_args &a
 				lo = __fmul_rn(lo, size);
 				hi = __fmul_rn(hi, size);
 			}
-			if (clamp)
-			{
-				lo = fminf(fmaxf(lo, floorf(rlo) + 0.5f), floorf(rhi) + 0.5f);
-				hi = fminf(fmaxf(hi, floorf(rlo) + 0.5f), floorf(rhi) + 0.5f);
-			}
+			// The descriptor's bounds are floor(region) + 0.5, infinite without clamping.
+			lo = fminf(fmaxf(lo, rlo), rhi);
+			hi = fminf(fmaxf(hi, rlo), rhi);
 			// Bilinear reads x0 = floor(n * extent - 0.5) and x0 + 1; one texel more each way
 			// absorbs the rounding of the corner evaluation.
 			const int32_t tlo = feedback_texel(lo, inv, bias, float(extent)) - 2;
 			const int32_t thi = feedback_texel(hi, inv, bias, float(extent)) + 2;
 			return wrap_ranges(tlo, thi, extent, r);
 		};
-		nx = axis(umin, umax, td.size[0], (pa.tex & TEX_CLAMP_S) != 0, td.region[0], td.region[2], td.size[2],
-		          td.bias[0], w, xr);
-		ny = axis(vmin, vmax, td.size[1], (pa.tex & TEX_CLAMP_T) != 0, td.region[1], td.region[3], td.size[3],
-		          td.bias[1], h, yr);
+		nx = axis(umin, umax, td.size[0], td.region[0], td.region[2], td.size[2], td.bias[0], w, xr);
+		ny = axis(vmin, vmax, td.size[1], td.region[1], td.region[3], td.size[3], td.bias[1], h, yr);
 	}
 	else
 	{

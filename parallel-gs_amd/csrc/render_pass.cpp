@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <cstdlib>
+#include <limits>
 
 namespace pgs
 {
@@ -601,10 +602,14 @@ int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 	td.image_offset = im.offset;
 	td.width = l.width;
 	td.height = l.height;
-	if (wms == 1) { td.region[0] = 0.0f; td.region[2] = float(l.width) - 1.0f; }
-	else if (wms == 2) { td.region[0] = float(field(keyc, 4, 10)); td.region[2] = float(field(keyc, 14, 10)); }
-	if (wmt == 1) { td.region[1] = 0.0f; td.region[3] = float(l.height) - 1.0f; }
-	else if (wmt == 2) { td.region[1] = float(field(keyc, 24, 10)); td.region[3] = float(field(keyc, 34, 10)); }
+	// Clamp bounds floor(region) + 0.5 (the regions are whole texels, so the sum is exact).
+	const float inf = std::numeric_limits<float>::infinity();
+	td.region[0] = td.region[1] = -inf;
+	td.region[2] = td.region[3] = inf;
+	if (wms == 1) { td.region[0] = 0.5f; td.region[2] = float(l.width - 1u) + 0.5f; }
+	else if (wms == 2) { td.region[0] = float(field(keyc, 4, 10)) + 0.5f; td.region[2] = float(field(keyc, 14, 10)) + 0.5f; }
+	if (wmt == 1) { td.region[1] = 0.5f; td.region[3] = float(l.height - 1u) + 0.5f; }
+	else if (wmt == 2) { td.region[1] = float(field(keyc, 24, 10)) + 0.5f; td.region[3] = float(field(keyc, 34, 10)) + 0.5f; }
 	td.size[0] = float(1u << l.tw);
 	td.size[1] = float(1u << l.th);
 	td.size[2] = 1.0f / float(l.width);
