@@ -24,6 +24,82 @@ namespace
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
 __device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+
+// A primitive's setup record (80 bytes) and state word, both wave-uniform, in one round trip
+// through the scalar cache. Left to itself the compiler places each scalar load at the first
+// use of its field, and since scalar loads return out of order every wait is a wait for all of
+// them: one round trip per use. The three loads and their wait sit in one asm statement, so
+// the outputs are complete when it ends; the statement has no side effects (the records are
+// read-only while shading).
+__device__ __forceinline__ void load_setup_and_state(const PrimitiveSetup *sp, const PrimitiveAttribute *pp,
+                                                     PrimitiveSetup &s, uint32_t &state)
+{
+	u32x16 lo;
+	u32x4 hi;
+	uint32_t st;
+	asm("s_load_dwordx16 %0, %3, 0x0\n\t"
+	    "s_load_dwordx4 %1, %3, 0x40\n\t"
+	    "s_load_dword %2, %4, 0x8\n\t"
+	    "s_waitcnt lgkmcnt(0)"
+	    : "=&s"(lo), "=&s"(hi), "=&s"(st)
+	    : "s"(sp), "s"(pp));
+	__builtin_memcpy(&s, &lo, 64);
+	__builtin_memcpy(reinterpret_cast<uint8_t *>(&s) + 64, &hi, 16);
+	state = st;
+}
+
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+
+// The second round, once the state word is known: the attribute record's colour terms, the
+// state vector, and the primitive's texture, alpha, frame-buffer mask and fog colour words.
+struct PrimitiveShadeData
+{
+	float col0[4], dcol1[4], dcol2[4];
+	StateVector sv;
+	uint32_t tex, tex2, alpha, fbmsk, fogcol;
+};
+__device__ __forceinline__ void load_shade_data(const TransformedAttributes *tp, const StateVector *svp,
+                                                const PrimitiveAttribute *pp, PrimitiveShadeData &d)
+{
+	u32x8 col_a;
+	u32x4 col_b, sv, pa;
+	uint32_t tex;
+	asm("s_load_dwordx8 %0, %5, 0x40\n\t"
+	    "s_load_dwordx4 %1, %5, 0x60\n\t"
+	    "s_load_dwordx4 %2, %6, 0x0\n\t"
+	    "s_load_dwordx4 %3, %7, 0x10\n\t"
+	    "s_load_dword %4, %7, 0xc\n\t"
+	    "s_waitcnt lgkmcnt(0)"
+	    : "=&s"(col_a), "=&s"(col_b), "=&s"(sv), "=&s"(pa), "=&s"(tex)
+	    : "s"(tp), "s"(svp), "s"(pp));
+	__builtin_memcpy(&d.col0[0], &col_a, 32);
+	__builtin_memcpy(&d.dcol2[0], &col_b, 16);
+	__builtin_memcpy(&d.sv, &sv, 16);
+	__builtin_memcpy(&d.tex2, &pa, 16);
+	d.tex = tex;
+}
+
+// The third round, for a textured primitive: the S, T, Q terms of the attribute record and the
+// texture descriptor of the primitive's slot.
+__device__ __forceinline__ void load_texture_data(const TransformedAttributes *tp, const TextureDesc *tdp,
+                                                  float stq[3][4], TextureDesc &td)
+{
+	u32x8 stq_a;
+	u32x4 stq_b;
+	u32x16 desc;
+	asm("s_load_dwordx8 %0, %3, 0x0\n\t"
+	    "s_load_dwordx4 %1, %3, 0x20\n\t"
+	    "s_load_dwordx16 %2, %4, 0x0\n\t"
+	    "s_waitcnt lgkmcnt(0)"
+	    : "=&s"(stq_a), "=&s"(stq_b), "=&s"(desc)
+	    : "s"(tp), "s"(tdp));
+	__builtin_memcpy(&stq[0][0], &stq_a, 32);
+	__builtin_memcpy(&stq[2][0], &stq_b, 16);
+	__builtin_memcpy(&td, &desc, 64);
+}
 // Edge function at a sample, wrapping in 32 bits like the reference's int arithmetic. The
 // products use the full-rate 24-bit multiply, whose low 32 bits equal the 32-bit product when
 // both operands fit in signed 24 bits: the edge slopes are differences of vertex coordinates
@@ -217,10 +293,9 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 {
 	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
 	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
-	// Copies: the records are wave-uniform, so they come in with a few wide scalar loads issued
-	// together before any test reads them, instead of one load and wait per field.
-	const uint32_t st = prims[prim_index].state;
-	const PrimitiveSetup s = setups[prim_index];
+	uint32_t st;
+	PrimitiveSetup s;
+	load_setup_and_state(setups + prim_index, prims + prim_index, s, st);
 
 	// Snapped primitives rasterize at the pixel's first sample (ubershader.comp:815-825).
 	const int32_t snap_mask = -(1 << rate);
@@ -299,9 +374,11 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	// The primitive's other records, read in one place: every interpolant is computed from them
 	// here for each primitive, so their scalar loads leave together (one round trip) instead of
 	// one wait at each first use further down.
-	const PrimitiveAttribute &pa = prims[prim_index];
-	const StateVector sv = static_cast<const StateVector *>(a.states)[st & STATE_INDEX_MASK];
-	const TransformedAttributes &ta = static_cast<const TransformedAttributes *>(a.tattr)[prim_index];
+	PrimitiveShadeData ta;
+	load_shade_data(static_cast<const TransformedAttributes *>(a.tattr) + prim_index,
+	                static_cast<const StateVector *>(a.states) + (st & STATE_INDEX_MASK), prims + prim_index, ta);
+	const uint32_t sv_combiner = ta.sv.combiner, sv_blend = ta.sv.blend_mode;
+	const uint32_t sv_dimx0 = ta.sv.dimx0, sv_dimx1 = ta.sv.dimx1;
 	// Attribute snapping without raster snapping: attributes at the pixel's first sample,
 	// depth per sample (ubershader.comp:888-898).
 	float ai = fi, aj = fj;
@@ -322,16 +399,20 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		const float sum = __fadd_rn(__fadd_rn(ta.col0[ch], e1), e2);
 		col[ch] = floorf(__fadd_rn(__fmul_rn(255.0f, sum), 1.0f / 256.0f));
 	}
-	const uint32_t p_tex = pa.tex, p_alpha = pa.alpha, p_fbmsk = pa.fbmsk, p_fogcol = pa.fogcol;
-	// S, T, Q and fog at the attribute position.
-	auto stqf = [&](int k) {
-		return __fadd_rn(__fadd_rn(ta.stqf0[k], __fmul_rn(ta.stqf1[k], ai)), __fmul_rn(ta.stqf2[k], aj));
-	};
+	const uint32_t p_tex = ta.tex, p_alpha = ta.alpha, p_fbmsk = ta.fbmsk, p_fogcol = ta.fogcol;
+	const TransformedAttributes *tap = static_cast<const TransformedAttributes *>(a.tattr) + prim_index;
 
 	// Combined colour (ubershader.comp:1260-1339).
 	float comb[4] = { col[0], col[1], col[2], col[3] };
-	if (sv.combiner & COMB_TME)
+	if (sv_combiner & COMB_TME)
 	{
+		float stq[3][4];
+		TextureDesc td;
+		load_texture_data(tap, static_cast<const TextureDesc *>(a.textures) + (p_tex & TEX_INDEX_MASK), stq, td);
+		// S, T, Q at the attribute position.
+		auto stqf = [&](int k) {
+			return __fadd_rn(__fadd_rn(stq[0][k], __fmul_rn(stq[1][k], ai)), __fmul_rn(stq[2][k], aj));
+		};
 		const bool persp = (st & ST_PERSPECTIVE) != 0;
 		float u = stqf(0), v = stqf(1);
 		if (persp) // q == 1 divides exactly, as the reference's q != 1 test skips
@@ -342,11 +423,10 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		}
 		u = fminf(fmaxf(u, -2047.0f), 2047.0f);
 		v = fminf(fmaxf(v, -2047.0f), 2047.0f);
-		const TextureDesc &td = static_cast<const TextureDesc *>(a.textures)[p_tex & TEX_INDEX_MASK];
 		float tx[4];
 		sample_texture(td, a.tex_images, p_tex, u, v, persp, tx);
-		const bool tcc = (sv.combiner & COMB_TCC) != 0;
-		const uint32_t mode = sv.combiner & COMB_MODE_MASK;
+		const bool tcc = (sv_combiner & COMB_TCC) != 0;
+		const uint32_t mode = sv_combiner & COMB_MODE_MASK;
 		if (mode == 1) // DECAL
 		{
 			for (int k = 0; k < 4; k++) comb[k] = tx[k];
@@ -369,9 +449,10 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 			}
 		}
 	}
-	if (sv.combiner & COMB_FOG)
+	if (sv_combiner & COMB_FOG)
 	{
-		const float fog = rintf(stqf(3));
+		const float fog = rintf(__fadd_rn(__fadd_rn(tap->stqf0[3], __fmul_rn(tap->stqf1[3], ai)),
+		                                  __fmul_rn(tap->stqf2[3], aj)));
 		for (uint32_t k = 0; k < 3; k++)
 		{
 			const float fc = byte_f(p_fogcol, k);
@@ -385,7 +466,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	for (uint32_t k = 0; k < 4; k++)
 		fb[k] = byte_f(color, k);
 	bool write_fb = true, write_zb = true, write_alpha = true;
-	const uint32_t bm = sv.blend_mode;
+	const uint32_t bm = sv_blend;
 	bool blending = bm & BLEND_ABE;
 	if (bm & BLEND_ATE)
 	{
@@ -437,7 +518,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	{
 		const int32_t dxp = px & 3, dyp = py & 3;
 		const int32_t bit_index = ((dyp * 4 + dxp) * 4) & 31;
-		const uint32_t dmx = dyp < 2 ? sv.dimx0 : sv.dimx1;
+		const uint32_t dmx = dyp < 2 ? sv_dimx0 : sv_dimx1;
 		const int32_t dv = int32_t(dmx << (29 - bit_index)) >> 29;
 		for (int k = 0; k < 3; k++)
 			o[k] = __fadd_rn(o[k], float(dv));
