@@ -632,7 +632,7 @@ __device__ __forceinline__ void shade_1x_block(const pgs_shade_args &a, uint32_t
 		{
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
-			const uint32_t prim = uni(uint32_t(__shfl(int(idx), int(bit))));
+			const uint32_t prim = uint32_t(__builtin_amdgcn_readlane(int(idx), int(bit)));
 			shade_pixel<true>(a, pi, prim, px, py, 0u, false, color, depth, cdirty, zdirty);
 		}
 	}
@@ -841,7 +841,7 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 		{
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
-			const uint32_t prim = uni(uint32_t(__shfl(int(idx), int(bit))));
+			const uint32_t prim = uint32_t(__builtin_amdgcn_readlane(int(idx), int(bit)));
 			shade_pixel<false>(a, pi, prim, hx, hy, rate, !single, color, depth, cdirty, zdirty);
 		}
 	}
