@@ -94,11 +94,12 @@ struct TextureDesc
 	uint32_t pad0;
 	// Clamp bounds of the texel coordinate (min u, min v, max u, max v): the sampler's
 	// floor(region) + 0.5 on an axis with CLAMP or REGION_CLAMP, -inf / +inf on the others, so
-	// that the clamp is two unconditional min/max operations.
+	// that the clamp is unconditional; min <= max (a region with min > max clamps everything to
+	// max, so it is stored as [max, max]).
 	float region[4];
 	float size[4];         // 2^TW, 2^TH, 1 / width, 1 / height
 	float bias[2];         // -rectangle origin / rectangle size
-	uint32_t pad1[2];
+	float fwidth, fheight; // width and height as floats
 };
 static_assert(sizeof(TextureDesc) == 64, "TextureDesc layout");
 }

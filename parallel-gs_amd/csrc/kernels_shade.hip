@@ -239,14 +239,16 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 	float u1 = perspective ? __fmul_rn(u, t.size[0]) : u;
 	float v1 = perspective ? __fmul_rn(v, t.size[1]) : v;
 	// CLAMP / REGION_CLAMP; the descriptor holds floor(region) + 0.5, or infinities on an axis
-	// that does not clamp (u1 and v1 are finite: u and v were clamped to +-2047).
-	u1 = fminf(fmaxf(u1, t.region[0]), t.region[2]);
-	v1 = fminf(fmaxf(v1, t.region[1]), t.region[3]);
+	// that does not clamp, with min <= max (a reversed region is stored as [max, max], which
+	// min(max(x, lo), hi) also gives). u1 and v1 are finite (u and v were clamped to +-2047), so
+	// the median of (x, lo, hi) is that clamp.
+	u1 = __builtin_amdgcn_fmed3f(u1, t.region[0], t.region[2]);
+	v1 = __builtin_amdgcn_fmed3f(v1, t.region[1], t.region[3]);
 	float un = floorf(__fadd_rn(__fmul_rn(u1, 16.0f), 1.0f / 64.0f));
 	float vn = floorf(__fadd_rn(__fmul_rn(v1, 16.0f), 1.0f / 64.0f));
 	un = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(un, 16.0f / 1024.0f), t.size[2]), 0.0625f), t.bias[0]);
 	vn = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(vn, 16.0f / 1024.0f), t.size[3]), 0.0625f), t.bias[1]);
-	const float fw = float(t.width), fh = float(t.height);
+	const float fw = t.fwidth, fh = t.fheight;
 	uint32_t c;
 	if (!(tex & TEX_MAG_LINEAR))
 	{
@@ -277,6 +279,7 @@ struct PassInfo
 {
 	uint32_t fb_psm, z_psm;
 	bool fb16, fb24, fb32, z16, z24, zsens;
+	uint32_t zmax; // largest depth the Z format holds
 };
 
 // Applies one primitive to this lane's sample (ubershader.comp shade_early + shade_resolve).
@@ -346,8 +349,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		uint32_t dz = uint32_t(zf);
 		dz = min(dz, 0xffffffffu - z);
 		z += dz;
-		if (pi.z16) z = min(z, 0xffffu);
-		else if (pi.z24) z = min(z, 0xffffffu);
+		z = min(z, pi.zmax);
 	}
 	if constexpr (kPredicated)
 	{
@@ -361,7 +363,9 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	const bool z_write = pi.zsens && (st & ST_Z_WRITE);
 	if (z_test)
 	{
-		alive = alive && ((st & ST_Z_GREATER) ? z > depth : z >= depth);
+		// GREATER or GEQUAL as lane masks combined with the wave-uniform choice.
+		const bool equal_passes = (st & ST_Z_GREATER) == 0;
+		alive = alive & ((z > depth) | ((z == depth) & equal_passes));
 		if constexpr (kPredicated)
 		{
 			if (__ballot(alive) == 0)
@@ -591,6 +595,7 @@ __device__ __forceinline__ void shade_1x_block(const pgs_shade_args &a, uint32_t
 	pi.zsens = a.z_psm != 0xffffffffu;
 	pi.z16 = pi.zsens && psm_is_16(a.z_psm);
 	pi.z24 = pi.zsens && (a.z_psm == PSMCT24 || a.z_psm == PSMZ24);
+	pi.zmax = pi.z16 ? 0xffffu : pi.z24 ? 0xffffffu : 0xffffffffu;
 
 	// begin_tile
 	const uint32_t caddr = swizzle_addr(uint32_t(px), uint32_t(py), a.fbp * 32u, a.fbw, a.fb_psm);
@@ -777,6 +782,7 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 	pi.zsens = a.z_psm != 0xffffffffu;
 	pi.z16 = pi.zsens && psm_is_16(a.z_psm);
 	pi.z24 = pi.zsens && (a.z_psm == PSMCT24 || a.z_psm == PSMZ24);
+	pi.zmax = pi.z16 ? 0xffffu : pi.z24 ? 0xffffffu : 0xffffffffu;
 
 	// begin_tile: samples come from their planes where the reference plane marks the pixel; a
 	// single-sampled tile reads plane 0 only.

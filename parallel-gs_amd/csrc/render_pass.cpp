@@ -610,6 +610,10 @@ int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 	else if (wms == 2) { td.region[0] = float(field(keyc, 4, 10)) + 0.5f; td.region[2] = float(field(keyc, 14, 10)) + 0.5f; }
 	if (wmt == 1) { td.region[1] = 0.5f; td.region[3] = float(l.height - 1u) + 0.5f; }
 	else if (wmt == 2) { td.region[1] = float(field(keyc, 24, 10)) + 0.5f; td.region[3] = float(field(keyc, 34, 10)) + 0.5f; }
+	td.region[0] = std::min(td.region[0], td.region[2]);
+	td.region[1] = std::min(td.region[1], td.region[3]);
+	td.fwidth = float(l.width);
+	td.fheight = float(l.height);
 	td.size[0] = float(1u << l.tw);
 	td.size[1] = float(1u << l.th);
 	td.size[2] = 1.0f / float(l.width);
