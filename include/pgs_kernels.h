@@ -50,6 +50,27 @@ typedef struct pgs_vram_snapshot_args
 
 int pgs_k_vram_snapshot(const pgs_vram_snapshot_args *args, hipStream_t stream);
 
+/* Copies a list of byte ranges (one workgroup per range; sizes a multiple of 16 bytes at
+ * 16-byte aligned addresses). Used to restore the VRAM pages and the CLUT a
+ * recording changes before each replay, for one context or for every context of a batch in
+ * one launch. No counterpart in the reference (backend extension for replays). */
+typedef struct pgs_copy_range
+{
+	const uint8_t *src;
+	uint8_t *dst;
+	uint32_t bytes;
+	uint32_t pad;
+} pgs_copy_range;
+
+typedef struct pgs_restore_args
+{
+	const pgs_copy_range *ranges; /* device array */
+	uint32_t num_ranges;
+	uint32_t pad;
+} pgs_restore_args;
+
+int pgs_k_restore(const pgs_restore_args *args, hipStream_t stream);
+
 /* Uploads the reciprocal lookup tables the raster kernels read (the 256-entry fixed-point
  * seed table and the 256-entry float seed LUT). Replaces GSRenderer::init_luts
  * (gs/gs_renderer.cpp:3600-3660). Must run once per device before any setup launch. */

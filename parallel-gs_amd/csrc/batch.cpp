@@ -84,6 +84,8 @@ Batch::~Batch()
 	destroy_graph();
 	if (dev_)
 		(void)hipFree(dev_);
+	if (restore_dev_)
+		(void)hipFree(restore_dev_);
 	if (sched_)
 		(void)hipFree(sched_);
 	if (stream_)
@@ -105,6 +107,27 @@ bool Batch::build(Renderer *const *contexts, uint32_t count, bool merge)
 		r->sync();
 	if (!stream_)
 		ok(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+	// The contexts' restore ranges (pages their recordings write, and the CLUT) in one list,
+	// restored by one launch per replay; contexts without such a list restore whole planes.
+	if (restore_dev_)
+		ok(hipFree(restore_dev_));
+	restore_dev_ = nullptr;
+	restore_count_ = 0;
+	std::vector<pgs_copy_range> ranges;
+	bool all_listed = true;
+	for (Renderer *r : contexts_)
+	{
+		if (r->restore_ranges().empty())
+			all_listed = false;
+		ranges.insert(ranges.end(), r->restore_ranges().begin(), r->restore_ranges().end());
+	}
+	if (all_listed && !ranges.empty())
+	{
+		ok(hipMalloc(&restore_dev_, ranges.size() * sizeof(pgs_copy_range)));
+		ok(hipMemcpy(restore_dev_, ranges.data(), ranges.size() * sizeof(pgs_copy_range), hipMemcpyHostToDevice));
+		restore_count_ = uint32_t(ranges.size());
+	}
 
 	merged_.clear();
 	if (!merge)
@@ -416,8 +439,7 @@ uint32_t Batch::collect_costs(float *features_out, float *log2_costs, uint32_t m
 	collecting_ = true;
 	try
 	{
-		for (Renderer *r : contexts_)
-			r->restore_snapshot(stream_);
+		restore(stream_);
 		for (auto &m : merged_)
 			launch(m);
 		ok(hipStreamSynchronize(stream_));
@@ -491,13 +513,25 @@ void Batch::launch(const Merged &m)
 		throw std::runtime_error("multi-launch failed");
 }
 
+void Batch::restore(hipStream_t stream)
+{
+	if (restore_dev_)
+	{
+		pgs_restore_args r{ restore_dev_, restore_count_, 0 };
+		if (pgs_k_restore(&r, stream) != 0)
+			throw std::runtime_error("restore launch failed");
+		return;
+	}
+	for (Renderer *r : contexts_)
+		r->restore_snapshot(stream);
+}
+
 int Batch::replay(bool use_graph)
 {
 	try
 	{
 		auto issue = [&]() {
-			for (Renderer *r : contexts_)
-				r->restore_snapshot(stream_);
+			restore(stream_);
 			for (auto &m : merged_)
 				launch(m);
 		};
@@ -578,8 +612,7 @@ void Batch::profile(double *ms, uint64_t *launches, uint32_t kinds)
 		ms[k] = 0.0;
 		launches[k] = 0;
 	}
-	for (Renderer *r : contexts_)
-		r->restore_snapshot(stream_);
+	restore(stream_);
 	std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> ev;
 	for (auto &m : merged_)
 	{

@@ -84,6 +84,9 @@ private:
 	BatchTuning tuning_;
 	hipStream_t stream_ = nullptr;
 	uint8_t *dev_ = nullptr;       // argument blocks + workgroup prefixes of every merged launch
+	pgs_copy_range *restore_dev_ = nullptr; // every context's restore ranges, one launch
+	void restore(hipStream_t stream);
+	uint32_t restore_count_ = 0;
 	size_t dev_bytes_ = 0;
 	hipGraph_t graph_ = nullptr;
 	hipGraphExec_t graph_exec_ = nullptr;

@@ -13,6 +13,7 @@
 struct pgs_context
 {
 	pgs::GSInterface gs;
+	uint64_t record_clock = 0; // VRAM write-log time at pgs_begin_recording
 };
 
 struct pgs_batch
@@ -205,6 +206,7 @@ int pgs_begin_recording(pgs_context *ctx)
 		ctx->gs.flush();
 		ctx->gs.render_pass().clear_texture_cache();
 		ctx->gs.renderer().begin_recording();
+		ctx->record_clock = ctx->gs.render_pass().writes().now();
 	});
 }
 
@@ -213,6 +215,9 @@ int pgs_end_recording(pgs_context *ctx)
 	return guard(ctx, [&] {
 		ctx->gs.flush();
 		ctx->gs.renderer().end_recording();
+		// Replays restore only the pages the recording wrote (every VRAM write of the
+		// backend, transfers and render passes, is logged with its page span).
+		ctx->gs.renderer().set_restore_pages(ctx->gs.render_pass().writes().pages_written_since(ctx->record_clock));
 	});
 }
 

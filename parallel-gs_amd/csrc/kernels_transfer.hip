@@ -167,6 +167,16 @@ __device__ __forceinline__ void snapshot_blocks_body(const pgs_vram_snapshot_arg
 	reinterpret_cast<uint32_t *>(a.shadow)[word] = reinterpret_cast<const uint32_t *>(a.vram)[word];
 }
 
+// One workgroup per range: 16-byte vector copies, every thread 16 bytes per pass.
+__global__ void __launch_bounds__(256) k_restore(pgs_restore_args a)
+{
+	const pgs_copy_range r = a.ranges[blockIdx.x];
+	const uint4 *src = reinterpret_cast<const uint4 *>(r.src);
+	uint4 *dst = reinterpret_cast<uint4 *>(r.dst);
+	for (uint32_t i = threadIdx.x; i < r.bytes / 16u; i += 256u)
+		dst[i] = src[i];
+}
+
 __global__ void __launch_bounds__(256) k_resolve_blocks(pgs_vram_copy_args a) { resolve_blocks_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(256) k_snapshot_blocks(pgs_vram_snapshot_args a) { snapshot_blocks_body(a, blockIdx.x, gridDim.x); }
 PGS_DEFINE_MULTI(256, k_resolve_blocks_multi, resolve_blocks_body, pgs_vram_copy_args)
@@ -200,6 +210,17 @@ extern "C" int pgs_k_vram_copy(const pgs_vram_copy_args *args, hipStream_t strea
 	const uint32_t waves_per_group = 4;
 	const uint32_t groups = (args->num_blocks + waves_per_group - 1) / waves_per_group;
 	hipLaunchKernelGGL(pgs::k_resolve_blocks, dim3(groups), dim3(256), 0, stream, *args);
+	hipError_t err = hipGetLastError();
+	return err == hipSuccess ? 0 : -int(err);
+}
+
+extern "C" int pgs_k_restore(const pgs_restore_args *args, hipStream_t stream)
+{
+	if (!args || (args->num_ranges && !args->ranges))
+		return -int(hipErrorInvalidValue);
+	if (args->num_ranges == 0)
+		return 0;
+	hipLaunchKernelGGL(pgs::k_restore, dim3(args->num_ranges), dim3(256), 0, stream, *args);
 	hipError_t err = hipGetLastError();
 	return err == hipSuccess ? 0 : -int(err);
 }

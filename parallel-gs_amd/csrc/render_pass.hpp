@@ -68,6 +68,15 @@ public:
 	void clut_loaded() { clut_gen_++; }
 	uint64_t now() const { return clock_; }
 	uint64_t clut_generation() const { return clut_gen_; }
+	// Pages written after time t (now() at that point), ascending.
+	std::vector<uint32_t> pages_written_since(uint64_t t) const
+	{
+		std::vector<uint32_t> pages;
+		for (uint32_t p = 0; p < kNumPages; p++)
+			if (page_[p] > t)
+				pages.push_back(p);
+		return pages;
+	}
 	bool written_since(PageSpan s, uint64_t t) const
 	{
 		if (s.last - s.first + 1u >= kNumPages)

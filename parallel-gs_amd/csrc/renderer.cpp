@@ -55,7 +55,8 @@ Renderer::~Renderer()
 		(void)hipEventDestroy(e);
 	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_,
 	                 (void *)clut_buf_, (void *)tex_images_, (void *)state_copy_, (void *)flat_buckets_,
-	                 (void *)tile_modes_, (void *)coarse_mask_, (void *)feedback_masks_, (void *)snapshot_ })
+	                 (void *)tile_modes_, (void *)coarse_mask_, (void *)feedback_masks_, (void *)snapshot_,
+	                 (void *)restore_dev_ })
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
@@ -849,6 +850,10 @@ void Renderer::begin_recording()
 	destroy_graph();
 	recorded_.clear();
 	replay_.clear();
+	restore_ranges_.clear();
+	if (restore_dev_)
+		check(hipFree(restore_dev_));
+	restore_dev_ = nullptr;
 	const size_t vram_bytes = size_t(num_planes_) * kVramSize;
 	if (!snapshot_)
 		check(hipMalloc(&snapshot_, vram_bytes + 1024u));
@@ -858,8 +863,43 @@ void Renderer::begin_recording()
 	recording_ = true;
 }
 
+void Renderer::set_restore_pages(const std::vector<uint32_t> &pages)
+{
+	restore_ranges_.clear();
+	const size_t vram_bytes = size_t(num_planes_) * kVramSize;
+	constexpr uint32_t kRunPages = 8; // 64 KiB per range
+	for (uint32_t plane = 0; plane < num_planes_; plane++)
+	{
+		const size_t base = size_t(plane) * kVramSize;
+		for (size_t i = 0; i < pages.size();)
+		{
+			size_t j = i + 1;
+			while (j < pages.size() && pages[j] == pages[j - 1] + 1 && j - i < kRunPages)
+				j++;
+			const size_t off = base + size_t(pages[i]) * kPageBytes;
+			restore_ranges_.push_back({ snapshot_ + off, vram_ + off, uint32_t((j - i) * kPageBytes), 0 });
+			i = j;
+		}
+	}
+	restore_ranges_.push_back({ snapshot_ + vram_bytes, reinterpret_cast<uint8_t *>(clut_buf_), 1024u, 0 });
+	if (restore_dev_)
+		check(hipFree(restore_dev_));
+	restore_dev_ = nullptr;
+	check(hipMalloc(&restore_dev_, restore_ranges_.size() * sizeof(pgs_copy_range)));
+	check(hipMemcpy(restore_dev_, restore_ranges_.data(), restore_ranges_.size() * sizeof(pgs_copy_range),
+	                hipMemcpyHostToDevice));
+}
+
 void Renderer::restore_snapshot(hipStream_t stream)
 {
+	if (restore_dev_)
+	{
+		pgs_restore_args r{ restore_dev_, uint32_t(restore_ranges_.size()), 0 };
+		const int rc = pgs_k_restore(&r, stream);
+		if (rc != 0)
+			check(hipError_t(-rc));
+		return;
+	}
 	const size_t vram_bytes = size_t(num_planes_) * kVramSize;
 	check(hipMemcpyAsync(vram_, snapshot_, vram_bytes, hipMemcpyDeviceToDevice, stream));
 	check(hipMemcpyAsync(clut_buf_, snapshot_ + vram_bytes, 1024u, hipMemcpyDeviceToDevice, stream));

@@ -130,6 +130,10 @@ public:
 	const std::vector<Launch> &recorded() const { return recorded_; }
 	// Restores VRAM planes and the CLUT to their state at begin_recording() (on `stream`).
 	void restore_snapshot(hipStream_t stream);
+	// After end_recording(): the 8 KiB pages (of every plane) the recording wrote; replays then
+	// restore only those and the CLUT, with one copy launch.
+	void set_restore_pages(const std::vector<uint32_t> &pages);
+	const std::vector<pgs_copy_range> &restore_ranges() const { return restore_ranges_; }
 
 private:
 	struct Arena
@@ -163,6 +167,8 @@ public:
 private:
 	static constexpr uint32_t kMaxCoarseTiles = 256, kMaxMaskWords = 1024;
 	uint8_t *snapshot_ = nullptr;     // VRAM planes + CLUT at begin_recording()
+	std::vector<pgs_copy_range> restore_ranges_; // the pages a recording writes, and the CLUT
+	pgs_copy_range *restore_dev_ = nullptr;
 	size_t lists_cap_ = 0;            // entries
 	size_t counts_cap_ = 0;
 	uint32_t tile_log2_1x_ = 5;

@@ -55,6 +55,7 @@ CLOSED_FORM = [
     (T.test_wrap_modes, dict(mode="repeat")),
     (T.test_wrap_modes, dict(mode="clamp")),
     (T.test_wrap_modes, dict(mode="region_clamp")),
+    (T.test_wrap_modes, dict(mode="region_clamp_reversed")),
     (T.test_modulate_closed_form, {}),
     (T.test_bilinear_at_texel_centres_is_exact, {}),
     (T.test_bilinear_midpoints_of_linear_ramp, {}),

@@ -204,12 +204,14 @@ def test_cld_cached_cbp():
     assert np.array_equal(img[:32, 64:], pal_b[idx])
 
 
-@pytest.mark.parametrize("mode", ["repeat", "clamp", "region_clamp"])
+@pytest.mark.parametrize("mode", ["repeat", "clamp", "region_clamp", "region_clamp_reversed"])
 def test_wrap_modes(mode):
     rng = np.random.default_rng(21)
     tex = rng.integers(0, 2**32, size=(16, 16), dtype=np.uint64)
     cl = {"repeat": rs.clamp_reg(0, 0), "clamp": rs.clamp_reg(1, 1),
-          "region_clamp": rs.clamp_reg(2, 2, minu=2, maxu=9, minv=3, maxv=12)}[mode]
+          "region_clamp": rs.clamp_reg(2, 2, minu=2, maxu=9, minv=3, maxv=12),
+          # MINU > MAXU: clamp(u, min, max) = min(max(u, min), max) is max everywhere.
+          "region_clamp_reversed": rs.clamp_reg(2, 2, minu=9, maxu=2, minv=12, maxv=3)}[mode]
     s = gif.ad(base_regs([(0x08, cl), (0x06, rs.tex0(TBP, 1, 0x00, 4, 4))])) + upload(TBP, 1, 0x00, 16, 16, tex)
     s += gif.ad(sprite(0, 0, 64 * 16, 64 * 16, 0, 0, 64 * 16, 64 * 16))
     img = frame_image(run(s))[:64, :64]
@@ -218,8 +220,10 @@ def test_wrap_modes(mode):
         ex = tex[ys % 16, xs % 16]
     elif mode == "clamp":
         ex = tex[np.minimum(ys, 15), np.minimum(xs, 15)]
-    else:
+    elif mode == "region_clamp":
         ex = tex[np.clip(ys, 3, 12), np.clip(xs, 2, 9)]
+    else:
+        ex = np.broadcast_to(tex[3, 2], (64, 64))
     assert np.array_equal(img, ex)
 
 
