@@ -566,7 +566,7 @@ void Batch::sync()
 // fastest is kept. Candidates whose tile lists do not fit are skipped.
 bool Batch::autotune(uint32_t reps, std::vector<TuningResult> *table)
 {
-	static const uint32_t tiles[] = { 0, 3, 4, 5 };
+	static const uint32_t tiles[] = { 0, 3, 4, 5, 6 };
 	static const uint32_t waves[] = { 1, 2, 4 };
 	hipEvent_t e0, e1;
 	ok(hipEventCreate(&e0));
