@@ -388,6 +388,51 @@ typedef struct pgs_mlp_adam_args
 int pgs_k_mlp_adam(const pgs_mlp_adam_args *args, hipStream_t stream);
 int pgs_k_mlp_quantize(const float *master, uint16_t *weights, hipStream_t stream);
 
+/* Fused launch (no counterpart in the reference, which records these dispatches one by one):
+ * one launch of 256-thread workgroups runs several independent parts, each the work of one
+ * launch of the kernels above (VRAM copy or snapshot, CLUT write, texture decode, triangle
+ * setup, single-sample heuristic, binning). Part p owns workgroups [begin, begin + groups);
+ * parts must not depend on each other (no part reads what another writes), and each part's
+ * arguments must satisfy its own launcher's checks. Workgroups per part: copy and snapshot
+ * (num_blocks + 3) / 4, CLUT 1, decode num_groups, setup (num_primitives + 255) / 256,
+ * heuristic 256, binning tiles_x * tiles_y. Replays use it to run dispatches whose order
+ * does not matter as one launch. */
+#define PGS_FUSED_VRAM_COPY 0u
+#define PGS_FUSED_VRAM_SNAPSHOT 1u
+#define PGS_FUSED_CLUT_WRITE 2u
+#define PGS_FUSED_TEXTURE_DECODE 3u
+#define PGS_FUSED_TRIANGLE_SETUP 4u
+#define PGS_FUSED_SSAA_HEURISTIC 5u
+#define PGS_FUSED_BINNING 6u
+#define PGS_FUSED_MAX_PARTS 8u
+
+typedef struct pgs_fused_part
+{
+	uint32_t kind;   /* PGS_FUSED_* */
+	uint32_t begin;  /* first workgroup of the part */
+	uint32_t groups; /* workgroups of the part */
+	uint32_t pad;
+	union
+	{
+		pgs_vram_copy_args copy;
+		pgs_vram_snapshot_args snapshot;
+		pgs_clut_args clut;
+		pgs_tex_decode_args decode;
+		pgs_setup_args setup;
+		pgs_ssaa_heuristic_args heuristic;
+		pgs_bin_args bin;
+	} args;
+} pgs_fused_part;
+
+typedef struct pgs_fused_args
+{
+	const pgs_fused_part *parts; /* device array, parts in workgroup order */
+	uint32_t num_parts;          /* 1 .. PGS_FUSED_MAX_PARTS */
+	uint32_t groups;             /* total workgroups (sum over the parts) */
+} pgs_fused_args;
+
+int pgs_k_fused(const pgs_fused_args *args, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ uint32_t grid_of(const Renderer::Launch &l)
 	case K::TexDecode: return pgs_grid_texture_decode(l.decode);
 	case K::SsaaHeuristic: return pgs_grid_ssaa_heuristic(l.heuristic);
 	case K::SamplerFeedback: return pgs_grid_sampler_feedback(l.feedback);
-	case K::Fork: case K::Join: break; // not in recorded kernel lists
+	case K::Fork: case K::Join: case K::Fused: break; // not in recorded kernel lists
 	}
 	return 0;
 }
@@ -50,7 +50,7 @@ size_t args_size(Renderer::LaunchKind k)
 	case K::TexDecode: return sizeof(pgs_tex_decode_args);
 	case K::SsaaHeuristic: return sizeof(pgs_ssaa_heuristic_args);
 	case K::SamplerFeedback: return sizeof(pgs_feedback_args);
-	case K::Fork: case K::Join: break;
+	case K::Fork: case K::Join: case K::Fused: break;
 	}
 	return 0;
 }
@@ -507,7 +507,7 @@ void Batch::launch(const Merged &m)
 	case K::SamplerFeedback:
 		rc = pgs_multi_sampler_feedback(static_cast<const pgs_feedback_args *>(args), prefix, n, m.total, stream_);
 		break;
-	case K::Fork: case K::Join: break;
+	case K::Fork: case K::Join: case K::Fused: break;
 	}
 	if (rc != 0)
 		throw std::runtime_error("multi-launch failed");

@@ -10,6 +10,8 @@
 #include "pgs_kernels.h"
 #include "multi_launch.hpp"
 #include "multi_dispatch.hpp"
+#include "transfer_bodies.hpp"
+#include "texture_bodies.hpp"
 
 namespace pgs
 {
@@ -414,7 +416,10 @@ __device__ __forceinline__ void triangle_setup_one(const pgs_setup_args &args, u
 // Largest coarse grid side (the launcher checks coarse_x, coarse_y and their product).
 constexpr uint32_t kMaxCoarseSide = 256;
 
-__device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uint32_t wg, uint32_t index)
+// col_bits / row_bits: this wave's kMaxCoarseSide words of LDS each; `word` is the wave's mask
+// word (primitives 64 word .. 64 word + 63). Every wave of the workgroup calls it (barrier).
+__device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uint32_t word, uint32_t index,
+                                                 uint64_t *col_bits, uint64_t *row_bits)
 {
 	int32_t cx0 = 1, cy0 = 1, cx1 = 0, cy1 = 0; // empty range
 	if (index < args.num_primitives)
@@ -432,8 +437,7 @@ __device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uin
 	// A bounding box is a column range times a row range, so the word of coarse tile (cx, cy)
 	// is the ballot of the lanes whose columns cover cx AND the ballot of the lanes whose rows
 	// cover cy: coarse_x + coarse_y ballots, kept in LDS, then every lane writes the words of
-	// tiles lane, lane + 64, ... (the launch is one wave per workgroup).
-	__shared__ uint64_t col_bits[kMaxCoarseSide], row_bits[kMaxCoarseSide];
+	// tiles lane, lane + 64, ...
 	const uint32_t lane = threadIdx.x & 63u;
 	for (uint32_t cx = 0; cx < args.coarse_x; cx++)
 	{
@@ -448,7 +452,9 @@ __device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uin
 			row_bits[cy] = bits;
 	}
 	__syncthreads();
-	uint64_t *mask = args.coarse_mask + wg;
+	if (word >= args.mask_words)
+		return; // a wave past the last primitive (workgroups of several waves)
+	uint64_t *mask = args.coarse_mask + word;
 	const uint32_t tiles = args.coarse_x * args.coarse_y;
 	for (uint32_t t = lane; t < tiles; t += 64u)
 	{
@@ -457,16 +463,27 @@ __device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uin
 	}
 }
 
-__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t nwg)
+// LDS of a setup workgroup of up to kSetupMaxWaves waves: each wave's coarse-mask ballots.
+constexpr uint32_t kSetupMaxWaves = 4;
+struct SetupLds
 {
-	const uint32_t index = wg * 64u + threadIdx.x;
+	uint64_t col[kSetupMaxWaves][kMaxCoarseSide], row[kSetupMaxWaves][kMaxCoarseSide];
+};
+
+// One thread per primitive, any workgroup size of whole waves (at most kSetupMaxWaves).
+__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t nwg,
+                                                    SetupLds &lds)
+{
+	const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6;
+	const uint32_t word = wg * waves + wave;
+	const uint32_t index = word * 64u + (threadIdx.x & 63u);
 	if (index < args.num_primitives)
 		triangle_setup_one(args, index);
 	if (args.coarse_mask)
-		coarse_mask_wave(args, wg, index);
+		coarse_mask_wave(args, word, index, lds.col[wave], lds.row[wave]);
 	// The pass's sampler-feedback masks start from zero; the feedback launch follows setup.
 	if (args.clear_words)
-		for (uint32_t w = index; w < args.clear_count; w += nwg * 64u)
+		for (uint32_t w = index; w < args.clear_count; w += nwg * blockDim.x)
 			args.clear_words[w] = 0;
 }
 
@@ -489,15 +506,24 @@ __device__ __forceinline__ uint32_t hash_vertex(const VertexPosition &v)
 	return x >> 16;
 }
 
-__device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_args &a, uint32_t wg, uint32_t)
+struct HeuristicLds
+{
+	uint32_t full3d[kMaskWords], cand[kMaskWords], shared_v[kMaskWords];
+	uint32_t any3d, min3d, max3d;
+};
+
+// Any workgroup size of whole waves (the loops stride by the workgroup size).
+__device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_args &a, uint32_t wg, uint32_t,
+                                                    HeuristicLds &lds)
 {
 	const uint32_t bucket = wg;
 	if (((a.flat_buckets[bucket >> 5] >> (bucket & 31u)) & 1u) == 0)
 		return;
 
-	__shared__ uint32_t full3d[kMaskWords], cand[kMaskWords], shared_v[kMaskWords];
-	__shared__ uint32_t any3d, min3d, max3d;
-	for (uint32_t i = threadIdx.x; i < kMaskWords; i += kHeurThreads)
+	uint32_t *full3d = lds.full3d, *cand = lds.cand, *shared_v = lds.shared_v;
+	uint32_t &any3d = lds.any3d, &min3d = lds.min3d, &max3d = lds.max3d;
+	const uint32_t threads = blockDim.x;
+	for (uint32_t i = threadIdx.x; i < kMaskWords; i += threads)
 	{
 		full3d[i] = 0;
 		cand[i] = 0;
@@ -524,7 +550,7 @@ __device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_arg
 	// per wave instead of one per vertex on the same two words.
 	uint32_t zmin = ~0u, zmax = 0u;
 	bool has3d = false;
-	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
+	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += threads)
 	{
 		const uint32_t st = a.state_copy[i];
 		if ((st & NO_FULL_3D) == 0)
@@ -560,7 +586,7 @@ __device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_arg
 	}
 	__syncthreads();
 
-	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
+	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += threads)
 	{
 		if ((a.state_copy[i] & NO_FULL_3D) != 0)
 			continue;
@@ -581,7 +607,7 @@ __device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_arg
 		const uint32_t h = hash_vertex(v);
 		return ((shared_v[h >> 5] >> (h & 31u)) & 1u) != 0;
 	};
-	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
+	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += threads)
 	{
 		if ((a.state_copy[i] & (SA | SR)) != SA)
 			continue;
@@ -595,7 +621,7 @@ __device__ __forceinline__ void ssaa_heuristic_body(const pgs_ssaa_heuristic_arg
 	__syncthreads();
 
 	const bool force_single = any3d == 0;
-	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += kHeurThreads)
+	for (uint32_t i = threadIdx.x; i < a.num_primitives; i += threads)
 	{
 		const uint32_t z = pos[3u * i].z;
 		const uint32_t sz = snapped(z);
@@ -666,7 +692,14 @@ __device__ __forceinline__ bool intersects_tile(const PrimitiveSetup &s, int32_t
 // the exact test.
 constexpr uint32_t kBinWindowWords = 256;
 
-__device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t wg, uint32_t)
+struct BinningLds
+{
+	uint32_t wave_counts[4], wave_flags[4], window_counts[4];
+	uint16_t candidates[kBinWindowWords * 64];
+};
+
+// Workgroups of 256 threads.
+__device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t wg, uint32_t, BinningLds &lds)
 {
 	const uint32_t tile = wg;
 	const uint32_t tx = tile % args.tiles_x, ty = tile / args.tiles_x;
@@ -677,10 +710,8 @@ __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t 
 	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(args.prims);
 	uint16_t *list = args.lists + size_t(tile) * args.list_stride;
 
-	__shared__ uint32_t wave_counts[4];
-	__shared__ uint32_t wave_flags[4];
-	__shared__ uint32_t window_counts[4];
-	__shared__ uint16_t candidates[kBinWindowWords * 64];
+	uint32_t *wave_counts = lds.wave_counts, *wave_flags = lds.wave_flags, *window_counts = lds.window_counts;
+	uint16_t *candidates = lds.candidates;
 	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 	uint32_t written = 0;
 	// binning.comp:87-125: flags of the primitives binned to this tile.
@@ -782,12 +813,58 @@ __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t 
 		args.counts[tile] = written;
 }
 
-__global__ void __launch_bounds__(64) k_triangle_setup(pgs_setup_args a) { triangle_setup_body(a, blockIdx.x, gridDim.x); }
-__global__ void __launch_bounds__(kHeurThreads) k_ssaa_heuristic(pgs_ssaa_heuristic_args a) { ssaa_heuristic_body(a, blockIdx.x, gridDim.x); }
-__global__ void __launch_bounds__(256) k_binning(pgs_bin_args a) { binning_body(a, blockIdx.x, gridDim.x); }
-PGS_DEFINE_MULTI(64, k_triangle_setup_multi, triangle_setup_body, pgs_setup_args)
-PGS_DEFINE_MULTI(kHeurThreads, k_ssaa_heuristic_multi, ssaa_heuristic_body, pgs_ssaa_heuristic_args)
-PGS_DEFINE_MULTI(256, k_binning_multi, binning_body, pgs_bin_args)
+// The bodies with their own LDS, for their own launches.
+__device__ __forceinline__ void setup_own_lds(const pgs_setup_args &a, uint32_t wg, uint32_t nwg)
+{
+	__shared__ SetupLds lds;
+	triangle_setup_body(a, wg, nwg, lds);
+}
+__device__ __forceinline__ void heuristic_own_lds(const pgs_ssaa_heuristic_args &a, uint32_t wg, uint32_t nwg)
+{
+	__shared__ HeuristicLds lds;
+	ssaa_heuristic_body(a, wg, nwg, lds);
+}
+__device__ __forceinline__ void binning_own_lds(const pgs_bin_args &a, uint32_t wg, uint32_t nwg)
+{
+	__shared__ BinningLds lds;
+	binning_body(a, wg, nwg, lds);
+}
+
+__global__ void __launch_bounds__(64) k_triangle_setup(pgs_setup_args a) { setup_own_lds(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kHeurThreads) k_ssaa_heuristic(pgs_ssaa_heuristic_args a) { heuristic_own_lds(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) k_binning(pgs_bin_args a) { binning_own_lds(a, blockIdx.x, gridDim.x); }
+PGS_DEFINE_MULTI(64, k_triangle_setup_multi, setup_own_lds, pgs_setup_args)
+PGS_DEFINE_MULTI(kHeurThreads, k_ssaa_heuristic_multi, heuristic_own_lds, pgs_ssaa_heuristic_args)
+PGS_DEFINE_MULTI(256, k_binning_multi, binning_own_lds, pgs_bin_args)
+
+// Fused launch: the parts' workgroups side by side, 256 threads each, one LDS area shared by
+// the parts' bodies (a workgroup runs one part).
+union FusedLds
+{
+	SetupLds setup;
+	HeuristicLds heuristic;
+	BinningLds binning;
+};
+__global__ void __launch_bounds__(256) k_fused(pgs_fused_args f)
+{
+	__shared__ FusedLds lds;
+	uint32_t p = 0;
+	while (p + 1u < f.num_parts && blockIdx.x >= uni(f.parts[p + 1u].begin))
+		p++;
+	const pgs_fused_part &part = f.parts[p];
+	const uint32_t wg = blockIdx.x - uni(part.begin), n = uni(part.groups);
+	switch (uni(part.kind))
+	{
+	case PGS_FUSED_VRAM_COPY: xfer::resolve_blocks_body(part.args.copy, wg, n); break;
+	case PGS_FUSED_VRAM_SNAPSHOT: xfer::snapshot_blocks_body(part.args.snapshot, wg, n); break;
+	case PGS_FUSED_CLUT_WRITE: texb::clut_write_body(part.args.clut, wg, n); break;
+	case PGS_FUSED_TEXTURE_DECODE: texb::texture_decode_body(part.args.decode, wg, n); break;
+	case PGS_FUSED_TRIANGLE_SETUP: triangle_setup_body(part.args.setup, wg, n, lds.setup); break;
+	case PGS_FUSED_SSAA_HEURISTIC: ssaa_heuristic_body(part.args.heuristic, wg, n, lds.heuristic); break;
+	case PGS_FUSED_BINNING: binning_body(part.args.bin, wg, n, lds.binning); break;
+	default: break;
+	}
+}
 }
 }
 
@@ -818,6 +895,19 @@ extern "C" int pgs_k_ssaa_heuristic(const pgs_ssaa_heuristic_args *args, hipStre
 	if (args->z_shift_bucket > 16 || args->z_shift > 8)
 		return -int(hipErrorInvalidValue);
 	hipLaunchKernelGGL(pgs::k_ssaa_heuristic, dim3(256), dim3(pgs::kHeurThreads), 0, stream, *args);
+	hipError_t err = hipGetLastError();
+	return err == hipSuccess ? 0 : -int(err);
+}
+
+// The parts live in device memory: their layout is the caller's (the renderer checks it on the
+// host before the upload); here only the shape of the launch is checked.
+extern "C" int pgs_k_fused(const pgs_fused_args *args, hipStream_t stream)
+{
+	if (!args || !args->parts || args->num_parts == 0 || args->num_parts > PGS_FUSED_MAX_PARTS)
+		return -int(hipErrorInvalidValue);
+	if (args->groups == 0)
+		return 0;
+	hipLaunchKernelGGL(pgs::k_fused, dim3(args->groups), dim3(256), 0, stream, *args);
 	hipError_t err = hipGetLastError();
 	return err == hipSuccess ? 0 : -int(err);
 }

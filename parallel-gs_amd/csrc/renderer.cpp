@@ -3,6 +3,7 @@
 #include "render_pass.hpp"
 #include "raster_tables.hpp"
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <stdexcept>
 
@@ -56,7 +57,7 @@ Renderer::~Renderer()
 	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_,
 	                 (void *)clut_buf_, (void *)tex_images_, (void *)state_copy_, (void *)flat_buckets_,
 	                 (void *)tile_modes_, (void *)coarse_mask_, (void *)feedback_masks_, (void *)snapshot_,
-	                 (void *)restore_dev_ })
+	                 (void *)restore_dev_, (void *)fused_dev_ })
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
@@ -331,6 +332,7 @@ void Renderer::issue(const Launch &l)
 	case LaunchKind::SsaaHeuristic: rc = pgs_k_ssaa_heuristic(&l.heuristic, s); break;
 	case LaunchKind::ShadeSuper: rc = pgs_k_shade(&l.shade, s); break;
 	case LaunchKind::SamplerFeedback: rc = pgs_k_sampler_feedback(&l.feedback, s); break;
+	case LaunchKind::Fused: rc = pgs_k_fused(&l.fused, s); break;
 	case LaunchKind::Fork:
 	case LaunchKind::Join: break;
 	}
@@ -912,6 +914,177 @@ void Renderer::end_recording()
 	arena_upload();
 	sync_streams();
 	recording_ = false;
+	if (!side_)
+		fuse_replay();
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused replays. Each launch reads and writes a few resources; a launch's level is one more
+// than the highest level of an earlier launch it depends on (reads what that one writes, or
+// writes what that one reads or writes). Launches of one level are independent, so the
+// fusable ones among them (every kernel but shading and sampler feedback, whose workgroups
+// have other shapes) become one launch of 256-thread workgroups. A textured frame's copy,
+// CLUT load, decode, setup, heuristic, binning and shading (seven launches) replay as four:
+// {copy, setup}, {CLUT, heuristic}, {decode, binning}, shading.
+
+namespace
+{
+enum : uint32_t
+{
+	R_VRAM = 1u << 0, R_SHADOW = 1u << 1, R_CLUT = 1u << 2, R_TEXIMG = 1u << 3, R_PRIMS = 1u << 4,
+	R_SETUP = 1u << 5, R_FLAT = 1u << 6, R_FEEDBACK = 1u << 7, R_BINS = 1u << 8, R_COUNT = 9
+};
+
+// Resources a launch reads and writes. PRIMS: the primitive records (state words); SETUP:
+// setup records, attributes, state copies and coarse masks; FLAT: the heuristic's bucket mask;
+// BINS: tile lists, counts and modes; TEXIMG: decoded images.
+void launch_resources(const Renderer::Launch &l, uint32_t &reads, uint32_t &writes)
+{
+	using K = Renderer::LaunchKind;
+	reads = writes = 0;
+	switch (l.kind)
+	{
+	case K::VramCopy: reads = R_VRAM | R_SHADOW; writes = R_VRAM; break;
+	case K::VramSnapshot: reads = R_VRAM; writes = R_SHADOW; break;
+	case K::ClutWrite: reads = R_VRAM; writes = R_CLUT; break;
+	case K::TexDecode:
+		reads = R_VRAM | R_CLUT | (l.decode.feedback ? R_FEEDBACK : 0u);
+		writes = R_TEXIMG;
+		break;
+	case K::Setup:
+		reads = R_PRIMS;
+		writes = R_PRIMS | R_SETUP | R_FLAT | (l.setup.clear_words ? R_FEEDBACK : 0u);
+		break;
+	case K::SamplerFeedback: reads = R_SETUP | R_PRIMS; writes = R_FEEDBACK; break;
+	case K::SsaaHeuristic: reads = R_PRIMS | R_SETUP | R_FLAT; writes = R_PRIMS | R_FLAT; break;
+	case K::Binning: reads = R_SETUP | R_PRIMS; writes = R_BINS; break;
+	case K::Shade:
+	case K::ShadeSuper: reads = R_VRAM | R_SETUP | R_PRIMS | R_BINS | R_TEXIMG; writes = R_VRAM; break;
+	default: reads = writes = ~0u; break; // anything else orders against everything
+	}
+}
+
+// The part of a fusable launch (256-thread workgroups); false for launches that do not fuse.
+bool fused_part(const Renderer::Launch &l, pgs_fused_part &p)
+{
+	using K = Renderer::LaunchKind;
+	p = {};
+	switch (l.kind)
+	{
+	case K::VramCopy: p.kind = PGS_FUSED_VRAM_COPY; p.args.copy = l.copy; p.groups = (l.copy.num_blocks + 3u) / 4u; break;
+	case K::VramSnapshot:
+		p.kind = PGS_FUSED_VRAM_SNAPSHOT; p.args.snapshot = l.snapshot; p.groups = (l.snapshot.num_blocks + 3u) / 4u;
+		break;
+	case K::ClutWrite: p.kind = PGS_FUSED_CLUT_WRITE; p.args.clut = l.clut; p.groups = 1; break;
+	case K::TexDecode: p.kind = PGS_FUSED_TEXTURE_DECODE; p.args.decode = l.decode; p.groups = l.decode.num_groups; break;
+	case K::Setup:
+		p.kind = PGS_FUSED_TRIANGLE_SETUP; p.args.setup = l.setup; p.groups = (l.setup.num_primitives + 255u) / 256u;
+		break;
+	case K::SsaaHeuristic: p.kind = PGS_FUSED_SSAA_HEURISTIC; p.args.heuristic = l.heuristic; p.groups = 256; break;
+	case K::Binning: p.kind = PGS_FUSED_BINNING; p.args.bin = l.bin; p.groups = l.bin.tiles_x * l.bin.tiles_y; break;
+	default: return false;
+	}
+	return true;
+}
+}
+
+void Renderer::fuse_replay()
+{
+	if (fused_dev_)
+		check(hipFree(fused_dev_));
+	fused_dev_ = nullptr;
+	std::array<int32_t, R_COUNT> last_read, last_write;
+	last_read.fill(-1);
+	last_write.fill(-1);
+	std::vector<int32_t> level(replay_.size());
+	int32_t max_level = -1;
+	for (size_t i = 0; i < replay_.size(); i++)
+	{
+		uint32_t reads, writes;
+		launch_resources(replay_[i], reads, writes);
+		int32_t lv = 0;
+		for (uint32_t r = 0; r < R_COUNT; r++)
+		{
+			if (reads & (1u << r))
+				lv = std::max(lv, last_write[r] + 1);
+			if (writes & (1u << r))
+				lv = std::max(lv, std::max(last_write[r], last_read[r]) + 1);
+		}
+		for (uint32_t r = 0; r < R_COUNT; r++)
+		{
+			if (reads & (1u << r))
+				last_read[r] = std::max(last_read[r], lv);
+			if (writes & (1u << r))
+				last_write[r] = std::max(last_write[r], lv);
+		}
+		level[i] = lv;
+		max_level = std::max(max_level, lv);
+	}
+	std::vector<std::vector<size_t>> by_level(size_t(max_level + 1));
+	for (size_t i = 0; i < replay_.size(); i++)
+		by_level[size_t(level[i])].push_back(i);
+
+	std::vector<Launch> out;
+	std::vector<pgs_fused_part> parts;
+	std::vector<std::pair<size_t, size_t>> fused_spans; // (first part, parts) per fused launch, by out index
+	std::vector<size_t> fused_out;
+	for (const auto &members : by_level)
+	{
+		std::vector<pgs_fused_part> group;
+		std::vector<size_t> single;
+		for (size_t i : members)
+		{
+			pgs_fused_part p;
+			if (fused_part(replay_[i], p))
+			{
+				if (p.groups)
+					group.push_back(p);
+			}
+			else
+				single.push_back(i);
+		}
+		if (group.size() == 1)
+		{
+			// One part: its own launch, as recorded.
+			for (size_t i : members)
+			{
+				pgs_fused_part p;
+				if (fused_part(replay_[i], p) && p.groups)
+					out.push_back(replay_[i]);
+			}
+		}
+		else if (group.size() > 1)
+		{
+			for (size_t first = 0; first < group.size(); first += PGS_FUSED_MAX_PARTS)
+			{
+				const size_t n = std::min<size_t>(PGS_FUSED_MAX_PARTS, group.size() - first);
+				uint32_t begin = 0;
+				for (size_t k = 0; k < n; k++)
+				{
+					group[first + k].begin = begin;
+					begin += group[first + k].groups;
+				}
+				fused_spans.push_back({ parts.size(), n });
+				fused_out.push_back(out.size());
+				parts.insert(parts.end(), group.begin() + first, group.begin() + first + n);
+				Launch l{};
+				l.kind = LaunchKind::Fused;
+				l.fused.num_parts = uint32_t(n);
+				l.fused.groups = begin;
+				out.push_back(l);
+			}
+		}
+		for (size_t i : single)
+			out.push_back(replay_[i]);
+	}
+	if (!parts.empty())
+	{
+		check(hipMalloc(&fused_dev_, parts.size() * sizeof(pgs_fused_part)));
+		check(hipMemcpy(fused_dev_, parts.data(), parts.size() * sizeof(pgs_fused_part), hipMemcpyHostToDevice));
+		for (size_t k = 0; k < fused_out.size(); k++)
+			out[fused_out[k]].fused.parts = fused_dev_ + fused_spans[k].first;
+	}
+	replay_ = std::move(out);
 }
 
 void Renderer::destroy_graph()

@@ -108,7 +108,9 @@ public:
 	enum class LaunchKind : uint32_t
 	{
 		VramCopy = 0, VramSnapshot = 1, Setup = 2, Binning = 3, Shade = 4, ClutWrite = 5, TexDecode = 6,
-		SsaaHeuristic = 7, ShadeSuper = 8, SamplerFeedback = 9, Fork = 10, Join = 11
+		SsaaHeuristic = 7, ShadeSuper = 8, SamplerFeedback = 9, Fork = 10, Join = 11,
+		// Replays only: independent launches of one dependency level run as one (pgs_k_fused).
+		Fused = 12
 	};
 	struct Launch
 	{
@@ -124,6 +126,7 @@ public:
 			pgs_tex_decode_args decode;
 			pgs_ssaa_heuristic_args heuristic;
 			pgs_feedback_args feedback;
+			pgs_fused_args fused;
 		};
 	};
 	// The recorded kernel launches, in issue order (without the stream ordering points).
@@ -229,6 +232,10 @@ private:
 	// Makes the main stream wait for the texture stream's outstanding work.
 	void join_texture_stream();
 	void sync_streams();
+	// Regroups the replay list by dependency level: launches of one level that read nothing
+	// another writes become one fused launch (one stream only).
+	void fuse_replay();
+	pgs_fused_part *fused_dev_ = nullptr; // the fused launches' parts of the current recording
 	// Bump-allocates `bytes` in the arena; returns the host pointer and device offset.
 	uint8_t *arena_alloc(size_t bytes, size_t &dev_offset);
 	void arena_make_room(size_t bytes);

@@ -61,3 +61,33 @@ def test_texture_stream_replays(hip_device, use_graph):
             ctx.replay(use_graph)
             ctx.sync()
             assert_equal(ctx.vram_planes(), live)
+
+
+REPLAY_CASES = {
+    "textured_frames": (lambda: pgs.gen_textured(0x51E0, 700, 3, 320, 224, 0.0), 4, "auto"),
+    "random_textured_1x": (lambda: R.random_textured_stream(44), 1, "auto"),
+    "random_textured_4x": (lambda: R.random_textured_stream(45), 4, "auto"),
+    "transfers": (lambda: pgs.gen_transfers_wild(46, 80), 1, "auto"),
+    "feedback_forced": (lambda: R.random_textured_stream(47), 4, "always"),
+}
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("case", sorted(REPLAY_CASES))
+def test_fused_replays_match_live(hip_device, case, use_graph):
+    """On one stream a recording's replay runs its independent launches of each dependency level
+    as one fused launch (copy with setup, CLUT load with heuristic, decode with binning): every
+    replay ends with the VRAM of the live run, which equals the oracle's."""
+    make, ssaa, feedback = REPLAY_CASES[case]
+    stream = make()
+    print(f"[case] fused replay {case} ssaa={ssaa} feedback={feedback} use_graph={use_graph}", flush=True)
+    with pgs.Context(super_sampling=ssaa, sampler_feedback=feedback) as ctx:
+        ctx.begin_recording()
+        ctx.gif_transfer(stream)
+        ctx.end_recording()
+        live = ctx.vram_planes()
+        assert_equal(live, oracle(stream, ssaa))
+        for _ in range(2):
+            ctx.replay(use_graph)
+            ctx.sync()
+            assert_equal(ctx.vram_planes(), live)
