@@ -27,8 +27,8 @@ KIND_NAMES = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade"
 # 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), and the kernel's waves and VALU
 # instructions per wave from an SQ_WAVES / SQ_INSTS_VALU run; the runs are summarised in
 # profiles/hbm_traffic_pmc__plain_and_textured4x.md.
-MEASURED_TRAFFIC = {"plain": 6915392, "textured": 20956781}
-MEASURED_VALU = {"plain": (4480, 10676), "textured": (17920, 2179)}  # (waves, VALU instructions per wave)
+MEASURED_TRAFFIC = {"plain": 6915392, "textured": 20951962}
+MEASURED_VALU = {"plain": (4480, 10676), "textured": (17920, 2011)}  # (waves, VALU instructions per wave)
 # MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a SIMD for 4 cycles at 2.4 GHz.
 VALU_ISSUE_S_PER_WAVE_INST = 4.0 / 1024 / 2.4e9
 
