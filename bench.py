@@ -21,7 +21,7 @@ METRIC = "shaded Mpix/s + frames/s at bit-exact VRAM; batched/learned-sched vs p
 HBM_PEAK_GBS = 8000.0
 # Launch kinds of the renderer's profile counters (LaunchKind in renderer.hpp).
 KIND_NAMES = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade", "clut_write", "texture_decode",
-              "ssaa_heuristic", "shade_super", "sampler_feedback", "fork", "join", "fused"]
+              "ssaa_heuristic", "shade_super", "sampler_feedback", "fork", "join", "fused", "restore"]
 # L2-to-fabric bytes per launch of each workload's dominant kernel, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by what the counter reads for a
 # 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), and the kernel's waves and VALU

@@ -100,7 +100,8 @@ int pgs_replay(pgs_context *ctx, int use_graph);
  * bracketed by events on its stream; pgs_profile_collect synchronizes and writes the summed
  * milliseconds and launch counts per kernel kind (0 = VRAM copy, 1 = snapshot, 2 = triangle
  * setup, 3 = binning, 4 = shading, 5 = CLUT write, 6 = texture decode, 7 = heuristic,
- * 8 = super-sampled shading, 9 = sampler feedback, 12 = fused launch of a replay). */
+ * 8 = super-sampled shading, 9 = sampler feedback, 12 = fused launch of a replay, 13 = a replay's
+ * VRAM restore, timed in replays without a graph). */
 int pgs_profile_enable(pgs_context *ctx, int enable);
 int pgs_profile_collect(pgs_context *ctx, double *ms, uint64_t *launches, uint32_t kinds);
 /* Blocks until all submitted work has finished. */

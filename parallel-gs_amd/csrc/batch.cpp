@@ -260,6 +260,45 @@ bool Batch::upload()
 		m.tile_prefix_offset = bytes;
 		if (is_shade(m))
 			bytes = align256(bytes + sizeof(uint32_t) * (m.refs.size() + 1u));
+		m.run = m.run_n = m.run_total = 0;
+		m.in_run = false;
+	}
+	// Runs of adjacent shading launches whose contexts are disjoint: passes of different
+	// sampling grids that no context orders against each other.
+	for (size_t i = 0; i < merged_.size();)
+	{
+		if (!is_shade(merged_[i]))
+		{
+			i++;
+			continue;
+		}
+		std::vector<bool> used(contexts_.size(), false);
+		size_t j = i;
+		for (; j < merged_.size() && is_shade(merged_[j]); j++)
+		{
+			bool clash = false;
+			for (const Ref &r : merged_[j].refs)
+				clash = clash || used[r.context];
+			if (clash)
+				break;
+			for (const Ref &r : merged_[j].refs)
+				used[r.context] = true;
+		}
+		if (j - i >= 2)
+		{
+			Merged &h = merged_[i];
+			h.run = uint32_t(j - i);
+			for (size_t k = i; k < j; k++)
+			{
+				h.run_n += uint32_t(merged_[k].refs.size());
+				merged_[k].in_run = k > i;
+			}
+			h.run_args_offset = bytes;
+			bytes = align256(bytes + sizeof(pgs_shade_args) * h.run_n);
+			h.run_prefix_offset = bytes;
+			bytes = align256(bytes + sizeof(uint32_t) * (h.run_n + 1u));
+		}
+		i = j;
 	}
 	std::vector<uint8_t> host(bytes);
 	for (auto &m : merged_)
@@ -299,6 +338,25 @@ bool Batch::upload()
 			m.rate_x = l0.shade.rate_x_log2;
 			m.rate_y = l0.shade.rate_log2;
 		}
+	}
+	for (size_t i = 0; i < merged_.size(); i++)
+	{
+		Merged &h = merged_[i];
+		if (h.run < 2)
+			continue;
+		pgs_shade_args *args = reinterpret_cast<pgs_shade_args *>(host.data() + h.run_args_offset);
+		uint32_t *prefix = reinterpret_cast<uint32_t *>(host.data() + h.run_prefix_offset);
+		uint32_t at = 0, total = 0;
+		for (size_t k = i; k < i + h.run; k++)
+			for (const Ref &r : merged_[k].refs)
+			{
+				const Renderer::Launch &l = launches_[r.context][r.index];
+				args[at] = l.shade;
+				prefix[at++] = total;
+				total += grid_of(l);
+			}
+		prefix[at] = total;
+		h.run_total = total;
 	}
 	if (bytes > dev_bytes_)
 	{
@@ -346,6 +404,14 @@ void Batch::launch_shade(const Merged &m)
 	const void *args = dev_ + m.args_offset;
 	const uint32_t *prefix = reinterpret_cast<const uint32_t *>(dev_ + m.prefix_offset);
 	const uint32_t n = uint32_t(m.refs.size());
+	if (schedule_ == Schedule::Hardware && !collecting_ && m.run >= 2)
+	{
+		if (pgs_multi_shade_mixed(reinterpret_cast<const pgs_shade_args *>(dev_ + m.run_args_offset),
+		                          reinterpret_cast<const uint32_t *>(dev_ + m.run_prefix_offset), m.run_n,
+		                          m.run_total, tuning_.waves_per_group, stream_) != 0)
+			throw std::runtime_error("shade multi-launch failed");
+		return;
+	}
 	if (schedule_ == Schedule::Hardware && !collecting_)
 	{
 		if (pgs_multi_shade(static_cast<const pgs_shade_args *>(args), prefix, n, m.total, m.rate_x, m.rate_y,
@@ -477,9 +543,29 @@ uint32_t Batch::collect_costs(float *features_out, float *log2_costs, uint32_t m
 	return written;
 }
 
+uint32_t Batch::num_launches() const
+{
+	uint32_t n = 0;
+	for (const Merged &m : merged_)
+		n += (schedule_ == Schedule::Hardware && m.in_run) ? 0u : 1u;
+	return n;
+}
+
+bool Batch::issues(const Merged &m) const
+{
+	if (schedule_ == Schedule::Hardware && !collecting_)
+	{
+		if (m.in_run)
+			return false;
+		if (m.run >= 2)
+			return m.run_total != 0;
+	}
+	return m.total != 0;
+}
+
 void Batch::launch(const Merged &m)
 {
-	if (m.total == 0)
+	if (!issues(m))
 		return;
 	using K = Renderer::LaunchKind;
 	const void *args = dev_ + m.args_offset;
@@ -612,11 +698,19 @@ void Batch::profile(double *ms, uint64_t *launches, uint32_t kinds)
 		ms[k] = 0.0;
 		launches[k] = 0;
 	}
-	restore(stream_);
 	std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> ev;
+	{
+		hipEvent_t a, b;
+		ok(hipEventCreate(&a));
+		ok(hipEventCreate(&b));
+		ok(hipEventRecord(a, stream_));
+		restore(stream_);
+		ok(hipEventRecord(b, stream_));
+		ev.push_back({ Renderer::kProfileRestore, { a, b } });
+	}
 	for (auto &m : merged_)
 	{
-		if (m.total == 0)
+		if (!issues(m))
 			continue;
 		hipEvent_t a, b;
 		ok(hipEventCreate(&a));

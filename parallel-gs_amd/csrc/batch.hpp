@@ -59,7 +59,8 @@ public:
 	uint32_t collect_costs(float *features, float *log2_costs, uint32_t max_items);
 	void sync();
 	hipStream_t stream() const { return stream_; }
-	uint32_t num_launches() const { return uint32_t(merged_.size()); }
+	// Kernel launches per replay (a shading run counts once under the hardware schedule).
+	uint32_t num_launches() const;
 	uint32_t num_contexts() const { return uint32_t(contexts_.size()); }
 	const BatchTuning &tuning() const { return tuning_; }
 
@@ -76,6 +77,12 @@ private:
 		size_t tile_prefix_offset = 0; // shading launches: coarse-tile prefix over the launches
 		uint32_t tiles = 0;
 		uint32_t waves = 0; // shading launches: waves of all tiles (bound on dispatch-order entries)
+		// Runs of adjacent shading launches over disjoint contexts: the first launch of a run
+		// holds the run's combined argument blocks and prefix (run = launches in the run, >= 2);
+		// the others are marked in_run. The hardware schedule issues a run as one launch.
+		uint32_t run = 0, run_n = 0, run_total = 0;
+		bool in_run = false;
+		size_t run_args_offset = 0, run_prefix_offset = 0;
 	};
 
 	std::vector<Renderer *> contexts_;
@@ -98,6 +105,8 @@ private:
 
 	bool upload();
 	void destroy_graph();
+	// Whether launch(m) issues a kernel (shading runs are issued by their first launch).
+	bool issues(const Merged &m) const;
 	void launch(const Merged &m);
 	void launch_shade(const Merged &m);
 };

@@ -689,8 +689,11 @@ __device__ __forceinline__ bool intersects_tile(const PrimitiveSetup &s, int32_t
 // word, a block-wide prefix sum of the word popcounts places the set bits' indices in LDS in
 // ascending order, and only those primitives are tested. Without one, every primitive is a
 // candidate. Both give the same lists: the coarse bit is set for every primitive that passes
-// the exact test.
-constexpr uint32_t kBinWindowWords = 256;
+// the exact test. A window is 128 mask words (8192 primitives): its 16 KiB of candidates let 8
+// workgroups share a CU (32 KiB windows allowed 5), which hides more of the latency of the
+// candidates' setup-record loads; threads past the window load no word.
+constexpr uint32_t kBinWindowWords = 128;
+static_assert(kBinWindowWords <= 256, "a window holds at most one mask word per thread");
 
 struct BinningLds
 {
@@ -764,7 +767,7 @@ __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t 
 		for (uint32_t win = 0; win < args.mask_words; win += kBinWindowWords)
 		{
 			const uint32_t w = win + threadIdx.x;
-			uint64_t m = w < args.mask_words ? cmask[w] : 0ull;
+			uint64_t m = (threadIdx.x < kBinWindowWords && w < args.mask_words) ? cmask[w] : 0ull;
 			const uint32_t c = uint32_t(__popcll(m));
 			uint32_t incl = c;
 			for (uint32_t d = 1; d < 64u; d <<= 1)

@@ -22,6 +22,16 @@ namespace pgs
 namespace
 {
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// A wave-uniform pointer in scalar registers (an operand of a scalar load). Free where the
+// compiler already knows the pointer is uniform; needed where it cannot tell (a kernel holding
+// the shading bodies of several sampling grids).
+template <typename T>
+__device__ __forceinline__ const T *uni_ptr(const T *p)
+{
+	const uint64_t v = reinterpret_cast<uint64_t>(p);
+	const uint64_t lo = uni(uint32_t(v)), hi = uni(uint32_t(v >> 32));
+	return reinterpret_cast<const T *>(hi << 32 | lo);
+}
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
 __device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
 
@@ -45,7 +55,7 @@ __device__ __forceinline__ void load_setup_and_state(const PrimitiveSetup *sp, c
 	    "s_load_dword %2, %4, 0x8\n\t"
 	    "s_waitcnt lgkmcnt(0)"
 	    : "=&s"(lo), "=&s"(hi), "=&s"(st)
-	    : "s"(sp), "s"(pp));
+	    : "s"(uni_ptr(sp)), "s"(uni_ptr(pp)));
 	__builtin_memcpy(&s, &lo, 64);
 	__builtin_memcpy(reinterpret_cast<uint8_t *>(&s) + 64, &hi, 16);
 	state = st;
@@ -74,7 +84,7 @@ __device__ __forceinline__ void load_shade_data(const TransformedAttributes *tp,
 	    "s_load_dword %4, %7, 0xc\n\t"
 	    "s_waitcnt lgkmcnt(0)"
 	    : "=&s"(col_a), "=&s"(col_b), "=&s"(sv), "=&s"(pa), "=&s"(tex)
-	    : "s"(tp), "s"(svp), "s"(pp));
+	    : "s"(uni_ptr(tp)), "s"(uni_ptr(svp)), "s"(uni_ptr(pp)));
 	__builtin_memcpy(&d.col0[0], &col_a, 32);
 	__builtin_memcpy(&d.dcol2[0], &col_b, 16);
 	__builtin_memcpy(&d.sv, &sv, 16);
@@ -95,7 +105,7 @@ __device__ __forceinline__ void load_texture_data(const TransformedAttributes *t
 	    "s_load_dwordx16 %2, %4, 0x0\n\t"
 	    "s_waitcnt lgkmcnt(0)"
 	    : "=&s"(stq_a), "=&s"(stq_b), "=&s"(desc)
-	    : "s"(tp), "s"(tdp));
+	    : "s"(uni_ptr(tp)), "s"(uni_ptr(tdp)));
 	__builtin_memcpy(&stq[0][0], &stq_a, 32);
 	__builtin_memcpy(&stq[2][0], &stq_b, 16);
 	__builtin_memcpy(&td, &desc, 64);
@@ -943,9 +953,9 @@ __device__ __forceinline__ void shade_ssaa_body(const pgs_shade_args &a, uint32_
 }
 
 __global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a) { shade_1x_body(a, blockIdx.x, gridDim.x); }
-template <uint32_t RX, uint32_t RY>
 // Held to 8 waves per SIMD (64 VGPRs, 78 SGPRs with a few spilled to VGPR lanes): faster than
 // the 7 waves its unbounded 106 SGPRs allow (textured 4x: 86.5 against 91 us per launch).
+template <uint32_t RX, uint32_t RY>
 __global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 k_shade_ssaa(pgs_shade_args a)
 {
 	shade_ssaa_body<RX, RY>(a, blockIdx.x, gridDim.x);
@@ -955,6 +965,25 @@ PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_01_multi, (shade_ssaa_b
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_11_multi, (shade_ssaa_body<1, 1>), pgs_shade_args)
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_args)
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
+
+// Launches of different sampling grids in one multi-launch (independent passes of different
+// contexts): each workgroup runs the body of its own launch's grid. One launch instead of one
+// per grid leaves no partly idle tail between the grids.
+__device__ __forceinline__ void shade_any_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
+{
+	const uint32_t rx = a.rate_x_log2, ry = a.rate_log2;
+	if (ry == 0)
+		shade_1x_body(a, wg, nwg);
+	else if (rx == 0 && ry == 1)
+		shade_ssaa_body<0, 1>(a, wg, nwg);
+	else if (rx == 1 && ry == 1)
+		shade_ssaa_body<1, 1>(a, wg, nwg);
+	else if (rx == 1 && ry == 2)
+		shade_ssaa_body<1, 2>(a, wg, nwg);
+	else if (rx == 2 && ry == 2)
+		shade_ssaa_body<2, 2>(a, wg, nwg);
+}
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_mixed_multi, shade_any_body, pgs_shade_args)
 
 // Whether a launch shading `want_mode` tiles works on a tile of `mode`: a super-sampled launch
 // with single_tiles also takes the single-sampled tiles (with the same S waves per block, of
@@ -1262,6 +1291,15 @@ int pgs_multi_shade(const pgs_shade_args *args, const uint32_t *prefix, uint32_t
 		hipLaunchKernelGGL(pgs::k_shade_ssaa_22_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
 	else
 		return -int(hipErrorNotSupported);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+}
+
+// Launches of any supported sampling grids; every launch shares the waves per group.
+int pgs_multi_shade_mixed(const pgs_shade_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
+                          uint32_t waves, hipStream_t stream)
+{
+	const uint32_t threads = 64u * (waves ? waves : 4u);
+	hipLaunchKernelGGL(pgs::k_shade_mixed_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
 	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }
 

@@ -21,13 +21,25 @@ namespace
 using xfer::resolve_blocks_body;
 using xfer::snapshot_blocks_body;
 
-// One workgroup per range: 16-byte vector copies, every thread 16 bytes per pass.
+// One workgroup per range: 16-byte vector copies. Source (the snapshot) and destination (the
+// VRAM planes and CLUT) are separate allocations, so every thread issues four loads before its
+// four stores instead of waiting out each load's latency.
 __global__ void __launch_bounds__(256) k_restore(pgs_restore_args a)
 {
 	const pgs_copy_range r = a.ranges[blockIdx.x];
-	const uint4 *src = reinterpret_cast<const uint4 *>(r.src);
-	uint4 *dst = reinterpret_cast<uint4 *>(r.dst);
-	for (uint32_t i = threadIdx.x; i < r.bytes / 16u; i += 256u)
+	const uint4 *__restrict__ src = reinterpret_cast<const uint4 *>(r.src);
+	uint4 *__restrict__ dst = reinterpret_cast<uint4 *>(r.dst);
+	const uint32_t n = r.bytes / 16u;
+	uint32_t i = threadIdx.x;
+	for (; i + 768u < n; i += 1024u)
+	{
+		const uint4 v0 = src[i], v1 = src[i + 256u], v2 = src[i + 512u], v3 = src[i + 768u];
+		dst[i] = v0;
+		dst[i + 256u] = v1;
+		dst[i + 512u] = v2;
+		dst[i + 768u] = v3;
+	}
+	for (; i < n; i += 256u)
 		dst[i] = src[i];
 }
 

@@ -1108,7 +1108,19 @@ int Renderer::replay(bool use_graph)
 		join_texture_stream();
 		if (!use_graph)
 		{
+			TimedLaunch t{ kProfileRestore, nullptr, nullptr };
+			if (profiling_)
+			{
+				t.start = get_event();
+				t.stop = get_event();
+				check(hipEventRecord(t.start, stream_));
+			}
 			restore_snapshot(stream_);
+			if (profiling_)
+			{
+				check(hipEventRecord(t.stop, stream_));
+				timed_.push_back(t);
+			}
 			for (auto &l : replay_)
 				issue(l);
 			join_texture_stream();

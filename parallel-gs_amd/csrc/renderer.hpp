@@ -112,6 +112,8 @@ public:
 		// Replays only: independent launches of one dependency level run as one (pgs_k_fused).
 		Fused = 12
 	};
+	// Profile counter of a replay's VRAM restore (after the launch kinds).
+	static constexpr uint32_t kProfileRestore = 13;
 	struct Launch
 	{
 		LaunchKind kind;

@@ -118,3 +118,27 @@ def test_schedules_are_bit_exact(hip_device):
     batch.close()
     for c in ctxs:
         c.close()
+
+
+def test_large_contexts_bin_in_several_windows(hip_device):
+    """Contexts of 40000 primitives: binning takes a tile's coarse-mask candidates in several
+    windows of primitives, live (own launches) and batched (multi-launches). Every primitive is
+    binned once: with blending, a primitive listed twice would blend twice."""
+    specs = []
+    for ssaa, blend in [(1, 0), (4, 1)]:
+        sp = pgs.batched_specs(0x5EED0011 + ssaa, 1, n_min=40000, n_max=40000, ssaa_choices=(ssaa,),
+                               blend_presets=(blend,))[0]
+        specs.append(sp)
+    ctxs, live = make_contexts(specs)
+    for sp, planes in zip(specs, live):
+        ref = oracle_planes(sp)
+        bad = np.flatnonzero(planes != ref)
+        assert bad.size == 0, (sp, bad[:8])
+    batch = pgs.Batch(ctxs)
+    batch.replay(True)
+    batch.sync()
+    for i, ctx in enumerate(ctxs):
+        assert np.array_equal(ctx.vram_planes(), live[i]), i
+    batch.close()
+    for c in ctxs:
+        c.close()
