@@ -262,19 +262,28 @@ bool Batch::upload()
 			bytes = align256(bytes + sizeof(uint32_t) * (m.refs.size() + 1u));
 		m.run = m.run_n = m.run_total = 0;
 		m.in_run = false;
+		m.run_all_grids = false;
 	}
 	// Runs of adjacent shading launches whose contexts are disjoint: passes of different
-	// sampling grids that no context orders against each other.
+	// sampling grids that no context orders against each other. Only grids the mixed-grid
+	// kernel holds (1x, 2x, 4x, 8x, 16x) join a run.
+	auto mixable = [&](const Merged &m) {
+		if (!is_shade(m))
+			return false;
+		const pgs_shade_args &s = launches_[m.refs[0].context][m.refs[0].index].shade;
+		const uint32_t rx = s.rate_x_log2, ry = s.rate_log2;
+		return ry == 0 || (ry == 1 && rx <= 1) || (ry == 2 && (rx == 1 || rx == 2));
+	};
 	for (size_t i = 0; i < merged_.size();)
 	{
-		if (!is_shade(merged_[i]))
+		if (!mixable(merged_[i]))
 		{
 			i++;
 			continue;
 		}
 		std::vector<bool> used(contexts_.size(), false);
 		size_t j = i;
-		for (; j < merged_.size() && is_shade(merged_[j]); j++)
+		for (; j < merged_.size() && mixable(merged_[j]); j++)
 		{
 			bool clash = false;
 			for (const Ref &r : merged_[j].refs)
@@ -351,6 +360,8 @@ bool Batch::upload()
 			for (const Ref &r : merged_[k].refs)
 			{
 				const Renderer::Launch &l = launches_[r.context][r.index];
+				const uint32_t rx = l.shade.rate_x_log2, ry = l.shade.rate_log2;
+				h.run_all_grids = h.run_all_grids || !(ry == 0 || (ry == 1 && rx <= 1));
 				args[at] = l.shade;
 				prefix[at++] = total;
 				total += grid_of(l);
@@ -408,7 +419,7 @@ void Batch::launch_shade(const Merged &m)
 	{
 		if (pgs_multi_shade_mixed(reinterpret_cast<const pgs_shade_args *>(dev_ + m.run_args_offset),
 		                          reinterpret_cast<const uint32_t *>(dev_ + m.run_prefix_offset), m.run_n,
-		                          m.run_total, tuning_.waves_per_group, stream_) != 0)
+		                          m.run_total, tuning_.waves_per_group, m.run_all_grids, stream_) != 0)
 			throw std::runtime_error("shade multi-launch failed");
 		return;
 	}

@@ -82,6 +82,7 @@ private:
 		// the others are marked in_run. The hardware schedule issues a run as one launch.
 		uint32_t run = 0, run_n = 0, run_total = 0;
 		bool in_run = false;
+		bool run_all_grids = false; // the run has an 8x or 16x launch
 		size_t run_args_offset = 0, run_prefix_offset = 0;
 	};
 

@@ -968,7 +968,10 @@ PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_22_multi, (shade_ssaa_b
 
 // Launches of different sampling grids in one multi-launch (independent passes of different
 // contexts): each workgroup runs the body of its own launch's grid. One launch instead of one
-// per grid leaves no partly idle tail between the grids.
+// per grid leaves no partly idle tail between the grids. The bodies share one register
+// allocation, so a kernel holding only the 1x, 2x and 4x bodies (kAllGrids false) spills less
+// than one holding all five.
+template <bool kAllGrids>
 __device__ __forceinline__ void shade_any_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
 {
 	const uint32_t rx = a.rate_x_log2, ry = a.rate_log2;
@@ -978,12 +981,16 @@ __device__ __forceinline__ void shade_any_body(const pgs_shade_args &a, uint32_t
 		shade_ssaa_body<0, 1>(a, wg, nwg);
 	else if (rx == 1 && ry == 1)
 		shade_ssaa_body<1, 1>(a, wg, nwg);
-	else if (rx == 1 && ry == 2)
-		shade_ssaa_body<1, 2>(a, wg, nwg);
-	else if (rx == 2 && ry == 2)
-		shade_ssaa_body<2, 2>(a, wg, nwg);
+	else if constexpr (kAllGrids)
+	{
+		if (rx == 1 && ry == 2)
+			shade_ssaa_body<1, 2>(a, wg, nwg);
+		else if (rx == 2 && ry == 2)
+			shade_ssaa_body<2, 2>(a, wg, nwg);
+	}
 }
-PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_mixed_multi, shade_any_body, pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_mixed_multi, shade_any_body<true>, pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_mixed_low_multi, shade_any_body<false>, pgs_shade_args)
 
 // Whether a launch shading `want_mode` tiles works on a tile of `mode`: a super-sampled launch
 // with single_tiles also takes the single-sampled tiles (with the same S waves per block, of
@@ -1296,10 +1303,13 @@ int pgs_multi_shade(const pgs_shade_args *args, const uint32_t *prefix, uint32_t
 
 // Launches of any supported sampling grids; every launch shares the waves per group.
 int pgs_multi_shade_mixed(const pgs_shade_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
-                          uint32_t waves, hipStream_t stream)
+                          uint32_t waves, bool all_grids, hipStream_t stream)
 {
 	const uint32_t threads = 64u * (waves ? waves : 4u);
-	hipLaunchKernelGGL(pgs::k_shade_mixed_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	if (all_grids)
+		hipLaunchKernelGGL(pgs::k_shade_mixed_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	else
+		hipLaunchKernelGGL(pgs::k_shade_mixed_low_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
 	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }
 

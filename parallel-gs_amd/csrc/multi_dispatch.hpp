@@ -25,9 +25,10 @@ int pgs_multi_ssaa_heuristic(const pgs_ssaa_heuristic_args *args, const uint32_t
 int pgs_multi_binning(const pgs_bin_args *args, const uint32_t *prefix, uint32_t n, uint32_t total, hipStream_t stream);
 int pgs_multi_shade(const pgs_shade_args *args, const uint32_t *prefix, uint32_t n, uint32_t total, uint32_t rate_x,
                     uint32_t rate_y, uint32_t waves, hipStream_t stream);
-// Shading launches of different sampling grids (1x, 2x, 4x, 8x, 16x) in one multi-launch.
+// Shading launches of different sampling grids in one multi-launch: all_grids false allows only
+// 1x, 2x and 4x launches, true also 8x and 16x.
 int pgs_multi_shade_mixed(const pgs_shade_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
-                          uint32_t waves, hipStream_t stream);
+                          uint32_t waves, bool all_grids, hipStream_t stream);
 // Resident workgroups the work-queue shading launch of these rates uses (0: unsupported).
 uint32_t pgs_queue_slots(uint32_t rate_x, uint32_t rate_y, uint32_t waves_per_group, uint32_t total);
 int pgs_multi_clut_write(const pgs_clut_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
