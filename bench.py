@@ -27,7 +27,7 @@ KIND_NAMES = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade"
 # 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), and the kernel's waves and VALU
 # instructions per wave from an SQ_WAVES / SQ_INSTS_VALU run; the runs are summarised in
 # profiles/hbm_traffic_pmc__plain_and_textured4x.md.
-MEASURED_TRAFFIC = {"plain": 6915392, "textured": 20951962}
+MEASURED_TRAFFIC = {"plain": 6915392, "textured": 20951962, "batched": 1537190605}
 MEASURED_VALU = {"plain": (4480, 10676), "textured": (17920, 2011)}  # (waves, VALU instructions per wave)
 # MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a SIMD for 4 cycles at 2.4 GHz.
 VALU_ISSUE_S_PER_WAVE_INST = 4.0 / 1024 / 2.4e9
@@ -336,8 +336,9 @@ def bench_batched_common(args, specs, workload, seed, cpu_contexts=2):
                      "table_ms": [[t, w, round(m, 4)] for t, w, m in table]},
         "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "kernel": "k_shade_mixed_multi" if k_n == 1 else "k_shade_*_multi",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": MEASURED_TRAFFIC["batched"] if k_n == 1 else None,
+                     "kernel": "k_shade_mixed_low_multi" if k_n == 1 else "k_shade_*_multi",
                      "launches": k_n,
                      "note": "VALU-issue bound; byte roofline fraction is small by construction"},
         "cpu_baseline": {"value": round(cpu_pix / cpu_s / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
