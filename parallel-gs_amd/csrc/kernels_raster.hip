@@ -465,14 +465,17 @@ __device__ __forceinline__ void coarse_mask_wave(const pgs_setup_args &args, uin
 
 // LDS of a setup workgroup of up to kSetupMaxWaves waves: each wave's coarse-mask ballots.
 constexpr uint32_t kSetupMaxWaves = 4;
-struct SetupLds
+// Per-wave coarse-mask scratch of a setup workgroup of up to W waves.
+template <uint32_t W>
+struct SetupLdsW
 {
-	uint64_t col[kSetupMaxWaves][kMaxCoarseSide], row[kSetupMaxWaves][kMaxCoarseSide];
+	uint64_t col[W][kMaxCoarseSide], row[W][kMaxCoarseSide];
 };
+using SetupLds = SetupLdsW<kSetupMaxWaves>;
 
-// One thread per primitive, any workgroup size of whole waves (at most kSetupMaxWaves).
-__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t nwg,
-                                                    SetupLds &lds)
+// One thread per primitive, any workgroup size of whole waves (at most the LDS type's W).
+template <typename Lds>
+__device__ __forceinline__ void triangle_setup_body(const pgs_setup_args &args, uint32_t wg, uint32_t nwg, Lds &lds)
 {
 	const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6;
 	const uint32_t word = wg * waves + wave;
@@ -817,9 +820,11 @@ __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t 
 }
 
 // The bodies with their own LDS, for their own launches.
+// Setup's own launches are one wave per workgroup: 4 KiB of LDS each (the 4-wave layout's
+// 16 KiB held a CU to 10 of them).
 __device__ __forceinline__ void setup_own_lds(const pgs_setup_args &a, uint32_t wg, uint32_t nwg)
 {
-	__shared__ SetupLds lds;
+	__shared__ SetupLdsW<1> lds;
 	triangle_setup_body(a, wg, nwg, lds);
 }
 __device__ __forceinline__ void heuristic_own_lds(const pgs_ssaa_heuristic_args &a, uint32_t wg, uint32_t nwg)
