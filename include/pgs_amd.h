@@ -32,7 +32,17 @@ typedef struct pgs_options
 	 * setup, the sampling heuristic and binning, with event waits between the streams; 0: one
 	 * stream. Never changes a pixel. */
 	uint32_t texture_stream;
+	/* GSOptions::dynamic_super_sampling (gs/gs_interface.hpp:204): 1 allocates planes for 16
+	 * samples so that pgs_set_super_sampling_rate can raise the rate later; 0 allocates for
+	 * `super_sampling` only. */
+	uint32_t dynamic_super_sampling;
 } pgs_options;
+
+/* Error codes beyond negative hipError_t values. */
+#define PGS_ERROR_INVALID_ARGUMENT (-1)
+/* A request the reference accepts but this backend does not implement (the call changes
+ * nothing). */
+#define PGS_ERROR_UNSUPPORTED (-1000)
 
 /* FlushStats (gs/gs_renderer.hpp:55-65), extended with the backend's own counters. */
 typedef struct pgs_flush_stats
@@ -81,6 +91,124 @@ int pgs_clobber_register_state(pgs_context *ctx);
 
 /* GSInterface::consume_flush_stats (gs/gs_interface.cpp:4697-4700). */
 int pgs_consume_flush_stats(pgs_context *ctx, pgs_flush_stats *stats);
+
+/* ---- Register state, GIF paths, options (gs/gs_interface.hpp:251-305). ---- */
+
+/* ContextState (gs/gs_interface.hpp:20-34): the per-context registers as 64-bit GS words. */
+typedef struct pgs_context_registers
+{
+	uint64_t tex0, tex1, clamp, xyoffset, miptbp1, miptbp2, scissor, alpha, test, fba, frame, zbuf;
+} pgs_context_registers;
+
+/* RegisterState (gs/gs_interface.hpp:36-61) with every register as its 64-bit GS word. */
+typedef struct pgs_register_state
+{
+	pgs_context_registers ctx[2];
+	uint64_t prim, rgbaq, st, uv, fog, prmodecont, texclut, texa, fogcol, dimx, dthe, colclamp, pabe;
+	uint64_t bitbltbuf, trxpos, trxreg, trxdir, scanmsk;
+	float internal_q;
+	uint32_t cached_cbp[2];
+	uint32_t pad;
+} pgs_register_state;
+
+/* GSInterface::get_register_state (gs/gs_interface.hpp:289-290). Writing the state back
+ * (a save-state restore) takes effect like the reference's assignment through the returned
+ * reference: no register side effects run; call pgs_clobber_register_state afterwards, as the
+ * reference's callers do. */
+int pgs_get_register_state(pgs_context *ctx, pgs_register_state *out);
+int pgs_set_register_state(pgs_context *ctx, const pgs_register_state *in);
+
+/* GIFPath (gs/gs_interface.hpp:63-68): the current GIFtag as two 64-bit words and the
+ * position inside it. */
+typedef struct pgs_gif_path
+{
+	uint64_t tag[2];
+	uint32_t reg;
+	uint32_t loop;
+} pgs_gif_path;
+
+/* GSInterface::get_gif_path (gs/gs_interface.hpp:295-296). path < 4. */
+int pgs_get_gif_path(pgs_context *ctx, uint32_t path, pgs_gif_path *out);
+int pgs_set_gif_path(pgs_context *ctx, uint32_t path, const pgs_gif_path *in);
+
+/* PrivRegisterState (gs/gs_interface.hpp:70-111): the privileged (CRTC) registers, as the
+ * 8 KiB block the EE sees them in: PMODE at byte 0x0000, SMODE1 0x0010, SMODE2 0x0020, SRFSH
+ * 0x0030, SYNCH1 0x0040, SYNCH2 0x0050, SYNCV 0x0060, DISPFB1 0x0070, DISPLAY1 0x0080, DISPFB2
+ * 0x0090, DISPLAY2 0x00a0, EXTBUF 0x00b0, EXTDATA 0x00c0, EXTWRITE 0x00d0, BGCOLOR 0x00e0, CSR
+ * 0x1000, IMR 0x1010, BUSDIR 0x1040, SIGLBLID 0x1080 (GSInterface::get_priv_register_state,
+ * gs/gs_interface.hpp:292-293). */
+#define PGS_PRIV_REGISTER_BYTES 8192u
+int pgs_get_priv_register_state(pgs_context *ctx, void *out, size_t size);
+int pgs_set_priv_register_state(pgs_context *ctx, const void *in, size_t size);
+
+/* GSInterface::set_super_sampling_rate (gs/gs_interface.hpp:258, gs/gs_interface.cpp:65-118).
+ * Submits pending work, then switches the sample count (1, 2, 4, 8, 16) and grid, clearing
+ * every super-sample plane. The rate may not exceed what the context allocated (its
+ * `super_sampling`, or 16 with `dynamic_super_sampling`). Returns PGS_ERROR_UNSUPPORTED for
+ * super-sampled textures and for the sparse 4x / 16x grids (ordered_grid = 0), which this
+ * backend does not build, and PGS_ERROR_INVALID_ARGUMENT for other rates; the rate is then
+ * unchanged. Not allowed while recording. */
+int pgs_set_super_sampling_rate(pgs_context *ctx, uint32_t super_sampling, uint32_t ordered_grid,
+                                uint32_t super_sampled_textures);
+/* The current sample count (1, 2, 4, 8 or 16). */
+uint32_t pgs_get_super_sampling_rate(pgs_context *ctx);
+
+/* Hacks (gs/gs_interface.hpp:211-240). disable_mipmaps is accepted (this backend samples
+ * LOD 0 only); unsynced_readbacks makes LOCAL->HOST transfers return zeros without waiting
+ * for the GPU (the reference does so when the data is not ready); backbuffer_promotion and
+ * allow_blend_demote are not built: asking for them returns PGS_ERROR_UNSUPPORTED and the
+ * other fields still apply. */
+typedef struct pgs_hacks
+{
+	uint32_t disable_mipmaps;
+	uint32_t unsynced_readbacks;
+	uint32_t backbuffer_promotion;
+	uint32_t allow_blend_demote;
+} pgs_hacks;
+int pgs_set_hacks(pgs_context *ctx, const pgs_hacks *hacks);
+
+/* DebugMode (gs/gs_interface.hpp:137-153). timestamps enables the per-kernel timers read by
+ * pgs_get_accumulated_timestamps; deterministic_timeline_query is accepted (every host access
+ * here waits for the GPU, which is what it asks for); disable_sampler_feedback turns sparse
+ * texture decoding off. feedback_render_target and a draw_mode other than 0 (debug captures
+ * of every draw) are not built: PGS_ERROR_UNSUPPORTED, the other fields still apply. */
+typedef struct pgs_debug_mode
+{
+	uint32_t feedback_render_target;
+	uint32_t timestamps;
+	uint32_t deterministic_timeline_query;
+	uint32_t disable_sampler_feedback;
+	uint32_t draw_mode;
+} pgs_debug_mode;
+int pgs_set_debug_mode(pgs_context *ctx, const pgs_debug_mode *mode);
+
+/* SignalInterface (gs/gs_interface.hpp:242-250): called on SIGNAL, FINISH and LABEL register
+ * writes with the written value; a nonzero return submits pending work (GSInterface::flush).
+ * NULL callbacks are skipped. The pointer is copied; `user` is passed back. */
+typedef struct pgs_signal_interface
+{
+	int (*on_signal)(void *user, uint64_t payload);
+	int (*on_finish)(void *user, uint64_t payload);
+	int (*on_label)(void *user, uint64_t payload);
+	void *user;
+} pgs_signal_interface;
+int pgs_set_signal_interface(pgs_context *ctx, const pgs_signal_interface *iface);
+
+/* TimestampType (gs/gs_renderer.hpp:67-79). */
+#define PGS_TIMESTAMP_SYNC_HOST_TO_VRAM 0u
+#define PGS_TIMESTAMP_COPY_VRAM 1u
+#define PGS_TIMESTAMP_PALETTE_UPDATE 2u
+#define PGS_TIMESTAMP_TEXTURE_UPLOAD 3u
+#define PGS_TIMESTAMP_TRIANGLE_SETUP 4u
+#define PGS_TIMESTAMP_BINNING 5u
+#define PGS_TIMESTAMP_SHADING 6u
+#define PGS_TIMESTAMP_READBACK 7u
+#define PGS_TIMESTAMP_VSYNC 8u
+#define PGS_TIMESTAMP_COUNT 9u
+/* GSInterface::get_accumulated_timestamps (gs/gs_interface.cpp:4702-4705): seconds of GPU time
+ * spent in the kernels of `type` since timestamps were enabled (pgs_set_debug_mode), measured
+ * with HIP events around each launch. Negative on error. */
+double pgs_get_accumulated_timestamps(pgs_context *ctx, uint32_t type);
 
 /* Number of 4 MiB VRAM planes: 1 at 1x, 2 + samples with super-sampling
  * (plane layout of GSRenderer::init_vram, gs/gs_renderer.cpp:360-427). */

@@ -234,12 +234,24 @@ GS::GS(uint32_t ssaa_log2)
 void GS::reset_registers()
 {
 	memset(&regs, 0, sizeof(regs));
-	regs.rgbaq_q = 1.0f;
+	regs.rgbaq_q = 0.0f; // RGBAQ is zero after reset (RegisterState{}, gs/gs_interface.hpp:36-61)
 	internal_q = 1.0f;
 	for (auto &p : paths)
 		p = {};
 	vq_count = 0;
 	xfer = {};
+}
+
+void GS::reset_context_state()
+{
+	flush();
+	memset(&regs, 0, sizeof(regs));
+	internal_q = 1.0f;
+	for (auto &p : paths)
+		p = {};
+	vq_count = 0;
+	cached_cbp[0] = cached_cbp[1] = 0;
+	state_dirty = true;
 }
 
 void GS::set_reg(uint64_t &reg, uint64_t v, bool marks_dirty)
@@ -259,7 +271,18 @@ void GS::write_register(uint32_t addr, uint64_t v)
 {
 	switch (addr)
 	{
-	case 0x00: set_reg(regs.prim, v & 0x7ffu, true); vq_count = 0; break;
+	case 0x00:
+		// gs/gs_interface.cpp:3507-3556: PRMODECONT.AC = 0 keeps every PRIM bit but the type.
+		if (regs.prmodecont & 1u)
+			set_reg(regs.prim, v & 0x7ffu, true);
+		else
+		{
+			if ((regs.prim & 7u) != (v & 7u) && ((v >> 7) & 1u))
+				state_dirty = true;
+			regs.prim = (regs.prim & ~7ull) | (v & 7u);
+		}
+		vq_count = 0;
+		break;
 	case 0x01: case 0x11:
 		regs.rgbaq = uint32_t(v);
 		{ uint32_t q = uint32_t(v >> 32); memcpy(&regs.rgbaq_q, &q, 4); }
@@ -277,11 +300,16 @@ void GS::write_register(uint32_t addr, uint64_t v)
 	case 0x16: case 0x17: write_tex2(addr - 0x16, v); break;
 	case 0x18: case 0x19: regs.xyoffset[addr - 0x18] = v; break;
 	case 0x1a: regs.prmodecont = v; break;
-	case 0x1b: regs.prmode = v; break;
+	case 0x1b:
+		// gs/gs_interface.cpp:3692-3722: PRMODE supplies PRIM's other bits while AC = 0.
+		regs.prmode = v;
+		if (!(regs.prmodecont & 1u))
+			set_reg(regs.prim, (v & 0x7f8u) | (regs.prim & 7u), true);
+		break;
 	case 0x1c: regs.texclut = v; break;
-	case 0x22: regs.scanmsk = v; break;
-	case 0x34: case 0x35: regs.miptbp1[addr - 0x34] = v; break;
-	case 0x36: case 0x37: regs.miptbp2[addr - 0x36] = v; break;
+	case 0x22: set_reg(regs.scanmsk, v, true); break;
+	case 0x34: case 0x35: set_reg(regs.miptbp1[addr - 0x34], v, true); break;
+	case 0x36: case 0x37: set_reg(regs.miptbp2[addr - 0x36], v, true); break;
 	case 0x3b: set_reg(regs.texa, v, true); break;
 	case 0x3d: regs.fogcol = v; break;
 	case 0x40: case 0x41: regs.scissor[addr - 0x40] = v; break;
@@ -328,6 +356,11 @@ void GS::init_transfer()
 		flush_render_pass();
 		local_copy(c);
 	}
+	else if (xdir == 1)
+	{
+		flush_render_pass();
+		local_to_host(c);
+	}
 	else if (xdir == 0)
 	{
 		xfer.copy = c;
@@ -336,6 +369,64 @@ void GS::init_transfer()
 		xfer.payload.clear();
 		xfer.written_pixels = 0;
 	}
+}
+
+// Local->host readback into the FIFO, pixels in raster order packed at the source PSM's
+// bits per pixel; 24-bit formats store 3 bytes per pixel; 4-bit formats are not read back
+// (their FIFO stays zero) (gs/gs_util.hpp:49-110 vram_readback; gs/gs_interface.cpp:3213-3275).
+void GS::local_to_host(const Copy &c)
+{
+	const uint32_t bytes = uint32_t((uint64_t(c.w) * c.h * bits_per_pixel(c.spsm)) / 8u);
+	fifo.assign(bytes, 0);
+	fifo_offset_128 = 0;
+	const uint8_t *v = vram.data();
+	for (uint32_t y = 0; y < c.h; y++)
+		for (uint32_t x = 0; x < c.w; x++)
+		{
+			const size_t i = size_t(y) * c.w + x;
+			const uint32_t px = (x + c.ssax) & 2047u, py = (y + c.ssay) & 2047u;
+			switch (c.spsm)
+			{
+			case CT32: case Z32:
+			{
+				uint32_t d = read_local_pixel(v, c.spsm, px, py, c.sbp, c.sbw);
+				memcpy(&fifo[i * 4], &d, 4);
+				break;
+			}
+			case CT24: case Z24:
+			{
+				const uint32_t d = read_local_pixel(v, c.spsm, px, py, c.sbp, c.sbw); // whole word
+				if (i * 3 + 3 <= bytes)
+					for (uint32_t k = 0; k < 3; k++)
+						fifo[i * 3 + k] = uint8_t(d >> (8 * k));
+				break;
+			}
+			case CT16: case CT16S: case Z16: case Z16S:
+			{
+				uint32_t d = read_local_pixel(v, c.spsm, px, py, c.sbp, c.sbw);
+				fifo[i * 2] = uint8_t(d);
+				fifo[i * 2 + 1] = uint8_t(d >> 8);
+				break;
+			}
+			case T8: case T8H:
+				fifo[i] = uint8_t(read_local_pixel(v, c.spsm, px, py, c.sbp, c.sbw));
+				break;
+			default:
+				break;
+			}
+		}
+}
+
+void GS::read_transfer_fifo(void *data, uint32_t n)
+{
+	const uint32_t size_128 = uint32_t(fifo.size() / 16u);
+	const uint32_t avail = size_128 - std::min(size_128, fifo_offset_128);
+	const uint32_t copy = std::min(n, avail);
+	if (copy)
+		memcpy(data, fifo.data() + 16u * fifo_offset_128, copy * 16u);
+	if (copy < n)
+		memset(static_cast<uint8_t *>(data) + 16u * copy, 0, (n - copy) * 16u);
+	fifo_offset_128 += copy;
 }
 
 // Local->local: the whole source rectangle is read before any destination pixel is
@@ -571,6 +662,38 @@ uint32_t orc_swizzle(uint32_t x, uint32_t y, uint32_t base, uint32_t stride, uin
 	return orc::swizzle(x, y, base, stride, psm);
 }
 uint64_t orc_stat(void *g, uint32_t which) { return static_cast<orc::GS *>(g)->stat(which); }
+// GSInterface::reset_context_state (gs/gs_interface.cpp:26-39): pending work is flushed (a
+// host->local transfer keeps what arrived and stays active), then the register file, GIF paths,
+// vertex queue and cached CLUT base pointers are reset.
+void orc_reset_context_state(void *g) { static_cast<orc::GS *>(g)->reset_context_state(); }
+void orc_read_transfer_fifo(void *g, void *data, uint32_t n) { static_cast<orc::GS *>(g)->read_transfer_fifo(data, n); }
+// Host VRAM write (gs/gs_interface.cpp:4184-4228): the bytes land in plane 0 after all earlier
+// work; every 8 KiB page the range touches loses its super-samples (reference plane cleared,
+// gs/gs_renderer.cpp:1478-1492).
+void orc_host_write(void *g, size_t offset, const void *data, size_t size)
+{
+	auto *gs = static_cast<orc::GS *>(g);
+	if (!size || offset + size > orc::VRAM_SIZE)
+		return;
+	gs->flush();
+	memcpy(gs->vram.data() + offset, data, size);
+	if (gs->sample_log2)
+	{
+		const size_t first = offset / 8192u * 8192u, last = ((offset + size - 1u) / 8192u + 1u) * 8192u;
+		memset(gs->vram.data() + orc::VRAM_SIZE + first, 0, last - first);
+	}
+}
+// GSInterface::set_super_sampling_rate (gs/gs_interface.cpp:65-118): pending work is done at the
+// old rate, then every plane above plane 0 is cleared (gs/gs_renderer.cpp:315-342).
+void orc_set_sample_log2(void *g, uint32_t sample_log2)
+{
+	auto *gs = static_cast<orc::GS *>(g);
+	gs->flush();
+	gs->sample_log2 = sample_log2;
+	const size_t planes = sample_log2 ? 2u + (1u << sample_log2) : 1u;
+	gs->vram.resize(planes * orc::VRAM_SIZE);
+	std::fill(gs->vram.begin() + orc::VRAM_SIZE, gs->vram.end(), uint8_t(0));
+}
 }
 
 extern "C"

@@ -88,6 +88,7 @@ public:
 	void gif_transfer(uint32_t path, const void *data, size_t size);
 	void write_register(uint32_t addr, uint64_t v);
 	void flush();
+	void reset_context_state();
 	uint64_t stat(uint32_t which) const;
 
 	std::vector<uint8_t> vram;   // plane 0 = VRAM; at SSAA: plane 1 = reference, 2.. = samples
@@ -96,6 +97,11 @@ public:
 
 	void (*xfer_hook)(void *, uint32_t dir, uint32_t w, uint32_t h, uint32_t psm) = nullptr;
 	void *xfer_hook_user = nullptr;
+
+	// LOCAL->HOST transfer FIFO (gs/gs_interface.cpp:3150-3166, 3213-3275).
+	std::vector<uint8_t> fifo;
+	uint32_t fifo_offset_128 = 0;
+	void read_transfer_fifo(void *data, uint32_t num_128b_words);
 
 private:
 	struct Path { uint32_t nloop = 0, loop = 0, flg = 0, nreg = 0, reg = 0; uint64_t regs = 0; };
@@ -135,6 +141,7 @@ private:
 	void packed_write(uint32_t addr, const uint64_t *w);
 	void init_transfer();
 	void local_copy(const Copy &c);
+	void local_to_host(const Copy &c);
 	void hwreg(const uint64_t *data, size_t count);
 	void write_host_pixels();
 	void finish_host_transfer();

@@ -5,6 +5,7 @@
 #include "batch.hpp"
 #include "cost_model.hpp"
 #include <vector>
+#include <cstring>
 #include <exception>
 #include <cstdio>
 #include <new>
@@ -86,16 +87,11 @@ pgs_context *pgs_create(const pgs_options *options)
 	pgs::GSOptions o;
 	if (options)
 	{
-		switch (options->super_sampling)
-		{
-		case 2: o.super_sampling = pgs::SuperSampling::X2; break;
-		case 4: o.super_sampling = pgs::SuperSampling::X4; break;
-		case 8: o.super_sampling = pgs::SuperSampling::X8; break;
-		case 16: o.super_sampling = pgs::SuperSampling::X16; break;
-		default: o.super_sampling = pgs::SuperSampling::X1; break;
-		}
+		// Sample counts other than 1/2/4/8/16 and super-sampled textures make init fail.
+		o.super_sampling = pgs::SuperSampling(options->super_sampling ? options->super_sampling : 1u);
 		o.ordered_super_sampling = options->ordered_super_sampling != 0;
 		o.super_sampled_textures = options->super_sampled_textures != 0;
+		o.dynamic_super_sampling = options->dynamic_super_sampling != 0;
 		if (options->upload_arena_bytes)
 			o.upload_arena_bytes = options->upload_arena_bytes;
 		if (options->sampler_feedback > 2u)
@@ -191,6 +187,177 @@ int pgs_consume_flush_stats(pgs_context *ctx, pgs_flush_stats *out)
 			out->texture_cache_hits = s.texture_cache_hits;
 		}
 	});
+}
+
+int pgs_get_register_state(pgs_context *ctx, pgs_register_state *out)
+{
+	if (!out)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return guard(ctx, [&] {
+		const pgs::RegisterState &r = ctx->gs.get_register_state();
+		pgs_register_state s{};
+		for (uint32_t c = 0; c < 2; c++)
+		{
+			pgs_context_registers &o = s.ctx[c];
+			o.tex0 = r.tex0[c]; o.tex1 = r.tex1[c]; o.clamp = r.clamp[c]; o.xyoffset = r.xyoffset[c];
+			o.miptbp1 = r.miptbp1[c]; o.miptbp2 = r.miptbp2[c]; o.scissor = r.scissor[c]; o.alpha = r.alpha[c];
+			o.test = r.test[c]; o.fba = r.fba[c]; o.frame = r.frame[c]; o.zbuf = r.zbuf[c];
+		}
+		s.prim = r.prim; s.rgbaq = r.rgbaq; s.st = r.st; s.uv = r.uv; s.fog = r.fog; s.prmodecont = r.prmodecont;
+		s.texclut = r.texclut; s.texa = r.texa; s.fogcol = r.fogcol; s.dimx = r.dimx; s.dthe = r.dthe;
+		s.colclamp = r.colclamp; s.pabe = r.pabe; s.bitbltbuf = r.bitbltbuf; s.trxpos = r.trxpos;
+		s.trxreg = r.trxreg; s.trxdir = r.trxdir; s.scanmsk = r.scanmsk;
+		s.internal_q = r.internal_q;
+		s.cached_cbp[0] = ctx->gs.cached_cbp()[0];
+		s.cached_cbp[1] = ctx->gs.cached_cbp()[1];
+		*out = s;
+	});
+}
+
+int pgs_set_register_state(pgs_context *ctx, const pgs_register_state *in)
+{
+	if (!in)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return guard(ctx, [&] {
+		pgs::RegisterState &r = ctx->gs.get_register_state();
+		for (uint32_t c = 0; c < 2; c++)
+		{
+			const pgs_context_registers &o = in->ctx[c];
+			r.tex0[c] = o.tex0; r.tex1[c] = o.tex1; r.clamp[c] = o.clamp; r.xyoffset[c] = o.xyoffset;
+			r.miptbp1[c] = o.miptbp1; r.miptbp2[c] = o.miptbp2; r.scissor[c] = o.scissor; r.alpha[c] = o.alpha;
+			r.test[c] = o.test; r.fba[c] = o.fba; r.frame[c] = o.frame; r.zbuf[c] = o.zbuf;
+		}
+		r.prim = in->prim; r.rgbaq = in->rgbaq; r.st = in->st; r.uv = in->uv; r.fog = in->fog;
+		r.prmodecont = in->prmodecont; r.texclut = in->texclut; r.texa = in->texa; r.fogcol = in->fogcol;
+		r.dimx = in->dimx; r.dthe = in->dthe; r.colclamp = in->colclamp; r.pabe = in->pabe;
+		r.bitbltbuf = in->bitbltbuf; r.trxpos = in->trxpos; r.trxreg = in->trxreg; r.trxdir = in->trxdir;
+		r.scanmsk = in->scanmsk;
+		r.internal_q = in->internal_q;
+		ctx->gs.cached_cbp()[0] = in->cached_cbp[0];
+		ctx->gs.cached_cbp()[1] = in->cached_cbp[1];
+	});
+}
+
+int pgs_get_gif_path(pgs_context *ctx, uint32_t path, pgs_gif_path *out)
+{
+	if (!out || path >= 4)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return guard(ctx, [&] {
+		const pgs::GIFPath &p = ctx->gs.get_gif_path(path);
+		out->tag[0] = p.tag_lo;
+		out->tag[1] = p.regs;
+		out->reg = p.reg;
+		out->loop = p.loop;
+	});
+}
+
+int pgs_set_gif_path(pgs_context *ctx, uint32_t path, const pgs_gif_path *in)
+{
+	if (!in || path >= 4)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return guard(ctx, [&] {
+		pgs::GIFPath &p = ctx->gs.get_gif_path(path);
+		p.tag_lo = in->tag[0];
+		p.nloop = uint32_t(in->tag[0] & 0x7fff);
+		p.flg = uint32_t((in->tag[0] >> 58) & 3u);
+		p.nreg = uint32_t((in->tag[0] >> 60) & 0xf);
+		p.regs = in->tag[1];
+		p.reg = in->reg;
+		p.loop = in->loop;
+	});
+}
+
+int pgs_get_priv_register_state(pgs_context *ctx, void *out, size_t size)
+{
+	if (!out || size > PGS_PRIV_REGISTER_BYTES)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return guard(ctx, [&] { memcpy(out, ctx->gs.get_priv_register_state(), size); });
+}
+
+int pgs_set_priv_register_state(pgs_context *ctx, const void *in, size_t size)
+{
+	if (!in || size > PGS_PRIV_REGISTER_BYTES)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return guard(ctx, [&] { memcpy(ctx->gs.get_priv_register_state(), in, size); });
+}
+
+int pgs_set_super_sampling_rate(pgs_context *ctx, uint32_t super_sampling, uint32_t ordered_grid,
+                                uint32_t super_sampled_textures)
+{
+	if (!ctx)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	const bool known = super_sampling == 1 || super_sampling == 2 || super_sampling == 4 || super_sampling == 8 ||
+	                   super_sampling == 16;
+	if (!known)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	if (super_sampled_textures && super_sampling != 1)
+		return PGS_ERROR_UNSUPPORTED;
+	if (!ordered_grid && (super_sampling == 4 || super_sampling == 16))
+		return PGS_ERROR_UNSUPPORTED;
+	bool ok = false;
+	const int rc = guard(ctx, [&] {
+		ok = ctx->gs.set_super_sampling_rate(pgs::SuperSampling(super_sampling), ordered_grid != 0,
+		                                     super_sampled_textures != 0);
+	});
+	if (rc != 0)
+		return rc;
+	return ok ? 0 : PGS_ERROR_INVALID_ARGUMENT;
+}
+
+uint32_t pgs_get_super_sampling_rate(pgs_context *ctx) { return ctx ? ctx->gs.super_sampling_rate() : 0; }
+
+int pgs_set_hacks(pgs_context *ctx, const pgs_hacks *hacks)
+{
+	if (!hacks)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	const int rc = guard(ctx, [&] {
+		pgs::Hacks h;
+		h.disable_mipmaps = hacks->disable_mipmaps != 0;
+		h.unsynced_readbacks = hacks->unsynced_readbacks != 0;
+		ctx->gs.set_hacks(h);
+	});
+	if (rc != 0)
+		return rc;
+	return (hacks->backbuffer_promotion || hacks->allow_blend_demote) ? PGS_ERROR_UNSUPPORTED : 0;
+}
+
+int pgs_set_debug_mode(pgs_context *ctx, const pgs_debug_mode *mode)
+{
+	if (!mode)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	const int rc = guard(ctx, [&] {
+		ctx->gs.set_timestamps(mode->timestamps != 0);
+		if (mode->disable_sampler_feedback)
+			ctx->gs.set_sampler_feedback(2u);
+	});
+	if (rc != 0)
+		return rc;
+	return (mode->feedback_render_target || mode->draw_mode) ? PGS_ERROR_UNSUPPORTED : 0;
+}
+
+int pgs_set_signal_interface(pgs_context *ctx, const pgs_signal_interface *iface)
+{
+	return guard(ctx, [&] {
+		pgs::SignalCallbacks cb;
+		if (iface)
+		{
+			cb.on_signal = iface->on_signal;
+			cb.on_finish = iface->on_finish;
+			cb.on_label = iface->on_label;
+			cb.user = iface->user;
+		}
+		ctx->gs.set_signal_interface(cb);
+	});
+}
+
+double pgs_get_accumulated_timestamps(pgs_context *ctx, uint32_t type)
+{
+	if (!ctx || type >= PGS_TIMESTAMP_COUNT)
+		return -1.0;
+	double v = -1.0;
+	if (guard(ctx, [&] { v = ctx->gs.get_accumulated_timestamps(pgs::TimestampType(type)); }) != 0)
+		return -1.0;
+	return v;
 }
 
 uint32_t pgs_num_planes(pgs_context *ctx) { return ctx ? ctx->gs.renderer().num_planes() : 0; }

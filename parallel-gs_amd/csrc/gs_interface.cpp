@@ -2,41 +2,127 @@
 #include "gs_interface.hpp"
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 
 namespace pgs
 {
+namespace
+{
+// log2 of a sample count 1/2/4/8/16, or -1 for any other value.
+int sample_count_log2(uint32_t s)
+{
+	switch (s)
+	{
+	case 1: return 0;
+	case 2: return 1;
+	case 4: return 2;
+	case 8: return 3;
+	case 16: return 4;
+	default: return -1;
+	}
+}
+}
+
 bool GSInterface::init(const GSOptions &options)
 {
 	options_ = options;
-	uint32_t s = uint32_t(options.super_sampling);
-	uint32_t sample_log2 = s >= 16 ? 4 : s >= 8 ? 3 : s >= 4 ? 2 : s >= 2 ? 1 : 0;
+	const int l = sample_count_log2(uint32_t(options.super_sampling));
+	if (l < 0)
+		return false;
+	const uint32_t sample_log2 = uint32_t(l);
 	// Sampling grids (gs/gs_interface.cpp:65-117): 2x = 1x2, 4x = 2x2, 8x = 2x4, 16x = 4x4.
-	// The sparse 4x (1x4) and 16x (2x8) grids of the reference are not provided.
+	// The sparse 4x (1x4) and 16x (2x8) grids and super-sampled textures (texture arrays per
+	// sample, ubershader.comp:1155-1212) are not built: asking for them fails.
 	if ((sample_log2 == 2 || sample_log2 == 4) && !options.ordered_super_sampling)
 		return false;
-	if (!renderer_.init(sample_log2, options.upload_arena_bytes, options.sampler_feedback, options.texture_stream))
+	if (options.super_sampled_textures && sample_log2 != 0)
+		return false;
+	const uint32_t capacity_log2 = options.dynamic_super_sampling ? 4u : sample_log2;
+	if (!renderer_.init(sample_log2, options.upload_arena_bytes, options.sampler_feedback, options.texture_stream,
+	                    capacity_log2))
 		return false;
 	rp_.init(&renderer_, sample_log2);
 	rp_.host = this;
-	reset_context_state();
+	reset_registers();
 	return true;
 }
 
-// gs/gs_interface.cpp:26-39
+// gs/gs_interface.cpp:26-39: pending work is submitted first (a host->local transfer keeps
+// what arrived and stays active, as the reference's flush leaves it), then the register file,
+// GIF paths, vertex queue and cached CLUT base pointers are reset.
 void GSInterface::reset_context_state()
+{
+	flush();
+	reset_registers();
+}
+
+void GSInterface::reset_registers()
 {
 	regs_ = {};
 	for (auto &p : paths_)
 		p = {};
-	xfer_.host_to_local_active = false;
-	xfer_.payload.clear();
+	cached_cbp_[0] = cached_cbp_[1] = 0;
 	rp_.reset_vertex_queue();
+	rp_.mark_state_dirty();
 }
 
 void GSInterface::clobber_register_state()
 {
 	flush();
 	rp_.reset_vertex_queue();
+}
+
+bool GSInterface::set_super_sampling_rate(SuperSampling rate, bool ordered_grid, bool super_sampled_textures)
+{
+	const int l = sample_count_log2(uint32_t(rate));
+	if (l < 0)
+		return false;
+	const uint32_t sample_log2 = uint32_t(l);
+	if ((sample_log2 == 2 || sample_log2 == 4) && !ordered_grid)
+		return false;
+	// At 1x the reference turns super-sampled textures off (gs/gs_interface.cpp:74-78).
+	if (super_sampled_textures && sample_log2 != 0)
+		return false;
+	if (sample_log2 > renderer_.capacity_log2())
+		return false;
+	flush();
+	if (!renderer_.set_sample_log2(sample_log2))
+		return false;
+	rp_.set_sample_log2(sample_log2);
+	options_.super_sampling = rate;
+	options_.ordered_super_sampling = ordered_grid;
+	return true;
+}
+
+// Kernel kinds of the renderer's launch timers (Renderer::LaunchKind) by TimestampType.
+void GSInterface::poll_timestamps(bool wait)
+{
+	double ms[16] = {};
+	renderer_.poll_timings(ms, 16, wait);
+	auto add = [&](TimestampType t, double v) { timestamp_ms_[uint32_t(t)] += v; };
+	add(TimestampType::CopyVRAM, ms[0] + ms[1]);
+	add(TimestampType::TriangleSetup, ms[2] + ms[7]);
+	add(TimestampType::Binning, ms[3]);
+	add(TimestampType::Shading, ms[4] + ms[8]);
+	add(TimestampType::PaletteUpdate, ms[5]);
+	add(TimestampType::TextureUpload, ms[6] + ms[9]);
+}
+
+void GSInterface::set_timestamps(bool enable)
+{
+	if (timestamps_ && !enable)
+		poll_timestamps(true);
+	timestamps_ = enable;
+	renderer_.profile_enable(enable);
+}
+
+double GSInterface::get_accumulated_timestamps(TimestampType type)
+{
+	if (uint32_t(type) >= uint32_t(TimestampType::Count))
+		return -1.0;
+	if (timestamps_)
+		poll_timestamps(true);
+	return timestamp_ms_[uint32_t(type)] * 1e-3;
 }
 
 // Transfers and draws execute in submission order on one stream. The host keeps at most one
@@ -217,7 +303,7 @@ void GSInterface::write_register(uint32_t addr, uint64_t v)
 {
 	switch (addr)
 	{
-	case 0x00: set_state_reg(regs_.prim, v & 0x7ffu); rp_.reset_vertex_queue(); break;
+	case 0x00: write_prim(v); break;
 	case 0x01: case 0x11: regs_.rgbaq = v; break;
 	case 0x02: regs_.st = v; break;
 	case 0x03: regs_.uv = v; break;
@@ -243,13 +329,15 @@ void GSInterface::write_register(uint32_t addr, uint64_t v)
 	}
 	case 0x18: case 0x19: regs_.xyoffset[addr - 0x18] = v; break;
 	case 0x1a: regs_.prmodecont = v; break;
-	case 0x1b: regs_.prmode = v; break;
+	case 0x1b: write_prmode(v); break;
 	case 0x1c: regs_.texclut = v; break;
-	case 0x22: regs_.scanmsk = v; break;
-	case 0x34: case 0x35: regs_.miptbp1[addr - 0x34] = v; break;
-	case 0x36: case 0x37: regs_.miptbp2[addr - 0x36] = v; break;
+	case 0x22: set_state_reg(regs_.scanmsk, v); break;
+	case 0x34: case 0x35: set_state_reg(regs_.miptbp1[addr - 0x34], v); break;
+	case 0x36: case 0x37: set_state_reg(regs_.miptbp2[addr - 0x36], v); break;
 	case 0x3b: set_state_reg(regs_.texa, v); break;
 	case 0x3d: regs_.fogcol = v; break;
+	// SCISSOR does not block parallelogram fusion: the reference clears its dirty bit when the
+	// next primitive is clipped, before the fusion test (gs/gs_interface.cpp:2546, 2584).
 	case 0x40: case 0x41: regs_.scissor[addr - 0x40] = v; break;
 	case 0x42: case 0x43: set_state_reg(regs_.alpha[addr - 0x42], v); break;
 	case 0x44: set_state_reg(regs_.dimx, v); break;
@@ -265,8 +353,40 @@ void GSInterface::write_register(uint32_t addr, uint64_t v)
 	case 0x52: regs_.trxreg = v; break;
 	case 0x53: regs_.trxdir = v; init_transfer(); break;
 	case 0x54: hwreg(&v, 1); break;
+	// SIGNAL / FINISH / LABEL (gs/gs_interface.cpp:3970-3986): the caller's handler decides
+	// whether pending work is submitted.
+	case 0x60: if (signal_.on_signal && signal_.on_signal(signal_.user, v)) flush(); break;
+	case 0x61: if (signal_.on_finish && signal_.on_finish(signal_.user, v)) flush(); break;
+	case 0x62: if (signal_.on_label && signal_.on_label(signal_.user, v)) flush(); break;
 	default: break;
 	}
+}
+
+// PRIM (gs/gs_interface.cpp:3507-3556). With PRMODECONT.AC = 1 the whole register is written;
+// with AC = 0 only the primitive type changes and the other bits keep what PRMODE set. A type
+// change marks the state dirty only when the written AA1 bit is set (PRIM_TEMPLATE there).
+void GSInterface::write_prim(uint64_t v)
+{
+	v &= 0x7ffu;
+	if (regs_.prmodecont & 1u)
+		set_state_reg(regs_.prim, v);
+	else
+	{
+		if ((regs_.prim & 7u) != (v & 7u) && ((v >> 7) & 1u))
+			rp_.mark_state_dirty();
+		regs_.prim = (regs_.prim & ~7ull) | (v & 7u);
+	}
+	rp_.reset_vertex_queue();
+}
+
+// PRMODE (gs/gs_interface.cpp:3692-3722): with PRMODECONT.AC = 0 it supplies every PRIM bit but
+// the primitive type; with AC = 1 the write is ignored.
+void GSInterface::write_prmode(uint64_t v)
+{
+	regs_.prmode = v;
+	if (regs_.prmodecont & 1u)
+		return;
+	set_state_reg(regs_.prim, (v & 0x7f8u) | (regs_.prim & 7u));
 }
 
 // TEX0 write: texture state change plus the CLUT load its CLD field asks for.
@@ -373,14 +493,25 @@ void GSInterface::init_transfer()
 	}
 	else if (xdir == 1)
 	{
-		// Local->host readback into the transfer FIFO (gs/gs_interface.cpp:3213-3275).
+		// Local->host readback into the transfer FIFO (gs/gs_interface.cpp:3213-3275). The read
+		// waits for every earlier draw and transfer, as the reference's host-read timeline does;
+		// 4-bit formats are not read back (gs/gs_util.hpp:49-110) and leave the FIFO zero.
 		uint32_t bytes = uint32_t((uint64_t(c.w) * c.h * psm_bpp(c.spsm)) / 8u);
 		xfer_.fifo.assign(bytes, 0);
 		xfer_.fifo_offset_128 = 0;
 		xfer_.fifo_size_128 = bytes / 16u;
+		// Hacks::unsynced_readbacks: zeros instead of waiting for unfinished GPU work.
+		if (hacks_.unsynced_readbacks && (rp_.has_primitives() || renderer_.has_pending_transfers() || !renderer_.idle()))
+			return;
+		const auto t0 = std::chrono::steady_clock::now();
+		flush_render_pass();
+		renderer_.flush_transfers();
 		if (host_vram_.size() < kVramSize)
 			host_vram_.resize(kVramSize);
 		renderer_.read_vram(host_vram_.data(), 0, 1);
+		if (timestamps_)
+			timestamp_ms_[uint32_t(TimestampType::Readback)] +=
+			    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 		const uint8_t *v = host_vram_.data();
 		for (uint32_t y = 0; y < c.h; y++)
 			for (uint32_t x = 0; x < c.w; x++)
@@ -471,6 +602,8 @@ void GSInterface::flush()
 	flush_pending_transfer(true);
 	flush_render_pass();
 	renderer_.flush_transfers();
+	if (timestamps_)
+		poll_timestamps(false);
 }
 
 const void *GSInterface::map_vram_read(size_t offset, size_t size)
@@ -500,9 +633,18 @@ void *GSInterface::map_vram_write(size_t offset, size_t size)
 
 void GSInterface::end_vram_write(size_t offset, size_t size)
 {
-	if (offset + size > kVramSize)
+	if (offset + size > kVramSize || size == 0)
 		return;
-	rp_.writes().write_all();
-	renderer_.write_vram(host_vram_.data() + offset, offset, size);
+	// The written 8 KiB pages go back to the device whole, as the reference commits a host write
+	// with every block of its pages marked (gs/gs_interface.cpp:4210-4228, gs/gs_renderer.cpp:
+	// 1478-1539); the texture cache forgets images decoded from them.
+	const uint32_t first_page = uint32_t(offset / kPageBytes), last_page = uint32_t((offset + size - 1u) / kPageBytes);
+	rp_.writes().write(PageSpan{ first_page, last_page });
+	const auto t0 = std::chrono::steady_clock::now();
+	renderer_.write_vram(host_vram_.data() + size_t(first_page) * kPageBytes, size_t(first_page) * kPageBytes,
+	                     size_t(last_page - first_page + 1u) * kPageBytes);
+	if (timestamps_)
+		timestamp_ms_[uint32_t(TimestampType::SyncHostToVRAM)] +=
+		    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 }

@@ -5,6 +5,7 @@
 #include "gs_defs.hpp"
 #include "renderer.hpp"
 #include "render_pass.hpp"
+#include <array>
 #include <vector>
 #include <cstdint>
 
@@ -34,10 +35,33 @@ struct RegisterState
 	uint64_t fog = 0;
 	uint64_t tex0[2] = {}, clamp[2] = {}, tex1[2] = {}, xyoffset[2] = {}, miptbp1[2] = {}, miptbp2[2] = {};
 	uint64_t scissor[2] = {}, alpha[2] = {}, test[2] = {}, fba[2] = {}, frame[2] = {}, zbuf[2] = {};
-	uint64_t prmodecont = 1, prmode = 0, texclut = 0, scanmsk = 0, texa = 0, fogcol = 0;
+	uint64_t prmodecont = 0, prmode = 0, texclut = 0, scanmsk = 0, texa = 0, fogcol = 0;
 	uint64_t dimx = 0, dthe = 0, colclamp = 0, pabe = 0;
 	uint64_t bitbltbuf = 0, trxpos = 0, trxreg = 0, trxdir = 0;
 	float internal_q = 1.0f;
+};
+
+// SignalInterface (gs/gs_interface.hpp:242-250) as C callbacks.
+struct SignalCallbacks
+{
+	int (*on_signal)(void *, uint64_t) = nullptr;
+	int (*on_finish)(void *, uint64_t) = nullptr;
+	int (*on_label)(void *, uint64_t) = nullptr;
+	void *user = nullptr;
+};
+
+// Hacks (gs/gs_interface.hpp:211-240) and DebugMode (gs/gs_interface.hpp:137-153), the fields
+// this backend acts on.
+struct Hacks
+{
+	bool disable_mipmaps = false;
+	bool unsynced_readbacks = false;
+};
+
+// TimestampType (gs/gs_renderer.hpp:67-79).
+enum class TimestampType : uint32_t
+{
+	SyncHostToVRAM, CopyVRAM, PaletteUpdate, TextureUpload, TriangleSetup, Binning, Shading, Readback, VSync, Count
 };
 
 struct GIFPath
@@ -67,8 +91,20 @@ public:
 	void clobber_register_state();
 	void read_transfer_fifo(void *data, uint32_t num_128b_words);
 
+	// gs/gs_interface.cpp:65-118. False (nothing changes) for a rate or grid this backend does
+	// not build or that exceeds the allocated planes.
+	bool set_super_sampling_rate(SuperSampling rate, bool ordered_grid, bool super_sampled_textures);
+	uint32_t super_sampling_rate() const { return 1u << renderer_.sample_log2(); }
+	void set_hacks(const Hacks &h) { hacks_ = h; }
+	void set_signal_interface(const SignalCallbacks &cb) { signal_ = cb; }
+	void set_timestamps(bool enable);
+	void set_sampler_feedback(uint32_t mode) { renderer_.set_sampler_feedback(mode); }
+	double get_accumulated_timestamps(TimestampType type);
+
 	RegisterState &get_register_state() { return regs_; }
 	GIFPath &get_gif_path(uint32_t path) { return paths_[path & 3u]; }
+	uint8_t *get_priv_register_state() { return priv_.data(); }
+	uint32_t *cached_cbp() { return cached_cbp_; }
 	FlushStats consume_flush_stats() { return renderer_.consume_stats(); }
 	Renderer &renderer() { return renderer_; }
 	RenderPassBuilder &render_pass() { return rp_; }
@@ -87,6 +123,12 @@ private:
 	};
 
 	GSOptions options_;
+	Hacks hacks_;
+	SignalCallbacks signal_;
+	std::array<uint8_t, 8192> priv_{};
+	bool timestamps_ = false;
+	double timestamp_ms_[uint32_t(TimestampType::Count)] = {};
+	void poll_timestamps(bool wait);
 	Renderer renderer_;
 	RenderPassBuilder rp_;
 	RegisterState regs_;
@@ -104,6 +146,9 @@ private:
 	void begin_transfer_work();
 	void flush_render_pass();
 	void set_state_reg(uint64_t &reg, uint64_t v);
+	void reset_registers();
+	void write_prim(uint64_t v);
+	void write_prmode(uint64_t v);
 	void before_first_primitive() override;
 	void before_texture_read() override;
 };

@@ -99,13 +99,18 @@ bool triangles_form_parallelogram(const VertexPosition *p, const VertexAttribute
 }
 }
 
-void RenderPassBuilder::init(Renderer *renderer, uint32_t sample_log2)
+void RenderPassBuilder::set_sample_log2(uint32_t sample_log2)
 {
-	renderer_ = renderer;
 	sample_log2_ = sample_log2;
 	// Samples per axis of the ordered grids: 2x and 4x use 2, 8x and 16x use 4
 	// (gs/gs_interface.cpp:65-118).
 	rate_log2_ = sample_log2 ? (sample_log2 + 1) / 2 : 0;
+}
+
+void RenderPassBuilder::init(Renderer *renderer, uint32_t sample_log2)
+{
+	renderer_ = renderer;
+	set_sample_log2(sample_log2);
 	pos_.reserve(3u * 65536u);
 	attr_.reserve(3u * 65536u);
 	prims_.reserve(65536u);

@@ -77,13 +77,16 @@ void Renderer::check(hipError_t err)
 	}
 }
 
-bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback, bool texture_stream)
+bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback, bool texture_stream,
+                    uint32_t capacity_log2)
 {
 	try
 	{
 		sample_log2_ = sample_log2;
+		capacity_log2_ = std::max(capacity_log2, sample_log2);
 		sampler_feedback_ = sampler_feedback;
 		num_planes_ = sample_log2 ? 2u + (1u << sample_log2) : 1u;
+		const uint32_t capacity_planes = capacity_log2_ ? 2u + (1u << capacity_log2_) : 1u;
 		check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 		if (texture_stream)
 		{
@@ -91,10 +94,11 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_f
 			check(hipEventCreateWithFlags(&fork_event_, hipEventDisableTiming));
 			check(hipEventCreateWithFlags(&join_event_, hipEventDisableTiming));
 		}
-		check(hipMalloc(&vram_, size_t(num_planes_) * kVramSize));
-		check(hipMemsetAsync(vram_, 0, size_t(num_planes_) * kVramSize, stream_));
+		check(hipMalloc(&vram_, size_t(capacity_planes) * kVramSize));
+		check(hipMemsetAsync(vram_, 0, size_t(capacity_planes) * kVramSize, stream_));
 		check(hipMalloc(&shadow_, kVramSize));
 		arena_.cap = std::max<size_t>(arena_bytes, 16u << 20);
+		arena_bytes_ = arena_.cap;
 		check(hipHostMalloc(&arena_.host, arena_.cap, hipHostMallocDefault));
 		check(hipMalloc(&arena_.dev, arena_.cap));
 		batch_written_.assign(kNumBlocks / 64, 0);
@@ -141,6 +145,14 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_f
 uint8_t *Renderer::arena_alloc(size_t bytes, size_t &dev_offset)
 {
 	bytes = align_up(std::max<size_t>(bytes, 1), kAlign);
+	if (!arena_.host)
+	{
+		// The arena handed to the last recording was detached (end_recording): work after the
+		// recording gets a new one of the configured size.
+		arena_.cap = std::max(arena_bytes_, align_up(bytes, 1u << 20));
+		check(hipHostMalloc(&arena_.host, arena_.cap, hipHostMallocDefault));
+		check(hipMalloc(&arena_.dev, arena_.cap));
+	}
 	if (arena_.used + bytes > arena_.cap)
 		arena_make_room(bytes);
 	dev_offset = arena_.used;
@@ -827,6 +839,39 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 // ---------------------------------------------------------------------------------------
 // Host VRAM access.
 
+bool Renderer::set_sample_log2(uint32_t sample_log2)
+{
+	if (sample_log2 > capacity_log2_ || recording_)
+		return false;
+	sync();
+	const uint32_t capacity_planes = capacity_log2_ ? 2u + (1u << capacity_log2_) : 1u;
+	if (capacity_planes > 1u)
+		check(hipMemset(vram_ + kVramSize, 0, size_t(capacity_planes - 1u) * kVramSize));
+	sample_log2_ = sample_log2;
+	num_planes_ = sample_log2 ? 2u + (1u << sample_log2) : 1u;
+	return true;
+}
+
+void Renderer::poll_timings(double *ms, uint32_t kinds, bool wait)
+{
+	if (wait)
+		sync_streams();
+	size_t done = 0;
+	for (; done < timed_.size(); done++)
+	{
+		TimedLaunch &t = timed_[done];
+		if (!wait && hipEventQuery(t.stop) != hipSuccess)
+			break;
+		float e = 0.0f;
+		check(hipEventElapsedTime(&e, t.start, t.stop));
+		if (t.kind < kinds)
+			ms[t.kind] += e;
+		event_pool_.push_back(t.start);
+		event_pool_.push_back(t.stop);
+	}
+	timed_.erase(timed_.begin(), timed_.begin() + std::ptrdiff_t(done));
+}
+
 void Renderer::read_vram(void *dst, uint32_t first_plane, uint32_t planes)
 {
 	sync();
@@ -838,6 +883,10 @@ void Renderer::write_vram(const void *src, size_t offset, size_t size)
 {
 	sync();
 	check(hipMemcpy(vram_ + offset, src, size, hipMemcpyHostToDevice));
+	// A host write invalidates the super-samples of what it overwrites: the reference plane of
+	// the range is cleared (gs/gs_renderer.cpp:1478-1492).
+	if (sample_log2_)
+		check(hipMemset(vram_ + kVramSize + offset, 0, size));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -850,6 +899,13 @@ void Renderer::begin_recording()
 {
 	sync();
 	destroy_graph();
+	// The previous recording's arenas hold nothing a later launch reads any more.
+	for (auto &a : retired_arenas_)
+	{
+		if (a.host) (void)hipHostFree(a.host);
+		if (a.dev) (void)hipFree(a.dev);
+	}
+	retired_arenas_.clear();
 	recorded_.clear();
 	replay_.clear();
 	restore_ranges_.clear();
@@ -857,8 +913,16 @@ void Renderer::begin_recording()
 		check(hipFree(restore_dev_));
 	restore_dev_ = nullptr;
 	const size_t vram_bytes = size_t(num_planes_) * kVramSize;
+	if (snapshot_ && snapshot_bytes_ < vram_bytes + 1024u)
+	{
+		check(hipFree(snapshot_));
+		snapshot_ = nullptr;
+	}
 	if (!snapshot_)
+	{
 		check(hipMalloc(&snapshot_, vram_bytes + 1024u));
+		snapshot_bytes_ = vram_bytes + 1024u;
+	}
 	check(hipMemcpyAsync(snapshot_, vram_, vram_bytes, hipMemcpyDeviceToDevice, stream_));
 	check(hipMemcpyAsync(snapshot_ + vram_bytes, clut_buf_, 1024u, hipMemcpyDeviceToDevice, stream_));
 	main_pending_ = true;
@@ -914,6 +978,10 @@ void Renderer::end_recording()
 	arena_upload();
 	sync_streams();
 	recording_ = false;
+	// The recorded launches point into the arena: it stays with the recording, and work after
+	// the recording allocates a fresh arena on its first upload (arena_make_room).
+	retired_arenas_.push_back(arena_);
+	arena_ = {};
 	if (!side_)
 		fuse_replay();
 }

@@ -43,7 +43,20 @@ public:
 	// texels, 1 always, 2 never.
 	// texture_stream: run VRAM copies, CLUT loads and texture decodes on a second stream,
 	// overlapping triangle setup, the sampling heuristic and binning.
-	bool init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback = 0, bool texture_stream = false);
+	// capacity_log2 (>= sample_log2): the largest sample count set_sample_log2 may switch to;
+	// its planes are allocated up front.
+	bool init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_feedback = 0, bool texture_stream = false,
+	          uint32_t capacity_log2 = 0);
+	// Switches the sample count (GSRenderer::invalidate_super_sampling_state,
+	// gs/gs_renderer.cpp:315-342): waits for the GPU, then clears every plane above plane 0.
+	// False when the rate exceeds the allocated capacity or a recording is open.
+	bool set_sample_log2(uint32_t sample_log2);
+	uint32_t capacity_log2() const { return capacity_log2_; }
+	void set_sampler_feedback(uint32_t mode) { sampler_feedback_ = mode; }
+	// Adds the elapsed time of every timed launch whose events have completed to ms[kind]
+	// (without waiting), in launch order; with wait, first waits for the GPU and drains all.
+	void poll_timings(double *ms, uint32_t kinds, bool wait);
+	bool profiling() const { return profiling_; }
 
 	// ---- transfers -------------------------------------------------------------------
 	// Queue a host->local upload. The payload bytes are copied into the upload arena.
@@ -53,6 +66,11 @@ public:
 	// Close the current transfer batch: build the per-block work lists and launch.
 	void flush_transfers();
 	bool has_pending_transfers() const { return !batch_.empty(); }
+	// No submitted work is still running.
+	bool idle() const
+	{
+		return hipStreamQuery(stream_) == hipSuccess && (!side_ || hipStreamQuery(side_) == hipSuccess);
+	}
 
 	// ---- render passes ----------------------------------------------------------------
 	// Uploads the pass's primitive arrays and runs triangle setup, binning and shading.
@@ -185,7 +203,9 @@ private:
 	void emit_transfer_level(const std::vector<uint32_t> &members);
 
 	uint32_t sample_log2_ = 0;
+	uint32_t capacity_log2_ = 0;
 	uint32_t num_planes_ = 1;
+	size_t snapshot_bytes_ = 0;
 	uint8_t *vram_ = nullptr;    // num_planes_ * 4 MiB
 	uint8_t *shadow_ = nullptr;  // 4 MiB
 	hipStream_t stream_ = nullptr;
@@ -202,7 +222,8 @@ private:
 	int last_error_ = 0;
 
 	Arena arena_;
-	std::vector<Arena> retired_arenas_; // kept alive while recording
+	size_t arena_bytes_ = 0;            // configured arena size
+	std::vector<Arena> retired_arenas_; // arenas of the current recording (read by replays)
 	bool recording_ = false;
 	std::vector<Launch> recorded_;   // kernels
 	std::vector<Launch> replay_;     // kernels and stream ordering points

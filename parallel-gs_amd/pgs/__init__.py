@@ -39,7 +39,51 @@ class _Options(ctypes.Structure):
         ("upload_arena_bytes", ctypes.c_uint64),
         ("sampler_feedback", ctypes.c_uint32),
         ("texture_stream", ctypes.c_uint32),
+        ("dynamic_super_sampling", ctypes.c_uint32),
     ]
+
+
+ERROR_INVALID_ARGUMENT = -1
+ERROR_UNSUPPORTED = -1000
+
+
+class ContextRegisters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("tex0", "tex1", "clamp", "xyoffset", "miptbp1", "miptbp2", "scissor",
+                                               "alpha", "test", "fba", "frame", "zbuf")]
+
+
+class RegisterState(ctypes.Structure):
+    _fields_ = ([("ctx", ContextRegisters * 2)] +
+                [(n, ctypes.c_uint64) for n in ("prim", "rgbaq", "st", "uv", "fog", "prmodecont", "texclut", "texa",
+                                                "fogcol", "dimx", "dthe", "colclamp", "pabe", "bitbltbuf", "trxpos",
+                                                "trxreg", "trxdir", "scanmsk")] +
+                [("internal_q", ctypes.c_float), ("cached_cbp", ctypes.c_uint32 * 2), ("pad", ctypes.c_uint32)])
+
+
+class GifPath(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64 * 2), ("reg", ctypes.c_uint32), ("loop", ctypes.c_uint32)]
+
+
+class Hacks(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("disable_mipmaps", "unsynced_readbacks", "backbuffer_promotion",
+                                               "allow_blend_demote")]
+
+
+class DebugMode(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("feedback_render_target", "timestamps", "deterministic_timeline_query",
+                                               "disable_sampler_feedback", "draw_mode")]
+
+
+SIGNAL_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64)
+
+
+class SignalInterface(ctypes.Structure):
+    _fields_ = [("on_signal", SIGNAL_CB), ("on_finish", SIGNAL_CB), ("on_label", SIGNAL_CB), ("user", ctypes.c_void_p)]
+
+
+PRIV_REGISTER_BYTES = 8192
+TIMESTAMPS = ("sync_host_to_vram", "copy_vram", "palette_update", "texture_upload", "triangle_setup", "binning",
+              "shading", "readback", "vsync")
 
 
 class FlushStats(ctypes.Structure):
@@ -70,6 +114,10 @@ EXPORTED_SYMBOLS = [
     "pgs_write_register", "pgs_flush", "pgs_map_vram_read", "pgs_map_vram_write",
     "pgs_end_vram_write", "pgs_read_transfer_fifo", "pgs_clobber_register_state",
     "pgs_consume_flush_stats", "pgs_num_planes", "pgs_read_vram_planes",
+    "pgs_get_register_state", "pgs_set_register_state", "pgs_get_gif_path", "pgs_set_gif_path",
+    "pgs_get_priv_register_state", "pgs_set_priv_register_state", "pgs_set_super_sampling_rate",
+    "pgs_get_super_sampling_rate", "pgs_set_hacks", "pgs_set_debug_mode", "pgs_set_signal_interface",
+    "pgs_get_accumulated_timestamps",
     "pgs_begin_recording", "pgs_end_recording", "pgs_replay", "pgs_sync", "pgs_stream",
     "pgs_profile_enable", "pgs_profile_collect",
     "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
@@ -124,6 +172,20 @@ def lib() -> ctypes.CDLL:
         L.pgs_host_swizzle_addr.restype = u32
         L.pgs_host_block_lane_pixel.argtypes = [u32, u32, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]
         L.pgs_host_block_lane_pixel.restype = None
+        L.pgs_get_register_state.argtypes = [vp, ctypes.POINTER(RegisterState)]
+        L.pgs_set_register_state.argtypes = [vp, ctypes.POINTER(RegisterState)]
+        L.pgs_get_gif_path.argtypes = [vp, u32, ctypes.POINTER(GifPath)]
+        L.pgs_set_gif_path.argtypes = [vp, u32, ctypes.POINTER(GifPath)]
+        L.pgs_get_priv_register_state.argtypes = [vp, vp, sz]
+        L.pgs_set_priv_register_state.argtypes = [vp, vp, sz]
+        L.pgs_set_super_sampling_rate.argtypes = [vp, u32, u32, u32]
+        L.pgs_get_super_sampling_rate.argtypes = [vp]
+        L.pgs_get_super_sampling_rate.restype = u32
+        L.pgs_set_hacks.argtypes = [vp, ctypes.POINTER(Hacks)]
+        L.pgs_set_debug_mode.argtypes = [vp, ctypes.POINTER(DebugMode)]
+        L.pgs_set_signal_interface.argtypes = [vp, ctypes.POINTER(SignalInterface)]
+        L.pgs_get_accumulated_timestamps.argtypes = [vp, u32]
+        L.pgs_get_accumulated_timestamps.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -197,11 +259,11 @@ class Context:
 
     def __init__(self, super_sampling: int = 1, device_index: int = 0, arena_bytes: int = 0,
                  ordered: bool = True, super_sampled_textures: bool = False, sampler_feedback: str = "auto",
-                 texture_stream: bool = False):
+                 texture_stream: bool = False, dynamic_super_sampling: bool = False):
         """texture_stream=True runs VRAM copies, CLUT loads and texture decodes on a second HIP
         stream beside setup and binning; by default every launch goes to one stream."""
         opts = _Options(super_sampling, int(ordered), int(super_sampled_textures), device_index, arena_bytes,
-                        self.SAMPLER_FEEDBACK[sampler_feedback], int(texture_stream))
+                        self.SAMPLER_FEEDBACK[sampler_feedback], int(texture_stream), int(dynamic_super_sampling))
         self._L = lib()
         self._h = self._L.pgs_create(ctypes.byref(opts))
         if not self._h:
@@ -237,6 +299,12 @@ class Context:
 
     def flush(self):
         self._check(self._L.pgs_flush(self._h), "flush")
+
+    def reset_context_state(self):
+        self._check(self._L.pgs_reset_context_state(self._h), "reset_context_state")
+
+    def clobber_register_state(self):
+        self._check(self._L.pgs_clobber_register_state(self._h), "clobber_register_state")
 
     def sync(self):
         self._check(self._L.pgs_sync(self._h), "sync")
@@ -296,6 +364,57 @@ class Context:
     @property
     def stream(self) -> int:
         return int(self._L.pgs_stream(self._h) or 0)
+
+    def register_state(self) -> RegisterState:
+        r = RegisterState()
+        self._check(self._L.pgs_get_register_state(self._h, ctypes.byref(r)), "get_register_state")
+        return r
+
+    def set_register_state(self, r: RegisterState):
+        self._check(self._L.pgs_set_register_state(self._h, ctypes.byref(r)), "set_register_state")
+
+    def gif_path(self, path: int) -> GifPath:
+        g = GifPath()
+        self._check(self._L.pgs_get_gif_path(self._h, path, ctypes.byref(g)), "get_gif_path")
+        return g
+
+    def set_gif_path(self, path: int, g: GifPath):
+        self._check(self._L.pgs_set_gif_path(self._h, path, ctypes.byref(g)), "set_gif_path")
+
+    def priv_registers(self) -> np.ndarray:
+        out = np.zeros(PRIV_REGISTER_BYTES, np.uint8)
+        self._check(self._L.pgs_get_priv_register_state(self._h, out.ctypes.data, out.nbytes), "get_priv")
+        return out
+
+    def set_priv_registers(self, data: np.ndarray):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self._check(self._L.pgs_set_priv_register_state(self._h, data.ctypes.data, data.nbytes), "set_priv")
+
+    def set_super_sampling_rate(self, samples: int, ordered: bool = True, textures: bool = False) -> int:
+        """Returns the C return code (0, ERROR_UNSUPPORTED or ERROR_INVALID_ARGUMENT)."""
+        return int(self._L.pgs_set_super_sampling_rate(self._h, samples, int(ordered), int(textures)))
+
+    @property
+    def super_sampling_rate(self) -> int:
+        return int(self._L.pgs_get_super_sampling_rate(self._h))
+
+    def set_hacks(self, **kw) -> int:
+        h = Hacks(**{k: int(v) for k, v in kw.items()})
+        return int(self._L.pgs_set_hacks(self._h, ctypes.byref(h)))
+
+    def set_debug_mode(self, **kw) -> int:
+        d = DebugMode(**{k: int(v) for k, v in kw.items()})
+        return int(self._L.pgs_set_debug_mode(self._h, ctypes.byref(d)))
+
+    def set_signal_interface(self, on_signal=None, on_finish=None, on_label=None):
+        """Each callback takes the 64-bit payload and returns True to flush."""
+        def wrap(f):
+            return SIGNAL_CB(lambda _user, v: int(bool(f(int(v))))) if f else SIGNAL_CB()
+        self._signal = SignalInterface(wrap(on_signal), wrap(on_finish), wrap(on_label), None)
+        self._check(self._L.pgs_set_signal_interface(self._h, ctypes.byref(self._signal)), "set_signal_interface")
+
+    def accumulated_timestamps(self) -> dict:
+        return {name: float(self._L.pgs_get_accumulated_timestamps(self._h, i)) for i, name in enumerate(TIMESTAMPS)}
 
 
 def host_swizzle_addr(x: int, y: int, base: int, stride: int, psm: int) -> int:

@@ -43,6 +43,14 @@ def oracle_lib() -> ctypes.CDLL:
         L.orc_swizzle.restype = u32
         L.orc_stat.argtypes = [vp, u32]
         L.orc_stat.restype = u64
+        L.orc_read_transfer_fifo.argtypes = [vp, vp, u32]
+        L.orc_read_transfer_fifo.restype = None
+        L.orc_host_write.argtypes = [vp, sz, vp, sz]
+        L.orc_host_write.restype = None
+        L.orc_set_sample_log2.argtypes = [vp, u32]
+        L.orc_set_sample_log2.restype = None
+        L.orc_reset_context_state.argtypes = [vp]
+        L.orc_reset_context_state.restype = None
         _L = L
     return _L
 
@@ -84,6 +92,21 @@ class Oracle:
 
     def stat(self, which: int) -> int:
         return int(self._L.orc_stat(self._h, which))
+
+    def read_transfer_fifo(self, n128: int) -> np.ndarray:
+        out = np.zeros(n128 * 16, dtype=np.uint8)
+        self._L.orc_read_transfer_fifo(self._h, out.ctypes.data, n128)
+        return out
+
+    def host_write(self, data: np.ndarray, offset: int = 0):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self._L.orc_host_write(self._h, offset, data.ctypes.data, data.nbytes)
+
+    def reset_context_state(self):
+        self._L.orc_reset_context_state(self._h)
+
+    def set_sample_log2(self, sample_log2: int):
+        self._L.orc_set_sample_log2(self._h, sample_log2)
 
 
 def swizzle(x, y, base, stride, psm) -> int:
