@@ -375,6 +375,17 @@ typedef struct pgs_mlp_grad_args
 
 int pgs_k_mlp_gradients(const pgs_mlp_grad_args *args, hipStream_t stream);
 
+/* pgs_k_mlp_gradients without a workspace: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, training loops that forbid
+ * allocation). `scratch` is ignored; the per-sample training scratch is a fixed device array
+ * of the library (PGS_MLP_NOALLOC_MAX_SAMPLES x PGS_MLP_SAMPLE_FLOATS floats, 4.6 MB per GPU,
+ * held until the process ends). count above PGS_MLP_NOALLOC_MAX_SAMPLES returns
+ * -hipErrorInvalidValue. The call allocates nothing. Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other streams, other threads) race. Results are identical to
+ * pgs_k_mlp_gradients. */
+#define PGS_MLP_NOALLOC_MAX_SAMPLES 4096u
+int pgs_k_mlp_gradients_noalloc(const pgs_mlp_grad_args *args, hipStream_t stream);
+
 typedef struct pgs_mlp_adam_args
 {
 	float *master;           /* FP32 parameters */

@@ -10,7 +10,15 @@
 // sample order, and Adam on FP32 master weights that refreshes the BF16 copy.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstdio>
+#include <mutex>
 #include "pgs_kernels.h"
+
+// Per-sample training scratch of pgs_k_mlp_gradients_noalloc: a fixed device array (each GPU
+// the library runs on holds its own copy until the process ends). Host code never passes the
+// array's name to a kernel; it passes the device address hipGetSymbolAddress returns for the
+// current device.
+static __device__ __align__(256) float scratch[size_t(PGS_MLP_NOALLOC_MAX_SAMPLES) * PGS_MLP_SAMPLE_FLOATS];
 
 namespace pgs
 {
@@ -206,6 +214,60 @@ extern "C" int pgs_k_mlp_gradients(const pgs_mlp_grad_args *a, hipStream_t strea
 		return 0;
 	hipLaunchKernelGGL(pgs::k_mlp_backward, dim3((a->count + 127u) / 128u), dim3(128), 0, stream, *a);
 	hipLaunchKernelGGL(pgs::k_mlp_reduce, dim3((PGS_MLP_PARAMS + 255u) / 256u), dim3(256), 0, stream, *a);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+}
+
+namespace
+{
+// Device address of `scratch` per device, looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+float *scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(float **out)
+{
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+		return -int(hipErrorInvalidDevice);
+	std::lock_guard<std::mutex> lock(scratch_mutex);
+	if (!scratch_ptr[dev])
+	{
+		void *ptr = nullptr;
+		const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (rc != hipSuccess || !ptr)
+		{
+			fprintf(stderr, "pgs: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev, hipGetErrorString(rc));
+			return -int(rc != hipSuccess ? rc : hipErrorInvalidSymbol);
+		}
+		scratch_ptr[dev] = static_cast<float *>(ptr);
+	}
+	*out = scratch_ptr[dev];
+	return 0;
+}
+}
+
+extern "C" int pgs_k_mlp_gradients_noalloc(const pgs_mlp_grad_args *a, hipStream_t stream)
+{
+	if (!a || !a->weights || !a->features || !a->targets || !a->grads)
+		return -int(hipErrorInvalidValue);
+	if (a->count > PGS_MLP_NOALLOC_MAX_SAMPLES)
+	{
+		fprintf(stderr, "pgs: pgs_k_mlp_gradients_noalloc: %u samples exceed the fixed scratch of %u samples\n",
+		        a->count, PGS_MLP_NOALLOC_MAX_SAMPLES);
+		return -int(hipErrorInvalidValue);
+	}
+	if (a->count == 0)
+		return 0;
+	float *dev_scratch = nullptr;
+	const int rc = scratch_address(&dev_scratch);
+	if (rc != 0)
+		return rc;
+	// The backward kernel writes every scratch float of samples [0, count) before the reduction
+	// reads them, so nothing needs clearing between calls.
+	pgs_mlp_grad_args b = *a;
+	b.scratch = dev_scratch;
+	hipLaunchKernelGGL(pgs::k_mlp_backward, dim3((b.count + 127u) / 128u), dim3(128), 0, stream, b);
+	hipLaunchKernelGGL(pgs::k_mlp_reduce, dim3((PGS_MLP_PARAMS + 255u) / 256u), dim3(256), 0, stream, b);
 	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }
 

@@ -127,7 +127,7 @@ EXPORTED_SYMBOLS = [
     "pgs_batch_sync", "pgs_batch_stream", "pgs_batch_num_launches", "pgs_batch_profile",
     "pgs_cost_model_create", "pgs_cost_model_destroy", "pgs_cost_model_set_params", "pgs_cost_model_get_params",
     "pgs_cost_model_predict", "pgs_cost_model_gradients", "pgs_cost_model_train",
-    "pgs_k_mlp_forward", "pgs_k_mlp_gradients", "pgs_k_mlp_adam", "pgs_k_mlp_quantize",
+    "pgs_k_mlp_forward", "pgs_k_mlp_gradients", "pgs_k_mlp_gradients_noalloc", "pgs_k_mlp_adam", "pgs_k_mlp_quantize",
     "pgs_batch_set_schedule", "pgs_batch_collect_costs", "pgs_k_shade_queue", "pgs_k_tile_features",
     "pgs_k_queue_order", "pgs_k_fill_u32", "pgs_k_shade_ordered", "pgs_k_sampler_feedback", "pgs_k_restore", "pgs_k_fused",
 ]
