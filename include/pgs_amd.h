@@ -288,9 +288,13 @@ int pgs_batch_profile(pgs_batch *batch, double *ms, uint64_t *launches, uint32_t
  * step), PGS_SCHEDULE_QUEUE_STATIC (a work queue of resident workgroups in item order),
  * PGS_SCHEDULE_QUEUE_LEARNED (the queue ordered by a cost model's predictions, largest first,
  * heavy tiles split), PGS_SCHEDULE_DISPATCH_STATIC (one workgroup per chunk of a tile's waves,
- * started by the hardware dispatcher in tile order) or PGS_SCHEDULE_DISPATCH_LEARNED (the same
- * with tiles in predicted-cost order, largest first). Learned schedules need `model`, which
- * must outlive the batch. Pixels are identical under every schedule. */
+ * started by the hardware dispatcher, each XCD taking an eighth of the tiles in tile order) or
+ * PGS_SCHEDULE_DISPATCH_LEARNED (the same with the tiles placed on XCDs by predicted cost:
+ * heavy tiles first, largest first, to the least loaded XCD; the rest in tile order filling each
+ * XCD to the mean). The order is built once, by the first replay after this call (or after a
+ * tuning change or a cost collection), which runs eagerly; later replays only shade. Learned
+ * schedules need `model`, which must outlive the batch; set the schedule again after changing
+ * the model. Pixels are identical under every schedule. */
 #define PGS_SCHEDULE_HARDWARE 0u
 #define PGS_SCHEDULE_QUEUE_STATIC 1u
 #define PGS_SCHEDULE_QUEUE_LEARNED 2u

@@ -228,8 +228,9 @@ int pgs_k_shade_queue(const pgs_queue_args *args, hipStream_t stream);
 
 /* Ordered dispatch: the same shading with the hardware dispatcher as the queue. Workgroup b
  * (of `grid`, at least counter[1]) shades chunk order[b] & 0xfff of tile order[b] >> 12, the
- * chunk being waves_per_group consecutive waves of the tile; workgroups from counter[1] on exit
- * at once. The dispatcher starts workgroups in index order, so `order` is the start order. */
+ * chunk being waves_per_group consecutive waves of the tile; workgroups from counter[1] on, and
+ * those whose entry is 0xffffffff, exit at once. The dispatcher starts workgroups in index order
+ * and hands workgroup b to XCD b % 8, so `order` is the start order of every XCD. */
 int pgs_k_shade_ordered(const pgs_queue_args *args, uint32_t grid, hipStream_t stream);
 
 /* Features of every coarse tile of a shading multi-launch for the tile-cost model: one wave

@@ -82,6 +82,7 @@ uint32_t order_entries(uint32_t tiles, uint32_t waves)
 Batch::~Batch()
 {
 	destroy_graph();
+	free_plans();
 	if (dev_)
 		(void)hipFree(dev_);
 	if (restore_dev_)
@@ -395,6 +396,12 @@ bool Batch::upload()
 		ok(hipMalloc(&sched_, sb));
 		sched_bytes_ = sb;
 	}
+	if (plans_.size() != merged_.size())
+	{
+		free_plans();
+		plans_.resize(merged_.size());
+	}
+	planned_ = false;
 	destroy_graph();
 	return true;
 }
@@ -405,8 +412,129 @@ bool Batch::set_schedule(Schedule s, const CostModel *model)
 		return false;
 	schedule_ = s;
 	model_ = model;
+	planned_ = false;
 	destroy_graph();
 	return true;
+}
+
+void Batch::free_plans()
+{
+	for (Plan &p : plans_)
+		if (p.order)
+			(void)hipFree(p.order);
+	plans_.clear();
+	planned_ = false;
+}
+
+// Dispatch order placed per XCD. The dispatcher hands workgroup b to XCD b % 8 in index order,
+// so XCD x runs the entries at 8 k + x one after another; unused places hold 0xffffffff (the
+// workgroup exits). Each context's tiles with work are cut into eight runs in tile order, run x
+// going to XCD x (the screen bands the hardware schedule's remap gives the XCDs, so every XCD
+// gets a share of every context). Static order cuts the runs by chunk count and keeps them in
+// context order. Learned order cuts them by predicted cost, so that every XCD gets the same
+// predicted work, takes the contexts by predicted cost, largest first, and starts, on each XCD,
+// the tiles whose predicted cost is above half of what one resident workgroup of the XCD does on
+// average, largest first (so that no heavy tile starts last); the other tiles follow in context
+// and tile order. A tile's chunks stay together.
+void Batch::plan_dispatch(const Merged &m, Plan &plan, const float *features, const float *pred,
+                          const uint32_t *tile_waves, uint32_t *counter, uint32_t group_waves)
+{
+	constexpr uint32_t kXcds = 8;
+	const uint32_t n = m.tiles;
+	std::vector<float> f(size_t(n) * PGS_MLP_INPUTS), p;
+	std::vector<uint32_t> waves(n);
+	ok(hipStreamSynchronize(stream_));
+	ok(hipMemcpy(f.data(), features, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+	ok(hipMemcpy(waves.data(), tile_waves, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+	if (pred)
+	{
+		p.resize(n);
+		ok(hipMemcpy(p.data(), pred, n * sizeof(float), hipMemcpyDeviceToHost));
+	}
+	auto has_work = [&](uint32_t i) { return f[size_t(i) * PGS_MLP_INPUTS] != 0.0f && waves[i] != 0; };
+	auto chunks = [&](uint32_t i) { return (waves[i] + group_waves - 1u) / group_waves; };
+	// Static runs are cut by chunks, learned runs by predicted cost.
+	auto weight = [&](uint32_t i) {
+		return pred ? std::exp2(double(std::min(p[i], 40.0f))) : double(chunks(i));
+	};
+	std::vector<std::vector<uint32_t>> seq(kXcds); // tiles per XCD, in run order
+	double load[kXcds] = {};
+	// Contexts (launches of the multi-launch) in launch order, or, learned, by predicted cost,
+	// largest first, so that the last contexts to start are the light ones.
+	std::vector<uint32_t> firsts(m.refs.size() + 1u, 0u);
+	for (size_t k = 0; k < m.refs.size(); k++)
+	{
+		const pgs_shade_args &sa = launches_[m.refs[k].context][m.refs[k].index].shade;
+		firsts[k + 1] = firsts[k] + sa.tiles_x * sa.tiles_y;
+	}
+	std::vector<uint32_t> launch_order(m.refs.size());
+	std::vector<double> launch_cost(m.refs.size(), 0.0);
+	for (size_t k = 0; k < m.refs.size(); k++)
+	{
+		launch_order[k] = uint32_t(k);
+		for (uint32_t i = firsts[k]; i < firsts[k + 1]; i++)
+			if (has_work(i))
+				launch_cost[k] += weight(i);
+	}
+	if (pred)
+		std::stable_sort(launch_order.begin(), launch_order.end(),
+		                 [&](uint32_t a, uint32_t b) { return launch_cost[a] > launch_cost[b]; });
+	for (uint32_t k : launch_order)
+	{
+		const uint32_t first = firsts[k], end = firsts[k + 1];
+		const double total = launch_cost[k];
+		double at = 0.0;
+		for (uint32_t i = first; i < end; i++)
+			if (has_work(i))
+			{
+				const double w = weight(i);
+				// The run a tile falls in by the midpoint of its weight.
+				const uint32_t x = total > 0.0 ? std::min(kXcds - 1u, uint32_t((at + 0.5 * w) * kXcds / total)) : 0u;
+				seq[x].push_back(i);
+				load[x] += w;
+				at += w;
+			}
+	}
+	if (pred)
+	{
+		const uint32_t slots = std::max(1u, pgs_queue_slots(m.rate_x, m.rate_y, tuning_.waves_per_group, m.tiles) / kXcds);
+		for (uint32_t x = 0; x < kXcds; x++)
+		{
+			const double heavy = 0.5 * load[x] / slots;
+			std::stable_partition(seq[x].begin(), seq[x].end(), [&](uint32_t i) { return weight(i) > heavy; });
+			const auto mid = std::find_if(seq[x].begin(), seq[x].end(), [&](uint32_t i) { return weight(i) <= heavy; });
+			std::stable_sort(seq[x].begin(), mid, [&](uint32_t a, uint32_t b) { return weight(a) > weight(b); });
+		}
+	}
+	size_t longest = 0;
+	std::vector<std::vector<uint32_t>> entries(kXcds);
+	for (uint32_t x = 0; x < kXcds; x++)
+	{
+		for (uint32_t i : seq[x])
+			for (uint32_t k = 0; k < chunks(i); k++)
+				entries[x].push_back((i << 12) | k);
+		longest = std::max(longest, entries[x].size());
+	}
+	const size_t grid = longest * kXcds;
+	if (grid > 0x7fffffffu)
+		throw std::runtime_error("dispatch plan: too many workgroups");
+	std::vector<uint32_t> order(grid, 0xffffffffu);
+	for (uint32_t x = 0; x < kXcds; x++)
+		for (size_t k = 0; k < entries[x].size(); k++)
+			order[k * kXcds + x] = entries[x][k];
+	if (plan.capacity < grid)
+	{
+		if (plan.order)
+			ok(hipFree(plan.order));
+		plan.order = nullptr;
+		ok(hipMalloc(&plan.order, std::max<size_t>(grid, 1u) * sizeof(uint32_t)));
+		plan.capacity = grid;
+	}
+	plan.grid = uint32_t(grid);
+	if (grid)
+		ok(hipMemcpy(plan.order, order.data(), grid * sizeof(uint32_t), hipMemcpyHostToDevice));
+	const uint32_t limits[2] = { 0u, uint32_t(grid) };
+	ok(hipMemcpy(counter, limits, sizeof(limits), hipMemcpyHostToDevice));
 }
 
 // Shading launch under the current schedule.
@@ -446,6 +574,34 @@ void Batch::launch_shade(const Merged &m)
 	                      !collecting_;
 	if (m.tiles >= (dispatch ? (1u << 20) : (1u << 24)))
 		throw std::runtime_error("scheduled shading: too many tiles");
+	const uint32_t group_waves = tuning_.waves_per_group ? tuning_.waves_per_group : 4u;
+	Plan &plan = plans_[size_t(&m - merged_.data())];
+
+	pgs_queue_args q{};
+	q.args = args;
+	q.prefix = tile_prefix;
+	q.order = dispatch ? plan.order : order;
+	q.counter = counter;
+	q.costs = collecting_ ? costs : nullptr;
+	q.num_launches = n;
+	q.total = m.tiles;
+	q.rate_x = m.rate_x;
+	q.rate_y = m.rate_y;
+	q.waves_per_group = tuning_.waves_per_group;
+	if (planned_ && !collecting_)
+	{
+		// The order of an earlier planning replay: a dispatch plan needs nothing more, a queue
+		// only its pop counter reset.
+		if (dispatch)
+		{
+			if (plan.grid && pgs_k_shade_ordered(&q, plan.grid, stream_) != 0)
+				throw std::runtime_error("ordered shading launch failed");
+			return;
+		}
+		if (pgs_k_fill_u32(counter, 1, 0u, stream_) != 0 || pgs_k_shade_queue(&q, stream_) != 0)
+			throw std::runtime_error("work-queue shading launch failed");
+		return;
+	}
 
 	// Features of every tile (the queue holds only tiles with work), the model's predictions,
 	// the queue, then the resident-workgroup shading.
@@ -469,44 +625,24 @@ void Batch::launch_shade(const Merged &m)
 	o.count = m.tiles;
 	o.slots = pgs_queue_slots(m.rate_x, m.rate_y, tuning_.waves_per_group, m.tiles);
 	o.capacity = order_entries(m.tiles, m.waves);
-	const uint32_t group_waves = tuning_.waves_per_group ? tuning_.waves_per_group : 4u;
 	o.tile_waves = tile_waves;
 	o.run_start = run_start;
 	o.run_weight = run_weight;
 	o.group_waves = group_waves;
 	o.dispatch = dispatch ? 1u : 0u;
+	if (dispatch)
+	{
+		// The host places the chunks per XCD (plan_dispatch) from the features and predictions.
+		plan_dispatch(m, plan, features, learned ? pred : nullptr, tile_waves, counter, group_waves);
+		q.order = plan.order;
+		if (plan.grid && pgs_k_shade_ordered(&q, plan.grid, stream_) != 0)
+			throw std::runtime_error("ordered shading launch failed");
+		return;
+	}
 	if (pgs_k_queue_order(&o, stream_) != 0)
 		throw std::runtime_error("order launch failed");
 	if (collecting_ && pgs_k_fill_u32(costs, m.tiles, 0u, stream_) != 0)
 		throw std::runtime_error("cost clear failed");
-
-	pgs_queue_args q{};
-	q.args = args;
-	q.prefix = tile_prefix;
-	q.order = order;
-	q.counter = counter;
-	q.costs = collecting_ ? costs : nullptr;
-	q.num_launches = n;
-	q.total = m.tiles;
-	q.rate_x = m.rate_x;
-	q.rate_y = m.rate_y;
-	q.waves_per_group = tuning_.waves_per_group;
-	if (dispatch)
-	{
-		// One workgroup per chunk; the grid counts the chunks of every tile, with or without
-		// work (each tile's waves are rounded up to whole chunks separately).
-		uint64_t grid = 0;
-		for (const Ref &r : m.refs)
-		{
-			const pgs_shade_args &sa = launches_[r.context][r.index].shade;
-			const uint64_t B = 1ull << (sa.tile_log2 - 3u);
-			const uint64_t w = B * B << (sa.rate_x_log2 + sa.rate_log2);
-			grid += uint64_t(sa.tiles_x) * sa.tiles_y * ((w + group_waves - 1u) / group_waves);
-		}
-		if (grid > 0x7fffffffull || pgs_k_shade_ordered(&q, uint32_t(grid), stream_) != 0)
-			throw std::runtime_error("ordered shading launch failed");
-		return;
-	}
 	if (pgs_k_shade_queue(&q, stream_) != 0)
 		throw std::runtime_error("work-queue shading launch failed");
 }
@@ -527,6 +663,9 @@ uint32_t Batch::collect_costs(float *features_out, float *log2_costs, uint32_t m
 		throw;
 	}
 	collecting_ = false;
+	// The collection replay rebuilt the queues in tile order: plan again before the next replay.
+	planned_ = false;
+	destroy_graph();
 	uint32_t written = 0;
 	std::vector<float> f;
 	std::vector<uint32_t> c;
@@ -632,6 +771,17 @@ int Batch::replay(bool use_graph)
 			for (auto &m : merged_)
 				launch(m);
 		};
+		if (schedule_ != Schedule::Hardware && !planned_)
+		{
+			// Planning replay (eager): builds every shading launch's order once.
+			if (plans_.size() != merged_.size())
+				plans_.resize(merged_.size());
+			issue();
+			ok(hipStreamSynchronize(stream_));
+			planned_ = true;
+			destroy_graph();
+			return 0;
+		}
 		if (!use_graph)
 		{
 			issue();

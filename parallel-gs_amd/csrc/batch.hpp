@@ -103,6 +103,20 @@ private:
 	uint8_t *sched_ = nullptr;    // work-queue buffers of the shading launches
 	size_t sched_bytes_ = 0;
 	bool collecting_ = false;
+	// A schedule's queue or dispatch order depends only on the recording, the tuning, the
+	// schedule and the model, so it is built once (a planning replay) and replays then only
+	// shade. Dispatch orders are placed per XCD on the host (plans_, one per merged launch).
+	bool planned_ = false;
+	struct Plan
+	{
+		uint32_t *order = nullptr; // device: grid entries, entry k of XCD x at 8 k + x
+		uint32_t grid = 0;
+		size_t capacity = 0;
+	};
+	std::vector<Plan> plans_;
+	void free_plans();
+	void plan_dispatch(const Merged &m, Plan &plan, const float *features, const float *pred, const uint32_t *tile_waves,
+	                   uint32_t *counter, uint32_t group_waves);
 
 	bool upload();
 	void destroy_graph();

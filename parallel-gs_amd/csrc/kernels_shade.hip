@@ -1071,6 +1071,8 @@ __device__ __forceinline__ void ordered_chunk(const pgs_queue_args &q, const pgs
 	if (blockIdx.x >= uni(q.counter[1]))
 		return;
 	const uint32_t e = uni(q.order[blockIdx.x]);
+	if (e == 0xffffffffu) // an unused place of a per-XCD order
+		return;
 	const uint32_t g = e >> 12, chunk = e & 0xfffu;
 	const uint32_t l = uni(find_launch(q.prefix, q.num_launches, g));
 	const pgs_shade_args &a = args[l];
