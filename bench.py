@@ -381,18 +381,88 @@ def spearman(a, b):
     return float((ra * rb).sum() / np.sqrt((ra * ra).sum() * (rb * rb).sum()))
 
 
+def oracle_contexts_parallel(specs, threads):
+    """Runs the CPU oracle over several contexts at once, one context per thread (the ctypes calls
+    release the GIL, so the threads run on separate cores). Returns (wall seconds, planes per
+    context)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import pgs
+    from oracle_lib import Oracle
+
+    streams = [pgs.spec_stream(sp) for sp in specs]
+
+    def run(k):
+        o = Oracle({1: 0, 2: 1, 4: 2, 8: 3, 16: 4}[specs[k]["ssaa"]])
+        o.gif_transfer(streams[k])
+        o.flush()
+        return o.planes()
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        planes = list(ex.map(run, range(len(specs))))
+    return time.perf_counter() - t0, planes
+
+
+def live_parallel(ctxs, specs, threads):
+    """Steady-state drop-in path: every context parses its GIF stream through gif_transfer and
+    flushes (no recording), contexts on parallel host threads. Returns the wall seconds of the
+    second of two passes."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import pgs
+
+    streams = [pgs.spec_stream(sp) for sp in specs]
+
+    def run(k):
+        ctxs[k].gif_transfer(streams[k])
+        ctxs[k].flush()
+        ctxs[k].sync()
+
+    wall = 0.0
+    for _ in range(2):
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            list(ex.map(run, range(len(ctxs))))
+        wall = time.perf_counter() - t0
+    return wall
+
+
 def bench_learned(args):
-    """16x SSAA, 64 contexts with Pareto(1.2) primitive areas: the tile-cost MLP is trained on
-    the GPU from costs measured on a separate training batch, then orders the work queue of
-    the evaluation batch; compared with the static queue, hardware dispatch and the sequential
-    single-stream replay. Pixels are compared across schedules."""
+    """Config 5: 64 contexts at 16x SSAA with Pareto(1.2) primitive areas, replayed as one merged
+    batch. The tile-cost MLP is trained on the GPU from costs measured on a separate training
+    batch; every schedule (hardware dispatch, work queue and ordered dispatch, each in static and
+    learned order) is timed, the fastest is `value`, and the single-stream replay of the same
+    recordings is the plain baseline. Afterwards the first contexts are rendered by the CPU
+    oracle on parallel threads (the CPU baseline) and compared with the GPU's planes byte for
+    byte (`bit_exact`)."""
     import numpy as np
     import pgs
 
     seed_train, seed = 0x5EED0015, 0x5EED0005
     n_ctx = 64
+    threads = max(1, min(16, os.cpu_count() or 1))
     common = dict(ssaa_choices=(16,), pareto_alpha=1.2, tex_psms=(0x13,), blend_presets=(0,))
     t0 = time.perf_counter()
+    if args.quick:
+        # Profiling runs: only the evaluation batch, autotuned, under the hardware schedule.
+        specs = pgs.batched_specs(seed, n_ctx, **common)
+        ctxs = record_contexts(specs)
+        batch = pgs.Batch(ctxs)
+        table = batch.autotune(2)
+        best_tuning = min(table, key=lambda r: r[2])
+        ms = time_batch(batch, args.steps, args.warmup)
+        kms, kn = batch.profile()
+        batch.close()
+        for c in ctxs:
+            c.close()
+        pixels = sum(sp["width"] * sp["height"] for sp in specs)
+        return {"metric": METRIC, "value": round(pixels / (ms / 1e3) / 1e6, 2), "unit": "Mpix/s",
+                "ms_per_step": round(ms, 4), "dtype": "fp32+int",
+                "config": {"workload": "config 5 evaluation batch, autotuned, hardware schedule only (profiling run)",
+                           "tuning": {"tile_log2_1x": best_tuning[0], "waves_per_group": best_tuning[1]}},
+                "kernels_ms": {KIND_NAMES[i]: round(kms[i] / kn[i], 4) for i in range(len(KIND_NAMES)) if kn[i]},
+                "roofline": None, "cpu_baseline": None}
     train_ctxs = record_contexts(pgs.batched_specs(seed_train, n_ctx, **common))
     train_batch = pgs.Batch(train_ctxs)
     feats, costs = train_batch.collect_costs()
@@ -402,6 +472,7 @@ def bench_learned(args):
     rng = np.random.default_rng(1)
     pick = rng.choice(feats.shape[0], size=min(200000, feats.shape[0]), replace=False)
     model = pgs.CostModel(7)
+    untrained = pgs.CostModel(7)
     t1 = time.perf_counter()
     mse = model.train(feats[pick], costs[pick], epochs=8, batch=1024, lr=2e-3, seed=3)
     train_s = time.perf_counter() - t1
@@ -411,72 +482,103 @@ def bench_learned(args):
     specs = pgs.batched_specs(seed, n_ctx, **common)
     ctxs = record_contexts(specs)
     prep_s = time.perf_counter() - t0
-
     seq = pgs.Batch(ctxs, sequential=True)
     ms_plain = time_batch(seq, args.steps, args.warmup)
     seq.close()
 
     batch = pgs.Batch(ctxs)
+    table = batch.autotune(2)
+    best_tuning = min(table, key=lambda r: r[2])
     eval_feats, eval_costs = batch.collect_costs()
     rho = spearman(model.predict(eval_feats), eval_costs)
     times, planes = {}, {}
+    check = (0, n_ctx // 2, n_ctx - 1)
     for name in ["hardware", "static", "learned", "dispatch_static", "dispatch_learned"]:
         batch.set_schedule(name, model if "learned" in name else None)
         times[name] = time_batch(batch, args.steps, args.warmup)
         batch.sync()
-        planes[name] = [ctxs[i].vram_planes() for i in (0, n_ctx // 2, n_ctx - 1)]
-    exact = all(np.array_equal(a, b) for name in ("static", "learned", "dispatch_static", "dispatch_learned")
-                for a, b in zip(planes["hardware"], planes[name]))
-    batch.set_schedule("learned", model)
+        planes[name] = [ctxs[i].vram_planes() for i in check]
+    # The learned queue with the untrained network: what training gains.
+    batch.set_schedule("learned", untrained)
+    times["learned_untrained"] = time_batch(batch, args.steps, args.warmup)
+    exact_schedules = all(np.array_equal(a, b) for name in ("static", "learned", "dispatch_static", "dispatch_learned")
+                          for a, b in zip(planes["hardware"], planes[name]))
+    best = min(("hardware", "static", "learned", "dispatch_static", "dispatch_learned"), key=lambda k: times[k])
+    batch.set_schedule(best, model if "learned" in best else None)
+    ms = time_batch(batch, args.steps, args.warmup)
     kms, kn = batch.profile()
     prof = {KIND_NAMES[i]: (kms[i], kn[i]) for i in range(len(KIND_NAMES)) if kn[i]}
     launches = batch.num_launches
     batch.close()
+
+    # GPU planes of the contexts the CPU baseline renders (each context's VRAM is its recording's
+    # result after every replay).
+    sample_n = threads
+    gpu_planes = [ctxs[i].vram_planes() for i in range(sample_n)]
+    live_s = live_parallel(ctxs, specs, threads)
     for c in ctxs:
         c.close()
 
-    ms = times["learned"]
     pixels = sum(sp["width"] * sp["height"] for sp in specs)
     value = pixels / (ms / 1e3) / 1e6
+    # Dominant kernel: the 16x shading (one merged launch). Algorithmic bytes: every plane of
+    # colour and depth of every context read and written once per pixel (2 + 16 planes x 8 B x 2),
+    # plus setup and attributes read once per primitive (144 B).
     k_ms, k_n = prof["shade_super"]
     bytes_total = sum(sp["width"] * sp["height"] * (2 + sp["ssaa"]) * 16 + sp["n_prims"] * 144 for sp in specs)
     achieved = bytes_total / k_n / (k_ms / k_n / 1e3) / 1e9
+    kernel = {"hardware": "k_shade_ssaa_22_multi", "static": "k_shade_ssaa_22_queue",
+              "learned": "k_shade_ssaa_22_queue", "dispatch_static": "k_shade_ssaa_22_ordered",
+              "dispatch_learned": "k_shade_ssaa_22_ordered"}[best]
 
-    from oracle_lib import Oracle
-    cpu_s = 0.0
-    sp = specs[0]
-    o = Oracle(4)
-    st = pgs.spec_stream(sp)
-    t0 = time.perf_counter()
-    o.gif_transfer(st)
-    o.flush()
-    cpu_s = time.perf_counter() - t0
+    # CPU baseline: the oracle over the first `threads` contexts, one per thread, and context 0
+    # alone on one thread.
+    cpu_wall, cpu_planes = oracle_contexts_parallel(specs[:sample_n], threads)
+    t2 = time.perf_counter()
+    oracle_contexts_parallel(specs[:1], 1)
+    cpu1_s = time.perf_counter() - t2
+    sample_px = sum(sp["width"] * sp["height"] for sp in specs[:sample_n])
+    mismatched = [i for i in range(sample_n) if not np.array_equal(gpu_planes[i], cpu_planes[i])]
     return {
         "metric": METRIC,
         "value": round(value, 2),
         "unit": "Mpix/s",
         "ms_per_step": round(ms, 4),
         "dtype": "fp32+int (tile-cost MLP: bf16 weights, fp32 accumulation)",
-        "config": {"workload": "learned schedule: 64 contexts, 16x SSAA, Pareto(1.2) primitive areas, N log-uniform "
-                               "500..50000, PSMT8 + CLUT, (Cs-Cd)*As>>7+Cd", "seed": seed, "train_seed": seed_train,
-                   "contexts": n_ctx, "primitives": sum(s_["n_prims"] for s_ in specs), "merged_launches": launches},
+        "config": {"workload": "config 5, learned schedule: 64 contexts, 16x SSAA, 640/512x448, Pareto(1.2) "
+                               "primitive areas, N log-uniform 500..50000, PSMT8 + CLUT, (Cs-Cd)*As>>7+Cd",
+                   "seed": seed, "train_seed": seed_train, "contexts": n_ctx,
+                   "primitives": sum(s_["n_prims"] for s_ in specs), "merged_launches": launches,
+                   "schedule": best, "tuning": {"tile_log2_1x": best_tuning[0], "waves_per_group": best_tuning[1]}},
+        "bit_exact": not mismatched,
+        "bit_exact_checked": {"contexts": sample_n, "bytes_per_context": int(gpu_planes[0].size),
+                              "mismatched_contexts": mismatched, "schedules_bit_exact": bool(exact_schedules)},
         "plain_single_stream": {"value": round(pixels / (ms_plain / 1e3) / 1e6, 2), "unit": "Mpix/s",
                                 "ms_per_step": round(ms_plain, 4)},
         "speedup_vs_plain": round(ms_plain / ms, 3),
         "schedules_ms": {k: round(v, 4) for k, v in times.items()},
         "learned_gain_vs_static_queue": round(times["static"] / times["learned"], 4),
+        "learned_gain_vs_untrained": round(times["learned_untrained"] / times["learned"], 4),
         "learned_gain_vs_hardware": round(times["hardware"] / times["learned"], 4),
-        "schedules_bit_exact": bool(exact),
         "mlp": {"train_samples": int(pick.size), "collected": int(feats.shape[0]), "train_s": round(train_s, 3),
                 "train_mse": round(mse, 5), "cost_var": round(float(np.var(costs[pick])), 5),
                 "eval_spearman": round(rho, 4)},
         "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_shade_ssaa_22_queue",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": MEASURED_TRAFFIC.get(("learned", kernel)),
+                     "kernel": kernel, "launches": k_n, "avg_launch_ms": round(k_ms / k_n, 4),
+                     "algorithmic_bytes_per_launch": int(bytes_total / k_n),
                      "note": "VALU-issue bound; byte roofline fraction is small by construction"},
-        "cpu_baseline": {"value": round(sp["width"] * sp["height"] / cpu_s / 1e6, 4), "unit": "Mpix/s", "cores": 1,
-                         "kind": "port", "sample": f"context 0 of the evaluation batch ({sp['n_prims']} primitives), "
-                                                   "single-threaded CPU oracle"},
+        "cpu_baseline": {"value": round(sample_px / cpu_wall / 1e6, 4), "unit": "Mpix/s", "cores": threads,
+                         "kind": "port",
+                         "sample": f"first {sample_n} contexts of the batch ({sum(s_['n_prims'] for s_ in specs[:sample_n])} "
+                                   f"primitives), CPU oracle, one context per thread",
+                         "single_core": round(specs[0]["width"] * specs[0]["height"] / cpu1_s / 1e6, 4),
+                         "single_core_sample": f"context 0 ({specs[0]['n_prims']} primitives), one thread"},
+        "live_ms": round(live_s * 1e3, 2),
+        "live_value": round(pixels / live_s / 1e6, 2),
+        "live_note": f"steady state: all {n_ctx} contexts parse their GIF streams through gif_transfer and flush "
+                     f"(no recording), one context per host thread ({threads} threads), second of two passes",
         "prep_s": round(prep_s, 2),
     }
 
@@ -519,10 +621,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="textured", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="learned", choices=sorted(WORKLOADS))
     ap.add_argument("--texture-stream", action="store_true",
                     help="run copies, CLUT loads and decodes on a second stream beside setup and binning")
     ap.add_argument("--no-graph", action="store_true", help="time eager replays instead of hipGraph replays")
+    ap.add_argument("--quick", action="store_true",
+                    help="learned workload: time the evaluation batch under hardware dispatch only (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported; GS contexts are independent and this build "

@@ -302,6 +302,21 @@ int pgs_batch_profile(pgs_batch *batch, double *ms, uint64_t *launches, uint32_t
 #define PGS_SCHEDULE_DISPATCH_LEARNED 4u
 typedef struct pgs_cost_model pgs_cost_model;
 int pgs_batch_set_schedule(pgs_batch *batch, uint32_t schedule, const pgs_cost_model *model);
+/* What the current schedule's order holds, after its planning replay (the first replay after
+ * pgs_batch_set_schedule): shading launches, tiles with work, entries (queue items or dispatch
+ * workgroups), tiles the learned queue split beyond their base items (heavy tiles), and tiles
+ * the learned dispatch order starts first on their XCD. Returns PGS_ERROR_INVALID_ARGUMENT under
+ * the hardware schedule or before the planning replay. */
+typedef struct pgs_schedule_stats
+{
+	uint32_t shading_launches;
+	uint32_t tiles;
+	uint32_t entries;
+	uint32_t split_tiles;
+	uint32_t heavy_first;
+} pgs_schedule_stats;
+int pgs_batch_schedule_stats(pgs_batch *batch, pgs_schedule_stats *stats);
+
 /* One replay with the static work queue measuring every shading workgroup: writes up to
  * `max_items` feature rows (PGS_MLP_INPUTS floats) and log2 shader-clock costs; returns the
  * number of items written (or a negative error). */

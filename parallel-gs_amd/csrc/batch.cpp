@@ -515,6 +515,18 @@ void Batch::plan_dispatch(const Merged &m, Plan &plan, const float *features, co
 				entries[x].push_back((i << 12) | k);
 		longest = std::max(longest, entries[x].size());
 	}
+	plan.tiles = 0;
+	plan.heavy = 0;
+	for (uint32_t x = 0; x < kXcds; x++)
+		plan.tiles += uint32_t(seq[x].size());
+	if (pred)
+		for (uint32_t x = 0; x < kXcds; x++)
+		{
+			const uint32_t slots = std::max(1u, pgs_queue_slots(m.rate_x, m.rate_y, tuning_.waves_per_group, m.tiles) / kXcds);
+			const double heavy = 0.5 * load[x] / slots;
+			for (uint32_t i : seq[x])
+				plan.heavy += weight(i) > heavy ? 1u : 0u;
+		}
 	const size_t grid = longest * kXcds;
 	if (grid > 0x7fffffffu)
 		throw std::runtime_error("dispatch plan: too many workgroups");
@@ -691,6 +703,45 @@ uint32_t Batch::collect_costs(float *features_out, float *log2_costs, uint32_t m
 		}
 	}
 	return written;
+}
+
+bool Batch::schedule_stats(ScheduleStats &out)
+{
+	out = {};
+	if (schedule_ == Schedule::Hardware || !planned_)
+		return false;
+	const bool dispatch = schedule_ == Schedule::DispatchStatic || schedule_ == Schedule::DispatchLearned;
+	for (size_t k = 0; k < merged_.size(); k++)
+	{
+		const Merged &m = merged_[k];
+		if (!(m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper) || m.tiles == 0)
+			continue;
+		out.shading_launches++;
+		if (dispatch)
+		{
+			out.tiles += plans_[k].tiles;
+			out.heavy_first += plans_[k].heavy;
+			out.entries += plans_[k].grid;
+			continue;
+		}
+		// Queue: weights per tile (run_weight) against the base items of its waves.
+		const uint8_t *base = sched_ + m.sched_offset;
+		const uint32_t *tile_waves = reinterpret_cast<const uint32_t *>(base + size_t(m.tiles) * (PGS_MLP_INPUTS + 2u) * 4u);
+		const uint32_t *run_weight = tile_waves + 2u * size_t(m.tiles);
+		std::vector<uint32_t> w(m.tiles), rw(m.tiles);
+		ok(hipMemcpy(w.data(), tile_waves, m.tiles * 4u, hipMemcpyDeviceToHost));
+		ok(hipMemcpy(rw.data(), run_weight, m.tiles * 4u, hipMemcpyDeviceToHost));
+		for (uint32_t i = 0; i < m.tiles; i++)
+		{
+			if (!rw[i])
+				continue;
+			out.tiles++;
+			out.entries += rw[i];
+			const uint32_t base_items = std::min(16u, std::max(1u, (w[i] + 15u) / 16u));
+			out.split_tiles += rw[i] > base_items ? 1u : 0u;
+		}
+	}
+	return true;
 }
 
 uint32_t Batch::num_launches() const

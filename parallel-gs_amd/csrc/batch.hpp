@@ -57,6 +57,14 @@ public:
 	// One replay with the static work queue that measures every shading workgroup: features
 	// (PGS_MLP_INPUTS floats) and log2 shader cycles per item. Returns the item count.
 	uint32_t collect_costs(float *features, float *log2_costs, uint32_t max_items);
+	// What the current schedule's plan holds (after the planning replay): tiles with work,
+	// queue items or dispatch workgroups, queue tiles split beyond their base items, dispatch
+	// tiles started first. False before the first replay of a planned schedule.
+	struct ScheduleStats
+	{
+		uint32_t shading_launches = 0, tiles = 0, entries = 0, split_tiles = 0, heavy_first = 0;
+	};
+	bool schedule_stats(ScheduleStats &out);
 	void sync();
 	hipStream_t stream() const { return stream_; }
 	// Kernel launches per replay (a shading run counts once under the hardware schedule).
@@ -112,6 +120,8 @@ private:
 		uint32_t *order = nullptr; // device: grid entries, entry k of XCD x at 8 k + x
 		uint32_t grid = 0;
 		size_t capacity = 0;
+		uint32_t tiles = 0;        // tiles with work
+		uint32_t heavy = 0;        // learned: tiles started first on their XCD
 	};
 	std::vector<Plan> plans_;
 	void free_plans();

@@ -592,6 +592,23 @@ int pgs_batch_set_schedule(pgs_batch *batch, uint32_t schedule, const pgs_cost_m
 	});
 }
 
+int pgs_batch_schedule_stats(pgs_batch *batch, pgs_schedule_stats *stats)
+{
+	return batch_guard(batch, [&] {
+		if (!stats)
+			return PGS_ERROR_INVALID_ARGUMENT;
+		pgs::Batch::ScheduleStats s;
+		if (!batch->batch.schedule_stats(s))
+			return PGS_ERROR_INVALID_ARGUMENT;
+		stats->shading_launches = s.shading_launches;
+		stats->tiles = s.tiles;
+		stats->entries = s.entries;
+		stats->split_tiles = s.split_tiles;
+		stats->heavy_first = s.heavy_first;
+		return 0;
+	});
+}
+
 int pgs_batch_collect_costs(pgs_batch *batch, float *features, float *log2_costs, uint32_t max_items)
 {
 	return batch_guard(batch, [&] {

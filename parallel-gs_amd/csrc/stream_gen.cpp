@@ -155,15 +155,15 @@ uint64_t scissor_reg(uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1)
 	return uint64_t(x0) | uint64_t(x1) << 16 | uint64_t(y0) << 32 | uint64_t(y1) << 48;
 }
 
-std::vector<uint8_t> *g_last = nullptr;
+// The last stream generated on each thread (generators may run on several threads at once).
+thread_local std::vector<uint8_t> g_last;
 
 const uint8_t *publish(Stream &s, size_t *size)
 {
-	delete g_last;
-	g_last = new std::vector<uint8_t>(s.q.size() * 8);
-	memcpy(g_last->data(), s.q.data(), g_last->size());
-	*size = g_last->size();
-	return g_last->data();
+	g_last.assign(s.q.size() * 8, 0);
+	memcpy(g_last.data(), s.q.data(), g_last.size());
+	*size = g_last.size();
+	return g_last.data();
 }
 }
 
@@ -172,7 +172,7 @@ extern "C"
 // VRAM transfer microbenchmark: n_h2l host->local uploads cycling through 12 PSMs with
 // random w,h in 8..max_dim (multiples of 8), interleaved with one local->local copy after
 // every `h2l_per_l2l` uploads (same PSM, page-overlapping source and destination).
-// Returns a pointer to the stream (valid until the next generator call).
+// Returns a pointer to the stream (valid until the next generator call on the same thread).
 const uint8_t *pgs_gen_transfers(uint64_t seed, uint32_t n_h2l, uint32_t n_l2l, uint32_t max_dim, size_t *size)
 {
 	static const uint32_t psms[12] = { CT32, CT24, CT16, CT16S, T8, T4, T8H, T4HL, T4HH, Z32, Z24, Z16 };

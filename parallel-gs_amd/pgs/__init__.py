@@ -128,7 +128,7 @@ EXPORTED_SYMBOLS = [
     "pgs_cost_model_create", "pgs_cost_model_destroy", "pgs_cost_model_set_params", "pgs_cost_model_get_params",
     "pgs_cost_model_predict", "pgs_cost_model_gradients", "pgs_cost_model_train",
     "pgs_k_mlp_forward", "pgs_k_mlp_gradients", "pgs_k_mlp_gradients_noalloc", "pgs_k_mlp_adam", "pgs_k_mlp_quantize",
-    "pgs_batch_set_schedule", "pgs_batch_collect_costs", "pgs_k_shade_queue", "pgs_k_tile_features",
+    "pgs_batch_set_schedule", "pgs_batch_collect_costs", "pgs_batch_schedule_stats", "pgs_k_shade_queue", "pgs_k_tile_features",
     "pgs_k_queue_order", "pgs_k_fill_u32", "pgs_k_shade_ordered", "pgs_k_sampler_feedback", "pgs_k_restore", "pgs_k_fused",
 ]
 
@@ -507,6 +507,18 @@ class Batch:
         self._model = model  # kept alive with the batch
         if self._L.pgs_batch_set_schedule(self._h, self.SCHEDULES[name], model._h if model else None) != 0:
             raise PgsError("pgs_batch_set_schedule failed")
+
+    def schedule_stats(self):
+        """Contents of the current schedule's order after its planning replay (dict), or None
+        under the hardware schedule / before planning."""
+        class _S(ctypes.Structure):
+            _fields_ = [(n, ctypes.c_uint32) for n in ("shading_launches", "tiles", "entries", "split_tiles",
+                                                        "heavy_first")]
+        self._L.pgs_batch_schedule_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(_S)]
+        s = _S()
+        if self._L.pgs_batch_schedule_stats(self._h, ctypes.byref(s)) != 0:
+            return None
+        return {n: int(getattr(s, n)) for n, _ in _S._fields_}
 
     def collect_costs(self, max_items: int = 1 << 22):
         fp = ctypes.POINTER(ctypes.c_float)
