@@ -27,7 +27,11 @@ KIND_NAMES = ["vram_copy", "vram_snapshot", "triangle_setup", "binning", "shade"
 # 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), and the kernel's waves and VALU
 # instructions per wave from an SQ_WAVES / SQ_INSTS_VALU run; the runs are summarised in
 # profiles/hbm_traffic_pmc__plain_and_textured4x.md.
-MEASURED_TRAFFIC = {"plain": 6915392, "textured": 20951962, "batched": 1537190605}
+MEASURED_TRAFFIC = {"plain": 6915392, "textured": 20951962, "batched": 1537190605,
+                    ("learned", "k_shade_ssaa_22_multi"): 2268539747}
+# Config 5's dominant kernel under hardware dispatch (profiles/shade_ssaa22_multi__config5_64ctx_16x.md):
+# waves, VALU instructions per wave and the share of wave cycles spent waiting.
+MEASURED_SQ = {("learned", "k_shade_ssaa_22_multi"): {"waves": 4085760, "valu_per_wave": 1376, "wait_frac": 0.329}}
 MEASURED_VALU = {"plain": (4480, 10676), "textured": (17920, 2011)}  # (waves, VALU instructions per wave)
 # MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a SIMD for 4 cycles at 2.4 GHz.
 VALU_ISSUE_S_PER_WAVE_INST = 4.0 / 1024 / 2.4e9
@@ -568,7 +572,10 @@ def bench_learned(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": MEASURED_TRAFFIC.get(("learned", kernel)),
                      "kernel": kernel, "launches": k_n, "avg_launch_ms": round(k_ms / k_n, 4),
                      "algorithmic_bytes_per_launch": int(bytes_total / k_n),
-                     "note": "VALU-issue bound; byte roofline fraction is small by construction"},
+                     "sq": MEASURED_SQ.get(("learned", kernel)),
+                     "note": "VALU-issue bound (byte roofline fraction is small by construction): the measured VALU "
+                             "instructions at 4 cycles per wave64 instruction over 1024 SIMDs at 2.4 GHz take as long "
+                             "as the launch; traffic from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0"},
         "cpu_baseline": {"value": round(sample_px / cpu_wall / 1e6, 4), "unit": "Mpix/s", "cores": threads,
                          "kind": "port",
                          "sample": f"first {sample_n} contexts of the batch ({sum(s_['n_prims'] for s_ in specs[:sample_n])} "
