@@ -76,6 +76,14 @@ int pgs_k_restore(const pgs_restore_args *args, hipStream_t stream);
  * (gs/gs_renderer.cpp:3600-3660). Must run once per device before any setup launch. */
 int pgs_k_init_tables(const uint8_t *rcp_seed, const float *rcp_float_lut);
 
+/* Test hook (no counterpart in the reference): evaluates the device reciprocals triangle setup
+ * uses — rcp_fixed (mode 0, the fixed-point Newton reciprocal of math_utils.h:27-61) or rcp_float
+ * (mode 1, inputs are float bits; math_utils.h:65-101) — over `count` consecutive inputs from
+ * `first` (at most 2^32) and adds the wrapping 64-bit sum over inputs of
+ * ((output bits << 32) | input) * 0x9E3779B97F4A7C15 to *hash (device memory, cleared by the
+ * caller). Needs pgs_k_init_tables. Lets a test compare the device arithmetic with a CPU sweep. */
+int pgs_k_rcp_eval(uint32_t mode, uint32_t first, uint64_t count, uint64_t *hash, hipStream_t stream);
+
 /* Triangle / sprite / line / point setup: one thread per primitive.
  * Replaces triangle_setup.comp dispatched by GSRenderer::dispatch_triangle_setup
  * (gs/gs_renderer.cpp:1898-1977). Writes PrimitiveSetup (80 B) and TransformedAttributes

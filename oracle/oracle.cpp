@@ -14,6 +14,8 @@
 #include <cstring>
 #include <cstdio>
 #include <algorithm>
+#include <cmath>
+#include <limits>
 
 namespace orc
 {
@@ -696,8 +698,55 @@ void orc_set_sample_log2(void *g, uint32_t sample_log2)
 }
 }
 
+// Exhaustive sweeps of the reciprocals for the tests: the hash is the wrapping sum over inputs of
+// ((output bits << 32) | input bits) * 0x9E3779B97F4A7C15, which the device evaluation kernel
+// (pgs_k_rcp_eval) computes the same way; max_ulp is the largest |result - 1/x| in units of the
+// last place of the correctly rounded float 1/x.
+static double ulps_off(float r, double exact)
+{
+	const float f = float(exact);
+	const double ulp = double(std::nextafter(f, std::numeric_limits<float>::infinity())) - double(f);
+	return std::fabs(double(r) - exact) / ulp;
+}
+
 extern "C"
 {
+void orc_rcp_fixed_sweep(uint32_t first, uint64_t count, uint64_t *hash, double *max_ulp)
+{
+	uint64_t h = 0;
+	double worst = 0.0;
+	for (uint64_t k = 0; k < count; k++)
+	{
+		const uint32_t b = uint32_t(first + k);
+		const float r = orc::rcp_fixed(b);
+		uint32_t bits;
+		memcpy(&bits, &r, 4);
+		h += ((uint64_t(bits) << 32) | b) * 0x9E3779B97F4A7C15ull;
+		worst = std::max(worst, ulps_off(r, 1.0 / double(b)));
+	}
+	*hash = h;
+	*max_ulp = worst;
+}
+
+void orc_rcp_float_sweep(uint32_t first_bits, uint64_t count, uint64_t *hash, double *max_ulp)
+{
+	uint64_t h = 0;
+	double worst = 0.0;
+	for (uint64_t k = 0; k < count; k++)
+	{
+		const uint32_t u = uint32_t(first_bits + k);
+		float v;
+		memcpy(&v, &u, 4);
+		const float r = orc::rcp_float(v);
+		uint32_t bits;
+		memcpy(&bits, &r, 4);
+		h += ((uint64_t(bits) << 32) | u) * 0x9E3779B97F4A7C15ull;
+		worst = std::max(worst, ulps_off(r, 1.0 / double(v)));
+	}
+	*hash = h;
+	*max_ulp = worst;
+}
+
 const uint8_t *orc_rcp_table() { return orc::rcp_table(); }
 float orc_rcp_fixed(uint32_t v) { return orc::rcp_fixed(v); }
 float orc_rcp_float(float v) { return orc::rcp_float(v); }

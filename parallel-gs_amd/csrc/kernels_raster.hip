@@ -73,6 +73,26 @@ __device__ float rcp_float(float v)
 	return ldexpf(x, 127 - e);
 }
 
+// Test hook (pgs_k_rcp_eval): evaluates rcp_fixed (mode 0) or rcp_float (mode 1, inputs are float
+// bits) over `count` consecutive inputs from `first` and adds the wrapping sum of
+// ((output bits << 32) | input) * 0x9E3779B97F4A7C15 to *hash.
+__global__ void __launch_bounds__(256) k_rcp_eval(uint32_t mode, uint32_t first, uint64_t count,
+                                                  unsigned long long *hash)
+{
+	uint64_t h = 0;
+	const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+	for (uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < count; k += stride)
+	{
+		const uint32_t in = first + uint32_t(k);
+		const float r = mode ? rcp_float(__uint_as_float(in)) : rcp_fixed(in);
+		h += ((uint64_t(__float_as_uint(r)) << 32) | in) * 0x9E3779B97F4A7C15ull;
+	}
+	for (int m = 32; m >= 1; m >>= 1)
+		h += __shfl_xor(h, m);
+	if ((threadIdx.x & 63u) == 0)
+		atomicAdd(hash, (unsigned long long)h);
+}
+
 __device__ __forceinline__ bool tie_break(int32_t ex, int32_t ey) { return ey > 0 || (ey == 0 && ex < 0); }
 __device__ __forceinline__ bool order_less(int32_t ax, int32_t ay, int32_t bx, int32_t by, uint32_t az, uint32_t bz)
 {
@@ -946,6 +966,18 @@ extern "C" int pgs_k_init_tables(const uint8_t *seed, const float *lut)
 	if (e == hipSuccess)
 		e = hipMemcpyToSymbol(HIP_SYMBOL(pgs::c_rcp_float_lut), lut, 256 * sizeof(float));
 	return e == hipSuccess ? 0 : -int(e);
+}
+
+extern "C" int pgs_k_rcp_eval(uint32_t mode, uint32_t first, uint64_t count, uint64_t *hash, hipStream_t stream)
+{
+	if (!hash || mode > 1 || count > (uint64_t(1) << 32))
+		return -int(hipErrorInvalidValue);
+	if (count == 0)
+		return 0;
+	const uint64_t blocks = std::min<uint64_t>(4096, (count + 255u) / 256u);
+	hipLaunchKernelGGL(pgs::k_rcp_eval, dim3(uint32_t(blocks)), dim3(256), 0, stream, mode, first, count,
+	                   reinterpret_cast<unsigned long long *>(hash));
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }
 
 uint32_t pgs_grid_triangle_setup(const pgs_setup_args &a) { return (a.num_primitives + 63u) / 64u; }
