@@ -210,6 +210,58 @@ int pgs_set_signal_interface(pgs_context *ctx, const pgs_signal_interface *iface
  * with HIP events around each launch. Negative on error. */
 double pgs_get_accumulated_timestamps(pgs_context *ctx, uint32_t type);
 
+/* VSyncInfo (gs/gs_interface.hpp:155-198) without the Vulkan image layout / stage / access
+ * fields: the scanout is a device buffer ordered on pgs_stream(ctx). Booleans are 0 / 1. */
+typedef struct pgs_vsync_info
+{
+	uint32_t phase;                 /* field: 0 top, 1 bottom */
+	uint32_t force_progressive;
+	uint32_t overscan;
+	uint32_t anti_blur;
+	uint32_t crtc_offsets;
+	uint32_t adapt_to_internal_horizontal_resolution;
+	uint32_t raw_circuit_scanout;
+	uint32_t high_resolution_scanout;
+	uint32_t skip_deinterlace;
+} pgs_vsync_info;
+
+/* ScanoutResult (gs/gs_renderer.hpp:23-52). `image` is a device pointer to width x height
+ * RGBA8 pixels (R in the low byte, row pitch width * 4 bytes), owned by the context and valid
+ * until its next pgs_vsync or pgs_destroy; it is NULL (and everything else 0) when the video
+ * mode is unknown. */
+typedef struct pgs_scanout
+{
+	const void *image;
+	uint32_t width, height;
+	uint32_t internal_width, internal_height; /* single-sampled domain */
+	uint32_t mode_width, mode_height;
+	uint32_t high_resolution_scanout;
+	uint32_t interlaced;
+	uint32_t double_strike;
+	uint32_t interlace_phase;
+	float sampling_rate_mhz;
+} pgs_scanout;
+
+/* GSInterface::vsync (gs/gs_interface.cpp:4626-4690, gs/gs_renderer.cpp:4189-5055): scans out
+ * the privileged registers' display circuits (pgs_set_priv_register_state) into a frame:
+ * read circuits (super-sample aware at high resolution), merge over BGCOLOR, EXTWRITE feedback
+ * into VRAM, and FastMAD deinterlacing with a four-field history. Pending transfers and
+ * primitives are flushed first. The kernels run asynchronously on pgs_stream(ctx).
+ * Returns 0, a negative hipError_t, or PGS_ERROR_UNSUPPORTED for a request that would turn
+ * on the reference's field-aware super-sampled rendering (high_resolution_scanout of an
+ * interlaced FFMD = 1 mode at 2x or more), for an enabled read circuit whose DISPFB.PSM is not
+ * PSMCT32/24/16/16S, and while recording. */
+int pgs_vsync(pgs_context *ctx, const pgs_vsync_info *info, pgs_scanout *result);
+
+/* GSInterface::vsync_can_skip (gs/gs_interface.cpp:4692-4695): 1 when a frame with no new
+ * work may reuse the previous scanout (the output is not interlaced and no scan-masked
+ * primitive asked for field order), else 0; negative on error. */
+int pgs_vsync_can_skip(pgs_context *ctx, const pgs_vsync_info *info);
+
+/* Copies the last scanout image to host memory (width * height * 4 bytes). Blocks until the
+ * scanout kernels have finished. Backend extension: the reference hands out a Vulkan image. */
+int pgs_read_scanout(pgs_context *ctx, void *dst, size_t size);
+
 /* Number of 4 MiB VRAM planes: 1 at 1x, 2 + samples with super-sampling
  * (plane layout of GSRenderer::init_vram, gs/gs_renderer.cpp:360-427). */
 uint32_t pgs_num_planes(pgs_context *ctx);

@@ -453,6 +453,85 @@ typedef struct pgs_fused_args
 
 int pgs_k_fused(const pgs_fused_args *args, hipStream_t stream);
 
+/* ---- CRTC scanout (vsync) --------------------------------------------------------------
+ * Images are RGBA8 (R in the low byte), row pitch = width. Every float step rounds once
+ * (division, multiplication, addition in the order written in the kernel source), so the
+ * results are defined bit for bit. */
+
+/* Reads one read circuit's DISPFB region into an image.
+ * Replaces sample_crtc_circuit + sample_circuit.frag (gs/gs_renderer.cpp:4055-4112,
+ * gs/shaders/sample_circuit.frag:40-148). Pixels outside the valid extent are 0. At 4x, 8x
+ * and 16x (high-resolution scanout) each 2x2 quad of image pixels reads one single-sampled
+ * pixel and averages the quad's share of its super-samples where the reference plane marks
+ * them valid. */
+typedef struct pgs_scanout_circuit_args
+{
+	const uint8_t *vram;       /* plane 0 = VRAM, 1 = super-sample reference, 2.. = samples */
+	uint32_t *image;
+	uint32_t width, height;    /* image extent */
+	uint32_t valid_width, valid_height;
+	uint32_t psm;              /* DISPFB.PSM: PSMCT32/24/16/16S or PSMZ32/24/16/16S */
+	uint32_t base_block;       /* DISPFB.FBP x 32 */
+	uint32_t fbw, dbx, dby;
+	uint32_t phase, phase_stride;
+	uint32_t super_samples;    /* 1, 4, 8 or 16 */
+} pgs_scanout_circuit_args;
+
+int pgs_k_scanout_circuit(const pgs_scanout_circuit_args *args, hipStream_t stream);
+
+/* Merges the read circuits over the background colour into one frame.
+ * Replaces the merge render pass of GSRenderer::vsync (gs/gs_renderer.cpp:4760-4905) with
+ * blit_circuit.frag's linear-clamp sampling: circuit 2 is copied first (SLBG = 0), then
+ * circuit 1 is drawn over it replacing (MMOD = 1, ALP = 255), blended with the constant ALP
+ * (MMOD = 1) or with its own alpha (MMOD = 0). The filter takes 8-bit sub-texel weights
+ * (the weight of texel i0 + 1 is round(256 * frac) / 256). */
+typedef struct pgs_scanout_merge_args
+{
+	uint32_t *merged;
+	uint32_t width, height;
+	const uint32_t *circuit[2];
+	uint32_t circuit_width[2], circuit_height[2];
+	int32_t vp_x[2], vp_y[2];
+	uint32_t vp_width[2], vp_height[2];
+	uint32_t draw;             /* bit 0: circuit 1, bit 1: circuit 2 */
+	uint32_t blend;            /* circuit 1: 0 replace, 1 constant ALP, 2 source alpha */
+	uint32_t alp;              /* PMODE.ALP */
+	uint32_t bgcolor;          /* BGCOLOR: R | G << 8 | B << 16 */
+} pgs_scanout_merge_args;
+
+int pgs_k_scanout_merge(const pgs_scanout_merge_args *args, hipStream_t stream);
+
+/* EXTWRITE feedback: writes a scanout image back into VRAM as PSMCT32.
+ * Replaces extwrite.comp (gs/gs_renderer.cpp:4921-4999, gs/shaders/extwrite.comp). The source
+ * is sampled nearest (texel floor(u * size), clamped); EMODC / EMODA select RGB, luma, YCbCr or
+ * alpha conversions. With clear_reference the super-sample reference word of every written
+ * pixel is cleared. */
+typedef struct pgs_scanout_extwrite_args
+{
+	uint8_t *vram;
+	const uint32_t *view;
+	uint32_t view_width, view_height;
+	uint32_t width, height;    /* pixels written */
+	uint32_t wdx, wdy, exbp, exbw, emoda, emodc;
+	uint32_t clear_reference;
+	float uv_base[2], uv_scale[2];
+} pgs_scanout_extwrite_args;
+
+int pgs_k_scanout_extwrite(const pgs_scanout_extwrite_args *args, hipStream_t stream);
+
+/* FastMAD deinterlace of the last four fields into one frame of twice the field height.
+ * Replaces GSRenderer::fastmad_deinterlace + weave.frag (gs/gs_renderer.cpp:5057-5099,
+ * gs/shaders/weave.frag). field[0] is the newest field. */
+typedef struct pgs_scanout_weave_args
+{
+	const uint32_t *field[4];
+	uint32_t *out;
+	uint32_t width, height;    /* output extent: height = 2 x field height */
+	uint32_t phase;
+} pgs_scanout_weave_args;
+
+int pgs_k_scanout_weave(const pgs_scanout_weave_args *args, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

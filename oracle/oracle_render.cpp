@@ -553,7 +553,12 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 		state |= S_MULTISAMPLE;
 	uint32_t msk = field(regs.scanmsk, 0, 2);
 	if (msk >= 2)
+	{
 		state |= 1u << (23 + msk - 2);
+		// gs/gs_interface.cpp:1947-1960.
+		if (p.bb[1] + 400 < p.bb[3])
+			complex_scanmsk_timeout = 4;
+	}
 	if (!field(prim, 8, 1))
 		state |= S_PERSPECTIVE;
 	if (field(prim, 3, 1))

@@ -3,6 +3,7 @@
 #include "oracle_core.hpp"
 #include <cstdint>
 #include <vector>
+#include <memory>
 
 namespace orc
 {
@@ -81,6 +82,26 @@ PageSpan page_span(uint32_t base_block, uint32_t stride, uint32_t psm, uint32_t 
                    uint32_t y1);
 bool spans_overlap(PageSpan a, PageSpan b);
 
+// VSyncInfo (gs/gs_interface.hpp:155-198) and ScanoutResult (gs/gs_renderer.hpp:23-52) as
+// flat words for ctypes.
+struct VsyncInfo
+{
+	uint32_t phase, force_progressive, overscan, anti_blur, crtc_offsets, adapt_to_internal_horizontal_resolution,
+	    raw_circuit_scanout, high_resolution_scanout, skip_deinterlace;
+};
+struct ScanoutOut
+{
+	uint32_t has_image, width, height, internal_width, internal_height, mode_width, mode_height;
+	uint32_t high_resolution_scanout, interlaced, double_strike, interlace_phase;
+	float sampling_rate_mhz;
+};
+struct ScanImage
+{
+	uint32_t w = 0, h = 0;
+	std::vector<uint32_t> px; // RGBA8, row-major
+	ScanImage(uint32_t w_, uint32_t h_) : w(w_), h(h_), px(size_t(w_) * h_, 0u) {}
+};
+
 class GS
 {
 public:
@@ -102,6 +123,14 @@ public:
 	std::vector<uint8_t> fifo;
 	uint32_t fifo_offset_128 = 0;
 	void read_transfer_fifo(void *data, uint32_t num_128b_words);
+
+	// CRTC scanout (oracle_vsync.cpp). priv: the 8 KiB privileged register block.
+	std::vector<uint8_t> priv = std::vector<uint8_t>(8192, 0);
+	uint32_t complex_scanmsk_timeout = 0; // gs/gs_interface.cpp:1947-1960
+	// Returns 0, or -1000 where the reference would switch to field-aware rendering.
+	int vsync(const VsyncInfo &info, ScanoutOut &out, std::vector<uint32_t> &image);
+	bool vsync_can_skip(const VsyncInfo &info) const;
+	std::shared_ptr<ScanImage> last_fields[4];
 
 private:
 	struct Path { uint32_t nloop = 0, loop = 0, flg = 0, nreg = 0, reg = 0; uint64_t regs = 0; };

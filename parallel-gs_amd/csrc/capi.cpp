@@ -15,6 +15,7 @@ struct pgs_context
 {
 	pgs::GSInterface gs;
 	uint64_t record_clock = 0; // VRAM write-log time at pgs_begin_recording
+	pgs::ScanoutResult last_scanout;
 };
 
 struct pgs_batch
@@ -358,6 +359,73 @@ double pgs_get_accumulated_timestamps(pgs_context *ctx, uint32_t type)
 	if (guard(ctx, [&] { v = ctx->gs.get_accumulated_timestamps(pgs::TimestampType(type)); }) != 0)
 		return -1.0;
 	return v;
+}
+
+namespace
+{
+pgs::VSyncInfo to_vsync_info(const pgs_vsync_info &in)
+{
+	pgs::VSyncInfo v;
+	v.phase = in.phase;
+	v.force_progressive = in.force_progressive != 0;
+	v.overscan = in.overscan != 0;
+	v.anti_blur = in.anti_blur != 0;
+	v.crtc_offsets = in.crtc_offsets != 0;
+	v.adapt_to_internal_horizontal_resolution = in.adapt_to_internal_horizontal_resolution != 0;
+	v.raw_circuit_scanout = in.raw_circuit_scanout != 0;
+	v.high_resolution_scanout = in.high_resolution_scanout != 0;
+	v.skip_deinterlace = in.skip_deinterlace != 0;
+	return v;
+}
+}
+
+int pgs_vsync(pgs_context *ctx, const pgs_vsync_info *info, pgs_scanout *result)
+{
+	if (!ctx || !info || !result)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	int rc = 0;
+	const int g = guard(ctx, [&] {
+		pgs::ScanoutResult r;
+		rc = ctx->gs.vsync(to_vsync_info(*info), r);
+		ctx->last_scanout = r;
+		memset(result, 0, sizeof(*result));
+		result->image = r.image;
+		result->width = r.width;
+		result->height = r.height;
+		result->internal_width = r.internal_width;
+		result->internal_height = r.internal_height;
+		result->mode_width = r.mode_width;
+		result->mode_height = r.mode_height;
+		result->high_resolution_scanout = r.high_resolution_scanout;
+		result->interlaced = r.interlaced;
+		result->double_strike = r.double_strike;
+		result->interlace_phase = r.interlace_phase;
+		result->sampling_rate_mhz = r.sampling_rate_mhz;
+	});
+	if (g != 0)
+		return g;
+	return rc == pgs::GSInterface::kVsyncUnsupported ? PGS_ERROR_UNSUPPORTED : rc;
+}
+
+int pgs_vsync_can_skip(pgs_context *ctx, const pgs_vsync_info *info)
+{
+	if (!ctx || !info)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return ctx->gs.vsync_can_skip(to_vsync_info(*info)) ? 1 : 0;
+}
+
+int pgs_read_scanout(pgs_context *ctx, void *dst, size_t size)
+{
+	if (!ctx || !dst)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	const pgs::ScanoutResult &r = ctx->last_scanout;
+	const size_t bytes = size_t(r.width) * r.height * 4u;
+	if (!r.image || size < bytes)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	hipError_t e = hipStreamSynchronize(ctx->gs.renderer().stream());
+	if (e == hipSuccess)
+		e = hipMemcpy(dst, r.image, bytes, hipMemcpyDeviceToHost);
+	return e == hipSuccess ? 0 : -int(e);
 }
 
 uint32_t pgs_num_planes(pgs_context *ctx) { return ctx ? ctx->gs.renderer().num_planes() : 0; }

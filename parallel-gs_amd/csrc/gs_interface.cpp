@@ -106,6 +106,7 @@ void GSInterface::poll_timestamps(bool wait)
 	add(TimestampType::Shading, ms[4] + ms[8]);
 	add(TimestampType::PaletteUpdate, ms[5]);
 	add(TimestampType::TextureUpload, ms[6] + ms[9]);
+	add(TimestampType::VSync, scanout_.poll_ms(wait));
 }
 
 void GSInterface::set_timestamps(bool enable)
@@ -114,6 +115,62 @@ void GSInterface::set_timestamps(bool enable)
 		poll_timestamps(true);
 	timestamps_ = enable;
 	renderer_.profile_enable(enable);
+	scanout_.set_timing(enable);
+}
+
+namespace
+{
+void rate_log2_xy(uint32_t sample_log2, uint32_t &x, uint32_t &y)
+{
+	// Ordered grids (gs/gs_interface.cpp:65-118): 2x = 1x2, 4x = 2x2, 8x = 2x4, 16x = 4x4.
+	x = sample_log2 >> 1;
+	y = sample_log2 - x;
+}
+}
+
+int GSInterface::vsync(const VSyncInfo &info_in, ScanoutResult &result)
+{
+	result = ScanoutResult{};
+	const CrtcRegisters r = decode_crtc_registers(priv_.data());
+	uint32_t xl, yl;
+	rate_log2_xy(renderer_.sample_log2(), xl, yl);
+	// The reference switches its renderer to field-aware super-sampling here
+	// (gs/gs_interface.cpp:4666-4672, gs/gs_renderer.cpp:1953, 2123, 2605); not built.
+	if (info_in.high_resolution_scanout && yl && r.ffmd && r.interlace && r.cmod != 0)
+		return kVsyncUnsupported;
+	// A read circuit shows PSMCT32/24/16/16S; the reference reads other formats' addresses as
+	// word indices (sample_circuit.frag:50-71), which this backend does not reproduce.
+	auto displayable = [](uint32_t psm) {
+		return psm == PSMCT32 || psm == PSMCT24 || psm == PSMCT16 || psm == PSMCT16S;
+	};
+	for (uint32_t c = 0; c < 2; c++)
+		if (r.en[c] && !displayable(r.dispfb[c].psm))
+			return kVsyncUnsupported;
+	if (renderer_.recording())
+		return kVsyncUnsupported;
+
+	VSyncInfo info = info_in;
+	if (rp_.complex_scanmsk_timeout)
+	{
+		info.force_progressive = false;
+		rp_.complex_scanmsk_timeout--;
+	}
+	flush();
+	const ScanoutPlan plan = plan_scanout(r, info, xl, yl);
+	hipStream_t stream = renderer_.begin_external_work();
+	const int rc = scanout_.run(plan, info, renderer_.vram_device(), renderer_.sample_log2(), stream, result);
+	renderer_.end_external_work(plan.extwrite);
+	if (plan.extwrite)
+		rp_.writes().write(page_span(plan.exbp, plan.exbw, PSMCT32, plan.wdx, plan.wdy, plan.wdx + plan.ext_width - 1u,
+		                             plan.wdy + plan.ext_height - 1u));
+	if (timestamps_)
+		poll_timestamps(false);
+	return rc;
+}
+
+bool GSInterface::vsync_can_skip(const VSyncInfo &info) const
+{
+	return rp_.complex_scanmsk_timeout == 0 && !scanout_is_interlaced(decode_crtc_registers(priv_.data()), info);
 }
 
 double GSInterface::get_accumulated_timestamps(TimestampType type)

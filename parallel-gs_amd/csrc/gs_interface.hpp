@@ -5,6 +5,7 @@
 #include "gs_defs.hpp"
 #include "renderer.hpp"
 #include "render_pass.hpp"
+#include "scanout.hpp"
 #include <array>
 #include <vector>
 #include <cstdint>
@@ -101,6 +102,14 @@ public:
 	void set_sampler_feedback(uint32_t mode) { renderer_.set_sampler_feedback(mode); }
 	double get_accumulated_timestamps(TimestampType type);
 
+	// CRTC scanout (gs/gs_interface.cpp:4626-4695). Flushes pending work first. Returns 0, a
+	// negative hipError_t, or kVsyncUnsupported when the request would switch the renderer to
+	// field-aware super-sampling (high-resolution scanout of a field-rendered interlaced mode at
+	// 2x or more), which this backend does not build.
+	static constexpr int kVsyncUnsupported = -1000;
+	int vsync(const VSyncInfo &info, ScanoutResult &result);
+	bool vsync_can_skip(const VSyncInfo &info) const;
+
 	RegisterState &get_register_state() { return regs_; }
 	GIFPath &get_gif_path(uint32_t path) { return paths_[path & 3u]; }
 	uint8_t *get_priv_register_state() { return priv_.data(); }
@@ -131,6 +140,7 @@ private:
 	void poll_timestamps(bool wait);
 	Renderer renderer_;
 	RenderPassBuilder rp_;
+	Scanout scanout_;
 	RegisterState regs_;
 	GIFPath paths_[4];
 	TransferState xfer_;

@@ -400,7 +400,13 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 		state |= ST_MULTISAMPLE;
 	const uint32_t msk = field(r.scanmsk, 0, 2);
 	if (msk >= 2)
+	{
 		state |= 1u << (23 + msk - 2);
+		// A scan-masked primitive taller than 400 lines means the frame depends on field order:
+		// the next 4 vsyncs do not force progressive output (gs/gs_interface.cpp:1947-1960).
+		if (loy + 400 < hiy)
+			complex_scanmsk_timeout = 4;
+	}
 	if (!field(prim, 8, 1))
 		state |= ST_PERSPECTIVE;
 	if (field(prim, 3, 1))

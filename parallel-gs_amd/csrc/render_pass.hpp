@@ -131,6 +131,10 @@ public:
 	};
 	Host *host = nullptr;
 
+	// Vsyncs left during which progressive output is not forced (set by tall scan-masked
+	// primitives, counted down by GSInterface::vsync).
+	uint32_t complex_scanmsk_timeout = 0;
+
 private:
 	struct QueuedVertex
 	{

@@ -98,6 +98,23 @@ public:
 	// ---- launch recording for graph replay ----------------------------------------------
 	// While recording, uploads are kept alive and every launch is appended to a list that
 	// replay() re-issues (optionally captured into one hipGraph).
+	// Device work issued by the front end itself (the CRTC scanout): returns the stream on which
+	// it runs after every queued transfer and launch; end_external_work() records that it wrote
+	// VRAM, so later texture work on the side stream waits for it.
+	hipStream_t begin_external_work()
+	{
+		flush_transfers();
+		arena_upload();
+		join_texture_stream();
+		return stream_;
+	}
+	void end_external_work(bool wrote_vram)
+	{
+		if (wrote_vram)
+			main_pending_ = true;
+	}
+	bool recording() const { return recording_; }
+
 	void begin_recording();
 	void end_recording();
 	int replay(bool use_graph);

@@ -82,6 +82,22 @@ class SignalInterface(ctypes.Structure):
 
 
 PRIV_REGISTER_BYTES = 8192
+VSYNC_FIELDS = ("phase", "force_progressive", "overscan", "anti_blur", "crtc_offsets",
+                "adapt_to_internal_horizontal_resolution", "raw_circuit_scanout", "high_resolution_scanout",
+                "skip_deinterlace")
+SCANOUT_FIELDS = ("width", "height", "internal_width", "internal_height", "mode_width", "mode_height",
+                  "high_resolution_scanout", "interlaced", "double_strike", "interlace_phase")
+
+
+class VSyncInfo(ctypes.Structure):
+    """pgs_vsync_info."""
+    _fields_ = [(n, ctypes.c_uint32) for n in VSYNC_FIELDS]
+
+
+class Scanout(ctypes.Structure):
+    """pgs_scanout."""
+    _fields_ = [("image", ctypes.c_void_p)] + [(n, ctypes.c_uint32) for n in SCANOUT_FIELDS] + \
+        [("sampling_rate_mhz", ctypes.c_float)]
 TIMESTAMPS = ("sync_host_to_vram", "copy_vram", "palette_update", "texture_upload", "triangle_setup", "binning",
               "shading", "readback", "vsync")
 
@@ -117,7 +133,7 @@ EXPORTED_SYMBOLS = [
     "pgs_get_register_state", "pgs_set_register_state", "pgs_get_gif_path", "pgs_set_gif_path",
     "pgs_get_priv_register_state", "pgs_set_priv_register_state", "pgs_set_super_sampling_rate",
     "pgs_get_super_sampling_rate", "pgs_set_hacks", "pgs_set_debug_mode", "pgs_set_signal_interface",
-    "pgs_get_accumulated_timestamps",
+    "pgs_get_accumulated_timestamps", "pgs_vsync", "pgs_vsync_can_skip", "pgs_read_scanout",
     "pgs_begin_recording", "pgs_end_recording", "pgs_replay", "pgs_sync", "pgs_stream",
     "pgs_profile_enable", "pgs_profile_collect",
     "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
@@ -130,6 +146,7 @@ EXPORTED_SYMBOLS = [
     "pgs_k_mlp_forward", "pgs_k_mlp_gradients", "pgs_k_mlp_gradients_noalloc", "pgs_k_mlp_adam", "pgs_k_mlp_quantize",
     "pgs_batch_set_schedule", "pgs_batch_collect_costs", "pgs_batch_schedule_stats", "pgs_k_shade_queue", "pgs_k_tile_features",
     "pgs_k_queue_order", "pgs_k_fill_u32", "pgs_k_shade_ordered", "pgs_k_sampler_feedback", "pgs_k_restore", "pgs_k_fused", "pgs_k_rcp_eval",
+    "pgs_k_scanout_circuit", "pgs_k_scanout_merge", "pgs_k_scanout_extwrite", "pgs_k_scanout_weave",
 ]
 
 
@@ -186,6 +203,9 @@ def lib() -> ctypes.CDLL:
         L.pgs_set_signal_interface.argtypes = [vp, ctypes.POINTER(SignalInterface)]
         L.pgs_get_accumulated_timestamps.argtypes = [vp, u32]
         L.pgs_get_accumulated_timestamps.restype = ctypes.c_double
+        L.pgs_vsync.argtypes = [vp, ctypes.POINTER(VSyncInfo), ctypes.POINTER(Scanout)]
+        L.pgs_vsync_can_skip.argtypes = [vp, ctypes.POINTER(VSyncInfo)]
+        L.pgs_read_scanout.argtypes = [vp, vp, sz]
         _lib = L
     return _lib
 
@@ -415,6 +435,35 @@ class Context:
 
     def accumulated_timestamps(self) -> dict:
         return {name: float(self._L.pgs_get_accumulated_timestamps(self._h, i)) for i, name in enumerate(TIMESTAMPS)}
+
+    @staticmethod
+    def _vsync_info(info: dict) -> VSyncInfo:
+        bad = set(info) - set(VSYNC_FIELDS)
+        if bad:
+            raise TypeError(f"unknown vsync fields {bad}")
+        return VSyncInfo(**{k: int(v) for k, v in info.items()})
+
+    def vsync(self, read: bool = True, **info):
+        """pgs_vsync. Returns (status, metadata dict, image): the image is the frame as a
+        (height, width) uint32 RGBA8 array copied to the host when `read`, else the device
+        pointer; None when there is no frame."""
+        out = Scanout()
+        rc = int(self._L.pgs_vsync(self._h, ctypes.byref(self._vsync_info(info)), ctypes.byref(out)))
+        meta = {n: int(getattr(out, n)) for n in SCANOUT_FIELDS}
+        meta["sampling_rate_mhz"] = float(out.sampling_rate_mhz)
+        if rc != 0 or not out.image:
+            return rc, meta, None
+        if not read:
+            return rc, meta, int(out.image)
+        img = np.zeros((out.height, out.width), np.uint32)
+        self._check(self._L.pgs_read_scanout(self._h, img.ctypes.data, img.nbytes), "read_scanout")
+        return rc, meta, img
+
+    def vsync_can_skip(self, **info) -> bool:
+        rc = int(self._L.pgs_vsync_can_skip(self._h, ctypes.byref(self._vsync_info(info))))
+        if rc < 0:
+            raise PgsError(f"vsync_can_skip failed: {rc}")
+        return rc == 1
 
 
 def host_swizzle_addr(x: int, y: int, base: int, stride: int, psm: int) -> int:

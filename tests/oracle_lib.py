@@ -15,6 +15,28 @@ VRAM_SIZE = 4 * 1024 * 1024
 
 _L = None
 
+VSYNC_FIELDS = ("phase", "force_progressive", "overscan", "anti_blur", "crtc_offsets",
+                "adapt_to_internal_horizontal_resolution", "raw_circuit_scanout", "high_resolution_scanout",
+                "skip_deinterlace")
+SCANOUT_FIELDS = ("width", "height", "internal_width", "internal_height", "mode_width", "mode_height",
+                  "high_resolution_scanout", "interlaced", "double_strike", "interlace_phase")
+
+
+class VsyncInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in VSYNC_FIELDS]
+
+
+class ScanoutOut(ctypes.Structure):
+    _fields_ = [("has_image", ctypes.c_uint32)] + [(n, ctypes.c_uint32) for n in SCANOUT_FIELDS] + \
+        [("sampling_rate_mhz", ctypes.c_float)]
+
+
+def vsync_info(**kw) -> VsyncInfo:
+    bad = set(kw) - set(VSYNC_FIELDS)
+    if bad:
+        raise TypeError(f"unknown vsync fields {bad}")
+    return VsyncInfo(**{k: int(v) for k, v in kw.items()})
+
 
 def oracle_lib() -> ctypes.CDLL:
     global _L
@@ -51,6 +73,14 @@ def oracle_lib() -> ctypes.CDLL:
         L.orc_set_sample_log2.restype = None
         L.orc_reset_context_state.argtypes = [vp]
         L.orc_reset_context_state.restype = None
+        L.orc_set_priv.argtypes = [vp, vp, sz]
+        L.orc_set_priv.restype = None
+        L.orc_complex_scanmsk_timeout.argtypes = [vp]
+        L.orc_complex_scanmsk_timeout.restype = u32
+        L.orc_vsync_can_skip.argtypes = [vp, ctypes.POINTER(VsyncInfo)]
+        L.orc_vsync_can_skip.restype = ctypes.c_int
+        L.orc_vsync.argtypes = [vp, ctypes.POINTER(VsyncInfo), ctypes.POINTER(ScanoutOut), vp, sz]
+        L.orc_vsync.restype = ctypes.c_int
         _L = L
     return _L
 
@@ -107,6 +137,29 @@ class Oracle:
 
     def set_sample_log2(self, sample_log2: int):
         self._L.orc_set_sample_log2(self._h, sample_log2)
+
+    def set_priv(self, block: bytes):
+        buf = np.frombuffer(bytes(block), dtype=np.uint8)
+        self._L.orc_set_priv(self._h, buf.ctypes.data, buf.nbytes)
+
+    def complex_scanmsk_timeout(self) -> int:
+        return int(self._L.orc_complex_scanmsk_timeout(self._h))
+
+    def vsync_can_skip(self, **info) -> bool:
+        i = vsync_info(**info)
+        return bool(self._L.orc_vsync_can_skip(self._h, ctypes.byref(i)))
+
+    def vsync(self, **info):
+        """Returns (status, metadata dict, image as (height, width) uint32 RGBA8 or None)."""
+        i = vsync_info(**info)
+        out = ScanoutOut()
+        cap = 8192 * 4096
+        pixels = np.zeros(cap, dtype=np.uint32)
+        rc = self._L.orc_vsync(self._h, ctypes.byref(i), ctypes.byref(out), pixels.ctypes.data, cap)
+        meta = {n: int(getattr(out, n)) for n in SCANOUT_FIELDS}
+        meta["sampling_rate_mhz"] = float(out.sampling_rate_mhz)
+        img = pixels[:out.width * out.height].reshape(out.height, out.width).copy() if out.has_image else None
+        return int(rc), meta, img
 
 
 def swizzle(x, y, base, stride, psm) -> int:
