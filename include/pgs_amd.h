@@ -262,6 +262,45 @@ int pgs_vsync_can_skip(pgs_context *ctx, const pgs_vsync_info *info);
  * scanout kernels have finished. Backend extension: the reference hands out a Vulkan image. */
 int pgs_read_scanout(pgs_context *ctx, void *dst, size_t size);
 
+/* ---- GS dump files: the PCSX2 GS-dump format, state versions 8 and 9 ----------------------
+ * A dump is a header, the register state, 4 MiB of VRAM, the four GIF paths, the internal Q
+ * and the privileged registers, then Transfer / Vsync / ReadFIFO / PrivRegisters packets. */
+typedef struct pgs_dump_reader pgs_dump_reader;
+
+/* GSDumpParser::open / open_raw (dump/gs_dump_parser.cpp:63-82): opens `path` and, unless
+ * `raw` (packets only), loads its start state into `ctx` (VRAM, registers, GIF paths, privileged
+ * registers). NULL on failure. The context must outlive the reader. */
+pgs_dump_reader *pgs_dump_open(pgs_context *ctx, const char *path, uint32_t raw);
+/* GSDumpParser::restart (dump/gs_dump_parser.cpp:6-60): rewinds and loads the start state again
+ * (a save-state restore). 0 on success. */
+int pgs_dump_restart(pgs_dump_reader *reader);
+/* GSDumpParser::iterate_until_vsync (dump/gs_dump_parser.cpp:137-206): replays packets until a
+ * vsync that follows GIF traffic (every vsync with `conservative_crtc`) or a ReadFIFO packet.
+ * Vsyncs run pgs_vsync with the parser's VSyncInfo. Returns 1 when it stopped there, 0 at the end
+ * of the file, negative on error. */
+int pgs_dump_iterate_until_vsync(pgs_dump_reader *reader, uint32_t high_res_scanout, uint32_t conservative_crtc);
+/* GSDumpParser::consume_vsync_result (dump/gs_dump_parser.cpp:208-213): the last vsync's frame
+ * (then cleared); it also becomes the frame pgs_read_scanout copies. */
+int pgs_dump_consume_vsync_result(pgs_dump_reader *reader, pgs_scanout *result);
+void pgs_dump_close(pgs_dump_reader *reader);
+
+typedef struct pgs_dump_writer pgs_dump_writer;
+/* GSDumpGenerator::init (dump/gs_dump_generator.cpp:199-237): writes the header and the
+ * context's current state (flushes it first). NULL on failure. */
+pgs_dump_writer *pgs_dump_writer_create(pgs_context *ctx, const char *path);
+/* GSDumpGenerator::write_register / write_packed / write_vsync / write_image_upload
+ * (dump/gs_dump_generator.cpp:107-197). The writer only records: it does not send the packets
+ * to the context. write_vsync stores the context's privileged registers before the vsync
+ * packet. Image uploads use IMAGE packets of at most 32767 qwords. */
+int pgs_dump_write_register(pgs_dump_writer *writer, uint32_t addr, uint64_t payload);
+int pgs_dump_write_packed(pgs_dump_writer *writer, uint64_t prim, const uint32_t *registers,
+                          uint32_t num_registers, uint32_t num_loops, const void *data);
+int pgs_dump_write_vsync(pgs_dump_writer *writer, uint32_t field);
+int pgs_dump_write_image_upload(pgs_dump_writer *writer, uint32_t addr, uint32_t psm, uint32_t width,
+                                uint32_t height, const void *data, size_t size);
+/* Flushes and closes the file; 0 when every write succeeded. */
+int pgs_dump_writer_close(pgs_dump_writer *writer);
+
 /* Number of 4 MiB VRAM planes: 1 at 1x, 2 + samples with super-sampling
  * (plane layout of GSRenderer::init_vram, gs/gs_renderer.cpp:360-427). */
 uint32_t pgs_num_planes(pgs_context *ctx);

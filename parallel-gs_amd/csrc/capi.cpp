@@ -4,6 +4,7 @@
 #include "gs_interface.hpp"
 #include "batch.hpp"
 #include "cost_model.hpp"
+#include "gs_dump.hpp"
 #include <vector>
 #include <cstring>
 #include <exception>
@@ -16,6 +17,18 @@ struct pgs_context
 	pgs::GSInterface gs;
 	uint64_t record_clock = 0; // VRAM write-log time at pgs_begin_recording
 	pgs::ScanoutResult last_scanout;
+};
+
+struct pgs_dump_reader
+{
+	pgs_context *ctx;
+	pgs::DumpReader reader;
+};
+
+struct pgs_dump_writer
+{
+	pgs_context *ctx;
+	pgs::DumpWriter writer;
 };
 
 struct pgs_batch
@@ -51,6 +64,54 @@ int batch_guard(pgs_batch *b, F &&f)
 {
 	if (!b)
 		return -1;
+	try
+	{
+		return f();
+	}
+	catch (const std::exception &e)
+	{
+		fprintf(stderr, "pgs: %s\n", e.what());
+		return -1;
+	}
+}
+
+pgs::VSyncInfo to_vsync_info(const pgs_vsync_info &in)
+{
+	pgs::VSyncInfo v;
+	v.phase = in.phase;
+	v.force_progressive = in.force_progressive != 0;
+	v.overscan = in.overscan != 0;
+	v.anti_blur = in.anti_blur != 0;
+	v.crtc_offsets = in.crtc_offsets != 0;
+	v.adapt_to_internal_horizontal_resolution = in.adapt_to_internal_horizontal_resolution != 0;
+	v.raw_circuit_scanout = in.raw_circuit_scanout != 0;
+	v.high_resolution_scanout = in.high_resolution_scanout != 0;
+	v.skip_deinterlace = in.skip_deinterlace != 0;
+	return v;
+}
+
+void to_c_scanout(const pgs::ScanoutResult &r, pgs_scanout *out)
+{
+	memset(out, 0, sizeof(*out));
+	out->image = r.image;
+	out->width = r.width;
+	out->height = r.height;
+	out->internal_width = r.internal_width;
+	out->internal_height = r.internal_height;
+	out->mode_width = r.mode_width;
+	out->mode_height = r.mode_height;
+	out->high_resolution_scanout = r.high_resolution_scanout;
+	out->interlaced = r.interlaced;
+	out->double_strike = r.double_strike;
+	out->interlace_phase = r.interlace_phase;
+	out->sampling_rate_mhz = r.sampling_rate_mhz;
+}
+
+template <typename T, typename F>
+int handle_guard(T *h, F &&f)
+{
+	if (!h)
+		return PGS_ERROR_INVALID_ARGUMENT;
 	try
 	{
 		return f();
@@ -361,23 +422,6 @@ double pgs_get_accumulated_timestamps(pgs_context *ctx, uint32_t type)
 	return v;
 }
 
-namespace
-{
-pgs::VSyncInfo to_vsync_info(const pgs_vsync_info &in)
-{
-	pgs::VSyncInfo v;
-	v.phase = in.phase;
-	v.force_progressive = in.force_progressive != 0;
-	v.overscan = in.overscan != 0;
-	v.anti_blur = in.anti_blur != 0;
-	v.crtc_offsets = in.crtc_offsets != 0;
-	v.adapt_to_internal_horizontal_resolution = in.adapt_to_internal_horizontal_resolution != 0;
-	v.raw_circuit_scanout = in.raw_circuit_scanout != 0;
-	v.high_resolution_scanout = in.high_resolution_scanout != 0;
-	v.skip_deinterlace = in.skip_deinterlace != 0;
-	return v;
-}
-}
 
 int pgs_vsync(pgs_context *ctx, const pgs_vsync_info *info, pgs_scanout *result)
 {
@@ -388,19 +432,7 @@ int pgs_vsync(pgs_context *ctx, const pgs_vsync_info *info, pgs_scanout *result)
 		pgs::ScanoutResult r;
 		rc = ctx->gs.vsync(to_vsync_info(*info), r);
 		ctx->last_scanout = r;
-		memset(result, 0, sizeof(*result));
-		result->image = r.image;
-		result->width = r.width;
-		result->height = r.height;
-		result->internal_width = r.internal_width;
-		result->internal_height = r.internal_height;
-		result->mode_width = r.mode_width;
-		result->mode_height = r.mode_height;
-		result->high_resolution_scanout = r.high_resolution_scanout;
-		result->interlaced = r.interlaced;
-		result->double_strike = r.double_strike;
-		result->interlace_phase = r.interlace_phase;
-		result->sampling_rate_mhz = r.sampling_rate_mhz;
+		to_c_scanout(r, result);
 	});
 	if (g != 0)
 		return g;
@@ -426,6 +458,118 @@ int pgs_read_scanout(pgs_context *ctx, void *dst, size_t size)
 	if (e == hipSuccess)
 		e = hipMemcpy(dst, r.image, bytes, hipMemcpyDeviceToHost);
 	return e == hipSuccess ? 0 : -int(e);
+}
+
+
+pgs_dump_reader *pgs_dump_open(pgs_context *ctx, const char *path, uint32_t raw)
+{
+	if (!ctx || !path)
+		return nullptr;
+	try
+	{
+		auto *r = new pgs_dump_reader{ ctx, {} };
+		if (!r->reader.open(path, ctx->gs, raw != 0))
+		{
+			delete r;
+			return nullptr;
+		}
+		return r;
+	}
+	catch (const std::exception &e)
+	{
+		fprintf(stderr, "pgs: %s\n", e.what());
+		return nullptr;
+	}
+}
+
+int pgs_dump_restart(pgs_dump_reader *r)
+{
+	return handle_guard(r, [&] { return r->reader.restart() ? 0 : -1; });
+}
+
+int pgs_dump_iterate_until_vsync(pgs_dump_reader *r, uint32_t high_res_scanout, uint32_t conservative_crtc)
+{
+	return handle_guard(r, [&] {
+		const bool more = r->reader.iterate_until_vsync(high_res_scanout != 0, conservative_crtc != 0);
+		const int rc = r->reader.vsync_status();
+		if (rc != 0 && rc != pgs::GSInterface::kVsyncUnsupported)
+			return rc;
+		return more ? 1 : 0;
+	});
+}
+
+int pgs_dump_consume_vsync_result(pgs_dump_reader *r, pgs_scanout *result)
+{
+	if (!result)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return handle_guard(r, [&] {
+		const pgs::ScanoutResult s = r->reader.consume_vsync_result();
+		r->ctx->last_scanout = s;
+		to_c_scanout(s, result);
+		return 0;
+	});
+}
+
+void pgs_dump_close(pgs_dump_reader *r) { delete r; }
+
+pgs_dump_writer *pgs_dump_writer_create(pgs_context *ctx, const char *path)
+{
+	if (!ctx || !path)
+		return nullptr;
+	try
+	{
+		auto *w = new pgs_dump_writer{ ctx, {} };
+		if (!w->writer.init(path, ctx->gs))
+		{
+			delete w;
+			return nullptr;
+		}
+		return w;
+	}
+	catch (const std::exception &e)
+	{
+		fprintf(stderr, "pgs: %s\n", e.what());
+		return nullptr;
+	}
+}
+
+int pgs_dump_write_register(pgs_dump_writer *w, uint32_t addr, uint64_t payload)
+{
+	return handle_guard(w, [&] { return w->writer.write_register(addr, payload) ? 0 : -1; });
+}
+
+int pgs_dump_write_packed(pgs_dump_writer *w, uint64_t prim, const uint32_t *registers, uint32_t num_registers,
+                          uint32_t num_loops, const void *data)
+{
+	if (!registers || (num_loops && !data))
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return handle_guard(w, [&] {
+		return w->writer.write_packed(prim, registers, num_registers, num_loops, data) ? 0 : -1;
+	});
+}
+
+int pgs_dump_write_vsync(pgs_dump_writer *w, uint32_t field)
+{
+	return handle_guard(w, [&] { return w->writer.write_vsync(field, w->ctx->gs) ? 0 : -1; });
+}
+
+int pgs_dump_write_image_upload(pgs_dump_writer *w, uint32_t addr, uint32_t psm, uint32_t width, uint32_t height,
+                                const void *data, size_t size)
+{
+	if (!data && size)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	return handle_guard(w, [&] {
+		return w->writer.write_image_upload(addr, psm, width, height, data, size) ? 0 : -1;
+	});
+}
+
+int pgs_dump_writer_close(pgs_dump_writer *w)
+{
+	if (!w)
+		return PGS_ERROR_INVALID_ARGUMENT;
+	const int rc = w->writer.close() ? 0 : -1;
+	delete w;
+	return rc;
 }
 
 uint32_t pgs_num_planes(pgs_context *ctx) { return ctx ? ctx->gs.renderer().num_planes() : 0; }

@@ -113,6 +113,9 @@ public:
 	RegisterState &get_register_state() { return regs_; }
 	GIFPath &get_gif_path(uint32_t path) { return paths_[path & 3u]; }
 	uint8_t *get_priv_register_state() { return priv_.data(); }
+	const uint8_t *get_priv_register_state() const { return priv_.data(); }
+	const RegisterState &get_register_state() const { return regs_; }
+	const GIFPath &get_gif_path(uint32_t path) const { return paths_[path & 3u]; }
 	uint32_t *cached_cbp() { return cached_cbp_; }
 	FlushStats consume_flush_stats() { return renderer_.consume_stats(); }
 	Renderer &renderer() { return renderer_; }

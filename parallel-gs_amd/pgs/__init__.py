@@ -134,6 +134,9 @@ EXPORTED_SYMBOLS = [
     "pgs_get_priv_register_state", "pgs_set_priv_register_state", "pgs_set_super_sampling_rate",
     "pgs_get_super_sampling_rate", "pgs_set_hacks", "pgs_set_debug_mode", "pgs_set_signal_interface",
     "pgs_get_accumulated_timestamps", "pgs_vsync", "pgs_vsync_can_skip", "pgs_read_scanout",
+    "pgs_dump_open", "pgs_dump_restart", "pgs_dump_iterate_until_vsync", "pgs_dump_consume_vsync_result",
+    "pgs_dump_close", "pgs_dump_writer_create", "pgs_dump_write_register", "pgs_dump_write_packed",
+    "pgs_dump_write_vsync", "pgs_dump_write_image_upload", "pgs_dump_writer_close",
     "pgs_begin_recording", "pgs_end_recording", "pgs_replay", "pgs_sync", "pgs_stream",
     "pgs_profile_enable", "pgs_profile_collect",
     "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
@@ -206,6 +209,21 @@ def lib() -> ctypes.CDLL:
         L.pgs_vsync.argtypes = [vp, ctypes.POINTER(VSyncInfo), ctypes.POINTER(Scanout)]
         L.pgs_vsync_can_skip.argtypes = [vp, ctypes.POINTER(VSyncInfo)]
         L.pgs_read_scanout.argtypes = [vp, vp, sz]
+        cp = ctypes.c_char_p
+        L.pgs_dump_open.argtypes = [vp, cp, u32]
+        L.pgs_dump_open.restype = vp
+        L.pgs_dump_restart.argtypes = [vp]
+        L.pgs_dump_iterate_until_vsync.argtypes = [vp, u32, u32]
+        L.pgs_dump_consume_vsync_result.argtypes = [vp, ctypes.POINTER(Scanout)]
+        L.pgs_dump_close.argtypes = [vp]
+        L.pgs_dump_close.restype = None
+        L.pgs_dump_writer_create.argtypes = [vp, cp]
+        L.pgs_dump_writer_create.restype = vp
+        L.pgs_dump_write_register.argtypes = [vp, u32, u64]
+        L.pgs_dump_write_packed.argtypes = [vp, u64, ctypes.POINTER(u32), u32, u32, vp]
+        L.pgs_dump_write_vsync.argtypes = [vp, u32]
+        L.pgs_dump_write_image_upload.argtypes = [vp, u32, u32, u32, u32, vp, sz]
+        L.pgs_dump_writer_close.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -449,6 +467,9 @@ class Context:
         pointer; None when there is no frame."""
         out = Scanout()
         rc = int(self._L.pgs_vsync(self._h, ctypes.byref(self._vsync_info(info)), ctypes.byref(out)))
+        return self._scanout_result(rc, out, read)
+
+    def _scanout_result(self, rc: int, out: "Scanout", read: bool):
         meta = {n: int(getattr(out, n)) for n in SCANOUT_FIELDS}
         meta["sampling_rate_mhz"] = float(out.sampling_rate_mhz)
         if rc != 0 or not out.image:
@@ -464,6 +485,83 @@ class Context:
         if rc < 0:
             raise PgsError(f"vsync_can_skip failed: {rc}")
         return rc == 1
+
+
+class DumpReader:
+    """GS dump reader bound to a context (pgs_dump_open)."""
+
+    def __init__(self, ctx: "Context", path: str, raw: bool = False):
+        self._L, self._ctx = ctx._L, ctx
+        self._h = self._L.pgs_dump_open(ctx._h, os.fsencode(path), int(raw))
+        if not self._h:
+            raise PgsError(f"cannot open GS dump {path}")
+
+    def restart(self) -> bool:
+        return self._L.pgs_dump_restart(self._h) == 0
+
+    def iterate_until_vsync(self, high_res_scanout: bool = False, conservative_crtc: bool = False) -> bool:
+        rc = int(self._L.pgs_dump_iterate_until_vsync(self._h, int(high_res_scanout), int(conservative_crtc)))
+        if rc < 0:
+            raise PgsError(f"dump replay failed: {rc}")
+        return rc == 1
+
+    def consume_vsync_result(self, read: bool = True):
+        """(metadata, image) of the last vsync; image None when there was none."""
+        out = Scanout()
+        rc = int(self._L.pgs_dump_consume_vsync_result(self._h, ctypes.byref(out)))
+        _, meta, img = self._ctx._scanout_result(rc, out, read)
+        return meta, img
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.pgs_dump_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class DumpWriter:
+    """GS dump writer bound to a context (pgs_dump_writer_create): the header holds the
+    context's state at creation."""
+
+    def __init__(self, ctx: "Context", path: str):
+        self._L = ctx._L
+        self._h = self._L.pgs_dump_writer_create(ctx._h, os.fsencode(path))
+        if not self._h:
+            raise PgsError(f"cannot create GS dump {path}")
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise PgsError(f"dump {what} failed: {rc}")
+
+    def write_register(self, addr: int, payload: int):
+        self._ok(self._L.pgs_dump_write_register(self._h, addr, payload & ((1 << 64) - 1)), "write_register")
+
+    def write_packed(self, prim: int, registers, num_loops: int, data: np.ndarray):
+        regs = (ctypes.c_uint32 * len(registers))(*registers)
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self._ok(self._L.pgs_dump_write_packed(self._h, prim, regs, len(registers), num_loops, data.ctypes.data),
+                 "write_packed")
+
+    def write_vsync(self, field: int):
+        self._ok(self._L.pgs_dump_write_vsync(self._h, field), "write_vsync")
+
+    def write_image_upload(self, addr: int, psm: int, width: int, height: int, data: np.ndarray):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        self._ok(self._L.pgs_dump_write_image_upload(self._h, addr, psm, width, height, data.ctypes.data,
+                                                     data.nbytes), "write_image_upload")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            rc = self._L.pgs_dump_writer_close(self._h)
+            self._h = None
+            self._ok(rc, "close")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.pgs_dump_writer_close(self._h)
+            self._h = None
 
 
 def host_swizzle_addr(x: int, y: int, base: int, stride: int, psm: int) -> int:
