@@ -144,10 +144,10 @@ int pgs_set_priv_register_state(pgs_context *ctx, const void *in, size_t size);
 /* GSInterface::set_super_sampling_rate (gs/gs_interface.hpp:258, gs/gs_interface.cpp:65-118).
  * Submits pending work, then switches the sample count (1, 2, 4, 8, 16) and grid, clearing
  * every super-sample plane. The rate may not exceed what the context allocated (its
- * `super_sampling`, or 16 with `dynamic_super_sampling`). Returns PGS_ERROR_UNSUPPORTED for
- * super-sampled textures and for the sparse 4x / 16x grids (ordered_grid = 0), which this
- * backend does not build, and PGS_ERROR_INVALID_ARGUMENT for other rates; the rate is then
- * unchanged. Not allowed while recording. */
+ * `super_sampling`, or 16 with `dynamic_super_sampling`). ordered_grid = 0 selects the sparse
+ * 1x4 (4x) and 2x8 (16x) grids (gs/gs_interface.cpp:84-113). Returns PGS_ERROR_UNSUPPORTED for
+ * super-sampled textures, which this backend does not build, and PGS_ERROR_INVALID_ARGUMENT
+ * for other rates; the rate is then unchanged. Not allowed while recording. */
 int pgs_set_super_sampling_rate(pgs_context *ctx, uint32_t super_sampling, uint32_t ordered_grid,
                                 uint32_t super_sampled_textures);
 /* The current sample count (1, 2, 4, 8 or 16). */

@@ -687,6 +687,7 @@ void orc_host_write(void *g, size_t offset, const void *data, size_t size)
 }
 // GSInterface::set_super_sampling_rate (gs/gs_interface.cpp:65-118): pending work is done at the
 // old rate, then every plane above plane 0 is cleared (gs/gs_renderer.cpp:315-342).
+void orc_set_sparse_grid(void *g, uint32_t sparse) { static_cast<orc::GS *>(g)->sparse_grid = sparse != 0; }
 void orc_set_sample_log2(void *g, uint32_t sample_log2)
 {
 	auto *gs = static_cast<orc::GS *>(g);

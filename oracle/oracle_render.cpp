@@ -408,8 +408,8 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 	}
 	const int32_t pre_lo[2] = { lox, loy }, pre_hi[2] = { hix, hiy };
 	hix -= 1; hiy -= 1;
-	lox += (1 << (4 - int32_t(sample_log2 ? sample_log2 / 2 + (sample_log2 & 1) : 0))) - 1;
-	loy += (1 << (4 - int32_t(sample_log2 ? sample_log2 / 2 + (sample_log2 & 1) : 0))) - 1;
+	lox += (1 << (4 - int32_t(rate_y_log2()))) - 1;
+	loy += (1 << (4 - int32_t(rate_y_log2()))) - 1;
 	lox >>= 4; loy >>= 4; hix >>= 4; hiy >>= 4;
 	bool aa1 = field(prim, 7, 1) && field(prim, 0, 3) != 6 && field(prim, 0, 3) != 0;
 	if (is_line || aa1)
@@ -1195,6 +1195,20 @@ void sample_offset(uint32_t rate_x, uint32_t rate_y, uint32_t s, int32_t &dx, in
 		dx = int32_t(((s >> 1) & 1u) + 2u * ((s >> 3) & 1u));
 		dy = int32_t((s & 1u) + 2u * ((s >> 2) & 1u));
 	}
+	else if (rate_x == 0 && rate_y == 2)
+	{
+		// Sparse 1 x 4: one sample per quarter row at x quarters 0, 2, 3, 1 (the 0x78 offsets of
+		// ubershader.comp:346-351).
+		dx = int32_t((0x78u >> (2u * s)) & 3u);
+		dy = int32_t(s);
+	}
+	else if (rate_x == 1 && rate_y == 3)
+	{
+		// Sparse 2 x 8: per eighth row two samples half a pixel apart, shifted by the same
+		// four-row pattern.
+		dx = int32_t(4u * ((s >> 3) & 1u) + ((0x78u >> (2u * (s & 3u))) & 3u));
+		dy = int32_t(s & 7u);
+	}
 	else { dx = 0; dy = 0; }
 }
 // A render pass of a super-sampled context (planes: 0 = single-sampled VRAM, 1 = reference
@@ -1478,9 +1492,10 @@ void GS::render(const std::vector<Prim> &prims_in)
 	const uint32_t opaque_fbmask = ~write_mask;
 
 	// Sampling rate of the pass (gs/gs_interface.cpp:212-260): x/y log2 from the context's
-	// sample count (2x: 0/1, 4x: 1/1, 8x: 1/2, 16x: 2/2); channel-sliced 16-bit colour masks
+	// sample count and grid (2x: 0/1, 4x: 1/1 or sparse 0/2, 8x: 1/2, 16x: 2/2 or sparse 1/3);
+	// channel-sliced 16-bit colour masks
 	// and colour/depth aliasing with depth writes render single-sampled.
-	uint32_t rate_x = sample_log2 / 2, rate_y = (sample_log2 + 1) / 2;
+	uint32_t rate_x = rate_x_log2(), rate_y = rate_y_log2();
 	if (sample_log2)
 	{
 		const bool fb_32_24 = f.fb_psm == CT32 || f.fb_psm == CT24 || f.fb_psm == Z32 || f.fb_psm == Z24;
@@ -1711,7 +1726,7 @@ void GS::render(const std::vector<Prim> &prims_in)
 
 	if (sample_log2 != 0)
 	{
-		primitives_drawn += render_super_sampled_pass(vram.data(), rp_bb, (sample_log2 + 1) / 2, prims, setups, live, f,
+		primitives_drawn += render_super_sampled_pass(vram.data(), rp_bb, rate_y_log2(), prims, setups, live, f,
 		                                              rate_x, rate_y, shade);
 		return;
 	}

@@ -115,6 +115,17 @@ public:
 	std::vector<uint8_t> vram;   // plane 0 = VRAM; at SSAA: plane 1 = reference, 2.. = samples
 	std::vector<uint16_t> clut;  // current CLUT: 512 x 16-bit entries (gs/shaders/clut_write.comp)
 	uint32_t sample_log2 = 0;
+	// Sparse sampling grids at 4x (1 x 4) and 16x (2 x 8) instead of the ordered 2 x 2 / 4 x 4
+	// (gs/gs_interface.cpp:84-113).
+	bool sparse_grid = false;
+	// log2 of the samples per pixel along x and y (gs/gs_interface.cpp:65-117).
+	uint32_t rate_x_log2() const
+	{
+		if (!sample_log2)
+			return 0;
+		return sparse_grid && (sample_log2 == 2 || sample_log2 == 4) ? sample_log2 / 2 - 1 : sample_log2 / 2;
+	}
+	uint32_t rate_y_log2() const { return sample_log2 - rate_x_log2(); }
 
 	void (*xfer_hook)(void *, uint32_t dir, uint32_t w, uint32_t h, uint32_t psm) = nullptr;
 	void *xfer_hook_user = nullptr;

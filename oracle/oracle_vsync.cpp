@@ -398,7 +398,7 @@ int GS::vsync(const VsyncInfo &info_in, ScanoutOut &out, std::vector<uint32_t> &
 	image.clear();
 	flush();
 	Priv pr{ priv.data() };
-	const uint32_t rate_x_log2 = sample_log2 >> 1, rate_y_log2 = sample_log2 - (sample_log2 >> 1);
+	const uint32_t rate_x_log2 = this->rate_x_log2(), rate_y_log2 = this->rate_y_log2();
 
 	// GSInterface::vsync (gs/gs_interface.cpp:4626-4690); no promoted back buffers.
 	if (info_in.high_resolution_scanout && rate_y_log2 && pr.FFMD() && pr.INT() && pr.CMOD() != CMOD_PROGRESSIVE)

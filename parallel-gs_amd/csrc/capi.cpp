@@ -354,8 +354,6 @@ int pgs_set_super_sampling_rate(pgs_context *ctx, uint32_t super_sampling, uint3
 		return PGS_ERROR_INVALID_ARGUMENT;
 	if (super_sampled_textures && super_sampling != 1)
 		return PGS_ERROR_UNSUPPORTED;
-	if (!ordered_grid && (super_sampling == 4 || super_sampling == 16))
-		return PGS_ERROR_UNSUPPORTED;
 	bool ok = false;
 	const int rc = guard(ctx, [&] {
 		ok = ctx->gs.set_super_sampling_rate(pgs::SuperSampling(super_sampling), ordered_grid != 0,

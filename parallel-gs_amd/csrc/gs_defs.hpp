@@ -41,6 +41,16 @@ PGS_HD uint32_t psm_bpp(uint32_t psm)
 	}
 }
 
+// Samples per pixel along x and y (log2) of a sampling grid (gs/gs_interface.cpp:65-117):
+// ordered 2x = 1 x 2, 4x = 2 x 2, 8x = 2 x 4, 16x = 4 x 4; sparse 4x = 1 x 4, 16x = 2 x 8.
+PGS_HD void grid_rates(uint32_t sample_log2, bool sparse, uint32_t &rate_x, uint32_t &rate_y)
+{
+	rate_x = sample_log2 / 2u;
+	if (sparse && (sample_log2 == 2u || sample_log2 == 4u))
+		rate_x--;
+	rate_y = sample_log2 - rate_x;
+}
+
 PGS_HD bool psm_is_16(uint32_t psm) { return psm == PSMCT16 || psm == PSMCT16S || psm == PSMZ16 || psm == PSMZ16S; }
 PGS_HD bool psm_is_z(uint32_t psm) { return psm == PSMZ32 || psm == PSMZ24 || psm == PSMZ16 || psm == PSMZ16S; }
 

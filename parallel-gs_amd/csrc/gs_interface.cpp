@@ -30,18 +30,19 @@ bool GSInterface::init(const GSOptions &options)
 	if (l < 0)
 		return false;
 	const uint32_t sample_log2 = uint32_t(l);
-	// Sampling grids (gs/gs_interface.cpp:65-117): 2x = 1x2, 4x = 2x2, 8x = 2x4, 16x = 4x4.
-	// The sparse 4x (1x4) and 16x (2x8) grids and super-sampled textures (texture arrays per
-	// sample, ubershader.comp:1155-1212) are not built: asking for them fails.
-	if ((sample_log2 == 2 || sample_log2 == 4) && !options.ordered_super_sampling)
-		return false;
+	// Sampling grids (gs/gs_interface.cpp:65-117): 2x = 1x2, 4x = 2x2 or sparse 1x4, 8x = 2x4,
+	// 16x = 4x4 or sparse 2x8. Super-sampled textures (texture arrays per sample,
+	// ubershader.comp:1155-1212) are not built: asking for them fails.
 	if (options.super_sampled_textures && sample_log2 != 0)
 		return false;
+	const bool sparse = !options.ordered_super_sampling && (sample_log2 == 2 || sample_log2 == 4);
+	renderer_.set_sparse_grid(sparse);
 	const uint32_t capacity_log2 = options.dynamic_super_sampling ? 4u : sample_log2;
 	if (!renderer_.init(sample_log2, options.upload_arena_bytes, options.sampler_feedback, options.texture_stream,
 	                    capacity_log2))
 		return false;
 	rp_.init(&renderer_, sample_log2);
+	rp_.set_sample_log2(sample_log2, sparse);
 	rp_.host = this;
 	reset_registers();
 	return true;
@@ -78,17 +79,17 @@ bool GSInterface::set_super_sampling_rate(SuperSampling rate, bool ordered_grid,
 	if (l < 0)
 		return false;
 	const uint32_t sample_log2 = uint32_t(l);
-	if ((sample_log2 == 2 || sample_log2 == 4) && !ordered_grid)
-		return false;
 	// At 1x the reference turns super-sampled textures off (gs/gs_interface.cpp:74-78).
 	if (super_sampled_textures && sample_log2 != 0)
 		return false;
 	if (sample_log2 > renderer_.capacity_log2())
 		return false;
 	flush();
+	const bool sparse = !ordered_grid && (sample_log2 == 2 || sample_log2 == 4);
+	renderer_.set_sparse_grid(sparse);
 	if (!renderer_.set_sample_log2(sample_log2))
 		return false;
-	rp_.set_sample_log2(sample_log2);
+	rp_.set_sample_log2(sample_log2, sparse);
 	options_.super_sampling = rate;
 	options_.ordered_super_sampling = ordered_grid;
 	return true;
@@ -118,22 +119,12 @@ void GSInterface::set_timestamps(bool enable)
 	scanout_.set_timing(enable);
 }
 
-namespace
-{
-void rate_log2_xy(uint32_t sample_log2, uint32_t &x, uint32_t &y)
-{
-	// Ordered grids (gs/gs_interface.cpp:65-118): 2x = 1x2, 4x = 2x2, 8x = 2x4, 16x = 4x4.
-	x = sample_log2 >> 1;
-	y = sample_log2 - x;
-}
-}
-
 int GSInterface::vsync(const VSyncInfo &info_in, ScanoutResult &result)
 {
 	result = ScanoutResult{};
 	const CrtcRegisters r = decode_crtc_registers(priv_.data());
 	uint32_t xl, yl;
-	rate_log2_xy(renderer_.sample_log2(), xl, yl);
+	renderer_.rates(xl, yl);
 	// The reference switches its renderer to field-aware super-sampling here
 	// (gs/gs_interface.cpp:4666-4672, gs/gs_renderer.cpp:1953, 2123, 2605); not built.
 	if (info_in.high_resolution_scanout && yl && r.ffmd && r.interlace && r.cmod != 0)

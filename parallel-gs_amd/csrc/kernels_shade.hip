@@ -716,6 +716,18 @@ __device__ __forceinline__ void sample_offset(uint32_t s, int32_t &dx, int32_t &
 		dx = int32_t(2u * ((s >> 1) & 1u) + (s & 1u));
 		dy = int32_t((s & 1u) + 2u * ((s >> 2) & 1u));
 	}
+	else if (RX == 0 && RY == 2)
+	{
+		// Sparse 1 x 4: one sample per quarter row, at x quarters 0, 2, 3, 1 (the 0x78 offsets).
+		dx = int32_t((0x78u >> (2u * s)) & 3u);
+		dy = int32_t(s);
+	}
+	else if (RX == 1 && RY == 3)
+	{
+		// Sparse 2 x 8: two samples half a pixel apart per eighth row, with the same offsets.
+		dx = int32_t(4u * ((s >> 3) & 1u) + ((0x78u >> (2u * (s & 3u))) & 3u));
+		dy = int32_t(s & 7u);
+	}
 	else
 	{
 		dx = int32_t(((s >> 1) & 1u) + 2u * ((s >> 3) & 1u));
@@ -965,6 +977,8 @@ PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_01_multi, (shade_ssaa_b
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_11_multi, (shade_ssaa_body<1, 1>), pgs_shade_args)
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_12_multi, (shade_ssaa_body<1, 2>), pgs_shade_args)
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_02_multi, (shade_ssaa_body<0, 2>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_13_multi, (shade_ssaa_body<1, 3>), pgs_shade_args)
 
 // Launches of different sampling grids in one multi-launch (independent passes of different
 // contexts): each workgroup runs the body of its own launch's grid. One launch instead of one
@@ -1105,6 +1119,8 @@ PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_01_queue, 0, 1)
 PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_11_queue, 1, 1)
 PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_12_queue, 1, 2)
 PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_22_queue, 2, 2)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_02_queue, 0, 2)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_13_queue, 1, 3)
 
 __global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 k_shade_1x_ordered(pgs_queue_args q,
                                                                           const pgs_shade_args *__restrict__ args)
@@ -1122,6 +1138,8 @@ PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_01_ordered, 0, 1)
 PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_11_ordered, 1, 1)
 PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_12_ordered, 1, 2)
 PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_22_ordered, 2, 2)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_02_ordered, 0, 2)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_13_ordered, 1, 3)
 
 __device__ __forceinline__ float wave_sum(float v)
 {
@@ -1277,6 +1295,10 @@ extern "C" int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream)
 		hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 2>), dim3(groups), dim3(threads), 0, stream, *args);
 	else if (rx == 2 && ry == 2)
 		hipLaunchKernelGGL((pgs::k_shade_ssaa<2, 2>), dim3(groups), dim3(threads), 0, stream, *args);
+	else if (rx == 0 && ry == 2)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<0, 2>), dim3(groups), dim3(threads), 0, stream, *args);
+	else if (rx == 1 && ry == 3)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 3>), dim3(groups), dim3(threads), 0, stream, *args);
 	else
 		return -int(hipErrorNotSupported);
 	hipError_t err = hipGetLastError();
@@ -1298,6 +1320,10 @@ int pgs_multi_shade(const pgs_shade_args *args, const uint32_t *prefix, uint32_t
 		hipLaunchKernelGGL(pgs::k_shade_ssaa_12_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
 	else if (rate_x == 2 && rate_y == 2)
 		hipLaunchKernelGGL(pgs::k_shade_ssaa_22_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	else if (rate_x == 0 && rate_y == 2)
+		hipLaunchKernelGGL(pgs::k_shade_ssaa_02_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
+	else if (rate_x == 1 && rate_y == 3)
+		hipLaunchKernelGGL(pgs::k_shade_ssaa_13_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
 	else
 		return -int(hipErrorNotSupported);
 	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
@@ -1332,6 +1358,8 @@ static const void *queue_kernel(uint32_t rate_x, uint32_t rate_y)
 	if (rate_x == 1 && rate_y == 1) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_queue);
 	if (rate_x == 1 && rate_y == 2) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_queue);
 	if (rate_x == 2 && rate_y == 2) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_queue);
+	if (rate_x == 0 && rate_y == 2) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_02_queue);
+	if (rate_x == 1 && rate_y == 3) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_13_queue);
 	return nullptr;
 }
 
@@ -1379,6 +1407,8 @@ extern "C" int pgs_k_shade_ordered(const pgs_queue_args *q, uint32_t grid, hipSt
 	else if (q->rate_x == 1 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_ordered);
 	else if (q->rate_x == 1 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_ordered);
 	else if (q->rate_x == 2 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_ordered);
+	else if (q->rate_x == 0 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_02_ordered);
+	else if (q->rate_x == 1 && q->rate_y == 3) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_13_ordered);
 	else return -int(hipErrorNotSupported);
 	const pgs_shade_args *shade_args = static_cast<const pgs_shade_args *>(q->args);
 	void *params[] = { const_cast<pgs_queue_args *>(q), &shade_args };

@@ -99,12 +99,13 @@ bool triangles_form_parallelogram(const VertexPosition *p, const VertexAttribute
 }
 }
 
-void RenderPassBuilder::set_sample_log2(uint32_t sample_log2)
+void RenderPassBuilder::set_sample_log2(uint32_t sample_log2, bool sparse)
 {
 	sample_log2_ = sample_log2;
-	// Samples per axis of the ordered grids: 2x and 4x use 2, 8x and 16x use 4
+	// Samples along y, which sets the raster snap and bounding-box rounding
 	// (gs/gs_interface.cpp:65-118).
-	rate_log2_ = sample_log2 ? (sample_log2 + 1) / 2 : 0;
+	uint32_t rate_x;
+	grid_rates(sample_log2, sparse, rate_x, rate_log2_);
 }
 
 void RenderPassBuilder::init(Renderer *renderer, uint32_t sample_log2)

@@ -102,7 +102,7 @@ class RenderPassBuilder
 public:
 	void init(Renderer *renderer, uint32_t sample_log2);
 	// A new sample count; the caller has flushed the pass.
-	void set_sample_log2(uint32_t sample_log2);
+	void set_sample_log2(uint32_t sample_log2, bool sparse = false);
 	bool has_primitives() const { return !prims_.empty(); }
 	void reset_vertex_queue() { vq_count_ = 0; }
 

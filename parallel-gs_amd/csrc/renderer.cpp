@@ -589,7 +589,8 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	// Sampling rate of the pass (gs/gs_interface.cpp:212-260): channel-sliced 16-bit colour
 	// masks and colour/depth aliasing with depth writes render single-sampled.
 	const bool ssaa_context = num_planes_ > 1;
-	uint32_t rate_x = sample_log2_ / 2u, rate_y = (sample_log2_ + 1u) / 2u;
+	uint32_t rate_x, rate_y;
+	rates(rate_x, rate_y);
 	if (ssaa_context)
 	{
 		const bool fb_32_24 = d.fb_psm == PSMCT32 || d.fb_psm == PSMCT24 || d.fb_psm == PSMZ32 || d.fb_psm == PSMZ24;

@@ -51,6 +51,9 @@ public:
 	// gs/gs_renderer.cpp:315-342): waits for the GPU, then clears every plane above plane 0.
 	// False when the rate exceeds the allocated capacity or a recording is open.
 	bool set_sample_log2(uint32_t sample_log2);
+	// Sparse sampling grids for 4x and 16x (set before the sample count).
+	void set_sparse_grid(bool sparse) { sparse_ = sparse; }
+	void rates(uint32_t &rate_x, uint32_t &rate_y) const { grid_rates(sample_log2_, sparse_, rate_x, rate_y); }
 	uint32_t capacity_log2() const { return capacity_log2_; }
 	void set_sampler_feedback(uint32_t mode) { sampler_feedback_ = mode; }
 	// Adds the elapsed time of every timed launch whose events have completed to ms[kind]
@@ -220,6 +223,7 @@ private:
 	void emit_transfer_level(const std::vector<uint32_t> &members);
 
 	uint32_t sample_log2_ = 0;
+	bool sparse_ = false;
 	uint32_t capacity_log2_ = 0;
 	uint32_t num_planes_ = 1;
 	size_t snapshot_bytes_ = 0;

@@ -73,6 +73,8 @@ def oracle_lib() -> ctypes.CDLL:
         L.orc_set_sample_log2.restype = None
         L.orc_reset_context_state.argtypes = [vp]
         L.orc_reset_context_state.restype = None
+        L.orc_set_sparse_grid.argtypes = [vp, u32]
+        L.orc_set_sparse_grid.restype = None
         L.orc_set_priv.argtypes = [vp, vp, sz]
         L.orc_set_priv.restype = None
         L.orc_complex_scanmsk_timeout.argtypes = [vp]
@@ -88,9 +90,12 @@ def oracle_lib() -> ctypes.CDLL:
 class Oracle:
     """Immediate-mode CPU GS (one context)."""
 
-    def __init__(self, sample_log2: int = 0):
+    def __init__(self, sample_log2: int = 0, sparse: bool = False):
+        """sparse: the 1x4 (4x) / 2x8 (16x) sampling grids instead of the ordered ones."""
         self._L = oracle_lib()
         self._h = self._L.orc_create(sample_log2)
+        if sparse:
+            self._L.orc_set_sparse_grid(self._h, 1)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -135,7 +140,9 @@ class Oracle:
     def reset_context_state(self):
         self._L.orc_reset_context_state(self._h)
 
-    def set_sample_log2(self, sample_log2: int):
+    def set_sample_log2(self, sample_log2: int, sparse: bool = False):
+        self.flush()  # pending primitives render on the old grid
+        self._L.orc_set_sparse_grid(self._h, int(sparse))
         self._L.orc_set_sample_log2(self._h, sample_log2)
 
     def set_priv(self, block: bytes):

@@ -235,7 +235,6 @@ def test_dynamic_super_sampling_rate_change(hip_device):
         ctx.gif_transfer(stream2)
         o.gif_transfer(stream2)
         got = ctx.vram_planes()
-        assert ctx.set_super_sampling_rate(4, ordered=False) == pgs.ERROR_UNSUPPORTED
         assert ctx.set_super_sampling_rate(4, textures=True) == pgs.ERROR_UNSUPPORTED
         assert ctx.set_super_sampling_rate(3) == pgs.ERROR_INVALID_ARGUMENT
         assert ctx.super_sampling_rate == 4
@@ -247,8 +246,7 @@ def test_dynamic_super_sampling_rate_change(hip_device):
 
 
 def test_create_rejects_unbuilt_options(hip_device):
-    for kw in (dict(super_sampling=3), dict(super_sampling=4, ordered=False), dict(super_sampling=16, ordered=False),
-               dict(super_sampling=4, super_sampled_textures=True)):
+    for kw in (dict(super_sampling=3), dict(super_sampling=4, super_sampled_textures=True)):
         with pytest.raises(pgs.PgsError):
             pgs.Context(**kw)
 

@@ -142,3 +142,36 @@ def test_large_contexts_bin_in_several_windows(hip_device):
     batch.close()
     for c in ctxs:
         c.close()
+
+
+def test_sparse_grid_contexts_in_a_batch(hip_device):
+    """Contexts on the sparse 1 x 4 and 2 x 8 grids batched beside ordered ones: live runs equal
+    the oracle, and every schedule replays them to the same VRAM."""
+    specs = pgs.batched_specs(0x5EED0007, 4, n_min=300, n_max=2000, ssaa_choices=(4, 16))
+    ctxs, live = [], []
+    for k, sp in enumerate(specs):
+        sparse = k % 2 == 0
+        print(f"[case] context n={sp['n_prims']} ssaa={sp['ssaa']} sparse={sparse}", flush=True)
+        ctx = pgs.Context(super_sampling=sp["ssaa"], ordered=not sparse, arena_bytes=32 << 20)
+        ctx.begin_recording()
+        ctx.gif_transfer(pgs.spec_stream(sp))
+        ctx.end_recording()
+        live.append(ctx.vram_planes())
+        ctxs.append(ctx)
+        o = Oracle(LOG2[sp["ssaa"]], sparse)
+        o.gif_transfer(pgs.spec_stream(sp))
+        o.flush()
+        assert np.array_equal(live[-1], o.planes()), k
+    batch = pgs.Batch(ctxs)
+    model = pgs.CostModel(5)
+    for name in ("hardware", "static", "learned", "dispatch_static", "dispatch_learned"):
+        print(f"[case] sparse batch schedule {name}", flush=True)
+        batch.set_schedule(name, model if "learned" in name else None)
+        batch.replay(True)
+        batch.replay(True)
+        batch.sync()
+        for i, ctx in enumerate(ctxs):
+            assert np.array_equal(ctx.vram_planes(), live[i]), (name, i)
+    batch.close()
+    for c in ctxs:
+        c.close()

@@ -24,9 +24,9 @@ LOG2 = {1: 0, 2: 1, 4: 2, 8: 3, 16: 4}
 class Pair:
     """A GPU context and an oracle fed the same VRAM, streams and registers."""
 
-    def __init__(self, samples=1):
-        self.ctx = pgs.Context(super_sampling=samples)
-        self.orc = Oracle(LOG2[samples])
+    def __init__(self, samples=1, sparse=False):
+        self.ctx = pgs.Context(super_sampling=samples, ordered=not sparse)
+        self.orc = Oracle(LOG2[samples], sparse)
 
     def close(self):
         self.ctx.close()
@@ -193,6 +193,23 @@ def test_high_resolution_scanout_of_rendered_frames(hip_device, samples):
             raw_circuit_scanout=1, adapt_to_internal_horizontal_resolution=1)
     p.vsync(f"{samples}x field", phase=1, high_resolution_scanout=1)
     p.vsync(f"{samples}x progressive", force_progressive=1)
+    p.close()
+
+
+def test_high_resolution_scanout_of_a_sparse_16x_frame(hip_device):
+    """The sparse 2 x 8 grid at 16x: high-resolution scanout averages each quad's four sample
+    planes as for the ordered grid (sample_circuit.frag:110-140)."""
+    p = Pair(16, sparse=True)
+    p.stream(pgs.gen_textured(0x5EED0051, 1500, 1, 640, 448, 0.0))
+    p.priv(crtc.ntsc_interlaced_640x448())
+    img, meta = p.vsync("sparse 16x high resolution", force_progressive=1, high_resolution_scanout=1)
+    assert meta["high_resolution_scanout"] == 1 and img.shape == (896, 1280)
+    p.close()
+    p = Pair(4, sparse=True)  # 1 x 4: no x samples, so no high-resolution scanout
+    p.stream(pgs.gen_textured(0x5EED0052, 800, 1, 640, 448, 0.0))
+    p.priv(crtc.ntsc_interlaced_640x448())
+    img, meta = p.vsync("sparse 4x", force_progressive=1, high_resolution_scanout=1)
+    assert meta["high_resolution_scanout"] == 0
     p.close()
 
 

@@ -42,7 +42,14 @@ SAMPLES = {
 }
 
 
+SPARSE_X = [0, 2, 3, 1]  # quarter-pixel x offsets of the sparse rows (the 0x78 pattern)
+
+
 def sample_positions(n):
+    if n == "4s":   # sparse 1 x 4: quarter rows
+        return [(SPARSE_X[s] * 4, s * 4) for s in range(4)]
+    if n == "16s":  # sparse 2 x 8: eighth rows, two samples half a pixel apart
+        return [((4 * ((s >> 3) & 1) + SPARSE_X[s & 3]) * 2, (s & 7) * 2) for s in range(16)]
     if n == 2:
         return [(0, 0), (8, 8)]
     if n == 4:
@@ -68,8 +75,8 @@ def covered(verts, px, py):
 
 
 def draw_triangle(n_samples, verts, rgba, zs=(0x1000, 0x2000, 0x3000), iip=0, extra=()):
-    log2 = {1: 0, 2: 1, 4: 2, 16: 4}[n_samples]
-    o = Oracle(log2)
+    log2 = {1: 0, 2: 1, 4: 2, 16: 4, "4s": 2, "16s": 4}[n_samples]
+    o = Oracle(log2, sparse=isinstance(n_samples, str))
     writes = setup_regs(extra) + [(0x00, rs.prim_reg(3, iip=iip))]
     for (x, y), z in zip(verts, zs):
         writes.append((0x01, rs.rgbaq(rgba)))
@@ -79,10 +86,12 @@ def draw_triangle(n_samples, verts, rgba, zs=(0x1000, 0x2000, 0x3000), iip=0, ex
     return o
 
 
-@pytest.mark.parametrize("n_samples,seed", [(4, 0), (4, 1), (4, 2), (16, 3), (16, 4), (2, 5), (2, 6)])
+@pytest.mark.parametrize("n_samples,seed", [(4, 0), (4, 1), (4, 2), (16, 3), (16, 4), (2, 5), (2, 6),
+                                             ("4s", 7), ("4s", 8), ("16s", 9), ("16s", 10)])
 def test_edge_antialiasing_resolve(n_samples, seed):
     """A 3D triangle (depth varies, so no single-sample demotion) in flat grey 0x80: each
-    pixel resolves to 0x80 * k / n for k covered samples, exactly."""
+    pixel resolves to 0x80 * k / n for k covered samples, exactly. Sparse grids ("4s", "16s")
+    place their samples on the rows of a 4 x 4 / 8 x 8 sub-grid (ubershader.comp:333-352)."""
     rng = np.random.default_rng(seed)
     verts = [(int(rng.integers(0, W * 8)) * 2, int(rng.integers(0, H * 8)) * 2) for _ in range(3)]
     o = draw_triangle(n_samples, verts, 0x80808080)
@@ -90,13 +99,14 @@ def test_edge_antialiasing_resolve(n_samples, seed):
     img = plane_image(planes, 0)
     ref = plane_image(planes, 1)
     pos = sample_positions(n_samples)
+    n = len(pos)
     for y in range(H):
         for x in range(W):
             k = sum(covered(verts, x * 16 + dx, y * 16 + dy) for dx, dy in pos)
-            v = 0x80 * k // n_samples
+            v = 0x80 * k // n
             assert img[y, x] == v * 0x01010101, (x, y, k, hex(int(img[y, x])))
-            assert ref[y, x] == (0xFFFFFFFF if 0 < k < n_samples else 0), (x, y, k)
-            if 0 < k < n_samples:
+            assert ref[y, x] == (0xFFFFFFFF if 0 < k < n else 0), (x, y, k)
+            if 0 < k < n:
                 for s, (dx, dy) in enumerate(pos):
                     smp = plane_image(planes, 2 + s)[y, x] if s < 2 else None
                     if smp is not None:
