@@ -962,9 +962,13 @@ struct PassFormat
 	uint32_t fbp, fbw, zbp;
 };
 
-// intersect.h:106-244 (AA1 coverage is outside this build's scope; AA1 primitives rasterize
-// with plain coverage).
-int32_t evaluate_coverage(const Setup &s, int32_t cx, int32_t cy, uint32_t rate, float &i, float &j)
+// intersect.h:106-244: the coverage count of a sample, 0x80 where the sample centre is inside.
+// With AA1 (multisample) a line's coverage falls off linearly across its two-pixel-wide
+// parallelogram (128 - 256 |0.5 - j|), and a triangle sample that fails exactly one edge gets
+// a feathered coverage from its distance to that edge, measured along the edge's major axis in
+// 1/8-pixel units (never 0x80, so such samples never write depth).
+int32_t evaluate_coverage(const Setup &s, int32_t cx, int32_t cy, uint32_t rate, float &i, float &j,
+                          bool multisample = false, bool line = false)
 {
 	int32_t sx = cx >> rate, sy = cy >> rate;
 	if (!(sx >= s.bb[0] && sy >= s.bb[1] && sx <= s.bb[2] && sy <= s.bb[3]))
@@ -975,6 +979,7 @@ int32_t evaluate_coverage(const Setup &s, int32_t cx, int32_t cy, uint32_t rate,
 	int32_t a_eq[3] = { s.a[0] >> 6, s.a[1], s.a[2] };
 	int32_t px = int32_t(uint32_t(cx) << (3 - rate)), py = int32_t(uint32_t(cy) << (3 - rate));
 	int32_t A = idot3(a_eq, px, py), B = idot3(s.b, px, py), C = idot3(s.c, px, py);
+	const int32_t A0 = A, B0 = B, C0 = C;
 	float ti = float(B) * s.inv_area;
 	float tj = float(C) * s.inv_area;
 	i = ti + s.error_i;
@@ -985,7 +990,42 @@ int32_t evaluate_coverage(const Setup &s, int32_t cx, int32_t cy, uint32_t rate,
 		C = wadd(C, wadd(A, offc));
 		A = 0;
 	}
-	return (A >= 0 && B >= 0 && C >= 0) ? 0x80 : 0;
+	const bool centre = A >= 0 && B >= 0 && C >= 0;
+	if (multisample && line)
+	{
+		if (!centre)
+			return 0;
+		float d = std::fabs(0.5f - j);
+		return int32_t(std::max(0.0f, 128.0f - 256.0f * d));
+	}
+	if (centre)
+		return 0x80;
+	if (!multisample || para)
+		return 0;
+	const bool ea = A0 >= 0, eb = B0 >= 0, ec = C0 >= 0;
+	if (int(ea) + int(eb) + int(ec) != 2)
+		return 0;
+	float negative_bary;
+	int32_t max_delta;
+	if (!ea)
+	{
+		max_delta = std::max(std::abs(s.b[0] + s.c[0]), std::abs(s.b[1] + s.c[1]));
+		negative_bary = (i + j) - 1.0f;
+	}
+	else if (!eb)
+	{
+		max_delta = std::max(std::abs(s.b[0]), std::abs(s.b[1]));
+		negative_bary = -i;
+	}
+	else
+	{
+		max_delta = std::max(std::abs(s.c[0]), std::abs(s.c[1]));
+		negative_bary = -j;
+	}
+	float scaled_delta = float(max_delta) * s.inv_area;
+	float subpixels = negative_bary / scaled_delta;
+	float coverage = 8.0f - subpixels;
+	return int32_t(std::min(std::max(16.0f * coverage, 0.0f), 127.0f));
 }
 
 void barycentric(const Setup &s, int32_t cx, int32_t cy, uint32_t rate, float &i, float &j)
@@ -1520,7 +1560,7 @@ void GS::render(const std::vector<Prim> &prims_in)
 	                 auto &&get_regs) {
 		const uint32_t st = p.state;
 		const bool z_test = f.z_psm != 0xffffffffu && (st & S_Z_TEST);
-		const bool z_write = f.z_psm != 0xffffffffu && (st & S_Z_WRITE);
+		bool z_write = f.z_psm != 0xffffffffu && (st & S_Z_WRITE);
 		const bool z_greater = st & S_Z_GREATER;
 		const bool persp = st & S_PERSPECTIVE;
 		// shade_early (ubershader.comp:782-910): snapped primitives rasterize at the
@@ -1528,7 +1568,8 @@ void GS::render(const std::vector<Prim> &prims_in)
 		const bool snap_raster = super_sampled && (st & S_SNAP_RASTER);
 		const int32_t rx = snap_raster ? (hx & snap_mask) : hx, ry = snap_raster ? (hy & snap_mask) : hy;
 		float fi, fj;
-		int32_t cov = evaluate_coverage(s, rx, ry, rate, fi, fj);
+		const bool multisample = (st & S_MULTISAMPLE) != 0;
+		int32_t cov = evaluate_coverage(s, rx, ry, rate, fi, fj, multisample, (st & S_LINE) != 0);
 		if (cov == 0)
 			return;
 		if (st & (S_SCANMSK_EVEN | S_SCANMSK_ODD))
@@ -1537,6 +1578,9 @@ void GS::render(const std::vector<Prim> &prims_in)
 			if ((sy & 1) == 0 ? (st & S_SCANMSK_EVEN) : (st & S_SCANMSK_ODD))
 				return;
 		}
+		// AA1 without full coverage does not write depth (ubershader.comp:848-851).
+		if (multisample && cov < 0x80)
+			z_write = false;
 		covered_samples++;
 		Regs &r = get_regs();
 
@@ -1636,6 +1680,14 @@ void GS::render(const std::vector<Prim> &prims_in)
 		bool write_fb = true, write_zb = true, write_alpha = true;
 		const uint32_t bm = p.blend_mode;
 		bool blending = bm & B_ABE;
+		// AA1: the coverage replaces alpha 0x80 (or any alpha when not blending) and blending is
+		// forced on (ubershader.comp:1497-1505).
+		if (multisample)
+		{
+			if (comb[3] == 128.0f || !blending)
+				comb[3] = float(cov);
+			blending = true;
+		}
 		bool keep_all = false;
 		if (bm & B_ATE)
 		{

@@ -292,6 +292,43 @@ struct PassInfo
 	uint32_t zmax; // largest depth the Z format holds
 };
 
+// AA1 coverage of a sample (intersect.h:156-244): a line's coverage falls off linearly across
+// its two-pixel-wide parallelogram; a triangle sample outside exactly one edge gets a feathered
+// coverage from its distance to that edge along the edge's major axis (below 0x80).
+__device__ __forceinline__ uint32_t aa1_coverage(const PrimitiveSetup &s, bool line, bool centre, bool para,
+                                                 bool ea, bool eb, bool ec, float fi, float fj)
+{
+	if (line)
+	{
+		const float d = fabsf(__fsub_rn(0.5f, fj));
+		return centre ? uint32_t(int32_t(fmaxf(0.0f, __fsub_rn(128.0f, __fmul_rn(256.0f, d))))) : 0u;
+	}
+	if (centre)
+		return 0x80u;
+	if (para || int(ea) + int(eb) + int(ec) != 2)
+		return 0u;
+	int32_t max_delta;
+	float negative_bary;
+	if (!ea)
+	{
+		max_delta = max(abs(s.b[0] + s.c[0]), abs(s.b[1] + s.c[1]));
+		negative_bary = __fsub_rn(__fadd_rn(fi, fj), 1.0f);
+	}
+	else if (!eb)
+	{
+		max_delta = max(abs(s.b[0]), abs(s.b[1]));
+		negative_bary = -fi;
+	}
+	else
+	{
+		max_delta = max(abs(s.c[0]), abs(s.c[1]));
+		negative_bary = -fj;
+	}
+	const float scaled_delta = __fmul_rn(float(max_delta), s.inv_area);
+	const float cov = __fsub_rn(8.0f, __fdiv_rn(negative_bary, scaled_delta));
+	return uint32_t(int32_t(fminf(fmaxf(__fmul_rn(16.0f, cov), 0.0f), 127.0f)));
+}
+
 // Applies one primitive to this lane's sample (ubershader.comp shade_early + shade_resolve).
 // (hx, hy) is the sample's coordinate on the grid of 2^rate samples per pixel axis; the pixel
 // is (hx >> rate, hy >> rate). Single-sampled shading passes rate 0 and super_sampled false.
@@ -340,13 +377,20 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	int32_t C = idot3(s.c[0], s.c[1], s.c[2], sx, sy);
 	float fi = __fadd_rn(__fmul_rn(float(B), s.inv_area), s.error_i);
 	float fj = __fadd_rn(__fmul_rn(float(C), s.inv_area), s.error_j);
+	const bool edge_a = A >= 0, edge_b = B >= 0, edge_c = C >= 0;
 	if (para && A < 0)
 	{
 		B = wadd(B, wadd(A, offb));
 		C = wadd(C, wadd(A, offc));
 		A = 0;
 	}
-	alive = alive && A >= 0 && B >= 0 && C >= 0;
+	const bool centre = A >= 0 && B >= 0 && C >= 0;
+	// AA1 (wave-uniform state bit): coverage below 0x80 on line fringes and near triangle edges.
+	const bool multisample = (st & ST_MULTISAMPLE) != 0;
+	uint32_t coverage = centre ? 0x80u : 0u;
+	if (multisample)
+		coverage = aa1_coverage(s, (st & ST_LINE) != 0, centre, para, edge_a, edge_b, edge_c, fi, fj);
+	alive = alive && coverage != 0;
 
 	// Z (ubershader.comp:852-871), before the coverage exit: it reads the rest of the setup
 	// record, which then arrives with the first part.
@@ -482,6 +526,14 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	bool write_fb = true, write_zb = true, write_alpha = true;
 	const uint32_t bm = sv_blend;
 	bool blending = bm & BLEND_ABE;
+	if (multisample)
+	{
+		// AA1: coverage replaces alpha 0x80 (any alpha without blending), blending is forced on,
+		// and partial coverage does not write depth (ubershader.comp:848-851, 1497-1505).
+		comb[3] = (comb[3] == 128.0f || !blending) ? float(coverage) : comb[3];
+		blending = true;
+		write_zb = coverage >= 0x80u;
+	}
 	if (bm & BLEND_ATE)
 	{
 		const bool pass = alpha_test((bm >> 2) & 7u, p_alpha & 0xffu, comb[3]);
@@ -493,11 +545,11 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 				if (!alive)
 					return;
 			break;
-		case 1: write_zb = pass; break;
+		case 1: write_zb = write_zb && pass; break;
 		case 2: write_fb = pass; write_alpha = pass; break;
 		default:
 			if (pi.fb32) write_alpha = pass;
-			write_zb = pass;
+			write_zb = write_zb && pass;
 			break;
 		}
 	}

@@ -49,9 +49,10 @@ def rgbaq(rgba, q=1.0):
 
 
 def random_render_stream(seed: int, n_batches: int = 12, prims_per_batch: int = 40, width: int = 256,
-                         height: int = 192, textured: bool = False) -> np.ndarray:
+                         height: int = 192, textured: bool = False, aa1: bool = False) -> np.ndarray:
     """A sequence of batches; each batch randomises FB/Z format, test, blend, dither, masks
-    and primitive type, then draws primitives with A+D vertex writes."""
+    and primitive type, then draws primitives with A+D vertex writes. aa1: half of the
+    batches set PRIM.AA1 (the other seeds' streams are unchanged)."""
     rng = np.random.default_rng(seed)
     ofs = 2048 * 16
     out = [gif.ad([(0x1A, 1), (0x18, ofs | ofs << 32)])]
@@ -80,6 +81,8 @@ def random_render_stream(seed: int, n_batches: int = 12, prims_per_batch: int = 
         ptype = int(rng.choice([0, 1, 2, 3, 3, 3, 4, 5, 6, 6]))
         prim = prim_reg(ptype, iip=int(rng.integers(0, 2)), abe=int(rng.integers(0, 2)),
                         fge=int(rng.integers(0, 2)), fst=1, fix=int(rng.integers(0, 6) == 0))
+        if aa1 and rng.integers(0, 2):
+            prim |= 1 << 7
         regs.append((0x00, prim))
         out.append(gif.ad(regs))
         writes = []
