@@ -153,8 +153,8 @@ int pgs_set_super_sampling_rate(pgs_context *ctx, uint32_t super_sampling, uint3
 /* The current sample count (1, 2, 4, 8 or 16). */
 uint32_t pgs_get_super_sampling_rate(pgs_context *ctx);
 
-/* Hacks (gs/gs_interface.hpp:211-240). disable_mipmaps is accepted (this backend samples
- * LOD 0 only); unsynced_readbacks makes LOCAL->HOST transfers return zeros without waiting
+/* Hacks (gs/gs_interface.hpp:211-240). disable_mipmaps decodes level 0 only and samples it at
+ * LOD 0 with the magnification filter (gs/gs_interface.cpp:1539-1540, 1834); unsynced_readbacks makes LOCAL->HOST transfers return zeros without waiting
  * for the GPU (the reference does so when the data is not ready); backbuffer_promotion and
  * allow_blend_demote are not built: asking for them returns PGS_ERROR_UNSUPPORTED and the
  * other fields still apply. */

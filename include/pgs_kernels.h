@@ -84,6 +84,13 @@ int pgs_k_init_tables(const uint8_t *rcp_seed, const float *rcp_float_lut);
  * caller). Needs pgs_k_init_tables. Lets a test compare the device arithmetic with a CPU sweep. */
 int pgs_k_rcp_eval(uint32_t mode, uint32_t first, uint64_t count, uint64_t *hash, hipStream_t stream);
 
+/* Test hook: the shading kernel's Q-derived mipmap level of detail, ceil(K - 2^(L+4) log2|q|)
+ * in 1/16 units (gs/shaders/ubershader.comp:1076-1087, evaluated exactly), for L = l (0..3) and
+ * K = k (-2048..2047) over `count` consecutive float bit patterns from `first`; adds the wrapping
+ * 64-bit sum of ((lod bits << 32) | input) * 0x9E3779B97F4A7C15 to *hash (device memory, cleared
+ * by the caller). */
+int pgs_k_lod_eval(uint32_t l, int32_t k, uint32_t first, uint64_t count, uint64_t *hash, hipStream_t stream);
+
 /* Triangle / sprite / line / point setup: one thread per primitive.
  * Replaces triangle_setup.comp dispatched by GSRenderer::dispatch_triangle_setup
  * (gs/gs_renderer.cpp:1898-1977). Writes PrimitiveSetup (80 B) and TransformedAttributes

@@ -748,6 +748,30 @@ void orc_rcp_float_sweep(uint32_t first_bits, uint64_t count, uint64_t *hash, do
 	*max_ulp = worst;
 }
 
+// The oracle's Q-derived mipmap LOD over consecutive float bit patterns, hashed like the device
+// evaluation kernel (pgs_k_lod_eval); one evaluation through orc_lod_from_q.
+void orc_lod_sweep(uint32_t l, int32_t k, uint32_t first_bits, uint64_t count, uint64_t *hash)
+{
+	uint64_t h = 0;
+	for (uint64_t i = 0; i < count; i++)
+	{
+		const uint32_t u = uint32_t(first_bits + i);
+		float q;
+		memcpy(&q, &u, 4);
+		int32_t n;
+		orc::lod_from_q(q, k, l, n);
+		h += ((uint64_t(uint32_t(n)) << 32) | u) * 0x9E3779B97F4A7C15ull;
+	}
+	*hash = h;
+}
+
+int32_t orc_lod_from_q(float q, int32_t k, uint32_t l, int32_t *positive)
+{
+	int32_t n;
+	*positive = orc::lod_from_q(q, k, l, n) ? 1 : 0;
+	return n;
+}
+
 const uint8_t *orc_rcp_table() { return orc::rcp_table(); }
 float orc_rcp_fixed(uint32_t v) { return orc::rcp_fixed(v); }
 float orc_rcp_float(float v) { return orc::rcp_float(v); }

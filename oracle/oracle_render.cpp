@@ -520,8 +520,18 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 		uint32_t wms = field(cl, 0, 2), wmt = field(cl, 2, 2);
 		p.tex |= (wms == 1 || wms == 2) ? T_CLAMP_S : 0;
 		p.tex |= (wmt == 1 || wmt == 2) ? T_CLAMP_T : 0;
-		// Mip-mapping is outside this build's scope; LOD is fixed at 0.
-		p.tex2 = 1u;
+		// Mipmapped filters (gs/gs_interface.cpp:1834-1863): LCM | L << 1 | K << 3 in tex2, MXL
+		// at bit 21 of tex with the MMIN filter bits; otherwise a fixed LOD of 0.
+		const uint32_t mmin = field(t1, 6, 3);
+		if (mmin >= 2 && mmin <= 5)
+		{
+			p.tex2 = field(t1, 0, 1) | field(t1, 19, 2) << 1 | field(t1, 32, 12) << 3;
+			p.tex |= field(t1, 2, 3) << 21;
+			p.tex |= (mmin == 4 || mmin == 5) ? (1u << 17) : 0;
+			p.tex |= (mmin == 3 || mmin == 5) ? (1u << 18) : 0;
+		}
+		else
+			p.tex2 = 1u;
 	}
 	if (zte)
 	{
@@ -1501,7 +1511,8 @@ uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t cont
 }
 
 // Texture sampling lives in oracle_texture.cpp.
-void sample_texture(const TextureSlot &t, uint32_t tex, float u, float v, float q, bool perspective, float out[4]);
+void sample_texture(const TextureSlot &t, uint32_t tex, uint32_t tex2, float u, float v, float q, bool perspective,
+                    float out[4]);
 
 void GS::render(const std::vector<Prim> &prims_in)
 {
@@ -1637,7 +1648,7 @@ void GS::render(const std::vector<Prim> &prims_in)
 			u = std::fmin(std::fmax(u, -2047.0f), 2047.0f);
 			vv = std::fmin(std::fmax(vv, -2047.0f), 2047.0f);
 			float tx[4];
-			sample_texture(rp_textures[p.tex_slot], p.tex, u, vv, sq[2], persp, tx);
+			sample_texture(rp_textures[p.tex_slot], p.tex, p.tex2, u, vv, sq[2], persp, tx);
 			const bool tcc = p.combiner & C_TCC;
 			switch (p.combiner & 3u)
 			{

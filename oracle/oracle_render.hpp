@@ -67,9 +67,11 @@ struct Setup
 // A decoded texture of a render pass (gs/gs_interface.cpp:1754-1781 TexInfo content).
 struct TextureSlot
 {
-	uint64_t tex0, clamp, texa;
-	uint32_t width, height;     // effective rectangle size
-	std::vector<uint32_t> rgba; // decoded RGBA8 image (width x height)
+	uint64_t tex0, clamp, texa, mip1, mip2;
+	uint32_t levels;            // mip levels (1 without mipmapping)
+	uint32_t width, height;     // effective rectangle size (level 0)
+	std::vector<uint32_t> rgba; // decoded RGBA8 levels, one after another
+	uint32_t level_offset[7], level_width[7], level_height[7];
 	float size[4];              // 2^TW, 2^TH, 1 / width, 1 / height
 	float bias[2];              // -rect origin / rectangle size
 	float region[4];            // clamp region: min u, min v, max u, max v
@@ -81,6 +83,9 @@ struct PageSpan { uint32_t first, last; };
 PageSpan page_span(uint32_t base_block, uint32_t stride, uint32_t psm, uint32_t x0, uint32_t y0, uint32_t x1,
                    uint32_t y1);
 bool spans_overlap(PageSpan a, PageSpan b);
+
+// Q-derived mipmap level of detail in 1/16 units (oracle_texture.cpp); true when above 0.
+bool lod_from_q(float q, int32_t k, uint32_t l, int32_t &n);
 
 // VSyncInfo (gs/gs_interface.hpp:155-198) and ScanoutResult (gs/gs_renderer.hpp:23-52) as
 // flat words for ctypes.
@@ -202,7 +207,7 @@ private:
 	void render(const std::vector<Prim> &prims);
 	void load_clut(uint32_t ctx);
 	uint32_t texture_slot(uint32_t ctx);
-	PageSpan texture_span(uint32_t ctx) const;
+	std::vector<PageSpan> texture_span(uint32_t ctx) const;
 	bool texture_hazard(uint32_t ctx) const;
 };
 

@@ -131,24 +131,38 @@ void GS::load_clut(uint32_t ctx)
 
 namespace
 {
-// Effective texture rectangle and texel addressing of TEX0 / CLAMP (gs/gs_util.cpp:31-79 and
-// the upload addressing of gs/shaders/upload.comp:121-123).
+// Effective texture rectangle and texel addressing of TEX0 / CLAMP (gs/gs_util.cpp:31-79,
+// gs/gs_renderer.cpp:1235-1265 and the upload addressing of gs/shaders/upload.comp:121-123).
+// Level l of a mipmapped texture reads from its own TBP / TBW (MIPTBP1 for levels 1..3,
+// MIPTBP2 for 4..6) with the origin, MSK and FIX shifted right by l
+// (gs/gs_renderer.cpp:3502-3593).
 struct TexLayout
 {
-	uint32_t tbp, tbw, psm, tw, th;
+	uint32_t psm, tw, th, levels;
+	uint32_t tbp[7], tbw[7];
 	uint32_t width, height, bx, by;
 	uint32_t umsk, ufix, vmsk, vfix;
-	uint32_t tex_x(uint32_t x) const { return ((x & umsk) + bx) | ufix; }
-	uint32_t tex_y(uint32_t y) const { return ((y & vmsk) + by) | vfix; }
+	uint32_t tex_x(uint32_t x, uint32_t l) const { return ((x & (umsk >> l)) + (bx >> l)) | (ufix >> l); }
+	uint32_t tex_y(uint32_t y, uint32_t l) const { return ((y & (vmsk >> l)) + (by >> l)) | (vfix >> l); }
+	uint32_t level_width(uint32_t l) const { return std::max(width >> l, 1u); }
+	uint32_t level_height(uint32_t l) const { return std::max(height >> l, 1u); }
 };
 
-uint32_t effective_extent(uint32_t ext, uint32_t mode, uint32_t lo, uint32_t hi, uint32_t &base)
+uint32_t effective_extent(uint32_t ext, uint32_t mode, uint32_t lo, uint32_t hi, uint32_t levels, uint32_t &base)
 {
 	base = 0;
 	if (mode == 2)
 	{
 		lo = std::min(lo, hi);
 		ext = std::max(hi, lo) - lo + 1u;
+		if (levels > 1)
+		{
+			// Align the region to the coarsest level so every level has whole texels.
+			uint32_t mask = (1u << (levels - 1u)) - 1u;
+			ext += lo & mask;
+			lo &= ~mask;
+			ext = (ext + mask) & ~mask;
+		}
 		base = lo;
 	}
 	else if (mode == 3)
@@ -172,19 +186,38 @@ uint32_t effective_extent(uint32_t ext, uint32_t mode, uint32_t lo, uint32_t hi,
 	return ext;
 }
 
-TexLayout tex_layout(uint64_t tex0, uint64_t clamp)
+// Levels of the texture image: MXL + 1 when TEX1.MMIN is a mipmapped filter (2..5), at most
+// min(TW, TH) + 1 and 7 (gs/gs_renderer.cpp:1240-1243, gs/gs_interface.cpp:1539-1540).
+uint32_t texture_levels(uint64_t tex0, uint64_t tex1)
+{
+	uint32_t mmin = field(tex1, 6, 3);
+	if (mmin < 2 || mmin > 5)
+		return 1;
+	uint32_t tw = std::min(field(tex0, 26, 4), 10u), th = std::min(field(tex0, 30, 4), 10u);
+	return std::min(std::min(field(tex1, 2, 3) + 1u, std::min(tw, th) + 1u), 7u);
+}
+
+TexLayout tex_layout(uint64_t tex0, uint64_t clamp, uint32_t levels, uint64_t mip1, uint64_t mip2)
 {
 	TexLayout l{};
-	l.tbp = field(tex0, 0, 14);
-	l.tbw = field(tex0, 14, 6);
+	l.tbp[0] = field(tex0, 0, 14);
+	l.tbw[0] = field(tex0, 14, 6);
+	for (uint32_t i = 1; i < 7; i++)
+	{
+		uint64_t m = i <= 3 ? mip1 : mip2;
+		uint32_t o = ((i - 1) % 3) * 20;
+		l.tbp[i] = field(m, o, 14);
+		l.tbw[i] = field(m, o + 14, 6);
+	}
 	l.psm = field(tex0, 20, 6);
 	l.tw = std::min(field(tex0, 26, 4), 10u);
 	l.th = std::min(field(tex0, 30, 4), 10u);
+	l.levels = levels;
 	const uint32_t wms = field(clamp, 0, 2), wmt = field(clamp, 2, 2);
 	const uint32_t minu = field(clamp, 4, 10), maxu = field(clamp, 14, 10);
 	const uint32_t minv = field(clamp, 24, 10), maxv = field(clamp, 34, 10);
-	l.width = effective_extent(1u << l.tw, wms, minu, maxu, l.bx);
-	l.height = effective_extent(1u << l.th, wmt, minv, maxv, l.by);
+	l.width = effective_extent(1u << l.tw, wms, minu, maxu, levels, l.bx);
+	l.height = effective_extent(1u << l.th, wmt, minv, maxv, levels, l.by);
 	l.umsk = wms == 3 ? minu : 0xffffffffu;
 	l.ufix = wms == 3 ? maxu : 0u;
 	l.vmsk = wmt == 3 ? minv : 0xffffffffu;
@@ -221,39 +254,41 @@ bool spans_overlap(PageSpan a, PageSpan b)
 	return hit(a0, a1, b0, b1) || hit(a0 + pages, a1 + pages, b0, b1) || hit(a0, a1, b0 + pages, b1 + pages);
 }
 
-PageSpan GS::texture_span(uint32_t ctx) const
+std::vector<PageSpan> GS::texture_span(uint32_t ctx) const
 {
-	const TexLayout l = tex_layout(regs.tex0[ctx], regs.clamp[ctx]);
-	uint32_t x0 = ~0u, x1 = 0, y0 = ~0u, y1 = 0;
-	for (uint32_t x = 0; x < l.width; x++)
+	const uint32_t levels = texture_levels(regs.tex0[ctx], regs.tex1[ctx]);
+	const TexLayout l = tex_layout(regs.tex0[ctx], regs.clamp[ctx], levels, regs.miptbp1[ctx], regs.miptbp2[ctx]);
+	std::vector<PageSpan> spans;
+	for (uint32_t lv = 0; lv < levels; lv++)
 	{
-		x0 = std::min(x0, l.tex_x(x));
-		x1 = std::max(x1, l.tex_x(x));
+		uint32_t x0 = ~0u, x1 = 0, y0 = ~0u, y1 = 0;
+		for (uint32_t x = 0; x < l.level_width(lv); x++)
+		{
+			x0 = std::min(x0, l.tex_x(x, lv));
+			x1 = std::max(x1, l.tex_x(x, lv));
+		}
+		for (uint32_t y = 0; y < l.level_height(lv); y++)
+		{
+			y0 = std::min(y0, l.tex_y(y, lv));
+			y1 = std::max(y1, l.tex_y(y, lv));
+		}
+		spans.push_back(page_span(l.tbp[lv], l.tbw[lv], l.psm, x0, y0, x1, y1));
 	}
-	for (uint32_t y = 0; y < l.height; y++)
-	{
-		y0 = std::min(y0, l.tex_y(y));
-		y1 = std::max(y1, l.tex_y(y));
-	}
-	return page_span(l.tbp, l.tbw, l.psm, x0, y0, x1, y1);
+	return spans;
 }
 
-// Does the texture of context `ctx` read pages the pending render pass writes?
+// Does the texture of context `ctx` (any of its levels) read pages the pending render pass
+// writes?
 bool GS::texture_hazard(uint32_t ctx) const
 {
-	const PageSpan t = texture_span(ctx);
 	const uint32_t x0 = uint32_t(rp_bb[0]), y0 = uint32_t(rp_bb[1]), x1 = uint32_t(rp_bb[2]), y1 = uint32_t(rp_bb[3]);
 	const PageSpan fb = page_span(field(rp_frame, 0, 9) * 32u, field(rp_frame, 16, 6), field(rp_frame, 24, 6), x0, y0,
 	                              x1, y1);
-	if (spans_overlap(t, fb))
-		return true;
-	if (rp_zwrite)
-	{
-		const PageSpan z = page_span(field(rp_zbuf, 0, 9) * 32u, field(rp_frame, 16, 6),
-		                             0x30u | field(rp_zbuf, 24, 4), x0, y0, x1, y1);
-		if (spans_overlap(t, z))
+	const PageSpan z = page_span(field(rp_zbuf, 0, 9) * 32u, field(rp_frame, 16, 6), 0x30u | field(rp_zbuf, 24, 4), x0,
+	                             y0, x1, y1);
+	for (const PageSpan &t : texture_span(ctx))
+		if (spans_overlap(t, fb) || (rp_zwrite && spans_overlap(t, z)))
 			return true;
-	}
 	return false;
 }
 
@@ -263,15 +298,27 @@ uint32_t GS::texture_slot(uint32_t ctx)
 {
 	const uint64_t key0 = regs.tex0[ctx] & ((1ull << 61) - 1ull); // CLD does not affect the texture
 	const uint64_t keyc = regs.clamp[ctx], keya = regs.texa;
+	const uint32_t levels = texture_levels(key0, regs.tex1[ctx]);
+	// Only the used levels' MIPTBP fields select the image.
+	const uint32_t n1 = std::min(levels - 1u, 3u), n2 = levels > 4 ? levels - 4u : 0u;
+	const uint64_t mip1 = regs.miptbp1[ctx] & ((1ull << (20 * n1)) - 1ull);
+	const uint64_t mip2 = regs.miptbp2[ctx] & ((1ull << (20 * n2)) - 1ull);
 	for (uint32_t i = 0; i < rp_textures.size(); i++)
-		if (rp_textures[i].tex0 == key0 && rp_textures[i].clamp == keyc && rp_textures[i].texa == keya)
+	{
+		const TextureSlot &o = rp_textures[i];
+		if (o.tex0 == key0 && o.clamp == keyc && o.texa == keya && o.levels == levels && o.mip1 == mip1 &&
+		    o.mip2 == mip2)
 			return i;
+	}
 
 	TextureSlot t{};
 	t.tex0 = key0;
 	t.clamp = keyc;
 	t.texa = keya;
-	const TexLayout l = tex_layout(key0, keyc);
+	t.mip1 = mip1;
+	t.mip2 = mip2;
+	t.levels = levels;
+	const TexLayout l = tex_layout(key0, keyc, levels, mip1, mip2);
 	const uint32_t psm = l.psm;
 	const uint32_t cpsm0 = field(key0, 51, 4);
 	const bool clut16 = cpsm0 == CT16 || cpsm0 == CT16S;
@@ -306,12 +353,21 @@ uint32_t GS::texture_slot(uint32_t ctx)
 		return uint32_t(clut[bi & 511u]) | (uint32_t(clut[(bi + 256u) & 511u]) << 16);
 	};
 
-	t.rgba.resize(size_t(t.width) * t.height);
+	size_t total = 0;
+	for (uint32_t lv = 0; lv < levels; lv++)
+	{
+		t.level_offset[lv] = uint32_t(total);
+		t.level_width[lv] = l.level_width(lv);
+		t.level_height[lv] = l.level_height(lv);
+		total += size_t(t.level_width[lv]) * t.level_height[lv];
+	}
+	t.rgba.resize(total);
 	const uint8_t *v = vram.data();
-	for (uint32_t y = 0; y < t.height; y++)
-		for (uint32_t x = 0; x < t.width; x++)
+	for (uint32_t lv = 0; lv < levels; lv++)
+	for (uint32_t y = 0; y < t.level_height[lv]; y++)
+		for (uint32_t x = 0; x < t.level_width[lv]; x++)
 		{
-			uint32_t a = swizzle(l.tex_x(x), l.tex_y(y), l.tbp, l.tbw, psm);
+			uint32_t a = swizzle(l.tex_x(x, lv), l.tex_y(y, lv), l.tbp[lv], l.tbw[lv], psm);
 			uint32_t c = 0;
 			switch (psm)
 			{
@@ -343,7 +399,7 @@ uint32_t GS::texture_slot(uint32_t ctx)
 				memcpy(&c, v + a * 4u, 4);
 				break;
 			}
-			t.rgba[size_t(y) * t.width + x] = c;
+			t.rgba[t.level_offset[lv] + size_t(y) * t.level_width[lv] + x] = c;
 		}
 
 	rp_textures.push_back(std::move(t));
@@ -352,11 +408,13 @@ uint32_t GS::texture_slot(uint32_t ctx)
 
 namespace
 {
-inline uint32_t texel(const TextureSlot &t, int32_t x, int32_t y)
+// Texel (x, y) of an image level with REPEAT addressing.
+inline uint32_t texel(const TextureSlot &t, uint32_t lv, int32_t x, int32_t y)
 {
-	uint32_t xx = uint32_t(((x % int32_t(t.width)) + int32_t(t.width)) % int32_t(t.width));
-	uint32_t yy = uint32_t(((y % int32_t(t.height)) + int32_t(t.height)) % int32_t(t.height));
-	return t.rgba[size_t(yy) * t.width + xx];
+	const int32_t w = int32_t(t.level_width[lv]), h = int32_t(t.level_height[lv]);
+	uint32_t xx = uint32_t(((x % w) + w) % w);
+	uint32_t yy = uint32_t(((y % h) + h) % h);
+	return t.rgba[t.level_offset[lv] + size_t(yy) * uint32_t(w) + xx];
 }
 
 inline uint32_t lerp8(uint32_t a, uint32_t b, uint32_t w)
@@ -364,52 +422,139 @@ inline uint32_t lerp8(uint32_t a, uint32_t b, uint32_t w)
 	return uint32_t(int32_t(a) + ((int32_t(b) - int32_t(a)) * int32_t(w) + 128) / 256 -
 	                (((int32_t(b) - int32_t(a)) * int32_t(w) + 128) % 256 < 0 ? 1 : 0));
 }
-}
 
-// ubershader.comp:1042-1258 (non-array path, LOD 0).
-void sample_texture(const TextureSlot &t, uint32_t tex, float u, float v, float, bool perspective, float out[4])
+// A Vulkan textureLod fetch with REPEAT addressing at integer level `lod` of the image, which
+// reads level min(lod, levels - 1), for normalized coordinates (un, vn).
+uint32_t fetch(const TextureSlot &t, uint32_t lod, float un, float vn, bool linear)
 {
-	const bool clamp_s = (tex & (1u << 19)) != 0, clamp_t = (tex & (1u << 20)) != 0;
-	const bool linear = (tex & (1u << 16)) != 0;
-	float u1 = perspective ? u * t.size[0] : u;
-	float v1 = perspective ? v * t.size[1] : v;
-	if (clamp_s)
-		u1 = std::min(std::max(u1, std::floor(t.region[0]) + 0.5f), std::floor(t.region[2]) + 0.5f);
-	if (clamp_t)
-		v1 = std::min(std::max(v1, std::floor(t.region[1]) + 0.5f), std::floor(t.region[3]) + 0.5f);
-	float un = std::floor(u1 * 16.0f + 1.0f / 64.0f);
-	float vn = std::floor(v1 * 16.0f + 1.0f / 64.0f);
-	un = un + 16.0f / 1024.0f;
-	vn = vn + 16.0f / 1024.0f;
-	un = un * t.size[2];
-	vn = vn * t.size[3];
-	un = un * 0.0625f;
-	vn = vn * 0.0625f;
-	un = un + t.bias[0];
-	vn = vn + t.bias[1];
-
-	const float fw = float(t.width), fh = float(t.height);
-	uint32_t c;
+	const uint32_t lv = std::min(lod, t.levels - 1u);
+	const float fw = float(t.level_width[lv]), fh = float(t.level_height[lv]);
 	if (!linear)
 	{
 		float xs = std::floor(un * fw), ys = std::floor(vn * fh);
-		c = texel(t, int32_t(xs), int32_t(ys));
+		return texel(t, lv, int32_t(xs), int32_t(ys));
 	}
-	else
+	float xs = un * fw - 0.5f, ys = vn * fh - 0.5f;
+	float x0 = std::floor(xs), y0 = std::floor(ys);
+	uint32_t wx = uint32_t((xs - x0) * 256.0f), wy = uint32_t((ys - y0) * 256.0f);
+	uint32_t c00 = texel(t, lv, int32_t(x0), int32_t(y0)), c10 = texel(t, lv, int32_t(x0) + 1, int32_t(y0));
+	uint32_t c01 = texel(t, lv, int32_t(x0), int32_t(y0) + 1), c11 = texel(t, lv, int32_t(x0) + 1, int32_t(y0) + 1);
+	uint32_t c = 0;
+	for (uint32_t ch = 0; ch < 4; ch++)
 	{
-		float xs = un * fw - 0.5f, ys = vn * fh - 0.5f;
-		float x0 = std::floor(xs), y0 = std::floor(ys);
-		uint32_t wx = uint32_t((xs - x0) * 256.0f), wy = uint32_t((ys - y0) * 256.0f);
-		uint32_t c00 = texel(t, int32_t(x0), int32_t(y0)), c10 = texel(t, int32_t(x0) + 1, int32_t(y0));
-		uint32_t c01 = texel(t, int32_t(x0), int32_t(y0) + 1), c11 = texel(t, int32_t(x0) + 1, int32_t(y0) + 1);
-		c = 0;
+		uint32_t sh = ch * 8u;
+		uint32_t top = lerp8((c00 >> sh) & 0xffu, (c10 >> sh) & 0xffu, wx);
+		uint32_t bot = lerp8((c01 >> sh) & 0xffu, (c11 >> sh) & 0xffu, wx);
+		c |= lerp8(top, bot, wy) << sh;
+	}
+	return c;
+}
+
+// texture_clamp (ubershader.comp:947-953): the region scaled to the level.
+inline float texture_clamp(float coord, float lo, float hi, float lod)
+{
+	float a = std::floor(std::ldexp(lo, -int(lod))) + 0.5f;
+	float b = std::floor(std::ldexp(hi, -int(lod))) + 0.5f;
+	return std::min(std::max(coord, a), b);
+}
+}
+
+// The Q-derived level of detail ceil(ldexp(-log2|q|, L + 4) + K) (ubershader.comp:1076-1087)
+// evaluated exactly rather than with a rounded log2 (the precision of log2 is the
+// implementation's): with |q| = m * 2^e, m in [1, 2), it is K - 2^(L+4) e - floor(2^(L+4) log2 m).
+// 2^(L+4) log2 m is irrational for m != 1, so the long-double floor is exact unless it lies
+// within 2^-50 or so of an integer (the tests check every threshold). A zero, subnormal,
+// infinite or NaN q gives a formula of 0, as the reference's non-finite flush does (a subnormal
+// q taken as flushed to zero). Returns whether the formula is above 0.
+bool lod_from_q(float q, int32_t k, uint32_t l, int32_t &n)
+{
+	const float a = std::fabs(q);
+	if (!(a >= 0x1p-126f) || std::isinf(a))
+	{
+		n = 0;
+		return false;
+	}
+	int e;
+	const float m = std::frexp(a, &e) * 2.0f; // in [1, 2)
+	e -= 1;
+	const long double s = std::ldexp((long double)std::log2((long double)m), int(l) + 4);
+	n = k - (16 << l) * e - int32_t(std::floor(s));
+	return n > 0;
+}
+
+// ubershader.comp:1042-1258 (non-array path).
+void sample_texture(const TextureSlot &t, uint32_t tex, uint32_t tex2, float u, float v, float q, bool perspective,
+                    float out[4])
+{
+	const bool clamp_s = (tex & (1u << 19)) != 0, clamp_t = (tex & (1u << 20)) != 0;
+	const bool mag_linear = (tex & (1u << 16)) != 0, min_linear = (tex & (1u << 17)) != 0;
+	const bool trilinear = (tex & (1u << 18)) != 0;
+	const int32_t max_miplevel = int32_t((tex >> 21) & 7u);
+
+	// LOD (ubershader.comp:1066-1131).
+	float lod = 0.0f;
+	bool bilinear = mag_linear;
+	if (max_miplevel > 0)
+	{
+		const int32_t k = int32_t(tex2 << 17) >> 20; // bits 3..14, signed
+		if (!(tex2 & 1u))
+		{
+			int32_t n;
+			const bool positive = lod_from_q(q, k, (tex2 >> 1) & 3u, n);
+			lod = float(n) * 0.0625f;
+			bilinear = positive ? min_linear : mag_linear;
+		}
+		else
+		{
+			lod = float(k >> 4);
+			bilinear = lod > 0.0f ? min_linear : mag_linear;
+		}
+	}
+	lod = std::min(std::max(lod, 0.0f), float(max_miplevel));
+	float lod_1 = std::floor(lod), lod_2 = std::ceil(lod);
+	float fract_lod = lod - lod_1;
+	if (!trilinear)
+	{
+		fract_lod = 0.0f;
+		lod_1 = float(int(lod + 0.5f));
+		lod_2 = lod_1;
+	}
+
+	const float bs[2] = { perspective ? t.size[0] : 1.0f, perspective ? t.size[1] : 1.0f };
+	float u1 = u * std::ldexp(bs[0], -int(lod_1));
+	float v1 = v * std::ldexp(bs[1], -int(lod_1));
+	if (clamp_s)
+		u1 = texture_clamp(u1, t.region[0], t.region[2], lod_1);
+	if (clamp_t)
+		v1 = texture_clamp(v1, t.region[1], t.region[3], lod_1);
+	const float inv1[2] = { std::ldexp(t.size[2], int(lod_1)), std::ldexp(t.size[3], int(lod_1)) };
+	float un = std::floor(u1 * 16.0f + 1.0f / 64.0f);
+	float vn = std::floor(v1 * 16.0f + 1.0f / 64.0f);
+	un = (un + 16.0f / 1024.0f) * inv1[0] * 0.0625f + t.bias[0];
+	vn = (vn + 16.0f / 1024.0f) * inv1[1] * 0.0625f + t.bias[1];
+	uint32_t c = fetch(t, uint32_t(lod_1), un, vn, bilinear);
+
+	if (fract_lod != 0.0f)
+	{
+		float u2 = u * std::ldexp(bs[0], -int(lod_2));
+		float v2 = v * std::ldexp(bs[1], -int(lod_2));
+		if (clamp_s)
+			u2 = texture_clamp(u2, t.region[0], t.region[2], lod_2);
+		if (clamp_t)
+			v2 = texture_clamp(v2, t.region[1], t.region[3], lod_2);
+		const float inv2[2] = { std::ldexp(t.size[2], int(lod_2)), std::ldexp(t.size[3], int(lod_2)) };
+		// The second level is normalized without the rectangle origin (bias), as in the reference.
+		float un2 = (std::floor(u2 * 16.0f + 1.0f / 64.0f) + 16.0f / 1024.0f) * inv2[0] * 0.0625f;
+		float vn2 = (std::floor(v2 * 16.0f + 1.0f / 64.0f) + 16.0f / 1024.0f) * inv2[1] * 0.0625f;
+		const uint32_t c2 = fetch(t, uint32_t(lod_2), un2, vn2, bilinear);
+		const float weight = std::floor(fract_lod * 16.0f);
+		uint32_t r = 0;
 		for (uint32_t ch = 0; ch < 4; ch++)
 		{
-			uint32_t sh = ch * 8u;
-			uint32_t top = lerp8((c00 >> sh) & 0xffu, (c10 >> sh) & 0xffu, wx);
-			uint32_t bot = lerp8((c01 >> sh) & 0xffu, (c11 >> sh) & 0xffu, wx);
-			c |= lerp8(top, bot, wy) << sh;
+			const float a = float((c >> (8 * ch)) & 0xffu), b = float((c2 >> (8 * ch)) & 0xffu);
+			r |= uint32_t(std::nearbyint(((16.0f - weight) * a + b * weight) * (1.0f / 16.0f))) << (8 * ch);
 		}
+		c = r;
 	}
 	for (uint32_t ch = 0; ch < 4; ch++)
 		out[ch] = float((c >> (8 * ch)) & 0xffu);

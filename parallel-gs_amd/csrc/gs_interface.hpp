@@ -96,7 +96,11 @@ public:
 	// not build or that exceeds the allocated planes.
 	bool set_super_sampling_rate(SuperSampling rate, bool ordered_grid, bool super_sampled_textures);
 	uint32_t super_sampling_rate() const { return 1u << renderer_.sample_log2(); }
-	void set_hacks(const Hacks &h) { hacks_ = h; }
+	void set_hacks(const Hacks &h)
+	{
+		hacks_ = h;
+		rp_.disable_mipmaps = h.disable_mipmaps;
+	}
 	void set_signal_interface(const SignalCallbacks &cb) { signal_ = cb; }
 	void set_timestamps(bool enable);
 	void set_sampler_feedback(uint32_t mode) { renderer_.set_sampler_feedback(mode); }

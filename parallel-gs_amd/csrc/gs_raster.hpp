@@ -23,7 +23,9 @@ enum : uint32_t
 	TEX_INDEX_MASK = 0xffffu,
 	TEX_MAG_LINEAR = 1u << 16, TEX_MIN_LINEAR = 1u << 17, TEX_MIP_LINEAR = 1u << 18,
 	TEX_CLAMP_S = 1u << 19, TEX_CLAMP_T = 1u << 20,
-	TEX2_FIXED_LOD = 1u << 0,
+	TEX_MAX_MIP_LEVEL_SHIFT = 21, // TEX1.MXL of a mipmapped primitive, 3 bits
+	TEX2_FIXED_LOD = 1u << 0,     // LCM; L in bits 1..2, K (signed, 1/16 units) in bits 3..14
+	TEX2_L_SHIFT = 1, TEX2_K_SHIFT = 3,
 };
 
 enum : uint32_t
@@ -90,8 +92,8 @@ static_assert(sizeof(TransformedAttributes) == 112, "TransformedAttributes layou
 struct TextureDesc
 {
 	uint32_t image_offset; // offset in dwords into the texture image arena
-	uint32_t width, height; // effective rectangle size (the decoded image)
-	uint32_t pad0;
+	uint32_t width, height; // effective rectangle size (the decoded image's level 0)
+	uint32_t levels;        // mip levels, stored one after another from image_offset
 	// Clamp bounds of the texel coordinate (min u, min v, max u, max v): the sampler's
 	// floor(region) + 0.5 on an axis with CLAMP or REGION_CLAMP, -inf / +inf on the others, so
 	// that the clamp is unconditional; min <= max (a region with min > max clamps everything to

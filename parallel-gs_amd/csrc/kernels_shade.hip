@@ -12,6 +12,7 @@
 // keep that band's primitive data in their own L2.
 #include "gs_defs.hpp"
 #include "gs_raster.hpp"
+#include "lod_table.hpp"
 #include "pgs_kernels.h"
 #include "multi_launch.hpp"
 #include <algorithm>
@@ -238,48 +239,150 @@ __device__ __forceinline__ uint32_t lerp8x4(uint32_t a, uint32_t b, uint32_t w)
 	return ((even >> 8) & M) | (odd & ~M);
 }
 
-// Non-array sampling path of sample_texture (ubershader.comp:1042-1258) at LOD 0: UV snap to
-// 1/16 texel with the reference's epsilons, then a NEAREST or LINEAR fetch with REPEAT
-// addressing. LINEAR uses 8-bit fractional weights and an integer lerp (U, then V). Decoded
-// images are at most 1024 texels on a side, so texel offsets fit the 24-bit multiply.
-__device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint32_t *images, uint32_t tex, float u,
-                                               float v, bool perspective, float out[4])
+// One level's texel fetch for normalized coordinates (un, vn) on a w x h image: NEAREST, or
+// LINEAR with 8-bit fractional weights and an integer lerp (U, then V), both with REPEAT
+// addressing. Images are at most 1024 texels on a side, so texel offsets fit the 24-bit
+// multiply. fw, fh are the sizes as floats and inv_w, inv_h their rounded reciprocals.
+__device__ __forceinline__ uint32_t fetch_level(const uint32_t *img, uint32_t w, uint32_t h, float fw, float fh,
+                                                float inv_w, float inv_h, float un, float vn, bool linear)
 {
-	const uint32_t *img = images + t.image_offset;
-	float u1 = perspective ? __fmul_rn(u, t.size[0]) : u;
-	float v1 = perspective ? __fmul_rn(v, t.size[1]) : v;
-	// CLAMP / REGION_CLAMP; the descriptor holds floor(region) + 0.5, or infinities on an axis
-	// that does not clamp, with min <= max (a reversed region is stored as [max, max], which
-	// min(max(x, lo), hi) also gives). u1 and v1 are finite (u and v were clamped to +-2047), so
-	// the median of (x, lo, hi) is that clamp.
-	u1 = __builtin_amdgcn_fmed3f(u1, t.region[0], t.region[2]);
-	v1 = __builtin_amdgcn_fmed3f(v1, t.region[1], t.region[3]);
+	if (!linear)
+	{
+		const uint32_t x = wrap_texel(floorf(__fmul_rn(un, fw)), fw, inv_w);
+		const uint32_t y = wrap_texel(floorf(__fmul_rn(vn, fh)), fh, inv_h);
+		return img[__umul24(y, w) + x];
+	}
+	const float xs = __fsub_rn(__fmul_rn(un, fw), 0.5f), ys = __fsub_rn(__fmul_rn(vn, fh), 0.5f);
+	const float x0 = floorf(xs), y0 = floorf(ys);
+	const uint32_t wx = uint32_t(__fmul_rn(__fsub_rn(xs, x0), 256.0f));
+	const uint32_t wy = uint32_t(__fmul_rn(__fsub_rn(ys, y0), 256.0f));
+	// REPEAT on the decoded image; the second texel of each axis wraps from the first.
+	const uint32_t ax = wrap_texel(x0, fw, inv_w), ay = wrap_texel(y0, fh, inv_h);
+	const uint32_t bx = ax + 1u == w ? 0u : ax + 1u, by = ay + 1u == h ? 0u : ay + 1u;
+	const uint32_t r0 = __umul24(ay, w), r1 = __umul24(by, w);
+	const uint32_t c00 = img[r0 + ax], c10 = img[r0 + bx], c01 = img[r1 + ax], c11 = img[r1 + bx];
+	// wx, wy: fractions below 1 times 256, so at most 256 (a fraction just below 1 can round
+	// to 1).
+	return lerp8x4(lerp8x4(c00, c10, wx), lerp8x4(c01, c11, wx), wy);
+}
+
+// The Q-derived level of detail in 1/16 units, ceil(K - 2^(L+4) * log2|q|) (ubershader.comp:
+// 1076-1087), exactly: with |q| = m * 2^e, m in [1, 2), it is K - 2^(L+4) * e -
+// floor(2^(L+4) * log2 m), and floor(128 * log2 m) is the largest k with m >= kLodThreshold[k].
+// A zero, subnormal, infinite or NaN q gives 0 (the reference flushes non-finite LODs to 0; a
+// subnormal q is taken as flushed to zero).
+__device__ __forceinline__ int32_t lod_from_q(float q, int32_t k, uint32_t l)
+{
+	const uint32_t b = __float_as_uint(q) & 0x7fffffffu;
+	const uint32_t e = b >> 23;
+	if (e == 0u || e == 255u)
+		return 0;
+	const float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);
+	// An estimate from the hardware log, then corrected against the exact thresholds.
+	int32_t j = min(max(int32_t(__fmul_rn(__log2f(m), 128.0f)), 0), 127);
+	while (j > 0 && m < kLodThreshold[j])
+		j--;
+	while (j < 127 && m >= kLodThreshold[j + 1])
+		j++;
+	return k - (int32_t(e) - 127) * (16 << l) - (j >> (3u - l));
+}
+
+// Level `lod` of a mipmapped texture (ubershader.comp:1133-1254): coordinates scaled by 2^-lod,
+// clamped to the region scaled to the level (texture_clamp, ubershader.comp:947-953), snapped
+// to 1/16 texel, normalized with the level's 2^lod / width; the image level read is lod clamped
+// to the image's last level. `bias` is false for the second level of a trilinear blend, which
+// the reference normalizes without the rectangle origin.
+__device__ __forceinline__ uint32_t sample_level(const TextureDesc &t, const uint32_t *images, uint32_t tex, float u,
+                                                 float v, bool perspective, uint32_t lod, bool linear, bool bias)
+{
+	const int32_t nl = -int32_t(lod);
+	float u1 = __fmul_rn(u, ldexpf(perspective ? t.size[0] : 1.0f, nl));
+	float v1 = __fmul_rn(v, ldexpf(perspective ? t.size[1] : 1.0f, nl));
+	// The descriptor's bounds are floor(region) + 0.5 (or infinite): region - 0.5 is the region.
+	auto bound = [&](float r) { return __fadd_rn(floorf(ldexpf(__fsub_rn(r, 0.5f), nl)), 0.5f); };
+	if (tex & TEX_CLAMP_S)
+		u1 = __builtin_amdgcn_fmed3f(u1, bound(t.region[0]), bound(t.region[2]));
+	if (tex & TEX_CLAMP_T)
+		v1 = __builtin_amdgcn_fmed3f(v1, bound(t.region[1]), bound(t.region[3]));
 	float un = floorf(__fadd_rn(__fmul_rn(u1, 16.0f), 1.0f / 64.0f));
 	float vn = floorf(__fadd_rn(__fmul_rn(v1, 16.0f), 1.0f / 64.0f));
-	un = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(un, 16.0f / 1024.0f), t.size[2]), 0.0625f), t.bias[0]);
-	vn = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(vn, 16.0f / 1024.0f), t.size[3]), 0.0625f), t.bias[1]);
-	const float fw = t.fwidth, fh = t.fheight;
-	uint32_t c;
-	if (!(tex & TEX_MAG_LINEAR))
+	un = __fmul_rn(__fmul_rn(__fadd_rn(un, 16.0f / 1024.0f), ldexpf(t.size[2], int32_t(lod))), 0.0625f);
+	vn = __fmul_rn(__fmul_rn(__fadd_rn(vn, 16.0f / 1024.0f), ldexpf(t.size[3], int32_t(lod))), 0.0625f);
+	if (bias)
 	{
-		const uint32_t x = wrap_texel(floorf(__fmul_rn(un, fw)), fw, t.size[2]);
-		const uint32_t y = wrap_texel(floorf(__fmul_rn(vn, fh)), fh, t.size[3]);
-		c = img[__umul24(y, t.width) + x];
+		un = __fadd_rn(un, t.bias[0]);
+		vn = __fadd_rn(vn, t.bias[1]);
 	}
+	// The image level: lod, at most the last. Every level but a 1-texel-wide clamped one divides
+	// the level-0 size exactly (sizes are powers of two, or multiples of 2^(levels - 1)), so the
+	// level's reciprocal size is the level-0 one scaled.
+	const uint32_t lv = min(lod, t.levels - 1u);
+	uint32_t off = t.image_offset;
+	for (uint32_t i = 0; i < lv; i++)
+		off += max(t.width >> i, 1u) * max(t.height >> i, 1u);
+	const uint32_t w = t.width >> lv, h = t.height >> lv;
+	const float inv_w = w ? ldexpf(t.size[2], int32_t(lv)) : 1.0f, inv_h = h ? ldexpf(t.size[3], int32_t(lv)) : 1.0f;
+	const uint32_t w1 = max(w, 1u), h1 = max(h, 1u);
+	return fetch_level(images + off, w1, h1, float(w1), float(h1), inv_w, inv_h, un, vn, linear);
+}
+
+// Mipmapped sampling (ubershader.comp:1066-1131, 1233-1254): the level of detail from LCM / K
+// or from Q, clamped to [0, MXL]; the filter is MMIN's when the LOD is above 0, MMAG's
+// otherwise. Trilinear blends the two nearest levels with floor(fract * 16) / 16 weights,
+// rounding half to even; otherwise the nearest level is read.
+__device__ __forceinline__ uint32_t sample_mipmapped(const TextureDesc &t, const uint32_t *images, uint32_t tex,
+                                                     uint32_t tex2, float u, float v, float q, bool perspective)
+{
+	const uint32_t mxl = (tex >> TEX_MAX_MIP_LEVEL_SHIFT) & 7u;
+	const int32_t k = int32_t(tex2 << (32u - TEX2_K_SHIFT - 12u)) >> 20; // signed 12-bit K
+	int32_t n;
+	if (tex2 & TEX2_FIXED_LOD)
+		n = (k >> 4) * 16;
+	else
+		n = lod_from_q(q, k, (tex2 >> TEX2_L_SHIFT) & 3u);
+	const bool linear = (tex & (n > 0 ? TEX_MIN_LINEAR : TEX_MAG_LINEAR)) != 0;
+	const uint32_t lod16 = uint32_t(min(max(n, 0), int32_t(mxl * 16u)));
+	if (!(tex & TEX_MIP_LINEAR) || (lod16 & 15u) == 0u)
+		return sample_level(t, images, tex, u, v, perspective, (tex & TEX_MIP_LINEAR) ? lod16 >> 4 : (lod16 + 8u) >> 4,
+		                    linear, true);
+	const uint32_t c1 = sample_level(t, images, tex, u, v, perspective, lod16 >> 4, linear, true);
+	const uint32_t c2 = sample_level(t, images, tex, u, v, perspective, (lod16 + 15u) >> 4, linear, false);
+	const uint32_t w = lod16 & 15u;
+	uint32_t c = 0;
+	for (uint32_t ch = 0; ch < 32u; ch += 8u)
+	{
+		const uint32_t x = ((c1 >> ch) & 0xffu) * (16u - w) + ((c2 >> ch) & 0xffu) * w; // x / 16, half to even
+		c |= ((x + 7u + ((x >> 4) & 1u)) >> 4) << ch;
+	}
+	return c;
+}
+
+// Non-array sampling path of sample_texture (ubershader.comp:1042-1258). At LOD 0 (no mipmap):
+// UV snap to 1/16 texel with the reference's epsilons, then a NEAREST or LINEAR fetch.
+__device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint32_t *images, uint32_t tex,
+                                               uint32_t tex2, float u, float v, float q, bool perspective,
+                                               float out[4])
+{
+	uint32_t c;
+	if (tex & (7u << TEX_MAX_MIP_LEVEL_SHIFT))
+		c = sample_mipmapped(t, images, tex, tex2, u, v, q, perspective);
 	else
 	{
-		const float xs = __fsub_rn(__fmul_rn(un, fw), 0.5f), ys = __fsub_rn(__fmul_rn(vn, fh), 0.5f);
-		const float x0 = floorf(xs), y0 = floorf(ys);
-		const uint32_t wx = uint32_t(__fmul_rn(__fsub_rn(xs, x0), 256.0f));
-		const uint32_t wy = uint32_t(__fmul_rn(__fsub_rn(ys, y0), 256.0f));
-		// REPEAT on the decoded image; the second texel of each axis wraps from the first.
-		const uint32_t ax = wrap_texel(x0, fw, t.size[2]), ay = wrap_texel(y0, fh, t.size[3]);
-		const uint32_t bx = ax + 1u == t.width ? 0u : ax + 1u, by = ay + 1u == t.height ? 0u : ay + 1u;
-		const uint32_t r0 = __umul24(ay, t.width), r1 = __umul24(by, t.width);
-		const uint32_t c00 = img[r0 + ax], c10 = img[r0 + bx], c01 = img[r1 + ax], c11 = img[r1 + bx];
-		// wx, wy: fractions below 1 times 256, so at most 256 (a fraction just below 1 can round
-		// to 1).
-		c = lerp8x4(lerp8x4(c00, c10, wx), lerp8x4(c01, c11, wx), wy);
+		const uint32_t *img = images + t.image_offset;
+		float u1 = perspective ? __fmul_rn(u, t.size[0]) : u;
+		float v1 = perspective ? __fmul_rn(v, t.size[1]) : v;
+		// CLAMP / REGION_CLAMP; the descriptor holds floor(region) + 0.5, or infinities on an
+		// axis that does not clamp, with min <= max (a reversed region is stored as [max, max],
+		// which min(max(x, lo), hi) also gives). u1 and v1 are finite (u and v were clamped to
+		// +-2047), so the median of (x, lo, hi) is that clamp.
+		u1 = __builtin_amdgcn_fmed3f(u1, t.region[0], t.region[2]);
+		v1 = __builtin_amdgcn_fmed3f(v1, t.region[1], t.region[3]);
+		float un = floorf(__fadd_rn(__fmul_rn(u1, 16.0f), 1.0f / 64.0f));
+		float vn = floorf(__fadd_rn(__fmul_rn(v1, 16.0f), 1.0f / 64.0f));
+		un = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(un, 16.0f / 1024.0f), t.size[2]), 0.0625f), t.bias[0]);
+		vn = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(vn, 16.0f / 1024.0f), t.size[3]), 0.0625f), t.bias[1]);
+		c = fetch_level(img, t.width, t.height, t.fwidth, t.fheight, t.size[2], t.size[3], un, vn,
+		                (tex & TEX_MAG_LINEAR) != 0);
 	}
 	for (uint32_t ch = 0; ch < 4; ch++)
 		out[ch] = float((c >> (8 * ch)) & 0xffu);
@@ -457,7 +560,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		const float sum = __fadd_rn(__fadd_rn(ta.col0[ch], e1), e2);
 		col[ch] = floorf(__fadd_rn(__fmul_rn(255.0f, sum), 1.0f / 256.0f));
 	}
-	const uint32_t p_tex = ta.tex, p_alpha = ta.alpha, p_fbmsk = ta.fbmsk, p_fogcol = ta.fogcol;
+	const uint32_t p_tex = ta.tex, p_tex2 = ta.tex2, p_alpha = ta.alpha, p_fbmsk = ta.fbmsk, p_fogcol = ta.fogcol;
 	const TransformedAttributes *tap = static_cast<const TransformedAttributes *>(a.tattr) + prim_index;
 
 	// Combined colour (ubershader.comp:1260-1339).
@@ -482,7 +585,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 		u = fminf(fmaxf(u, -2047.0f), 2047.0f);
 		v = fminf(fmaxf(v, -2047.0f), 2047.0f);
 		float tx[4];
-		sample_texture(td, a.tex_images, p_tex, u, v, persp, tx);
+		sample_texture(td, a.tex_images, p_tex, p_tex2, u, v, stqf(2), persp, tx);
 		const bool tcc = (sv_combiner & COMB_TCC) != 0;
 		const uint32_t mode = sv_combiner & COMB_MODE_MASK;
 		if (mode == 1) // DECAL
@@ -1200,6 +1303,26 @@ __device__ __forceinline__ float wave_sum(float v)
 	return v;
 }
 
+// Test hook (pgs_k_lod_eval): the Q-derived level of detail lod_from_q for K = k and L = l over
+// `count` consecutive float bit patterns from `first`, summed into *hash as the wrapping sum of
+// ((n << 32) | input) * 0x9E3779B97F4A7C15 (n as 32-bit two's complement).
+__global__ void __launch_bounds__(256) k_lod_eval(uint32_t l, int32_t k, uint32_t first, uint64_t count,
+                                                  unsigned long long *hash)
+{
+	uint64_t h = 0;
+	const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+	for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+	{
+		const uint32_t in = first + uint32_t(i);
+		const uint32_t n = uint32_t(lod_from_q(__uint_as_float(in), k, l));
+		h += ((uint64_t(n) << 32) | in) * 0x9E3779B97F4A7C15ull;
+	}
+	for (int m = 32; m >= 1; m >>= 1)
+		h += __shfl_xor(h, m);
+	if ((threadIdx.x & 63u) == 0)
+		atomicAdd(hash, (unsigned long long)h);
+}
+
 // Per-tile features of a shading multi-launch's work for the tile-cost model (one wave per
 // coarse tile): primitive count, covered area and the state mix of the tile's primitives.
 // Tiles without work for this launch (no primitives, or the other shading mode) get all-zero
@@ -1476,5 +1599,17 @@ extern "C" int pgs_k_tile_features(const pgs_features_args *f, hipStream_t strea
 	if (f->total == 0)
 		return 0;
 	hipLaunchKernelGGL(pgs::k_tile_features, dim3((f->total + 3u) / 4u), dim3(256), 0, stream, *f);
+	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+}
+
+extern "C" int pgs_k_lod_eval(uint32_t l, int32_t k, uint32_t first, uint64_t count, uint64_t *hash, hipStream_t stream)
+{
+	if (!hash || l > 3u || k < -2048 || k > 2047 || count > (uint64_t(1) << 32))
+		return -int(hipErrorInvalidValue);
+	if (count == 0)
+		return 0;
+	const uint64_t blocks = std::min<uint64_t>(4096, (count + 255u) / 256u);
+	hipLaunchKernelGGL(pgs::k_lod_eval, dim3(uint32_t(blocks)), dim3(256), 0, stream, l, k, first, count,
+	                   reinterpret_cast<unsigned long long *>(hash));
 	return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
 }

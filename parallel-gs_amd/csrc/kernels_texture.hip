@@ -77,7 +77,7 @@ __device__ __forceinline__ void sampler_feedback_body(const pgs_feedback_args &a
 	if (slot >= a.num_textures)
 		return;
 	const TextureDesc td = static_cast<const TextureDesc *>(a.textures)[slot];
-	if (td.width > 1024u || td.height > 1024u || td.width == 0 || td.height == 0)
+	if (td.width > 1024u || td.height > 1024u || td.width == 0 || td.height == 0 || td.levels > 1u)
 		return; // decoded whole
 	uint32_t *mask = a.masks + slot * PGS_FEEDBACK_WORDS;
 	const int32_t w = int32_t(td.width), h = int32_t(td.height);

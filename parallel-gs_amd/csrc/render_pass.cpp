@@ -373,7 +373,18 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 		const uint32_t wms = field(cl, 0, 2), wmt = field(cl, 2, 2);
 		pa.tex |= (wms == 1 || wms == 2) ? TEX_CLAMP_S : 0;
 		pa.tex |= (wmt == 1 || wmt == 2) ? TEX_CLAMP_T : 0;
-		pa.tex2 = TEX2_FIXED_LOD;
+		// Mipmapped sampling (gs/gs_interface.cpp:1834-1863): LCM, L and K go to tex2, MXL and
+		// the MMIN filters to the texture word; otherwise the LOD is fixed at 0.
+		if (mmin_has_mipmap(t1) && !disable_mipmaps)
+		{
+			pa.tex2 = field(t1, 0, 1) | field(t1, 19, 2) << TEX2_L_SHIFT | field(t1, 32, 12) << TEX2_K_SHIFT;
+			pa.tex |= field(t1, 2, 3) << TEX_MAX_MIP_LEVEL_SHIFT;
+			const uint32_t mmin = field(t1, 6, 3);
+			pa.tex |= (mmin == 4 || mmin == 5) ? TEX_MIN_LINEAR : 0;
+			pa.tex |= (mmin == 3 || mmin == 5) ? TEX_MIP_LINEAR : 0;
+		}
+		else
+			pa.tex2 = TEX2_FIXED_LOD;
 	}
 	if (zte)
 	{
@@ -524,23 +535,37 @@ void RenderPassBuilder::flush()
 
 namespace pgs
 {
+RenderPassBuilder::SlotKey RenderPassBuilder::slot_key(const RegisterState &r, uint32_t ctx) const
+{
+	SlotKey k{};
+	k.tex0 = r.tex0[ctx] & ((1ull << 61) - 1ull); // CLD does not change the texture
+	k.clamp = r.clamp[ctx];
+	k.texa = r.texa;
+	k.levels = texture_levels(k.tex0, r.tex1[ctx], disable_mipmaps);
+	// Levels 1..3 in MIPTBP1, 4..6 in MIPTBP2, 20 bits each.
+	const uint32_t l1 = std::min(k.levels - 1u, 3u), l2 = k.levels > 4u ? k.levels - 4u : 0u;
+	k.mip1 = r.miptbp1[ctx] & ((1ull << (20u * l1)) - 1ull);
+	k.mip2 = r.miptbp2[ctx] & ((1ull << (20u * l2)) - 1ull);
+	return k;
+}
+
 bool RenderPassBuilder::texture_hazard(const RegisterState &r, uint32_t ctx)
 {
-	if (r.tex0[ctx] != span_tex0_ || r.clamp[ctx] != span_clamp_)
+	const SlotKey k = slot_key(r, ctx);
+	if (!(k == span_key_))
 	{
-		span_tex0_ = r.tex0[ctx];
-		span_clamp_ = r.clamp[ctx];
-		span_ = texture_span(tex_layout(span_tex0_, span_clamp_));
+		span_key_ = k;
+		span_ = texture_span(tex_layout(k.tex0, k.clamp, k.levels, k.mip1, k.mip2));
 	}
 	const uint32_t x0 = uint32_t(bb_[0]), y0 = uint32_t(bb_[1]), x1 = uint32_t(bb_[2]), y1 = uint32_t(bb_[3]);
 	const PageSpan fb = page_span(field(frame_, 0, 9) * 32u, field(frame_, 16, 6), field(frame_, 24, 6), x0, y0, x1, y1);
-	if (spans_overlap(span_, fb))
+	if (span_.overlaps(fb))
 		return true;
 	if (pass_zwrite_)
 	{
 		const PageSpan z = page_span(field(zbuf_, 0, 9) * 32u, field(frame_, 16, 6), 0x30u | field(zbuf_, 24, 4), x0,
 		                             y0, x1, y1);
-		if (spans_overlap(span_, z))
+		if (span_.overlaps(z))
 			return true;
 	}
 	return false;
@@ -568,28 +593,30 @@ bool RenderPassBuilder::allocate_image(uint32_t texels, uint32_t &offset)
 
 int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 {
-	const uint64_t key0 = r.tex0[ctx] & ((1ull << 61) - 1ull); // CLD does not change the texture
-	const uint64_t keyc = r.clamp[ctx], keya = r.texa;
+	const SlotKey key = slot_key(r, ctx);
 	for (uint32_t i = 0; i < slot_keys_.size(); i++)
-		if (slot_keys_[i].tex0 == key0 && slot_keys_[i].clamp == keyc && slot_keys_[i].texa == keya)
+		if (slot_keys_[i] == key)
 			return int32_t(i);
+	const uint64_t key0 = key.tex0, keyc = key.clamp, keya = key.texa;
 
-	const TexLayout l = tex_layout(key0, keyc);
-	const uint32_t texels = l.width * l.height;
+	const TexLayout l = tex_layout(key0, keyc, key.levels, key.mip1, key.mip2);
+	const uint32_t texels = l.texels();
 	if (slot_keys_.size() >= kMaxPassTextures || texels > kTexImageTexels)
 		return prims_.empty() ? -2 : -1;
 
 	// A valid cached image of an earlier pass is reused; otherwise the pass decodes one.
 	const bool palette = l.psm == PSMT8 || l.psm == PSMT4 || l.psm == PSMT8H || l.psm == PSMT4HL || l.psm == PSMT4HH;
-	const SlotKey key{ key0, keyc, keya };
 	SlotImage im{};
 	bool hit = false;
 	for (size_t i = 0; i < cache_.size(); i++)
 	{
 		const CachedTexture &c = cache_[i];
-		if (c.key.tex0 != key0 || c.key.clamp != keyc || c.key.texa != keya)
+		if (!(c.key == key))
 			continue;
-		if (!writes_.written_since(c.span, c.decoded_at) && (!c.palette || c.clut_gen == writes_.clut_generation()))
+		bool stale = false;
+		for (uint32_t lv = 0; lv < c.span.n; lv++)
+			stale = stale || writes_.written_since(c.span.s[lv], c.decoded_at);
+		if (!stale && (!c.palette || c.clut_gen == writes_.clut_generation()))
 		{
 			im.offset = c.offset;
 			im.texels = c.texels;
@@ -614,6 +641,7 @@ int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 	td.image_offset = im.offset;
 	td.width = l.width;
 	td.height = l.height;
+	td.levels = l.levels;
 	// Clamp bounds floor(region) + 0.5 (the regions are whole texels, so the sum is exact).
 	const float inf = std::numeric_limits<float>::infinity();
 	td.region[0] = td.region[1] = -inf;
@@ -637,8 +665,12 @@ int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 	// descriptor and sampler-feedback indices equal to the slot.
 	TexDecode dd{};
 	dd.image_offset = im.offset;
-	dd.width = im.decoded ? l.width : 0u;
-	dd.height = im.decoded ? l.height : 0u;
+	dd.width = l.width;
+	dd.height = l.height;
+	dd.levels = l.levels;
+	dd.texels = im.decoded ? texels : 0u;
+	for (uint32_t lv = 1; lv < l.levels; lv++)
+		dd.level_buf[lv - 1u] = l.level_tbp[lv] | l.level_tbw[lv] << 14;
 	dd.tbp = l.tbp;
 	dd.tbw = l.tbw;
 	dd.psm = l.psm;

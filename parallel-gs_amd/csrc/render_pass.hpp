@@ -135,6 +135,9 @@ public:
 	// primitives, counted down by GSInterface::vsync).
 	uint32_t complex_scanmsk_timeout = 0;
 
+	// Hacks::disable_mipmaps: textures decode level 0 only and sample it at LOD 0.
+	bool disable_mipmaps = false;
+
 private:
 	struct QueuedVertex
 	{
@@ -159,7 +162,19 @@ private:
 	bool pass_z_ = false;      // some primitive tests or writes depth
 	bool pass_zwrite_ = false; // some primitive writes depth
 
-	struct SlotKey { uint64_t tex0, clamp, texa; };
+	// The image depends on TEX0 (without CLD), CLAMP, TEXA, the level count and the level
+	// addresses in MIPTBP1 / MIPTBP2 (unused levels' fields cleared).
+	struct SlotKey
+	{
+		uint64_t tex0, clamp, texa, mip1, mip2;
+		uint32_t levels;
+		bool operator==(const SlotKey &o) const
+		{
+			return tex0 == o.tex0 && clamp == o.clamp && texa == o.texa && mip1 == o.mip1 && mip2 == o.mip2 &&
+			       levels == o.levels;
+		}
+	};
+	SlotKey slot_key(const RegisterState &r, uint32_t ctx) const;
 	std::vector<SlotKey> slot_keys_;
 	// Texture cache across passes (gs/gs_renderer.cpp:1317-1386 create_cached_texture, with
 	// page_tracker.cpp:842-933 invalidation): fully decoded images of earlier passes, kept in
@@ -170,7 +185,7 @@ private:
 		uint32_t offset, texels;
 		uint64_t decoded_at, clut_gen;
 		bool palette;
-		PageSpan span;
+		TexSpans span;
 	};
 	struct SlotImage
 	{
@@ -186,8 +201,8 @@ private:
 	std::vector<TexDecode> decodes_;
 	std::vector<TextureDesc> tex_descs_;
 	uint32_t texel_total_ = 0, group_total_ = 0;
-	uint64_t span_tex0_ = ~0ull, span_clamp_ = ~0ull; // last texture whose page span is cached
-	PageSpan span_{};
+	SlotKey span_key_{ ~0ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0u }; // last texture whose page spans are cached
+	TexSpans span_{};
 	bool state_dirty_ = true;
 	bool last_candidate_ = false;
 	int32_t last_order_[3] = { 0, 1, 2 };

@@ -90,7 +90,10 @@ __device__ __forceinline__ uint32_t clut_color(const uint16_t *clut, const TexDe
 	return uint32_t(clut[bi & 511u]) | (uint32_t(clut[(bi + 256u) & 511u]) << 16);
 }
 
-// upload.comp:108-228 (single-sampled source).
+// upload.comp:108-228 (single-sampled source). A mipmapped texture's levels follow each other
+// in the image, level l being max(width >> l, 1) x max(height >> l, 1) texels read from its own
+// TBP / TBW with the rectangle origin, MSK and FIX shifted right by l
+// (gs/gs_renderer.cpp:3543-3593).
 __device__ __forceinline__ void texture_decode_body(const pgs_tex_decode_args &a, uint32_t wg, uint32_t)
 {
 	const TexDecode *decodes = static_cast<const TexDecode *>(a.decodes);
@@ -98,18 +101,34 @@ __device__ __forceinline__ void texture_decode_body(const pgs_tex_decode_args &a
 	while (s + 1u < a.num_textures && decodes[s + 1u].group_begin <= wg)
 		s++;
 	const TexDecode d = decodes[s];
-	const uint32_t t = (wg - d.group_begin) * 256u + threadIdx.x;
-	if (t >= d.width * d.height)
+	const uint32_t t0 = (wg - d.group_begin) * 256u + threadIdx.x;
+	if (t0 >= d.texels)
 		return;
-	const uint32_t x = t % d.width, y = t / d.width;
-	if (a.feedback && d.width <= 1024u && d.height <= 1024u)
+	uint32_t t = t0, level = 0, w = d.width, h = d.height, tbp = d.tbp, tbw = d.tbw;
+	while (t >= w * h && level + 1u < d.levels)
+	{
+		t -= w * h;
+		level++;
+		w = max(d.width >> level, 1u);
+		h = max(d.height >> level, 1u);
+		// Read from the record in memory: a variable index into the local copy would place it
+		// in scratch.
+		const uint32_t buf = decodes[s].level_buf[level - 1u];
+		tbp = buf & 0x3fffu;
+		tbw = buf >> 14;
+	}
+	if (t >= w * h)
+		return;
+	const uint32_t x = t % w, y = t / w;
+	if (a.feedback && d.levels == 1u && w <= 1024u && h <= 1024u)
 	{
 		const uint32_t word = a.feedback[s * PGS_FEEDBACK_WORDS + (y >> 3) * 4u + (x >> 8)];
 		if (!((word >> ((x >> 3) & 31u)) & 1u))
 			return; // no primitive of the pass can sample this 8x8 block
 	}
-	const uint32_t tx = ((x & d.umsk) + d.bx) | d.ufix, ty = ((y & d.vmsk) + d.by) | d.vfix;
-	const uint32_t addr = swizzle_addr(tx, ty, d.tbp, d.tbw, d.psm);
+	const uint32_t tx = ((x & (d.umsk >> level)) + (d.bx >> level)) | (d.ufix >> level);
+	const uint32_t ty = ((y & (d.vmsk >> level)) + (d.by >> level)) | (d.vfix >> level);
+	const uint32_t addr = swizzle_addr(tx, ty, tbp, tbw, d.psm);
 	const uint8_t *v = a.vram;
 	uint32_t c;
 	switch (d.psm)
@@ -127,7 +146,7 @@ __device__ __forceinline__ void texture_decode_body(const pgs_tex_decode_args &a
 	case PSMT4HH: c = clut_color(a.clut, d, *reinterpret_cast<const uint32_t *>(v + addr) >> 28); break;
 	default: c = *reinterpret_cast<const uint32_t *>(v + addr); break;
 	}
-	a.images[d.image_offset + t] = c;
+	a.images[d.image_offset + t0] = c;
 }
 
 }

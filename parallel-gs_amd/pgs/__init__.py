@@ -148,7 +148,7 @@ EXPORTED_SYMBOLS = [
     "pgs_cost_model_predict", "pgs_cost_model_gradients", "pgs_cost_model_train",
     "pgs_k_mlp_forward", "pgs_k_mlp_gradients", "pgs_k_mlp_gradients_noalloc", "pgs_k_mlp_adam", "pgs_k_mlp_quantize",
     "pgs_batch_set_schedule", "pgs_batch_collect_costs", "pgs_batch_schedule_stats", "pgs_k_shade_queue", "pgs_k_tile_features",
-    "pgs_k_queue_order", "pgs_k_fill_u32", "pgs_k_shade_ordered", "pgs_k_sampler_feedback", "pgs_k_restore", "pgs_k_fused", "pgs_k_rcp_eval",
+    "pgs_k_queue_order", "pgs_k_fill_u32", "pgs_k_shade_ordered", "pgs_k_sampler_feedback", "pgs_k_restore", "pgs_k_fused", "pgs_k_rcp_eval", "pgs_k_lod_eval",
     "pgs_k_scanout_circuit", "pgs_k_scanout_merge", "pgs_k_scanout_extwrite", "pgs_k_scanout_weave",
 ]
 

@@ -127,8 +127,18 @@ def clamp_reg(wms=0, wmt=0, minu=0, maxu=0, minv=0, maxv=0):
     return wms | wmt << 2 | minu << 4 | maxu << 14 | minv << 24 | maxv << 34
 
 
-def tex1(mmag=0, mmin=0):
-    return mmag << 5 | mmin << 6
+def tex1(mmag=0, mmin=0, lcm=0, mxl=0, l=0, k=0):
+    """TEX1: LCM bit 0, MXL bits 2..4, MMAG bit 5, MMIN bits 6..8, L bits 19..20, K (signed
+    7.4 fixed point, 12 bits) bits 32..43."""
+    return lcm | mxl << 2 | mmag << 5 | mmin << 6 | l << 19 | (k & 0xFFF) << 32
+
+
+def miptbp(levels):
+    """MIPTBP1 / MIPTBP2: three (TBP, TBW) pairs of 14 + 6 bits."""
+    v = 0
+    for i, (tbp, tbw) in enumerate(levels):
+        v |= (tbp | tbw << 14) << (20 * i)
+    return v
 
 
 def texa(ta0=0, aem=0, ta1=0x80):
@@ -163,10 +173,12 @@ def _random_payload(rng, nbytes):
 
 
 def random_textured_stream(seed: int, n_batches: int = 10, prims_per_batch: int = 30, width: int = 256,
-                           height: int = 192, feedback: bool = True) -> np.ndarray:
+                           height: int = 192, feedback: bool = True, mipmap: bool = False) -> np.ndarray:
     """Textured drawing: random texture formats / sizes / wrap modes / filters / combiners,
     CLUT loads (CSM1, CSM2, CSA, CLD 0..5), UV and ST/Q coordinates, and (optionally) textures
-    read from the frame buffer being drawn to (render-pass hazard flushes)."""
+    read from the frame buffer being drawn to (render-pass hazard flushes). With `mipmap`, TEX1
+    takes random mipmapped filters (MMIN 2..5 mostly), MXL, LCM, L and K, MIPTBP1 / MIPTBP2 point
+    the levels at random texture pages, and Q spreads over 2^-3..2^3 so the LOD varies."""
     rng = np.random.default_rng(seed)
     ofs = 2048 * 16
     out = [gif.ad([(0x1A, 1), (0x18, ofs | ofs << 32)])]
@@ -210,12 +222,22 @@ def random_textured_stream(seed: int, n_batches: int = 10, prims_per_batch: int 
             (0x42, alpha_reg(0, 1, 0, 1, 0)),
             (0x3B, texa(int(rng.integers(0, 256)), int(rng.integers(0, 2)), int(rng.integers(0, 256)))),
             (0x1C, texclut(int(rng.integers(1, 4)), int(rng.integers(0, 4)), int(rng.integers(0, 8)))),
-            (0x14, tex1(int(rng.integers(0, 2)), int(rng.integers(0, 2)))),
+            (0x14, tex1(int(rng.integers(0, 2)), int(rng.integers(0, 2))) if not mipmap else
+             tex1(int(rng.integers(0, 2)), int(rng.choice([2, 3, 4, 5, 2, 3, 4, 5, 0, 1, 6])),
+                  lcm=int(rng.integers(0, 2)), mxl=int(rng.integers(0, 8)), l=int(rng.integers(0, 4)),
+                  k=int(rng.integers(-96, 160)))),
             (0x08, cl),
             (0x06, tex0(tbp, tbw, psm, tw, th, tcc=int(rng.integers(0, 2)), tfx=int(rng.integers(0, 4)), cbp=cbp,
                         cpsm=cpsm, csm=csm, csa=csa, cld=cld)),
             (0x3D, int(rng.integers(0, 2**24))),
         ]
+        if mipmap:
+            # Now and then a level is read from the frame buffer being drawn (a hazard on a level
+            # other than 0).
+            lv = lambda: ((0, fbw) if rng.integers(0, 6) == 0 else
+                          (int(rng.choice(tex_pages)) * 32 + int(rng.integers(0, 8)) * 4, int(rng.integers(1, 5))))
+            regs.insert(0, (0x34, miptbp([lv(), lv(), lv()])))
+            regs.insert(0, (0x36, miptbp([lv(), lv(), lv()])))
         ptype = int(rng.choice([3, 3, 4, 5, 6, 6, 1, 2]))
         fst = int(rng.integers(0, 3) != 0)
         prim = prim_reg(ptype, iip=int(rng.integers(0, 2)), tme=1, abe=int(rng.integers(0, 2)),
@@ -227,11 +249,15 @@ def random_textured_stream(seed: int, n_batches: int = 10, prims_per_batch: int 
         tsz_u, tsz_v = (1 << tw) * 16, (1 << th) * 16
         for p in range(prims_per_batch):
             q_const = float(rng.choice([1.0, 0.5, 2.0]))
+            if mipmap:
+                q_const = float(2.0 ** rng.uniform(-3, 3)) if rng.integers(0, 2) else float(2.0 ** rng.integers(-3, 4))
             for v in range(nv):
                 x = ofs + int(rng.integers(-16 * 16, (width + 16) * 16))
                 y = ofs + int(rng.integers(-16 * 16, (height + 16) * 16))
                 z = int(rng.integers(0, 2**24))
                 q = q_const if rng.integers(0, 2) else float(rng.uniform(0.5, 2.0))
+                if mipmap and rng.integers(0, 2):
+                    q = q_const * float(2.0 ** rng.uniform(-1, 1))
                 writes.append((0x01, rgbaq(int(rng.integers(0, 2**32)), q)))
                 if fst:
                     writes.append((0x03, uv(int(rng.integers(0, 2 * tsz_u)), int(rng.integers(0, 2 * tsz_v)))))
