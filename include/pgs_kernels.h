@@ -16,21 +16,23 @@
 extern "C" {
 #endif
 
-/* Host->local and local->local VRAM copies of one transfer batch.
+/* Host->local and local->local VRAM copies of one dependency level of a transfer batch.
  * Replaces the PREPARE + EXECUTE dispatches of vram_copy.comp issued by
  * GSRenderer::flush_transfer / emit_copy_vram (gs/gs_renderer.cpp:3866-4047, 2721-2827).
- * `blocks` lists every destination 256-byte block of the batch with the ordered slice of
- * `records` (copy index + block origin) that writes it; `descs` holds the copies. */
+ * `items` lists every destination 256-byte block of the level (CopyItem: the first record
+ * inline, further records in `records`, copy index + block origin) and the blocks the next
+ * level's snapshot copies from VRAM; `descs` holds the copies. Each wave takes two items. */
 typedef struct pgs_vram_copy_args
 {
 	uint8_t *vram;             /* plane 0, 4 MiB */
 	uint8_t *ref_plane;        /* super-sample reference plane, or NULL at 1x */
-	const uint8_t *shadow;     /* batch-start snapshot for hazardous local copies */
+	const uint8_t *shadow;     /* level-start snapshot for hazardous local copies */
+	uint8_t *shadow_next;      /* the next level's snapshot, or NULL */
 	const uint8_t *payload;    /* host->local payload arena */
 	const void *descs;         /* CopyDesc[] */
 	const void *records;       /* BlockRecord[] */
-	const void *blocks;        /* DestBlock[] */
-	uint32_t num_blocks;
+	const void *items;         /* CopyItem[num_items] */
+	uint32_t num_items;
 	uint32_t pad;
 } pgs_vram_copy_args;
 

@@ -255,12 +255,23 @@ struct BlockRecord
 	uint16_t x0, y0;
 };
 
-// A destination block and the ordered slice of records that write to it.
-struct DestBlock
+// One wave item of a transfer level: a destination block with its first record inline and the
+// ordered slice of further records that write it, or a block the next level's snapshot copies.
+struct CopyItem
 {
 	uint32_t block;
-	uint32_t rec_begin;
+	uint32_t flags;     // COPY_ITEM_*
+	uint32_t copy;      // first record: copy index
+	uint32_t xy;        // first record: block origin x | y << 16
+	uint32_t rec_begin; // further records
 	uint32_t rec_count;
-	uint32_t pad;
+	uint32_t pad[2];
+};
+static_assert(sizeof(CopyItem) == 32, "CopyItem layout");
+
+enum : uint32_t
+{
+	COPY_ITEM_TO_NEXT_SNAPSHOT = 1u << 0, // the next level snapshots this block: store the result there too
+	COPY_ITEM_SNAPSHOT = 1u << 1,         // copy the block from VRAM into the next snapshot only
 };
 }

@@ -2,11 +2,13 @@
 //
 // Replaces gs/shaders/vram_copy.comp. The reference resolves write-after-write races between
 // copies of one batch with an atomic linked list per dword and a second "execute" pass
-// (vram_copy.comp:115-197). Here the host groups the batch by destination block instead:
-// one wave64 owns one 256-byte VRAM block, lane L owns dword L of it, and the wave applies
-// every copy that touches the block in submission order while the dword stays in a VGPR.
-// The block is read once and written once, with fully coalesced 256-byte accesses, and no
-// atomics are needed. Per-pixel read/merge rules follow vram_copy.comp:199-613.
+// (vram_copy.comp:115-197). Here the host groups each dependency level of the batch by
+// destination block instead: a wave64 owns one 256-byte VRAM block, lane L owns dword L of it,
+// and the wave applies every copy that touches the block in submission order while the dword
+// stays in a VGPR. A block is read once and written once, with fully coalesced 256-byte
+// accesses, and no atomics are needed. Per-pixel read/merge rules follow vram_copy.comp:199-613.
+// A level is latency-bound (a chain of dependent loads per wave): the first record of a block is
+// inline in its work item, and a dword's pixels are gathered with all loads in flight at once.
 #include "gs_defs.hpp"
 #include "pgs_kernels.h"
 #include "multi_launch.hpp"
@@ -50,7 +52,7 @@ PGS_DEFINE_MULTI(256, k_snapshot_blocks_multi, snapshot_blocks_body, pgs_vram_sn
 }
 }
 
-uint32_t pgs_grid_vram_copy(const pgs_vram_copy_args &a) { return (a.num_blocks + 3u) / 4u; }
+uint32_t pgs_grid_vram_copy(const pgs_vram_copy_args &a) { return (a.num_items + 3u) / 4u; }
 uint32_t pgs_grid_vram_snapshot(const pgs_vram_snapshot_args &a) { return (a.num_blocks + 3u) / 4u; }
 
 int pgs_multi_vram_copy(const pgs_vram_copy_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
@@ -69,12 +71,11 @@ int pgs_multi_vram_snapshot(const pgs_vram_snapshot_args *args, const uint32_t *
 
 extern "C" int pgs_k_vram_copy(const pgs_vram_copy_args *args, hipStream_t stream)
 {
-	if (!args || !args->vram || !args->blocks || !args->records || !args->descs)
+	if (!args || !args->vram || (args->num_items && (!args->items || !args->descs)))
 		return -int(hipErrorInvalidValue);
-	if (args->num_blocks == 0)
+	if (args->num_items == 0)
 		return 0;
-	const uint32_t waves_per_group = 4;
-	const uint32_t groups = (args->num_blocks + waves_per_group - 1) / waves_per_group;
+	const uint32_t groups = pgs_grid_vram_copy(*args); // four waves of one item each
 	hipLaunchKernelGGL(pgs::k_resolve_blocks, dim3(groups), dim3(256), 0, stream, *args);
 	hipError_t err = hipGetLastError();
 	return err == hipSuccess ? 0 : -int(err);

@@ -23,12 +23,14 @@ inline size_t rect_block_bound(uint32_t psm, uint32_t w, uint32_t h)
 }
 
 // Arena bytes a copy's work lists can need at batch close: its descriptor plus, per touched
-// block, one record, one destination-block entry and one snapshot-list entry.
+// block, one record and one item, and per source block one snapshot-list entry and one
+// snapshot item.
 inline size_t copy_meta_bound(const CopyDesc &d)
 {
 	size_t blocks = rect_block_bound(d.dpsm, d.w, d.h);
 	size_t src_blocks = (d.flags & COPY_LOCAL) ? rect_block_bound(d.spsm, d.w, d.h) : 0;
-	return sizeof(CopyDesc) + blocks * (sizeof(BlockRecord) + sizeof(DestBlock)) + src_blocks * 4 + 4 * kAlign;
+	return sizeof(CopyDesc) + blocks * (sizeof(BlockRecord) + sizeof(CopyItem)) + src_blocks * (4 + sizeof(CopyItem)) +
+	       6 * kAlign;
 }
 }
 
@@ -96,12 +98,11 @@ bool Renderer::init(uint32_t sample_log2, size_t arena_bytes, uint32_t sampler_f
 		}
 		check(hipMalloc(&vram_, size_t(capacity_planes) * kVramSize));
 		check(hipMemsetAsync(vram_, 0, size_t(capacity_planes) * kVramSize, stream_));
-		check(hipMalloc(&shadow_, kVramSize));
+		check(hipMalloc(&shadow_, 2u * size_t(kVramSize))); // two snapshot planes, alternating by level
 		arena_.cap = std::max<size_t>(arena_bytes, 16u << 20);
 		arena_bytes_ = arena_.cap;
 		check(hipHostMalloc(&arena_.host, arena_.cap, hipHostMallocDefault));
 		check(hipMalloc(&arena_.dev, arena_.cap));
-		batch_written_.assign(kNumBlocks / 64, 0);
 		level_written_.assign(kNumBlocks, -1);
 		level_read_.assign(kNumBlocks, -1);
 		check(hipMalloc(&setup_buf_, 65536u * sizeof(PrimitiveSetup)));
@@ -452,56 +453,67 @@ void Renderer::flush_transfers()
 	std::vector<std::vector<uint32_t>> members(levels);
 	for (uint32_t j = 0; j < uint32_t(batch_.size()); j++)
 		members[level[j]].push_back(j);
-	for (const auto &m : members)
-		emit_transfer_level(m);
+	std::vector<TransferLevel> plan(levels);
+	for (uint32_t i = 0; i < levels; i++)
+		plan_transfer_level(members[i], plan[i]);
+	for (uint32_t i = 0; i < levels; i++)
+		emit_transfer_level(plan[i], i, i + 1u < levels ? &plan[i + 1u] : nullptr);
 	batch_.clear();
 	batch_meta_ = 0;
 }
 
-// One level of queued copies: one snapshot launch (if a local copy reads a block the level
-// writes) and one copy launch.
-void Renderer::emit_transfer_level(const std::vector<uint32_t> &members)
+// The blocks a level writes, and its local copies whose source the level also overwrites:
+// those read the level-start snapshot, whose blocks are listed.
+void Renderer::plan_transfer_level(const std::vector<uint32_t> &members, TransferLevel &lv)
 {
-	const uint32_t ncopies = uint32_t(members.size());
-	std::fill(batch_written_.begin(), batch_written_.end(), 0);
-	for (uint32_t j : members)
+	lv.written.assign(kNumBlocks / 64, 0);
+	lv.descs.resize(members.size());
+	for (size_t ci = 0; ci < members.size(); ci++)
 	{
-		const CopyDesc &d = batch_[j].desc;
+		lv.descs[ci] = batch_[members[ci]].desc;
+		const CopyDesc &d = lv.descs[ci];
 		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h,
-		                    [&](uint32_t b, uint32_t, uint32_t) { batch_written_[b >> 6] |= 1ull << (b & 63); });
+		                    [&](uint32_t b, uint32_t, uint32_t) { lv.written[b >> 6] |= 1ull << (b & 63); });
 	}
-
-	// 1. Local copies whose source the level overwrites read from the level-start snapshot.
-	std::vector<CopyDesc> level_descs(ncopies);
-	std::vector<uint32_t> snapshot;
 	std::vector<uint64_t> snap_mask(kNumBlocks / 64, 0);
-	for (uint32_t ci = 0; ci < ncopies; ci++)
+	for (CopyDesc &d : lv.descs)
 	{
-		CopyDesc &d = level_descs[ci];
-		d = batch_[members[ci]].desc;
 		if (!(d.flags & COPY_LOCAL))
 			continue;
 		bool hazard = false;
 		for_each_rect_block(d.spsm, d.sbp, d.sbw, d.ssax, d.ssay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
-			if (batch_written_[b >> 6] & (1ull << (b & 63)))
+			if (lv.written[b >> 6] & (1ull << (b & 63)))
 				hazard = true;
 		});
 		if (!hazard)
 			continue;
 		d.flags |= COPY_SOURCE_SHADOW;
 		for_each_rect_block(d.spsm, d.sbp, d.sbw, d.ssax, d.ssay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
-			uint64_t bit = 1ull << (b & 63);
+			const uint64_t bit = 1ull << (b & 63);
 			if (!(snap_mask[b >> 6] & bit))
 			{
 				snap_mask[b >> 6] |= bit;
-				snapshot.push_back(b);
+				lv.snapshot.push_back(b);
 			}
 		});
 	}
+}
 
-	// 2. Count records per destination block.
+// One level of queued copies as one launch. Level i reads its snapshot from shadow plane i & 1;
+// the launch of level i - 1 fills it: blocks that level writes store their result there too,
+// and the others are copied by extra waves of the same launch (the level does not write them,
+// so their value at its end is their value while it runs). Only a first level with a snapshot
+// needs a snapshot launch of its own.
+void Renderer::emit_transfer_level(const TransferLevel &lv, uint32_t index, const TransferLevel *next)
+{
+	const uint32_t ncopies = uint32_t(lv.descs.size());
+	uint8_t *shadow = shadow_ + size_t(index & 1u) * kVramSize;
+	uint8_t *shadow_next = shadow_ + size_t((index + 1u) & 1u) * kVramSize;
+	const bool has_next = next && !next->snapshot.empty();
+
+	// Count records per destination block.
 	size_t nrec = 0;
-	for (const CopyDesc &d : level_descs)
+	for (const CopyDesc &d : lv.descs)
 		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h, [&](uint32_t b, uint32_t, uint32_t) {
 			if (block_count_[b]++ == 0)
 				touched_blocks_.push_back(b);
@@ -509,45 +521,78 @@ void Renderer::emit_transfer_level(const std::vector<uint32_t> &members)
 		});
 	std::sort(touched_blocks_.begin(), touched_blocks_.end());
 
-	size_t off_desc, off_rec, off_blk, off_snap = 0;
-	auto *descs = reinterpret_cast<CopyDesc *>(arena_alloc(ncopies * sizeof(CopyDesc), off_desc));
-	auto *recs = reinterpret_cast<BlockRecord *>(arena_alloc(nrec * sizeof(BlockRecord), off_rec));
-	auto *blks = reinterpret_cast<DestBlock *>(arena_alloc(touched_blocks_.size() * sizeof(DestBlock), off_blk));
-	uint32_t *snap = nullptr;
-	if (!snapshot.empty())
-		snap = reinterpret_cast<uint32_t *>(arena_alloc(snapshot.size() * 4, off_snap));
-
-	// 3. Prefix sum, then scatter records stably (copy order, then raster order).
-	uint32_t run = 0;
-	for (size_t i = 0; i < touched_blocks_.size(); i++)
+	// The next level's snapshot: blocks this level writes keep a flag, the rest are copied.
+	std::vector<uint32_t> snap_copy;
+	std::vector<uint64_t> next_mask;
+	if (has_next)
 	{
-		uint32_t b = touched_blocks_[i];
-		blks[i] = { b, run, block_count_[b], 0 };
-		uint32_t c = block_count_[b];
-		block_count_[b] = run; // reuse as write cursor
-		run += c;
+		next_mask.assign(kNumBlocks / 64, 0);
+		for (uint32_t b : next->snapshot)
+		{
+			next_mask[b >> 6] |= 1ull << (b & 63);
+			if (!(lv.written[b >> 6] & (1ull << (b & 63))))
+				snap_copy.push_back(b);
+		}
+	}
+
+	// Items: one per destination block with its first record inline, then the snapshot copies.
+	// Further records follow in `recs`, stably ordered (copy order, then raster order).
+	const size_t nblocks = touched_blocks_.size(), nitems = nblocks + snap_copy.size();
+	const size_t nextra = nrec - nblocks;
+	size_t off_desc, off_rec, off_item, off_first = 0;
+	auto *descs = reinterpret_cast<CopyDesc *>(arena_alloc(ncopies * sizeof(CopyDesc), off_desc));
+	auto *recs = reinterpret_cast<BlockRecord *>(arena_alloc(nextra * sizeof(BlockRecord), off_rec));
+	auto *items = reinterpret_cast<CopyItem *>(arena_alloc(nitems * sizeof(CopyItem), off_item));
+	uint32_t *first_snap = nullptr;
+	if (index == 0 && !lv.snapshot.empty())
+		first_snap = reinterpret_cast<uint32_t *>(arena_alloc(lv.snapshot.size() * 4, off_first));
+
+	// block_count_ becomes the index of the block's item (cursor = records written so far in
+	// item_fill_, kept in the item's rec_count until the scatter ends).
+	uint32_t run = 0;
+	for (size_t i = 0; i < nblocks; i++)
+	{
+		const uint32_t b = touched_blocks_[i];
+		const bool to_next = has_next && (next_mask[b >> 6] & (1ull << (b & 63)));
+		const uint32_t extra = block_count_[b] - 1u;
+		items[i] = { b, to_next ? COPY_ITEM_TO_NEXT_SNAPSHOT : 0u, 0u, 0u, run, 0u, { 0u, 0u } };
+		block_count_[b] = uint32_t(i);
+		run += extra;
 	}
 	for (uint32_t ci = 0; ci < ncopies; ci++)
 	{
-		const CopyDesc &d = level_descs[ci];
+		const CopyDesc &d = lv.descs[ci];
 		descs[ci] = d;
 		for_each_rect_block(d.dpsm, d.dbp, d.dbw, d.dsax, d.dsay, d.w, d.h, [&](uint32_t b, uint32_t x0, uint32_t y0) {
-			recs[block_count_[b]++] = { ci, uint16_t(x0), uint16_t(y0) };
+			CopyItem &it = items[block_count_[b]];
+			if (it.pad[0] == 0u)
+			{
+				it.copy = ci;
+				it.xy = x0 | (y0 << 16);
+				it.pad[0] = 1u;
+			}
+			else
+				recs[it.rec_begin + it.rec_count++] = { ci, uint16_t(x0), uint16_t(y0) };
 		});
 	}
-	for (uint32_t b : touched_blocks_)
-		block_count_[b] = 0;
-	if (snap)
-		memcpy(snap, snapshot.data(), snapshot.size() * 4);
+	for (size_t i = 0; i < nblocks; i++)
+	{
+		items[i].pad[0] = 0u;
+		block_count_[touched_blocks_[i]] = 0;
+	}
+	for (size_t i = 0; i < snap_copy.size(); i++)
+		items[nblocks + i] = { snap_copy[i], COPY_ITEM_SNAPSHOT, 0u, 0u, 0u, 0u, { 0u, 0u } };
+	if (first_snap)
+		memcpy(first_snap, lv.snapshot.data(), lv.snapshot.size() * 4);
 
 	arena_upload();
 
-	if (snap)
+	if (first_snap)
 	{
 		Launch l{};
 		l.kind = LaunchKind::VramSnapshot;
-		l.snapshot = { vram_, shadow_, reinterpret_cast<const uint32_t *>(arena_.dev + off_snap),
-		               uint32_t(snapshot.size()), 0 };
+		l.snapshot = { vram_, shadow, reinterpret_cast<const uint32_t *>(arena_.dev + off_first),
+		               uint32_t(lv.snapshot.size()), 0 };
 		launch(l);
 	}
 
@@ -555,20 +600,21 @@ void Renderer::emit_transfer_level(const std::vector<uint32_t> &members)
 	l.kind = LaunchKind::VramCopy;
 	l.copy.vram = vram_;
 	l.copy.ref_plane = sample_log2_ ? vram_ + kVramSize : nullptr;
-	l.copy.shadow = shadow_;
+	l.copy.shadow = shadow;
+	l.copy.shadow_next = has_next ? shadow_next : nullptr;
 	l.copy.payload = arena_.dev;
 	l.copy.descs = arena_.dev + off_desc;
 	l.copy.records = arena_.dev + off_rec;
-	l.copy.blocks = arena_.dev + off_blk;
-	l.copy.num_blocks = uint32_t(touched_blocks_.size());
+	l.copy.items = arena_.dev + off_item;
+	l.copy.num_items = uint32_t(nitems);
 	launch(l);
 
 	stats_.transfer_batches++;
 	stats_.copies += ncopies;
 	stats_.copy_blocks += touched_blocks_.size();
 	stats_.copy_records += nrec;
-	stats_.snapshot_blocks += snapshot.size();
-	for (const CopyDesc &d : level_descs)
+	stats_.snapshot_blocks += lv.snapshot.size();
+	for (const CopyDesc &d : lv.descs)
 	{
 		uint64_t pixels = (d.flags & COPY_LOCAL) ? uint64_t(d.w) * d.h : uint64_t(d.end_pixel - d.first_pixel);
 		stats_.copy_bytes += 2u * pixels * psm_bpp(d.dpsm) / 8u;
@@ -1013,7 +1059,7 @@ void launch_resources(const Renderer::Launch &l, uint32_t &reads, uint32_t &writ
 	reads = writes = 0;
 	switch (l.kind)
 	{
-	case K::VramCopy: reads = R_VRAM | R_SHADOW; writes = R_VRAM; break;
+	case K::VramCopy: reads = R_VRAM | R_SHADOW; writes = R_VRAM | (l.copy.shadow_next ? R_SHADOW : 0u); break;
 	case K::VramSnapshot: reads = R_VRAM; writes = R_SHADOW; break;
 	case K::ClutWrite: reads = R_VRAM; writes = R_CLUT; break;
 	case K::TexDecode:
@@ -1040,7 +1086,9 @@ bool fused_part(const Renderer::Launch &l, pgs_fused_part &p)
 	p = {};
 	switch (l.kind)
 	{
-	case K::VramCopy: p.kind = PGS_FUSED_VRAM_COPY; p.args.copy = l.copy; p.groups = (l.copy.num_blocks + 3u) / 4u; break;
+	case K::VramCopy:
+		p.kind = PGS_FUSED_VRAM_COPY; p.args.copy = l.copy; p.groups = (l.copy.num_items + 3u) / 4u;
+		break;
 	case K::VramSnapshot:
 		p.kind = PGS_FUSED_VRAM_SNAPSHOT; p.args.snapshot = l.snapshot; p.groups = (l.snapshot.num_blocks + 3u) / 4u;
 		break;

@@ -220,7 +220,16 @@ private:
 	{
 		CopyDesc desc;
 	};
-	void emit_transfer_level(const std::vector<uint32_t> &members);
+	// One dependency level of a transfer flush: its copies (hazardous local copies flagged to
+	// read the level-start snapshot), the blocks it writes and the blocks its snapshot holds.
+	struct TransferLevel
+	{
+		std::vector<CopyDesc> descs;
+		std::vector<uint64_t> written;   // bitset over blocks
+		std::vector<uint32_t> snapshot;  // blocks the level's hazardous local copies read
+	};
+	void plan_transfer_level(const std::vector<uint32_t> &members, TransferLevel &lv);
+	void emit_transfer_level(const TransferLevel &lv, uint32_t index, const TransferLevel *next);
 
 	uint32_t sample_log2_ = 0;
 	bool sparse_ = false;
@@ -259,7 +268,6 @@ private:
 
 	// Current transfer batch.
 	std::vector<PendingCopy> batch_;
-	std::vector<uint64_t> batch_written_;   // bitset over 16384 blocks: written by the level being emitted
 	std::vector<int32_t> level_written_;    // per block: level of the last queued copy writing it (-1: none)
 	std::vector<int32_t> level_read_;       // per block: highest level of a queued copy reading it
 	size_t batch_meta_ = 0;                 // arena bytes the queued copies' metadata needs at flush
