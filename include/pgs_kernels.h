@@ -196,6 +196,7 @@ typedef struct pgs_shade_args
 	const uint32_t *counts;
 	const void *textures;    /* TextureDesc[] */
 	const uint32_t *tex_images;
+	const uint16_t *clut;    /* the CLUT (512 entries), read by on-tile palette feedback */
 	uint32_t list_stride;
 	int32_t origin_x, origin_y;
 	uint32_t tile_log2;

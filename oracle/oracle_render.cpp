@@ -392,6 +392,190 @@ bool triangles_form_parallelogram(const Vertex *v, const int32_t o[3], const Ver
 
 // gs/gs_interface.cpp:2473-2836 (single frame-buffer instance per render pass; a frame or
 // depth buffer change ends the pass).
+// On-tile feedback texel (ubershader.comp:955-1040): the pixel's (or sample's) current colour or
+// depth read as the texture's format. Palette formats take the index from the word's top bits
+// (8H: 8 bits, 4HL / 4HH: the nibbles at 24 and 28) plus CSA * 16 and look the CLUT up (8-bit
+// indices past the last bank clamp to it; a 32-bit CLUT has 16 banks and two halves 256 entries
+// apart); 24-bit formats get TA0 as alpha, 16-bit ones TA0 or TA1 by the alpha bit, AEM zeroing
+// alpha for black.
+void GS::feedback_texture(const Prim &p, uint32_t in, float out[4]) const
+{
+	const uint32_t psm = p.fb_tex_psm;
+	uint32_t cpsm = p.fb_tex_cpsm == CT24 ? CT32 : p.fb_tex_cpsm;
+	const uint32_t aem = field(p.fb_texa, 15, 1), ta0 = field(p.fb_texa, 0, 8), ta1 = field(p.fb_texa, 32, 8);
+	uint32_t c;
+	if (psm == T8 || psm == T4 || psm == T8H || psm == T4HL || psm == T4HH)
+	{
+		uint32_t csa = p.fb_csa;
+		if (cpsm == CT32)
+			csa &= 15u;
+		uint32_t idx = 0;
+		if (psm == T8H) idx = in >> 24;
+		else if (psm == T4HL) idx = (in >> 24) & 0xfu;
+		else if (psm == T4HH) idx = in >> 28;
+		idx += csa * 16u;
+		if (psm == T8 || psm == T8H)
+		{
+			uint32_t bank = std::min(idx >> 4, cpsm == CT32 ? 15u : 31u);
+			idx = bank * 16u + (idx & 15u);
+		}
+		if (cpsm == CT32)
+			c = uint32_t(clut[idx]) | (uint32_t(clut[idx + 256u]) << 16);
+		else
+			c = rgba16_to_rgba32(clut[idx], aem, ta0, ta1);
+	}
+	else
+	{
+		c = in;
+		const bool zero_alpha = aem && (c & 0xffffffu) == 0;
+		if (psm == CT24 || psm == Z24)
+			c = (c & 0xffffffu) | ((zero_alpha ? 0u : ta0) << 24);
+		else if (psm == CT16 || psm == CT16S || psm == Z16 || psm == Z16S)
+			c = (c & 0xffffffu) | ((zero_alpha ? 0u : ((c >> 24) == 0 ? ta0 : ta1)) << 24);
+	}
+	for (uint32_t ch = 0; ch < 4; ch++)
+		out[ch] = float((c >> (8 * ch)) & 0xffu);
+}
+
+// Frame-buffer feedback classification (gs/gs_interface.cpp:2141-2307 and 2388-2470).
+namespace
+{
+uint32_t compat_key(uint32_t psm)
+{
+	switch (psm)
+	{
+	case Z32: return 2;
+	case CT16: return 3;
+	case Z16: return 4;
+	case CT16S: return 5;
+	case Z16S: return 6;
+	case T8: return 7;
+	case T4: return 8;
+	case Z24: return 9;
+	default: return 1;
+	}
+}
+
+uint32_t word_mask(uint32_t psm)
+{
+	switch (psm)
+	{
+	case CT24: case Z24: return 0xffffffu;
+	case T8H: return 0xff000000u;
+	case T4HL: return 0x0f000000u;
+	case T4HH: return 0xf0000000u;
+	default: return 0xffffffffu;
+	}
+}
+}
+
+uint32_t GS::feedback_kind(uint32_t ctx, const Vertex *v, uint32_t n, bool quad, const int32_t bb[4],
+                           bool &depth) const
+{
+	depth = false;
+	const uint64_t t0 = regs.tex0[ctx], t1 = regs.tex1[ctx], cl = regs.clamp[ctx];
+	const uint64_t frame = regs.frame[ctx], zbuf = regs.zbuf[ctx], test = regs.test[ctx];
+	const uint32_t mmin = field(t1, 6, 3);
+	if (mmin >= 2 && mmin <= 5 && field(t1, 2, 3) != 0)
+		return 0;
+	const uint32_t psm = field(t0, 20, 6), tbw = field(t0, 14, 6), fpsm = field(frame, 24, 6);
+	const uint32_t wms = field(cl, 0, 2), wmt = field(cl, 2, 2);
+	const bool palette = psm == T8 || psm == T4 || psm == T8H || psm == T4HL || psm == T4HH;
+	uint32_t base = field(t0, 0, 14);
+	if (!palette)
+	{
+		// Page size of the texture format: 64 or 128 pixels wide, 32, 64 or 128 high.
+		uint32_t pw = 6, ph = 5;
+		if (psm == CT16 || psm == CT16S || psm == Z16 || psm == Z16S) ph = 6;
+		if (wms == 2 && field(cl, 4, 10))
+			base += 32u * (field(cl, 4, 10) >> pw);
+		if (wmt == 2 && field(cl, 24, 10))
+			base += 32u * tbw * (field(cl, 24, 10) >> ph);
+	}
+	bool z_sensitive = false;
+	for (const Prim &q : rp_prims)
+		z_sensitive |= (q.state & (S_Z_TEST | S_Z_WRITE)) != 0;
+	if (field(test, 16, 1))
+	{
+		const bool implied_mask = field(test, 0, 1) && field(test, 1, 3) == 0 && field(test, 12, 2) != 2;
+		if (field(test, 17, 2) > 1 || (!implied_mask && !field(zbuf, 32, 1)))
+			z_sensitive = true;
+	}
+	const bool on_color = base == field(frame, 0, 9) * 32u;
+	const bool on_depth = base == field(zbuf, 0, 9) * 32u && z_sensitive;
+	const bool awkward = on_color && (compat_key(psm) != compat_key(fpsm) || tbw != field(frame, 16, 6));
+	if (!awkward)
+	{
+		if (on_depth && (compat_key(psm) != compat_key(field(zbuf, 24, 4) | 0x30u) || tbw != field(frame, 16, 6)))
+			return 0;
+		if (wms == 3 || wmt == 3 || (!on_color && !on_depth))
+			return 0;
+	}
+	const bool color_fb = on_color && (awkward || (word_mask(psm) & word_mask(fpsm)));
+	const bool depth_fb = !awkward && on_depth && (word_mask(psm) & word_mask(field(zbuf, 24, 4)));
+	if (!color_fb && !depth_fb)
+		return 0;
+	// Affine triangles and sprites only.
+	if (!(n == 3 || (quad && n == 2)))
+		return 0;
+	if (awkward)
+		return 2;
+	const int32_t tw = 1 << field(t0, 26, 4), th = 1 << field(t0, 30, 4);
+	const uint32_t nv = quad ? 2 : 3;
+	int32_t uv[3][2];
+	if (field(regs.prim, 8, 1))
+	{
+		for (uint32_t i = 0; i < nv; i++)
+		{
+			uv[i][0] = int32_t(v[i].u);
+			uv[i][1] = int32_t(v[i].v);
+		}
+	}
+	else
+	{
+		if (!quad && (v[0].q != v[1].q || v[1].q != v[2].q))
+			return 0;
+		for (uint32_t i = 0; i < nv; i++)
+		{
+			const float inv_q = 1.0f / v[i].q;
+			const float fu = float(tw * 16) * (v[i].s * inv_q), fv = float(th * 16) * (v[i].t * inv_q);
+			if (!(std::fabs(fu) < 1073741824.0f) || !(std::fabs(fv) < 1073741824.0f))
+				return 0;
+			uv[i][0] = int32_t(fu);
+			uv[i][1] = int32_t(fv);
+		}
+	}
+	if (wms == 2)
+	{
+		if (bb[0] < int32_t(field(cl, 4, 10)) || bb[2] > int32_t(field(cl, 14, 10)))
+			return 0;
+	}
+	else if (bb[2] >= tw)
+		return 0;
+	if (wmt == 2)
+	{
+		if (bb[1] < int32_t(field(cl, 24, 10)) || bb[3] > int32_t(field(cl, 34, 10)))
+			return 0;
+	}
+	else if (bb[3] >= th)
+		return 0;
+	int32_t lo = 1 << 30, hi = -(1 << 30);
+	for (uint32_t i = 0; i < nv; i++)
+	{
+		const int32_t d[2] = { uv[i][0] - v[i].x, uv[i][1] - v[i].y };
+		for (int32_t dd : d)
+		{
+			lo = std::min(lo, dd);
+			hi = std::max(hi, dd);
+		}
+	}
+	const bool ok = field(t1, 5, 1) ? (lo == 8 && hi == 8) : (lo >= 0 && hi < 16);
+	if (!ok)
+		return 0;
+	depth = !color_fb;
+	return 1;
+}
+
 void GS::append(const Vertex *vin, uint32_t n, bool quad)
 {
 	Vertex v[3] = { vin[0], vin[1], vin[2] };
@@ -461,7 +645,30 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 	// sees every earlier primitive (the reference's page tracker, gs/page_tracker.cpp, orders
 	// texture reads after pending writes the same way). Fused parallelograms skip the check,
 	// as in the reference (gs/gs_interface.cpp:2570-2599).
-	if (field(prim, 4, 1) && !rp_prims.empty() && texture_hazard(ctx))
+	uint32_t fbk = 0;
+	bool fb_depth = false;
+	if (field(prim, 4, 1))
+	{
+		const int32_t bbox[4] = { lox, loy, hix, hiy };
+		fbk = feedback_kind(ctx, v, n, quad, bbox, fb_depth);
+	}
+	if (fbk == 1)
+	{
+		// A pass holds one kind and texture format of on-tile feedback
+		// (gs/gs_interface.cpp:909-933).
+		const uint64_t t0 = regs.tex0[ctx];
+		const uint32_t psm = field(t0, 20, 6);
+		const bool palette = psm == T8 || psm == T4 || psm == T8H || psm == T4HL || psm == T4HH;
+		const uint32_t cpsm = palette ? field(t0, 51, 4) : 0u;
+		for (const Prim &q : rp_prims)
+			if (q.feedback)
+			{
+				if (q.feedback != (fb_depth ? 2u : 1u) || q.fb_tex_psm != psm || (palette && q.fb_tex_cpsm != cpsm))
+					flush_render_pass();
+				break;
+			}
+	}
+	else if (field(prim, 4, 1) && fbk == 0 && !rp_prims.empty() && texture_hazard(ctx))
 		flush_render_pass();
 
 	if (rp_prims.empty())
@@ -511,7 +718,26 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 
 	// Primitive state (gs/gs_interface.cpp:1804-1968).
 	uint32_t state = 0;
-	if (tme)
+	if (fbk == 1)
+	{
+		// The texel is the pixel's own colour or depth (ubershader.comp:955-1040).
+		const uint64_t t0 = regs.tex0[ctx];
+		p.feedback = fb_depth ? 2u : 1u;
+		p.fb_tex_psm = field(t0, 20, 6);
+		const bool palette = p.fb_tex_psm == T8 || p.fb_tex_psm == T4 || p.fb_tex_psm == T8H ||
+		                     p.fb_tex_psm == T4HL || p.fb_tex_psm == T4HH;
+		p.fb_tex_cpsm = palette ? field(t0, 51, 4) : 0u;
+		p.fb_csa = field(t0, 56, 5);
+		p.fb_texa = regs.texa;
+		p.tex = 1u << 15;
+		// Sampler bits as for a texture (triangle setup reads the magnification filter).
+		const uint64_t t1 = regs.tex1[ctx], cl = regs.clamp[ctx];
+		p.tex |= field(t1, 5, 1) ? T_MAG_LINEAR : 0;
+		p.tex |= (field(cl, 0, 2) == 1 || field(cl, 0, 2) == 2) ? T_CLAMP_S : 0;
+		p.tex |= (field(cl, 2, 2) == 1 || field(cl, 2, 2) == 2) ? T_CLAMP_T : 0;
+		p.tex2 = 1u;
+	}
+	else if (tme)
 	{
 		p.tex_slot = texture_slot(ctx);
 		uint64_t t1 = regs.tex1[ctx], cl = regs.clamp[ctx];
@@ -557,6 +783,8 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 		if (can_mask_color || can_mask_z)
 			needs_prev = true;
 	}
+	if (fbk == 1)
+		needs_prev = true; // reading the pixel's own value blends with it
 	if (!needs_prev)
 		state |= S_OPAQUE;
 	if (aa1)
@@ -1553,7 +1781,10 @@ void GS::render(const std::vector<Prim> &prims_in)
 		const uint32_t m = write_mask & 0xf8f8f8u;
 		const bool channel_slice = !fb_32_24 && m != 0xf8f8f8u && m != 0;
 		const bool aliased = z_write_any && f.zbp == f.fbp;
-		if (channel_slice || aliased)
+		bool z_feedback = false;
+		for (const Prim &q : prims)
+			z_feedback |= q.feedback == 2;
+		if (channel_slice || z_feedback || aliased)
 			rate_x = rate_y = 0;
 	}
 
@@ -1648,7 +1879,10 @@ void GS::render(const std::vector<Prim> &prims_in)
 			u = std::fmin(std::fmax(u, -2047.0f), 2047.0f);
 			vv = std::fmin(std::fmax(vv, -2047.0f), 2047.0f);
 			float tx[4];
-			sample_texture(rp_textures[p.tex_slot], p.tex, p.tex2, u, vv, sq[2], persp, tx);
+			if (p.feedback)
+				feedback_texture(p, p.feedback == 2 ? r.depth : r.color, tx);
+			else
+				sample_texture(rp_textures[p.tex_slot], p.tex, p.tex2, u, vv, sq[2], persp, tx);
 			const bool tcc = p.combiner & C_TCC;
 			switch (p.combiner & 3u)
 			{

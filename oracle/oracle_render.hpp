@@ -49,6 +49,10 @@ struct Prim
 	uint32_t state, tex, tex2, alpha, fbmsk, fogcol;
 	uint32_t combiner, blend_mode, dimx[2];
 	uint32_t tex_slot;    // texture slot in the render pass
+	// On-tile feedback: 0 none, 1 the pixel's colour, 2 its depth is the texel; with the
+	// texture's PSM, CLUT format and CSA.
+	uint32_t feedback, fb_tex_psm, fb_tex_cpsm, fb_csa;
+	uint64_t fb_texa;     // TEXA at the draw
 };
 
 // Result of triangle setup (gs/shaders/data_structures.h:32-52 content).
@@ -209,6 +213,11 @@ private:
 	uint32_t texture_slot(uint32_t ctx);
 	std::vector<PageSpan> texture_span(uint32_t ctx) const;
 	bool texture_hazard(uint32_t ctx) const;
+	// 0: ordinary texture read; 1: on-tile feedback (depth set for the depth buffer); 2: the
+	// texture aliases the frame buffer with another layout and is read as the pass starts.
+	uint32_t feedback_kind(uint32_t ctx, const Vertex *v, uint32_t n, bool quad, const int32_t bb[4],
+	                       bool &depth) const;
+	void feedback_texture(const Prim &p, uint32_t in, float out[4]) const;
 };
 
 // Reciprocal helpers (gs/shaders/math_utils.h:27-101) and their tables

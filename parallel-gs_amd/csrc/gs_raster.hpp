@@ -26,6 +26,12 @@ enum : uint32_t
 	TEX_MAX_MIP_LEVEL_SHIFT = 21, // TEX1.MXL of a mipmapped primitive, 3 bits
 	TEX2_FIXED_LOD = 1u << 0,     // LCM; L in bits 1..2, K (signed, 1/16 units) in bits 3..14
 	TEX2_L_SHIFT = 1, TEX2_K_SHIFT = 3,
+	// On-tile frame-buffer feedback (the texel is the pixel's own colour or depth): bit 15 of the
+	// index marks it and bits 0..4 hold CSA; the texture's PSM, CPSM and the depth source sit in
+	// bits 21..31 (a feedback primitive is never mipmapped), TEXA in tex2.
+	TEX_FEEDBACK = 1u << 15, TEX_FEEDBACK_PSM_SHIFT = 21, TEX_FEEDBACK_CPSM_SHIFT = 27,
+	TEX_FEEDBACK_DEPTH = 1u << 31,
+	TEX2_FEEDBACK_AEM = 1u << 1, TEX2_FEEDBACK_TA0_SHIFT = 2, TEX2_FEEDBACK_TA1_SHIFT = 10,
 };
 
 enum : uint32_t

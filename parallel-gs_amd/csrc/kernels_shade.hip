@@ -388,6 +388,50 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 		out[ch] = float((c >> (8 * ch)) & 0xffu);
 }
 
+// On-tile frame-buffer feedback (ubershader.comp:955-1040): the texel is the pixel's (in a
+// super-sampled pass the sample's) current colour, or depth, read as the texture's format:
+// palette formats look up the CLUT with the index bits of the word (CSA banks as in the texture
+// decode, 8-bit indices clamped to the last bank), 24-bit formats take TA0 as alpha, 16-bit ones
+// TA0 / TA1 by the alpha bit, with AEM zeroing alpha for black.
+__device__ __forceinline__ void feedback_texel(const pgs_shade_args &a, uint32_t tex, uint32_t tex2, uint32_t color,
+                                               uint32_t depth, float out[4])
+{
+	const uint32_t in = (tex & TEX_FEEDBACK_DEPTH) ? depth : color;
+	const uint32_t psm = (tex >> TEX_FEEDBACK_PSM_SHIFT) & 63u, cpsm = (tex >> TEX_FEEDBACK_CPSM_SHIFT) & 15u;
+	const bool aem = (tex2 & TEX2_FEEDBACK_AEM) != 0;
+	const uint32_t ta0 = (tex2 >> TEX2_FEEDBACK_TA0_SHIFT) & 0xffu, ta1 = (tex2 >> TEX2_FEEDBACK_TA1_SHIFT) & 0xffu;
+	uint32_t c;
+	if (psm == PSMT8 || psm == PSMT8H || psm == PSMT4 || psm == PSMT4HL || psm == PSMT4HH)
+	{
+		uint32_t csa = tex & 31u;
+		if (cpsm == PSMCT32)
+			csa &= 15u;
+		uint32_t index = psm == PSMT8H ? in >> 24 : psm == PSMT4HL ? (in >> 24) & 15u : psm == PSMT4HH ? in >> 28 : 0u;
+		index += csa * 16u;
+		if (psm == PSMT8 || psm == PSMT8H)
+			index = min(index >> 4, cpsm == PSMCT32 ? 15u : 31u) * 16u + (index & 15u);
+		if (cpsm == PSMCT32)
+			c = uint32_t(a.clut[index]) | (uint32_t(a.clut[index + 256u]) << 16);
+		else
+		{
+			const uint32_t lo = a.clut[index];
+			const uint32_t alpha = (aem && lo == 0u) ? 0u : ((lo >> 15) ? ta1 : ta0);
+			c = ((lo & 0x1fu) << 3) | (((lo >> 5) & 0x1fu) << 11) | (((lo >> 10) & 0x1fu) << 19) | (alpha << 24);
+		}
+	}
+	else
+	{
+		c = in;
+		const bool zero_alpha = aem && (c & 0xffffffu) == 0u;
+		if (psm == PSMCT24 || psm == PSMZ24)
+			c = (c & 0xffffffu) | ((zero_alpha ? 0u : ta0) << 24);
+		else if (psm == PSMCT16 || psm == PSMCT16S || psm == PSMZ16 || psm == PSMZ16S)
+			c = (c & 0xffffffu) | ((zero_alpha ? 0u : ((c >> 24) == 0u ? ta0 : ta1)) << 24);
+	}
+	for (uint32_t ch = 0; ch < 4; ch++)
+		out[ch] = float((c >> (8 * ch)) & 0xffu);
+}
+
 struct PassInfo
 {
 	uint32_t fb_psm, z_psm;
@@ -567,25 +611,30 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 	float comb[4] = { col[0], col[1], col[2], col[3] };
 	if (sv_combiner & COMB_TME)
 	{
-		float stq[3][4];
-		TextureDesc td;
-		load_texture_data(tap, static_cast<const TextureDesc *>(a.textures) + (p_tex & TEX_INDEX_MASK), stq, td);
-		// S, T, Q at the attribute position.
-		auto stqf = [&](int k) {
-			return __fadd_rn(__fadd_rn(stq[0][k], __fmul_rn(stq[1][k], ai)), __fmul_rn(stq[2][k], aj));
-		};
-		const bool persp = (st & ST_PERSPECTIVE) != 0;
-		float u = stqf(0), v = stqf(1);
-		if (persp) // q == 1 divides exactly, as the reference's q != 1 test skips
-		{
-			const float q = stqf(2);
-			u = __fdiv_rn(u, q);
-			v = __fdiv_rn(v, q);
-		}
-		u = fminf(fmaxf(u, -2047.0f), 2047.0f);
-		v = fminf(fmaxf(v, -2047.0f), 2047.0f);
 		float tx[4];
-		sample_texture(td, a.tex_images, p_tex, p_tex2, u, v, stqf(2), persp, tx);
+		if (p_tex & TEX_FEEDBACK)
+			feedback_texel(a, p_tex, p_tex2, color, depth, tx);
+		else
+		{
+			float stq[3][4];
+			TextureDesc td;
+			load_texture_data(tap, static_cast<const TextureDesc *>(a.textures) + (p_tex & TEX_INDEX_MASK), stq, td);
+			// S, T, Q at the attribute position.
+			auto stqf = [&](int k) {
+				return __fadd_rn(__fadd_rn(stq[0][k], __fmul_rn(stq[1][k], ai)), __fmul_rn(stq[2][k], aj));
+			};
+			const bool persp = (st & ST_PERSPECTIVE) != 0;
+			float u = stqf(0), v = stqf(1);
+			if (persp) // q == 1 divides exactly, as the reference's q != 1 test skips
+			{
+				const float q = stqf(2);
+				u = __fdiv_rn(u, q);
+				v = __fdiv_rn(v, q);
+			}
+			u = fminf(fmaxf(u, -2047.0f), 2047.0f);
+			v = fminf(fmaxf(v, -2047.0f), 2047.0f);
+			sample_texture(td, a.tex_images, p_tex, p_tex2, u, v, stqf(2), persp, tx);
+		}
 		const bool tcc = (sv_combiner & COMB_TCC) != 0;
 		const uint32_t mode = sv_combiner & COMB_MODE_MASK;
 		if (mode == 1) // DECAL

@@ -238,6 +238,151 @@ uint32_t RenderPassBuilder::state_index(const StateVector &sv)
 	return idx;
 }
 
+namespace
+{
+// Formats sharing a swizzle layout (gs/shaders/swizzle_utils.h:298-325): colour-32 and the
+// 8H / 4HL / 4HH / 24 formats living in its words share one key.
+uint32_t swizzle_compat_key(uint32_t psm)
+{
+	switch (psm)
+	{
+	case PSMZ32: return 2;
+	case PSMCT16: return 3;
+	case PSMZ16: return 4;
+	case PSMCT16S: return 5;
+	case PSMZ16S: return 6;
+	case PSMT8: return 7;
+	case PSMT4: return 8;
+	case PSMZ24: return 9;
+	default: return 1;
+	}
+}
+
+// Bits of a 32-bit word a format writes (gs/gs_util.cpp:13-29).
+uint32_t psm_word_write_mask(uint32_t psm)
+{
+	switch (psm)
+	{
+	case PSMCT24: case PSMZ24: return 0xffffffu;
+	case PSMT8H: return 0xffu << 24;
+	case PSMT4HL: return 0xfu << 24;
+	case PSMT4HH: return 0xfu << 28;
+	default: return 0xffffffffu;
+	}
+}
+}
+
+// Whether a textured primitive samples its own frame buffer (colour) or depth buffer pixel for
+// pixel (gs/gs_interface.cpp:2141-2307 update_color_feedback_state and 2388-2470
+// deduce_color_feedback_mode). Pixel: the texture starts at the frame (or depth) buffer with the
+// same layout and stride, the primitive is an affine triangle or a sprite inside the texture
+// whose UVs land on the pixels' own texels (offsets in [0, 1) texel with NEAREST, exactly half a
+// texel with LINEAR); the shader then reads the pixel's current value, per sample in a
+// super-sampled pass. Bypass: the texture aliases the frame buffer with another layout; it is
+// read as decoded at the start of the pass (no ordering against the pass's own writes).
+// Everything else (None) is an ordinary texture read, ordered by the pass's hazard checks.
+RenderPassBuilder::Feedback RenderPassBuilder::feedback_mode(const RegisterState &r, uint32_t ctx,
+                                                             const VertexPosition *p, const VertexAttribute *a,
+                                                             uint32_t n, bool quad, const int32_t bb[4],
+                                                             bool &depth) const
+{
+	depth = false;
+	const uint64_t t0 = r.tex0[ctx], t1 = r.tex1[ctx], cl = r.clamp[ctx], frame = r.frame[ctx], zbuf = r.zbuf[ctx];
+	if (mmin_has_mipmap(t1) && field(t1, 2, 3) != 0 && !disable_mipmaps)
+		return Feedback::None;
+	const uint32_t tex_psm = field(t0, 20, 6), tbw = field(t0, 14, 6), fb_psm = field(frame, 24, 6);
+	const uint32_t wms = field(cl, 0, 2), wmt = field(cl, 2, 2);
+	const bool palette = tex_psm == PSMT8 || tex_psm == PSMT4 || tex_psm == PSMT8H || tex_psm == PSMT4HL ||
+	                     tex_psm == PSMT4HH;
+	uint32_t tbp0 = field(t0, 0, 14);
+	// REGION_CLAMP can align the texture's effective base with the frame buffer.
+	if (!palette)
+	{
+		const Geometry g = psm_geometry(tex_psm);
+		if (wms == 2 && field(cl, 4, 10) != 0)
+			tbp0 += 32u * (field(cl, 4, 10) >> g.pw);
+		if (wmt == 2 && field(cl, 24, 10) != 0)
+			tbp0 += 32u * tbw * (field(cl, 24, 10) >> g.ph);
+	}
+	const uint64_t test = r.test[ctx];
+	bool z_sensitive = pass_z_;
+	if (field(test, 16, 1))
+	{
+		const uint32_t ate = field(test, 0, 1), atst = field(test, 1, 3), afail = field(test, 12, 2);
+		const bool implied_z_mask = ate && atst == 0 && afail != 2;
+		z_sensitive = z_sensitive || field(test, 17, 2) > 1 || (!implied_z_mask && field(zbuf, 32, 1) == 0);
+	}
+	const bool equal_color = tbp0 == field(frame, 0, 9) * 32u;
+	const bool equal_depth = tbp0 == field(zbuf, 0, 9) * 32u && z_sensitive;
+	bool awkward = false;
+	if (equal_color && (swizzle_compat_key(tex_psm) != swizzle_compat_key(fb_psm) || tbw != field(frame, 16, 6)))
+		awkward = true;
+	else
+	{
+		if (equal_depth &&
+		    (swizzle_compat_key(tex_psm) != swizzle_compat_key(field(zbuf, 24, 4) | 0x30u) || tbw != field(frame, 16, 6)))
+			return Feedback::None;
+		if (wms == 3 || wmt == 3 || (!equal_color && !equal_depth))
+			return Feedback::None;
+	}
+	const bool color_fb = equal_color && (awkward || (psm_word_write_mask(tex_psm) & psm_word_write_mask(fb_psm)) != 0);
+	const bool depth_fb = !awkward && equal_depth &&
+	                      (psm_word_write_mask(tex_psm) & psm_word_write_mask(field(zbuf, 24, 4))) != 0;
+	if (!color_fb && !depth_fb)
+		return Feedback::None;
+
+	// Lines and points never read feedback per pixel.
+	if (!(n == 3 || (quad && n == 2)))
+		return Feedback::None;
+	if (awkward)
+		return Feedback::Bypass;
+	const int32_t width = 1 << int32_t(field(t0, 26, 4)), height = 1 << int32_t(field(t0, 30, 4));
+	// Texel positions of the vertices in 1/16 texel (compute_uv_bb, gs/gs_interface.cpp:2309-2380).
+	int32_t uv[3][2] = {};
+	const uint32_t nv = quad ? 2u : 3u;
+	if (field(r.prim, 8, 1))
+	{
+		for (uint32_t i = 0; i < nv; i++)
+		{
+			uv[i][0] = int32_t(a[i].uv & 0xffffu);
+			uv[i][1] = int32_t(a[i].uv >> 16);
+		}
+	}
+	else
+	{
+		// Perspective varies across a triangle whose Q differ: no pixel correspondence.
+		if (!quad && (a[0].q != a[1].q || a[1].q != a[2].q))
+			return Feedback::None;
+		const float fw = float(width * 16), fh = float(height * 16);
+		for (uint32_t i = 0; i < nv; i++)
+		{
+			const float inv_q = 1.0f / a[i].q;
+			const float u = fw * (a[i].s * inv_q), v = fh * (a[i].t * inv_q);
+			// Anything not a modest finite position cannot land on the pixel's own texel.
+			if (!(std::fabs(u) < 1073741824.0f) || !(std::fabs(v) < 1073741824.0f))
+				return Feedback::None;
+			uv[i][0] = int32_t(u);
+			uv[i][1] = int32_t(v);
+		}
+	}
+	// A primitive reaching outside the texture (or its clamp region) sees clamping or wrapping.
+	if (wms == 2 ? (bb[0] < int32_t(field(cl, 4, 10)) || bb[2] > int32_t(field(cl, 14, 10))) : bb[2] >= width)
+		return Feedback::None;
+	if (wmt == 2 ? (bb[1] < int32_t(field(cl, 24, 10)) || bb[3] > int32_t(field(cl, 34, 10))) : bb[3] >= height)
+		return Feedback::None;
+	int32_t lo = INT32_MAX, hi = INT32_MIN;
+	for (uint32_t i = 0; i < nv; i++)
+	{
+		const int32_t dx = uv[i][0] - p[i].x, dy = uv[i][1] - p[i].y;
+		lo = std::min(lo, std::min(dx, dy));
+		hi = std::max(hi, std::max(dx, dy));
+	}
+	if (field(t1, 5, 1) ? (lo != 8 || hi != 8) : (lo < 0 || hi >= 16))
+		return Feedback::None;
+	depth = !color_fb;
+	return Feedback::Pixel;
+}
+
 // gs/gs_interface.cpp:2473-2836 (one frame-buffer instance per render pass).
 void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, uint32_t n, bool quad)
 {
@@ -306,9 +451,28 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 	// parallelograms above skip the check, as in gs/gs_interface.cpp:2570-2599.
 	const bool tme = field(prim, 4, 1);
 	int32_t slot = -1;
+	Feedback feedback = Feedback::None;
+	bool feedback_depth = false;
 	if (tme)
 	{
-		if (!prims_.empty() && texture_hazard(r, ctx))
+		const int32_t pbb[4] = { lox, loy, hix, hiy };
+		feedback = feedback_mode(r, ctx, p, a, n, quad, pbb, feedback_depth);
+	}
+	if (feedback == Feedback::Pixel)
+	{
+		// One feedback kind and format per pass (gs/gs_interface.cpp:909-933).
+		const uint64_t t0 = r.tex0[ctx];
+		const uint32_t kind = feedback_depth ? 2u : 1u, psm = field(t0, 20, 6);
+		const bool palette = psm == PSMT8 || psm == PSMT4 || psm == PSMT8H || psm == PSMT4HL || psm == PSMT4HH;
+		const uint32_t cpsm = palette ? field(t0, 51, 4) : 0u;
+		if (!prims_.empty() && pass_feedback_ &&
+		    (kind != pass_feedback_ || psm != pass_feedback_psm_ || cpsm != pass_feedback_cpsm_))
+			flush();
+	}
+	else if (tme)
+	{
+		// Bypass reads the texture as it is when the pass starts, whatever the pass writes.
+		if (!prims_.empty() && feedback == Feedback::None && texture_hazard(r, ctx))
 			flush();
 		slot = texture_slot(r, ctx);
 		if (slot < 0)
@@ -365,7 +529,34 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 
 	PrimitiveAttribute pa{};
 	uint32_t state = state_index(sv);
-	if (tme)
+	if (feedback == Feedback::Pixel)
+	{
+		// The texel is the pixel's own colour or depth (ubershader.comp:955-1040): the index
+		// carries the feedback bit and CSA, the word the formats, tex2 the TEXA expansion.
+		const uint64_t t0 = r.tex0[ctx];
+		const uint32_t psm = field(t0, 20, 6);
+		const bool palette = psm == PSMT8 || psm == PSMT4 || psm == PSMT8H || psm == PSMT4HL || psm == PSMT4HH;
+		uint32_t cpsm = palette ? field(t0, 51, 4) : 0u;
+		if (cpsm == PSMCT24)
+			cpsm = PSMCT32;
+		pa.tex = TEX_FEEDBACK | field(t0, 56, 5) | psm << TEX_FEEDBACK_PSM_SHIFT | cpsm << TEX_FEEDBACK_CPSM_SHIFT |
+		         (feedback_depth ? TEX_FEEDBACK_DEPTH : 0u);
+		// The sampler bits stay as for a texture: triangle setup reads the magnification filter.
+		const uint64_t t1 = r.tex1[ctx], cl = r.clamp[ctx];
+		pa.tex |= field(t1, 5, 1) ? TEX_MAG_LINEAR : 0;
+		pa.tex |= (field(cl, 0, 2) == 1 || field(cl, 0, 2) == 2) ? TEX_CLAMP_S : 0;
+		pa.tex |= (field(cl, 2, 2) == 1 || field(cl, 2, 2) == 2) ? TEX_CLAMP_T : 0;
+		pa.tex2 = TEX2_FIXED_LOD | (field(r.texa, 15, 1) ? TEX2_FEEDBACK_AEM : 0u) |
+		          field(r.texa, 0, 8) << TEX2_FEEDBACK_TA0_SHIFT | field(r.texa, 32, 8) << TEX2_FEEDBACK_TA1_SHIFT;
+		if (!pass_feedback_)
+		{
+			pass_feedback_ = feedback_depth ? 2u : 1u;
+			pass_feedback_psm_ = psm;
+			pass_feedback_cpsm_ = palette ? field(t0, 51, 4) : 0u;
+		}
+		pass_z_feedback_ = pass_z_feedback_ || feedback_depth;
+	}
+	else if (tme)
 	{
 		const uint64_t t1 = r.tex1[ctx], cl = r.clamp[ctx];
 		pa.tex = uint32_t(slot);
@@ -406,6 +597,9 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 		if (afail != 1 || (can_write_z && afail != 2))
 			needs_prev = true;
 	}
+	// Reading the pixel's own value is a kind of blending (gs/gs_interface.cpp:1928-1931).
+	if (feedback == Feedback::Pixel)
+		needs_prev = true;
 	if (!needs_prev)
 		state |= ST_OPAQUE;
 	if (aa1)
@@ -486,6 +680,7 @@ void RenderPassBuilder::flush()
 		d.rate_log2 = rate_log2_;
 		d.z_sensitive = pass_z_;
 		d.z_write = pass_zwrite_;
+		d.z_feedback = pass_z_feedback_;
 		d.zbuf_psm = field(zbuf_, 24, 4) | 0x30u;
 		d.decodes = decodes_.data();
 		d.tex_descs = tex_descs_.data();
@@ -523,6 +718,9 @@ void RenderPassBuilder::flush()
 		tex_descs_.clear();
 		texel_total_ = group_total_ = 0;
 		pass_z_ = pass_zwrite_ = false;
+		pass_feedback_ = 0;
+		pass_feedback_psm_ = pass_feedback_cpsm_ = 0;
+		pass_z_feedback_ = false;
 		pos_.clear();
 		attr_.clear();
 		states_.clear();

@@ -33,6 +33,7 @@ struct RenderPassDesc
 	uint32_t rate_log2;
 	bool z_sensitive;         // some primitive tests or writes depth
 	bool z_write;             // some primitive writes depth
+	bool z_feedback;          // some primitive reads its own depth as a texture (renders single-sampled)
 	uint32_t zbuf_psm;        // ZBUF.PSM as written (0x30 | low bits), before format deduction
 	// Textures of the pass, decoded from VRAM before shading.
 	const TexDecode *decodes;
@@ -161,6 +162,11 @@ private:
 	uint32_t color_write_mask_ = 0;
 	bool pass_z_ = false;      // some primitive tests or writes depth
 	bool pass_zwrite_ = false; // some primitive writes depth
+	// On-tile texture feedback of the pass (gs/gs_interface.cpp:909-933): one kind (colour or
+	// depth) and one texture format per pass; depth feedback renders the pass single-sampled.
+	uint32_t pass_feedback_ = 0; // 0 none, 1 colour, 2 depth
+	uint32_t pass_feedback_psm_ = 0, pass_feedback_cpsm_ = 0;
+	bool pass_z_feedback_ = false;
 
 	// The image depends on TEX0 (without CLD), CLAMP, TEXA, the level count and the level
 	// addresses in MIPTBP1 / MIPTBP2 (unused levels' fields cleared).
@@ -212,6 +218,10 @@ private:
 	uint32_t state_index(const StateVector &sv);
 	// Texture slot of the current TEX0 / CLAMP / TEXA; -1 when the pass has no room for it.
 	int32_t texture_slot(const RegisterState &r, uint32_t ctx);
+	// How a textured primitive reads the frame or depth buffer it draws to.
+	enum class Feedback { None, Pixel, Bypass };
+	Feedback feedback_mode(const RegisterState &r, uint32_t ctx, const VertexPosition *p, const VertexAttribute *a,
+	                       uint32_t n, bool quad, const int32_t bb[4], bool &depth) const;
 	// Does the texture of `ctx` read pages the pending pass writes?
 	bool texture_hazard(const RegisterState &r, uint32_t ctx);
 };

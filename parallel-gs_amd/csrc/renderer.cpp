@@ -633,7 +633,8 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 		return;
 
 	// Sampling rate of the pass (gs/gs_interface.cpp:212-260): channel-sliced 16-bit colour
-	// masks and colour/depth aliasing with depth writes render single-sampled.
+	// masks, depth read back as a texture on tile and colour/depth aliasing with depth writes
+	// render single-sampled.
 	const bool ssaa_context = num_planes_ > 1;
 	uint32_t rate_x, rate_y;
 	rates(rate_x, rate_y);
@@ -643,7 +644,7 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 		const uint32_t m = ~d.opaque_fbmask & 0xf8f8f8u;
 		const bool channel_slice = !fb_32_24 && m != 0xf8f8f8u && m != 0;
 		const bool aliased = d.z_write && d.zbp == d.fbp;
-		if (channel_slice || aliased)
+		if (channel_slice || d.z_feedback || aliased)
 			rate_x = rate_y = 0;
 	}
 	const bool super_pass = rate_y != 0;
@@ -848,6 +849,7 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	a.counts = counts_buf_;
 	a.textures = d.num_textures ? static_cast<const void *>(arena_.dev + off_tex) : dummy_buf_;
 	a.tex_images = d.num_textures ? tex_images_ : dummy_buf_;
+	a.clut = reinterpret_cast<const uint16_t *>(clut_buf_);
 	a.list_stride = n;
 	a.origin_x = ox;
 	a.origin_y = oy;
@@ -1074,7 +1076,7 @@ void launch_resources(const Renderer::Launch &l, uint32_t &reads, uint32_t &writ
 	case K::SsaaHeuristic: reads = R_PRIMS | R_SETUP | R_FLAT; writes = R_PRIMS | R_FLAT; break;
 	case K::Binning: reads = R_SETUP | R_PRIMS; writes = R_BINS; break;
 	case K::Shade:
-	case K::ShadeSuper: reads = R_VRAM | R_SETUP | R_PRIMS | R_BINS | R_TEXIMG; writes = R_VRAM; break;
+	case K::ShadeSuper: reads = R_VRAM | R_CLUT | R_SETUP | R_PRIMS | R_BINS | R_TEXIMG; writes = R_VRAM; break;
 	default: reads = writes = ~0u; break; // anything else orders against everything
 	}
 }

@@ -267,3 +267,79 @@ def random_textured_stream(seed: int, n_batches: int = 10, prims_per_batch: int 
         for i in range(0, len(writes), 240):
             out.append(gif.ad(writes[i:i + 240]))
     return gif.arr(b"".join(out))
+
+
+def random_fb_feedback_stream(seed: int, n_batches: int = 10, prims_per_batch: int = 12) -> np.ndarray:
+    """Frame-buffer feedback: a 128 x 64 frame (CT32, CT24 or CT16) with random contents and 3D
+    triangles (varying depth), then sprites and triangles textured from the frame buffer itself
+    (same layout, 24-bit or palette views in the top bits of the word, a 16-bit view of a 32-bit
+    frame) or from the depth buffer, with UVs on the pixels' own texels (offsets 0..15 / 16, half
+    a texel with LINEAR) or off them, in UV and ST form, with random combiners, TEXA, blending
+    and depth tests."""
+    rng = np.random.default_rng(seed)
+    ofs = 2048 * 16
+    fw, fh = 128, 64
+    zbp = 64
+    out = [gif.ad([(0x1A, 1), (0x18, ofs | ofs << 32)])]
+    out.append(gif.upload(0, 2, 0x00, 0, 0, fw, fh, _random_payload(rng, fw * fh * 4)))
+    out.append(gif.upload(zbp * 32, 2, 0x00, 0, 0, fw, fh, _random_payload(rng, fw * fh * 4)))
+    clut_bp = 2048
+    out.append(gif.upload(clut_bp, 1, 0x00, 0, 0, 16, 16, _random_payload(rng, 16 * 16 * 4)))
+    for b in range(n_batches):
+        fb_psm = int(rng.choice([0x00, 0x00, 0x01, 0x02]))
+        z_psm = int(rng.integers(0, 3))
+        regs = [(0x4C, frame(0, 2, fb_psm)), (0x4E, zbuf(zbp, z_psm, int(rng.integers(0, 2)))),
+                (0x40, scissor(0, fw - 1, 0, fh - 1)),
+                (0x47, test_reg(zte=1, ztst=int(rng.integers(1, 4)))),
+                (0x42, alpha_reg(0, 1, 0, 1, 0)),
+                (0x3B, texa(int(rng.integers(0, 256)), int(rng.integers(0, 2)), int(rng.integers(0, 256))))]
+        out.append(gif.ad(regs))
+        writes = []
+        for p in range(prims_per_batch):
+            kind = int(rng.integers(0, 4))
+            if kind == 0:
+                # A 3D triangle.
+                writes.append((0x00, prim_reg(3, iip=1, abe=int(rng.integers(0, 2)))))
+                for v in range(3):
+                    writes.append((0x01, rgbaq(int(rng.integers(0, 2**32)))))
+                    writes.append((0x05, xyz2(ofs + int(rng.integers(-8 * 16, (fw + 8) * 16)),
+                                              ofs + int(rng.integers(-8 * 16, (fh + 8) * 16)),
+                                              int(rng.integers(0, 2**24)))))
+                continue
+            depth = rng.integers(0, 5) == 0
+            if depth:
+                tpsm = [0x30, 0x31, 0x32][z_psm]
+                tbp, cpsm = zbp * 32, 0
+            else:
+                choices = [fb_psm]
+                if fb_psm == 0x00:
+                    choices += [0x01, 0x1B, 0x24, 0x2C, 0x02]
+                tpsm = int(rng.choice(choices))
+                tbp, cpsm = 0, int(rng.choice([0x00, 0x02]))
+            mmag = int(rng.integers(0, 2))
+            d = int(rng.choice([0, 3, 8, 8, 15, 24, 8 if mmag else 5]))
+            t1 = tex1(mmag, 0)
+            t0 = tex0(tbp, 2, tpsm, 7, 6, tcc=int(rng.integers(0, 2)), tfx=int(rng.integers(0, 4)), cbp=clut_bp,
+                      cpsm=cpsm, csa=int(rng.integers(0, 16)), cld=int(rng.choice([0, 1])))
+            fst = int(rng.integers(0, 4) != 0)
+            writes += [(0x14, t1), (0x08, 0), (0x06, t0)]
+            x0, y0 = int(rng.integers(0, fw - 8)), int(rng.integers(0, fh - 8))
+            x1, y1 = x0 + int(rng.integers(1, fw - x0)), y0 + int(rng.integers(1, fh - y0))
+            if kind == 1 or kind == 3:
+                corners = [(x0, y0), (x1, y1)]
+                writes.append((0x00, prim_reg(6, iip=0, tme=1, abe=int(rng.integers(0, 2)), fst=fst)))
+            else:
+                corners = [(x0, y0), (x1, y0), (x0, y1)]
+                writes.append((0x00, prim_reg(3, iip=1, tme=1, abe=int(rng.integers(0, 2)), fst=fst)))
+            q = float(rng.choice([1.0, 0.5, 2.0]))
+            for (x, y) in corners:
+                u, v = x * 16 + d, y * 16 + d
+                writes.append((0x01, rgbaq(int(rng.integers(0, 2**32)), q)))
+                if fst:
+                    writes.append((0x03, uv(u, v)))
+                else:
+                    writes.append((0x02, st(u / (fw * 16.0) * q, v / (fh * 16.0) * q)))
+                writes.append((0x05, xyz2(ofs + x * 16, ofs + y * 16, int(rng.integers(0, 2**24)))))
+        for i in range(0, len(writes), 240):
+            out.append(gif.ad(writes[i:i + 240]))
+    return gif.arr(b"".join(out))
