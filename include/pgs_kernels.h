@@ -214,6 +214,19 @@ typedef struct pgs_shade_args
 	uint32_t single_tiles;   /* super-sampled launch only: 1 also shades the tiles whose mode is 1,
 	                            single-sampled, with the first of each block's sample waves (the
 	                            pass then needs no single-sampled launch); 0 leaves them */
+	/* Passes whose depth buffer aliases the colour buffer (FBP == ZBP) shade in several
+	 * launches over primitive ranges (gs/gs_renderer.cpp:2525-2690, the lo/hi primitive index
+	 * of ubershader.comp's push constants): */
+	const uint8_t *src;      /* NULL: colour and depth are read from vram; else from this copy of
+	                            VRAM with the same planes (a single-stepped primitive reads the
+	                            VRAM as it was before the primitive) */
+	const uint8_t *zsrc;     /* NULL: depth is read where colour is; else from this copy (the
+	                            read-only depth cache, gs/gs_renderer.cpp:2184-2232) */
+	uint32_t prim_first;     /* shade primitives prim_first .. prim_first + prim_count - 1 of the
+	                            tile lists; prim_count 0: every primitive from prim_first */
+	uint32_t prim_count;
+	uint32_t skip_stores;    /* bit 0: leave colour unstored, bit 1: leave depth unstored */
+	uint32_t pad_alias;
 } pgs_shade_args;
 
 int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);

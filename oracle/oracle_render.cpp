@@ -1497,10 +1497,15 @@ void sample_offset(uint32_t rate_x, uint32_t rate_y, uint32_t s, int32_t &dx, in
 // begin_tile / end_tile follow ubershader.comp:435-780: samples load from their planes where
 // the reference marks the pixel, and a dirty pixel whose samples differ stores every sample,
 // the reference mask and the resolved value (float average in butterfly order, minimum depth).
+//
+// Only primitives lo..hi shade (the tile lists and tile modes still come from the whole pass);
+// colour is read from csrc and depth from zsrc (copies of the planes, or v0 itself); bit 0 of
+// skip leaves colour unstored, bit 1 depth.
 template <typename Shade>
-uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t context_rate_y, std::vector<Prim> &prims,
-                                   const std::vector<Setup> &setups, const std::vector<bool> &live,
-                                   const PassFormat &f, uint32_t rate_x, uint32_t rate_y, Shade &&shade)
+void render_super_sampled_pass(uint8_t *v0, const uint8_t *csrc, const uint8_t *zsrc, uint32_t lo, uint32_t hi,
+                               uint32_t skip, const int32_t *bb, uint32_t context_rate_y, std::vector<Prim> &prims,
+                               const std::vector<Setup> &setups, const std::vector<bool> &live, const PassFormat &f,
+                               uint32_t rate_x, uint32_t rate_y, Shade &&shade)
 {
 	const uint32_t n = uint32_t(prims.size());
 	const bool ssaa = rate_y != 0;
@@ -1510,10 +1515,6 @@ uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t cont
 	const bool z16 = zs && psm16(f.z_psm), z24 = zs && psm24(f.z_psm);
 	const uint32_t fb_base = f.fbp * 32u, z_base = f.zbp * 32u;
 	const size_t P = size_t(4) << 20;
-
-	uint64_t drawn = 0;
-	for (uint32_t i = 0; i < n; i++)
-		drawn += live[i] ? 1u : 0u;
 
 	// Coarse tile size (gs/gs_interface.cpp:148-187), in the reference's 32-bit arithmetic.
 	const uint32_t w8 = uint32_t((bb[2] - bb[0]) >> 3) + 1u, h8 = uint32_t((bb[3] - bb[1]) >> 3) + 1u;
@@ -1533,8 +1534,10 @@ uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t cont
 	const uint32_t cw = uint32_t((bb[2] - bb[0]) >> tl) + 1u, ch = uint32_t((bb[3] - bb[1]) >> tl) + 1u;
 	const int32_t tsize = 1 << tl;
 
-	auto rd32 = [&](size_t a) { uint32_t d; memcpy(&d, v0 + a, 4); return d; };
-	auto rd16 = [&](size_t a) { uint16_t d; memcpy(&d, v0 + a, 2); return uint32_t(d); };
+	auto rd32 = [&](size_t a) { uint32_t d; memcpy(&d, csrc + a, 4); return d; };
+	auto rd16 = [&](size_t a) { uint16_t d; memcpy(&d, csrc + a, 2); return uint32_t(d); };
+	auto zrd32 = [&](size_t a) { uint32_t d; memcpy(&d, zsrc + a, 4); return d; };
+	auto zrd16 = [&](size_t a) { uint16_t d; memcpy(&d, zsrc + a, 2); return uint32_t(d); };
 	auto wr32 = [&](size_t a, uint32_t d) { memcpy(v0 + a, &d, 4); };
 	auto wr16 = [&](size_t a, uint32_t d) { uint16_t h = uint16_t(d); memcpy(v0 + a, &h, 2); };
 
@@ -1622,16 +1625,16 @@ uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t cont
 						{
 							if (z16)
 							{
-								r.depth = rd16(za);
-								if (super && rd16(za + P) == 0xffffu)
-									r.depth = rd16(za + (2u + smp) * P);
+								r.depth = zrd16(za);
+								if (super && zrd16(za + P) == 0xffffu)
+									r.depth = zrd16(za + (2u + smp) * P);
 							}
 							else
 							{
 								const uint32_t mask = z24 ? 0xffffffu : ~0u;
-								r.depth = rd32(za) & mask;
-								if (super && (rd32(za + P) & mask) == mask)
-									r.depth = rd32(za + (2u + smp) * P) & mask;
+								r.depth = zrd32(za) & mask;
+								if (super && (zrd32(za + P) & mask) == mask)
+									r.depth = zrd32(za + (2u + smp) * P) & mask;
 							}
 						}
 					}
@@ -1639,7 +1642,7 @@ uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t cont
 					for (uint32_t i : list)
 					{
 						const Setup &s = setups[i];
-						if (!(px >= s.bb[0] && px <= s.bb[2] && py >= s.bb[1] && py <= s.bb[3]))
+						if (i < lo || i > hi || !(px >= s.bb[0] && px <= s.bb[2] && py >= s.bb[1] && py <= s.bb[3]))
 							continue;
 						for (uint32_t smp = 0; smp < ns; smp++)
 						{
@@ -1660,6 +1663,10 @@ uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t cont
 						cdirty |= regs[smp].cdirty;
 						zdirty |= zs && regs[smp].zdirty;
 					}
+					if (skip & 1u)
+						cdirty = false;
+					if (skip & 2u)
+						zdirty = false;
 					uint32_t color = regs[0].color, depth = regs[0].depth;
 					if (req_c)
 					{
@@ -1734,7 +1741,6 @@ uint64_t render_super_sampled_pass(uint8_t *v0, const int32_t *bb, uint32_t cont
 			}
 		}
 	}
-	return drawn;
 }
 }
 
@@ -2021,115 +2027,218 @@ void GS::render(const std::vector<Prim> &prims_in)
 		}
 	};
 
-	if (sample_log2 != 0)
-	{
-		primitives_drawn += render_super_sampled_pass(vram.data(), rp_bb, rate_y_log2(), prims, setups, live, f,
-		                                              rate_x, rate_y, shade);
-		return;
-	}
-
-	const uint32_t fb_base = f.fbp * 32u, z_base = f.zbp * 32u;
+	for (size_t i = 0; i < prims.size(); i++)
+		primitives_drawn += live[i] ? 1u : 0u;
 	uint8_t *v0 = vram.data();
-	std::unordered_map<uint32_t, Regs> cache;
-	std::vector<uint32_t> touch_order;
+	const size_t plane_bytes = size_t(4) << 20;
 
-	auto load = [&](int32_t x, int32_t y) -> Regs & {
-		uint32_t key = uint32_t(y) * 4096u + uint32_t(x);
-		Regs &r = cache[key];
-		if (!r.loaded)
+	// One shading launch over primitives lo..hi: colour read from csrc, depth from zsrc
+	// (copies of VRAM or VRAM itself), stores left out per skip (bit 0 colour, bit 1 depth).
+	auto run = [&](uint32_t lo, uint32_t hi, const uint8_t *csrc, const uint8_t *zsrc, uint32_t skip) {
+		if (sample_log2 != 0)
 		{
-			r.loaded = true;
-			touch_order.push_back(key);
-			uint32_t a = swizzle(uint32_t(x), uint32_t(y), fb_base, f.fbw, f.fb_psm);
-			if (psm16(f.fb_psm))
+			render_super_sampled_pass(v0, csrc, zsrc, lo, hi, skip, rp_bb, rate_y_log2(), prims, setups, live, f,
+			                          rate_x, rate_y, shade);
+			return;
+		}
+		const uint32_t fb_base = f.fbp * 32u, z_base = f.zbp * 32u;
+		std::unordered_map<uint32_t, Regs> cache;
+		std::vector<uint32_t> touch_order;
+
+		auto load = [&](int32_t x, int32_t y) -> Regs & {
+			uint32_t key = uint32_t(y) * 4096u + uint32_t(x);
+			Regs &r = cache[key];
+			if (!r.loaded)
 			{
-				uint16_t c16;
-				memcpy(&c16, v0 + a * 2u, 2);
-				r.color = rgba16_to_rgba32(c16, 0, 0, 0x80);
-			}
-			else
-				memcpy(&r.color, v0 + a * 4u, 4);
-			if (f.z_psm != 0xffffffffu)
-			{
-				uint32_t za = swizzle(uint32_t(x), uint32_t(y), z_base, f.fbw, f.z_psm);
-				if (psm16(f.z_psm))
+				r.loaded = true;
+				touch_order.push_back(key);
+				uint32_t a = swizzle(uint32_t(x), uint32_t(y), fb_base, f.fbw, f.fb_psm);
+				if (psm16(f.fb_psm))
 				{
-					uint16_t d16;
-					memcpy(&d16, v0 + za * 2u, 2);
-					r.depth = d16;
+					uint16_t c16;
+					memcpy(&c16, csrc + a * 2u, 2);
+					r.color = rgba16_to_rgba32(c16, 0, 0, 0x80);
 				}
 				else
+					memcpy(&r.color, csrc + a * 4u, 4);
+				if (f.z_psm != 0xffffffffu)
 				{
-					memcpy(&r.depth, v0 + za * 4u, 4);
-					if (psm24(f.z_psm))
-						r.depth &= 0xffffffu;
+					uint32_t za = swizzle(uint32_t(x), uint32_t(y), z_base, f.fbw, f.z_psm);
+					if (psm16(f.z_psm))
+					{
+						uint16_t d16;
+						memcpy(&d16, zsrc + za * 2u, 2);
+						r.depth = d16;
+					}
+					else
+					{
+						memcpy(&r.depth, zsrc + za * 4u, 4);
+						if (psm24(f.z_psm))
+							r.depth &= 0xffffffu;
+					}
 				}
 			}
+			return r;
+		};
+
+		for (size_t pi = lo; pi <= hi && pi < prims.size(); pi++)
+		{
+			if (!live[pi])
+				continue;
+			const Prim &p = prims[pi];
+			const Setup &s = setups[pi];
+			for (int32_t y = s.bb[1]; y <= s.bb[3]; y++)
+				for (int32_t x = s.bb[0]; x <= s.bb[2]; x++)
+					shade(p, s, x, y, 0, false, [&]() -> Regs & { return load(x, y); });
 		}
-		return r;
+
+		// Write-back in 8x8 tile order: colour then depth per tile.
+		std::sort(touch_order.begin(), touch_order.end(), [](uint32_t a, uint32_t b) {
+			uint32_t ax = a % 4096u, ay = a / 4096u, bx = b % 4096u, by = b / 4096u;
+			uint32_t ta = (ay >> 3) * 512u + (ax >> 3), tb = (by >> 3) * 512u + (bx >> 3);
+			if (ta != tb) return ta < tb;
+			return a < b;
+		});
+		size_t t0 = 0;
+		while (t0 < touch_order.size())
+		{
+			size_t t1 = t0;
+			auto tile = [](uint32_t k) { return ((k / 4096u) >> 3) * 512u + ((k % 4096u) >> 3); };
+			while (t1 < touch_order.size() && tile(touch_order[t1]) == tile(touch_order[t0]))
+				t1++;
+			for (size_t t = t0; t < t1 && !(skip & 1u); t++)
+			{
+				uint32_t key = touch_order[t];
+				Regs &r = cache[key];
+				if (!r.cdirty)
+					continue;
+				uint32_t x = key % 4096u, y = key / 4096u;
+				uint32_t a = swizzle(x, y, fb_base, f.fbw, f.fb_psm);
+				if (psm16(f.fb_psm))
+				{
+					uint16_t c16 = uint16_t(rgba32_to_rgba16(r.color));
+					memcpy(v0 + a * 2u, &c16, 2);
+				}
+				else
+					memcpy(v0 + a * 4u, &r.color, 4);
+			}
+			for (size_t t = t0; t < t1 && !(skip & 2u); t++)
+			{
+				uint32_t key = touch_order[t];
+				Regs &r = cache[key];
+				if (!r.zdirty)
+					continue;
+				uint32_t x = key % 4096u, y = key / 4096u;
+				uint32_t a = swizzle(x, y, z_base, f.fbw, f.z_psm);
+				if (psm16(f.z_psm))
+				{
+					uint16_t d16 = uint16_t(r.depth);
+					memcpy(v0 + a * 2u, &d16, 2);
+				}
+				else if (psm24(f.z_psm))
+					memcpy(v0 + a * 4u, &r.depth, 3);
+				else
+					memcpy(v0 + a * 4u, &r.depth, 4);
+			}
+			t0 = t1;
+		}
 	};
 
-	for (size_t pi = 0; pi < prims.size(); pi++)
+	const uint32_t n = uint32_t(prims.size());
+	// Colour and depth in one buffer (gs/gs_renderer.cpp:2525-2690): a primitive's colour may
+	// land on another pixel's depth, so the pass shades in launches whose reads do not see
+	// their own writes:
+	// * primitives before the first depth-sensitive one together (depth only read by on-tile
+	//   depth feedback, from a copy);
+	// * with depth writes, each primitive from the first depth-sensitive one to the last depth
+	//   writer alone, reading a copy of VRAM taken before it, all colour stored, then all depth;
+	// * later depth-testing primitives in runs reading depth from a copy taken as the run
+	//   starts, a run ending before a depth-sensitive sprite that overlaps the run's earlier
+	//   sprites (gs/gs_renderer.cpp:2418-2475);
+	// * with no later depth test, the rest without depth.
+	if (!(f.z_psm != 0xffffffffu && f.fbp == f.zbp))
 	{
-		if (!live[pi])
-			continue;
-		const Prim &p = prims[pi];
-		const Setup &s = setups[pi];
-		primitives_drawn++;
-		for (int32_t y = s.bb[1]; y <= s.bb[3]; y++)
-			for (int32_t x = s.bb[0]; x <= s.bb[2]; x++)
-				shade(p, s, x, y, 0, false, [&]() -> Regs & { return load(x, y); });
+		run(0, n - 1u, v0, v0, 0);
+		return;
 	}
-
-	// Write-back in 8x8 tile order: colour then depth per tile.
-	std::sort(touch_order.begin(), touch_order.end(), [](uint32_t a, uint32_t b) {
-		uint32_t ax = a % 4096u, ay = a / 4096u, bx = b % 4096u, by = b / 4096u;
-		uint32_t ta = (ay >> 3) * 512u + (ax >> 3), tb = (by >> 3) * 512u + (bx >> 3);
-		if (ta != tb) return ta < tb;
-		return a < b;
-	});
-	size_t t0 = 0;
-	while (t0 < touch_order.size())
+	// Copy of the planes the launch reads (plane 0 only for single-sampled shading).
+	std::vector<uint8_t> copy;
+	auto take_copy = [&]() -> const uint8_t * {
+		const size_t bytes = (sample_log2 != 0 && rate_y != 0) ? vram.size() : plane_bytes;
+		copy.assign(vram.begin(), vram.begin() + ptrdiff_t(bytes));
+		return copy.data();
+	};
+	uint32_t first_z = ~0u, last_zwrite = 0, last_z = 0;
+	bool single_step = false;
+	for (uint32_t i = 0; i < n; i++)
 	{
-		size_t t1 = t0;
-		auto tile = [](uint32_t k) { return ((k / 4096u) >> 3) * 512u + ((k % 4096u) >> 3); };
-		while (t1 < touch_order.size() && tile(touch_order[t1]) == tile(touch_order[t0]))
-			t1++;
-		for (size_t t = t0; t < t1; t++)
+		const uint32_t st = prims[i].state;
+		if (st & S_Z_WRITE)
 		{
-			uint32_t key = touch_order[t];
-			Regs &r = cache[key];
-			if (!r.cdirty)
+			single_step = true;
+			last_zwrite = i;
+		}
+		if (st & (S_Z_TEST | S_Z_WRITE))
+		{
+			if (first_z == ~0u)
+				first_z = i;
+			last_z = i;
+		}
+	}
+	if (first_z == ~0u)
+	{
+		run(0, n - 1u, v0, v0, 0);
+		return;
+	}
+	auto cached_runs = [&](uint32_t lo) {
+		int32_t bb[4] = { INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN };
+		for (uint32_t i = lo; i < n; i++)
+		{
+			const Prim &p = prims[i];
+			if (!(p.state & S_SPRITE))
 				continue;
-			uint32_t x = key % 4096u, y = key / 4096u;
-			uint32_t a = swizzle(x, y, fb_base, f.fbw, f.fb_psm);
-			if (psm16(f.fb_psm))
+			const bool overlap = std::max(bb[0], p.bb[0]) <= std::min(bb[2], p.bb[2]) &&
+			                     std::max(bb[1], p.bb[1]) <= std::min(bb[3], p.bb[3]) &&
+			                     (p.state & (S_Z_TEST | S_Z_WRITE)) != 0;
+			if (overlap)
 			{
-				uint16_t c16 = uint16_t(rgba32_to_rgba16(r.color));
-				memcpy(v0 + a * 2u, &c16, 2);
+				run(lo, i - 1u, v0, take_copy(), 0);
+				lo = i;
+				memcpy(bb, p.bb, sizeof(bb));
 			}
 			else
-				memcpy(v0 + a * 4u, &r.color, 4);
-		}
-		for (size_t t = t0; t < t1; t++)
-		{
-			uint32_t key = touch_order[t];
-			Regs &r = cache[key];
-			if (!r.zdirty)
-				continue;
-			uint32_t x = key % 4096u, y = key / 4096u;
-			uint32_t a = swizzle(x, y, z_base, f.fbw, f.z_psm);
-			if (psm16(f.z_psm))
 			{
-				uint16_t d16 = uint16_t(r.depth);
-				memcpy(v0 + a * 2u, &d16, 2);
+				bb[0] = std::min(bb[0], p.bb[0]);
+				bb[1] = std::min(bb[1], p.bb[1]);
+				bb[2] = std::max(bb[2], p.bb[2]);
+				bb[3] = std::max(bb[3], p.bb[3]);
 			}
-			else if (psm24(f.z_psm))
-				memcpy(v0 + a * 4u, &r.depth, 3);
-			else
-				memcpy(v0 + a * 4u, &r.depth, 4);
 		}
-		t0 = t1;
+		if (lo < n)
+			run(lo, n - 1u, v0, take_copy(), 0);
+	};
+	if (first_z != 0)
+		run(0, first_z - 1u, v0, take_copy(), 0);
+	if (!single_step)
+	{
+		cached_runs(first_z);
+		return;
+	}
+	for (uint32_t i = first_z; i <= last_zwrite; i++)
+	{
+		const uint8_t *c = take_copy();
+		run(i, i, c, c, 2u);
+		run(i, i, c, c, 1u);
+	}
+	if (last_zwrite + 1u < n)
+	{
+		if (last_z > last_zwrite)
+			cached_runs(last_zwrite + 1u);
+		else
+		{
+			f.z_psm = 0xffffffffu;
+			run(last_zwrite + 1u, n - 1u, v0, v0, 0);
+		}
 	}
 }
 

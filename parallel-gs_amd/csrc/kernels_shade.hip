@@ -811,27 +811,31 @@ __device__ __forceinline__ void shade_1x_block(const pgs_shade_args &a, uint32_t
 	pi.z24 = pi.zsens && (a.z_psm == PSMCT24 || a.z_psm == PSMZ24);
 	pi.zmax = pi.z16 ? 0xffffu : pi.z24 ? 0xffffffu : 0xffffffffu;
 
-	// begin_tile
+	// begin_tile. Colour and depth may come from copies of VRAM (a pass whose depth aliases
+	// its colour buffer).
+	const uint8_t *cv = a.src ? a.src : a.vram;
+	const uint8_t *zv = a.zsrc ? a.zsrc : cv;
 	const uint32_t caddr = swizzle_addr(uint32_t(px), uint32_t(py), a.fbp * 32u, a.fbw, a.fb_psm);
 	uint32_t color;
 	if (pi.fb16)
-		color = rgba16_to_rgba32(*reinterpret_cast<const uint16_t *>(a.vram + caddr));
+		color = rgba16_to_rgba32(*reinterpret_cast<const uint16_t *>(cv + caddr));
 	else
-		color = *reinterpret_cast<const uint32_t *>(a.vram + caddr);
+		color = *reinterpret_cast<const uint32_t *>(cv + caddr);
 	uint32_t zaddr = 0, depth = 0;
 	if (pi.zsens)
 	{
 		zaddr = swizzle_addr(uint32_t(px), uint32_t(py), a.zbp * 32u, a.fbw, a.z_psm);
 		if (pi.z16)
-			depth = *reinterpret_cast<const uint16_t *>(a.vram + zaddr);
+			depth = *reinterpret_cast<const uint16_t *>(zv + zaddr);
 		else
 		{
-			depth = *reinterpret_cast<const uint32_t *>(a.vram + zaddr);
+			depth = *reinterpret_cast<const uint32_t *>(zv + zaddr);
 			if (pi.z24)
 				depth &= 0xffffffu;
 		}
 	}
 	bool cdirty = false, zdirty = false;
+	const uint32_t prim_first = a.prim_first, prim_span = a.prim_count ? a.prim_count : ~0u;
 
 	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
 	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
@@ -844,7 +848,8 @@ __device__ __forceinline__ void shade_1x_block(const pgs_shade_args &a, uint32_t
 		{
 			idx = list[li];
 			const uint32_t st = prims[idx].state;
-			hit = intersects_block(setups[idx], blox, bloy, blox + 7, bloy + 7, (st & ST_MULTISAMPLE) && !(st & ST_LINE));
+			hit = idx - prim_first < prim_span &&
+			      intersects_block(setups[idx], blox, bloy, blox + 7, bloy + 7, (st & ST_MULTISAMPLE) && !(st & ST_LINE));
 		}
 		uint64_t ballot = __ballot(hit);
 		while (ballot)
@@ -857,6 +862,10 @@ __device__ __forceinline__ void shade_1x_block(const pgs_shade_args &a, uint32_t
 	}
 
 	// end_tile. In a super-sampled context the reference plane marks the pixel single-sampled.
+	if (a.skip_stores & 1u)
+		cdirty = false;
+	if (a.skip_stores & 2u)
+		zdirty = false;
 	if (cdirty)
 	{
 		if (pi.fb16)
@@ -1011,26 +1020,29 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 	pi.zmax = pi.z16 ? 0xffffu : pi.z24 ? 0xffffffu : 0xffffffffu;
 
 	// begin_tile: samples come from their planes where the reference plane marks the pixel; a
-	// single-sampled tile reads plane 0 only.
+	// single-sampled tile reads plane 0 only. Colour and depth may come from copies of VRAM
+	// with the same planes (a pass whose depth aliases its colour buffer).
 	uint8_t *v = a.vram;
+	const uint8_t *cv = a.src ? a.src : v;
+	const uint8_t *zv = a.zsrc ? a.zsrc : cv;
 	const size_t P = kVramSize;
 	const uint32_t caddr = swizzle_addr(uint32_t(px), uint32_t(py), a.fbp * 32u, a.fbw, a.fb_psm);
 	uint32_t color;
 	if (pi.fb16)
 	{
-		uint32_t c = *reinterpret_cast<const uint16_t *>(v + caddr);
-		if (!single && *reinterpret_cast<const uint16_t *>(v + P + caddr) == 0xffffu)
-			c = *reinterpret_cast<const uint16_t *>(v + (2u + smp) * P + caddr);
+		uint32_t c = *reinterpret_cast<const uint16_t *>(cv + caddr);
+		if (!single && *reinterpret_cast<const uint16_t *>(cv + P + caddr) == 0xffffu)
+			c = *reinterpret_cast<const uint16_t *>(cv + (2u + smp) * P + caddr);
 		color = rgba16_to_rgba32(c);
 	}
 	else
 	{
-		color = *reinterpret_cast<const uint32_t *>(v + caddr);
-		const uint32_t ref = single ? 0u : *reinterpret_cast<const uint32_t *>(v + P + caddr);
+		color = *reinterpret_cast<const uint32_t *>(cv + caddr);
+		const uint32_t ref = single ? 0u : *reinterpret_cast<const uint32_t *>(cv + P + caddr);
 		const uint32_t mask = pi.fb24 ? 0xffffffu : ~0u;
 		if ((ref & mask) == mask)
 		{
-			const uint32_t sc = *reinterpret_cast<const uint32_t *>(v + (2u + smp) * P + caddr);
+			const uint32_t sc = *reinterpret_cast<const uint32_t *>(cv + (2u + smp) * P + caddr);
 			// A 24-bit frame keeps the single-sampled alpha unless the whole word is marked.
 			color = (pi.fb24 && ref != ~0u) ? ((color & 0xff000000u) | (sc & 0xffffffu)) : sc;
 		}
@@ -1041,19 +1053,20 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 		zaddr = swizzle_addr(uint32_t(px), uint32_t(py), a.zbp * 32u, a.fbw, a.z_psm);
 		if (pi.z16)
 		{
-			depth = *reinterpret_cast<const uint16_t *>(v + zaddr);
-			if (!single && *reinterpret_cast<const uint16_t *>(v + P + zaddr) == 0xffffu)
-				depth = *reinterpret_cast<const uint16_t *>(v + (2u + smp) * P + zaddr);
+			depth = *reinterpret_cast<const uint16_t *>(zv + zaddr);
+			if (!single && *reinterpret_cast<const uint16_t *>(zv + P + zaddr) == 0xffffu)
+				depth = *reinterpret_cast<const uint16_t *>(zv + (2u + smp) * P + zaddr);
 		}
 		else
 		{
 			const uint32_t mask = pi.z24 ? 0xffffffu : ~0u;
-			depth = *reinterpret_cast<const uint32_t *>(v + zaddr) & mask;
-			if (!single && (*reinterpret_cast<const uint32_t *>(v + P + zaddr) & mask) == mask)
-				depth = *reinterpret_cast<const uint32_t *>(v + (2u + smp) * P + zaddr) & mask;
+			depth = *reinterpret_cast<const uint32_t *>(zv + zaddr) & mask;
+			if (!single && (*reinterpret_cast<const uint32_t *>(zv + P + zaddr) & mask) == mask)
+				depth = *reinterpret_cast<const uint32_t *>(zv + (2u + smp) * P + zaddr) & mask;
 		}
 	}
 	bool cdirty = false, zdirty = false;
+	const uint32_t prim_first = a.prim_first, prim_span = a.prim_count ? a.prim_count : ~0u;
 
 	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
 	const PrimitiveAttribute *prims = static_cast<const PrimitiveAttribute *>(a.prims);
@@ -1066,7 +1079,8 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 		{
 			idx = list[li];
 			const uint32_t st = prims[idx].state;
-			hit = intersects_block(setups[idx], rox, roy, rhx, rhy, (st & ST_MULTISAMPLE) && !(st & ST_LINE));
+			hit = idx - prim_first < prim_span &&
+			      intersects_block(setups[idx], rox, roy, rhx, rhy, (st & ST_MULTISAMPLE) && !(st & ST_LINE));
 		}
 		uint64_t ballot = __ballot(hit);
 		while (ballot)
@@ -1087,8 +1101,8 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 	const uint32_t first_depth = uint32_t(__shfl(int(depth), int(group)));
 	const bool req_c = (__ballot(color != first_color) & gmask) != 0;
 	const bool req_d = pi.zsens && (__ballot(depth != first_depth) & gmask) != 0;
-	const bool cd = (__ballot(cdirty) & gmask) != 0;
-	const bool zd = pi.zsens && (__ballot(zdirty) & gmask) != 0;
+	const bool cd = !(a.skip_stores & 1u) && (__ballot(cdirty) & gmask) != 0;
+	const bool zd = pi.zsens && !(a.skip_stores & 2u) && (__ballot(zdirty) & gmask) != 0;
 
 	float f[4];
 	for (uint32_t k = 0; k < 4; k++)
@@ -1496,6 +1510,8 @@ static bool shade_args_valid(const pgs_shade_args *args)
 	if (args->rate_log2 != 0 && !args->tile_modes)
 		return false;
 	if (args->single_tiles > 1u || (args->single_tiles && args->rate_log2 == 0))
+		return false;
+	if (args->skip_stores > 3u)
 		return false;
 	return true;
 }

@@ -63,6 +63,7 @@ Renderer::~Renderer()
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
+	if (alias_cache_) (void)hipFree(alias_cache_);
 	if (fork_event_) (void)hipEventDestroy(fork_event_);
 	if (join_event_) (void)hipEventDestroy(join_event_);
 	if (side_) (void)hipStreamDestroy(side_);
@@ -705,7 +706,27 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	const size_t bytes_state = size_t(d.num_states) * sizeof(StateVector);
 	const size_t bytes_dec = size_t(d.num_textures) * sizeof(TexDecode);
 	const size_t bytes_tex = size_t(d.num_textures) * sizeof(TextureDesc);
-	const size_t need = bytes_pos + bytes_attr + bytes_prim + bytes_state + bytes_dec + bytes_tex + 6 * kAlign;
+	// A pass whose depth buffer is its colour buffer (gs/gs_renderer.cpp:2525-2542) shades in
+	// several launches; those that must not see colour written earlier in the same launch read
+	// depth (and a single-stepped primitive also colour) from a copy of the pass's pages. The
+	// copy's block list (every plane the shading kernel reads) goes up with the pass.
+	const bool fb_z_alias = d.z_sensitive && d.fbp == d.zbp;
+	const uint32_t cache_planes = super_pass ? num_planes_ : 1u;
+	std::vector<uint32_t> cache_pages;
+	if (fb_z_alias)
+	{
+		const PageSpan cs = page_span(d.fbp * 32u, d.fbw, d.fb_psm, uint32_t(d.bb[0]), uint32_t(d.bb[1]),
+		                              uint32_t(d.bb[2]), uint32_t(d.bb[3]));
+		const PageSpan zs = page_span(d.zbp * 32u, d.fbw, d.zbuf_psm, uint32_t(d.bb[0]), uint32_t(d.bb[1]),
+		                              uint32_t(d.bb[2]), uint32_t(d.bb[3]));
+		const uint32_t first = std::min(cs.first, zs.first), last = std::max(cs.last, zs.last);
+		const uint32_t pages = std::min(last - first + 1u, kNumPages);
+		for (uint32_t k = 0; k < pages; k++)
+			cache_pages.push_back((first + k) % kNumPages);
+	}
+	const size_t bytes_cache = cache_pages.size() * 32u * cache_planes * sizeof(uint32_t);
+	const size_t need = bytes_pos + bytes_attr + bytes_prim + bytes_state + bytes_dec + bytes_tex + bytes_cache +
+	                    7 * kAlign;
 	if (arena_.used + need > arena_.cap)
 		arena_make_room(need);
 	size_t off_pos, off_attr, off_prim, off_state;
@@ -718,6 +739,15 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	{
 		memcpy(arena_alloc(bytes_dec, off_dec), d.decodes, bytes_dec);
 		memcpy(arena_alloc(bytes_tex, off_tex), d.tex_descs, bytes_tex);
+	}
+	size_t off_cache = 0;
+	if (bytes_cache)
+	{
+		uint32_t *blocks = reinterpret_cast<uint32_t *>(arena_alloc(bytes_cache, off_cache));
+		for (uint32_t plane = 0; plane < cache_planes; plane++)
+			for (uint32_t page : cache_pages)
+				for (uint32_t b = 0; b < 32u; b++)
+					*blocks++ = plane * kNumBlocks + page * 32u + b;
 	}
 	arena_upload();
 
@@ -867,22 +897,149 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	a.rate_x_log2 = 0;
 	a.tile_modes = super_pass ? tile_modes_ : nullptr;
 	a.has_ref_plane = ssaa_context ? 1u : 0u;
-	if (!super_pass)
-		launch(lsh);
-	else
+	if (super_pass)
 	{
 		// One launch for both kinds of tile: the first sample wave of each block shades the
 		// single-sampled tiles.
-		Launch lss = lsh;
-		lss.kind = LaunchKind::ShadeSuper;
-		lss.shade.rate_log2 = rate_y;
-		lss.shade.rate_x_log2 = rate_x;
-		lss.shade.single_tiles = 1;
-		launch(lss);
+		lsh.kind = LaunchKind::ShadeSuper;
+		a.rate_log2 = rate_y;
+		a.rate_x_log2 = rate_x;
+		a.single_tiles = 1;
 	}
+	if (!fb_z_alias)
+		launch(lsh);
+	else
+		shade_aliased_pass(d, lsh, arena_.dev + off_cache,
+		                   uint32_t(cache_pages.size()) * 32u * cache_planes);
 
 	stats_.render_passes++;
 	stats_.primitives += n;
+}
+
+// Shading of a pass whose depth buffer is its colour buffer (gs/gs_renderer.cpp:2525-2690).
+// A primitive's colour may land on another pixel's depth (the two formats swizzle
+// differently), so the order of the writes matters:
+// * primitives before the first depth-sensitive one shade together (depth unused);
+// * if any primitive writes depth, every primitive from the first depth-sensitive one to the
+//   last depth writer shades alone, reading the VRAM as it was before it (a copy), storing
+//   all colour first and then all depth;
+// * later primitives that only test depth shade in runs that read depth from a copy taken
+//   when the run starts; a run ends before a depth-sensitive sprite that overlaps the
+//   sprites already in the run (gs/gs_renderer.cpp:2418-2475), so the sprite sees their
+//   colour as depth;
+// * without any, the rest shades without depth.
+// Launches that read colour and write it to the same pixel only are race-free; the copies
+// make every other read independent of the launch's own writes.
+void Renderer::shade_aliased_pass(const RenderPassDesc &d, const Launch &base, const uint8_t *cache_blocks,
+                                  uint32_t num_cache_blocks)
+{
+	if (!alias_cache_)
+	{
+		const uint32_t capacity_planes = capacity_log2_ ? 2u + (1u << capacity_log2_) : 1u;
+		check(hipMalloc(&alias_cache_, size_t(capacity_planes) * kVramSize));
+	}
+	const uint32_t n = d.num_primitives;
+	auto refresh_cache = [&]() {
+		Launch lc{};
+		lc.kind = LaunchKind::VramSnapshot;
+		lc.snapshot = { vram_, alias_cache_, reinterpret_cast<const uint32_t *>(cache_blocks), num_cache_blocks, 0 };
+		launch(lc);
+	};
+	auto shade = [&](uint32_t first, uint32_t count, bool read_copy, bool depth_copy, uint32_t skip, bool no_depth) {
+		Launch l = base;
+		l.shade.prim_first = first;
+		l.shade.prim_count = count;
+		l.shade.src = read_copy ? alias_cache_ : nullptr;
+		l.shade.zsrc = depth_copy ? alias_cache_ : nullptr;
+		l.shade.skip_stores = skip;
+		if (no_depth)
+			l.shade.z_psm = 0xffffffffu;
+		launch(l);
+	};
+
+	uint32_t first_z = ~0u, last_zwrite = 0, last_z = 0;
+	bool single_step = false, depth_feedback_before = false;
+	for (uint32_t i = 0; i < n; i++)
+	{
+		const uint32_t st = d.prims[i].state;
+		if (st & ST_Z_WRITE)
+		{
+			single_step = true;
+			last_zwrite = i;
+		}
+		if (st & (ST_Z_TEST | ST_Z_WRITE))
+		{
+			if (first_z == ~0u)
+				first_z = i;
+			last_z = i;
+		}
+		else if (first_z == ~0u && (d.prims[i].tex & (TEX_FEEDBACK | TEX_FEEDBACK_DEPTH)) == (TEX_FEEDBACK | TEX_FEEDBACK_DEPTH))
+			depth_feedback_before = true;
+	}
+	if (first_z == ~0u)
+	{
+		launch(base);
+		return;
+	}
+
+	// Runs of primitives that read depth from a copy taken as the run starts.
+	auto cached_runs = [&](uint32_t lo) {
+		int32_t bb[4] = { INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN };
+		for (uint32_t i = lo; i < n; i++)
+		{
+			const PrimitiveAttribute &p = d.prims[i];
+			if (!(p.state & ST_SPRITE))
+				continue;
+			const bool overlap = std::max<int32_t>(bb[0], p.bb[0]) <= std::min<int32_t>(bb[2], p.bb[2]) &&
+			                     std::max<int32_t>(bb[1], p.bb[1]) <= std::min<int32_t>(bb[3], p.bb[3]) &&
+			                     (p.state & (ST_Z_TEST | ST_Z_WRITE)) != 0;
+			if (overlap)
+			{
+				refresh_cache();
+				shade(lo, i - lo, false, true, 0, false);
+				lo = i;
+				bb[0] = p.bb[0]; bb[1] = p.bb[1]; bb[2] = p.bb[2]; bb[3] = p.bb[3];
+			}
+			else
+			{
+				bb[0] = std::min<int32_t>(bb[0], p.bb[0]);
+				bb[1] = std::min<int32_t>(bb[1], p.bb[1]);
+				bb[2] = std::max<int32_t>(bb[2], p.bb[2]);
+				bb[3] = std::max<int32_t>(bb[3], p.bb[3]);
+			}
+		}
+		if (lo < n)
+		{
+			refresh_cache();
+			shade(lo, n - lo, false, true, 0, false);
+		}
+	};
+
+	if (first_z != 0)
+	{
+		// Depth is unused here except by on-tile depth feedback, which reads the copy.
+		if (depth_feedback_before)
+			refresh_cache();
+		shade(0, first_z, false, depth_feedback_before, 0, false);
+	}
+	if (!single_step)
+	{
+		cached_runs(first_z);
+		return;
+	}
+	for (uint32_t i = first_z; i <= last_zwrite; i++)
+	{
+		refresh_cache();
+		shade(i, 1, true, true, 2u, false); // colour
+		shade(i, 1, true, true, 1u, false); // then depth
+	}
+	if (last_zwrite + 1u < n)
+	{
+		if (last_z > last_zwrite)
+			cached_runs(last_zwrite + 1u);
+		else
+			shade(last_zwrite + 1u, n - last_zwrite - 1u, false, false, 0, true);
+	}
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1076,7 +1233,11 @@ void launch_resources(const Renderer::Launch &l, uint32_t &reads, uint32_t &writ
 	case K::SsaaHeuristic: reads = R_PRIMS | R_SETUP | R_FLAT; writes = R_PRIMS | R_FLAT; break;
 	case K::Binning: reads = R_SETUP | R_PRIMS; writes = R_BINS; break;
 	case K::Shade:
-	case K::ShadeSuper: reads = R_VRAM | R_CLUT | R_SETUP | R_PRIMS | R_BINS | R_TEXIMG; writes = R_VRAM; break;
+	case K::ShadeSuper:
+		// A shading launch of an aliased pass also reads the VRAM copy (a snapshot buffer).
+		reads = R_VRAM | R_CLUT | R_SETUP | R_PRIMS | R_BINS | R_TEXIMG | (l.shade.src || l.shade.zsrc ? R_SHADOW : 0u);
+		writes = R_VRAM;
+		break;
 	default: reads = writes = ~0u; break; // anything else orders against everything
 	}
 }

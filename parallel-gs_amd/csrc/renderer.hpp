@@ -230,6 +230,8 @@ private:
 	};
 	void plan_transfer_level(const std::vector<uint32_t> &members, TransferLevel &lv);
 	void emit_transfer_level(const TransferLevel &lv, uint32_t index, const TransferLevel *next);
+	void shade_aliased_pass(const RenderPassDesc &d, const Launch &base, const uint8_t *cache_blocks,
+	                        uint32_t num_cache_blocks);
 
 	uint32_t sample_log2_ = 0;
 	bool sparse_ = false;
@@ -238,6 +240,9 @@ private:
 	size_t snapshot_bytes_ = 0;
 	uint8_t *vram_ = nullptr;    // num_planes_ * 4 MiB
 	uint8_t *shadow_ = nullptr;  // 4 MiB
+	// Copy of VRAM (every plane of the context's capacity) read by passes whose depth buffer
+	// aliases their colour buffer; allocated by the first such pass.
+	uint8_t *alias_cache_ = nullptr;
 	hipStream_t stream_ = nullptr;
 	// Texture stream: VRAM copies, snapshots, CLUT loads and (without sampler feedback) texture
 	// decodes. Before one of them runs, the texture stream waits for the main stream when the

@@ -343,3 +343,54 @@ def random_fb_feedback_stream(seed: int, n_batches: int = 10, prims_per_batch: i
         for i in range(0, len(writes), 240):
             out.append(gif.ad(writes[i:i + 240]))
     return gif.arr(b"".join(out))
+
+
+def random_fbz_alias_stream(seed: int, n_batches: int = 10, prims_per_batch: int = 16) -> np.ndarray:
+    """Depth buffer at the colour buffer's base (FBP == ZBP == 0, FBW 2, 128 x 64 of random
+    contents): batches of random colour / depth formats whose primitives (sprites of every size,
+    flat and 3D triangles) switch between no depth test, test-only (ZMSK) and depth writes, with
+    random colours, depths, blending and alpha tests; some batches draw test-only sprites only
+    (runs that end at overlapping sprites)."""
+    rng = np.random.default_rng(seed)
+    ofs = 2048 * 16
+    fw, fh = 128, 64
+    out = [gif.ad([(0x1A, 1), (0x18, ofs | ofs << 32)])]
+    out.append(gif.upload(0, 2, 0x00, 0, 0, fw, fh, _random_payload(rng, fw * fh * 4)))
+    for b in range(n_batches):
+        fb_psm = int(rng.choice([0x00, 0x00, 0x01, 0x02, 0x0A]))
+        z_psm = int(rng.choice([0, 1, 2, 0xA] if fb_psm in (0x02, 0x0A) else [0, 0, 1, 2]))
+        zmax = 0xFFFF if z_psm in (2, 0xA) else 0xFFFFFF if z_psm == 1 else 0xFFFFFFFF
+        test_only = rng.integers(0, 4) == 0
+        out.append(gif.ad([(0x4C, frame(0, 2, fb_psm)), (0x40, scissor(0, fw - 1, 0, fh - 1)),
+                           (0x42, alpha_reg(0, 1, 0, 1, 0))]))
+        writes = []
+        for p in range(prims_per_batch):
+            mode = 1 if test_only else int(rng.choice([0, 1, 1, 2]))  # 0 no depth, 1 test only, 2 writes
+            ztst = 1 if mode == 0 else int(rng.integers(1 if mode == 2 else 2, 4))
+            ate = int(rng.integers(0, 4) == 0)
+            writes += [(0x4E, zbuf(0, z_psm, 0 if mode == 2 else 1)),
+                       (0x47, test_reg(ate=ate, atst=int(rng.integers(0, 8)), aref=int(rng.integers(0, 256)),
+                                       afail=int(rng.integers(0, 4)), zte=1, ztst=ztst))]
+            kind = 0 if test_only else int(rng.integers(0, 3))
+            abe = int(rng.integers(0, 3) == 0)
+            if kind == 0:
+                big = rng.integers(0, 3) == 0
+                w = int(rng.integers(8, fw)) if big else int(rng.integers(1, 24))
+                h = int(rng.integers(8, fh)) if big else int(rng.integers(1, 24))
+                x0, y0 = int(rng.integers(-4, fw - 2)), int(rng.integers(-4, fh - 2))
+                z = int(rng.integers(0, zmax + 1))
+                writes += [(0x00, prim_reg(6, iip=0, abe=abe)), (0x01, rgbaq(int(rng.integers(0, 2**32)))),
+                           (0x05, xyz2(ofs + x0 * 16, ofs + y0 * 16, z)),
+                           (0x05, xyz2(ofs + (x0 + w) * 16, ofs + (y0 + h) * 16, z))]
+            else:
+                flat = kind == 1
+                z = int(rng.integers(0, zmax + 1))
+                writes.append((0x00, prim_reg(3, iip=int(not flat), abe=abe)))
+                for v in range(3):
+                    writes.append((0x01, rgbaq(int(rng.integers(0, 2**32)))))
+                    writes.append((0x05, xyz2(ofs + int(rng.integers(-8 * 16, (fw + 8) * 16)),
+                                              ofs + int(rng.integers(-8 * 16, (fh + 8) * 16)),
+                                              z if flat else int(rng.integers(0, zmax + 1)))))
+        for i in range(0, len(writes), 240):
+            out.append(gif.ad(writes[i:i + 240]))
+    return gif.arr(b"".join(out))
