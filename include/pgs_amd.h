@@ -145,9 +145,10 @@ int pgs_set_priv_register_state(pgs_context *ctx, const void *in, size_t size);
  * Submits pending work, then switches the sample count (1, 2, 4, 8, 16) and grid, clearing
  * every super-sample plane. The rate may not exceed what the context allocated (its
  * `super_sampling`, or 16 with `dynamic_super_sampling`). ordered_grid = 0 selects the sparse
- * 1x4 (4x) and 2x8 (16x) grids (gs/gs_interface.cpp:84-113). Returns PGS_ERROR_UNSUPPORTED for
- * super-sampled textures, which this backend does not build, and PGS_ERROR_INVALID_ARGUMENT
- * for other rates; the rate is then unchanged. Not allowed while recording. */
+ * 1x4 (4x) and 2x8 (16x) grids (gs/gs_interface.cpp:84-113). super_sampled_textures lets sprites
+ * sample frame-buffer pages through textures with one layer per sample (off at 1x,
+ * gs/gs_interface.cpp:1593-1603). Returns PGS_ERROR_INVALID_ARGUMENT for other rates; the rate
+ * is then unchanged. Not allowed while recording. */
 int pgs_set_super_sampling_rate(pgs_context *ctx, uint32_t super_sampling, uint32_t ordered_grid,
                                 uint32_t super_sampled_textures);
 /* The current sample count (1, 2, 4, 8 or 16). */

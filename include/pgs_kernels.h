@@ -124,7 +124,10 @@ typedef struct pgs_setup_args
 	/* Optional: words zeroed by the launch (the sampler-feedback masks of the pass). */
 	uint32_t *clear_words;
 	uint32_t clear_count;
-	uint32_t pad;
+	/* Super-sampled textures (triangle_setup.comp:677-789): bit 0 set when a texture of the pass
+	 * is super-sampled (SUPER_SAMPLED_TEXTURES), bit 1 when the pass is a deduced blur
+	 * (gs/gs_renderer.cpp:1848-1885), bits 8..9 the x rate log2 of the sampling grid. */
+	uint32_t texture_flags;
 } pgs_setup_args;
 
 int pgs_k_triangle_setup(const pgs_setup_args *args, hipStream_t stream);
@@ -227,9 +230,22 @@ typedef struct pgs_shade_args
 	uint32_t prim_count;
 	uint32_t skip_stores;    /* bit 0: leave colour unstored, bit 1: leave depth unstored */
 	uint32_t pad_alias;
+	/* Super-sampled textures (ubershader.comp:1155-1212): the phase filter table of the context's
+	 * sampling grid (PGS_PHASE_LUT_WORDS words, pgs_phase_lut), read by primitives whose texture
+	 * word has TEX_PER_SAMPLE without TEX_SAMPLE_MAPPING / TEX_SAMPLE_RESOLVED; NULL when no
+	 * texture of the pass is super-sampled. */
+	const uint32_t *phase_lut;
 } pgs_shade_args;
 
 int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream);
+
+/* Phase filter table of super-sampled textures for a sampling grid of 2^rate_x x 2^rate_y
+ * samples (gs/gs_renderer.cpp:94-313, init_phase_lut): PGS_PHASE_LUT_WORDS words, two per entry,
+ * 64 x 64 entries for LINEAR magnification, then 32 x 32 for NEAREST. Entry word 0: sample ids
+ * of the four filter taps in bits 0..15, their signed 2-bit texel offsets (x, y) in bits
+ * 16..31; word 1: the four weights in unorm8, summing to 255. Host function. */
+#define PGS_PHASE_LUT_WORDS (2 * (64 * 64 + 32 * 32))
+void pgs_phase_lut(uint32_t rate_x, uint32_t rate_y, uint32_t *out);
 
 /* Work-queue shading of a multi-launch by coarse tile (batched renderer; no counterpart in
  * the reference): a grid of resident workgroups pops tiles (tile numbering: launch l owns

@@ -353,6 +353,10 @@ void GS::init_transfer()
 	c.w = uint32_t(r & 0xfff);
 	c.h = uint32_t((r >> 32) & 0xfff);
 
+	// Transfers leave their destination pages single-sampled (gs/page_tracker.cpp:255-265,
+	// 650-660); the draws before them marked their pages as they were drawn.
+	if ((xdir == 0 || xdir == 2) && c.w && c.h)
+		mark_pages(c.dbp, c.dbw, c.dpsm, c.dsax, c.dsay, c.dsax + c.w - 1u, c.dsay + c.h - 1u, false);
 	if (xdir == 2)
 	{
 		flush_render_pass();
@@ -679,6 +683,8 @@ void orc_host_write(void *g, size_t offset, const void *data, size_t size)
 		return;
 	gs->flush();
 	memcpy(gs->vram.data() + offset, data, size);
+	for (size_t page = offset / 8192u; page <= (offset + size - 1u) / 8192u; page++)
+		gs->may_super_sample[page % 512u] = 0; // gs/page_tracker.cpp:700-711
 	if (gs->sample_log2)
 	{
 		const size_t first = offset / 8192u * 8192u, last = ((offset + size - 1u) / 8192u + 1u) * 8192u;
@@ -688,6 +694,18 @@ void orc_host_write(void *g, size_t offset, const void *data, size_t size)
 // GSInterface::set_super_sampling_rate (gs/gs_interface.cpp:65-118): pending work is done at the
 // old rate, then every plane above plane 0 is cleared (gs/gs_renderer.cpp:315-342).
 void orc_set_sparse_grid(void *g, uint32_t sparse) { static_cast<orc::GS *>(g)->sparse_grid = sparse != 0; }
+void orc_phase_lut(uint32_t rate_x, uint32_t rate_y, uint32_t *out)
+{
+	std::vector<uint32_t> t;
+	orc::phase_lut(rate_x, rate_y, t);
+	std::copy(t.begin(), t.end(), out);
+}
+void orc_set_super_sampled_textures(void *g, uint32_t on)
+{
+	auto *gs = static_cast<orc::GS *>(g);
+	gs->flush();
+	gs->super_sampled_textures = on != 0;
+}
 void orc_set_sample_log2(void *g, uint32_t sample_log2)
 {
 	auto *gs = static_cast<orc::GS *>(g);

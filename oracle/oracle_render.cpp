@@ -37,6 +37,8 @@ enum : uint32_t
 {
 	T_MAG_LINEAR = 1u << 16, T_MIN_LINEAR = 1u << 17, T_MIP_LINEAR = 1u << 18, T_CLAMP_S = 1u << 19,
 	T_CLAMP_T = 1u << 20,
+	// data_structures.h:251-253
+	T_PER_SAMPLE = 1u << 24, T_SAMPLE_MAPPING = 1u << 25, T_SAMPLE_RESOLVED = 1u << 26,
 };
 enum : uint32_t
 {
@@ -758,6 +760,17 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 		}
 		else
 			p.tex2 = 1u;
+		// A super-sampled texture samples per sample; palette formats with REGION_REPEAT force
+		// one layer per sample (gs/gs_interface.cpp:1783-1790, 1824-1831).
+		if (rp_textures[p.tex_slot].samples)
+		{
+			const uint32_t psm = field(regs.tex0[ctx], 20, 6);
+			const bool palette = psm == T8 || psm == T4 || psm == T8H || psm == T4HL || psm == T4HH;
+			p.tex |= T_PER_SAMPLE;
+			if (palette && (wms == 3 || wmt == 3))
+				p.tex |= T_SAMPLE_MAPPING;
+			rp_arrays = true;
+		}
 	}
 	if (zte)
 	{
@@ -823,17 +836,29 @@ void GS::append(const Vertex *vin, uint32_t n, bool quad)
 		state |= S_FIX | S_SNAP_RASTER | S_SNAP_ATTRIBUTE;
 	p.state = state;
 
+	bool expands = rp_prims.empty();
 	if (rp_prims.empty())
 	{
 		memcpy(rp_bb, p.bb, sizeof(rp_bb));
 		rp_zwrite = false;
 	}
+	expands = expands || p.bb[0] < rp_bb[0] || p.bb[1] < rp_bb[1] || p.bb[2] > rp_bb[2] || p.bb[3] > rp_bb[3];
 	rp_bb[0] = std::min(rp_bb[0], p.bb[0]);
 	rp_bb[1] = std::min(rp_bb[1], p.bb[1]);
 	rp_bb[2] = std::max(rp_bb[2], p.bb[2]);
 	rp_bb[3] = std::max(rp_bb[3], p.bb[3]);
 	rp_zwrite |= (state & S_Z_WRITE) != 0;
 	rp_prims.push_back(p);
+	// A draw growing the pass's box marks the box's frame-buffer pages, and its depth pages once
+	// the pass writes depth, as possibly super-sampled (gs/gs_interface.cpp:2786-2816).
+	if (expands && rp_bb[2] >= rp_bb[0] && rp_bb[3] >= rp_bb[1])
+	{
+		const uint32_t x0 = uint32_t(rp_bb[0]), y0 = uint32_t(rp_bb[1]), x1 = uint32_t(rp_bb[2]), y1 = uint32_t(rp_bb[3]);
+		mark_pages(field(rp_frame, 0, 9) * 32u, field(rp_frame, 16, 6), field(rp_frame, 24, 6), x0, y0, x1, y1, true);
+		if (rp_zwrite)
+			mark_pages(field(rp_zbuf, 0, 9) * 32u, field(rp_frame, 16, 6), 0x30u | field(rp_zbuf, 24, 4), x0, y0, x1,
+			           y1, true);
+	}
 	last_parallelogram_candidate = candidate;
 	state_dirty = false;
 	if (rp_prims.size() >= 64u * 1024u)
@@ -853,6 +878,7 @@ void GS::flush_render_pass()
 		render(rp_prims);
 		rp_prims.clear();
 		rp_textures.clear();
+		rp_arrays = false;
 		render_passes++;
 	}
 	state_dirty = true;
@@ -887,7 +913,8 @@ bool order_less(int32_t ax, int32_t ay, int32_t bx, int32_t by, uint32_t az, uin
 int32_t quantize64(int64_t c) { return int32_t(uint32_t(uint64_t(c) >> 1)); }
 
 // Returns false for a degenerate primitive (no coverage).
-bool setup_primitive(Prim &p, Setup &s, uint32_t rate_log2, uint32_t opaque_fbmask)
+bool setup_primitive(Prim &p, Setup &s, uint32_t rate_log2, uint32_t opaque_fbmask, bool arrays = false,
+                     uint32_t rate_x = 0, bool blur = false)
 {
 	const uint32_t st = p.state;
 	const bool parallelogram = st & S_PARALLELOGRAM, perspective = st & S_PERSPECTIVE, iip = st & S_IIP;
@@ -1159,6 +1186,60 @@ bool setup_primitive(Prim &p, Setup &s, uint32_t rate_log2, uint32_t opaque_fbma
 		bool half_texel = (ns_u || cs_u) && (ns_v || cs_v);
 		if (half_texel && no_interp)
 			p.tex &= ~T_MAG_LINEAR;
+	}
+
+	// Per-sample texturing only where filtering does not break it (triangle_setup.comp:677-789,
+	// FORCE_LINEAR_SUPER_SAMPLE off).
+	if (rate_log2 != 0 && arrays && (p.tex & (T_PER_SAMPLE | T_SAMPLE_MAPPING)) == T_PER_SAMPLE)
+	{
+		const int32_t du = std::abs(int32_t(a1.u) - int32_t(a0.u)), dv = std::abs(int32_t(a1.v) - int32_t(a0.v));
+		const int32_t distx = std::abs(dx - ax), disty = std::abs(dy - ay);
+		if (!(p.tex & T_MAG_LINEAR))
+		{
+			if (perspective)
+				p.tex |= T_SAMPLE_MAPPING;
+			else if (du > distx + 1 || dv > disty + 1)
+				p.tex &= ~T_PER_SAMPLE;
+			else if (du >= distx - 16 || dv >= disty - 16)
+			{
+				const uint32_t bm = p.blend_mode;
+				const bool blur_kernel = blur && (bm & B_ABE) && !(p.state & S_Z_TEST) && ((bm >> 11) & 3u) == 0u &&
+				                         ((bm >> 13) & 3u) == 2u && ((bm >> 17) & 3u) == 1u;
+				if (blur_kernel)
+					p.tex &= ~T_PER_SAMPLE;
+				else
+					p.tex |= T_SAMPLE_MAPPING;
+			}
+		}
+		else if (!perspective)
+		{
+			const bool down = du > distx + 1 && dv > disty + 1;
+			const bool up = distx >= 2 * du && disty >= 2 * dv;
+			if (down)
+				p.tex &= ~T_PER_SAMPLE;
+			else if (up)
+				p.tex |= T_SAMPLE_RESOLVED;
+		}
+		if ((p.tex & (T_PER_SAMPLE | T_SAMPLE_MAPPING)) == T_PER_SAMPLE)
+		{
+			// get_average_sampling_offset (utils.h:108-125) times 2^3, on both axes.
+			float avg;
+			if (rate_x == rate_log2)
+				avg = 0.5f * float((1 << rate_log2) - 1) / float(1 << rate_log2);
+			else if (rate_log2 == 1)
+				avg = 0.25f;
+			else if (rate_x == 1 && rate_log2 == 2)
+				avg = (0.125f + 0.625f) * 0.5f;
+			else if (rate_log2 == 2)
+				avg = 6.0f / 16.0f;
+			else
+				avg = (6.0f + 22.0f) / 64.0f;
+			const float corner = 8.0f * avg;
+			p.state &= ~S_SNAP_ATTRIBUTE;
+			const float di_dc = float(s.b[0] + s.b[1]) * corner, dj_dc = float(s.c[0] + s.c[1]) * corner;
+			s.error_i = s.error_i - di_dc * s.inv_area;
+			s.error_j = s.error_j - dj_dc * s.inv_area;
+		}
 	}
 
 	if (p.fbmsk != opaque_fbmask && (p.state & S_OPAQUE))
@@ -1563,6 +1644,8 @@ void render_super_sampled_pass(uint8_t *v0, const uint8_t *csrc, const uint8_t *
 					req_zw = true;
 				if ((st & (S_SNAP_RASTER | S_OPAQUE)) != (S_SNAP_RASTER | S_OPAQUE))
 					req = true;
+				if (prims[i].tex & T_PER_SAMPLE) // binning.comp:123-124
+					req = true;
 				if (st & S_Z_WRITE)
 					has_zw = true;
 			}
@@ -1650,7 +1733,7 @@ void render_super_sampled_pass(uint8_t *v0, const uint8_t *csrc, const uint8_t *
 							if (super)
 								sample_offset(rate_x, rate_y, smp, dx, dy);
 							const int32_t hx = (px << rate) + dx, hy = (py << rate) + dy;
-							shade(prims[i], s, hx, hy, rate, super, [&]() -> Regs & { return regs[smp]; });
+							shade(prims[i], s, hx, hy, rate, super, smp, [&]() -> Regs & { return regs[smp]; });
 						}
 					}
 
@@ -1746,7 +1829,7 @@ void render_super_sampled_pass(uint8_t *v0, const uint8_t *csrc, const uint8_t *
 
 // Texture sampling lives in oracle_texture.cpp.
 void sample_texture(const TextureSlot &t, uint32_t tex, uint32_t tex2, float u, float v, float q, bool perspective,
-                    float out[4]);
+                    const uint32_t *lut, uint32_t smp, float shift, float out[4]);
 
 void GS::render(const std::vector<Prim> &prims_in)
 {
@@ -1790,22 +1873,76 @@ void GS::render(const std::vector<Prim> &prims_in)
 		bool z_feedback = false;
 		for (const Prim &q : prims)
 			z_feedback |= q.feedback == 2;
-		if (channel_slice || z_feedback || aliased)
+		// A pass sampling super-sampled textures keeps its rate (gs/gs_interface.cpp:224-226).
+		if (!rp_arrays && (channel_slice || z_feedback || aliased))
 			rate_x = rate_y = 0;
+	}
+
+	// A deduced blur (gs/gs_renderer.cpp:1848-1885): at most 64 primitives, one super-sampled
+	// texture read at two phases (first vertex UV less position), none perspective.
+	bool blur = false;
+	if (rp_arrays && prims.size() <= 64u)
+	{
+		uint32_t last = ~0u;
+		int32_t ph_u = 0, ph_v = 0;
+		for (const Prim &p : prims)
+		{
+			if (!(p.tex & T_PER_SAMPLE))
+				continue;
+			if (p.state & S_PERSPECTIVE)
+			{
+				blur = false;
+				break;
+			}
+			const int32_t pu = int32_t(p.v[0].u) - p.v[0].x, pv = int32_t(p.v[0].v) - p.v[0].y;
+			if (p.tex_slot == last && (pu != ph_u || pv != ph_v))
+			{
+				blur = true;
+				break;
+			}
+			if (p.tex_slot != last)
+			{
+				ph_u = pu;
+				ph_v = pv;
+			}
+			last = p.tex_slot;
+		}
+	}
+	static std::vector<uint32_t> luts[12];
+	const uint32_t *lut = nullptr;
+	if (rp_arrays && rate_y)
+	{
+		std::vector<uint32_t> &l = luts[rate_y * 3 + rate_x];
+		if (l.empty())
+			phase_lut(rate_x, rate_y, l);
+		lut = l.data();
 	}
 
 	std::vector<Setup> setups(prims.size());
 	std::vector<bool> live(prims.size());
 	for (size_t i = 0; i < prims.size(); i++)
-		live[i] = setup_primitive(prims[i], setups[i], rate_y, opaque_fbmask);
+		live[i] = setup_primitive(prims[i], setups[i], rate_y, opaque_fbmask, rp_arrays, rate_x, blur);
 	if (rate_y != 0)
 		single_sample_heuristic(prims, setups, z_sensitive ? (field(rp_zbuf, 24, 4) | 0x30u) : uint32_t(Z24));
 
 	const int32_t snap_mask = -(1 << rate_y);
 
 	// One primitive at one sample: hi-res coordinate (hx, hy) of pixel (hx >> rate, hy >> rate).
+	// 0.5 less the shading grid's mean sample offset (ubershader.comp:1167; 0.5 at one sample).
+	float phase_shift = 0.5f;
+	if (rate_y)
+	{
+		if (rate_x == rate_y)
+			phase_shift = 0.5f - 0.5f * float((1 << rate_y) - 1) / float(1 << rate_y);
+		else if (rate_y == 1)
+			phase_shift = 0.25f;
+		else if (rate_y == 2)
+			phase_shift = 0.5f - 6.0f / 16.0f;
+		else
+			phase_shift = 0.5f - 28.0f / 64.0f;
+	}
 	auto shade = [&](const Prim &p, const Setup &s, int32_t hx, int32_t hy, uint32_t rate, bool super_sampled,
-	                 auto &&get_regs) {
+	                 uint32_t smp, auto &&get_regs) {
 		const uint32_t st = p.state;
 		const bool z_test = f.z_psm != 0xffffffffu && (st & S_Z_TEST);
 		bool z_write = f.z_psm != 0xffffffffu && (st & S_Z_WRITE);
@@ -1888,7 +2025,8 @@ void GS::render(const std::vector<Prim> &prims_in)
 			if (p.feedback)
 				feedback_texture(p, p.feedback == 2 ? r.depth : r.color, tx);
 			else
-				sample_texture(rp_textures[p.tex_slot], p.tex, p.tex2, u, vv, sq[2], persp, tx);
+				sample_texture(rp_textures[p.tex_slot], p.tex, p.tex2, u, vv, sq[2], persp, lut, smp,
+				               super_sampled ? phase_shift : 0.5f, tx);
 			const bool tcc = p.combiner & C_TCC;
 			switch (p.combiner & 3u)
 			{
@@ -2089,7 +2227,7 @@ void GS::render(const std::vector<Prim> &prims_in)
 			const Setup &s = setups[pi];
 			for (int32_t y = s.bb[1]; y <= s.bb[3]; y++)
 				for (int32_t x = s.bb[0]; x <= s.bb[2]; x++)
-					shade(p, s, x, y, 0, false, [&]() -> Regs & { return load(x, y); });
+					shade(p, s, x, y, 0, false, 0u, [&]() -> Regs & { return load(x, y); });
 		}
 
 		// Write-back in 8x8 tile order: colour then depth per tile.

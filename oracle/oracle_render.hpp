@@ -79,6 +79,9 @@ struct TextureSlot
 	float size[4];              // 2^TW, 2^TH, 1 / width, 1 / height
 	float bias[2];              // -rect origin / rectangle size
 	float region[4];            // clamp region: min u, min v, max u, max v
+	// Super-sampled texture (gs/gs_interface.cpp:1593-1603): samples per pixel S, else 0; rgba
+	// then holds S + 1 images: the single-sampled one, then one per sample.
+	uint32_t samples = 0;
 };
 
 // Inclusive interval of 8 KiB pages touched by a rectangle of a buffer; `last` may run past
@@ -87,6 +90,9 @@ struct PageSpan { uint32_t first, last; };
 PageSpan page_span(uint32_t base_block, uint32_t stride, uint32_t psm, uint32_t x0, uint32_t y0, uint32_t x1,
                    uint32_t y1);
 bool spans_overlap(PageSpan a, PageSpan b);
+
+// Phase filter table of super-sampled textures (oracle_texture.cpp).
+void phase_lut(uint32_t rate_x, uint32_t rate_y, std::vector<uint32_t> &out);
 
 // Q-derived mipmap level of detail in 1/16 units (oracle_texture.cpp); true when above 0.
 bool lod_from_q(float q, int32_t k, uint32_t l, int32_t &n);
@@ -147,6 +153,12 @@ public:
 	// CRTC scanout (oracle_vsync.cpp). priv: the 8 KiB privileged register block.
 	std::vector<uint8_t> priv = std::vector<uint8_t>(8192, 0);
 	uint32_t complex_scanmsk_timeout = 0; // gs/gs_interface.cpp:1947-1960
+	// Super-sampled textures (gs/gs_interface.cpp:65-78; off at 1x) and the page tracker's
+	// "may super-sample" page bits (gs/page_tracker.cpp:147-176, 255-265, 650-660, 700-711).
+	bool super_sampled_textures = false;
+	std::vector<uint8_t> may_super_sample = std::vector<uint8_t>(512, 0);
+	void mark_pages(uint32_t base_block, uint32_t stride, uint32_t psm, uint32_t x0, uint32_t y0, uint32_t x1,
+	                uint32_t y1, bool fb);
 	// Returns 0, or -1000 where the reference would switch to field-aware rendering.
 	int vsync(const VsyncInfo &info, ScanoutOut &out, std::vector<uint32_t> &image);
 	bool vsync_can_skip(const VsyncInfo &info) const;
@@ -176,6 +188,7 @@ private:
 	uint32_t cached_cbp[2] = { 0, 0 }; // CBP0 / CBP1 of CLD 2..5
 	int32_t rp_bb[4] = { 0, 0, 0, 0 };  // union of the pass's primitive boxes
 	bool rp_zwrite = false;            // some primitive of the pass writes depth
+	bool rp_arrays = false;            // some texture of the pass is super-sampled
 	uint64_t rp_frame = 0, rp_zbuf = 0;
 	bool state_dirty = true;          // a draw-state register changed since the last append
 	bool last_parallelogram_candidate = false;

@@ -294,6 +294,23 @@ bool GS::texture_hazard(uint32_t ctx) const
 
 // Finds or creates the render pass's texture slot for the current TEX0 / CLAMP / TEXA state
 // and decodes the texture image from VRAM (gs/shaders/upload.comp:108-228).
+// The page tracker's page rectangle walk (gs/gs_util.cpp:141-171 compute_page_rect): the page
+// of the first pixel, then whole pages to the last pixel's, row by row.
+void GS::mark_pages(uint32_t base_block, uint32_t stride, uint32_t psm, uint32_t x0, uint32_t y0, uint32_t x1,
+                    uint32_t y1, bool fb)
+{
+	uint32_t pw = 6, ph = 5; // 32-bit formats: 64 x 32 pages
+	if (psm == CT16 || psm == CT16S || psm == Z16 || psm == Z16S) { pw = 6; ph = 6; }
+	else if (psm == T8) { pw = 7; ph = 6; }
+	else if (psm == T4) { pw = 7; ph = 7; }
+	if (psm == T8 || psm == T4)
+		stride >>= 1;
+	const uint32_t base = base_block / 32u + (x0 >> pw) + stride * (y0 >> ph);
+	for (uint32_t y = 0; y <= (y1 >> ph) - (y0 >> ph); y++)
+		for (uint32_t x = 0; x <= (x1 >> pw) - (x0 >> pw); x++)
+			may_super_sample[(base + y * stride + x) % 512u] = fb ? 1 : 0;
+}
+
 uint32_t GS::texture_slot(uint32_t ctx)
 {
 	const uint64_t key0 = regs.tex0[ctx] & ((1ull << 61) - 1ull); // CLD does not affect the texture
@@ -303,15 +320,37 @@ uint32_t GS::texture_slot(uint32_t ctx)
 	const uint32_t n1 = std::min(levels - 1u, 3u), n2 = levels > 4 ? levels - 4u : 0u;
 	const uint64_t mip1 = regs.miptbp1[ctx] & ((1ull << (20 * n1)) - 1ull);
 	const uint64_t mip2 = regs.miptbp2[ctx] & ((1ull << (20 * n2)) - 1ull);
+	// Super-sampled when super-sampled textures are on, the primitive is a sprite and the texture
+	// has one level of 8 bits per texel or more whose base page was last written by a render pass
+	// (gs/gs_interface.cpp:1593-1603; the base page of the level-0 page rectangle).
+	uint32_t samples = 0;
+	const uint32_t tpsm = field(key0, 20, 6);
+	const bool bpp8 = !(tpsm == T4 || tpsm == T4HL || tpsm == T4HH);
+	if (super_sampled_textures && sample_log2 != 0 && levels == 1 && bpp8 && field(regs.prim, 0, 3) == 6)
+	{
+		const TexLayout l0 = tex_layout(key0, keyc, 1, 0, 0);
+		uint32_t x0 = ~0u, y0 = ~0u;
+		for (uint32_t x = 0; x < l0.level_width(0); x++)
+			x0 = std::min(x0, l0.tex_x(x, 0));
+		for (uint32_t y = 0; y < l0.level_height(0); y++)
+			y0 = std::min(y0, l0.tex_y(y, 0));
+		uint32_t pw = 6, ph = 5, stride = l0.tbw[0];
+		if (tpsm == CT16 || tpsm == CT16S || tpsm == Z16 || tpsm == Z16S) { pw = 6; ph = 6; }
+		else if (tpsm == T8) { pw = 7; ph = 6; stride >>= 1; }
+		const uint32_t base = l0.tbp[0] / 32u + (x0 >> pw) + stride * (y0 >> ph);
+		if (may_super_sample[base % 512u])
+			samples = 1u << sample_log2;
+	}
 	for (uint32_t i = 0; i < rp_textures.size(); i++)
 	{
 		const TextureSlot &o = rp_textures[i];
 		if (o.tex0 == key0 && o.clamp == keyc && o.texa == keya && o.levels == levels && o.mip1 == mip1 &&
-		    o.mip2 == mip2)
+		    o.mip2 == mip2 && o.samples == samples)
 			return i;
 	}
 
 	TextureSlot t{};
+	t.samples = samples;
 	t.tex0 = key0;
 	t.clamp = keyc;
 	t.texa = keya;
@@ -361,32 +400,50 @@ uint32_t GS::texture_slot(uint32_t ctx)
 		t.level_height[lv] = l.level_height(lv);
 		total += size_t(t.level_width[lv]) * t.level_height[lv];
 	}
-	t.rgba.resize(total);
+	const uint32_t layers = samples ? samples + 1u : 1u;
+	t.rgba.resize(total * layers);
 	const uint8_t *v = vram.data();
+	for (uint32_t layer = 0; layer < layers; layer++)
 	for (uint32_t lv = 0; lv < levels; lv++)
 	for (uint32_t y = 0; y < t.level_height[lv]; y++)
 		for (uint32_t x = 0; x < t.level_width[lv]; x++)
 		{
 			uint32_t a = swizzle(l.tex_x(x, lv), l.tex_y(y, lv), l.tbp[lv], l.tbw[lv], psm);
+			// Layer k > 0 reads sample k - 1 where the reference plane marks the value
+			// super-sampled (upload.comp:133-228 with SUPER_SAMPLED).
+			const uint8_t *ref = v + VRAM_SIZE, *hi = v + size_t(layer + 1u) * VRAM_SIZE;
 			uint32_t c = 0;
 			switch (psm)
 			{
 			case CT24: case Z24:
 			{
-				uint32_t w;
+				uint32_t w, m = 0, h;
 				memcpy(&w, v + a * 4u, 4);
+				if (layer)
+				{
+					memcpy(&m, ref + a * 4u, 4);
+					memcpy(&h, hi + a * 4u, 4);
+					if (m != 0)
+						w = (h & m) | (w & ~m);
+				}
 				c = rgb24_to_32(w, aem, ta0);
 				break;
 			}
 			case CT16: case CT16S: case Z16: case Z16S:
 			{
-				uint16_t h;
+				uint16_t h, m = 0;
 				memcpy(&h, v + a * 2u, 2);
+				if (layer)
+				{
+					memcpy(&m, ref + a * 2u, 2);
+					if (m == 0xffffu)
+						memcpy(&h, hi + a * 2u, 2);
+				}
 				c = rgba16_to_32(h, aem, ta0, ta1);
 				break;
 			}
-			case T8: c = clut_color(v[a]); break;
-			case T8H: c = clut_color(v[a * 4u + 3u]); break;
+			case T8: c = clut_color((layer && ref[a] == 0xffu) ? hi[a] : v[a]); break;
+			case T8H: c = clut_color((layer && ref[a * 4u + 3u] == 0xffu) ? hi[a * 4u + 3u] : v[a * 4u + 3u]); break;
 			case T4: c = clut_color((v[a >> 1] >> ((a & 1u) * 4u)) & 0xfu); break;
 			case T4HL: case T4HH:
 			{
@@ -396,10 +453,19 @@ uint32_t GS::texture_slot(uint32_t ctx)
 				break;
 			}
 			default:
+			{
+				uint32_t m = 0;
 				memcpy(&c, v + a * 4u, 4);
+				if (layer)
+				{
+					memcpy(&m, ref + a * 4u, 4);
+					if (m == ~0u)
+						memcpy(&c, hi + a * 4u, 4);
+				}
 				break;
 			}
-			t.rgba[t.level_offset[lv] + size_t(y) * t.level_width[lv] + x] = c;
+			}
+			t.rgba[size_t(layer) * total + t.level_offset[lv] + size_t(y) * t.level_width[lv] + x] = c;
 		}
 
 	rp_textures.push_back(std::move(t));
@@ -482,9 +548,216 @@ bool lod_from_q(float q, int32_t k, uint32_t l, int32_t &n)
 	return n > 0;
 }
 
-// ubershader.comp:1042-1258 (non-array path).
+// Phase filter table of super-sampled textures for a 2^rate_x x 2^rate_y sampling grid
+// (gs/gs_renderer.cpp:94-313 init_phase_lut): 64 x 64 LINEAR entries of two words (tap sample
+// ids in bits 0..15, 2-bit texel offsets x, y in bits 16..31; unorm8 weights), then 32 x 32
+// NEAREST entries (the nearest sample id).
+void phase_lut(uint32_t rate_x, uint32_t rate_y, std::vector<uint32_t> &out)
+{
+	out.assign(2 * (64 * 64 + 32 * 32), 0u);
+	if (rate_y == 0 || rate_x > rate_y || rate_y > 3)
+		return;
+	const int ns = 1 << (rate_x + rate_y);
+	// compute_sample_points (gs_renderer.cpp:106-150) on a 64-step lattice per texel.
+	auto points = [&](bool nearest, std::vector<std::pair<int, int>> &pts) {
+		pts.clear();
+		const int step = 64 >> rate_y;
+		for (int i = 0; i < ns; i++)
+		{
+			int x, y;
+			if (rate_y == rate_x + 2)
+			{
+				const int sparse[4] = { 0, 2, 3, 1 };
+				x = 4 * (i / 8) + sparse[i % 4];
+				y = i % 8;
+			}
+			else
+			{
+				y = (i & 1) + 2 * ((i >> 2) & 1);
+				x = ((i >> 1) & 1) + 2 * ((i >> 3) & 1);
+				if (rate_y == rate_x + 1)
+					x = 2 * x + (i & 1);
+			}
+			x *= step;
+			y *= step;
+			if (nearest)
+			{
+				x += (step - 1) / 2;
+				y += (step - 1) / 2;
+			}
+			pts.emplace_back(x, y);
+		}
+	};
+	struct R { int id, tx, ty, dx, dy, m; };
+	// compute_phase_lut_samples (gs_renderer.cpp:152-187): every sample point of the texels
+	// within `reach`, sorted by Chebyshev distance then vertical distance (std::sort, as there).
+	auto gather = [&](int x, int y, const std::vector<std::pair<int, int>> &pts, int reach) {
+		std::vector<R> r;
+		for (int i = 0; i < ns; i++)
+			for (int ty = -reach; ty <= reach; ty++)
+				for (int tx = -reach; tx <= reach; tx++)
+				{
+					R e;
+					e.id = i;
+					e.tx = tx;
+					e.ty = ty;
+					e.dx = std::abs(pts[size_t(i)].first + 64 * tx - x);
+					e.dy = std::abs(pts[size_t(i)].second + 64 * ty - y);
+					e.m = std::max(e.dx, e.dy);
+					r.push_back(e);
+				}
+		std::sort(r.begin(), r.end(), [](const R &a, const R &b) { return a.m == b.m ? a.dy < b.dy : a.m < b.m; });
+		return r;
+	};
+	std::vector<std::pair<int, int>> pts, near;
+	points(false, pts);
+	points(true, near);
+	int falloff = 1 << 30;
+	for (int y = 0; y < 64; y++)
+		for (int x = 0; x < 64; x++)
+			falloff = std::min(falloff, gather(x, y, pts, 1)[4].m);
+	for (int y = 0; y < 64; y++)
+		for (int x = 0; x < 64; x++)
+		{
+			std::vector<R> r = gather(x, y, pts, 1);
+			if (r[0].tx != 0 || r[0].ty != 0)
+				for (size_t i = 1; i < r.size(); i++)
+					if (r[i].tx == 0 && r[i].ty == 0)
+					{
+						if (i < 4)
+							std::swap(r[0], r[i]);
+						else
+						{
+							std::swap(r[3], r[i]);
+							std::swap(r[0], r[3]);
+						}
+						break;
+					}
+			uint32_t w0 = 0;
+			float wt[4], sum = 0.0f;
+			for (int i = 0; i < 4; i++)
+			{
+				w0 |= uint32_t(r[size_t(i)].id) << (4 * i) | (uint32_t(r[size_t(i)].tx) & 3u) << (16 + 4 * i) |
+				      (uint32_t(r[size_t(i)].ty) & 3u) << (18 + 4 * i);
+				const float ex = float(r[size_t(i)].dx) / float(falloff), ey = float(r[size_t(i)].dy) / float(falloff);
+				wt[i] = std::max(0.0f, 1.0f - ex) * std::max(0.0f, 1.0f - ey);
+				sum += wt[i];
+			}
+			if (sum < 0.001f)
+			{
+				wt[0] = wt[1] = wt[2] = wt[3] = 1.0f;
+				sum = 4.0f;
+			}
+			uint32_t w1 = 0, used = 0;
+			for (int i = 0; i < 3; i++)
+			{
+				uint32_t w = uint32_t(wt[i] * 255.0f / sum + 0.5f);
+				w = std::min(w, 255u - used);
+				used += w;
+				w1 |= w << (8 * i);
+			}
+			w1 |= (255u - used) << 24;
+			out[size_t(2 * (64 * y + x))] = w0;
+			out[size_t(2 * (64 * y + x) + 1)] = w1;
+		}
+	for (int y = 0; y < 32; y++)
+		for (int x = 0; x < 32; x++)
+			out[size_t(2 * (4096 + 32 * y + x))] = uint32_t(gather(2 * x, 2 * y, near, 0)[0].id);
+}
+
+namespace
+{
+// Nearest texel of image `layer` of a super-sampled texture (REPEAT addressing).
+uint32_t layer_texel(const TextureSlot &t, uint32_t layer, float un, float vn)
+{
+	const float fw = float(t.width), fh = float(t.height);
+	const int32_t w = int32_t(t.width), h = int32_t(t.height);
+	const int32_t x = int32_t(std::floor(un * fw)), y = int32_t(std::floor(vn * fh));
+	const uint32_t xx = uint32_t(((x % w) + w) % w), yy = uint32_t(((y % h) + h) % h);
+	return t.rgba[size_t(layer) * t.width * t.height + size_t(yy) * t.width + xx];
+}
+}
+
+// Super-sampled texture read (ubershader.comp:1155-1211) at the clamped texel coordinate
+// (u1, v1) for sample `smp`; `shift` is 0.5 less the mean sample offset of the shading grid.
+uint32_t sample_texture_array(const TextureSlot &t, const uint32_t *lut, uint32_t tex, float u1, float v1, uint32_t smp,
+                              float shift)
+{
+	const bool linear = (tex & (1u << 16)) != 0;
+	const float u = u1 + 1.0f / 1024.0f, v = v1 + 1.0f / 1024.0f;
+	const bool per_sample = (tex & (1u << 24)) != 0, mapping = (tex & (1u << 25)) != 0;
+	const bool resolved = (tex & (1u << 26)) != 0;
+	if (!per_sample || mapping || resolved)
+	{
+		// A NEAREST or LINEAR (8-bit weights, round-half-up lerp) fetch of one image, with
+		// REPEAT addressing, at the unsnapped coordinate.
+		const uint32_t layer = mapping ? smp + 1u : 0u;
+		const float un = u * t.size[2] + t.bias[0], vn = v * t.size[3] + t.bias[1];
+		const float fw = float(t.width), fh = float(t.height);
+		auto at = [&](int32_t x, int32_t y) {
+			const int32_t w = int32_t(t.width), h = int32_t(t.height);
+			const uint32_t xx = uint32_t(((x % w) + w) % w), yy = uint32_t(((y % h) + h) % h);
+			return t.rgba[size_t(layer) * t.width * t.height + size_t(yy) * t.width + xx];
+		};
+		if (!linear)
+			return at(int32_t(std::floor(un * fw)), int32_t(std::floor(vn * fh)));
+		const float xs = un * fw - 0.5f, ys = vn * fh - 0.5f;
+		const float x0 = std::floor(xs), y0 = std::floor(ys);
+		const uint32_t wx = uint32_t((xs - x0) * 256.0f), wy = uint32_t((ys - y0) * 256.0f);
+		const int32_t ix = int32_t(x0), iy = int32_t(y0);
+		const uint32_t c00 = at(ix, iy), c10 = at(ix + 1, iy), c01 = at(ix, iy + 1), c11 = at(ix + 1, iy + 1);
+		uint32_t c = 0;
+		for (uint32_t ch = 0; ch < 4; ch++)
+		{
+			const uint32_t sh = 8 * ch;
+			const uint32_t top = lerp8((c00 >> sh) & 0xffu, (c10 >> sh) & 0xffu, wx);
+			const uint32_t bot = lerp8((c01 >> sh) & 0xffu, (c11 >> sh) & 0xffu, wx);
+			c |= lerp8(top, bot, wy) << sh;
+		}
+		return c;
+	}
+	const float su = u - shift, sv = v - shift;
+	const float fu = std::floor(su), fv = std::floor(sv);
+	const uint32_t n = linear ? 64u : 32u;
+	const uint32_t ix = uint32_t((su - fu) * float(n)), iy = uint32_t((sv - fv) * float(n));
+	const uint32_t *e = lut + 2u * (linear ? iy * 64u + ix : 4096u + iy * 32u + ix);
+	const float cu = (fu + 0.5f) * t.size[2] + t.bias[0];
+	const float cv = (fv + 0.5f) * t.size[3] + t.bias[1];
+	const uint32_t c0 = layer_texel(t, (e[0] & 15u) + 1u, cu, cv);
+	// The texel is on the clamp region's edge (a reversed region, min > max, clamps to [max, max],
+	// GLSL's clamp being undefined there, so its edge is max).
+	const bool clamp_any = (tex & ((1u << 19) | (1u << 20))) != 0;
+	const float lo_u = std::min(t.region[0], t.region[2]), lo_v = std::min(t.region[1], t.region[3]);
+	const bool edge = clamp_any && (fu == lo_u || fv == lo_v || fu == t.region[2] || fv == t.region[3]);
+	if (!linear || edge)
+		return c0;
+	float acc[4];
+	const float w0 = float(e[1] & 0xffu) / 255.0f;
+	for (uint32_t ch = 0; ch < 4; ch++)
+		acc[ch] = (float((c0 >> (8 * ch)) & 0xffu) / 255.0f) * w0;
+	for (uint32_t i = 1; i < 4; i++)
+	{
+		const uint32_t wb = (e[1] >> (8 * i)) & 0xffu;
+		if (!wb)
+			continue;
+		const float w = float(wb) / 255.0f;
+		int32_t ox = int32_t((e[0] >> (16 + 4 * i)) & 3u), oy = int32_t((e[0] >> (18 + 4 * i)) & 3u);
+		ox = ox >= 2 ? ox - 4 : ox;
+		oy = oy >= 2 ? oy - 4 : oy;
+		const uint32_t ci = layer_texel(t, ((e[0] >> (4 * i)) & 15u) + 1u, cu + float(ox) * t.size[2],
+		                                cv + float(oy) * t.size[3]);
+		for (uint32_t ch = 0; ch < 4; ch++)
+			acc[ch] = acc[ch] + w * (float((ci >> (8 * ch)) & 0xffu) / 255.0f);
+	}
+	uint32_t c = 0;
+	for (uint32_t ch = 0; ch < 4; ch++)
+		c |= uint32_t(std::nearbyint(255.0f * acc[ch])) << (8 * ch);
+	return c;
+}
+
+// ubershader.comp:1042-1258.
 void sample_texture(const TextureSlot &t, uint32_t tex, uint32_t tex2, float u, float v, float q, bool perspective,
-                    float out[4])
+                    const uint32_t *lut, uint32_t smp, float shift, float out[4])
 {
 	const bool clamp_s = (tex & (1u << 19)) != 0, clamp_t = (tex & (1u << 20)) != 0;
 	const bool mag_linear = (tex & (1u << 16)) != 0, min_linear = (tex & (1u << 17)) != 0;
@@ -527,6 +800,13 @@ void sample_texture(const TextureSlot &t, uint32_t tex, uint32_t tex2, float u, 
 		u1 = texture_clamp(u1, t.region[0], t.region[2], lod_1);
 	if (clamp_t)
 		v1 = texture_clamp(v1, t.region[1], t.region[3], lod_1);
+	if (t.samples)
+	{
+		const uint32_t c = sample_texture_array(t, lut, tex, u1, v1, smp, shift);
+		for (uint32_t ch = 0; ch < 4; ch++)
+			out[ch] = float((c >> (8 * ch)) & 0xffu);
+		return;
+	}
 	const float inv1[2] = { std::ldexp(t.size[2], int(lod_1)), std::ldexp(t.size[3], int(lod_1)) };
 	float un = std::floor(u1 * 16.0f + 1.0f / 64.0f);
 	float vn = std::floor(v1 * 16.0f + 1.0f / 64.0f);

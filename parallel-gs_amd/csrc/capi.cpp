@@ -149,7 +149,7 @@ pgs_context *pgs_create(const pgs_options *options)
 	pgs::GSOptions o;
 	if (options)
 	{
-		// Sample counts other than 1/2/4/8/16 and super-sampled textures make init fail.
+		// Sample counts other than 1/2/4/8/16 make init fail.
 		o.super_sampling = pgs::SuperSampling(options->super_sampling ? options->super_sampling : 1u);
 		o.ordered_super_sampling = options->ordered_super_sampling != 0;
 		o.super_sampled_textures = options->super_sampled_textures != 0;
@@ -352,8 +352,6 @@ int pgs_set_super_sampling_rate(pgs_context *ctx, uint32_t super_sampling, uint3
 	                   super_sampling == 16;
 	if (!known)
 		return PGS_ERROR_INVALID_ARGUMENT;
-	if (super_sampled_textures && super_sampling != 1)
-		return PGS_ERROR_UNSUPPORTED;
 	bool ok = false;
 	const int rc = guard(ctx, [&] {
 		ok = ctx->gs.set_super_sampling_rate(pgs::SuperSampling(super_sampling), ordered_grid != 0,

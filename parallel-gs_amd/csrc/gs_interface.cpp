@@ -31,10 +31,8 @@ bool GSInterface::init(const GSOptions &options)
 		return false;
 	const uint32_t sample_log2 = uint32_t(l);
 	// Sampling grids (gs/gs_interface.cpp:65-117): 2x = 1x2, 4x = 2x2 or sparse 1x4, 8x = 2x4,
-	// 16x = 4x4 or sparse 2x8. Super-sampled textures (texture arrays per sample,
-	// ubershader.comp:1155-1212) are not built: asking for them fails.
-	if (options.super_sampled_textures && sample_log2 != 0)
-		return false;
+	// 16x = 4x4 or sparse 2x8. Super-sampled textures are off at 1x (gs/gs_interface.cpp:74-78).
+	rp_.super_sampled_textures = options.super_sampled_textures && sample_log2 != 0;
 	const bool sparse = !options.ordered_super_sampling && (sample_log2 == 2 || sample_log2 == 4);
 	renderer_.set_sparse_grid(sparse);
 	const uint32_t capacity_log2 = options.dynamic_super_sampling ? 4u : sample_log2;
@@ -79,12 +77,12 @@ bool GSInterface::set_super_sampling_rate(SuperSampling rate, bool ordered_grid,
 	if (l < 0)
 		return false;
 	const uint32_t sample_log2 = uint32_t(l);
-	// At 1x the reference turns super-sampled textures off (gs/gs_interface.cpp:74-78).
-	if (super_sampled_textures && sample_log2 != 0)
-		return false;
 	if (sample_log2 > renderer_.capacity_log2())
 		return false;
 	flush();
+	// At 1x the reference turns super-sampled textures off (gs/gs_interface.cpp:74-78).
+	rp_.super_sampled_textures = super_sampled_textures && sample_log2 != 0;
+	options_.super_sampled_textures = rp_.super_sampled_textures;
 	const bool sparse = !ordered_grid && (sample_log2 == 2 || sample_log2 == 4);
 	renderer_.set_sparse_grid(sparse);
 	if (!renderer_.set_sample_log2(sample_log2))
@@ -526,6 +524,8 @@ void GSInterface::init_transfer()
 			return;
 		begin_transfer_work();
 		rp_.writes().write(page_span(c.dbp, c.dbw, c.dpsm, c.dsax, c.dsay, c.dsax + c.w - 1u, c.dsay + c.h - 1u));
+		rp_.writes().mark_super_sample(
+		    page_rect(c.dbp, c.dbw, c.dpsm, c.dsax, c.dsay, c.dsax + c.w - 1u, c.dsay + c.h - 1u), false);
 		renderer_.queue_local_copy(c);
 	}
 	else if (xdir == 0)
@@ -533,6 +533,9 @@ void GSInterface::init_transfer()
 		// The texture cache must not reuse an image of the destination from here on: a texture
 		// lookup can come before the payload is flushed (each flush logs its pages again).
 		rp_.writes().write(page_span(c.dbp, c.dbw, c.dpsm, c.dsax, c.dsay, c.dsax + c.w - 1u, c.dsay + c.h - 1u));
+		if (c.w && c.h)
+			rp_.writes().mark_super_sample(
+			    page_rect(c.dbp, c.dbw, c.dpsm, c.dsax, c.dsay, c.dsax + c.w - 1u, c.dsay + c.h - 1u), false);
 		xfer_.copy = c;
 		xfer_.required_qwords = uint32_t((uint64_t(c.w) * c.h * psm_bpp(c.dpsm)) / 64u);
 		xfer_.host_to_local_active = xfer_.required_qwords != 0;
@@ -688,6 +691,7 @@ void GSInterface::end_vram_write(size_t offset, size_t size)
 	// 1478-1539); the texture cache forgets images decoded from them.
 	const uint32_t first_page = uint32_t(offset / kPageBytes), last_page = uint32_t((offset + size - 1u) / kPageBytes);
 	rp_.writes().write(PageSpan{ first_page, last_page });
+	rp_.writes().mark_super_sample(PageRect{ first_page, 0u, last_page - first_page + 1u, 1u }, false);
 	const auto t0 = std::chrono::steady_clock::now();
 	renderer_.write_vram(host_vram_.data() + size_t(first_page) * kPageBytes, size_t(first_page) * kPageBytes,
 	                     size_t(last_page - first_page + 1u) * kPageBytes);

@@ -24,6 +24,15 @@ enum : uint32_t
 	TEX_MAG_LINEAR = 1u << 16, TEX_MIN_LINEAR = 1u << 17, TEX_MIP_LINEAR = 1u << 18,
 	TEX_CLAMP_S = 1u << 19, TEX_CLAMP_T = 1u << 20,
 	TEX_MAX_MIP_LEVEL_SHIFT = 21, // TEX1.MXL of a mipmapped primitive, 3 bits
+	// Super-sampled textures (data_structures.h:251-253): the texture has one layer per sample
+	// after the single-sampled layer 0 (TEX_ARRAYED, set for every primitive that samples it);
+	// sample through the phase filter (PER_SAMPLE), each sample from its own layer
+	// (SAMPLE_MAPPING), or layer 0 (SAMPLE_RESOLVED, or PER_SAMPLE cleared by setup).
+	TEX_PER_SAMPLE = 1u << 24, TEX_SAMPLE_MAPPING = 1u << 25, TEX_SAMPLE_RESOLVED = 1u << 26,
+	TEX_ARRAYED = 1u << 27,
+	// Host hint for triangle setup: the pass is a deduced blur (gs/gs_renderer.cpp:1848-1885) and
+	// the primitive blends Cs * alpha + Cd without a depth test (triangle_setup.comp:713-722).
+	TEX_BLUR_KERNEL = 1u << 28,
 	TEX2_FIXED_LOD = 1u << 0,     // LCM; L in bits 1..2, K (signed, 1/16 units) in bits 3..14
 	TEX2_L_SHIFT = 1, TEX2_K_SHIFT = 3,
 	// On-tile frame-buffer feedback (the texel is the pixel's own colour or depth): bit 15 of the

@@ -106,6 +106,20 @@ __device__ __forceinline__ int32_t quantize64(int64_t c) { return int32_t(uint32
 
 template <typename T> __device__ __forceinline__ void swp(T &a, T &b) { T t = a; a = b; b = t; }
 
+// 8 x the mean sample offset of a sampling grid, in pixels (utils.h:108-125
+// get_average_sampling_offset, the same on both axes): 2 at 2x and ordered 4x, 3 at ordered
+// 16x, 8x and sparse 4x, 3.5 at sparse 16x.
+__device__ __forceinline__ float mean_sample_offset8(uint32_t rate_x, uint32_t rate_y)
+{
+	if (rate_x == rate_y)
+		return 4.0f * float((1u << rate_y) - 1u) / float(1u << rate_y);
+	if (rate_y == 1u)
+		return 2.0f;
+	if (rate_y == 2u)
+		return 3.0f;
+	return 3.5f;
+}
+
 // ---------------------------------------------------------------------------------------
 // Triangle setup (gs/shaders/triangle_setup.comp:127-873), one thread per primitive.
 __device__ __forceinline__ void triangle_setup_one(const pgs_setup_args &args, uint32_t index)
@@ -386,7 +400,6 @@ __device__ __forceinline__ void triangle_setup_one(const pgs_setup_args &args, u
 	s.bb[1] = max(loy, int32_t(prim.bb[1]));
 	s.bb[2] = min(hix, int32_t(prim.bb[2]));
 	s.bb[3] = min(hiy, int32_t(prim.bb[3]));
-	out = s;
 
 	if (sprite && !perspective)
 	{
@@ -401,6 +414,47 @@ __device__ __forceinline__ void triangle_setup_one(const pgs_setup_args &args, u
 		if ((ns_u || cs_u) && (ns_v || cs_v) && no_interp)
 			tex &= ~TEX_MAG_LINEAR;
 	}
+
+	// Super-sampled textures (triangle_setup.comp:677-789; FORCE_LINEAR_SUPER_SAMPLE is off, it
+	// needs field-aware rendering). Sprites sampling per sample keep that for scales near 1:1,
+	// fall back to layer 0 for downscales and to the resolved layer for upscales by 2 or more;
+	// NEAREST sprites map samples to layers unless a deduced blur kernel blends them
+	// (TEX_BLUR_KERNEL, set by the host).
+	if (rate != 0 && (args.texture_flags & 1u) &&
+	    (tex & (TEX_PER_SAMPLE | TEX_SAMPLE_MAPPING | TEX_FEEDBACK)) == TEX_PER_SAMPLE)
+	{
+		const int32_t du = abs(int32_t(attr[o1].uv & 0xffffu) - int32_t(attr[o0].uv & 0xffffu));
+		const int32_t dv = abs(int32_t(attr[o1].uv >> 16) - int32_t(attr[o0].uv >> 16));
+		const int32_t distx = abs(dx - ax), disty = abs(dy - ay);
+		if (!(tex & TEX_MAG_LINEAR))
+		{
+			if (perspective)
+				tex |= TEX_SAMPLE_MAPPING;
+			else if (du > distx + 1 || dv > disty + 1)
+				tex &= ~TEX_PER_SAMPLE;
+			else if (du >= distx - 16 || dv >= disty - 16)
+				tex = (tex & TEX_BLUR_KERNEL) ? (tex & ~TEX_PER_SAMPLE) : (tex | TEX_SAMPLE_MAPPING);
+		}
+		else if (!perspective)
+		{
+			if (du > distx + 1 && dv > disty + 1)
+				tex &= ~TEX_PER_SAMPLE;
+			else if (distx >= 2 * du && disty >= 2 * dv)
+				tex |= TEX_SAMPLE_RESOLVED;
+		}
+		if ((tex & (TEX_PER_SAMPLE | TEX_SAMPLE_MAPPING)) == TEX_PER_SAMPLE)
+		{
+			// Per-sample interpolation, shifted so that the samples' mean position interpolates
+			// as the pixel's single sample would: 2^3 x the grid's mean sample offset.
+			state &= ~ST_SNAP_ATTRIBUTE;
+			const float corner = mean_sample_offset8(args.texture_flags >> 8 & 3u, rate);
+			const float di_dc = __fmul_rn(float(dbdx + dbdy), corner);
+			const float dj_dc = __fmul_rn(float(dcdx + dcdy), corner);
+			s.error_i = __fsub_rn(s.error_i, __fmul_rn(di_dc, inv_area_scaled));
+			s.error_j = __fsub_rn(s.error_j, __fmul_rn(dj_dc, inv_area_scaled));
+		}
+	}
+	out = s;
 
 	if (prim.fbmsk != args.opaque_fbmask && (state & ST_OPAQUE))
 		state &= ~ST_OPAQUE;
@@ -756,6 +810,9 @@ __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t 
 				if ((st & (ST_SNAP_RASTER | ST_Z_TEST | ST_OPAQUE)) != (ST_SNAP_RASTER | ST_OPAQUE))
 					req_zw = true;
 				if ((st & (ST_SNAP_RASTER | ST_OPAQUE)) != (ST_SNAP_RASTER | ST_OPAQUE))
+					req = true;
+				// Sampling a super-sampled texture per sample needs the samples (binning.comp:123-124).
+				if ((prims[i].tex & (TEX_PER_SAMPLE | TEX_FEEDBACK)) == TEX_PER_SAMPLE)
 					req = true;
 				if (st & ST_Z_WRITE)
 					has_zw = true;

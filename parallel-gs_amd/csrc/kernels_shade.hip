@@ -357,11 +357,85 @@ __device__ __forceinline__ uint32_t sample_mipmapped(const TextureDesc &t, const
 	return c;
 }
 
-// Non-array sampling path of sample_texture (ubershader.comp:1042-1258). At LOD 0 (no mipmap):
-// UV snap to 1/16 texel with the reference's epsilons, then a NEAREST or LINEAR fetch.
+// Nearest texel of layer `layer` of a super-sampled texture's image (REPEAT addressing).
+__device__ __forceinline__ uint32_t layer_texel(const TextureDesc &t, const uint32_t *images, uint32_t layer, float un,
+                                                float vn)
+{
+	const uint32_t x = wrap_texel(floorf(__fmul_rn(un, t.fwidth)), t.fwidth, t.size[2]);
+	const uint32_t y = wrap_texel(floorf(__fmul_rn(vn, t.fheight)), t.fheight, t.size[3]);
+	return images[t.image_offset + __umul24(layer, t.width * t.height) + __umul24(y, t.width) + x];
+}
+
+// The raw clamp region of one bound of a super-sampled texture's descriptor: the stored
+// floor(region) + 0.5 less 0.5 on a clamping axis, 0 on the others (whose stored bound is
+// infinite), as the reference's TexInfo region (gs/gs_interface.cpp:1758-1777).
+__device__ __forceinline__ float raw_region(float stored) { return __builtin_isinf(stored) ? 0.0f : __fsub_rn(stored, 0.5f); }
+
+// Super-sampled texture (ubershader.comp:1155-1211): layer 0 holds the single-sampled image,
+// layer k + 1 the image of sample k. PER_SAMPLE: a filter over the samples nearest to the
+// coordinate's phase (the phase table's four taps and unorm8 weights) with nearest fetches,
+// or the first tap alone when magnification is NEAREST or the texel is on a clamped edge;
+// otherwise a NEAREST / LINEAR fetch of the wave's own sample's layer (SAMPLE_MAPPING) or of
+// layer 0. The coordinates are not snapped to 1/16 texel. `smp` is the lane's sample and
+// `shift` 0.5 less the grid's mean sample offset (0.5 in a single-sampled wave).
+__device__ __forceinline__ uint32_t sample_arrayed(const TextureDesc &t, const uint32_t *images, const uint32_t *lut,
+                                                   uint32_t tex, float u1, float v1, uint32_t smp, float shift)
+{
+	const float uu = __fadd_rn(u1, 1.0f / 1024.0f), vv = __fadd_rn(v1, 1.0f / 1024.0f);
+	const bool linear = (tex & TEX_MAG_LINEAR) != 0;
+	if ((tex & (TEX_PER_SAMPLE | TEX_SAMPLE_MAPPING | TEX_SAMPLE_RESOLVED)) != TEX_PER_SAMPLE)
+	{
+		const float un = __fadd_rn(__fmul_rn(uu, t.size[2]), t.bias[0]);
+		const float vn = __fadd_rn(__fmul_rn(vv, t.size[3]), t.bias[1]);
+		const uint32_t layer = (tex & TEX_SAMPLE_MAPPING) ? smp + 1u : 0u;
+		return fetch_level(images + t.image_offset + __umul24(layer, t.width * t.height), t.width, t.height, t.fwidth,
+		                   t.fheight, t.size[2], t.size[3], un, vn, linear);
+	}
+	const float su = __fsub_rn(uu, shift), sv = __fsub_rn(vv, shift);
+	const float fu = floorf(su), fv = floorf(sv);
+	// Phase table entry: 64 x 64 for LINEAR, 32 x 32 (after it) for NEAREST.
+	const float n = linear ? 64.0f : 32.0f;
+	const uint32_t ix = uint32_t(__fmul_rn(__fsub_rn(su, fu), n)), iy = uint32_t(__fmul_rn(__fsub_rn(sv, fv), n));
+	const uint32_t e = linear ? iy * 64u + ix : 4096u + iy * 32u + ix;
+	const uint2 entry = reinterpret_cast<const uint2 *>(lut)[e];
+	const float cu = __fadd_rn(__fmul_rn(__fadd_rn(fu, 0.5f), t.size[2]), t.bias[0]);
+	const float cv = __fadd_rn(__fmul_rn(__fadd_rn(fv, 0.5f), t.size[3]), t.bias[1]);
+	const uint32_t c0 = layer_texel(t, images, (entry.x & 15u) + 1u, cu, cv);
+	const bool on_edge = (tex & (TEX_CLAMP_S | TEX_CLAMP_T)) &&
+	                     (fu == raw_region(t.region[0]) || fv == raw_region(t.region[1]) ||
+	                      fu == raw_region(t.region[2]) || fv == raw_region(t.region[3]));
+	if (!linear || on_edge)
+		return c0;
+	float acc[4];
+	const float w0 = __fdiv_rn(float(entry.y & 0xffu), 255.0f);
+	for (uint32_t ch = 0; ch < 4u; ch++)
+		acc[ch] = __fmul_rn(__fdiv_rn(float((c0 >> (8u * ch)) & 0xffu), 255.0f), w0);
+	for (uint32_t i = 1; i < 4u; i++)
+	{
+		const uint32_t wb = (entry.y >> (8u * i)) & 0xffu;
+		if (wb == 0u)
+			continue;
+		const float w = __fdiv_rn(float(wb), 255.0f);
+		// Signed 2-bit texel offsets of the tap.
+		const int32_t ox = int32_t(entry.x << (14u - 4u * i)) >> 30, oy = int32_t(entry.x << (12u - 4u * i)) >> 30;
+		const float tu = __fadd_rn(cu, __fmul_rn(float(ox), t.size[2]));
+		const float tv = __fadd_rn(cv, __fmul_rn(float(oy), t.size[3]));
+		const uint32_t ci = layer_texel(t, images, ((entry.x >> (4u * i)) & 15u) + 1u, tu, tv);
+		for (uint32_t ch = 0; ch < 4u; ch++)
+			acc[ch] = __fadd_rn(acc[ch], __fmul_rn(w, __fdiv_rn(float((ci >> (8u * ch)) & 0xffu), 255.0f)));
+	}
+	uint32_t c = 0;
+	for (uint32_t ch = 0; ch < 4u; ch++)
+		c |= uint32_t(rintf(__fmul_rn(255.0f, acc[ch]))) << (8u * ch);
+	return c;
+}
+
+// sample_texture (ubershader.comp:1042-1258). At LOD 0 (no mipmap): UV snap to 1/16 texel with
+// the reference's epsilons, then a NEAREST or LINEAR fetch; super-sampled textures take the
+// array path after the clamp.
 __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint32_t *images, uint32_t tex,
                                                uint32_t tex2, float u, float v, float q, bool perspective,
-                                               float out[4])
+                                               const uint32_t *lut, uint32_t smp, float shift, float out[4])
 {
 	uint32_t c;
 	if (tex & (7u << TEX_MAX_MIP_LEVEL_SHIFT))
@@ -377,6 +451,13 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 		// +-2047), so the median of (x, lo, hi) is that clamp.
 		u1 = __builtin_amdgcn_fmed3f(u1, t.region[0], t.region[2]);
 		v1 = __builtin_amdgcn_fmed3f(v1, t.region[1], t.region[3]);
+		if (tex & TEX_ARRAYED)
+		{
+			c = sample_arrayed(t, images, lut, tex, u1, v1, smp, shift);
+			for (uint32_t ch = 0; ch < 4; ch++)
+				out[ch] = float((c >> (8 * ch)) & 0xffu);
+			return;
+		}
 		float un = floorf(__fadd_rn(__fmul_rn(u1, 16.0f), 1.0f / 64.0f));
 		float vn = floorf(__fadd_rn(__fmul_rn(v1, 16.0f), 1.0f / 64.0f));
 		un = __fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(un, 16.0f / 1024.0f), t.size[2]), 0.0625f), t.bias[0]);
@@ -483,9 +564,20 @@ __device__ __forceinline__ uint32_t aa1_coverage(const PrimitiveSetup &s, bool l
 // left (measured faster for single-sampled shading, where large primitives cover whole waves);
 // otherwise every per-lane failure returns at once (faster for super-sampled shading of small
 // primitives). Both give the same pixels.
+// 0.5 less the mean sample offset of a sampling grid (utils.h:108-125): the phase of a
+// super-sampled texture's filter is taken relative to it (ubershader.comp:1167).
+__device__ __forceinline__ float phase_shift(uint32_t rate_x, uint32_t rate_y)
+{
+	if (rate_x == rate_y)
+		return rate_y == 1u ? 0.25f : 0.125f;
+	if (rate_y == 1u)
+		return 0.25f;
+	return rate_y == 2u ? 0.125f : 0.0625f;
+}
+
 template <bool kPredicated>
 __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassInfo &pi, uint32_t prim_index,
-                                            int32_t hx, int32_t hy, uint32_t rate, bool super_sampled,
+                                            int32_t hx, int32_t hy, uint32_t rate, bool super_sampled, uint32_t smp,
                                             uint32_t &color, uint32_t &depth, bool &cdirty, bool &zdirty)
 {
 	const PrimitiveSetup *setups = static_cast<const PrimitiveSetup *>(a.setup);
@@ -633,7 +725,8 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 			}
 			u = fminf(fmaxf(u, -2047.0f), 2047.0f);
 			v = fminf(fmaxf(v, -2047.0f), 2047.0f);
-			sample_texture(td, a.tex_images, p_tex, p_tex2, u, v, stqf(2), persp, tx);
+			sample_texture(td, a.tex_images, p_tex, p_tex2, u, v, stqf(2), persp, a.phase_lut, smp,
+			               super_sampled ? phase_shift(a.rate_x_log2, rate) : 0.5f, tx);
 		}
 		const bool tcc = (sv_combiner & COMB_TCC) != 0;
 		const uint32_t mode = sv_combiner & COMB_MODE_MASK;
@@ -857,7 +950,7 @@ __device__ __forceinline__ void shade_1x_block(const pgs_shade_args &a, uint32_t
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
 			const uint32_t prim = uint32_t(__builtin_amdgcn_readlane(int(idx), int(bit)));
-			shade_pixel<true>(a, pi, prim, px, py, 0u, false, color, depth, cdirty, zdirty);
+			shade_pixel<true>(a, pi, prim, px, py, 0u, false, 0u, color, depth, cdirty, zdirty);
 		}
 	}
 
@@ -1088,7 +1181,7 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
 			const uint32_t prim = uint32_t(__builtin_amdgcn_readlane(int(idx), int(bit)));
-			shade_pixel<false>(a, pi, prim, hx, hy, rate, !single, color, depth, cdirty, zdirty);
+			shade_pixel<false>(a, pi, prim, hx, hy, rate, !single, smp, color, depth, cdirty, zdirty);
 		}
 	}
 

@@ -474,14 +474,15 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 		// Bypass reads the texture as it is when the pass starts, whatever the pass writes.
 		if (!prims_.empty() && feedback == Feedback::None && texture_hazard(r, ctx))
 			flush();
-		slot = texture_slot(r, ctx);
+		slot = texture_slot(r, ctx, type == 6);
 		if (slot < 0)
 		{
 			flush();
-			slot = texture_slot(r, ctx);
+			slot = texture_slot(r, ctx, type == 6);
 		}
 	}
 
+	bool expands = prims_.empty();
 	if (prims_.empty())
 	{
 		if (host)
@@ -492,6 +493,7 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 		color_write_mask_ = 0;
 	}
 	zbuf_ = (zbuf_ & ~0x0f0001ffull) | (zbuf & 0x0f0001ffull);
+	expands = expands || lox < bb_[0] || loy < bb_[1] || hix > bb_[2] || hiy > bb_[3];
 	bb_[0] = std::min(bb_[0], lox); bb_[1] = std::min(bb_[1], loy);
 	bb_[2] = std::max(bb_[2], hix); bb_[3] = std::max(bb_[3], hiy);
 
@@ -576,6 +578,17 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 		}
 		else
 			pa.tex2 = TEX2_FIXED_LOD;
+		// A super-sampled texture is sampled per sample; palette formats with REGION_REPEAT
+		// map each sample to its own layer (gs/gs_interface.cpp:1783-1790, 1824-1831).
+		if (slot_keys_[uint32_t(slot)].samples)
+		{
+			const uint32_t psm = field(r.tex0[ctx], 20, 6);
+			const bool palette = psm == PSMT8 || psm == PSMT4 || psm == PSMT8H || psm == PSMT4HL || psm == PSMT4HH;
+			pa.tex |= TEX_ARRAYED | TEX_PER_SAMPLE;
+			if (palette && (wms == 3 || wmt == 3))
+				pa.tex |= TEX_SAMPLE_MAPPING;
+			pass_arrays_ = true;
+		}
 	}
 	if (zte)
 	{
@@ -643,6 +656,19 @@ void RenderPassBuilder::append(const RegisterState &r, const QueuedVertex *vin, 
 	pass_z_ |= (state & (ST_Z_TEST | ST_Z_WRITE)) != 0;
 	pass_zwrite_ |= (state & ST_Z_WRITE) != 0;
 	prims_.push_back(pa);
+	// A draw that grows the pass's box marks the box's frame-buffer pages (and depth pages,
+	// once the pass writes depth) as possibly super-sampled (gs/gs_interface.cpp:2786-2816).
+	if (expands && bb_[2] >= bb_[0] && bb_[3] >= bb_[1])
+	{
+		const uint32_t x0 = uint32_t(bb_[0]), y0 = uint32_t(bb_[1]), x1 = uint32_t(bb_[2]), y1 = uint32_t(bb_[3]);
+		writes_.mark_super_sample(page_rect(field(frame_, 0, 9) * 32u, field(frame_, 16, 6), field(frame_, 24, 6), x0,
+		                                    y0, x1, y1),
+		                          true);
+		if (pass_zwrite_)
+			writes_.mark_super_sample(page_rect(field(zbuf_, 0, 9) * 32u, field(frame_, 16, 6),
+			                                    0x30u | field(zbuf_, 24, 4), x0, y0, x1, y1),
+			                          true);
+	}
 	pos_.insert(pos_.end(), p, p + 3);
 	attr_.insert(attr_.end(), a, a + 3);
 	last_candidate_ = candidate;
@@ -681,6 +707,9 @@ void RenderPassBuilder::flush()
 		d.z_sensitive = pass_z_;
 		d.z_write = pass_zwrite_;
 		d.z_feedback = pass_z_feedback_;
+		d.arrays = pass_arrays_;
+		if (pass_arrays_)
+			mark_blur_kernels();
 		d.zbuf_psm = field(zbuf_, 24, 4) | 0x30u;
 		d.decodes = decodes_.data();
 		d.tex_descs = tex_descs_.data();
@@ -721,6 +750,7 @@ void RenderPassBuilder::flush()
 		pass_feedback_ = 0;
 		pass_feedback_psm_ = pass_feedback_cpsm_ = 0;
 		pass_z_feedback_ = false;
+		pass_arrays_ = false;
 		pos_.clear();
 		attr_.clear();
 		states_.clear();
@@ -733,7 +763,49 @@ void RenderPassBuilder::flush()
 
 namespace pgs
 {
-RenderPassBuilder::SlotKey RenderPassBuilder::slot_key(const RegisterState &r, uint32_t ctx) const
+// A pass of at most 64 primitives that samples one super-sampled texture at two different
+// phases (first vertex UV less position) is taken for a blur (gs/gs_renderer.cpp:1848-1885);
+// then its primitives that blend Cs * alpha + Cd without a depth test are blur taps, which
+// triangle setup keeps off per-sample sampling (triangle_setup.comp:705-757).
+void RenderPassBuilder::mark_blur_kernels()
+{
+	if (prims_.size() > 64u)
+		return;
+	uint32_t last = ~0u;
+	int32_t phase_u = 0, phase_v = 0;
+	bool blur = false;
+	for (size_t i = 0; i < prims_.size() && !blur; i++)
+	{
+		const PrimitiveAttribute &p = prims_[i];
+		if ((p.tex & (TEX_PER_SAMPLE | TEX_FEEDBACK)) != TEX_PER_SAMPLE)
+			continue;
+		if (p.state & ST_PERSPECTIVE)
+			return;
+		const uint32_t index = p.tex & TEX_INDEX_MASK;
+		const int32_t pu = int32_t(attr_[3 * i].uv & 0xffffu) - pos_[3 * i].x;
+		const int32_t pv = int32_t(attr_[3 * i].uv >> 16) - pos_[3 * i].y;
+		if (index == last)
+			blur = pu != phase_u || pv != phase_v;
+		else
+		{
+			phase_u = pu;
+			phase_v = pv;
+		}
+		last = index;
+	}
+	if (!blur)
+		return;
+	for (PrimitiveAttribute &p : prims_)
+	{
+		const uint32_t bm = states_[p.state & STATE_INDEX_MASK].blend_mode;
+		const bool taps = (bm & BLEND_ABE) && !(p.state & ST_Z_TEST) && ((bm >> 11) & 3u) == 0u &&
+		                  ((bm >> 13) & 3u) == 2u && ((bm >> 17) & 3u) == 1u;
+		if ((p.tex & (TEX_PER_SAMPLE | TEX_FEEDBACK)) == TEX_PER_SAMPLE && taps)
+			p.tex |= TEX_BLUR_KERNEL;
+	}
+}
+
+RenderPassBuilder::SlotKey RenderPassBuilder::slot_key(const RegisterState &r, uint32_t ctx, bool sprite) const
 {
 	SlotKey k{};
 	k.tex0 = r.tex0[ctx] & ((1ull << 61) - 1ull); // CLD does not change the texture
@@ -744,6 +816,16 @@ RenderPassBuilder::SlotKey RenderPassBuilder::slot_key(const RegisterState &r, u
 	const uint32_t l1 = std::min(k.levels - 1u, 3u), l2 = k.levels > 4u ? k.levels - 4u : 0u;
 	k.mip1 = r.miptbp1[ctx] & ((1ull << (20u * l1)) - 1ull);
 	k.mip2 = r.miptbp2[ctx] & ((1ull << (20u * l2)) - 1ull);
+	// A sprite's single-level texture of at least 8 bits per texel whose first page was last
+	// written by a render pass is super-sampled (gs/gs_interface.cpp:1593-1603; the first page of
+	// the level-0 span is the page rectangle's base page, gs/page_tracker.cpp:147-153).
+	k.samples = 0;
+	if (sprite && super_sampled_textures && sample_log2_ != 0 && k.levels == 1u && psm_bpp(field(k.tex0, 20, 6)) >= 8u)
+	{
+		const TexLayout l = tex_layout(k.tex0, k.clamp, 1u, 0u, 0u);
+		if (writes_.may_super_sample(texture_span(l).s[0].first))
+			k.samples = 1u << sample_log2_;
+	}
 	return k;
 }
 
@@ -789,16 +871,17 @@ bool RenderPassBuilder::allocate_image(uint32_t texels, uint32_t &offset)
 	return true;
 }
 
-int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
+int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx, bool sprite)
 {
-	const SlotKey key = slot_key(r, ctx);
+	const SlotKey key = slot_key(r, ctx, sprite);
 	for (uint32_t i = 0; i < slot_keys_.size(); i++)
 		if (slot_keys_[i] == key)
 			return int32_t(i);
 	const uint64_t key0 = key.tex0, keyc = key.clamp, keya = key.texa;
 
 	const TexLayout l = tex_layout(key0, keyc, key.levels, key.mip1, key.mip2);
-	const uint32_t texels = l.texels();
+	// A super-sampled texture holds samples + 1 layers of its single level.
+	const uint32_t texels = l.texels() * (key.samples ? key.samples + 1u : 1u);
 	if (slot_keys_.size() >= kMaxPassTextures || texels > kTexImageTexels)
 		return prims_.empty() ? -2 : -1;
 
@@ -882,6 +965,7 @@ int32_t RenderPassBuilder::texture_slot(const RegisterState &r, uint32_t ctx)
 	dd.ta0 = field(keya, 0, 8);
 	dd.ta1 = field(keya, 32, 8);
 	dd.group_begin = group_total_;
+	dd.samples = key.samples;
 
 	slot_keys_.push_back(key);
 	slot_images_.push_back(im);

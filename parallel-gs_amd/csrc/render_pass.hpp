@@ -34,6 +34,7 @@ struct RenderPassDesc
 	bool z_sensitive;         // some primitive tests or writes depth
 	bool z_write;             // some primitive writes depth
 	bool z_feedback;          // some primitive reads its own depth as a texture (renders single-sampled)
+	bool arrays;              // some texture of the pass is super-sampled (TEX_ARRAYED)
 	uint32_t zbuf_psm;        // ZBUF.PSM as written (0x30 | low bits), before format deduction
 	// Textures of the pass, decoded from VRAM before shading.
 	const TexDecode *decodes;
@@ -65,7 +66,18 @@ public:
 	{
 		clock_++;
 		page_.fill(clock_);
+		may_ss_.fill(false);
 	}
+	// Pages whose last write was a render pass's frame-buffer or depth write (`fb`), after which a
+	// texture based there may be super-sampled; transfers and host writes make them
+	// single-sampled (gs/page_tracker.cpp:147-176, 255-265, 650-660, 700-711).
+	void mark_super_sample(const PageRect &r, bool fb)
+	{
+		for (uint32_t y = 0; y < r.height; y++)
+			for (uint32_t x = 0; x < r.width; x++)
+				may_ss_[(r.base + y * r.stride + x) % kNumPages] = fb;
+	}
+	bool may_super_sample(uint32_t page) const { return may_ss_[page % kNumPages]; }
 	void clut_loaded() { clut_gen_++; }
 	uint64_t now() const { return clock_; }
 	uint64_t clut_generation() const { return clut_gen_; }
@@ -91,6 +103,7 @@ public:
 private:
 	uint64_t clock_ = 1, clut_gen_ = 0;
 	std::array<uint64_t, kNumPages> page_{};
+	std::array<bool, kNumPages> may_ss_{};
 };
 
 // Per-pass texture limits: slot index width of PrimitiveAttribute.tex and the decoded image
@@ -139,6 +152,12 @@ public:
 	// Hacks::disable_mipmaps: textures decode level 0 only and sample it at LOD 0.
 	bool disable_mipmaps = false;
 
+	// Super-sampled textures (gs/gs_interface.cpp:65-77, 1593-1603): at 2x or more, sprites may
+	// sample frame-buffer pages through a texture with one layer per sample.
+	bool super_sampled_textures = false;
+	// Does the pending pass sample a super-sampled texture?
+	bool pass_has_arrays() const { return pass_arrays_; }
+
 private:
 	struct QueuedVertex
 	{
@@ -167,6 +186,7 @@ private:
 	uint32_t pass_feedback_ = 0; // 0 none, 1 colour, 2 depth
 	uint32_t pass_feedback_psm_ = 0, pass_feedback_cpsm_ = 0;
 	bool pass_z_feedback_ = false;
+	bool pass_arrays_ = false; // some texture of the pass is super-sampled
 
 	// The image depends on TEX0 (without CLD), CLAMP, TEXA, the level count and the level
 	// addresses in MIPTBP1 / MIPTBP2 (unused levels' fields cleared).
@@ -174,13 +194,15 @@ private:
 	{
 		uint64_t tex0, clamp, texa, mip1, mip2;
 		uint32_t levels;
+		uint32_t samples; // super-sampled texture: samples per pixel, else 0
 		bool operator==(const SlotKey &o) const
 		{
 			return tex0 == o.tex0 && clamp == o.clamp && texa == o.texa && mip1 == o.mip1 && mip2 == o.mip2 &&
-			       levels == o.levels;
+			       levels == o.levels && samples == o.samples;
 		}
 	};
-	SlotKey slot_key(const RegisterState &r, uint32_t ctx) const;
+	// `sprite`: the primitive being drawn is a sprite (only sprites sample super-sampled textures).
+	SlotKey slot_key(const RegisterState &r, uint32_t ctx, bool sprite = false) const;
 	std::vector<SlotKey> slot_keys_;
 	// Texture cache across passes (gs/gs_renderer.cpp:1317-1386 create_cached_texture, with
 	// page_tracker.cpp:842-933 invalidation): fully decoded images of earlier passes, kept in
@@ -207,17 +229,18 @@ private:
 	std::vector<TexDecode> decodes_;
 	std::vector<TextureDesc> tex_descs_;
 	uint32_t texel_total_ = 0, group_total_ = 0;
-	SlotKey span_key_{ ~0ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0u }; // last texture whose page spans are cached
+	SlotKey span_key_{ ~0ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0u, 0u }; // last texture whose page spans are cached
 	TexSpans span_{};
 	bool state_dirty_ = true;
 	bool last_candidate_ = false;
 	int32_t last_order_[3] = { 0, 1, 2 };
 
 	bool draw_is_degenerate(const RegisterState &r) const;
+	void mark_blur_kernels();
 	void append(const RegisterState &r, const QueuedVertex *v, uint32_t n, bool quad);
 	uint32_t state_index(const StateVector &sv);
 	// Texture slot of the current TEX0 / CLAMP / TEXA; -1 when the pass has no room for it.
-	int32_t texture_slot(const RegisterState &r, uint32_t ctx);
+	int32_t texture_slot(const RegisterState &r, uint32_t ctx, bool sprite);
 	// How a textured primitive reads the frame or depth buffer it draws to.
 	enum class Feedback { None, Pixel, Bypass };
 	Feedback feedback_mode(const RegisterState &r, uint32_t ctx, const VertexPosition *p, const VertexAttribute *a,

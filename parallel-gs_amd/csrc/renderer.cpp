@@ -1,5 +1,6 @@
 // Device-side renderer: HBM buffers, upload arena, transfer batching and launch recording.
 #include "renderer.hpp"
+#include <mutex>
 #include "render_pass.hpp"
 #include "raster_tables.hpp"
 #include <algorithm>
@@ -59,7 +60,7 @@ Renderer::~Renderer()
 	for (void *p : { (void *)setup_buf_, (void *)tattr_buf_, (void *)lists_buf_, (void *)counts_buf_, (void *)dummy_buf_,
 	                 (void *)clut_buf_, (void *)tex_images_, (void *)state_copy_, (void *)flat_buckets_,
 	                 (void *)tile_modes_, (void *)coarse_mask_, (void *)feedback_masks_, (void *)snapshot_,
-	                 (void *)restore_dev_, (void *)fused_dev_ })
+	                 (void *)restore_dev_, (void *)fused_dev_, (void *)phase_lut_ })
 		if (p) (void)hipFree(p);
 	if (vram_) (void)hipFree(vram_);
 	if (shadow_) (void)hipFree(shadow_);
@@ -645,7 +646,8 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 		const uint32_t m = ~d.opaque_fbmask & 0xf8f8f8u;
 		const bool channel_slice = !fb_32_24 && m != 0xf8f8f8u && m != 0;
 		const bool aliased = d.z_write && d.zbp == d.fbp;
-		if (channel_slice || d.z_feedback || aliased)
+		// A pass sampling super-sampled textures stays super-sampled.
+		if (!d.arrays && (channel_slice || d.z_feedback || aliased))
 			rate_x = rate_y = 0;
 	}
 	const bool super_pass = rate_y != 0;
@@ -781,6 +783,7 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	ls.setup.tattr = tattr_buf_;
 	ls.setup.num_primitives = n;
 	ls.setup.rate_log2 = rate_y;
+	ls.setup.texture_flags = (d.arrays && super_pass ? 1u : 0u) | rate_x << 8;
 	ls.setup.opaque_fbmask = d.opaque_fbmask;
 	// Depth format of the single-sample heuristic (gs/gs_renderer.cpp:1918-1940, 2001-2031);
 	// 24-bit when the pass does not use depth.
@@ -905,6 +908,8 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 		a.rate_log2 = rate_y;
 		a.rate_x_log2 = rate_x;
 		a.single_tiles = 1;
+		if (d.arrays)
+			a.phase_lut = phase_lut(rate_x, rate_y);
 	}
 	if (!fb_z_alias)
 		launch(lsh);
@@ -914,6 +919,33 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 
 	stats_.render_passes++;
 	stats_.primitives += n;
+}
+
+// Phase filter table of a sampling grid for super-sampled textures (gs/gs_renderer.cpp:189-313):
+// built once per process and grid on the host, uploaded once per context and grid into its own
+// slot, so that recordings keep reading the table of their grid.
+const uint32_t *Renderer::phase_lut(uint32_t rate_x, uint32_t rate_y)
+{
+	static std::mutex mutex;
+	static std::vector<uint32_t> tables[kPhaseLutSlots];
+	const uint32_t slot = rate_y * 3u + rate_x;
+	if (slot >= kPhaseLutSlots)
+		throw std::runtime_error("sampling grid without a phase table");
+	if (!phase_lut_)
+		check(hipMalloc(&phase_lut_, size_t(kPhaseLutSlots) * PGS_PHASE_LUT_WORDS * sizeof(uint32_t)));
+	uint32_t *dst = phase_lut_ + size_t(slot) * PGS_PHASE_LUT_WORDS;
+	if (!(phase_lut_loaded_ & (1u << slot)))
+	{
+		std::lock_guard<std::mutex> lock(mutex);
+		if (tables[slot].empty())
+		{
+			tables[slot].resize(PGS_PHASE_LUT_WORDS);
+			pgs_phase_lut(rate_x, rate_y, tables[slot].data());
+		}
+		check(hipMemcpy(dst, tables[slot].data(), PGS_PHASE_LUT_WORDS * sizeof(uint32_t), hipMemcpyHostToDevice));
+		phase_lut_loaded_ |= 1u << slot;
+	}
+	return dst;
 }
 
 // Shading of a pass whose depth buffer is its colour buffer (gs/gs_renderer.cpp:2525-2690).

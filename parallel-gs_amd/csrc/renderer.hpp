@@ -201,6 +201,12 @@ private:
 	uint32_t *tile_modes_ = nullptr;  // super-sampled passes: per coarse tile
 	uint64_t *coarse_mask_ = nullptr; // first binning level: kMaxCoarseTiles x kMaxMaskWords
 	uint32_t *feedback_masks_ = nullptr; // sampler feedback: kMaxPassTextures x PGS_FEEDBACK_WORDS
+	// Phase filter tables of super-sampled textures, one slot per sampling grid (rate_y * 3 +
+	// rate_x), allocated and filled on first use.
+	static constexpr uint32_t kPhaseLutSlots = 12;
+	uint32_t *phase_lut_ = nullptr;
+	uint32_t phase_lut_loaded_ = 0;
+	const uint32_t *phase_lut(uint32_t rate_x, uint32_t rate_y);
 	uint32_t sampler_feedback_ = 0;
 
 public:

@@ -105,6 +105,14 @@ __device__ __forceinline__ void texture_decode_body(const pgs_tex_decode_args &a
 	if (t0 >= d.texels)
 		return;
 	uint32_t t = t0, level = 0, w = d.width, h = d.height, tbp = d.tbp, tbw = d.tbw;
+	// A super-sampled texture's layers follow each other: layer 0 from VRAM, layer k from the
+	// plane of sample k - 1 (upload.comp:133-228 with SUPER_SAMPLED).
+	uint32_t layer = 0;
+	if (d.samples)
+	{
+		layer = t / (w * h);
+		t -= layer * (w * h);
+	}
 	while (t >= w * h && level + 1u < d.levels)
 	{
 		t -= w * h;
@@ -120,7 +128,7 @@ __device__ __forceinline__ void texture_decode_body(const pgs_tex_decode_args &a
 	if (t >= w * h)
 		return;
 	const uint32_t x = t % w, y = t / w;
-	if (a.feedback && d.levels == 1u && w <= 1024u && h <= 1024u)
+	if (a.feedback && d.levels == 1u && !d.samples && w <= 1024u && h <= 1024u)
 	{
 		const uint32_t word = a.feedback[s * PGS_FEEDBACK_WORDS + (y >> 3) * 4u + (x >> 8)];
 		if (!((word >> ((x >> 3) & 31u)) & 1u))
@@ -130,21 +138,47 @@ __device__ __forceinline__ void texture_decode_body(const pgs_tex_decode_args &a
 	const uint32_t ty = ((y & (d.vmsk >> level)) + (d.by >> level)) | (d.vfix >> level);
 	const uint32_t addr = swizzle_addr(tx, ty, tbp, tbw, d.psm);
 	const uint8_t *v = a.vram;
+	// Sample plane and reference plane of a sample layer: the sample's value replaces the
+	// single-sampled one where the reference plane marks all its bits (24-bit formats: the
+	// marked bits).
+	const uint8_t *ref = v + kVramSize, *hi = v + size_t(layer + 1u) * kVramSize;
 	uint32_t c;
 	switch (d.psm)
 	{
 	case PSMCT24: case PSMZ24:
-		c = rgb24_expand(*reinterpret_cast<const uint32_t *>(v + addr), d.aem, d.ta0);
+	{
+		uint32_t w32 = *reinterpret_cast<const uint32_t *>(v + addr);
+		if (layer)
+		{
+			const uint32_t m = *reinterpret_cast<const uint32_t *>(ref + addr);
+			if (m)
+				w32 = (*reinterpret_cast<const uint32_t *>(hi + addr) & m) | (w32 & ~m);
+		}
+		c = rgb24_expand(w32, d.aem, d.ta0);
 		break;
+	}
 	case PSMCT16: case PSMCT16S: case PSMZ16: case PSMZ16S:
-		c = rgba16_expand(*reinterpret_cast<const uint16_t *>(v + addr), d.aem, d.ta0, d.ta1);
+	{
+		uint32_t w16 = *reinterpret_cast<const uint16_t *>(v + addr);
+		if (layer && *reinterpret_cast<const uint16_t *>(ref + addr) == 0xffffu)
+			w16 = *reinterpret_cast<const uint16_t *>(hi + addr);
+		c = rgba16_expand(w16, d.aem, d.ta0, d.ta1);
 		break;
-	case PSMT8: c = clut_color(a.clut, d, v[addr]); break;
-	case PSMT8H: c = clut_color(a.clut, d, v[addr + 3u]); break;
+	}
+	case PSMT8:
+		c = clut_color(a.clut, d, (layer && ref[addr] == 0xffu) ? hi[addr] : v[addr]);
+		break;
+	case PSMT8H:
+		c = clut_color(a.clut, d, (layer && ref[addr + 3u] == 0xffu) ? hi[addr + 3u] : v[addr + 3u]);
+		break;
 	case PSMT4: c = clut_color(a.clut, d, (v[addr >> 1] >> ((addr & 1u) * 4u)) & 0xfu); break;
 	case PSMT4HL: c = clut_color(a.clut, d, (*reinterpret_cast<const uint32_t *>(v + addr) >> 24) & 0xfu); break;
 	case PSMT4HH: c = clut_color(a.clut, d, *reinterpret_cast<const uint32_t *>(v + addr) >> 28); break;
-	default: c = *reinterpret_cast<const uint32_t *>(v + addr); break;
+	default:
+		c = *reinterpret_cast<const uint32_t *>(v + addr);
+		if (layer && *reinterpret_cast<const uint32_t *>(ref + addr) == ~0u)
+			c = *reinterpret_cast<const uint32_t *>(hi + addr);
+		break;
 	}
 	a.images[d.image_offset + t0] = c;
 }

@@ -143,6 +143,24 @@ inline PageSpan page_span(uint32_t base_block, uint32_t stride, uint32_t psm, ui
 	return { (base_block >> 5) + lo, ((base_block + 31u) >> 5) + hi };
 }
 
+// Rectangle of 8 KiB pages a pixel rectangle of a buffer touches, as the page tracker walks it
+// (gs/gs_util.cpp:141-171 compute_page_rect): the page of the first pixel (block base rounded
+// down to its page), `width` x `height` pages, rows `stride` pages apart; page numbers wrap.
+struct PageRect { uint32_t base, stride, width, height; };
+inline PageRect page_rect(uint32_t base_block, uint32_t stride, uint32_t psm, uint32_t x0, uint32_t y0, uint32_t x1,
+                          uint32_t y1)
+{
+	const Geometry g = psm_geometry(psm);
+	if (psm == PSMT8 || psm == PSMT4)
+		stride >>= 1;
+	PageRect r;
+	r.base = (base_block >> 5) + (x0 >> g.pw) + stride * (y0 >> g.ph);
+	r.stride = stride;
+	r.width = (x1 >> g.pw) - (x0 >> g.pw) + 1u;
+	r.height = (y1 >> g.ph) - (y0 >> g.ph) + 1u;
+	return r;
+}
+
 inline bool spans_overlap(PageSpan a, PageSpan b)
 {
 	const uint32_t pages = kNumPages;
@@ -203,6 +221,10 @@ struct TexDecode
 	uint32_t levels;         // mip levels, stored one after another from image_offset
 	uint32_t texels;         // texels of all levels (0: nothing to decode)
 	uint32_t level_buf[kMaxTexLevels - 1]; // TBP | TBW << 14 of levels 1..6
+	// Super-sampled texture: samples per pixel S (0 otherwise). The image then holds S + 1
+	// layers of width x height texels: layer 0 from VRAM, layer k from the plane of sample k - 1
+	// where the reference plane marks the word super-sampled.
+	uint32_t samples;
 };
-static_assert(sizeof(TexDecode) == 104, "TexDecode layout");
+static_assert(sizeof(TexDecode) == 108, "TexDecode layout");
 }

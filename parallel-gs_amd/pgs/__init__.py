@@ -141,7 +141,7 @@ EXPORTED_SYMBOLS = [
     "pgs_profile_enable", "pgs_profile_collect",
     "pgs_host_swizzle_addr", "pgs_host_block_lane_pixel",
     "pgs_k_vram_copy", "pgs_k_vram_snapshot", "pgs_k_init_tables", "pgs_k_triangle_setup",
-    "pgs_k_binning", "pgs_k_shade", "pgs_k_clut_write", "pgs_k_texture_decode", "pgs_k_ssaa_heuristic",
+    "pgs_k_binning", "pgs_k_shade", "pgs_phase_lut", "pgs_k_clut_write", "pgs_k_texture_decode", "pgs_k_ssaa_heuristic",
     "pgs_batch_create", "pgs_batch_destroy", "pgs_batch_apply_tuning", "pgs_batch_autotune", "pgs_batch_replay",
     "pgs_batch_sync", "pgs_batch_stream", "pgs_batch_num_launches", "pgs_batch_profile",
     "pgs_cost_model_create", "pgs_cost_model_destroy", "pgs_cost_model_set_params", "pgs_cost_model_get_params",
@@ -226,6 +226,17 @@ def lib() -> ctypes.CDLL:
         L.pgs_dump_writer_close.argtypes = [vp]
         _lib = L
     return _lib
+
+
+def phase_lut(rate_x: int, rate_y: int) -> np.ndarray:
+    """pgs_phase_lut: the phase filter table of super-sampled textures for a 2^rate_x x 2^rate_y
+    sampling grid (a host function: no device needed)."""
+    out = np.zeros(2 * (64 * 64 + 32 * 32), np.uint32)
+    L = lib()
+    L.pgs_phase_lut.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+    L.pgs_phase_lut.restype = None
+    L.pgs_phase_lut(rate_x, rate_y, out.ctypes.data)
+    return out
 
 
 def streams() -> ctypes.CDLL:

@@ -75,6 +75,10 @@ def oracle_lib() -> ctypes.CDLL:
         L.orc_reset_context_state.restype = None
         L.orc_set_sparse_grid.argtypes = [vp, u32]
         L.orc_set_sparse_grid.restype = None
+        L.orc_set_super_sampled_textures.argtypes = [vp, u32]
+        L.orc_set_super_sampled_textures.restype = None
+        L.orc_phase_lut.argtypes = [u32, u32, vp]
+        L.orc_phase_lut.restype = None
         L.orc_set_priv.argtypes = [vp, vp, sz]
         L.orc_set_priv.restype = None
         L.orc_complex_scanmsk_timeout.argtypes = [vp]
@@ -90,12 +94,15 @@ def oracle_lib() -> ctypes.CDLL:
 class Oracle:
     """Immediate-mode CPU GS (one context)."""
 
-    def __init__(self, sample_log2: int = 0, sparse: bool = False):
-        """sparse: the 1x4 (4x) / 2x8 (16x) sampling grids instead of the ordered ones."""
+    def __init__(self, sample_log2: int = 0, sparse: bool = False, super_sampled_textures: bool = False):
+        """sparse: the 1x4 (4x) / 2x8 (16x) sampling grids instead of the ordered ones;
+        super_sampled_textures: sprites may sample rendered pages per sample (off at 1x)."""
         self._L = oracle_lib()
         self._h = self._L.orc_create(sample_log2)
         if sparse:
             self._L.orc_set_sparse_grid(self._h, 1)
+        if super_sampled_textures and sample_log2:
+            self._L.orc_set_super_sampled_textures(self._h, 1)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -171,3 +178,10 @@ class Oracle:
 
 def swizzle(x, y, base, stride, psm) -> int:
     return int(oracle_lib().orc_swizzle(x, y, base, stride, psm))
+
+
+def oracle_phase_lut(rate_x: int, rate_y: int) -> np.ndarray:
+    """The oracle's phase filter table of super-sampled textures (2 * (64 * 64 + 32 * 32) words)."""
+    out = np.zeros(2 * (64 * 64 + 32 * 32), np.uint32)
+    oracle_lib().orc_phase_lut(rate_x, rate_y, out.ctypes.data)
+    return out

@@ -394,3 +394,83 @@ def random_fbz_alias_stream(seed: int, n_batches: int = 10, prims_per_batch: int
         for i in range(0, len(writes), 240):
             out.append(gif.ad(writes[i:i + 240]))
     return gif.arr(b"".join(out))
+
+
+def random_ssaa_texture_stream(seed: int, n_batches: int = 6, sprites_per_batch: int = 10) -> np.ndarray:
+    """Super-sampled textures: a 128 x 64 source frame (CT32, CT24 or CT16) drawn with 3D
+    triangles (whose edges leave super samples), sometimes partly overwritten by an upload
+    (those pages stop being super-sampled), then sprites into a second frame textured from the
+    source in its own format or a 24-bit / 8H / 8-bit view, with NEAREST or LINEAR, at 1:1,
+    near 1:1, 2x up and down, in UV and ST form, with CLAMP / REGION_CLAMP / REGION_REPEAT, and
+    blur-like passes (one texture at several phases, Cs * alpha + Cd blending, no depth test)."""
+    rng = np.random.default_rng(seed)
+    ofs = 2048 * 16
+    fw, fh = 128, 64
+    dst_fbp, zbp, clut_bp = 8, 16, 2048
+    out = [gif.ad([(0x1A, 1), (0x18, ofs | ofs << 32)])]
+    out.append(gif.upload(0, 2, 0x00, 0, 0, fw, fh, _random_payload(rng, fw * fh * 4)))
+    out.append(gif.upload(dst_fbp * 32, 2, 0x00, 0, 0, fw, fh, _random_payload(rng, fw * fh * 4)))
+    out.append(gif.upload(clut_bp, 1, 0x00, 0, 0, 16, 16, _random_payload(rng, 16 * 16 * 4)))
+    for b in range(n_batches):
+        src_psm = int(rng.choice([0x00, 0x00, 0x01, 0x02]))
+        out.append(gif.ad([(0x4C, frame(0, 2, src_psm)), (0x4E, zbuf(zbp, 0, 0)),
+                           (0x40, scissor(0, fw - 1, 0, fh - 1)), (0x47, test_reg(zte=1, ztst=1)),
+                           (0x42, alpha_reg(0, 1, 0, 1, 0))]))
+        writes = []
+        for t in range(int(rng.integers(4, 12))):
+            writes.append((0x00, prim_reg(3, iip=1, abe=int(rng.integers(0, 4) == 0))))
+            for v in range(3):
+                writes.append((0x01, rgbaq(int(rng.integers(0, 2**32)))))
+                writes.append((0x05, xyz2(ofs + int(rng.integers(-8 * 16, (fw + 8) * 16)),
+                                          ofs + int(rng.integers(-8 * 16, (fh + 8) * 16)), 0)))
+        out.append(gif.ad(writes))
+        if rng.integers(0, 4) == 0:
+            x, y = int(rng.integers(0, 2)) * 64, int(rng.integers(0, 2)) * 32
+            out.append(gif.upload(0, 2, 0x00, x, y, 64, 32, _random_payload(rng, 64 * 32 * 4)))
+        dst_psm = int(rng.choice([0x00, 0x00, 0x02]))
+        blur = rng.integers(0, 3) == 0
+        ztst = 1 if blur else int(rng.integers(1, 4))
+        regs = [(0x4C, frame(dst_fbp, 2, dst_psm)), (0x40, scissor(0, fw - 1, 0, fh - 1)),
+                (0x47, test_reg(zte=1, ztst=ztst)),
+                (0x42, alpha_reg(0, 2, int(rng.integers(0, 3)), 1, int(rng.integers(0, 256))) if blur
+                 else alpha_reg(*[int(rng.integers(0, 3)) for _ in range(4)], int(rng.integers(0, 256))))]
+        out.append(gif.ad(regs))
+        writes = []
+        views = {0x00: [0x00, 0x01, 0x1B, 0x13], 0x01: [0x01, 0x1B], 0x02: [0x02]}[src_psm]
+        mmag_pass = int(rng.integers(0, 2))
+        for p in range(sprites_per_batch):
+            tpsm = int(rng.choice(views))
+            mmag = mmag_pass if blur else int(rng.integers(0, 2))
+            wm = int(rng.choice([0, 0, 1, 2, 3]))
+            if wm == 3:
+                cl = clamp_reg(3, 3, int(rng.choice([0x3F, 0x7F, 0x1F])), 0, int(rng.choice([0x3F, 0x1F])), 0)
+            elif wm == 2:
+                lo, hi = sorted(int(v) for v in rng.integers(0, fw, 2))
+                lov, hiv = sorted(int(v) for v in rng.integers(0, fh, 2))
+                cl = clamp_reg(2, 2, lo, hi, lov, hiv)
+            else:
+                cl = clamp_reg(wm, wm)
+            t0 = tex0(0, 2, tpsm, 7, 6, tcc=int(rng.integers(0, 2)), tfx=int(rng.choice([0, 1, 1])),
+                      cbp=clut_bp, cpsm=0, csa=int(rng.integers(0, 8)), cld=int(rng.choice([0, 1])))
+            fst = int(rng.integers(0, 5) != 0)
+            writes += [(0x14, tex1(mmag, mmag)), (0x08, cl), (0x06, t0),
+                       (0x00, prim_reg(6, iip=0, tme=1, abe=int(blur or rng.integers(0, 2)), fst=fst))]
+            w = int(rng.choice([8, 16, 24, 32, 37, 64]))
+            h = int(rng.choice([8, 16, 20, 32]))
+            x0, y0 = int(rng.integers(0, fw - w + 1)), int(rng.integers(0, fh - h + 1))
+            scale = float(rng.choice([1.0, 1.0, 0.5, 2.0, 0.97, 1.03]))
+            d = int(rng.choice([0, 0, 3, 8, 15])) if not blur else int(rng.integers(0, 16))
+            u0, v0 = int(rng.integers(0, fw - 8)) * 16 + d, int(rng.integers(0, fh - 8)) * 16 + d
+            if blur:
+                u0, v0 = x0 * 16 + d, y0 * 16 + d
+            corners = [(x0, y0, u0, v0), (x0 + w, y0 + h, u0 + int(w * 16 * scale), v0 + int(h * 16 * scale))]
+            q = float(rng.choice([1.0, 1.0, 2.0]))
+            for (x, y, u, v) in corners:
+                writes.append((0x01, rgbaq(int(rng.integers(0, 2**32)), q)))
+                if fst:
+                    writes.append((0x03, uv(u, v)))
+                else:
+                    writes.append((0x02, st(u / (fw * 16.0) * q, v / (fh * 16.0) * q)))
+                writes.append((0x05, xyz2(ofs + x * 16, ofs + y * 16, int(rng.integers(0, 2**24)))))
+        out.append(gif.ad(writes))
+    return gif.arr(b"".join(out))

@@ -235,7 +235,7 @@ def test_dynamic_super_sampling_rate_change(hip_device):
         ctx.gif_transfer(stream2)
         o.gif_transfer(stream2)
         got = ctx.vram_planes()
-        assert ctx.set_super_sampling_rate(4, textures=True) == pgs.ERROR_UNSUPPORTED
+        assert ctx.set_super_sampling_rate(4, textures=True) == 0
         assert ctx.set_super_sampling_rate(3) == pgs.ERROR_INVALID_ARGUMENT
         assert ctx.super_sampling_rate == 4
     o.flush()
@@ -246,9 +246,11 @@ def test_dynamic_super_sampling_rate_change(hip_device):
 
 
 def test_create_rejects_unbuilt_options(hip_device):
-    for kw in (dict(super_sampling=3), dict(super_sampling=4, super_sampled_textures=True)):
-        with pytest.raises(pgs.PgsError):
-            pgs.Context(**kw)
+    with pytest.raises(pgs.PgsError):
+        pgs.Context(super_sampling=3)
+    # Super-sampled textures are built; at 1x the option is accepted and has no effect.
+    for kw in (dict(super_sampling=4, super_sampled_textures=True), dict(super_sampling=1, super_sampled_textures=True)):
+        pgs.Context(**kw).close()
 
 
 def test_signal_interface_and_options(hip_device):
