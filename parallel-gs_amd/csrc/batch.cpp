@@ -60,7 +60,7 @@ uint64_t key_of(const Renderer::Launch &l)
 {
 	uint64_t k = uint64_t(l.kind) << 16;
 	if (l.kind == Renderer::LaunchKind::Shade || l.kind == Renderer::LaunchKind::ShadeSuper)
-		k |= uint64_t(l.shade.rate_x_log2) << 8 | l.shade.rate_log2;
+		k |= uint64_t(l.shade.rate_x_log2) << 8 | l.shade.rate_log2 | (l.shade.phase_lut ? 1ull << 12 : 0ull);
 	return k;
 }
 
@@ -273,7 +273,7 @@ bool Batch::upload()
 			return false;
 		const pgs_shade_args &s = launches_[m.refs[0].context][m.refs[0].index].shade;
 		const uint32_t rx = s.rate_x_log2, ry = s.rate_log2;
-		return ry == 0 || (ry == 1 && rx <= 1) || (ry == 2 && (rx == 1 || rx == 2));
+		return !s.phase_lut && (ry == 0 || (ry == 1 && rx <= 1) || (ry == 2 && (rx == 1 || rx == 2)));
 	};
 	for (size_t i = 0; i < merged_.size();)
 	{
@@ -345,7 +345,7 @@ bool Batch::upload()
 		if (m.kind == Renderer::LaunchKind::Shade || m.kind == Renderer::LaunchKind::ShadeSuper)
 		{
 			const Renderer::Launch &l0 = launches_[m.refs[0].context][m.refs[0].index];
-			m.rate_x = l0.shade.rate_x_log2;
+			m.rate_x = l0.shade.rate_x_log2 | (l0.shade.phase_lut ? kRateArrays : 0u);
 			m.rate_y = l0.shade.rate_log2;
 		}
 	}

@@ -433,6 +433,7 @@ __device__ __forceinline__ uint32_t sample_arrayed(const TextureDesc &t, const u
 // sample_texture (ubershader.comp:1042-1258). At LOD 0 (no mipmap): UV snap to 1/16 texel with
 // the reference's epsilons, then a NEAREST or LINEAR fetch; super-sampled textures take the
 // array path after the clamp.
+template <bool kArrays>
 __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint32_t *images, uint32_t tex,
                                                uint32_t tex2, float u, float v, float q, bool perspective,
                                                const uint32_t *lut, uint32_t smp, float shift, float out[4])
@@ -451,12 +452,15 @@ __device__ __forceinline__ void sample_texture(const TextureDesc &t, const uint3
 		// +-2047), so the median of (x, lo, hi) is that clamp.
 		u1 = __builtin_amdgcn_fmed3f(u1, t.region[0], t.region[2]);
 		v1 = __builtin_amdgcn_fmed3f(v1, t.region[1], t.region[3]);
-		if (tex & TEX_ARRAYED)
+		if constexpr (kArrays)
 		{
-			c = sample_arrayed(t, images, lut, tex, u1, v1, smp, shift);
-			for (uint32_t ch = 0; ch < 4; ch++)
-				out[ch] = float((c >> (8 * ch)) & 0xffu);
-			return;
+			if (tex & TEX_ARRAYED)
+			{
+				c = sample_arrayed(t, images, lut, tex, u1, v1, smp, shift);
+				for (uint32_t ch = 0; ch < 4; ch++)
+					out[ch] = float((c >> (8 * ch)) & 0xffu);
+				return;
+			}
 		}
 		float un = floorf(__fadd_rn(__fmul_rn(u1, 16.0f), 1.0f / 64.0f));
 		float vn = floorf(__fadd_rn(__fmul_rn(v1, 16.0f), 1.0f / 64.0f));
@@ -575,7 +579,7 @@ __device__ __forceinline__ float phase_shift(uint32_t rate_x, uint32_t rate_y)
 	return rate_y == 2u ? 0.125f : 0.0625f;
 }
 
-template <bool kPredicated>
+template <bool kPredicated, bool kArrays = false>
 __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassInfo &pi, uint32_t prim_index,
                                             int32_t hx, int32_t hy, uint32_t rate, bool super_sampled, uint32_t smp,
                                             uint32_t &color, uint32_t &depth, bool &cdirty, bool &zdirty)
@@ -725,7 +729,7 @@ __device__ __forceinline__ void shade_pixel(const pgs_shade_args &a, const PassI
 			}
 			u = fminf(fmaxf(u, -2047.0f), 2047.0f);
 			v = fminf(fmaxf(v, -2047.0f), 2047.0f);
-			sample_texture(td, a.tex_images, p_tex, p_tex2, u, v, stqf(2), persp, a.phase_lut, smp,
+			sample_texture<kArrays>(td, a.tex_images, p_tex, p_tex2, u, v, stqf(2), persp, a.phase_lut, smp,
 			               super_sampled ? phase_shift(a.rate_x_log2, rate) : 0.5f, tx);
 		}
 		const bool tcc = (sv_combiner & COMB_TCC) != 0;
@@ -1048,7 +1052,7 @@ __device__ __forceinline__ void sample_offset(uint32_t s, int32_t &dx, int32_t &
 // with single_tiles, the first wave of a block of a single-sampled tile shades its 64 pixels at
 // 1x through the same loop (the sampling rate is then a wave-uniform 0), as the single-sampled
 // kernel would, and the block's other waves leave.
-template <uint32_t RX, uint32_t RY>
+template <uint32_t RX, uint32_t RY, bool kArrays = false>
 __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t sub)
 {
 	constexpr uint32_t SL = RX + RY, S = 1u << SL;
@@ -1181,7 +1185,7 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 			const uint32_t bit = uint32_t(__builtin_ctzll(ballot));
 			ballot &= ballot - 1u;
 			const uint32_t prim = uint32_t(__builtin_amdgcn_readlane(int(idx), int(bit)));
-			shade_pixel<false>(a, pi, prim, hx, hy, rate, !single, smp, color, depth, cdirty, zdirty);
+			shade_pixel<false, kArrays>(a, pi, prim, hx, hy, rate, !single, smp, color, depth, cdirty, zdirty);
 		}
 	}
 
@@ -1264,7 +1268,7 @@ __device__ __forceinline__ void shade_ssaa_wave(const pgs_shade_args &a, uint32_
 	}
 }
 
-template <uint32_t RX, uint32_t RY>
+template <uint32_t RX, uint32_t RY, bool kArrays = false>
 __device__ __forceinline__ void shade_ssaa_body(const pgs_shade_args &a, uint32_t wg, uint32_t nwg)
 {
 	constexpr uint32_t S = 1u << (RX + RY);
@@ -1272,16 +1276,18 @@ __device__ __forceinline__ void shade_ssaa_body(const pgs_shade_args &a, uint32_
 	const uint32_t logical = (wg % 8u) * per_xcd + wg / 8u;
 	const uint32_t wave_global = logical * waves_of(a) + (threadIdx.x >> 6);
 	const uint32_t block = wave_global / S, sub = wave_global % S;
-	shade_ssaa_wave<RX, RY>(a, block % a.blocks_x, block / a.blocks_x, sub);
+	shade_ssaa_wave<RX, RY, kArrays>(a, block % a.blocks_x, block / a.blocks_x, sub);
 }
 
 __global__ void __launch_bounds__(256) k_shade_1x(pgs_shade_args a) { shade_1x_body(a, blockIdx.x, gridDim.x); }
 // Held to 8 waves per SIMD (64 VGPRs, 78 SGPRs with a few spilled to VGPR lanes): faster than
 // the 7 waves its unbounded 106 SGPRs allow (textured 4x: 86.5 against 91 us per launch).
-template <uint32_t RX, uint32_t RY>
+// kArrays: the pass samples super-sampled textures (its own instantiation, so that passes
+// without them keep the register allocation of the plain sampling path).
+template <uint32_t RX, uint32_t RY, bool kArrays = false>
 __global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 k_shade_ssaa(pgs_shade_args a)
 {
-	shade_ssaa_body<RX, RY>(a, blockIdx.x, gridDim.x);
+	shade_ssaa_body<RX, RY, kArrays>(a, blockIdx.x, gridDim.x);
 }
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_1x_multi, shade_1x_body, pgs_shade_args)
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_01_multi, (shade_ssaa_body<0, 1>), pgs_shade_args)
@@ -1290,6 +1296,12 @@ PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_12_multi, (shade_ssaa_b
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_22_multi, (shade_ssaa_body<2, 2>), pgs_shade_args)
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_02_multi, (shade_ssaa_body<0, 2>), pgs_shade_args)
 PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_13_multi, (shade_ssaa_body<1, 3>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_01_multi_arrays, (shade_ssaa_body<0, 1, true>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_11_multi_arrays, (shade_ssaa_body<1, 1, true>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_12_multi_arrays, (shade_ssaa_body<1, 2, true>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_22_multi_arrays, (shade_ssaa_body<2, 2, true>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_02_multi_arrays, (shade_ssaa_body<0, 2, true>), pgs_shade_args)
+PGS_DEFINE_MULTI_ATTR(256, PGS_OCCUPANCY_8, k_shade_ssaa_13_multi_arrays, (shade_ssaa_body<1, 3, true>), pgs_shade_args)
 
 // Launches of different sampling grids in one multi-launch (independent passes of different
 // contexts): each workgroup runs the body of its own launch's grid. One launch instead of one
@@ -1419,38 +1431,50 @@ __global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 k_shade_1x_queue(pgs_queu
 {
 	tile_queue_loop<1>(q, args, 1u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t) { shade_1x_block(a, bx, by); });
 }
-#define PGS_SSAA_QUEUE_KERNEL(NAME, RX, RY)                                                         \
+#define PGS_SSAA_QUEUE_KERNEL(NAME, RX, RY, KA)                                                         \
 	__global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 NAME(pgs_queue_args q,                  \
 	                                                            const pgs_shade_args *__restrict__ args) \
 	{                                                                                              \
 		tile_queue_loop<(1u << (RX + RY))>(q, args, 2u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, \
-		                                               uint32_t sub) { shade_ssaa_wave<RX, RY>(a, bx, by, sub); }); \
+		                                               uint32_t sub) { shade_ssaa_wave<RX, RY, KA>(a, bx, by, sub); }); \
 	}
-PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_01_queue, 0, 1)
-PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_11_queue, 1, 1)
-PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_12_queue, 1, 2)
-PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_22_queue, 2, 2)
-PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_02_queue, 0, 2)
-PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_13_queue, 1, 3)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_01_queue, 0, 1, false)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_01_queue_arrays, 0, 1, true)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_11_queue, 1, 1, false)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_11_queue_arrays, 1, 1, true)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_12_queue, 1, 2, false)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_12_queue_arrays, 1, 2, true)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_22_queue, 2, 2, false)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_22_queue_arrays, 2, 2, true)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_02_queue, 0, 2, false)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_02_queue_arrays, 0, 2, true)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_13_queue, 1, 3, false)
+PGS_SSAA_QUEUE_KERNEL(k_shade_ssaa_13_queue_arrays, 1, 3, true)
 
 __global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 k_shade_1x_ordered(pgs_queue_args q,
                                                                           const pgs_shade_args *__restrict__ args)
 {
 	ordered_chunk<1>(q, args, 1u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, uint32_t) { shade_1x_block(a, bx, by); });
 }
-#define PGS_SSAA_ORDERED_KERNEL(NAME, RX, RY)                                                       \
+#define PGS_SSAA_ORDERED_KERNEL(NAME, RX, RY, KA)                                                       \
 	__global__ void __launch_bounds__(256) PGS_OCCUPANCY_8 NAME(pgs_queue_args q,                  \
 	                                                            const pgs_shade_args *__restrict__ args) \
 	{                                                                                              \
 		ordered_chunk<(1u << (RX + RY))>(q, args, 2u, [](const pgs_shade_args &a, uint32_t bx, uint32_t by, \
-		                                                 uint32_t sub) { shade_ssaa_wave<RX, RY>(a, bx, by, sub); }); \
+		                                                 uint32_t sub) { shade_ssaa_wave<RX, RY, KA>(a, bx, by, sub); }); \
 	}
-PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_01_ordered, 0, 1)
-PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_11_ordered, 1, 1)
-PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_12_ordered, 1, 2)
-PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_22_ordered, 2, 2)
-PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_02_ordered, 0, 2)
-PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_13_ordered, 1, 3)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_01_ordered, 0, 1, false)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_01_ordered_arrays, 0, 1, true)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_11_ordered, 1, 1, false)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_11_ordered_arrays, 1, 1, true)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_12_ordered, 1, 2, false)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_12_ordered_arrays, 1, 2, true)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_22_ordered, 2, 2, false)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_22_ordered_arrays, 2, 2, true)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_02_ordered, 0, 2, false)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_02_ordered_arrays, 0, 2, true)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_13_ordered, 1, 3, false)
+PGS_SSAA_ORDERED_KERNEL(k_shade_ssaa_13_ordered_arrays, 1, 3, true)
 
 __device__ __forceinline__ float wave_sum(float v)
 {
@@ -1609,6 +1633,28 @@ static bool shade_args_valid(const pgs_shade_args *args)
 	return true;
 }
 
+// A super-sampled pass that samples super-sampled textures.
+static int launch_shade_arrays(const pgs_shade_args &a, uint32_t rx, uint32_t ry, uint32_t groups, uint32_t threads,
+                               hipStream_t stream)
+{
+	if (rx == 0 && ry == 1)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<0, 1, true>), dim3(groups), dim3(threads), 0, stream, a);
+	else if (rx == 1 && ry == 1)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 1, true>), dim3(groups), dim3(threads), 0, stream, a);
+	else if (rx == 1 && ry == 2)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 2, true>), dim3(groups), dim3(threads), 0, stream, a);
+	else if (rx == 2 && ry == 2)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<2, 2, true>), dim3(groups), dim3(threads), 0, stream, a);
+	else if (rx == 0 && ry == 2)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<0, 2, true>), dim3(groups), dim3(threads), 0, stream, a);
+	else if (rx == 1 && ry == 3)
+		hipLaunchKernelGGL((pgs::k_shade_ssaa<1, 3, true>), dim3(groups), dim3(threads), 0, stream, a);
+	else
+		return -int(hipErrorNotSupported);
+	hipError_t err = hipGetLastError();
+	return err == hipSuccess ? 0 : -int(err);
+}
+
 extern "C" int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream)
 {
 	if (!shade_args_valid(args))
@@ -1618,6 +1664,8 @@ extern "C" int pgs_k_shade(const pgs_shade_args *args, hipStream_t stream)
 		return 0;
 	const uint32_t threads = 64u * (args->waves_per_group ? args->waves_per_group : 4u);
 	const uint32_t rx = args->rate_x_log2, ry = args->rate_log2;
+	if (ry != 0 && args->phase_lut)
+		return launch_shade_arrays(*args, rx, ry, groups, threads, stream);
 	if (ry == 0)
 		hipLaunchKernelGGL(pgs::k_shade_1x, dim3(groups), dim3(threads), 0, stream, *args);
 	else if (rx == 0 && ry == 1)
@@ -1643,6 +1691,21 @@ int pgs_multi_shade(const pgs_shade_args *args, const uint32_t *prefix, uint32_t
                     uint32_t rate_y, uint32_t waves, hipStream_t stream)
 {
 	const uint32_t threads = 64u * (waves ? waves : 4u);
+	const bool arrays = (rate_x & kRateArrays) != 0 && rate_y != 0;
+	rate_x &= ~kRateArrays;
+#define PGS_MULTI(K) hipLaunchKernelGGL(pgs::K, dim3(total), dim3(threads), 0, stream, args, prefix, n)
+	if (arrays)
+	{
+		if (rate_x == 0 && rate_y == 1) PGS_MULTI(k_shade_ssaa_01_multi_arrays);
+		else if (rate_x == 1 && rate_y == 1) PGS_MULTI(k_shade_ssaa_11_multi_arrays);
+		else if (rate_x == 1 && rate_y == 2) PGS_MULTI(k_shade_ssaa_12_multi_arrays);
+		else if (rate_x == 2 && rate_y == 2) PGS_MULTI(k_shade_ssaa_22_multi_arrays);
+		else if (rate_x == 0 && rate_y == 2) PGS_MULTI(k_shade_ssaa_02_multi_arrays);
+		else if (rate_x == 1 && rate_y == 3) PGS_MULTI(k_shade_ssaa_13_multi_arrays);
+		else return -int(hipErrorNotSupported);
+		return hipGetLastError() == hipSuccess ? 0 : -int(hipErrorLaunchFailure);
+	}
+#undef PGS_MULTI
 	if (rate_y == 0)
 		hipLaunchKernelGGL(pgs::k_shade_1x_multi, dim3(total), dim3(threads), 0, stream, args, prefix, n);
 	else if (rate_x == 0 && rate_y == 1)
@@ -1686,6 +1749,18 @@ static uint32_t queue_grid(const void *kernel, uint32_t threads)
 
 static const void *queue_kernel(uint32_t rate_x, uint32_t rate_y)
 {
+	if ((rate_x & kRateArrays) && rate_y != 0)
+	{
+		rate_x &= ~kRateArrays;
+		if (rate_x == 0 && rate_y == 1) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_queue_arrays);
+		if (rate_x == 1 && rate_y == 1) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_queue_arrays);
+		if (rate_x == 1 && rate_y == 2) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_queue_arrays);
+		if (rate_x == 2 && rate_y == 2) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_queue_arrays);
+		if (rate_x == 0 && rate_y == 2) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_02_queue_arrays);
+		if (rate_x == 1 && rate_y == 3) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_13_queue_arrays);
+		return nullptr;
+	}
+	rate_x &= ~kRateArrays;
 	if (rate_y == 0) return reinterpret_cast<const void *>(pgs::k_shade_1x_queue);
 	if (rate_x == 0 && rate_y == 1) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_queue);
 	if (rate_x == 1 && rate_y == 1) return reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_queue);
@@ -1735,13 +1810,24 @@ extern "C" int pgs_k_shade_ordered(const pgs_queue_args *q, uint32_t grid, hipSt
 	if (waves != 1 && waves != 2 && waves != 4)
 		return -int(hipErrorInvalidValue);
 	const void *k = nullptr;
-	if (q->rate_y == 0) k = reinterpret_cast<const void *>(pgs::k_shade_1x_ordered);
-	else if (q->rate_x == 0 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_ordered);
-	else if (q->rate_x == 1 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_ordered);
-	else if (q->rate_x == 1 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_ordered);
-	else if (q->rate_x == 2 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_ordered);
-	else if (q->rate_x == 0 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_02_ordered);
-	else if (q->rate_x == 1 && q->rate_y == 3) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_13_ordered);
+	const uint32_t rx = q->rate_x & ~kRateArrays;
+	if ((q->rate_x & kRateArrays) && q->rate_y != 0)
+	{
+		if (rx == 0 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_ordered_arrays);
+		else if (rx == 1 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_ordered_arrays);
+		else if (rx == 1 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_ordered_arrays);
+		else if (rx == 2 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_ordered_arrays);
+		else if (rx == 0 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_02_ordered_arrays);
+		else if (rx == 1 && q->rate_y == 3) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_13_ordered_arrays);
+		else return -int(hipErrorNotSupported);
+	}
+	else if (q->rate_y == 0) k = reinterpret_cast<const void *>(pgs::k_shade_1x_ordered);
+	else if (rx == 0 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_01_ordered);
+	else if (rx == 1 && q->rate_y == 1) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_11_ordered);
+	else if (rx == 1 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_12_ordered);
+	else if (rx == 2 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_22_ordered);
+	else if (rx == 0 && q->rate_y == 2) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_02_ordered);
+	else if (rx == 1 && q->rate_y == 3) k = reinterpret_cast<const void *>(pgs::k_shade_ssaa_13_ordered);
 	else return -int(hipErrorNotSupported);
 	const pgs_shade_args *shade_args = static_cast<const pgs_shade_args *>(q->args);
 	void *params[] = { const_cast<pgs_queue_args *>(q), &shade_args };

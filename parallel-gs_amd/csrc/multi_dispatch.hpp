@@ -23,6 +23,9 @@ int pgs_multi_triangle_setup(const pgs_setup_args *args, const uint32_t *prefix,
 int pgs_multi_ssaa_heuristic(const pgs_ssaa_heuristic_args *args, const uint32_t *prefix, uint32_t n, uint32_t total,
                              hipStream_t stream);
 int pgs_multi_binning(const pgs_bin_args *args, const uint32_t *prefix, uint32_t n, uint32_t total, hipStream_t stream);
+// Set in the x rate of a shading multi-launch, queue or dispatch order whose launches sample
+// super-sampled textures: the kernels instantiated with that sampling path are used.
+constexpr uint32_t kRateArrays = 1u << 8;
 int pgs_multi_shade(const pgs_shade_args *args, const uint32_t *prefix, uint32_t n, uint32_t total, uint32_t rate_x,
                     uint32_t rate_y, uint32_t waves, hipStream_t stream);
 // Shading launches of different sampling grids in one multi-launch: all_grids false allows only

@@ -106,12 +106,24 @@ def test_runtime_switch_and_batch_replay(hip_device):
         ctx.end_recording()
         live.append(ctx.vram_planes())
         ctxs.append(ctx)
+    # A context without super-sampled textures beside them: its passes keep the plain kernels
+    # and never share a multi-launch with the others'.
+    plain = pgs.Context(super_sampling=4, arena_bytes=32 << 20)
+    plain.begin_recording()
+    plain.gif_transfer(rs.random_ssaa_texture_stream(862))
+    plain.end_recording()
+    live.append(plain.vram_planes())
+    ctxs.append(plain)
     b = pgs.Batch(ctxs)
-    b.replay(True)
-    b.replay(True)
-    b.sync()
-    for ctx, want in zip(ctxs, live):
-        assert np.array_equal(ctx.vram_planes(), want)
+    model = pgs.CostModel(4)
+    for name in ("hardware", "static", "learned", "dispatch_static", "dispatch_learned"):
+        print(f"[case] batch schedule {name}", flush=True)
+        b.set_schedule(name, model if "learned" in name else None)
+        b.replay(True)
+        b.replay(True)
+        b.sync()
+        for i, (ctx, want) in enumerate(zip(ctxs, live)):
+            assert np.array_equal(ctx.vram_planes(), want), (name, i)
     b.close()
     for ctx in ctxs:
         ctx.close()
