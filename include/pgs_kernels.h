@@ -167,7 +167,9 @@ typedef struct pgs_bin_args
 	int32_t origin_x, origin_y;
 	uint32_t tile_log2;
 	uint32_t tiles_x, tiles_y;
-	uint32_t super_sampled;  /* pass is super-sampled: decide per tile (binning.comp:87-125) */
+	uint32_t super_sampled;  /* pass is super-sampled: decide per tile (binning.comp:87-125); bit 1:
+	                            the pass samples super-sampled textures (their sprites' tiles
+	                            are super-sampled, binning.comp:123-124) */
 	uint32_t *tile_modes;    /* per tile: 0 = no work, 1 = single-sampled, 2 = super-sampled */
 	/* Optional coarse mask from triangle setup (same grid origin; a coarse tile is
 	 * 2^coarse_shift by 2^coarse_shift tiles): only primitives with their bit set are tested.

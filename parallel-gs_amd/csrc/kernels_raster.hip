@@ -812,7 +812,7 @@ __device__ __forceinline__ void binning_body(const pgs_bin_args &args, uint32_t 
 				if ((st & (ST_SNAP_RASTER | ST_OPAQUE)) != (ST_SNAP_RASTER | ST_OPAQUE))
 					req = true;
 				// Sampling a super-sampled texture per sample needs the samples (binning.comp:123-124).
-				if ((prims[i].tex & (TEX_PER_SAMPLE | TEX_FEEDBACK)) == TEX_PER_SAMPLE)
+				if ((args.super_sampled & 2u) && (prims[i].tex & (TEX_PER_SAMPLE | TEX_FEEDBACK)) == TEX_PER_SAMPLE)
 					req = true;
 				if (st & ST_Z_WRITE)
 					has_zw = true;

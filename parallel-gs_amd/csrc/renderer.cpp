@@ -861,7 +861,7 @@ void Renderer::submit_render_pass(const RenderPassDesc &d)
 	lb.bin.tile_log2 = tile_log2;
 	lb.bin.tiles_x = tiles_x;
 	lb.bin.tiles_y = tiles_y;
-	lb.bin.super_sampled = super_pass ? 1u : 0u;
+	lb.bin.super_sampled = super_pass ? (d.arrays ? 3u : 1u) : 0u;
 	lb.bin.tile_modes = super_pass ? tile_modes_ : nullptr;
 	lb.bin.coarse_mask = coarse_mask_;
 	lb.bin.coarse_shift = coarse_shift;
