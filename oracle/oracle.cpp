@@ -314,7 +314,7 @@ void GS::write_register(uint32_t addr, uint64_t v)
 	case 0x36: case 0x37: set_reg(regs.miptbp2[addr - 0x36], v, true); break;
 	case 0x3b: set_reg(regs.texa, v, true); break;
 	case 0x3d: regs.fogcol = v; break;
-	case 0x40: case 0x41: regs.scissor[addr - 0x40] = v; break;
+	case 0x40: case 0x41: set_reg(regs.scissor[addr - 0x40], v, true); break; // STATE_DIRTY_SCISSOR_BIT
 	case 0x42: case 0x43: set_reg(regs.alpha[addr - 0x42], v, true); break;
 	case 0x44: set_reg(regs.dimx, v, true); break;
 	case 0x45: set_reg(regs.dthe, v, true); break;

@@ -384,7 +384,9 @@ void GSInterface::write_register(uint32_t addr, uint64_t v)
 	case 0x3d: regs_.fogcol = v; break;
 	// SCISSOR does not block parallelogram fusion: the reference clears its dirty bit when the
 	// next primitive is clipped, before the fusion test (gs/gs_interface.cpp:2546, 2584).
-	case 0x40: case 0x41: regs_.scissor[addr - 0x40] = v; break;
+	// A scissor change ends parallelogram fusion like any state change (STATE_DIRTY_SCISSOR_BIT,
+	// gs/gs_interface.cpp:2546, 2584).
+	case 0x40: case 0x41: set_state_reg(regs_.scissor[addr - 0x40], v); break;
 	case 0x42: case 0x43: set_state_reg(regs_.alpha[addr - 0x42], v); break;
 	case 0x44: set_state_reg(regs_.dimx, v); break;
 	case 0x45: set_state_reg(regs_.dthe, v); break;

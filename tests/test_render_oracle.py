@@ -267,3 +267,36 @@ def test_rcp_fixed_error_bound():
         r = L.orc_rcp_fixed(int(x))
         exact = 1.0 / float(x)
         assert abs(r - exact) <= 2.0 * np.spacing(np.float32(exact)), (int(x), r, exact)
+
+
+def test_scissor_change_ends_parallelogram_fusion():
+    """Two triangles forming a parallelogram with a SCISSOR write between them are not fused
+    (gs/gs_interface.cpp:2546, 2584): the second is clipped by the new scissor, so the right
+    half it covers stays untouched, while without the change the whole quad is drawn."""
+    import gif
+    import render_streams as rs
+    from oracle_lib import Oracle, swizzle
+    ofs = 2048 * 16
+
+    def run(second_scissor):
+        w = [(0x1A, 1), (0x18, ofs | ofs << 32), (0x4C, rs.frame(0, 1, 0)), (0x40, rs.scissor(0, 63, 0, 31)),
+             (0x47, rs.test_reg(zte=0)), (0x00, rs.prim_reg(4, iip=0))]  # triangle strip
+        quad = [(0, 0), (64, 0), (0, 32), (64, 32)]
+        for k, (x, y) in enumerate(quad):
+            if k == 3:
+                w.append((0x40, second_scissor))
+            w += [(0x01, rs.rgbaq(0xFF0000FF)), (0x05, rs.xyz2(ofs + x * 16, ofs + y * 16, 0))]
+        o = Oracle(0)
+        o.gif_transfer(gif.arr(gif.ad(w)))
+        o.flush()
+        v = o.vram()
+        return np.array([[int.from_bytes(v[swizzle(x, y, 0, 1, 0) * 4:][:4].tobytes(), "little")
+                          for x in range(64)] for y in range(32)])
+
+    full = run(rs.scissor(0, 63, 0, 31))
+    half = run(rs.scissor(0, 31, 0, 31))
+    assert (full == 0xFF0000FF).all()
+    # The first triangle (upper left of the diagonal) keeps the full scissor; the second is held
+    # to the left half, so the lower right corner stays clear.
+    assert (half[:, :32] == 0xFF0000FF).all()
+    assert half[0, 62] == 0xFF0000FF and half[31, 63] == 0 and (half[:, 32:] == 0).sum() > 0
